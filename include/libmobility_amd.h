@@ -1,0 +1,163 @@
+/*
+ * libmobility_amd -- C ABI of the MI355X NBody mobility solver.
+ *
+ * Drop-in boundary for libMobility's NBody solver and its default Lanczos
+ * fluctuation path.  Every entry point below replaces one member of the
+ * reference's C++ interface (paths relative to the reference repository);
+ * the Python class libmobility_amd.NBody binds exactly these symbols.
+ *
+ * Conventions
+ *   - All arrays are FP64, interleaved [x1,y1,z1,x2,...] (3N values), caller
+ *     owned.  `mem` says where they live: LM_MEM_HOST (pageable host memory,
+ *     staged through the device) or LM_MEM_DEVICE (HIP device memory).
+ *   - Mdot overwrites its outputs; sqrtMdotW and divM accumulate (+=), as
+ *     in MobilityInterface.h:179-187 and mobility.h:227-235.
+ *   - Every function returns LM_OK or LM_ERROR; lm_last_error() then returns
+ *     the message of the reference's std::runtime_error for the same
+ *     condition (thread-local).
+ *   - A handle is not thread-safe; all work is ordered on one HIP stream
+ *     (default: the null stream, see lm_nbody_set_stream).
+ */
+#ifndef LIBMOBILITY_AMD_H
+#define LIBMOBILITY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LM_OK 0
+#define LM_ERROR (-1)
+
+#define LM_MEM_HOST 0
+#define LM_MEM_DEVICE 1
+
+/* periodicity_mode, include/MobilityInterface/MobilityInterface.h:27-33 */
+#define LM_PERIODICITY_SINGLE_WALL 0
+#define LM_PERIODICITY_TWO_WALLS 1
+#define LM_PERIODICITY_OPEN 2
+#define LM_PERIODICITY_PERIODIC 3
+#define LM_PERIODICITY_UNSPECIFIED 4
+
+/* nbody_rpy::algorithm, solvers/NBody/extra/interface.h:10 */
+#define LM_ALGORITHM_FAST 0
+#define LM_ALGORITHM_NAIVE 1
+#define LM_ALGORITHM_BLOCK 2
+#define LM_ALGORITHM_ADVISE 3
+
+/* Pair arithmetic (extension; the reference fixes it at compile time,
+ * include/MobilityInterface/defines.h:5-13). */
+#define LM_PAIR_FP64 0
+#define LM_PAIR_FP32 1 /* FP32 pair terms, FP64 per-tile accumulation */
+
+typedef struct lm_nbody lm_nbody;
+
+/* Lanczos progress hook, Parameters::lanczosCallback
+ * (MobilityInterface.h:42, LanczosAlgorithm.cu:187-189). */
+typedef void (*lm_lanczos_callback)(int iteration, double residual, void* user);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* lm_last_error(void);
+
+/* Interface version, LIBMOBILITYVERSION (include/MobilityInterface/defines.h:4). */
+const char* lm_version(void);
+
+/* I/O precision, Mobility::precision (MobilityInterface.h:73-77): "double". */
+const char* lm_precision(void);
+
+/* NBody(Configuration), solvers/NBody/mobility.h:37-47. */
+int lm_nbody_create(int periodicity_x, int periodicity_y, int periodicity_z, lm_nbody** out);
+
+/* Destroys the handle and frees its device memory. */
+int lm_nbody_destroy(lm_nbody* h);
+
+/* NBody::setParametersNBody, solvers/NBody/mobility.h:78-97
+ * (Python: solvers/NBody/python_wrapper.cu:50-61).  delta is in units of the
+ * hydrodynamic radius. */
+int lm_nbody_set_parameters(lm_nbody* h, int algorithm, int nbatch, int nperbatch,
+                            int has_wall_height, double wall_height, double delta);
+
+/* Extension: pair arithmetic (LM_PAIR_FP64 default, LM_PAIR_FP32). */
+int lm_nbody_set_pair_precision(lm_nbody* h, int precision);
+
+/* Extension: order all work of this handle on `stream` (a hipStream_t). */
+int lm_nbody_set_stream(lm_nbody* h, void* stream);
+
+/* NBody::initialize + Mobility::initialize, mobility.h:99-114 and
+ * MobilityInterface.h:102-114.  seed 0 draws from std::random_device. */
+int lm_nbody_initialize(lm_nbody* h, double viscosity, double hydrodynamic_radius,
+                        int include_angular, double tolerance, uint64_t seed,
+                        lm_lanczos_callback callback, void* callback_user);
+
+/* NBody::setPositions, mobility.h:116-162.  n3 = 3N. */
+int lm_nbody_set_positions(lm_nbody* h, const double* positions, int64_t n3, int mem);
+
+/* Extension: ragged independent systems.  offsets[0] = 0 <= ... <=
+ * offsets[nsys] = N; pairs interact only inside a system (generalises the
+ * equal-size batches of mobility.h:56-77).  Applies to the next
+ * lm_nbody_set_positions. */
+int lm_nbody_set_systems(lm_nbody* h, const int64_t* offsets, int nsys);
+
+/* NBody::getNumberParticles, mobility.h:164. */
+int64_t lm_nbody_num_particles(lm_nbody* h);
+
+/* NBody::Mdot, mobility.h:166-182 (checks of pythonify.h:109-149).
+ * forces/torques may be NULL (n = 0); angular may be NULL when the solver was
+ * initialised without includeAngular. */
+int lm_nbody_mdot(lm_nbody* h, const double* forces, int64_t n_forces, const double* torques,
+                  int64_t n_torques, double* linear, double* angular, int mem);
+
+/* Mobility::sqrtMdotW, MobilityInterface.h:132-188 (Lanczos, lanczos.h:43-58):
+ * linear (and angular) += prefactor * M^{1/2} W, W ~ N(0,1) drawn on the device. */
+int lm_nbody_sqrt_mdot_w(lm_nbody* h, double* linear, double* angular, double prefactor, int mem);
+
+/* Extension for parity tests: as lm_nbody_sqrt_mdot_w with the noise W given
+ * (length 3N, or 6N with includeAngular; torque part after the forces). */
+int lm_nbody_sqrt_mdot_w_noise(lm_nbody* h, const double* noise, double* linear, double* angular,
+                               double prefactor, int mem);
+
+/* NBody::divM (random finite differences), mobility.h:184-236:
+ * outputs += prefactor * div M; no-op for open boundaries. */
+int lm_nbody_div_m(lm_nbody* h, double* linear, double* angular, double prefactor, int mem);
+
+/* Mobility::LangevinVelocities, MobilityInterface.h:191-206:
+ * Mdot (when forces/torques given) + sqrt(2 kbt/dt) M^{1/2} W + kbt div M. */
+int lm_nbody_langevin_velocities(lm_nbody* h, double dt, double kbt, const double* forces,
+                                 int64_t n_forces, const double* torques, int64_t n_torques,
+                                 double* linear, double* angular, int mem);
+
+/* Alias with the name used by earlier libMobility versions
+ * (include/MobilityInterface/README.md:16-17): LangevinVelocities with
+ * dt = 2, kbt = 1 (unit noise and drift prefactors). */
+int lm_nbody_hydrodynamic_velocities(lm_nbody* h, const double* forces, int64_t n_forces,
+                                     const double* torques, int64_t n_torques, double* linear,
+                                     double* angular, int mem);
+
+/* Mobility::clean, MobilityInterface.h:220-223. */
+int lm_nbody_clean(lm_nbody* h);
+
+/* Extension: iteration index at which the last Lanczos call converged
+ * (the return value of lanczos::Solver::run, LanczosAlgorithm.cu:192). */
+int lm_nbody_last_lanczos_iterations(lm_nbody* h);
+
+/* Extension: wait for all work of the handle's stream. */
+int lm_nbody_synchronize(lm_nbody* h);
+
+/* ---- self-checks of the kernel arithmetic, run on the host CPU -----------
+ * Not used by any solver path.  They evaluate the same pair and generator
+ * code that the kernels are compiled from, so CPU tests can check it
+ * against the oracle without a GPU. */
+
+/* 6x6 dimensionless coupling block (rows U,W of i; columns F,T of j) as the
+ * pair kernels compute it; fp32 != 0 runs the FP32 arithmetic. */
+int lm_selftest_pair_block_host(int wall, int fp32, const double* pi, const double* pj, double a,
+                                double* block36);
+
+/* Philox4x32-10 of the noise kernels: out = philox(ctr, key). */
+int lm_selftest_philox_host(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LIBMOBILITY_AMD_H */

@@ -1,0 +1,84 @@
+"""ctypes binding of the C ABI in include/libmobility_amd.h.
+
+The shared library is built in-tree (libmobility_amd/lib/liblmamd.so).  There
+is no fallback: if it is missing, importing the solver raises ImportError.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "liblmamd.so")
+
+LM_OK = 0
+LM_MEM_HOST = 0
+LM_MEM_DEVICE = 1
+
+PERIODICITY = {"single_wall": 0, "two_walls": 1, "open": 2, "periodic": 3, "unspecified": 4}
+ALGORITHM = {"fast": 0, "naive": 1, "block": 2, "advise": 3}
+PAIR_PRECISION = {"fp64": 0, "fp32": 1}
+
+LANCZOS_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_double, ctypes.c_void_p)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+
+# name -> (restype, argtypes); the list is also what the CPU tests check
+# against the declarations in include/libmobility_amd.h.
+SIGNATURES = {
+    "lm_last_error": (ctypes.c_char_p, []),
+    "lm_version": (ctypes.c_char_p, []),
+    "lm_precision": (ctypes.c_char_p, []),
+    "lm_nbody_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "lm_nbody_destroy": (ctypes.c_int, [_vp]),
+    "lm_nbody_set_parameters": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "lm_nbody_set_pair_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lm_nbody_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "lm_nbody_initialize": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_uint64, LANCZOS_CALLBACK, _vp]),
+    "lm_nbody_set_positions": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_int]),
+    "lm_nbody_set_systems": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
+    "lm_nbody_num_particles": (_i64, [_vp]),
+    "lm_nbody_mdot": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, ctypes.c_int]),
+    "lm_nbody_sqrt_mdot_w": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
+    "lm_nbody_sqrt_mdot_w_noise": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
+    "lm_nbody_div_m": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
+    "lm_nbody_langevin_velocities": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _i64,
+                                                    _vp, _i64, _vp, _vp, ctypes.c_int]),
+    "lm_nbody_hydrodynamic_velocities": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, ctypes.c_int]),
+    "lm_nbody_clean": (ctypes.c_int, [_vp]),
+    "lm_nbody_last_lanczos_iterations": (ctypes.c_int, [_vp]),
+    "lm_nbody_synchronize": (ctypes.c_int, [_vp]),
+    "lm_selftest_pair_block_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                                   ctypes.c_double, _dp]),
+    "lm_selftest_philox_host": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32),
+                                               ctypes.POINTER(ctypes.c_uint32),
+                                               ctypes.POINTER(ctypes.c_uint32)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises ImportError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"libmobility_amd: {LIB_PATH} is missing; build it with "
+            "`python -m libmobility_amd.build` (hipcc, gfx950). There is no CPU fallback.")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != LM_OK:
+        msg = load().lm_last_error().decode("utf-8", "replace")
+        raise RuntimeError(msg)

@@ -1,0 +1,196 @@
+// C ABI (include/libmobility_amd.h) over lm::NBodySolver.
+#include "../../include/libmobility_amd.h"
+
+#include <exception>
+#include <string>
+
+#include "pair_math.hpp"
+#include "philox.hpp"
+#include "solver.hpp"
+
+struct lm_nbody {
+  lm::NBodySolver solver;
+  lm_nbody(int px, int py, int pz) : solver(px, py, pz) {}
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return LM_OK;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+  } catch (...) {
+    g_last_error = "[libmobility_amd] unknown error";
+  }
+  return LM_ERROR;
+}
+
+lm::NBodySolver& S(lm_nbody* h) {
+  if (!h) throw lm::Error("[libmobility_amd] null solver handle");
+  return h->solver;
+}
+
+bool is_dev(int mem) {
+  if (mem != LM_MEM_HOST && mem != LM_MEM_DEVICE) throw lm::Error("[libmobility_amd] invalid memory space");
+  return mem == LM_MEM_DEVICE;
+}
+}  // namespace
+
+extern "C" {
+
+const char* lm_last_error(void) { return g_last_error.c_str(); }
+const char* lm_version(void) { return "1.1"; }
+const char* lm_precision(void) { return "double"; }
+
+int lm_nbody_create(int px, int py, int pz, lm_nbody** out) {
+  return guarded([&] {
+    if (!out) throw lm::Error("[libmobility_amd] null output handle");
+    *out = nullptr;
+    *out = new lm_nbody(px, py, pz);
+  });
+}
+
+int lm_nbody_destroy(lm_nbody* h) {
+  return guarded([&] { delete h; });
+}
+
+int lm_nbody_set_parameters(lm_nbody* h, int algorithm, int nbatch, int nperbatch,
+                            int has_wall_height, double wall_height, double delta) {
+  return guarded([&] {
+    S(h).setParameters(algorithm, nbatch, nperbatch, has_wall_height != 0, wall_height, delta);
+  });
+}
+
+int lm_nbody_set_pair_precision(lm_nbody* h, int precision) {
+  return guarded([&] { S(h).setPairPrecision(precision); });
+}
+
+int lm_nbody_set_stream(lm_nbody* h, void* stream) {
+  return guarded([&] { S(h).setStream(static_cast<hipStream_t>(stream)); });
+}
+
+int lm_nbody_initialize(lm_nbody* h, double viscosity, double radius, int include_angular,
+                        double tolerance, uint64_t seed, lm_lanczos_callback cb, void* user) {
+  return guarded([&] {
+    S(h).initialize(viscosity, radius, include_angular != 0, tolerance, seed, cb, user);
+  });
+}
+
+int lm_nbody_set_positions(lm_nbody* h, const double* pos, int64_t n3, int mem) {
+  return guarded([&] { S(h).setPositions(pos, n3, is_dev(mem)); });
+}
+
+int lm_nbody_set_systems(lm_nbody* h, const int64_t* offsets, int nsys) {
+  return guarded([&] {
+    if (!offsets) throw lm::Error("[libmobility_amd] null offsets");
+    S(h).setSystems(offsets, nsys);
+  });
+}
+
+int64_t lm_nbody_num_particles(lm_nbody* h) { return h ? h->solver.numParticles() : -1; }
+
+int lm_nbody_mdot(lm_nbody* h, const double* F, int64_t nF, const double* T, int64_t nT,
+                  double* U, double* W, int mem) {
+  return guarded([&] { S(h).Mdot(F, F ? nF : 0, T, T ? nT : 0, U, W, is_dev(mem)); });
+}
+
+int lm_nbody_sqrt_mdot_w(lm_nbody* h, double* U, double* W, double prefactor, int mem) {
+  return guarded([&] { S(h).sqrtMdotW(U, W, prefactor, is_dev(mem), nullptr); });
+}
+
+int lm_nbody_sqrt_mdot_w_noise(lm_nbody* h, const double* noise, double* U, double* W,
+                               double prefactor, int mem) {
+  return guarded([&] {
+    if (!noise) throw lm::Error("[libmobility_amd] null noise");
+    S(h).sqrtMdotW(U, W, prefactor, is_dev(mem), noise);
+  });
+}
+
+int lm_nbody_div_m(lm_nbody* h, double* U, double* W, double prefactor, int mem) {
+  return guarded([&] { S(h).divM(U, W, prefactor, is_dev(mem)); });
+}
+
+int lm_nbody_langevin_velocities(lm_nbody* h, double dt, double kbt, const double* F, int64_t nF,
+                                 const double* T, int64_t nT, double* U, double* W, int mem) {
+  return guarded(
+      [&] { S(h).langevin(dt, kbt, F, F ? nF : 0, T, T ? nT : 0, U, W, is_dev(mem)); });
+}
+
+int lm_nbody_hydrodynamic_velocities(lm_nbody* h, const double* F, int64_t nF, const double* T,
+                                     int64_t nT, double* U, double* W, int mem) {
+  return lm_nbody_langevin_velocities(h, 2.0, 1.0, F, nF, T, nT, U, W, mem);
+}
+
+int lm_nbody_clean(lm_nbody* h) {
+  return guarded([&] { S(h).clean(); });
+}
+
+int lm_nbody_last_lanczos_iterations(lm_nbody* h) { return h ? h->solver.lastLanczosIterations() : -1; }
+
+int lm_nbody_synchronize(lm_nbody* h) {
+  return guarded([&] { LM_HIP(hipStreamSynchronize(S(h).stream())); });
+}
+
+int lm_selftest_pair_block_host(int wall, int fp32, const double* pi, const double* pj, double a,
+                                double* blk) {
+  return guarded([&] {
+    auto run = [&](auto zero) {
+      using R = decltype(zero);
+      R ri[4], rj[4];
+      const double* p[2] = {pi, pj};
+      R* r[2] = {ri, rj};
+      for (int q = 0; q < 2; ++q) {
+        const double x = p[q][0] / a, y = p[q][1] / a, z = p[q][2];
+        if (wall) {
+          r[q][0] = R(x);
+          r[q][1] = R(y);
+          r[q][2] = R((z > a ? z : a) / a);
+          const double b = z / a;
+          r[q][3] = R(b < 0 ? 0 : (b > 1 ? 1 : b));
+        } else {
+          r[q][0] = R(x);
+          r[q][1] = R(y);
+          r[q][2] = R(z / a);
+          r[q][3] = R(1);
+        }
+      }
+      for (int col = 0; col < 6; ++col) {
+        R v[6] = {0, 0, 0, 0, 0, 0};
+        v[col] = R(1);
+        lm::PairAcc<R> acc;
+        lm::acc_zero(acc);
+        constexpr int all = lm::kUF | lm::kUT | lm::kWT | lm::kWF;
+        if (wall)
+          lm::wall_pair<R, all>(ri[0], ri[1], ri[2], ri[3], rj[0], rj[1], rj[2], rj[3], v[0], v[1],
+                                v[2], v[3], v[4], v[5], acc);
+        else
+          lm::open_pair<R, all>(ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2], v[0], v[1], v[2], v[3],
+                                v[4], v[5], acc);
+        const double u[6] = {double(acc.uf.x + acc.ut.x), double(acc.uf.y + acc.ut.y),
+                             double(acc.uf.z + acc.ut.z), double(acc.wt.x + acc.wf.x),
+                             double(acc.wt.y + acc.wf.y), double(acc.wt.z + acc.wf.z)};
+        for (int row = 0; row < 6; ++row) blk[row * 6 + col] = u[row];
+      }
+    };
+    if (fp32)
+      run(0.0f);
+    else
+      run(0.0);
+  });
+}
+
+int lm_selftest_philox_host(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4) {
+  return guarded([&] {
+    lm::U32x4 c;
+    for (int i = 0; i < 4; ++i) c.v[i] = ctr4[i];
+    const lm::U32x4 r = lm::philox4x32_10(c, key2[0], key2[1]);
+    for (int i = 0; i < 4; ++i) out4[i] = r.v[i];
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Launch interface of the gfx950 kernels (nbody_kernels.hip, lanczos_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace lm {
+
+// One unit of pair work: targets [tgt0, tgt0 + ntgt) against sources
+// [src0, src1) of the same system; the partial result goes to slab `chunk`.
+struct PairItem {
+  int64_t tgt0;
+  int64_t src0;
+  int64_t src1;
+  int32_t ntgt;
+  int32_t chunk;
+};
+
+struct PairArgs {
+  int wall;       // 0 open RPY, 1 bottom wall
+  int fp32;       // 1: FP32 pair terms, FP64 per-tile accumulation
+  int blocks;     // BlockBits mask
+  int tpl;        // targets per lane (1 or 2)
+  const void* pos;      // double4 or float4 records, N entries
+  const double* F;      // 3N AoS, nullptr when absent
+  const double* T;      // 3N AoS, nullptr when absent
+  double* U;            // output slab base, 3N per slab
+  double* W;            // output slab base, 3N per slab (nullptr if no angular)
+  int64_t slab;         // elements between slabs (3N)
+  const PairItem* items;
+  int nitems;
+  double t0, rt0, r0;
+};
+
+constexpr int kPairThreads = 256;  // threads per workgroup = sources per LDS tile
+
+void launch_pair(const PairArgs& a, hipStream_t s);
+void launch_chunk_reduce(const double* part, int nchunks, int64_t n, double* out, hipStream_t s);
+
+// setPositions: per-system bounding boxes (6 doubles each: min xyz, max xyz)
+// of the wall-shifted positions.
+void launch_system_bbox(const double* pos, const int64_t* sys_off, int nsys, double wall_shift,
+                        double* bbox, hipStream_t s);
+// Records for the pair kernels; `shifted` receives the wall-shifted AoS copy.
+void launch_make_records(const double* pos, int64_t N, const int64_t* sys_off, int nsys,
+                         const double* bbox, int wall, double wall_shift, double a,
+                         double* shifted, void* rec64, void* rec32, hipStream_t s);
+
+// ---- Lanczos / vector kernels (FP64) ---------------------------------------
+// Normal(0,1) samples from Philox4x32-10 + Box-Muller: out[k], k < n.
+void launch_normal_fill(double* out, int64_t n, uint64_t seed, uint64_t stream, hipStream_t s);
+
+// Scratch for the single-pass reductions: partial sums + a self-resetting
+// arrival counter.  kReduceBlocks partials per reduction slot.
+constexpr int kReduceBlocks = 256;
+struct ReduceScratch {
+  double* partial;       // kReduceBlocks * 4 doubles
+  unsigned int* counter; // 1 counter, zero at rest
+};
+
+// Lanczos scalar state resident on the device.
+struct LanczosScalars {
+  double normz;    // ||z||
+  double alpha[256];  // hdiag
+  double beta[256];   // hsup
+  double err;      // last error estimate
+  double errnum, errden;
+};
+
+void launch_lanczos_start(const double* z, double* v0, int64_t n, ReduceScratch rs,
+                          LanczosScalars* sc, hipStream_t s);
+// w -= beta[i-1] v_prev (if i > 0); alpha[i] = w . v_i
+void launch_lanczos_alpha(double* w, const double* v_i, const double* v_prev, int64_t n, int i,
+                          ReduceScratch rs, LanczosScalars* sc, hipStream_t s);
+// w -= alpha[i] v_i; beta[i] = ||w|| with the breakdown rule
+void launch_lanczos_beta(double* w, const double* v_i, int64_t n, int i, ReduceScratch rs,
+                         LanczosScalars* sc, hipStream_t s);
+// v_next = w / beta[i], or e1 after breakdown
+void launch_lanczos_next(const double* w, double* v_next, int64_t n, int i,
+                         const LanczosScalars* sc, hipStream_t s);
+// s[0..m) = T_m^{1/2} e1 (FP64, one workgroup); work has kTsqrtScratch doubles
+constexpr int kTsqrtNodes = 512;
+constexpr int kTsqrtScratch = 2 * 256 * kTsqrtNodes;
+void launch_tridiag_sqrt(const LanczosScalars* sc, int m, double* s_out, double* work,
+                         hipStream_t s);
+// y = normz * V[:, 0:m] s ; err = ||y - old|| / ||old||
+void launch_lanczos_estimate(const double* V, int64_t ldv, int m, const double* s_coef,
+                             double* y, const double* old, int64_t n, ReduceScratch rs,
+                             LanczosScalars* sc, hipStream_t s);
+// out[k] += pref * y[k]
+void launch_axpy_accumulate(double* out, const double* y, double pref, int64_t n, hipStream_t s);
+// dst = src + c * w
+void launch_axpy_to(double* dst, const double* src, const double* w, double c, int64_t n,
+                    hipStream_t s);
+// out += pref * (a - b)
+void launch_diff_accumulate(double* out, const double* a, const double* b, double pref,
+                            int64_t n, hipStream_t s);
+void launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
+
+}  // namespace lm

@@ -1,0 +1,480 @@
+// GPU-resident Lanczos square root (after include/lanczos/LanczosAlgorithm.cu,
+// Ando, Chow, Saad, Skolnick, J. Chem. Phys. 137, 064106 (2012)).
+//
+// Everything stays on the device: the Krylov basis V (column-major, n x m),
+// the three-term recurrence scalars alpha/beta, the tridiagonal square root
+// and the convergence estimate.  Reductions are single pass: each block
+// writes a partial, the last block to arrive (agent-scope counter that it
+// resets itself) sums the partials in a fixed order, so results are bitwise
+// reproducible and no host round trip is needed until the convergence test.
+//
+// Tridiagonal square root (replaces host LAPACKE ?steqr + cblas ?gemv,
+// LanczosAlgorithm.cu:48-85): for the SPD tridiagonal T,
+//     T^{1/2} e1 = T T^{-1/2} e1,   T^{-1/2} = (2/pi) Int_R e^u (e^{2u} I + T)^{-1} du,
+// evaluated with the trapezoidal rule in u (spacing 0.3, integrand analytic in
+// the strip |Im u| < pi/2, so the error is ~e^{-pi^2/0.3}), truncated 33 units
+// beyond [log sqrt(lmin), log sqrt(lmax)].  Each quadrature node is one
+// shifted tridiagonal solve (Thomas algorithm), one node per thread; the
+// spectrum bounds come from Gershgorin (upper) and Sturm-count multisection
+// (lower).  A negative eigenvalue (non-SPD operator) yields NaN, which the
+// convergence test reports as the reference does (LanczosAlgorithm.cu:217-220).
+#include <cmath>
+
+#include "kernels.hpp"
+#include "lm_common.hpp"
+#include "philox.hpp"
+
+namespace lm {
+
+// ---- Philox normal fill ------------------------------------------------------
+__global__ void k_normal_fill(double* __restrict__ out, int64_t n, uint32_t k0, uint32_t k1,
+                              uint32_t s0, uint32_t s1) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (2 * g >= n) return;
+  U32x4 c;
+  c.v[0] = (uint32_t)g;
+  c.v[1] = (uint32_t)((uint64_t)g >> 32);
+  c.v[2] = s0;
+  c.v[3] = s1;
+  const U32x4 r = philox4x32_10(c, k0, k1);
+  const double u1 = philox_u01(r.v[0], r.v[1]);
+  const double u2 = philox_u01(r.v[2], r.v[3]);
+  const double rad = sqrt(-2.0 * log(u1));
+  double sn, cs;
+  sincospi(2.0 * u2, &sn, &cs);
+  out[2 * g] = rad * cs;
+  if (2 * g + 1 < n) out[2 * g + 1] = rad * sn;
+}
+
+void launch_normal_fill(double* out, int64_t n, uint64_t seed, uint64_t stream, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t pairs = (n + 1) / 2;
+  const int nt = 256;
+  hipLaunchKernelGGL(k_normal_fill, dim3((unsigned)((pairs + nt - 1) / nt)), dim3(nt), 0, s, out,
+                     n, (uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)stream,
+                     (uint32_t)(stream >> 32));
+  LM_HIP(hipGetLastError());
+}
+
+// ---- single-pass reduction helpers -----------------------------------------
+constexpr int kRT = 256;  // threads per reduction block
+
+// Sum over the block; the result is valid in every thread of wave 0.
+__device__ inline double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double t = (threadIdx.x < (blockDim.x >> 6)) ? sh[threadIdx.x] : 0.0;
+  if (wid == 0)
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+  return t;
+}
+
+// Publish this block's partials; returns true in every thread of the last
+// block to arrive, which then sees all partials.
+__device__ inline bool publish_and_arrive(double* partial, int nvals, const double* vals,
+                                          unsigned int* counter) {
+  __shared__ bool s_last;
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < nvals; ++q)
+      __hip_atomic_store(&partial[q * kReduceBlocks + blockIdx.x], vals[q], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned prev =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  const bool last = s_last;
+  if (last) __threadfence();
+  return last;
+}
+
+// In the last block: fixed-order sum of the nb partials of slot q.
+__device__ inline double gather_partials(const double* partial, int q, int nb, double* sh) {
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    v += __hip_atomic_load(&partial[q * kReduceBlocks + b], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  return block_sum(v, sh);
+}
+
+static inline int reduce_blocks(int64_t n) {
+  int64_t b = (n + kRT - 1) / kRT;
+  if (b > kReduceBlocks) b = kReduceBlocks;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// ---- Lanczos steps ------------------------------------------------------------
+// ||z|| -> sc->normz
+__global__ __launch_bounds__(kRT) void k_norm_start(const double* __restrict__ z, int64_t n,
+                                                   double* partial, unsigned* counter,
+                                                   LanczosScalars* sc) {
+  __shared__ double sh[kRT / 64];
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT)
+    v += z[i] * z[i];
+  v = block_sum(v, sh);
+  if (publish_and_arrive(partial, 1, &v, counter)) {
+    const double t = gather_partials(partial, 0, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      sc->normz = sqrt(t);
+      *counter = 0;
+    }
+  }
+}
+
+__global__ void k_scale_start(const double* __restrict__ z, double* __restrict__ v0, int64_t n,
+                              const LanczosScalars* sc) {
+  const double inv = 1.0 / sc->normz;  // LanczosAlgorithm.cu:115-117
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v0[i] = z[i] * inv;
+}
+
+void launch_lanczos_start(const double* z, double* v0, int64_t n, ReduceScratch rs,
+                          LanczosScalars* sc, hipStream_t s) {
+  const int nb = reduce_blocks(n);
+  hipLaunchKernelGGL(k_norm_start, dim3(nb), dim3(kRT), 0, s, z, n, rs.partial, rs.counter, sc);
+  LM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scale_start, dim3(nb), dim3(kRT), 0, s, z, v0, n, sc);
+  LM_HIP(hipGetLastError());
+}
+
+// w -= beta[i-1] v_prev ; alpha[i] = w . v_i   (LanczosAlgorithm.cu:128-134)
+__global__ __launch_bounds__(kRT) void k_alpha(double* __restrict__ w,
+                                              const double* __restrict__ vi,
+                                              const double* __restrict__ vp, int64_t n, int i,
+                                              double* partial, unsigned* counter,
+                                              LanczosScalars* sc) {
+  __shared__ double sh[kRT / 64];
+  const double b = (i > 0) ? sc->beta[i - 1] : 0.0;
+  double v = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * kRT + threadIdx.x; k < n; k += (int64_t)gridDim.x * kRT) {
+    double wk = w[k];
+    if (i > 0) {
+      wk -= b * vp[k];
+      w[k] = wk;
+    }
+    v += wk * vi[k];
+  }
+  v = block_sum(v, sh);
+  if (publish_and_arrive(partial, 1, &v, counter)) {
+    const double t = gather_partials(partial, 0, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      sc->alpha[i] = t;
+      *counter = 0;
+    }
+  }
+}
+
+void launch_lanczos_alpha(double* w, const double* v_i, const double* v_prev, int64_t n, int i,
+                          ReduceScratch rs, LanczosScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_alpha, dim3(reduce_blocks(n)), dim3(kRT), 0, s, w, v_i, v_prev, n, i,
+                     rs.partial, rs.counter, sc);
+  LM_HIP(hipGetLastError());
+}
+
+// w -= alpha[i] v_i ; beta[i] = ||w||, zeroed below 1e-3 alpha/||z||
+// (LanczosAlgorithm.cu:136-143)
+__global__ __launch_bounds__(kRT) void k_beta(double* __restrict__ w,
+                                             const double* __restrict__ vi, int64_t n, int i,
+                                             double* partial, unsigned* counter,
+                                             LanczosScalars* sc) {
+  __shared__ double sh[kRT / 64];
+  const double al = sc->alpha[i];
+  double v = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * kRT + threadIdx.x; k < n; k += (int64_t)gridDim.x * kRT) {
+    const double wk = w[k] - al * vi[k];
+    w[k] = wk;
+    v += wk * wk;
+  }
+  v = block_sum(v, sh);
+  if (publish_and_arrive(partial, 1, &v, counter)) {
+    const double t = gather_partials(partial, 0, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      double be = sqrt(t);
+      const double tol = 1e-3 * sc->alpha[i] / sc->normz;
+      if (be < tol) be = 0.0;
+      sc->beta[i] = be;
+      *counter = 0;
+    }
+  }
+}
+
+void launch_lanczos_beta(double* w, const double* v_i, int64_t n, int i, ReduceScratch rs,
+                         LanczosScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_beta, dim3(reduce_blocks(n)), dim3(kRT), 0, s, w, v_i, n, i, rs.partial,
+                     rs.counter, sc);
+  LM_HIP(hipGetLastError());
+}
+
+// v_{i+1} = w / beta[i], or e1 after a breakdown (LanczosAlgorithm.cu:144-152)
+__global__ void k_next(const double* __restrict__ w, double* __restrict__ vn, int64_t n, int i,
+                       const LanczosScalars* sc) {
+  const double be = sc->beta[i];
+  const double inv = be > 0.0 ? 1.0 / be : 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    vn[k] = be > 0.0 ? w[k] * inv : (k == 0 ? 1.0 : 0.0);
+}
+
+void launch_lanczos_next(const double* w, double* v_next, int64_t n, int i,
+                         const LanczosScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_next, dim3(reduce_blocks(n)), dim3(kRT), 0, s, w, v_next, n, i, sc);
+  LM_HIP(hipGetLastError());
+}
+
+// ---- tridiagonal square root (one workgroup) -----------------------------------
+constexpr int kTS = kTsqrtNodes;  // threads = quadrature nodes
+
+__device__ inline int sturm_count(const double* a, const double* b2, int m, double x,
+                                  double pivmin) {
+  int cnt = 0;
+  double q = a[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  cnt += q < 0.0;
+  for (int i = 1; i < m; ++i) {
+    q = (a[i] - x) - b2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0.0;
+  }
+  return cnt;
+}
+
+__global__ __launch_bounds__(kTS) void k_tridiag_sqrt(const LanczosScalars* __restrict__ sc,
+                                                      int m, double* __restrict__ s_out,
+                                                      double* __restrict__ work) {
+  __shared__ double a[256], b[256], b2[256], xs[256];
+  __shared__ double red[kTS / 64];
+  __shared__ int ired[kTS / 64];
+  __shared__ double s_lo, s_hi;
+  __shared__ int s_bad;
+  const int t = threadIdx.x;
+  for (int i = t; i < m; i += kTS) {
+    a[i] = sc->alpha[i];
+    b[i] = (i < m - 1) ? sc->beta[i] : 0.0;
+    b2[i] = b[i] * b[i];
+  }
+  __syncthreads();
+  // Gershgorin interval
+  double glo = 1e300, ghi = -1e300, bmax = 0.0;
+  for (int i = t; i < m; i += kTS) {
+    const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
+    glo = fmin(glo, a[i] - r);
+    ghi = fmax(ghi, a[i] + r);
+    bmax = fmax(bmax, b2[i]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    glo = fmin(glo, __shfl_xor(glo, o));
+    ghi = fmax(ghi, __shfl_xor(ghi, o));
+    bmax = fmax(bmax, __shfl_xor(bmax, o));
+  }
+  __shared__ double rlo[kTS / 64], rhi[kTS / 64], rb[kTS / 64];
+  if ((t & 63) == 0) {
+    rlo[t >> 6] = glo;
+    rhi[t >> 6] = ghi;
+    rb[t >> 6] = bmax;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double lo = rlo[0], hi = rhi[0], bm = rb[0];
+    for (int w = 1; w < kTS / 64; ++w) {
+      lo = fmin(lo, rlo[w]);
+      hi = fmax(hi, rhi[w]);
+      bm = fmax(bm, rb[w]);
+    }
+    const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
+    s_lo = lo - pad;
+    s_hi = hi + pad;
+    red[0] = bm;
+    const double pivmin = 1e-290 * fmax(1.0, bm);
+    s_bad = sturm_count(a, b2, m, 0.0, pivmin) > 0;  // an eigenvalue below zero
+  }
+  __syncthreads();
+  const double pivmin = 1e-290 * fmax(1.0, red[0]);
+  if (s_bad) {
+    for (int i = t; i < m; i += kTS) s_out[i] = NAN;
+    return;
+  }
+  // Multisection for the smallest eigenvalue: 4 rounds over kTS points.
+  double L = s_lo, H = s_hi;
+  for (int round = 0; round < 4; ++round) {
+    const double x = L + (H - L) * (double)(t + 1) / (double)kTS;
+    const int c = sturm_count(a, b2, m, x, pivmin);
+    int first = (c >= 1) ? t : kTS;
+    for (int o = 32; o > 0; o >>= 1) first = min(first, __shfl_xor(first, o));
+    __syncthreads();
+    if ((t & 63) == 0) ired[t >> 6] = first;
+    __syncthreads();
+    int f = ired[0];
+    for (int w = 1; w < kTS / 64; ++w) f = min(f, ired[w]);
+    if (f >= kTS) f = kTS - 1;
+    const double nH = L + (H - L) * (double)(f + 1) / (double)kTS;
+    const double nL = L + (H - L) * (double)f / (double)kTS;
+    L = nL;
+    H = nH;
+  }
+  const double lmin = fmax(L, 1e-300 + 1e-16 * s_hi);
+  const double lmax = s_hi;
+  // Trapezoidal nodes u_k = u0 + k h
+  double h = 0.3;
+  const double u0 = 0.5 * log(lmin) - 33.0;
+  const double u1 = 0.5 * log(lmax) + 33.0;
+  int K = (int)ceil((u1 - u0) / h) + 1;
+  if (K > kTS) {
+    K = kTS;
+    h = (u1 - u0) / (double)(kTS - 1);
+  }
+  double* cp = work;                      // [m][kTS]
+  double* dp = work + 256 * kTS;          // [m][kTS]
+  if (t < K) {
+    const double u = u0 + h * (double)t;
+    const double sig = exp(2.0 * u);
+    const double wgt = (2.0 * h / M_PI) * exp(u);
+    // Thomas: (sig I + T) y = e1
+    double den = sig + a[0];
+    double c_prev = (m > 1) ? b[0] / den : 0.0;
+    double d_prev = 1.0 / den;
+    cp[t] = c_prev;
+    dp[t] = d_prev;
+    for (int i = 1; i < m; ++i) {
+      den = sig + a[i] - b[i - 1] * c_prev;
+      const double inv = 1.0 / den;
+      c_prev = (i < m - 1) ? b[i] * inv : 0.0;
+      d_prev = -b[i - 1] * d_prev * inv;
+      cp[i * kTS + t] = c_prev;
+      dp[i * kTS + t] = d_prev;
+    }
+    double y = d_prev;
+    dp[(m - 1) * kTS + t] = wgt * y;
+    for (int i = m - 2; i >= 0; --i) {
+      y = dp[i * kTS + t] - cp[i * kTS + t] * y;
+      dp[i * kTS + t] = wgt * y;
+    }
+  }
+  __syncthreads();
+  // x_j = sum_k dp[j][k] : one wave per component, fixed order
+  const int lane = t & 63, wid = t >> 6;
+  for (int j = wid; j < m; j += kTS / 64) {
+    double v = 0.0;
+    for (int k = lane; k < K; k += 64) v += dp[j * kTS + k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) xs[j] = v;
+  }
+  __syncthreads();
+  for (int j = t; j < m; j += kTS) {
+    double v = a[j] * xs[j];
+    if (j > 0) v += b[j - 1] * xs[j - 1];
+    if (j < m - 1) v += b[j] * xs[j + 1];
+    s_out[j] = v;
+  }
+}
+
+void launch_tridiag_sqrt(const LanczosScalars* sc, int m, double* s_out, double* work,
+                         hipStream_t s) {
+  if (m < 1 || m > 256) throw Error("[libmobility_amd] tridiagonal size out of range");
+  hipLaunchKernelGGL(k_tridiag_sqrt, dim3(1), dim3(kTS), 0, s, sc, m, s_out, work);
+  LM_HIP(hipGetLastError());
+}
+
+// y = normz V[:, :m] s ; old <- old - y ; err = ||old - y|| / ||old||
+// (LanczosAlgorithm.cu:158-167 and :202-222)
+__global__ __launch_bounds__(kRT) void k_estimate(const double* __restrict__ V, int64_t ldv,
+                                                 int m, const double* __restrict__ coef,
+                                                 double* __restrict__ y, double* __restrict__ old,
+                                                 int64_t n, double* partial, unsigned* counter,
+                                                 LanczosScalars* sc) {
+  __shared__ double sc_s[256];
+  __shared__ double sh[kRT / 64];
+  const double nz = sc->normz;
+  for (int k = threadIdx.x; k < m; k += kRT) sc_s[k] = nz * coef[k];
+  __syncthreads();
+  double num = 0.0, den = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+    double v = 0.0;
+    for (int k = 0; k < m; ++k) v += V[(int64_t)k * ldv + i] * sc_s[k];
+    y[i] = v;
+    const double o = old[i];
+    const double d = o - v;
+    old[i] = d;
+    num += d * d;
+    den += o * o;
+  }
+  num = block_sum(num, sh);
+  den = block_sum(den, sh);
+  double vals[2] = {num, den};
+  if (publish_and_arrive(partial, 2, vals, counter)) {
+    const double tn = gather_partials(partial, 0, gridDim.x, sh);
+    const double td = gather_partials(partial, 1, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      sc->errnum = tn;
+      sc->errden = td;
+      sc->err = fabs(sqrt(tn) / sqrt(td));
+      *counter = 0;
+    }
+  }
+}
+
+void launch_lanczos_estimate(const double* V, int64_t ldv, int m, const double* s_coef, double* y,
+                             const double* old, int64_t n, ReduceScratch rs, LanczosScalars* sc,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_estimate, dim3(reduce_blocks(n)), dim3(kRT), 0, s, V, ldv, m, s_coef, y,
+                     const_cast<double*>(old), n, rs.partial, rs.counter, sc);
+  LM_HIP(hipGetLastError());
+}
+
+// ---- small vector kernels ----------------------------------------------------------
+__global__ void k_axpy_acc(double* __restrict__ out, const double* __restrict__ y, double p,
+                           int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] += p * y[i];
+}
+void launch_axpy_accumulate(double* out, const double* y, double pref, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy_acc, dim3(reduce_blocks(n) * 4), dim3(kRT), 0, s, out, y, pref, n);
+  LM_HIP(hipGetLastError());
+}
+
+__global__ void k_axpy_to(double* __restrict__ dst, const double* __restrict__ src,
+                          const double* __restrict__ w, double c, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i] + c * w[i];
+}
+void launch_axpy_to(double* dst, const double* src, const double* w, double c, int64_t n,
+                    hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy_to, dim3(reduce_blocks(n) * 4), dim3(kRT), 0, s, dst, src, w, c, n);
+  LM_HIP(hipGetLastError());
+}
+
+__global__ void k_diff_acc(double* __restrict__ out, const double* __restrict__ a,
+                           const double* __restrict__ b, double p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] += p * (a[i] - b[i]);
+}
+void launch_diff_accumulate(double* out, const double* a, const double* b, double pref, int64_t n,
+                            hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_diff_acc, dim3(reduce_blocks(n) * 4), dim3(kRT), 0, s, out, a, b, pref, n);
+  LM_HIP(hipGetLastError());
+}
+
+__global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+void launch_copy(double* dst, const double* src, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy, dim3(reduce_blocks(n) * 4), dim3(kRT), 0, s, dst, src, n);
+  LM_HIP(hipGetLastError());
+}
+
+}  // namespace lm

@@ -1,0 +1,285 @@
+// gfx950 kernels of the O(N^2) mobility product and of setPositions.
+//
+// Pair kernel layout
+//   * One workgroup = one PairItem: a tile of targets (256 x TPL, each lane
+//     owns TPL targets held in registers) against one chunk of sources of the
+//     same system.
+//   * Sources stream through LDS in tiles of 256 records; every lane reads the
+//     same source at the same time, so LDS reads are broadcasts.
+//   * FP32 mode: pair terms and the per-tile sums are FP32, each tile's sum is
+//     added into FP64 accumulators (error grows with the tile, not with N).
+//   * Partial results of the chunks of one target go to separate slabs and
+//     are summed by k_chunk_reduce in a fixed order (bitwise reproducible).
+//   * Items are ordered target-tile-major, chunk-minor.  Workgroups are
+//     dispatched round-robin over the 8 XCDs, so with 8 chunks every XCD
+//     reads one chunk of the sources and its L2 holds only that chunk.
+#include <type_traits>
+
+#include "kernels.hpp"
+#include "lm_common.hpp"
+#include "pair_math.hpp"
+
+namespace lm {
+
+template <class R>
+struct Rec;
+template <>
+struct Rec<float> {
+  using T = float4;
+};
+template <>
+struct Rec<double> {
+  using T = double4;
+};
+
+// A record far from everything: padding sources (zero force) and idle target
+// lanes evaluate finite arithmetic against it and contribute nothing.
+template <class R4>
+__device__ inline R4 far_record() {
+  R4 r;
+  r.x = 1.0e6f;
+  r.y = 1.0e6f;
+  r.z = 1.0e6f;
+  r.w = 1.0f;
+  return r;
+}
+
+template <class R, bool WALL, int BLK, int TPL>
+__global__ __launch_bounds__(kPairThreads) void k_pair(
+    const PairItem* __restrict__ items, const typename Rec<R>::T* __restrict__ pos,
+    const double* __restrict__ F, const double* __restrict__ T, double* __restrict__ U,
+    double* __restrict__ W, int64_t slab, double t0, double rt0, double r0) {
+  using R4 = typename Rec<R>::T;
+  constexpr bool kNeedF = (BLK & (kUF | kWF)) != 0;
+  constexpr bool kNeedT = (BLK & (kUT | kWT)) != 0;
+  constexpr bool kAngular = (BLK & (kWT | kWF)) != 0;
+  __shared__ R4 s_pos[kPairThreads];
+  __shared__ R4 s_f[kNeedF ? kPairThreads : 1];
+  __shared__ R4 s_t[kNeedT ? kPairThreads : 1];
+
+  const PairItem it = items[blockIdx.x];
+  const int tid = threadIdx.x;
+
+  R4 pi[TPL];
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+    const int loc = tid + k * kPairThreads;
+    pi[k] = (loc < it.ntgt) ? pos[it.tgt0 + loc] : far_record<R4>();
+  }
+  PairAcc<R> acc[TPL];
+  PairAcc<double> acc64[TPL];
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+    acc_zero(acc[k]);
+    acc_zero(acc64[k]);
+  }
+
+  for (int64_t base = it.src0; base < it.src1; base += kPairThreads) {
+    const int64_t j = base + tid;
+    __syncthreads();
+    if (j < it.src1) {
+      s_pos[tid] = pos[j];
+      if (kNeedF) s_f[tid] = R4{R(F[3 * j]), R(F[3 * j + 1]), R(F[3 * j + 2]), R(0)};
+      if (kNeedT) s_t[tid] = R4{R(T[3 * j]), R(T[3 * j + 1]), R(T[3 * j + 2]), R(0)};
+    } else {
+      s_pos[tid] = far_record<R4>();
+      if (kNeedF) s_f[tid] = R4{R(0), R(0), R(0), R(0)};
+      if (kNeedT) s_t[tid] = R4{R(0), R(0), R(0), R(0)};
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int s = 0; s < kPairThreads; ++s) {
+      const R4 pj = s_pos[s];
+      R4 fj, tj;
+      if (kNeedF) fj = s_f[s]; else fj = R4{R(0), R(0), R(0), R(0)};
+      if (kNeedT) tj = s_t[s]; else tj = R4{R(0), R(0), R(0), R(0)};
+#pragma unroll
+      for (int k = 0; k < TPL; ++k) {
+        if (WALL)
+          wall_pair<R, BLK>(pi[k].x, pi[k].y, pi[k].z, pi[k].w, pj.x, pj.y, pj.z, pj.w, fj.x,
+                            fj.y, fj.z, tj.x, tj.y, tj.z, acc[k]);
+        else
+          open_pair<R, BLK>(pi[k].x - pj.x, pi[k].y - pj.y, pi[k].z - pj.z, fj.x, fj.y, fj.z,
+                            tj.x, tj.y, tj.z, acc[k]);
+      }
+    }
+    if constexpr (std::is_same<R, float>::value) {
+#pragma unroll
+      for (int k = 0; k < TPL; ++k) {
+        acc64[k].uf.x += acc[k].uf.x; acc64[k].uf.y += acc[k].uf.y; acc64[k].uf.z += acc[k].uf.z;
+        acc64[k].ut.x += acc[k].ut.x; acc64[k].ut.y += acc[k].ut.y; acc64[k].ut.z += acc[k].ut.z;
+        acc64[k].wt.x += acc[k].wt.x; acc64[k].wt.y += acc[k].wt.y; acc64[k].wt.z += acc[k].wt.z;
+        acc64[k].wf.x += acc[k].wf.x; acc64[k].wf.y += acc[k].wf.y; acc64[k].wf.z += acc[k].wf.z;
+        acc_zero(acc[k]);
+      }
+    }
+  }
+  if constexpr (std::is_same<R, double>::value) {
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) acc64[k] = acc[k];
+  }
+  double* Uo = U + (int64_t)it.chunk * slab;
+  double* Wo = kAngular ? W + (int64_t)it.chunk * slab : nullptr;
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+    const int loc = tid + k * kPairThreads;
+    if (loc < it.ntgt) {
+      const int64_t i = it.tgt0 + loc;
+      const PairAcc<double>& a = acc64[k];
+      Uo[3 * i + 0] = t0 * a.uf.x + rt0 * a.ut.x;
+      Uo[3 * i + 1] = t0 * a.uf.y + rt0 * a.ut.y;
+      Uo[3 * i + 2] = t0 * a.uf.z + rt0 * a.ut.z;
+      if (kAngular) {
+        Wo[3 * i + 0] = r0 * a.wt.x + rt0 * a.wf.x;
+        Wo[3 * i + 1] = r0 * a.wt.y + rt0 * a.wf.y;
+        Wo[3 * i + 2] = r0 * a.wt.z + rt0 * a.wf.z;
+      }
+    }
+  }
+}
+
+template <class R, bool WALL, int BLK, int TPL>
+static void launch_pair_t(const PairArgs& a, hipStream_t s) {
+  using R4 = typename Rec<R>::T;
+  hipLaunchKernelGGL((k_pair<R, WALL, BLK, TPL>), dim3(a.nitems), dim3(kPairThreads), 0, s,
+                     a.items, static_cast<const R4*>(a.pos), a.F, a.T, a.U, a.W, a.slab, a.t0,
+                     a.rt0, a.r0);
+  LM_HIP(hipGetLastError());
+}
+
+template <class R, bool WALL, int BLK>
+static void launch_pair_tpl(const PairArgs& a, hipStream_t s) {
+  if (a.tpl == 2)
+    launch_pair_t<R, WALL, BLK, 2>(a, s);
+  else
+    launch_pair_t<R, WALL, BLK, 1>(a, s);
+}
+
+template <class R, bool WALL>
+static void launch_pair_blk(const PairArgs& a, hipStream_t s) {
+  switch (a.blocks) {
+    case kUF: launch_pair_tpl<R, WALL, kUF>(a, s); break;
+    case kUF | kWF: launch_pair_tpl<R, WALL, kUF | kWF>(a, s); break;
+    case kUT | kWT: launch_pair_tpl<R, WALL, kUT | kWT>(a, s); break;
+    case kUF | kUT | kWT | kWF: launch_pair_tpl<R, WALL, kUF | kUT | kWT | kWF>(a, s); break;
+    default: throw Error("[libmobility_amd] unsupported block combination");
+  }
+}
+
+void launch_pair(const PairArgs& a, hipStream_t s) {
+  if (a.nitems <= 0) return;
+  if (a.fp32) {
+    if (a.wall) launch_pair_blk<float, true>(a, s);
+    else launch_pair_blk<float, false>(a, s);
+  } else {
+    if (a.wall) launch_pair_blk<double, true>(a, s);
+    else launch_pair_blk<double, false>(a, s);
+  }
+}
+
+__global__ void k_chunk_reduce(const double* __restrict__ part, int nch, int64_t n,
+                               double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int c = 0; c < nch; ++c) s += part[(int64_t)c * n + i];
+  out[i] = s;
+}
+
+void launch_chunk_reduce(const double* part, int nch, int64_t n, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  const int nt = 256;
+  hipLaunchKernelGGL(k_chunk_reduce, dim3((unsigned)((n + nt - 1) / nt)), dim3(nt), 0, s, part,
+                     nch, n, out);
+  LM_HIP(hipGetLastError());
+}
+
+// ---- setPositions -----------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_system_bbox(const double* __restrict__ pos,
+                                                     const int64_t* __restrict__ off,
+                                                     double wall_shift,
+                                                     double* __restrict__ bbox) {
+  const int s = blockIdx.x;
+  const int64_t b = off[s], e = off[s + 1];
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    for (int c = 0; c < 3; ++c) {
+      double v = pos[3 * i + c];
+      if (c == 2) v -= wall_shift;
+      mn[c] = fmin(mn[c], v);
+      mx[c] = fmax(mx[c], v);
+    }
+  }
+  __shared__ double sh[6][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int c = 0; c < 3; ++c) {
+    for (int o = 32; o > 0; o >>= 1) {
+      mn[c] = fmin(mn[c], __shfl_xor(mn[c], o));
+      mx[c] = fmax(mx[c], __shfl_xor(mx[c], o));
+    }
+    if (lane == 0) {
+      sh[c][wid] = mn[c];
+      sh[3 + c][wid] = mx[c];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int c = threadIdx.x;
+    double v = sh[c][0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = (c < 3) ? fmin(v, sh[c][w]) : fmax(v, sh[c][w]);
+    bbox[6 * s + c] = v;
+  }
+}
+
+void launch_system_bbox(const double* pos, const int64_t* sys_off, int nsys, double wall_shift,
+                        double* bbox, hipStream_t s) {
+  if (nsys <= 0) return;
+  hipLaunchKernelGGL(k_system_bbox, dim3(nsys), dim3(256), 0, s, pos, sys_off, wall_shift, bbox);
+  LM_HIP(hipGetLastError());
+}
+
+__global__ void k_make_records(const double* __restrict__ pos, int64_t N,
+                               const int64_t* __restrict__ off, int nsys,
+                               const double* __restrict__ bbox, int wall, double wall_shift,
+                               double a, double* __restrict__ shifted, double4* __restrict__ rec64,
+                               float4* __restrict__ rec32) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int lo = 0, hi = nsys;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid;
+  }
+  const double* bb = bbox + 6 * lo;
+  const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2] - (wall ? wall_shift : 0.0);
+  if (shifted) {
+    shifted[3 * i] = x;
+    shifted[3 * i + 1] = y;
+    shifted[3 * i + 2] = z;
+  }
+  const double ia = 1.0 / a;
+  const double cx = 0.5 * (bb[0] + bb[3]), cy = 0.5 * (bb[1] + bb[4]), cz = 0.5 * (bb[2] + bb[5]);
+  if (wall) {
+    const double h = fmax(z, a) * ia;
+    const double b = fmin(fmax(z * ia, 0.0), 1.0);
+    rec64[i] = double4{x * ia, y * ia, h, b};
+    rec32[i] = float4{(float)((x - cx) * ia), (float)((y - cy) * ia), (float)h, (float)b};
+  } else {
+    rec64[i] = double4{x * ia, y * ia, z * ia, 1.0};
+    rec32[i] = float4{(float)((x - cx) * ia), (float)((y - cy) * ia), (float)((z - cz) * ia), 1.0f};
+  }
+}
+
+void launch_make_records(const double* pos, int64_t N, const int64_t* sys_off, int nsys,
+                         const double* bbox, int wall, double wall_shift, double a,
+                         double* shifted, void* rec64, void* rec32, hipStream_t s) {
+  if (N <= 0) return;
+  const int nt = 256;
+  hipLaunchKernelGGL(k_make_records, dim3((unsigned)((N + nt - 1) / nt)), dim3(nt), 0, s, pos, N,
+                     sys_off, nsys, bbox, wall, wall_shift, a, shifted,
+                     static_cast<double4*>(rec64), static_cast<float4*>(rec32));
+  LM_HIP(hipGetLastError());
+}
+
+}  // namespace lm

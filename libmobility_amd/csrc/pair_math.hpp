@@ -1,0 +1,275 @@
+// Pair arithmetic of the NBody mobility operator, shared by the gfx950 kernels
+// and by the host-side arithmetic self-check (lm_selftest_pair_block_host).
+//
+// Geometry is in units of the hydrodynamic radius a.  Position records are
+// (x, y, z, b): for open boundaries z is the coordinate and b = 1; for the
+// bottom wall z is the clamped height max(z, a)/a and b = clamp(z/a, 0, 1) is
+// the damping factor of Usabiaga et al. 2017 (App. 1).
+//
+// The four 3x3 blocks follow solvers/NBody/extra/hydrodynamicKernels.cuh:
+//   open RPY       UF :41-57,143-150   WT :63-86,152-159   UT/WF :89-118,161-173
+//   wall terms     UF :187-249  WT :253-282  UT :287-319,416-431  WF :324-359
+//   damping/clamp  :362-390
+// The wall corrections are regrouped into vector form (fewer operations per
+// pair than the reference's entry-by-entry 3x3 application):
+//   UF: c = f1 v + f2 e (e.v) + f3 v_z e + z^ (f4 (e.v) + f5 v_z)
+//   WT: c = g1 v + g2 e (e.v) + g4 p (e_y, -e_x, 0) + z^ g3 (e.v),  p = e_y v_x - e_x v_y
+//   WF: c = (k3 e_y (e.v) - (k1 e_z + k4) v_y + (k1 + k2) e_y v_z,
+//            (k1 e_z + k4) v_x - k3 e_x (e.v) - (k1 + k2) e_x v_z,  -k1 p)
+//   UT: same factors with R' = (-d_x, -d_y, R_z) and h = h_i
+// with R = (d_x, d_y, h_i + h_j), e = R/|R|.  The regrouping is exact algebra;
+// the CPU oracle (explicit reference matrices) checks it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace lm {
+
+enum BlockBits : int { kUF = 1, kUT = 2, kWT = 4, kWF = 8 };
+
+template <class R>
+struct Vec3 {
+  R x, y, z;
+};
+
+template <class R>
+struct PairAcc {
+  Vec3<R> uf, ut, wt, wf;
+};
+
+template <class R>
+__host__ __device__ inline void acc_zero(PairAcc<R>& a) {
+  a.uf = {R(0), R(0), R(0)};
+  a.ut = {R(0), R(0), R(0)};
+  a.wt = {R(0), R(0), R(0)};
+  a.wf = {R(0), R(0), R(0)};
+}
+
+__host__ __device__ inline float lm_rsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / __builtin_sqrtf(x);
+#endif
+}
+__host__ __device__ inline double lm_rsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return rsqrt(x);
+#else
+  return 1.0 / __builtin_sqrt(x);
+#endif
+}
+__host__ __device__ inline float lm_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+__host__ __device__ inline double lm_rcp(double x) { return 1.0 / x; }
+__host__ __device__ inline float lm_sqrt(float x) { return __builtin_sqrtf(x); }
+__host__ __device__ inline double lm_sqrt(double x) { return __builtin_sqrt(x); }
+
+// Open-boundary RPY coefficients for r2 = |d|^2 (scaled units).
+//   fuf, guf : f and g/r^2 of M_UF
+//   fwt, gwt : f and g/r^2 of M_WT
+//   c        : coupling coefficient of U = c (T x d), W = c (F x d)
+// Far field (r >= 2) uses rsqrt; the overlap branch (r < 2, including the
+// self pair r = 0) is the regularised RPY of hydrodynamicKernels.cuh:51-55,
+// :73-84, :97-100.  It is a divergent branch so that waves with no near pair
+// skip it.
+template <class R, int BLK>
+__host__ __device__ inline void rpy_coeffs(R r2, R& fuf, R& guf, R& fwt, R& gwt, R& c) {
+  const R ir = lm_rsqrt(r2 >= R(4) ? r2 : R(4));
+  const R ir2 = ir * ir;
+  const R ir3 = ir * ir2;
+  if (BLK & kUF) {
+    fuf = ir * (R(0.75) + R(0.5) * ir2);
+    guf = ir3 * (R(0.75) - R(1.5) * ir2);
+  }
+  if (BLK & kWT) {
+    fwt = R(-0.5) * ir3;
+    gwt = R(1.5) * ir3 * ir2;
+  }
+  if (BLK & (kUT | kWF)) c = ir3;
+  if (r2 < R(4)) {
+    const R r = lm_sqrt(r2);
+    const R ir_n = r > R(0) ? R(1) / r : R(0);
+    if (BLK & kUF) {
+      fuf = R(1) - R(0.28125) * r;
+      guf = R(0.09375) * ir_n;
+    }
+    if (BLK & kWT) {
+      fwt = R(1) - R(0.84375) * r + R(0.078125) * r * r * r;
+      gwt = r > R(0) ? (R(0.28125) * ir_n - R(0.046875) * r) : R(0);
+    }
+    if (BLK & (kUT | kWF)) c = R(0.5) * (R(1) - R(0.375) * r);
+  }
+}
+
+// Open-boundary contribution of one source (f = force, t = torque) to one
+// target, added into a (dimensionless; the caller applies t0, rt0, r0).
+template <class R, int BLK>
+__host__ __device__ inline void open_pair(R dx, R dy, R dz, R fx, R fy, R fz, R tx, R ty, R tz,
+                                          PairAcc<R>& a) {
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  R fuf, guf, fwt, gwt, c;
+  rpy_coeffs<R, BLK>(r2, fuf, guf, fwt, gwt, c);
+  if (BLK & kUF) {
+    const R s = guf * (dx * fx + dy * fy + dz * fz);
+    a.uf.x += fuf * fx + s * dx;
+    a.uf.y += fuf * fy + s * dy;
+    a.uf.z += fuf * fz + s * dz;
+  }
+  if (BLK & kWT) {
+    const R s = gwt * (dx * tx + dy * ty + dz * tz);
+    a.wt.x += fwt * tx + s * dx;
+    a.wt.y += fwt * ty + s * dy;
+    a.wt.z += fwt * tz + s * dz;
+  }
+  if (BLK & kUT) {  // c (T x d)
+    a.ut.x += c * (ty * dz - tz * dy);
+    a.ut.y += c * (tz * dx - tx * dz);
+    a.ut.z += c * (tx * dy - ty * dx);
+  }
+  if (BLK & kWF) {  // c (F x d)
+    a.wf.x += c * (fy * dz - fz * dy);
+    a.wf.y += c * (fz * dx - fx * dz);
+    a.wf.z += c * (fx * dy - fy * dx);
+  }
+}
+
+// Bottom-wall contribution (open RPY + Rotne-Prager-Blake/Swan-Brady image
+// terms, damped by b_i b_j).  (xi, yi, hi, bi) and (xj, yj, hj, bj) are
+// position records as described at the top of this file.
+template <class R, int BLK>
+__host__ __device__ inline void wall_pair(R xi, R yi, R hi, R bi, R xj, R yj, R hj, R bj, R fx,
+                                          R fy, R fz, R tx, R ty, R tz, PairAcc<R>& a) {
+  const R dx = xi - xj, dy = yi - yj, dz = hi - hj;
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  const R bij = bi * bj;
+  R fuf, guf, fwt, gwt, c;
+  rpy_coeffs<R, BLK>(r2, fuf, guf, fwt, gwt, c);
+  Vec3<R> uf = {R(0), R(0), R(0)}, ut = uf, wt = uf, wf = uf;
+  if (r2 != R(0)) {
+    // Image system: R = (dx, dy, hi + hj), e = R/|R|, hh = h_j/R_z, hh' = h_i/R_z.
+    const R Rz = hi + hj;
+    const R iR = lm_rsqrt(dx * dx + dy * dy + Rz * Rz);
+    const R iR2 = iR * iR;
+    const R iR3 = iR2 * iR;
+    const R ex = dx * iR, ey = dy * iR, ez = Rz * iR;
+    const R ez2 = ez * ez;
+    const R iRz = lm_rcp(Rz);
+    if (BLK & kUF) {
+      const R hh = hj * iRz;
+      const R iR5 = iR3 * iR2;
+      const R t1 = R(5) * ez2 - R(1);  // -(1 - 5 ez2)
+      const R hq = hh * (R(1) - hh) * ez2;
+      const R f1 = R(-0.25) * (R(3) * (R(1) + R(2) * hq) * iR + R(2) * (R(1) - R(3) * ez2) * iR3 +
+                               R(2) * t1 * iR5);
+      const R f2 = R(-0.25) * (R(3) * (R(1) - R(6) * hq) * iR + R(6) * t1 * iR3 +
+                               R(10) * (R(1) - R(7) * ez2) * iR5);
+      const R f3 = R(0.5) * ez *
+                   (R(3) * hh * (R(1) - R(6) * (R(1) - hh) * ez2) * iR + R(6) * t1 * iR3 +
+                    R(10) * (R(2) - R(7) * ez2) * iR5);
+      const R f4 = R(0.5) * ez * (R(3) * hh * iR - R(10) * iR5);
+      const R f5 = -(R(3) * hh * hh * ez2 * iR + R(3) * ez2 * iR3 + (R(2) - R(15) * ez2) * iR5);
+      const R ev = ex * fx + ey * fy + ez * fz;
+      const R q = f2 * ev + f3 * fz;
+      uf.x = f1 * fx + q * ex;
+      uf.y = f1 * fy + q * ey;
+      uf.z = f1 * fz + q * ez + f4 * ev + f5 * fz;
+    }
+    if (BLK & kWT) {
+      const R g1 = R(0.5) * (R(1) - R(6) * ez2) * iR3;
+      const R g2 = R(-1.5) * iR3;
+      const R g3 = R(3) * iR3 * ez;
+      const R g4 = R(3) * iR3;
+      const R ev = ex * tx + ey * ty + ez * tz;
+      const R p = g4 * (ey * tx - ex * ty);
+      wt.x = g1 * tx + g2 * ev * ex + p * ey;
+      wt.y = g1 * ty + g2 * ev * ey - p * ex;
+      wt.z = g1 * tz + (g2 * ez + g3) * ev;
+    }
+    if (BLK & (kWF | kUT)) {
+      const R iR4 = iR2 * iR2;
+      const R k1 = iR2;
+      if (BLK & kWF) {
+        const R hh = hj * iRz;
+        const R k2 = R(2) * (R(6) * hh * ez2 * iR2 + (R(1) - R(10) * ez2) * iR4);
+        const R k3 = R(-2) * ez * (R(3) * hh * iR2 - R(5) * iR4);
+        const R k4 = R(-2) * ez * (hh * iR2 - iR4);
+        const R ev = ex * fx + ey * fy + ez * fz;
+        const R p = ey * fx - ex * fy;
+        const R kz = k1 * ez + k4;
+        const R k12 = k1 + k2;
+        wf.x = k3 * ey * ev - kz * fy + k12 * ey * fz;
+        wf.y = kz * fx - k3 * ex * ev - k12 * ex * fz;
+        wf.z = -k1 * p;
+      }
+      if (BLK & kUT) {
+        const R hh = hi * iRz;
+        const R k2 = R(2) * (R(6) * hh * ez2 * iR2 + (R(1) - R(10) * ez2) * iR4);
+        const R k3 = R(-2) * ez * (R(3) * hh * iR2 - R(5) * iR4);
+        const R k4 = R(-2) * ez * (hh * iR2 - iR4);
+        const R p = ey * tx - ex * ty;
+        const R kz = k1 * ez + k4;
+        ut.x = k3 * ex * p + kz * ty + k1 * ey * tz;
+        ut.y = k3 * ey * p - kz * tx - k1 * ex * tz;
+        ut.z = -(k1 + k2 + k3 * ez) * p;
+      }
+    }
+  } else {
+    // Self terms (r == 0), Swan & Brady B1, B2, B3 as in :190-202, :256-261,
+    // :290-294, :327-331.  h_i == h_j here.
+    const R ih = R(1) / hj;
+    const R ih2 = ih * ih;
+    const R ih3 = ih2 * ih;
+    if (BLK & kUF) {
+      const R ih5 = ih3 * ih2;
+      const R cp = R(-0.0625) * (R(9) * ih - R(2) * ih3 + ih5);
+      const R cz = R(-0.125) * (R(9) * ih - R(4) * ih3 + ih5);
+      uf.x = cp * fx;
+      uf.y = cp * fy;
+      uf.z = cz * fz;
+    }
+    if (BLK & kWT) {
+      wt.x = R(-0.3125) * ih3 * tx;
+      wt.y = R(-0.3125) * ih3 * ty;
+      wt.z = R(-0.125) * ih3 * tz;
+    }
+    const R c4 = R(0.125) * ih3 * ih;
+    if (BLK & kWF) {
+      wf.x = -c4 * fy;
+      wf.y = c4 * fx;
+    }
+    if (BLK & kUT) {
+      ut.x = c4 * ty;
+      ut.y = -c4 * tx;
+    }
+  }
+  // open RPY part
+  if (BLK & kUF) {
+    const R s = guf * (dx * fx + dy * fy + dz * fz);
+    a.uf.x += bij * (fuf * fx + s * dx + uf.x);
+    a.uf.y += bij * (fuf * fy + s * dy + uf.y);
+    a.uf.z += bij * (fuf * fz + s * dz + uf.z);
+  }
+  if (BLK & kWT) {
+    const R s = gwt * (dx * tx + dy * ty + dz * tz);
+    a.wt.x += bij * (fwt * tx + s * dx + wt.x);
+    a.wt.y += bij * (fwt * ty + s * dy + wt.y);
+    a.wt.z += bij * (fwt * tz + s * dz + wt.z);
+  }
+  if (BLK & kUT) {
+    a.ut.x += bij * (c * (ty * dz - tz * dy) + ut.x);
+    a.ut.y += bij * (c * (tz * dx - tx * dz) + ut.y);
+    a.ut.z += bij * (c * (tx * dy - ty * dx) + ut.z);
+  }
+  if (BLK & kWF) {
+    a.wf.x += bij * (c * (fy * dz - fz * dy) + wf.x);
+    a.wf.y += bij * (c * (fz * dx - fx * dz) + wf.y);
+    a.wf.z += bij * (c * (fx * dy - fy * dx) + wf.z);
+  }
+}
+
+}  // namespace lm

@@ -1,0 +1,624 @@
+// Host orchestration of the NBody solver.  Reference behaviour followed (paths
+// relative to the reference root):
+//   construction            solvers/NBody/mobility.h:37-47
+//   setParameters           solvers/NBody/mobility.h:78-97
+//   initialize + seeds      solvers/NBody/mobility.h:99-114, MobilityInterface.h:102-114
+//   setPositions            solvers/NBody/mobility.h:116-162
+//   Mdot + block selection  solvers/NBody/mobility.h:166-182, extra/NbodyRPY.cu:254-287
+//   sqrtMdotW               MobilityInterface.h:132-188, lanczos.h:43-58,
+//                           include/lanczos/LanczosAlgorithm.cu:173-232
+//   LangevinVelocities      MobilityInterface.h:191-206
+//   divM (RFD)              solvers/NBody/mobility.h:184-236,
+//                           include/MobilityInterface/random_finite_differences.h:30-63
+#include "solver.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+namespace lm {
+
+static constexpr int kIterationHardLimit = 200;  // LanczosAlgorithm.h:49
+
+NBodySolver::NBodySolver(int px, int py, int pz) {
+  if (px < 0 || px > 4 || py < 0 || py > 4 || pz < 0 || pz > 4)
+    throw Error("[libMobility] Invalid periodicity");
+  if (px != kOpen || py != kOpen) throw Error("[Mobility] NBody must be open in the plane");
+  if (pz == kOpen)
+    wall_ = false;
+  else if (pz == kSingleWall)
+    wall_ = true;
+  else
+    throw Error("[Mobility] Invalid periodicity");
+}
+
+NBodySolver::~NBodySolver() = default;
+
+void NBodySolver::setParameters(int algorithm, int nbatch, int nperbatch, bool has_wall_height,
+                                double wall_height, double delta) {
+  if (algorithm < kFast || algorithm > kAdvise) throw Error("Invalid algorithm selected");
+  algorithm_ = algorithm;
+  nbatch_ = nbatch;
+  nperbatch_ = nperbatch;
+  if (wall_) {
+    if (has_wall_height)
+      wall_height_ = wall_height;
+    else
+      throw Error(
+          "[Mobility] Wall height parameter is required for a bottom wall. If you want to use "
+          "a wall, set the wallHeight parameter.");
+  } else if (has_wall_height) {
+    throw Error(
+        "[Mobility] Wall height parameter is only valid for bottom wall. If you want to use a "
+        "wall, set periodicityZ to single_wall in the configuration.");
+  }
+  delta_ = delta;
+  items_tpl_ = -1;  // algorithm may change the work decomposition
+}
+
+void NBodySolver::setPairPrecision(int p) {
+  if (p != kPairFP64 && p != kPairFP32) throw Error("[libmobility_amd] Invalid pair precision");
+  pair_prec_ = p;
+  items_tpl_ = -1;
+}
+
+void NBodySolver::initialize(double viscosity, double radius, bool include_angular,
+                             double tolerance, uint64_t seed, LanczosCallback cb, void* cb_user) {
+  if (initialized_) clean();  // MobilityInterface.h:104-105
+  eta_ = viscosity;
+  a_ = radius;
+  t0_ = 1.0 / (6.0 * M_PI * viscosity * radius);
+  rt0_ = 1.0 / (8.0 * M_PI * viscosity * radius * radius);
+  r0_ = 1.0 / (8.0 * M_PI * viscosity * radius * radius * radius);
+  // Seed chain of mobility.h:107-113 and MobilityInterface.h:106-110.
+  if (seed == 0) seed = std::random_device()();
+  rng_nbody_ = std::mt19937(seed);
+  uint64_t base_seed = rng_nbody_();
+  if (base_seed == 0) base_seed = std::random_device()();
+  rng_base_ = std::mt19937(base_seed);
+  lanczos_seed_ = rng_base_();
+  tol_ = tolerance;
+  cb_ = cb;
+  cb_user_ = cb_user;
+  angular_ = include_angular;
+  initialized_ = true;
+  items_tpl_ = -1;
+}
+
+void NBodySolver::clean() {
+  // MobilityInterface.h:220-223: drop the Lanczos solver and its workspace.
+  lanczos_engine_live_ = false;
+  check_steps_ = 3;
+  old_n_ = 0;
+  V_.release();
+  w_.release();
+  bufA_.release();
+  bufB_.release();
+  y_buf_ = old_buf_ = nullptr;
+  V_cols_ = V_n_ = 0;
+}
+
+void NBodySolver::require_positions(const char* msg) const {
+  if (N_ <= 0) throw Error(msg);
+}
+
+void NBodySolver::build_systems(int64_t N, std::vector<int64_t>& off, bool& valid) const {
+  off.clear();
+  if (explicit_systems_ && !sys_off_host_.empty() && sys_off_host_.back() == N) {
+    off = sys_off_host_;
+    valid = true;
+    return;
+  }
+  const int64_t nb = nbatch_ < 0 ? 1 : nbatch_;
+  const int64_t npb = nperbatch_ < 0 ? N : nperbatch_;
+  valid = (nb * npb == N) && nb > 0;
+  if (!valid || N == 0) {
+    off = {0, N};
+    return;
+  }
+  off.resize(nb + 1);
+  for (int64_t b = 0; b <= nb; ++b) off[b] = b * npb;
+}
+
+void NBodySolver::setSystems(const int64_t* offsets, int nsys) {
+  if (nsys < 1) throw Error("[libmobility_amd] need at least one system");
+  if (offsets[0] != 0) throw Error("[libmobility_amd] system offsets must start at 0");
+  for (int s = 0; s < nsys; ++s)
+    if (offsets[s + 1] < offsets[s]) throw Error("[libmobility_amd] system offsets must be non-decreasing");
+  sys_off_host_.assign(offsets, offsets + nsys + 1);
+  explicit_systems_ = true;
+}
+
+void NBodySolver::setPositions(const double* pos, int64_t n3, bool dev) {
+  if (n3 % 3 != 0)
+    throw Error(
+        "[libMobility] The input arrays for positions, forces, and torques must each have "
+        "total size 3*N.");
+  const int64_t N = n3 / 3;
+  std::vector<int64_t> off;
+  bool valid = true;
+  build_systems(N, off, valid);
+  if (explicit_systems_ && off.back() != N)
+    throw Error("[libmobility_amd] system offsets do not cover the positions");
+  const int nsys = (int)off.size() - 1;
+  if (N > 0) {
+    const double* src = pos;
+    if (!dev) {
+      st_pos_.reserve(sizeof(double) * n3);
+      LM_HIP(hipMemcpyAsync(st_pos_.as<double>(), pos, sizeof(double) * n3,
+                            hipMemcpyHostToDevice, stream_));
+      src = st_pos_.as<double>();
+    }
+    sys_off_.reserve(sizeof(int64_t) * off.size());
+    LM_HIP(hipMemcpyAsync(sys_off_.as<int64_t>(), off.data(), sizeof(int64_t) * off.size(),
+                          hipMemcpyHostToDevice, stream_));
+    bbox_.reserve(sizeof(double) * 6 * nsys);
+    const double shift = wall_ ? wall_height_ : 0.0;
+    launch_system_bbox(src, sys_off_.as<int64_t>(), nsys, shift, bbox_.as<double>(), stream_);
+    if (wall_) {
+      std::vector<double> bb(6 * (size_t)nsys);
+      LM_HIP(hipMemcpyAsync(bb.data(), bbox_.as<double>(), sizeof(double) * bb.size(),
+                            hipMemcpyDeviceToHost, stream_));
+      LM_HIP(hipStreamSynchronize(stream_));
+      double minz = 1e300;
+      for (int s = 0; s < nsys; ++s)
+        if (off[s + 1] > off[s]) minz = std::min(minz, bb[6 * s + 2]);
+      if (minz < 0)
+        throw Error(
+            "[Mobility] The center of a particle has fallen below the wall. Check your "
+            "configuration and forces to ensure particle centers stay above the wall.");
+    }
+    shifted_.reserve(sizeof(double) * n3);
+    rec64_.reserve(sizeof(double) * 4 * N);
+    rec32_.reserve(sizeof(float) * 4 * N);
+    launch_make_records(src, N, sys_off_.as<int64_t>(), nsys, bbox_.as<double>(), wall_ ? 1 : 0,
+                        shift, a_, shifted_.as<double>(), rec64_.as<void>(), rec32_.as<void>(),
+                        stream_);
+    rec_a_ = a_;
+  }
+  N_ = N;
+  if (!explicit_systems_) sys_off_host_ = off;
+  ++layout_id_;
+  items_tpl_ = -1;
+  if (!valid)
+    throw Error(
+        "[Mobility] Invalid batch parameters for NBody. If in doubt, use the defaults.");
+  if (!dev || wall_) LM_HIP(hipStreamSynchronize(stream_));
+}
+
+// Work decomposition: target tiles of 256*tpl particles x source chunks.
+void NBodySolver::ensure_items(int tpl, int nch_hint) {
+  const std::vector<int64_t>& off = sys_off_host_;
+  const int nsys = (int)off.size() - 1;
+  const int64_t bt = (int64_t)kPairThreads * tpl;
+  int64_t tiles = 0, maxn = 0;
+  for (int s = 0; s < nsys; ++s) {
+    const int64_t ns = off[s + 1] - off[s];
+    tiles += (ns + bt - 1) / bt;
+    maxn = std::max(maxn, ns);
+  }
+  int nch = nch_hint;
+  if (nch <= 0) {
+    const int64_t want = 2048;  // ~8 workgroups per CU
+    nch = (int)std::max<int64_t>(1, (want + std::max<int64_t>(tiles, 1) - 1) / std::max<int64_t>(tiles, 1));
+  }
+  const int maxch = (int)std::max<int64_t>(1, (maxn + kPairThreads - 1) / kPairThreads);
+  nch = std::max(1, std::min({nch, 16, maxch}));
+  if (items_tpl_ == tpl && items_nch_ == nch && items_N_ == N_ && items_layout_ == layout_id_)
+    return;
+  std::vector<PairItem> items;
+  items.reserve((size_t)tiles * nch);
+  for (int s = 0; s < nsys; ++s) {
+    const int64_t b = off[s], e = off[s + 1], ns = e - b;
+    if (ns <= 0) continue;
+    int64_t cs = (ns + nch - 1) / nch;
+    cs = ((cs + kPairThreads - 1) / kPairThreads) * kPairThreads;
+    for (int64_t t0 = b; t0 < e; t0 += bt) {
+      for (int c = 0; c < nch; ++c) {
+        PairItem it;
+        it.tgt0 = t0;
+        it.ntgt = (int32_t)std::min<int64_t>(bt, e - t0);
+        it.src0 = std::min<int64_t>(b + (int64_t)c * cs, e);
+        it.src1 = std::min<int64_t>(b + (int64_t)(c + 1) * cs, e);
+        it.chunk = c;
+        items.push_back(it);
+      }
+    }
+  }
+  items_.reserve(sizeof(PairItem) * std::max<size_t>(items.size(), 1));
+  LM_HIP(hipMemcpyAsync(items_.as<PairItem>(), items.data(), sizeof(PairItem) * items.size(),
+                        hipMemcpyHostToDevice, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));  // the host vector goes out of scope
+  items_tpl_ = tpl;
+  items_nch_ = nch;
+  items_N_ = N_;
+  items_layout_ = layout_id_;
+  items_count_ = (int)items.size();
+}
+
+// U (3N) and, when angular, W (3N) are overwritten (NbodyRPY.cu:91-96).
+void NBodySolver::pair_product(const void* rec64, const void* rec32, const double* F,
+                               const double* T, double* U, double* W) {
+  int blocks = 0;
+  // Block selection of NbodyRPY.cu:264-275.
+  if (F) blocks |= kUF | (angular_ ? kWF : 0);
+  if (T && angular_) blocks |= kUT | kWT;
+  const int64_t n3 = 3 * N_;
+  if (blocks == 0) {
+    LM_HIP(hipMemsetAsync(U, 0, sizeof(double) * n3, stream_));
+    if (angular_ && W) LM_HIP(hipMemsetAsync(W, 0, sizeof(double) * n3, stream_));
+    return;
+  }
+  const bool fp32 = pair_prec_ == kPairFP32;
+  int tpl = 1, nch_hint = 0;
+  if (algorithm_ == kNaive) {
+    nch_hint = 1;
+  } else if (algorithm_ == kBlock) {
+    nch_hint = 16;
+  } else if (!wall_ && blocks == kUF && fp32) {
+    tpl = 2;
+  }
+  ensure_items(tpl, nch_hint);
+  PairArgs a{};
+  a.wall = wall_ ? 1 : 0;
+  a.fp32 = fp32 ? 1 : 0;
+  a.blocks = blocks;
+  a.tpl = tpl;
+  a.pos = fp32 ? rec32 : rec64;
+  a.F = F;
+  a.T = (blocks & (kUT | kWT)) ? T : nullptr;
+  a.items = items_.as<PairItem>();
+  a.nitems = items_count_;
+  a.t0 = t0_;
+  a.rt0 = rt0_;
+  a.r0 = r0_;
+  const bool ang = (blocks & (kWT | kWF)) != 0;
+  if (!U) throw Error("[libmobility_amd] Mdot needs a linear output array");
+  if (ang && !W)
+    throw Error("[libmobility_amd] the solver includes angular velocities: an angular output array is required");
+  if (items_count_ <= 0) throw Error("[libmobility_amd] empty pair work list");
+  if (items_nch_ == 1) {
+    a.U = U;
+    a.W = ang ? W : nullptr;
+    a.slab = 0;
+    launch_pair(a, stream_);
+  } else {
+    part_u_.reserve(sizeof(double) * n3 * items_nch_);
+    if (ang) part_w_.reserve(sizeof(double) * n3 * items_nch_);
+    a.U = part_u_.as<double>();
+    a.W = ang ? part_w_.as<double>() : nullptr;
+    a.slab = n3;
+    launch_pair(a, stream_);
+    launch_chunk_reduce(part_u_.as<double>(), items_nch_, n3, U, stream_);
+    if (ang) launch_chunk_reduce(part_w_.as<double>(), items_nch_, n3, W, stream_);
+  }
+  if (angular_ && W && !ang) LM_HIP(hipMemsetAsync(W, 0, sizeof(double) * n3, stream_));
+}
+
+// Records are scaled by 1/a; rebuild them if initialize() changed a after
+// setPositions() (the reference reads the radius only when it multiplies).
+void NBodySolver::ensure_records() {
+  if (N_ <= 0 || rec_a_ == a_) return;
+  const int nsys = (int)sys_off_host_.size() - 1;
+  launch_system_bbox(shifted_.as<double>(), sys_off_.as<int64_t>(), nsys, 0.0, bbox_.as<double>(), stream_);
+  launch_make_records(shifted_.as<double>(), N_, sys_off_.as<int64_t>(), nsys, bbox_.as<double>(),
+                      wall_ ? 1 : 0, 0.0, a_, nullptr, rec64_.as<void>(), rec32_.as<void>(), stream_);
+  rec_a_ = a_;
+}
+
+void NBodySolver::mdot_dev(const double* F, const double* T, double* U, double* W) {
+  require_positions("[Mobility] Positions have 0 particles. Did you call setPositions?");
+  ensure_records();
+  pair_product(rec64_.as<void>(), rec32_.as<void>(), F, T, U, W);
+}
+
+void NBodySolver::Mdot(const double* F, int64_t nF, const double* T, int64_t nT, double* U,
+                       double* W, bool dev) {
+  // pythonify.h:109-149 checks, then NBody::Mdot (mobility.h:166-182)
+  const int64_t n3 = 3 * N_;
+  if (nF > 0 && nF != n3) throw Error("The forces array must have size 3*N.");
+  if (nT > 0 && nT != n3) throw Error("The torques array must have size 3*N.");
+  if (!angular_ && nT > 0)
+    throw Error(
+        "The solver was configured without including angular velocities. Set includeAngular "
+        "to true when initializing if you want to use torques");
+  require_positions("[Mobility] Positions have 0 particles. Did you call setPositions?");
+  const double* f = nF > 0 ? F : nullptr;
+  const double* t = nT > 0 ? T : nullptr;
+  if (dev) {
+    mdot_dev(f, t, U, angular_ ? W : nullptr);
+    LM_HIP(hipStreamSynchronize(stream_));
+    return;
+  }
+  double *df = nullptr, *dt = nullptr;
+  if (f) {
+    st_f_.reserve(sizeof(double) * n3);
+    df = st_f_.as<double>();
+    LM_HIP(hipMemcpyAsync(df, f, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
+  }
+  if (t) {
+    st_t_.reserve(sizeof(double) * n3);
+    dt = st_t_.as<double>();
+    LM_HIP(hipMemcpyAsync(dt, t, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
+  }
+  st_u_.reserve(sizeof(double) * n3);
+  st_w_.reserve(sizeof(double) * n3);
+  mdot_dev(df, dt, st_u_.as<double>(), angular_ ? st_w_.as<double>() : nullptr);
+  LM_HIP(hipMemcpyAsync(U, st_u_.as<double>(), sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
+  if (angular_ && W)
+    LM_HIP(hipMemcpyAsync(W, st_w_.as<double>(), sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));
+}
+
+// ---- Lanczos ---------------------------------------------------------------------
+
+int NBodySolver::lanczos_run(const double* z, double** result) {
+  const int64_t n = (angular_ ? 6 : 3) * N_;
+  const int64_t N3 = 3 * N_;
+  ensure_records();
+  // workspace
+  red_partial_.reserve(sizeof(double) * 4 * kReduceBlocks);
+  if (red_counter_.capacity() == 0) {
+    red_counter_.reserve(sizeof(unsigned int) * 64);
+    LM_HIP(hipMemsetAsync(red_counter_.as<void>(), 0, sizeof(unsigned int) * 64, stream_));
+  }
+  scalars_.reserve(sizeof(LanczosScalars));
+  tsq_work_.reserve(sizeof(double) * kTsqrtScratch);
+  coef_.reserve(sizeof(double) * 256);
+  w_.reserve(sizeof(double) * n);
+  ReduceScratch rs{red_partial_.as<double>(), red_counter_.as<unsigned int>()};
+  LanczosScalars* sc = scalars_.as<LanczosScalars>();
+  // previous estimate (oldBz) keeps its prefix across calls (LanczosAlgorithm.cu:175)
+  if (old_n_ != n || old_buf_ == nullptr) {
+    const int64_t keep = old_buf_ ? std::min(old_n_, n) : 0;
+    std::vector<double> host_prev(keep);
+    if (keep > 0) {
+      LM_HIP(hipMemcpyAsync(host_prev.data(), old_buf_, sizeof(double) * keep,
+                            hipMemcpyDeviceToHost, stream_));
+      LM_HIP(hipStreamSynchronize(stream_));
+    }
+    bufA_.release();
+    bufB_.release();
+    bufA_.reserve(sizeof(double) * (n + 1));
+    bufB_.reserve(sizeof(double) * (n + 1));
+    LM_HIP(hipMemsetAsync(bufA_.as<void>(), 0, sizeof(double) * (n + 1), stream_));
+    LM_HIP(hipMemsetAsync(bufB_.as<void>(), 0, sizeof(double) * (n + 1), stream_));
+    old_buf_ = bufA_.as<double>();
+    y_buf_ = bufB_.as<double>();
+    if (keep > 0) {
+      LM_HIP(hipMemcpyAsync(old_buf_, host_prev.data(), sizeof(double) * keep,
+                            hipMemcpyHostToDevice, stream_));
+      LM_HIP(hipStreamSynchronize(stream_));
+    }
+    old_n_ = n;
+  }
+  // Krylov basis, column-major n x cols
+  auto ensure_cols = [&](int64_t cols) {
+    if (V_n_ != n) {
+      V_.release();
+      V_cols_ = 0;
+      V_n_ = n;
+    }
+    if (cols <= V_cols_) return;
+    int64_t want = std::max<int64_t>(cols, std::min<int64_t>(kIterationHardLimit + 1,
+                                                            std::max<int64_t>(32, 2 * V_cols_)));
+    V_.grow_keep(sizeof(double) * n * want, sizeof(double) * n * V_cols_, stream_);
+    V_cols_ = want;
+  };
+  ensure_cols(2);
+  double* V = V_.as<double>();
+  double* w = w_.as<double>();
+  launch_lanczos_start(z, V, n, rs, sc, stream_);  // LanczosAlgorithm.cu:110-118
+  const int check = std::min(check_steps_, kIterationHardLimit - 2);
+  for (int i = 0; i < kIterationHardLimit; ++i) {
+    ensure_cols(i + 2);
+    V = V_.as<double>();
+    const double* vi = V + (int64_t)i * n;
+    // w = M v_i ; torques are stored after the forces (MobilityInterface.h:164-176)
+    pair_product(rec64_.as<void>(), rec32_.as<void>(), vi, angular_ ? vi + N3 : nullptr, w,
+                 angular_ ? w + N3 : nullptr);
+    launch_lanczos_alpha(w, vi, i > 0 ? V + (int64_t)(i - 1) * n : nullptr, n, i, rs, sc, stream_);
+    launch_lanczos_beta(w, vi, n, i, rs, sc, stream_);
+    launch_lanczos_next(w, V + (int64_t)(i + 1) * n, n, i, sc, stream_);
+    if (i >= check || cb_ != nullptr) {
+      const int m = i + 1;
+      launch_tridiag_sqrt(sc, m, coef_.as<double>(), tsq_work_.as<double>(), stream_);
+      launch_lanczos_estimate(V, n, m, coef_.as<double>(), y_buf_, old_buf_, n, rs, sc, stream_);
+      if (i > 0) {
+        double err = 0.0;
+        LM_HIP(hipMemcpyAsync(&err, &sc->err, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        LM_HIP(hipStreamSynchronize(stream_));
+        if (std::isnan(err)) throw Error("[Lanczos] Unknown error (found NaN in result guess)");
+        if (cb_) cb_(i, err, cb_user_);
+        if (err <= tol_) {
+          // registerRequiredStepsForConverge, LanczosAlgorithm.cu:224-232
+          if (i - 2 > check_steps_)
+            check_steps_ += 1;
+          else
+            check_steps_ = std::max(1, check_steps_ - 2);
+          last_iters_ = i;
+          *result = y_buf_;
+          return i;
+        }
+      }
+      std::swap(y_buf_, old_buf_);  // oldBz <- Bz (LanczosAlgorithm.cu:196)
+    }
+  }
+  throw Error("[Lanczos] Could not converge");
+}
+
+void NBodySolver::sqrtMdotW_dev(double* U, double* W, double prefactor, const double* noise) {
+  const int64_t n = (angular_ ? 6 : 3) * N_;
+  const double* z = noise;
+  if (!z) {
+    // LanczosStochasticVelocities: per-call seeds from an mt19937 (lanczos.h:35-52)
+    if (!lanczos_engine_live_) {
+      lanczos_engine_ = std::mt19937(lanczos_seed_);
+      lanczos_engine_live_ = true;
+    }
+    const uint32_t s1 = std::uniform_int_distribution<uint32_t>(0, UINT32_MAX)(lanczos_engine_);
+    const uint32_t s2 = std::uniform_int_distribution<uint32_t>(0, UINT32_MAX)(lanczos_engine_);
+    noise_.reserve(sizeof(double) * n);
+    launch_normal_fill(noise_.as<double>(), n, ((uint64_t)s2 << 32) | s1, 0, stream_);
+    z = noise_.as<double>();
+  }
+  double* y = nullptr;
+  lanczos_run(z, &y);
+  // out += prefactor * y (MobilityInterface.h:179-187)
+  launch_axpy_accumulate(U, y, prefactor, 3 * N_, stream_);
+  if (angular_ && W) launch_axpy_accumulate(W, y + 3 * N_, prefactor, 3 * N_, stream_);
+}
+
+void NBodySolver::sqrtMdotW(double* U, double* W, double prefactor, bool dev,
+                            const double* noise) {
+  if (prefactor == 0) return;  // MobilityInterface.h:134-135
+  if (!initialized_)
+    throw Error(
+        "[libMobility] You must initialize the base class in order to use the default "
+        "stochastic displacement computation");
+  require_positions(
+      "[libMobility] The number of particles is not set. Did you forget to call setPositions?");
+  if (!U) throw Error("[libMobility] This solver requires linear velocities");
+  const int64_t n3 = 3 * N_;
+  const int64_t n = (angular_ ? 6 : 3) * N_;
+  if (dev) {
+    sqrtMdotW_dev(U, W, prefactor, noise);
+    LM_HIP(hipStreamSynchronize(stream_));
+    return;
+  }
+  st_u_.reserve(sizeof(double) * n3);
+  st_w_.reserve(sizeof(double) * n3);
+  LM_HIP(hipMemcpyAsync(st_u_.as<double>(), U, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
+  if (angular_ && W)
+    LM_HIP(hipMemcpyAsync(st_w_.as<double>(), W, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
+  const double* dnoise = nullptr;
+  if (noise) {
+    st_f_.reserve(sizeof(double) * n);
+    LM_HIP(hipMemcpyAsync(st_f_.as<double>(), noise, sizeof(double) * n, hipMemcpyHostToDevice, stream_));
+    dnoise = st_f_.as<double>();
+  }
+  sqrtMdotW_dev(st_u_.as<double>(), (angular_ && W) ? st_w_.as<double>() : nullptr, prefactor, dnoise);
+  LM_HIP(hipMemcpyAsync(U, st_u_.as<double>(), sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
+  if (angular_ && W)
+    LM_HIP(hipMemcpyAsync(W, st_w_.as<double>(), sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));
+}
+
+// ---- random finite differences -------------------------------------------------------
+
+void NBodySolver::divM_dev(double* U, double* W, double prefactor) {
+  const int64_t N = N_, n3 = 3 * N;
+  ensure_records();
+  const uint32_t seed = rng_nbody_();  // mobility.h:218
+  const double delta = delta_ * a_;    // mobility.h:224
+  rfd_noise_.reserve(sizeof(double) * n3);
+  rfd_pos_.reserve(sizeof(double) * n3);
+  rfd_rec64_.reserve(sizeof(double) * 4 * N);
+  rfd_rec32_.reserve(sizeof(float) * 4 * N);
+  rfd_mp_.reserve(sizeof(double) * n3);
+  rfd_mm_.reserve(sizeof(double) * n3);
+  if (angular_) {
+    rfd_wp_.reserve(sizeof(double) * n3);
+    rfd_wm_.reserve(sizeof(double) * n3);
+  }
+  double* noise = rfd_noise_.as<double>();
+  launch_normal_fill(noise, n3, seed, 1, stream_);
+  const int nsys = (int)sys_off_host_.size() - 1;
+  // q + (delta/2) W, then M(q + delta/2 W) W  (random_finite_differences.h:47-49)
+  launch_axpy_to(rfd_pos_.as<double>(), shifted_.as<double>(), noise, 0.5 * delta, n3, stream_);
+  launch_system_bbox(rfd_pos_.as<double>(), sys_off_.as<int64_t>(), nsys, 0.0, bbox_.as<double>(), stream_);
+  launch_make_records(rfd_pos_.as<double>(), N, sys_off_.as<int64_t>(), nsys, bbox_.as<double>(),
+                      wall_ ? 1 : 0, 0.0, a_, nullptr, rfd_rec64_.as<void>(), rfd_rec32_.as<void>(), stream_);
+  pair_product(rfd_rec64_.as<void>(), rfd_rec32_.as<void>(), noise, nullptr, rfd_mp_.as<double>(),
+               angular_ ? rfd_wp_.as<double>() : nullptr);
+  // q - (delta/2) W  (:50-52)
+  launch_axpy_to(rfd_pos_.as<double>(), shifted_.as<double>(), noise, -0.5 * delta, n3, stream_);
+  launch_system_bbox(rfd_pos_.as<double>(), sys_off_.as<int64_t>(), nsys, 0.0, bbox_.as<double>(), stream_);
+  launch_make_records(rfd_pos_.as<double>(), N, sys_off_.as<int64_t>(), nsys, bbox_.as<double>(),
+                      wall_ ? 1 : 0, 0.0, a_, nullptr, rfd_rec64_.as<void>(), rfd_rec32_.as<void>(), stream_);
+  pair_product(rfd_rec64_.as<void>(), rfd_rec32_.as<void>(), noise, nullptr, rfd_mm_.as<double>(),
+               angular_ ? rfd_wm_.as<double>() : nullptr);
+  // out += (prefactor/delta) (M+ W - M- W)  (:54-62, mobility.h:227-235)
+  launch_diff_accumulate(U, rfd_mp_.as<double>(), rfd_mm_.as<double>(), prefactor / delta, n3, stream_);
+  if (angular_ && W)
+    launch_diff_accumulate(W, rfd_wp_.as<double>(), rfd_wm_.as<double>(), prefactor / delta, n3, stream_);
+}
+
+void NBodySolver::divM(double* U, double* W, double prefactor, bool dev) {
+  if (prefactor == 0) return;  // mobility.h:186-188
+  if (!wall_) return;          // open: no thermal drift (mobility.h:189-190)
+  require_positions(
+      "[libMobility] The number of particles is not set. Did you forget to call setPositions?");
+  if (!U) throw Error("[libMobility] The number of forces does not match the number of particles");
+  const int64_t n3 = 3 * N_;
+  if (dev) {
+    divM_dev(U, W, prefactor);
+    LM_HIP(hipStreamSynchronize(stream_));
+    return;
+  }
+  st_u_.reserve(sizeof(double) * n3);
+  st_w_.reserve(sizeof(double) * n3);
+  LM_HIP(hipMemcpyAsync(st_u_.as<double>(), U, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
+  if (angular_ && W)
+    LM_HIP(hipMemcpyAsync(st_w_.as<double>(), W, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
+  divM_dev(st_u_.as<double>(), (angular_ && W) ? st_w_.as<double>() : nullptr, prefactor);
+  LM_HIP(hipMemcpyAsync(U, st_u_.as<double>(), sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
+  if (angular_ && W)
+    LM_HIP(hipMemcpyAsync(W, st_w_.as<double>(), sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));
+}
+
+void NBodySolver::langevin(double dt, double kbt, const double* F, int64_t nF, const double* T,
+                           int64_t nT, double* U, double* W, bool dev) {
+  // MobilityInterface.h:191-206
+  const int64_t n3 = 3 * N_;
+  if (nF > 0 && nF != n3) throw Error("The forces array must have size 3*N.");
+  if (nT > 0 && nT != n3) throw Error("The torques array must have size 3*N.");
+  if (!angular_ && nT > 0)
+    throw Error(
+        "The solver was configured without including angular velocities. Set includeAngular "
+        "to true when initializing if you want to use torques");
+  const bool have_ft = nF > 0 || nT > 0;
+  if (!have_ft && kbt == 0) return;
+  require_positions("[Mobility] Positions have 0 particles. Did you call setPositions?");
+  if (kbt != 0) {
+    if (!initialized_)
+      throw Error(
+          "[libMobility] You must initialize the base class in order to use the default "
+          "stochastic displacement computation");
+  }
+  double *dU = U, *dW = angular_ ? W : nullptr;
+  const double *dF = nF > 0 ? F : nullptr, *dT = nT > 0 ? T : nullptr;
+  if (!dev) {
+    st_u_.reserve(sizeof(double) * n3);
+    st_w_.reserve(sizeof(double) * n3);
+    dU = st_u_.as<double>();
+    dW = angular_ ? st_w_.as<double>() : nullptr;
+    LM_HIP(hipMemcpyAsync(dU, U, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
+    if (dW && W) LM_HIP(hipMemcpyAsync(dW, W, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
+    if (dF) {
+      st_f_.reserve(sizeof(double) * n3);
+      LM_HIP(hipMemcpyAsync(st_f_.as<double>(), F, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
+      dF = st_f_.as<double>();
+    }
+    if (dT) {
+      st_t_.reserve(sizeof(double) * n3);
+      LM_HIP(hipMemcpyAsync(st_t_.as<double>(), T, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
+      dT = st_t_.as<double>();
+    }
+  }
+  if (have_ft) mdot_dev(dF, dT, dU, dW);
+  if (kbt != 0) {
+    sqrtMdotW_dev(dU, dW, std::sqrt(2 * kbt / dt), nullptr);
+    if (wall_) divM_dev(dU, dW, kbt);
+  }
+  if (!dev) {
+    LM_HIP(hipMemcpyAsync(U, dU, sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
+    if (dW && W) LM_HIP(hipMemcpyAsync(W, dW, sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
+  }
+  LM_HIP(hipStreamSynchronize(stream_));
+}
+
+}  // namespace lm

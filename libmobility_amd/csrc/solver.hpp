@@ -1,0 +1,119 @@
+// Host-side NBody solver: the MI355X counterpart of libmobility::Mobility +
+// NBody (include/MobilityInterface/MobilityInterface.h:56-224,
+// solvers/NBody/mobility.h:11-237).  One instance = one solver handle of the
+// C ABI; calls on one instance are serialised by the caller (as with the
+// reference, which is not thread-safe either).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <random>
+#include <vector>
+
+#include "kernels.hpp"
+#include "lm_common.hpp"
+#include "pair_math.hpp"
+
+namespace lm {
+
+// Same numbering as libmobility::periodicity_mode (MobilityInterface.h:27-33).
+enum Periodicity : int { kSingleWall = 0, kTwoWalls = 1, kOpen = 2, kPeriodic = 3, kUnspecified = 4 };
+// Same numbering as nbody_rpy::algorithm (solvers/NBody/extra/interface.h:10).
+enum Algorithm : int { kFast = 0, kNaive = 1, kBlock = 2, kAdvise = 3 };
+enum PairPrecision : int { kPairFP64 = 0, kPairFP32 = 1 };
+
+using LanczosCallback = void (*)(int iteration, double residual, void* user);
+
+class NBodySolver {
+ public:
+  NBodySolver(int px, int py, int pz);
+  ~NBodySolver();
+
+  void setParameters(int algorithm, int nbatch, int nperbatch, bool has_wall_height,
+                     double wall_height, double delta);
+  void setPairPrecision(int p);
+  void setStream(hipStream_t s) { stream_ = s; }
+  hipStream_t stream() const { return stream_; }
+  void initialize(double viscosity, double radius, bool include_angular, double tolerance,
+                  uint64_t seed, LanczosCallback cb, void* cb_user);
+
+  void setPositions(const double* pos, int64_t n3, bool dev);
+  // Ragged systems (extension): offsets[0..nsys], offsets[nsys] = N.
+  void setSystems(const int64_t* offsets, int nsys);
+  int64_t numParticles() const { return N_; }
+  bool includeAngular() const { return angular_; }
+
+  void Mdot(const double* F, int64_t nF, const double* T, int64_t nT, double* U, double* W,
+            bool dev);
+  void sqrtMdotW(double* U, double* W, double prefactor, bool dev, const double* noise);
+  void divM(double* U, double* W, double prefactor, bool dev);
+  void langevin(double dt, double kbt, const double* F, int64_t nF, const double* T, int64_t nT,
+                double* U, double* W, bool dev);
+  void clean();
+  int lastLanczosIterations() const { return last_iters_; }
+  int checkConvergenceSteps() const { return check_steps_; }
+
+  // Device-pointer cores (no staging, no synchronisation beyond what the
+  // algorithm needs).
+  void mdot_dev(const double* F, const double* T, double* U, double* W);
+  void sqrtMdotW_dev(double* U, double* W, double prefactor, const double* noise);
+  void divM_dev(double* U, double* W, double prefactor);
+
+ private:
+  void require_positions(const char* msg) const;
+  void ensure_records();
+  void build_systems(int64_t N, std::vector<int64_t>& off, bool& valid) const;
+  void ensure_items(int tpl, int nch_hint);
+  void pair_product(const void* rec64, const void* rec32, const double* F, const double* T,
+                    double* U, double* W);
+  int lanczos_run(const double* z, double** result);
+
+  // configuration
+  bool wall_ = false;
+  int algorithm_ = kAdvise;
+  int nbatch_ = -1, nperbatch_ = -1;
+  double wall_height_ = 0.0;
+  double delta_ = 1e-3;
+  int pair_prec_ = kPairFP64;
+  hipStream_t stream_ = nullptr;
+
+  // initialize()
+  bool initialized_ = false;
+  double eta_ = 1.0, a_ = 1.0, t0_ = 0.0, rt0_ = 0.0, r0_ = 0.0;
+  bool angular_ = false;
+  double tol_ = 1e-4;
+  LanczosCallback cb_ = nullptr;
+  void* cb_user_ = nullptr;
+  std::mt19937 rng_nbody_;    // mobility.h:34,110
+  std::mt19937 rng_base_;     // MobilityInterface.h:64,108
+  uint64_t lanczos_seed_ = 0;
+  bool lanczos_engine_live_ = false;
+  std::mt19937 lanczos_engine_;  // lanczos.h:32,37
+
+  // positions
+  int64_t N_ = 0;
+  std::vector<int64_t> sys_off_host_;
+  bool explicit_systems_ = false;
+  DevBuf sys_off_, bbox_, shifted_, rec64_, rec32_;
+  double rec_a_ = 0.0;  // radius the records were scaled with
+  // pair work lists
+  DevBuf items_;
+  int items_tpl_ = -1, items_nch_ = 0, items_count_ = 0;
+  int64_t items_N_ = -1;
+  uint64_t items_layout_ = 0, layout_id_ = 0;
+  DevBuf part_u_, part_w_;
+  // staging for host arrays
+  DevBuf st_pos_, st_f_, st_t_, st_u_, st_w_;
+  // Lanczos
+  DevBuf V_, w_, coef_, bufA_, bufB_, noise_, red_partial_, red_counter_, scalars_, tsq_work_;
+  int64_t V_cols_ = 0, V_n_ = 0;
+  double* y_buf_ = nullptr;
+  double* old_buf_ = nullptr;
+  int64_t old_n_ = 0;
+  int check_steps_ = 3;          // LanczosAlgorithm.cu:171
+  int last_iters_ = 0;
+  // RFD
+  DevBuf rfd_noise_, rfd_pos_, rfd_rec64_, rfd_rec32_, rfd_mp_, rfd_mm_, rfd_wp_, rfd_wm_;
+};
+
+}  // namespace lm

@@ -1,0 +1,330 @@
+"""Python ``NBody`` class: the drop-in surface of libMobility's NBody solver.
+
+Mirrors the class that ``MOBILITY_PYTHONIFY_WITH_EXTRA_CODE`` generates
+(include/MobilityInterface/pythonify.h:364-407) plus NBody's
+``setParameters`` (solvers/NBody/python_wrapper.cu:50-61): same method names,
+argument names and defaults, shape rules, output framework/device, and
+``RuntimeError`` texts.  All arithmetic runs in the HIP library through the C
+ABI of include/libmobility_amd.h; arrays that already live on the GPU (torch
+ROCm tensors) are passed by device pointer without copies.
+
+Extensions (no reference counterpart): ``setPairPrecision``,
+``sqrtMdotW_with_noise``, ``hydrodynamicVelocities``, ``setSystems``,
+``lastLanczosIterations``.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _ffi
+
+# Module-level "last positions" state, as pythonify.h:34-36 keeps it: the
+# output shape/framework/device of sqrtMdotW and divM come from the last
+# setPositions (or Mdot) call of any solver in the process.
+_last_shape = ()
+_last_framework = "numpy"
+_last_device = "cpu"
+
+
+def _is_torch(x):
+    try:
+        import torch
+    except ImportError:
+        return False
+    return isinstance(x, torch.Tensor)
+
+
+class _Arr:
+    """An FP64, C-contiguous view of a caller array plus where it lives."""
+
+    __slots__ = ("obj", "framework", "device", "shape", "size", "ndim", "ptr", "mem")
+
+    def __init__(self, x):
+        if x is None:
+            x = np.zeros(0)
+        if _is_torch(x):
+            import torch
+
+            t = x.detach()
+            if t.dtype != torch.float64:
+                t = t.to(torch.float64)
+            t = t.contiguous()
+            self.obj = t
+            self.framework = "torch"
+            self.device = str(t.device)
+            self.shape = tuple(t.shape)
+            self.size = t.numel()
+            self.ndim = t.dim()
+            self.ptr = t.data_ptr() if self.size else None
+            self.mem = _ffi.LM_MEM_DEVICE if t.is_cuda else _ffi.LM_MEM_HOST
+        else:
+            a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+            self.obj = a
+            self.framework = "numpy"
+            self.device = "cpu"
+            self.shape = tuple(a.shape)
+            self.size = a.size
+            self.ndim = a.ndim
+            self.ptr = a.ctypes.data if a.size else None
+            self.mem = _ffi.LM_MEM_HOST
+
+    def moved_to(self, mem, device):
+        """Same data in the given memory space (torch tensors only move)."""
+        if self.size == 0 or self.mem == mem:
+            return self
+        import torch
+
+        src = self.obj if self.framework == "torch" else torch.from_numpy(self.obj)
+        return _Arr(src.to(device if mem == _ffi.LM_MEM_DEVICE else "cpu"))
+
+
+def _check_and_get_shape(arr):
+    """pythonify.h:83-107."""
+    if arr.size == 0:
+        return ()
+    if arr.size % 3 != 0:
+        raise RuntimeError(
+            "[libMobility] The input arrays for positions, forces, and torques must each have "
+            "total size 3*N.")
+    n = arr.size // 3
+    err = False
+    if arr.ndim == 1 and arr.shape[0] != 3 * n:
+        err = True
+    elif arr.ndim == 2 and arr.shape[1] != 3:
+        err = True
+    elif arr.ndim not in (1, 2):
+        err = True
+    if err:
+        raise RuntimeError(
+            "[libMobility] The input arrays for positions, forces, and torques must each have "
+            "shape (N, 3) or (3*N).")
+    return arr.shape
+
+
+def _zeros(shape, framework, device):
+    """python_tensor.h:30-56: output in the caller's framework and device."""
+    if framework == "torch":
+        import torch
+
+        return torch.zeros(shape, dtype=torch.float64, device=device)
+    return np.zeros(shape, dtype=np.float64)
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if _is_torch(x):
+        return x.data_ptr() if x.numel() else None
+    return x.ctypes.data if x.size else None
+
+
+def _mem_of(framework, device):
+    return _ffi.LM_MEM_DEVICE if (framework == "torch" and device.startswith("cuda")) else _ffi.LM_MEM_HOST
+
+
+class NBody:
+    """O(N^2) hydrodynamic interactions on one MI355X.
+
+    Open boundaries in x and y; z is ``open`` (Rotne-Prager-Yamakawa) or
+    ``single_wall`` (Rotne-Prager-Blake with Swan-Brady corrections, wall at
+    ``wallHeight``).
+    """
+
+    precision = "double"
+    version = "1.1"
+
+    def __init__(self, periodicityX, periodicityY, periodicityZ):
+        # string2Periodicity, pythonify.h:38-52
+        per = []
+        for p in (periodicityX, periodicityY, periodicityZ):
+            if p not in _ffi.PERIODICITY:
+                raise RuntimeError("[libMobility] Invalid periodicity")
+            per.append(_ffi.PERIODICITY[p])
+        self._lib = _ffi.load()
+        h = ctypes.c_void_p()
+        _ffi.check(self._lib.lm_nbody_create(per[0], per[1], per[2], ctypes.byref(h)))
+        self._h = h
+        self._cb = None
+        self._include_angular = False
+        self._stream_set = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.lm_nbody_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # -- configuration ---------------------------------------------------------
+    def setParameters(self, algorithm="advise", Nbatch=-1, NperBatch=-1, wallHeight=None,
+                      delta=1e-3):
+        """NBody.setParameters, solvers/NBody/python_wrapper.cu:50-61."""
+        if algorithm not in _ffi.ALGORITHM:
+            raise RuntimeError("Invalid algorithm selected")
+        has_wall = wallHeight is not None
+        _ffi.check(self._lib.lm_nbody_set_parameters(
+            self._h, _ffi.ALGORITHM[algorithm], int(Nbatch), int(NperBatch), int(has_wall),
+            float(wallHeight) if has_wall else 0.0, float(delta)))
+
+    def setPairPrecision(self, precision="fp64"):
+        """Extension: 'fp64' (default) or 'fp32' (FP32 pair terms, FP64 accumulation)."""
+        if precision not in _ffi.PAIR_PRECISION:
+            raise RuntimeError("[libmobility_amd] pair precision must be 'fp64' or 'fp32'")
+        _ffi.check(self._lib.lm_nbody_set_pair_precision(self._h, _ffi.PAIR_PRECISION[precision]))
+
+    def setSystems(self, sizes):
+        """Extension: independent systems of the given (ragged) sizes."""
+        sizes = np.asarray(sizes, dtype=np.int64)
+        off = np.zeros(sizes.size + 1, dtype=np.int64)
+        off[1:] = np.cumsum(sizes)
+        self._sys_off = off
+        _ffi.check(self._lib.lm_nbody_set_systems(self._h, off.ctypes.data, int(sizes.size)))
+
+    def initialize(self, viscosity, hydrodynamicRadius, includeAngular=False, tolerance=1e-4,
+                   seed=None, lanczos_callback=None):
+        """pythonify.h:241-260 -> NBody::initialize (mobility.h:99-114)."""
+        cb = _ffi.LANCZOS_CALLBACK()
+        if lanczos_callback is not None:
+            user_cb = lanczos_callback
+
+            def _tramp(i, err, _user):
+                user_cb(int(i), float(err))
+
+            cb = _ffi.LANCZOS_CALLBACK(_tramp)
+        self._cb = cb  # keep alive
+        s = 0 if seed is None else int(seed)
+        _ffi.check(self._lib.lm_nbody_initialize(
+            self._h, float(viscosity), float(hydrodynamicRadius), int(bool(includeAngular)),
+            float(tolerance), ctypes.c_uint64(s & 0xFFFFFFFFFFFFFFFF), cb, None))
+        self._include_angular = bool(includeAngular)
+
+    # -- helpers ------------------------------------------------------------------
+    def _bind_stream(self, mem, device):
+        if mem != _ffi.LM_MEM_DEVICE:
+            return
+        import torch
+
+        st = torch.cuda.current_stream(torch.device(device)).cuda_stream
+        if st != self._stream_set:
+            _ffi.check(self._lib.lm_nbody_set_stream(self._h, ctypes.c_void_p(st)))
+            self._stream_set = st
+
+    def getNumberParticles(self):
+        return int(self._lib.lm_nbody_num_particles(self._h))
+
+    # -- interface ----------------------------------------------------------------
+    def setPositions(self, positions):
+        """pythonify.h:262-271 -> NBody::setPositions (mobility.h:116-162)."""
+        global _last_shape, _last_framework, _last_device
+        arr = _Arr(positions)
+        shape = _check_and_get_shape(arr)
+        _last_shape = shape
+        _last_framework = arr.framework
+        _last_device = arr.device
+        self._bind_stream(arr.mem, arr.device)
+        _ffi.check(self._lib.lm_nbody_set_positions(self._h, arr.ptr, arr.size, arr.mem))
+        _last_shape = arr.shape
+
+    def _setup_arrays(self, forces, torques):
+        """pythonify.h:109-149."""
+        global _last_framework, _last_device
+        N = self.getNumberParticles()
+        f = _Arr(forces)
+        t = _Arr(torques)
+        if f.size > 0 and f.size != 3 * N:
+            raise RuntimeError("The forces array must have size 3*N.")
+        if t.size > 0 and t.size != 3 * N:
+            raise RuntimeError("The torques array must have size 3*N.")
+        lead = f if f.size > 0 else (t if t.size > 0 else f)
+        framework, device, mem = lead.framework, lead.device, lead.mem
+        _last_framework = framework
+        _last_device = device
+        f = f.moved_to(mem, device)
+        t = t.moved_to(mem, device)
+        f_shape = _check_and_get_shape(f) if f.size > 0 else _check_and_get_shape(t)
+        mf = _zeros(f_shape, framework, device)
+        mt = None
+        if not self._include_angular:
+            if t.size > 0:
+                raise RuntimeError(
+                    "The solver was configured without including angular velocities. Set "
+                    "includeAngular to true when initializing if you want to use torques")
+        else:
+            t_shape = f_shape if t.size == 0 else _check_and_get_shape(t)
+            mt = _zeros(t_shape, framework, device)
+        return f, t, mf, mt, mem, device
+
+    def Mdot(self, forces=None, torques=None):
+        """pythonify.h:208-216 -> NBody::Mdot (mobility.h:166-182)."""
+        f, t, mf, mt, mem, device = self._setup_arrays(forces, torques)
+        self._bind_stream(mem, device)
+        _ffi.check(self._lib.lm_nbody_mdot(self._h, f.ptr, f.size, t.ptr, t.size, _ptr(mf),
+                                           _ptr(mt), mem))
+        return mf, mt
+
+    def _outputs_like_positions(self):
+        linear = _zeros(_last_shape, _last_framework, _last_device)
+        angular = _zeros(_last_shape, _last_framework, _last_device) if self._include_angular else None
+        return linear, angular, _mem_of(_last_framework, _last_device)
+
+    def sqrtMdotW(self, prefactor=1.0):
+        """pythonify.h:170-185 -> Mobility::sqrtMdotW (MobilityInterface.h:132-188)."""
+        linear, angular, mem = self._outputs_like_positions()
+        self._bind_stream(mem, _last_device)
+        _ffi.check(self._lib.lm_nbody_sqrt_mdot_w(self._h, _ptr(linear), _ptr(angular),
+                                                  float(prefactor), mem))
+        return linear, angular
+
+    def sqrtMdotW_with_noise(self, noise, prefactor=1.0):
+        """Extension: prefactor * M^{1/2} W for a given W (3N, or 6N with angular)."""
+        linear, angular, mem = self._outputs_like_positions()
+        w = _Arr(noise).moved_to(mem, _last_device)
+        n = self.getNumberParticles() * (6 if self._include_angular else 3)
+        if w.size != n:
+            raise RuntimeError(f"[libmobility_amd] noise must have {n} entries")
+        self._bind_stream(mem, _last_device)
+        _ffi.check(self._lib.lm_nbody_sqrt_mdot_w_noise(self._h, w.ptr, _ptr(linear), _ptr(angular),
+                                                        float(prefactor), mem))
+        return linear, angular
+
+    def divM(self, prefactor=1):
+        """pythonify.h:329-345 -> NBody::divM (mobility.h:184-236)."""
+        if self.getNumberParticles() <= 0:
+            raise RuntimeError(
+                "[libMobility] The number of particles is not set. Did you forget to call "
+                "setPositions?")
+        linear, angular, mem = self._outputs_like_positions()
+        self._bind_stream(mem, _last_device)
+        _ffi.check(self._lib.lm_nbody_div_m(self._h, _ptr(linear), _ptr(angular), float(prefactor),
+                                            mem))
+        return linear, angular
+
+    def LangevinVelocities(self, dt, kbt, forces=None, torques=None):
+        """pythonify.h:273-294 -> Mobility::LangevinVelocities (MobilityInterface.h:191-206)."""
+        f, t, mf, mt, mem, device = self._setup_arrays(forces, torques)
+        if f.size == 0:
+            mf = _zeros(_last_shape, _last_framework, _last_device)
+            mem = _mem_of(_last_framework, _last_device)
+            device = _last_device
+        if self._include_angular and t.size == 0:
+            mt = _zeros(_last_shape, _last_framework, _last_device)
+        f = f.moved_to(mem, device)
+        t = t.moved_to(mem, device)
+        self._bind_stream(mem, device)
+        _ffi.check(self._lib.lm_nbody_langevin_velocities(
+            self._h, float(dt), float(kbt), f.ptr, f.size, t.ptr, t.size, _ptr(mf), _ptr(mt), mem))
+        return mf, mt
+
+    def hydrodynamicVelocities(self, forces=None, torques=None):
+        """Mdot + M^{1/2} W + div M with unit prefactors (LangevinVelocities, dt=2, kbt=1)."""
+        return self.LangevinVelocities(2.0, 1.0, forces, torques)
+
+    def clean(self):
+        """Mobility::clean (MobilityInterface.h:220-223)."""
+        _ffi.check(self._lib.lm_nbody_clean(self._h))
+
+    def lastLanczosIterations(self):
+        """Extension: iteration index at which the last sqrtMdotW converged."""
+        return int(self._lib.lm_nbody_last_lanczos_iterations(self._h))

@@ -1,0 +1,127 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol that
+include/libmobility_amd.h declares, and its pair/generator arithmetic (run on
+the host through the self-check entry points) agrees with the oracle and
+with the published Philox4x32-10 known-answer vectors.  No GPU work here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "libmobility_amd.h")
+
+
+def _lib():
+    from libmobility_amd import _ffi
+    return _ffi.load()
+
+
+def _declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    from libmobility_amd import _ffi
+    declared = set(_declared_symbols())
+    assert declared == set(_ffi.SIGNATURES), declared ^ set(_ffi.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib()
+    for name in _declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_version_precision():
+    lib = _lib()
+    assert lib.lm_version() == b"1.1"
+    assert lib.lm_precision() == b"double"
+
+
+def _blk(wall, fp32, pi, pj, a):
+    lib = _lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    pi = np.ascontiguousarray(pi, float)
+    pj = np.ascontiguousarray(pj, float)
+    out = np.zeros(36)
+    assert lib.lm_selftest_pair_block_host(wall, fp32, pi.ctypes.data_as(dp), pj.ctypes.data_as(dp),
+                                           a, out.ctypes.data_as(dp)) == 0
+    return out.reshape(6, 6)
+
+
+@pytest.mark.parametrize("wall", [0, 1])
+def test_kernel_pair_arithmetic_fp64(wall):
+    """The regrouped wall/open arithmetic of pair_math.hpp vs the oracle's
+    reference-form matrices, incl. self pairs, overlaps and z < a."""
+    rng = np.random.default_rng(10 + wall)
+    for t in range(3000):
+        a = rng.uniform(0.5, 2.0)
+        pi = rng.uniform(-4, 4, 3) * a
+        pj = rng.uniform(-4, 4, 3) * a
+        if wall:
+            pi[2] = rng.uniform(-0.2, 4) * a
+            pj[2] = rng.uniform(-0.2, 4) * a
+        if t % 40 == 0:
+            pj = pi.copy()
+        B = _blk(wall, 0, pi, pj, a)
+        O = oracle.pair_block(wall, pi, pj, a)
+        assert np.abs(B - O).max() <= 1e-13 * max(np.abs(O).max(), 1e-300)
+
+
+@pytest.mark.parametrize("wall", [0, 1])
+def test_kernel_pair_arithmetic_fp32(wall):
+    rng = np.random.default_rng(20 + wall)
+    worst = 0.0
+    for t in range(1000):
+        pi = rng.uniform(-6, 6, 3)
+        pj = rng.uniform(-6, 6, 3)
+        if wall:
+            pi[2] = rng.uniform(0.5, 6)
+            pj[2] = rng.uniform(0.5, 6)
+        B = _blk(wall, 1, pi, pj, 1.0)
+        O = oracle.pair_block(wall, pi, pj, 1.0)
+        worst = max(worst, np.abs(B - O).max() / np.abs(O).max())
+    assert worst < 1e-5
+
+
+def test_philox_known_answers():
+    """Published Philox4x32-10 test vectors (Random123 kat_vectors)."""
+    lib = _lib()
+    U4 = ctypes.c_uint32 * 4
+    K2 = ctypes.c_uint32 * 2
+    cases = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in cases:
+        out = U4()
+        assert lib.lm_selftest_philox_host(U4(*ctr), K2(*key), out) == 0
+        assert list(out) == want
+
+
+def test_python_errors_before_device_work():
+    """Configuration errors of pythonify.h / mobility.h raise RuntimeError."""
+    from libmobility_amd import NBody
+    with pytest.raises(RuntimeError, match="Invalid periodicity"):
+        NBody("periodicasdas", "periodic", "open")
+    with pytest.raises(RuntimeError):
+        NBody("periodic", "periodic", "open")
+    s = NBody("open", "open", "open")
+    with pytest.raises(RuntimeError, match="only valid for bottom wall"):
+        s.setParameters(wallHeight=0.5)
+    s.setParameters("advise")
+    w = NBody("open", "open", "single_wall")
+    with pytest.raises(RuntimeError, match="required for a bottom wall"):
+        w.setParameters("advise")
+    w.setParameters(wallHeight=0.5)
+    with pytest.raises(RuntimeError, match="Invalid algorithm"):
+        s.setParameters("fmm")
+    assert NBody.precision == "double"
