@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--N", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 2 GiB device-to-device copy (reference for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -76,6 +78,16 @@ def main():
     import torch
     from libmobility_amd import _ffi
     from libmobility_amd.workloads import rsa_cube
+
+    if args.copy_calibration:
+        n = 1 << 28  # doubles: 2 GiB read + 2 GiB written per copy
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}), flush=True)
+        return
 
     lib = _ffi.load()
     dev = torch.device("cuda:0")
@@ -102,7 +114,13 @@ def main():
         _ffi.check(lib.lm_nbody_set_stream(h, ctypes.c_void_p(stream.cuda_stream)))
         _ffi.check(lib.lm_nbody_initialize(h, eta, 1.0, 0, tol, 12345, _ffi.LANCZOS_CALLBACK(), None))
         _ffi.check(lib.lm_nbody_set_positions(h, pos_d.data_ptr(), 3 * N, _ffi.LM_MEM_DEVICE))
+        _ffi.check(lib.lm_nbody_set_autotune(h, 1))
         return h
+
+    def tiling(h):
+        t, c = ctypes.c_int(), ctypes.c_int()
+        _ffi.check(lib.lm_nbody_get_pair_tiling(h, ctypes.byref(t), ctypes.byref(c)))
+        return {"targets_per_lane": t.value, "source_chunks": c.value}
 
     def time_mdot(h, steps, warmup):
         for _ in range(warmup):
@@ -152,6 +170,8 @@ def main():
     log(f"  {ms64:.3f} ms/Mdot, {gpair64:.1f} Gpair/s")
 
     extra = {
+        "tiling_fp32pair": tiling(h32),
+        "tiling_fp64pair": tiling(h64),
         "mdot_fp32pair_ms": ms32,
         "mdot_fp64pair_ms": ms64,
         "mdot_fp64pair_gpair_s": gpair64,

@@ -81,6 +81,19 @@ int lm_nbody_set_parameters(lm_nbody* h, int algorithm, int nbatch, int nperbatc
 /* Extension: pair arithmetic (LM_PAIR_FP64 default, LM_PAIR_FP32). */
 int lm_nbody_set_pair_precision(lm_nbody* h, int precision);
 
+/* Extension: pair-kernel tiling.  targets_per_lane in {0 (auto), 1, 2, 4};
+ * source_chunks >= 0 (0 = auto).  Fixes the work decomposition (the
+ * reference's blockDim choice, NbodyRPY.cu:105-108). */
+int lm_nbody_set_pair_tiling(lm_nbody* h, int targets_per_lane, int source_chunks);
+
+/* Extension: when enabled, the first product for a new (N, systems, kernel,
+ * precision, blocks) combination times every tiling candidate on the device
+ * and keeps the fastest for later calls. */
+int lm_nbody_set_autotune(lm_nbody* h, int enable);
+
+/* Extension: tiling used by the last pair-kernel launch. */
+int lm_nbody_get_pair_tiling(lm_nbody* h, int* targets_per_lane, int* source_chunks);
+
 /* Extension: order all work of this handle on `stream` (a hipStream_t). */
 int lm_nbody_set_stream(lm_nbody* h, void* stream);
 

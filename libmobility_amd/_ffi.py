@@ -35,6 +35,10 @@ SIGNATURES = {
                                                ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "lm_nbody_set_pair_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lm_nbody_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "lm_nbody_set_pair_tiling": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "lm_nbody_set_autotune": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lm_nbody_get_pair_tiling": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
+                                                ctypes.POINTER(ctypes.c_int)]),
     "lm_nbody_initialize": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                            ctypes.c_double, ctypes.c_uint64, LANCZOS_CALLBACK, _vp]),
     "lm_nbody_set_positions": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_int]),

@@ -70,6 +70,21 @@ int lm_nbody_set_pair_precision(lm_nbody* h, int precision) {
   return guarded([&] { S(h).setPairPrecision(precision); });
 }
 
+int lm_nbody_set_pair_tiling(lm_nbody* h, int tpl, int nch) {
+  return guarded([&] { S(h).setPairTiling(tpl, nch); });
+}
+
+int lm_nbody_set_autotune(lm_nbody* h, int enable) {
+  return guarded([&] { S(h).setAutotune(enable != 0); });
+}
+
+int lm_nbody_get_pair_tiling(lm_nbody* h, int* tpl, int* nch) {
+  return guarded([&] {
+    if (!tpl || !nch) throw lm::Error("[libmobility_amd] null output");
+    S(h).lastPairTiling(tpl, nch);
+  });
+}
+
 int lm_nbody_set_stream(lm_nbody* h, void* stream) {
   return guarded([&] { S(h).setStream(static_cast<hipStream_t>(stream)); });
 }

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(
       if (kNeedT) s_t[tid] = R4{R(0), R(0), R(0), R(0)};
     }
     __syncthreads();
-#pragma unroll 4
+#pragma unroll (TPL >= 4 ? 1 : 4 / TPL)
     for (int s = 0; s < kPairThreads; ++s) {
       const R4 pj = s_pos[s];
       R4 fj, tj;
@@ -149,10 +149,17 @@ static void launch_pair_t(const PairArgs& a, hipStream_t s) {
 
 template <class R, bool WALL, int BLK>
 static void launch_pair_tpl(const PairArgs& a, hipStream_t s) {
-  if (a.tpl == 2)
-    launch_pair_t<R, WALL, BLK, 2>(a, s);
-  else
-    launch_pair_t<R, WALL, BLK, 1>(a, s);
+  switch (a.tpl) {
+    case 1: launch_pair_t<R, WALL, BLK, 1>(a, s); break;
+    case 2: launch_pair_t<R, WALL, BLK, 2>(a, s); break;
+    case 4:
+      if constexpr (!WALL && BLK == kUF) {
+        launch_pair_t<R, WALL, BLK, 4>(a, s);
+        break;
+      }
+      [[fallthrough]];
+    default: throw Error("[libmobility_amd] unsupported targets-per-lane for this kernel");
+  }
 }
 
 template <class R, bool WALL>

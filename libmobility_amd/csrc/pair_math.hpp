@@ -51,9 +51,15 @@ __host__ __device__ inline float lm_rsqrt(float x) {
   return 1.0f / __builtin_sqrtf(x);
 #endif
 }
+// FP64: hardware estimate refined by one Newton step, y (1 + h/2) with
+// h = 1 - x y^2 (relative error 1.5 e0^2 for an estimate error e0).  No
+// special-value handling: x = 0 only occurs for self pairs, whose
+// coefficients are replaced by the overlap branch.
 __host__ __device__ inline double lm_rsqrt(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return rsqrt(x);
+  const double y = __builtin_amdgcn_rsq(x);
+  const double h = __builtin_fma(-x * y, y, 1.0);
+  return __builtin_fma(0.5 * y, h, y);
 #else
   return 1.0 / __builtin_sqrt(x);
 #endif
@@ -66,6 +72,8 @@ __host__ __device__ inline float lm_rcp(float x) {
 #endif
 }
 __host__ __device__ inline double lm_rcp(double x) { return 1.0 / x; }
+__host__ __device__ inline float lm_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__host__ __device__ inline double lm_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __host__ __device__ inline float lm_sqrt(float x) { return __builtin_sqrtf(x); }
 __host__ __device__ inline double lm_sqrt(double x) { return __builtin_sqrt(x); }
 
@@ -76,10 +84,11 @@ __host__ __device__ inline double lm_sqrt(double x) { return __builtin_sqrt(x); 
 // Far field (r >= 2) uses rsqrt; the overlap branch (r < 2, including the
 // self pair r = 0) is the regularised RPY of hydrodynamicKernels.cuh:51-55,
 // :73-84, :97-100.  It is a divergent branch so that waves with no near pair
-// skip it.
+// skip it.  For r = 0 the far-field values are infinite; they are overwritten
+// (never combined) by the overlap branch.
 template <class R, int BLK>
 __host__ __device__ inline void rpy_coeffs(R r2, R& fuf, R& guf, R& fwt, R& gwt, R& c) {
-  const R ir = lm_rsqrt(r2 >= R(4) ? r2 : R(4));
+  const R ir = lm_rsqrt(r2);
   const R ir2 = ir * ir;
   const R ir3 = ir * ir2;
   if (BLK & kUF) {
@@ -91,9 +100,15 @@ __host__ __device__ inline void rpy_coeffs(R r2, R& fuf, R& guf, R& fwt, R& gwt,
     gwt = R(1.5) * ir3 * ir2;
   }
   if (BLK & (kUT | kWF)) c = ir3;
-  if (r2 < R(4)) {
-    const R r = lm_sqrt(r2);
-    const R ir_n = r > R(0) ? R(1) / r : R(0);
+  if (__builtin_expect(r2 < R(4), 0)) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // An opaque statement keeps this a branch: speculating it (if-conversion)
+    // would put the overlap arithmetic on every pair.
+    __asm__ __volatile__("");
+#endif
+    // r and 1/r from the rsqrt already computed (no sqrt, no division)
+    const R r = r2 > R(0) ? r2 * ir : R(0);
+    const R ir_n = r2 > R(0) ? ir : R(0);
     if (BLK & kUF) {
       fuf = R(1) - R(0.28125) * r;
       guf = R(0.09375) * ir_n;
@@ -114,27 +129,28 @@ __host__ __device__ inline void open_pair(R dx, R dy, R dz, R fx, R fy, R fz, R 
   const R r2 = dx * dx + dy * dy + dz * dz;
   R fuf, guf, fwt, gwt, c;
   rpy_coeffs<R, BLK>(r2, fuf, guf, fwt, gwt, c);
+  // Accumulate as two FMAs per component: acc = fma(f, v, acc); acc = fma(s, d, acc).
   if (BLK & kUF) {
     const R s = guf * (dx * fx + dy * fy + dz * fz);
-    a.uf.x += fuf * fx + s * dx;
-    a.uf.y += fuf * fy + s * dy;
-    a.uf.z += fuf * fz + s * dz;
+    a.uf.x = lm_fma(s, dx, lm_fma(fuf, fx, a.uf.x));
+    a.uf.y = lm_fma(s, dy, lm_fma(fuf, fy, a.uf.y));
+    a.uf.z = lm_fma(s, dz, lm_fma(fuf, fz, a.uf.z));
   }
   if (BLK & kWT) {
     const R s = gwt * (dx * tx + dy * ty + dz * tz);
-    a.wt.x += fwt * tx + s * dx;
-    a.wt.y += fwt * ty + s * dy;
-    a.wt.z += fwt * tz + s * dz;
+    a.wt.x = lm_fma(s, dx, lm_fma(fwt, tx, a.wt.x));
+    a.wt.y = lm_fma(s, dy, lm_fma(fwt, ty, a.wt.y));
+    a.wt.z = lm_fma(s, dz, lm_fma(fwt, tz, a.wt.z));
   }
   if (BLK & kUT) {  // c (T x d)
-    a.ut.x += c * (ty * dz - tz * dy);
-    a.ut.y += c * (tz * dx - tx * dz);
-    a.ut.z += c * (tx * dy - ty * dx);
+    a.ut.x = lm_fma(c, ty * dz - tz * dy, a.ut.x);
+    a.ut.y = lm_fma(c, tz * dx - tx * dz, a.ut.y);
+    a.ut.z = lm_fma(c, tx * dy - ty * dx, a.ut.z);
   }
   if (BLK & kWF) {  // c (F x d)
-    a.wf.x += c * (fy * dz - fz * dy);
-    a.wf.y += c * (fz * dx - fx * dz);
-    a.wf.z += c * (fx * dy - fy * dx);
+    a.wf.x = lm_fma(c, fy * dz - fz * dy, a.wf.x);
+    a.wf.y = lm_fma(c, fz * dx - fx * dz, a.wf.y);
+    a.wf.z = lm_fma(c, fx * dy - fy * dx, a.wf.z);
   }
 }
 

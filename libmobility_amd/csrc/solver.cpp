@@ -201,11 +201,11 @@ void NBodySolver::ensure_items(int tpl, int nch_hint) {
   }
   int nch = nch_hint;
   if (nch <= 0) {
-    const int64_t want = 2048;  // ~8 workgroups per CU
+    const int64_t want = 8192;  // ~32 workgroups per CU: short tail, balanced XCDs
     nch = (int)std::max<int64_t>(1, (want + std::max<int64_t>(tiles, 1) - 1) / std::max<int64_t>(tiles, 1));
   }
   const int maxch = (int)std::max<int64_t>(1, (maxn + kPairThreads - 1) / kPairThreads);
-  nch = std::max(1, std::min({nch, 16, maxch}));
+  nch = std::max(1, std::min({nch, 32, maxch}));
   if (items_tpl_ == tpl && items_nch_ == nch && items_N_ == N_ && items_layout_ == layout_id_)
     return;
   std::vector<PairItem> items;
@@ -238,35 +238,25 @@ void NBodySolver::ensure_items(int tpl, int nch_hint) {
   items_count_ = (int)items.size();
 }
 
-// U (3N) and, when angular, W (3N) are overwritten (NbodyRPY.cu:91-96).
-void NBodySolver::pair_product(const void* rec64, const void* rec32, const double* F,
-                               const double* T, double* U, double* W) {
-  int blocks = 0;
-  // Block selection of NbodyRPY.cu:264-275.
-  if (F) blocks |= kUF | (angular_ ? kWF : 0);
-  if (T && angular_) blocks |= kUT | kWT;
+void NBodySolver::setPairTiling(int tpl, int nch) {
+  if (tpl != 0 && tpl != 1 && tpl != 2 && tpl != 4)
+    throw Error("[libmobility_amd] targets per lane must be 0 (auto), 1, 2 or 4");
+  if (nch < 0) throw Error("[libmobility_amd] source chunks must be >= 0");
+  tile_tpl_ = tpl;
+  tile_nch_ = nch;
+  items_tpl_ = -1;
+}
+
+// Launch the pair kernel over the current work list (items_).
+void NBodySolver::launch_pair_items(int blocks, bool fp32, const void* rec, const double* F,
+                                    const double* T, double* U, double* W) {
   const int64_t n3 = 3 * N_;
-  if (blocks == 0) {
-    LM_HIP(hipMemsetAsync(U, 0, sizeof(double) * n3, stream_));
-    if (angular_ && W) LM_HIP(hipMemsetAsync(W, 0, sizeof(double) * n3, stream_));
-    return;
-  }
-  const bool fp32 = pair_prec_ == kPairFP32;
-  int tpl = 1, nch_hint = 0;
-  if (algorithm_ == kNaive) {
-    nch_hint = 1;
-  } else if (algorithm_ == kBlock) {
-    nch_hint = 16;
-  } else if (!wall_ && blocks == kUF && fp32) {
-    tpl = 2;
-  }
-  ensure_items(tpl, nch_hint);
   PairArgs a{};
   a.wall = wall_ ? 1 : 0;
   a.fp32 = fp32 ? 1 : 0;
   a.blocks = blocks;
-  a.tpl = tpl;
-  a.pos = fp32 ? rec32 : rec64;
+  a.tpl = items_tpl_;
+  a.pos = rec;
   a.F = F;
   a.T = (blocks & (kUT | kWT)) ? T : nullptr;
   a.items = items_.as<PairItem>();
@@ -275,10 +265,6 @@ void NBodySolver::pair_product(const void* rec64, const void* rec32, const doubl
   a.rt0 = rt0_;
   a.r0 = r0_;
   const bool ang = (blocks & (kWT | kWF)) != 0;
-  if (!U) throw Error("[libmobility_amd] Mdot needs a linear output array");
-  if (ang && !W)
-    throw Error("[libmobility_amd] the solver includes angular velocities: an angular output array is required");
-  if (items_count_ <= 0) throw Error("[libmobility_amd] empty pair work list");
   if (items_nch_ == 1) {
     a.U = U;
     a.W = ang ? W : nullptr;
@@ -294,6 +280,103 @@ void NBodySolver::pair_product(const void* rec64, const void* rec32, const doubl
     launch_chunk_reduce(part_u_.as<double>(), items_nch_, n3, U, stream_);
     if (ang) launch_chunk_reduce(part_w_.as<double>(), items_nch_, n3, W, stream_);
   }
+}
+
+// Tiling: user override, else the autotuned choice, else the default rule.
+void NBodySolver::choose_tiling(int blocks, bool fp32, const double* F, const double* T,
+                                double* U, double* W, const void* rec, int& tpl, int& nch) {
+  const bool tpl4_ok = !wall_ && blocks == kUF;
+  if (tile_tpl_ != 0 || tile_nch_ != 0) {
+    tpl = tile_tpl_ != 0 ? tile_tpl_ : 1;
+    if (tpl == 4 && !tpl4_ok) tpl = 2;
+    nch = tile_nch_;
+    return;
+  }
+  tpl = 1;
+  nch = 0;
+  if (algorithm_ == kNaive) {
+    nch = 1;
+    return;
+  }
+  if (algorithm_ == kBlock) {
+    nch = 32;
+    return;
+  }
+  // Measured on MI355X: two targets per lane pay off once the system is
+  // large enough to keep every CU busy with 512-target tiles.
+  int64_t maxn = 0;
+  for (size_t s = 0; s + 1 < sys_off_host_.size(); ++s)
+    maxn = std::max(maxn, sys_off_host_[s + 1] - sys_off_host_[s]);
+  if (maxn >= 16384) tpl = 2;
+  if (!autotune_) return;
+  const int nsys = (int)sys_off_host_.size() - 1;
+  const auto key = std::make_tuple(N_, nsys, wall_ ? 1 : 0, fp32 ? 1 : 0, blocks);
+  auto it = tuned_.find(key);
+  if (it != tuned_.end()) {
+    tpl = it->second.first;
+    nch = it->second.second;
+    return;
+  }
+  // Time every candidate on the live inputs (outputs are overwritten by the
+  // real launch that follows).
+  hipEvent_t e0, e1;
+  LM_HIP(hipEventCreate(&e0));
+  LM_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_tpl = tpl, best_nch = 0;
+  const int tpls[3] = {1, 2, 4};
+  const int nchs[6] = {0, 1, 2, 4, 8, 16};
+  for (int ti = 0; ti < 3; ++ti) {
+    if (tpls[ti] == 4 && !tpl4_ok) continue;
+    for (int ci = 0; ci < 6; ++ci) {
+      ensure_items(tpls[ti], nchs[ci]);
+      if (nchs[ci] != 0 && items_nch_ != nchs[ci]) continue;  // clamped duplicate
+      launch_pair_items(blocks, fp32, rec, F, T, U, W);
+      LM_HIP(hipEventRecord(e0, stream_));
+      launch_pair_items(blocks, fp32, rec, F, T, U, W);
+      launch_pair_items(blocks, fp32, rec, F, T, U, W);
+      LM_HIP(hipEventRecord(e1, stream_));
+      LM_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      LM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_tpl = items_tpl_;
+        best_nch = items_nch_;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  tuned_[key] = {best_tpl, best_nch};
+  tpl = best_tpl;
+  nch = best_nch;
+}
+
+// U (3N) and, when angular, W (3N) are overwritten (NbodyRPY.cu:91-96).
+void NBodySolver::pair_product(const void* rec64, const void* rec32, const double* F,
+                               const double* T, double* U, double* W) {
+  int blocks = 0;
+  // Block selection of NbodyRPY.cu:264-275.
+  if (F) blocks |= kUF | (angular_ ? kWF : 0);
+  if (T && angular_) blocks |= kUT | kWT;
+  const int64_t n3 = 3 * N_;
+  if (!U) throw Error("[libmobility_amd] Mdot needs a linear output array");
+  if (blocks == 0) {
+    LM_HIP(hipMemsetAsync(U, 0, sizeof(double) * n3, stream_));
+    if (angular_ && W) LM_HIP(hipMemsetAsync(W, 0, sizeof(double) * n3, stream_));
+    return;
+  }
+  const bool ang = (blocks & (kWT | kWF)) != 0;
+  if (ang && !W)
+    throw Error("[libmobility_amd] the solver includes angular velocities: an angular output array is required");
+  const bool fp32 = pair_prec_ == kPairFP32;
+  const void* rec = fp32 ? rec32 : rec64;
+  int tpl = 1, nch = 0;
+  choose_tiling(blocks, fp32, F, T, U, W, rec, tpl, nch);
+  ensure_items(tpl, nch);
+  if (items_count_ <= 0) throw Error("[libmobility_amd] empty pair work list");
+  launch_pair_items(blocks, fp32, rec, F, T, U, W);
   if (angular_ && W && !ang) LM_HIP(hipMemsetAsync(W, 0, sizeof(double) * n3, stream_));
 }
 

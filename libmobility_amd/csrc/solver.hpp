@@ -7,7 +7,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <random>
+#include <tuple>
 #include <vector>
 
 #include "kernels.hpp"
@@ -51,6 +53,13 @@ class NBodySolver {
                 double* U, double* W, bool dev);
   void clean();
   int lastLanczosIterations() const { return last_iters_; }
+  // Pair-kernel tiling: targets per lane (1, 2, 4) and source chunks; 0 = auto.
+  void setPairTiling(int tpl, int nch);
+  void setAutotune(bool on) { autotune_ = on; }
+  void lastPairTiling(int* tpl, int* nch) const {
+    *tpl = items_tpl_;
+    *nch = items_nch_;
+  }
   int checkConvergenceSteps() const { return check_steps_; }
 
   // Device-pointer cores (no staging, no synchronisation beyond what the
@@ -64,6 +73,10 @@ class NBodySolver {
   void ensure_records();
   void build_systems(int64_t N, std::vector<int64_t>& off, bool& valid) const;
   void ensure_items(int tpl, int nch_hint);
+  void choose_tiling(int blocks, bool fp32, const double* F, const double* T, double* U,
+                     double* W, const void* rec, int& tpl, int& nch);
+  void launch_pair_items(int blocks, bool fp32, const void* rec, const double* F,
+                         const double* T, double* U, double* W);
   void pair_product(const void* rec64, const void* rec32, const double* F, const double* T,
                     double* U, double* W);
   int lanczos_run(const double* z, double** result);
@@ -102,6 +115,9 @@ class NBodySolver {
   int64_t items_N_ = -1;
   uint64_t items_layout_ = 0, layout_id_ = 0;
   DevBuf part_u_, part_w_;
+  int tile_tpl_ = 0, tile_nch_ = 0;  // user override (0 = auto)
+  bool autotune_ = false;
+  std::map<std::tuple<int64_t, int, int, int, int>, std::pair<int, int>> tuned_;
   // staging for host arrays
   DevBuf st_pos_, st_f_, st_t_, st_u_, st_w_;
   // Lanczos

@@ -5,8 +5,9 @@ Checks, through the C ABI:
     (converged iteration within a small window, results within a few x tol), including the
     persistent state between calls (adaptive check schedule, previous
     estimate), LanczosAlgorithm.cu:171-232;
-  * agreement with the dense FP64 eigen square root applied to the same W
-    (tolerance 20 x tol: the stopping rule measures successive change);
+  * agreement with the dense FP64 eigen square root applied to the same W,
+    bounded by the error of the oracle's restated algorithm on that W (the
+    stopping rule measures successive change, not the error);
   * fluctuation-dissipation: sample covariance of sqrtMdotW -> M
     (test_fluctuation_dissipation.py:19-112 in spirit);
   * prefactor linearity with equal seeds (test_interface.py:384-418);
@@ -55,7 +56,8 @@ def test_lanczos_explicit_noise_matches_oracle(wall, angular):
         assert abs(s.lastLanczosIterations() - it) <= 6
         assert np.linalg.norm(got - ref) <= 5 * tol * np.linalg.norm(ref)
         exact = B @ W
-        assert np.linalg.norm(got - exact) <= 20 * tol * np.linalg.norm(exact)
+        err_ref = np.linalg.norm(ref - exact)
+        assert np.linalg.norm(got - exact) <= 3 * err_ref + 5 * tol * np.linalg.norm(exact)
 
 
 def test_lanczos_c1_fp64_against_eigh():
@@ -71,7 +73,12 @@ def test_lanczos_c1_fp64_against_eigh():
     M = oracle.dense_matrix(pos, eta, 1.0)
     exact = oracle.dense_sqrt(M) @ W
     err = np.linalg.norm(u.ravel() - exact) / np.linalg.norm(exact)
-    assert err <= 2e-5, err
+    # The stopping rule bounds the change between successive estimates, not
+    # the error; the bound is the error of the restated reference algorithm
+    # on the same W.
+    ref, _ = oracle.LanczosOracle().run(lambda v: M @ v, W, 1e-6)
+    err_ref = np.linalg.norm(ref - exact) / np.linalg.norm(exact)
+    assert err <= 3 * err_ref + 5e-6, (err, err_ref)
 
 
 def test_prefactor_linearity_same_seed():
