@@ -187,7 +187,16 @@ def main():
         ms_l64, it64 = time_lanczos(h64, max(1, args.lanczos_calls - 2))
         extra["sqrtMdotW_plain_fp64pair_ms"] = ms_l64
         extra["sqrtMdotW_plain_fp64pair_iterations"] = it64
-        extra["sqrtMdotW_learned_ms"] = None
+        log("learned-preconditioned Lanczos sqrtMdotW, FP32 pairs, tol 1e-6")
+        from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
+        net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+        hl = make_solver("fp32")
+        _ffi.check(lib.lm_nbody_set_preconditioner(hl, net._h))
+        ms_ll, itl = time_lanczos(hl, args.lanczos_calls)
+        lib.lm_nbody_destroy(hl)
+        extra["sqrtMdotW_learned_fp32pair_ms"] = ms_ll
+        extra["sqrtMdotW_learned_fp32pair_iterations"] = itl
+        extra["sqrtMdotW_learned_speedup_vs_plain"] = ms_l32 / ms_ll
     for h in (h32, h64):
         lib.lm_nbody_destroy(h)
 

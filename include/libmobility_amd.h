@@ -157,6 +157,43 @@ int lm_nbody_last_lanczos_iterations(lm_nbody* h);
 /* Extension: wait for all work of the handle's stream. */
 int lm_nbody_synchronize(lm_nbody* h);
 
+/* ---- learned block preconditioner (no counterpart in the reference) ------
+ * A per-particle network maps the relative positions of a particle's 16
+ * nearest neighbours (48 floats, units of a) through 48-128-128-6 (GELU,
+ * BF16 MFMA, FP32 accumulation) to the Cholesky factor of an SPD 3x3 block
+ * G_i.  With a preconditioner attached, sqrtMdotW runs Lanczos on
+ * G M G and returns G^{-1} (G M G)^{1/2} W, another square root of M.
+ * All arrays of this group are host arrays. */
+typedef struct lm_precond lm_precond;
+
+int lm_precond_create(lm_precond** out);
+int lm_precond_destroy(lm_precond* p);
+/* Number of FP32 weights: W1[48][128] b1[128] W2[128][128] b2[128] W3[128][6] b3[6]. */
+int64_t lm_precond_num_weights(void);
+int lm_precond_init_weights(lm_precond* p, uint64_t seed);
+int lm_precond_set_weights(lm_precond* p, const float* weights, int64_t n);
+int lm_precond_get_weights(lm_precond* p, float* weights, int64_t n);
+/* Mini-batch Adam on (X: n x 48, T9: n x 9) with the GPU forward/backward
+ * kernels; loss_history (epochs entries, may be NULL) gets each epoch's mean
+ * loss sum_ij (G - T)_ij^2. */
+int lm_precond_train(lm_precond* p, const float* X, const float* T9, int64_t n, int epochs,
+                     int batch, float learning_rate, uint64_t seed, float* loss_history);
+/* Network outputs (n x 6) for inputs X (n x 48). */
+int lm_precond_forward(lm_precond* p, const float* X, int64_t n, float* out6);
+/* Mean loss and its gradient w.r.t. all weights (lm_precond_num_weights
+ * entries) for one batch. */
+int lm_precond_loss_grad(lm_precond* p, const float* X, const float* T9, int64_t n, float* grad,
+                         float* loss);
+/* Training pairs from the solver's current positions: features of ncenters
+ * random particles and the center 3x3 block of (M_local/t0)^{-1/2} of each
+ * (open RPY, particle + 16 neighbours, FP64 on the device). */
+int lm_nbody_precond_dataset(lm_nbody* h, int64_t ncenters, uint64_t seed, float* X, float* T9);
+/* Attach (p != NULL) or detach (p == NULL) a preconditioner; p must outlive
+ * its use by h. */
+int lm_nbody_set_preconditioner(lm_nbody* h, lm_precond* p);
+/* The blocks G (9N doubles, row-major 3x3 per particle) for the positions. */
+int lm_nbody_precond_blocks(lm_nbody* h, double* G9);
+
 /* ---- self-checks of the kernel arithmetic, run on the host CPU -----------
  * Not used by any solver path.  They evaluate the same pair and generator
  * code that the kernels are compiled from, so CPU tests can check it

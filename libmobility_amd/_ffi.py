@@ -54,6 +54,19 @@ SIGNATURES = {
     "lm_nbody_clean": (ctypes.c_int, [_vp]),
     "lm_nbody_last_lanczos_iterations": (ctypes.c_int, [_vp]),
     "lm_nbody_synchronize": (ctypes.c_int, [_vp]),
+    "lm_precond_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "lm_precond_destroy": (ctypes.c_int, [_vp]),
+    "lm_precond_num_weights": (_i64, []),
+    "lm_precond_init_weights": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "lm_precond_set_weights": (ctypes.c_int, [_vp, _vp, _i64]),
+    "lm_precond_get_weights": (ctypes.c_int, [_vp, _vp, _i64]),
+    "lm_precond_train": (ctypes.c_int, [_vp, _vp, _vp, _i64, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_float, ctypes.c_uint64, _vp]),
+    "lm_precond_forward": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
+    "lm_precond_loss_grad": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    "lm_nbody_precond_dataset": (ctypes.c_int, [_vp, _i64, ctypes.c_uint64, _vp, _vp]),
+    "lm_nbody_set_preconditioner": (ctypes.c_int, [_vp, _vp]),
+    "lm_nbody_precond_blocks": (ctypes.c_int, [_vp, _vp]),
     "lm_selftest_pair_block_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp,
                                                    ctypes.c_double, _dp]),
     "lm_selftest_philox_host": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32),

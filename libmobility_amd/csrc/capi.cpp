@@ -13,6 +13,10 @@ struct lm_nbody {
   lm_nbody(int px, int py, int pz) : solver(px, py, pz) {}
 };
 
+struct lm_precond {
+  lm::LearnedPreconditioner net;
+};
+
 namespace {
 thread_local std::string g_last_error;
 
@@ -149,6 +153,85 @@ int lm_nbody_last_lanczos_iterations(lm_nbody* h) { return h ? h->solver.lastLan
 
 int lm_nbody_synchronize(lm_nbody* h) {
   return guarded([&] { LM_HIP(hipStreamSynchronize(S(h).stream())); });
+}
+
+int lm_precond_create(lm_precond** out) {
+  return guarded([&] {
+    if (!out) throw lm::Error("[libmobility_amd] null output handle");
+    *out = nullptr;
+    *out = new lm_precond();
+  });
+}
+
+int lm_precond_destroy(lm_precond* p) {
+  return guarded([&] { delete p; });
+}
+
+int64_t lm_precond_num_weights(void) { return lm::kPrecondNumWeights; }
+
+static lm::LearnedPreconditioner& P(lm_precond* p) {
+  if (!p) throw lm::Error("[libmobility_amd] null preconditioner handle");
+  return p->net;
+}
+
+int lm_precond_init_weights(lm_precond* p, uint64_t seed) {
+  return guarded([&] { P(p).initWeights(seed); });
+}
+
+int lm_precond_set_weights(lm_precond* p, const float* w, int64_t n) {
+  return guarded([&] { P(p).setWeights(w, n); });
+}
+
+int lm_precond_get_weights(lm_precond* p, float* w, int64_t n) {
+  return guarded([&] { P(p).getWeights(w, n); });
+}
+
+int lm_precond_train(lm_precond* p, const float* X, const float* T9, int64_t n, int epochs,
+                     int batch, float lr, uint64_t seed, float* loss_history) {
+  return guarded([&] { P(p).train(X, T9, n, epochs, batch, lr, seed, loss_history, nullptr); });
+}
+
+int lm_precond_forward(lm_precond* p, const float* X, int64_t n, float* out6) {
+  return guarded([&] {
+    lm::DevBuf x, o;
+    x.reserve(sizeof(float) * n * lm::kPrecondIn);
+    o.reserve(sizeof(float) * n * lm::kPrecondOut);
+    LM_HIP(hipMemcpy(x.as<float>(), X, sizeof(float) * n * lm::kPrecondIn, hipMemcpyHostToDevice));
+    P(p).forward(x.as<float>(), (int)n, o.as<float>(), nullptr);
+    LM_HIP(hipMemcpy(out6, o.as<float>(), sizeof(float) * n * lm::kPrecondOut, hipMemcpyDeviceToHost));
+  });
+}
+
+int lm_precond_loss_grad(lm_precond* p, const float* X, const float* T9, int64_t n, float* grad,
+                         float* loss) {
+  return guarded([&] {
+    lm::DevBuf x, t;
+    x.reserve(sizeof(float) * n * lm::kPrecondIn);
+    t.reserve(sizeof(float) * n * 9);
+    LM_HIP(hipMemcpy(x.as<float>(), X, sizeof(float) * n * lm::kPrecondIn, hipMemcpyHostToDevice));
+    LM_HIP(hipMemcpy(t.as<float>(), T9, sizeof(float) * n * 9, hipMemcpyHostToDevice));
+    const float l = P(p).lossGrad(x.as<float>(), t.as<float>(), (int)n, nullptr);
+    if (loss) *loss = l;
+    LM_HIP(hipMemcpy(grad, P(p).grad(), sizeof(float) * lm::kPrecondNumWeights, hipMemcpyDeviceToHost));
+  });
+}
+
+int lm_nbody_precond_dataset(lm_nbody* h, int64_t ncenters, uint64_t seed, float* X, float* T9) {
+  return guarded([&] {
+    if (!X || !T9) throw lm::Error("[libmobility_amd] null output");
+    S(h).precondDataset(ncenters, seed, X, T9);
+  });
+}
+
+int lm_nbody_set_preconditioner(lm_nbody* h, lm_precond* p) {
+  return guarded([&] { S(h).setPreconditioner(p ? &p->net : nullptr); });
+}
+
+int lm_nbody_precond_blocks(lm_nbody* h, double* G9) {
+  return guarded([&] {
+    if (!G9) throw lm::Error("[libmobility_amd] null output");
+    S(h).precondBlocks(G9);
+  });
 }
 
 int lm_selftest_pair_block_host(int wall, int fp32, const double* pi, const double* pj, double a,

@@ -389,6 +389,7 @@ void NBodySolver::ensure_records() {
   launch_make_records(shifted_.as<double>(), N_, sys_off_.as<int64_t>(), nsys, bbox_.as<double>(),
                       wall_ ? 1 : 0, 0.0, a_, nullptr, rec64_.as<void>(), rec32_.as<void>(), stream_);
   rec_a_ = a_;
+  ++layout_id_;  // features are in units of a
 }
 
 void NBodySolver::mdot_dev(const double* F, const double* T, double* U, double* W) {
@@ -490,6 +491,11 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     V_.grow_keep(sizeof(double) * n * want, sizeof(double) * n * V_cols_, stream_);
     V_cols_ = want;
   };
+  if (precond_) {
+    ensure_precond_blocks();
+    pc_t_.reserve(sizeof(double) * n);
+    pc_u_.reserve(sizeof(double) * n);
+  }
   ensure_cols(2);
   double* V = V_.as<double>();
   double* w = w_.as<double>();
@@ -500,8 +506,18 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     V = V_.as<double>();
     const double* vi = V + (int64_t)i * n;
     // w = M v_i ; torques are stored after the forces (MobilityInterface.h:164-176)
-    pair_product(rec64_.as<void>(), rec32_.as<void>(), vi, angular_ ? vi + N3 : nullptr, w,
-                 angular_ ? w + N3 : nullptr);
+    if (precond_) {
+      // w = G M G v_i with the learned block-diagonal G
+      double* t = pc_t_.as<double>();
+      double* u = pc_u_.as<double>();
+      launch_apply_blocks(pc_G_.as<double>(), vi, N_, n, t, stream_);
+      pair_product(rec64_.as<void>(), rec32_.as<void>(), t, angular_ ? t + N3 : nullptr, u,
+                   angular_ ? u + N3 : nullptr);
+      launch_apply_blocks(pc_G_.as<double>(), u, N_, n, w, stream_);
+    } else {
+      pair_product(rec64_.as<void>(), rec32_.as<void>(), vi, angular_ ? vi + N3 : nullptr, w,
+                   angular_ ? w + N3 : nullptr);
+    }
     launch_lanczos_alpha(w, vi, i > 0 ? V + (int64_t)(i - 1) * n : nullptr, n, i, rs, sc, stream_);
     launch_lanczos_beta(w, vi, n, i, rs, sc, stream_);
     launch_lanczos_next(w, V + (int64_t)(i + 1) * n, n, i, sc, stream_);
@@ -549,6 +565,11 @@ void NBodySolver::sqrtMdotW_dev(double* U, double* W, double prefactor, const do
   }
   double* y = nullptr;
   lanczos_run(z, &y);
+  if (precond_) {
+    // B W = G^{-1} (G M G)^{1/2} W
+    launch_apply_blocks(pc_Gi_.as<double>(), y, N_, n, pc_t_.as<double>(), stream_);
+    y = pc_t_.as<double>();
+  }
   // out += prefactor * y (MobilityInterface.h:179-187)
   launch_axpy_accumulate(U, y, prefactor, 3 * N_, stream_);
   if (angular_ && W) launch_axpy_accumulate(W, y + 3 * N_, prefactor, 3 * N_, stream_);
@@ -586,6 +607,71 @@ void NBodySolver::sqrtMdotW(double* U, double* W, double prefactor, bool dev,
   LM_HIP(hipMemcpyAsync(U, st_u_.as<double>(), sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
   if (angular_ && W)
     LM_HIP(hipMemcpyAsync(W, st_w_.as<double>(), sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));
+}
+
+// ---- learned preconditioner ------------------------------------------------------
+
+void NBodySolver::ensure_knn() {
+  require_positions("[Mobility] Positions have 0 particles. Did you call setPositions?");
+  ensure_records();
+  if (knn_layout_ == layout_id_ && knn_layout_ != 0) return;
+  pc_feat_.reserve(sizeof(float) * N_ * kPrecondIn);
+  pc_nbr_.reserve(sizeof(int) * N_ * kPrecondK);
+  const int nsys = (int)sys_off_host_.size() - 1;
+  launch_knn(rec64_.as<double4>(), sys_off_.as<int64_t>(), nsys, N_, pc_feat_.as<float>(),
+             pc_nbr_.as<int>(), stream_);
+  knn_layout_ = layout_id_;
+}
+
+void NBodySolver::ensure_precond_blocks() {
+  if (!precond_) throw Error("[libmobility_amd] no preconditioner attached");
+  ensure_knn();
+  if (pc_layout_ == layout_id_ && pc_version_ == precond_->version()) return;
+  pc_out_.reserve(sizeof(float) * N_ * kPrecondOut);
+  pc_G_.reserve(sizeof(double) * 9 * N_);
+  pc_Gi_.reserve(sizeof(double) * 9 * N_);
+  precond_->forward(pc_feat_.as<float>(), (int)N_, pc_out_.as<float>(), stream_);
+  launch_make_blocks(pc_out_.as<float>(), pc_nbr_.as<int>(), N_, pc_G_.as<double>(),
+                     pc_Gi_.as<double>(), stream_);
+  pc_layout_ = layout_id_;
+  pc_version_ = precond_->version();
+}
+
+void NBodySolver::precondDataset(int64_t ncenters, uint64_t seed, float* X, float* T9) {
+  ensure_knn();
+  if (ncenters <= 0 || ncenters > N_) throw Error("[libmobility_amd] invalid number of centers");
+  // distinct random centers
+  std::vector<int64_t> idx(N_);
+  for (int64_t i = 0; i < N_; ++i) idx[i] = i;
+  std::mt19937_64 rng(seed);
+  for (int64_t i = 0; i < ncenters; ++i) {
+    std::uniform_int_distribution<int64_t> d(i, N_ - 1);
+    std::swap(idx[i], idx[d(rng)]);
+  }
+  DevBuf centers, tg, st, xf;
+  centers.reserve(sizeof(int64_t) * ncenters);
+  tg.reserve(sizeof(float) * 9 * ncenters);
+  st.reserve(sizeof(int) * ncenters);
+  xf.reserve(sizeof(float) * kPrecondIn * ncenters);
+  LM_HIP(hipMemcpyAsync(centers.as<int64_t>(), idx.data(), sizeof(int64_t) * ncenters,
+                        hipMemcpyHostToDevice, stream_));
+  launch_local_targets(rec64_.as<double4>(), pc_nbr_.as<int>(), centers.as<int64_t>(), ncenters,
+                       tg.as<float>(), st.as<int>(), stream_);
+  launch_gather_rows(pc_feat_.as<float>(), centers.as<int64_t>(), (int)ncenters, kPrecondIn,
+                     xf.as<float>(), stream_);
+  std::vector<int> status(ncenters);
+  LM_HIP(hipMemcpyAsync(status.data(), st.as<int>(), sizeof(int) * ncenters, hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipMemcpyAsync(T9, tg.as<float>(), sizeof(float) * 9 * ncenters, hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipMemcpyAsync(X, xf.as<float>(), sizeof(float) * kPrecondIn * ncenters, hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));
+  for (int64_t c = 0; c < ncenters; ++c)
+    if (status[c] != 0) throw Error("[libmobility_amd] local inverse square root did not converge");
+}
+
+void NBodySolver::precondBlocks(double* G9) {
+  ensure_precond_blocks();
+  LM_HIP(hipMemcpyAsync(G9, pc_G_.as<double>(), sizeof(double) * 9 * N_, hipMemcpyDeviceToHost, stream_));
   LM_HIP(hipStreamSynchronize(stream_));
 }
 

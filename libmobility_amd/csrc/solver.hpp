@@ -15,6 +15,7 @@
 #include "kernels.hpp"
 #include "lm_common.hpp"
 #include "pair_math.hpp"
+#include "precond.hpp"
 
 namespace lm {
 
@@ -61,6 +62,15 @@ class NBodySolver {
     *nch = items_nch_;
   }
   int checkConvergenceSteps() const { return check_steps_; }
+
+  // Learned block preconditioner for sqrtMdotW (nullptr: plain Lanczos).  The
+  // caller keeps the object alive while it is attached.
+  void setPreconditioner(LearnedPreconditioner* p) { precond_ = p; pc_layout_ = 0; }
+  // Training pairs from the current positions: X (ncenters x 48) and the FP64
+  // local targets (ncenters x 9), host arrays.
+  void precondDataset(int64_t ncenters, uint64_t seed, float* X, float* T9);
+  // The 3x3 blocks G (9N, host) the attached network gives for the positions.
+  void precondBlocks(double* G9);
 
   // Device-pointer cores (no staging, no synchronisation beyond what the
   // algorithm needs).
@@ -128,6 +138,12 @@ class NBodySolver {
   int64_t old_n_ = 0;
   int check_steps_ = 3;          // LanczosAlgorithm.cu:171
   int last_iters_ = 0;
+  // learned preconditioner
+  void ensure_knn();
+  void ensure_precond_blocks();
+  LearnedPreconditioner* precond_ = nullptr;
+  DevBuf pc_feat_, pc_nbr_, pc_out_, pc_G_, pc_Gi_, pc_t_, pc_u_;
+  uint64_t knn_layout_ = 0, pc_layout_ = 0, pc_version_ = 0;
   // RFD
   DevBuf rfd_noise_, rfd_pos_, rfd_rec64_, rfd_rec32_, rfd_mp_, rfd_mm_, rfd_wp_, rfd_wm_;
 };

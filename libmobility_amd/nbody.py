@@ -145,6 +145,7 @@ class NBody:
         _ffi.check(self._lib.lm_nbody_create(per[0], per[1], per[2], ctypes.byref(h)))
         self._h = h
         self._cb = None
+        self._precond = None
         self._include_angular = False
         self._stream_set = None
 
@@ -181,6 +182,32 @@ class NBody:
         off[1:] = np.cumsum(sizes)
         self._sys_off = off
         _ffi.check(self._lib.lm_nbody_set_systems(self._h, off.ctypes.data, int(sizes.size)))
+
+    def setPreconditioner(self, preconditioner="learned"):
+        """Extension: attach a learned block preconditioner to sqrtMdotW.
+
+        ``"learned"`` loads the packaged trained weights; a
+        :class:`libmobility_amd.precond.LearnedPreconditioner` is used as is;
+        ``None`` returns to plain Lanczos.
+        """
+        from .precond import DEFAULT_WEIGHTS, LearnedPreconditioner
+        if preconditioner is None:
+            self._precond = None
+            _ffi.check(self._lib.lm_nbody_set_preconditioner(self._h, None))
+            return
+        if isinstance(preconditioner, str):
+            if preconditioner != "learned":
+                raise RuntimeError("[libmobility_amd] preconditioner must be 'learned', an object or None")
+            preconditioner = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+        self._precond = preconditioner  # keep alive while attached
+        _ffi.check(self._lib.lm_nbody_set_preconditioner(self._h, preconditioner._h))
+
+    def preconditionerBlocks(self):
+        """Extension: the 3x3 blocks G_i (N, 3, 3) of the attached preconditioner."""
+        N = self.getNumberParticles()
+        G = np.zeros((N, 3, 3))
+        _ffi.check(self._lib.lm_nbody_precond_blocks(self._h, G.ctypes.data))
+        return G
 
     def initialize(self, viscosity, hydrodynamicRadius, includeAngular=False, tolerance=1e-4,
                    seed=None, lanczos_callback=None):
