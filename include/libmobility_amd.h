@@ -108,9 +108,21 @@ int lm_nbody_set_positions(lm_nbody* h, const double* positions, int64_t n3, int
 
 /* Extension: ragged independent systems.  offsets[0] = 0 <= ... <=
  * offsets[nsys] = N; pairs interact only inside a system (generalises the
- * equal-size batches of mobility.h:56-77).  Applies to the next
+ * equal-size batches of mobility.h:56-77).  kinds[s] = 0 (open RPY) or 1
+ * (bottom wall at wallHeight) per system, or NULL for the solver's
+ * periodicity; mixed ensembles run in one launch.  Applies to the next
  * lm_nbody_set_positions. */
-int lm_nbody_set_systems(lm_nbody* h, const int64_t* offsets, int nsys);
+int lm_nbody_set_systems(lm_nbody* h, const int64_t* offsets, int nsys, const int* kinds);
+
+/* Extension: persistent pair launches (a fixed grid draining a work queue
+ * of items sorted longest first).  mode: -1 auto (mixed ensembles), 0 off,
+ * 1 on. */
+int lm_nbody_set_persistent(lm_nbody* h, int mode);
+
+/* lanczos::Solver::iterationHardLimit, LanczosAlgorithm.h:49 (a public member
+ * there, default 200): iterations before "[Lanczos] Could not converge".
+ * Range [3, 1024]; the default stays 200. */
+int lm_nbody_set_lanczos_iteration_limit(lm_nbody* h, int limit);
 
 /* NBody::getNumberParticles, mobility.h:164. */
 int64_t lm_nbody_num_particles(lm_nbody* h);

@@ -104,11 +104,22 @@ int lm_nbody_set_positions(lm_nbody* h, const double* pos, int64_t n3, int mem) 
   return guarded([&] { S(h).setPositions(pos, n3, is_dev(mem)); });
 }
 
-int lm_nbody_set_systems(lm_nbody* h, const int64_t* offsets, int nsys) {
+int lm_nbody_set_systems(lm_nbody* h, const int64_t* offsets, int nsys, const int* kinds) {
   return guarded([&] {
     if (!offsets) throw lm::Error("[libmobility_amd] null offsets");
-    S(h).setSystems(offsets, nsys);
+    S(h).setSystems(offsets, nsys, kinds);
   });
+}
+
+int lm_nbody_set_persistent(lm_nbody* h, int mode) {
+  return guarded([&] {
+    if (mode < -1 || mode > 1) throw lm::Error("[libmobility_amd] persistent mode must be -1, 0 or 1");
+    S(h).setPersistent(mode);
+  });
+}
+
+int lm_nbody_set_lanczos_iteration_limit(lm_nbody* h, int limit) {
+  return guarded([&] { S(h).setLanczosIterationLimit(limit); });
 }
 
 int64_t lm_nbody_num_particles(lm_nbody* h) { return h ? h->solver.numParticles() : -1; }

@@ -13,10 +13,15 @@ struct PairItem {
   int64_t src1;
   int32_t ntgt;
   int32_t chunk;
+  int32_t wall;  // kernel of the item's system: 0 open RPY, 1 bottom wall
+  int32_t pad;
 };
 
 struct PairArgs {
-  int wall;       // 0 open RPY, 1 bottom wall
+  int wall;       // 0 open RPY, 1 bottom wall, 2 per item (mixed ensembles)
+  int persistent; // 1: grid of `grid` workgroups fetching items through `counter`
+  int grid;
+  unsigned int* counter;  // zero before the launch (persistent mode)
   int fp32;       // 1: FP32 pair terms, FP64 per-tile accumulation
   int blocks;     // BlockBits mask
   int tpl;        // targets per lane (1 or 2)
@@ -38,12 +43,16 @@ void launch_chunk_reduce(const double* part, int nchunks, int64_t n, double* out
 
 // setPositions: per-system bounding boxes (6 doubles each: min xyz, max xyz)
 // of the wall-shifted positions.
+// kinds[s] (0 open, 1 wall) selects per system; nullptr: all systems `wall`.
 void launch_system_bbox(const double* pos, const int64_t* sys_off, int nsys, double wall_shift,
-                        double* bbox, hipStream_t s);
+                        const int* kinds, int wall, double* bbox, hipStream_t s);
 // Records for the pair kernels; `shifted` receives the wall-shifted AoS copy.
 void launch_make_records(const double* pos, int64_t N, const int64_t* sys_off, int nsys,
-                         const double* bbox, int wall, double wall_shift, double a,
-                         double* shifted, void* rec64, void* rec32, hipStream_t s);
+                         const double* bbox, const int* kinds, int wall, double wall_shift,
+                         double a, double* shifted, void* rec64, void* rec32, hipStream_t s);
+// v[3i..3i+2] = 0 for the particles of open systems (kinds[s] == 0)
+void launch_zero_open_systems(double* v, int64_t N, const int64_t* sys_off, int nsys,
+                              const int* kinds, hipStream_t s);
 
 // ---- Lanczos / vector kernels (FP64) ---------------------------------------
 // Normal(0,1) samples from Philox4x32-10 + Box-Muller: out[k], k < n.
@@ -57,11 +66,14 @@ struct ReduceScratch {
   unsigned int* counter; // 1 counter, zero at rest
 };
 
+// Largest Krylov dimension the device-side Lanczos state can hold.
+constexpr int kLanczosMaxIterations = 1024;
+
 // Lanczos scalar state resident on the device.
 struct LanczosScalars {
   double normz;    // ||z||
-  double alpha[256];  // hdiag
-  double beta[256];   // hsup
+  double alpha[kLanczosMaxIterations];  // hdiag
+  double beta[kLanczosMaxIterations];   // hsup
   double err;      // last error estimate
   double errnum, errden;
 };
@@ -77,9 +89,9 @@ void launch_lanczos_beta(double* w, const double* v_i, int64_t n, int i, ReduceS
 // v_next = w / beta[i], or e1 after breakdown
 void launch_lanczos_next(const double* w, double* v_next, int64_t n, int i,
                          const LanczosScalars* sc, hipStream_t s);
-// s[0..m) = T_m^{1/2} e1 (FP64, one workgroup); work has kTsqrtScratch doubles
+// s[0..m) = T_m^{1/2} e1 (FP64, one workgroup); work has tsqrt_scratch(m) doubles
 constexpr int kTsqrtNodes = 512;
-constexpr int kTsqrtScratch = 2 * 256 * kTsqrtNodes;
+constexpr int64_t tsqrt_scratch(int m) { return 2 * (int64_t)m * kTsqrtNodes; }
 void launch_tridiag_sqrt(const LanczosScalars* sc, int m, double* s_out, double* work,
                          hipStream_t s);
 // y = normz * V[:, 0:m] s ; err = ||y - old|| / ||old||

@@ -248,7 +248,8 @@ __device__ inline int sturm_count(const double* a, const double* b2, int m, doub
 __global__ __launch_bounds__(kTS) void k_tridiag_sqrt(const LanczosScalars* __restrict__ sc,
                                                       int m, double* __restrict__ s_out,
                                                       double* __restrict__ work) {
-  __shared__ double a[256], b[256], b2[256], xs[256];
+  __shared__ double a[kLanczosMaxIterations], b[kLanczosMaxIterations],
+      b2[kLanczosMaxIterations], xs[kLanczosMaxIterations];
   __shared__ double red[kTS / 64];
   __shared__ int ired[kTS / 64];
   __shared__ double s_lo, s_hi;
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(kTS) void k_tridiag_sqrt(const LanczosScalars* __re
     h = (u1 - u0) / (double)(kTS - 1);
   }
   double* cp = work;                      // [m][kTS]
-  double* dp = work + 256 * kTS;          // [m][kTS]
+  double* dp = work + (int64_t)m * kTS;   // [m][kTS]
   if (t < K) {
     const double u = u0 + h * (double)t;
     const double sig = exp(2.0 * u);
@@ -376,7 +377,8 @@ __global__ __launch_bounds__(kTS) void k_tridiag_sqrt(const LanczosScalars* __re
 
 void launch_tridiag_sqrt(const LanczosScalars* sc, int m, double* s_out, double* work,
                          hipStream_t s) {
-  if (m < 1 || m > 256) throw Error("[libmobility_amd] tridiagonal size out of range");
+  if (m < 1 || m > kLanczosMaxIterations)
+    throw Error("[libmobility_amd] tridiagonal size out of range");
   hipLaunchKernelGGL(k_tridiag_sqrt, dim3(1), dim3(kTS), 0, s, sc, m, s_out, work);
   LM_HIP(hipGetLastError());
 }
@@ -388,7 +390,7 @@ __global__ __launch_bounds__(kRT) void k_estimate(const double* __restrict__ V, 
                                                  double* __restrict__ y, double* __restrict__ old,
                                                  int64_t n, double* partial, unsigned* counter,
                                                  LanczosScalars* sc) {
-  __shared__ double sc_s[256];
+  __shared__ double sc_s[kLanczosMaxIterations];
   __shared__ double sh[kRT / 64];
   const double nz = sc->normz;
   for (int k = threadIdx.x; k < m; k += kRT) sc_s[k] = nz * coef[k];

@@ -44,20 +44,17 @@ __device__ inline R4 far_record() {
   return r;
 }
 
+// One item: targets [tgt0, tgt0 + ntgt) x sources [src0, src1) of one system.
 template <class R, bool WALL, int BLK, int TPL>
-__global__ __launch_bounds__(kPairThreads) void k_pair(
-    const PairItem* __restrict__ items, const typename Rec<R>::T* __restrict__ pos,
-    const double* __restrict__ F, const double* __restrict__ T, double* __restrict__ U,
-    double* __restrict__ W, int64_t slab, double t0, double rt0, double r0) {
+__device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* __restrict__ pos,
+                                 const double* __restrict__ F, const double* __restrict__ T,
+                                 double* __restrict__ U, double* __restrict__ W, int64_t slab,
+                                 double t0, double rt0, double r0, typename Rec<R>::T* s_pos,
+                                 typename Rec<R>::T* s_f, typename Rec<R>::T* s_t) {
   using R4 = typename Rec<R>::T;
   constexpr bool kNeedF = (BLK & (kUF | kWF)) != 0;
   constexpr bool kNeedT = (BLK & (kUT | kWT)) != 0;
   constexpr bool kAngular = (BLK & (kWT | kWF)) != 0;
-  __shared__ R4 s_pos[kPairThreads];
-  __shared__ R4 s_f[kNeedF ? kPairThreads : 1];
-  __shared__ R4 s_t[kNeedT ? kPairThreads : 1];
-
-  const PairItem it = items[blockIdx.x];
   const int tid = threadIdx.x;
 
   R4 pi[TPL];
@@ -138,23 +135,69 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(
   }
 }
 
-template <class R, bool WALL, int BLK, int TPL>
+// KIND: 0 open, 1 wall, 2 per item.  PERSIST: the grid loops over items,
+// fetching the next one from `counter` (work queue; every workgroup exits
+// once the counter passes nitems).
+template <class R, int KIND, int BLK, int TPL, bool PERSIST>
+__global__ __launch_bounds__(kPairThreads) void k_pair(
+    const PairItem* __restrict__ items, int nitems, unsigned int* __restrict__ counter,
+    const typename Rec<R>::T* __restrict__ pos, const double* __restrict__ F,
+    const double* __restrict__ T, double* __restrict__ U, double* __restrict__ W, int64_t slab,
+    double t0, double rt0, double r0) {
+  using R4 = typename Rec<R>::T;
+  constexpr bool kNeedF = (BLK & (kUF | kWF)) != 0;
+  constexpr bool kNeedT = (BLK & (kUT | kWT)) != 0;
+  __shared__ R4 s_pos[kPairThreads];
+  __shared__ R4 s_f[kNeedF ? kPairThreads : 1];
+  __shared__ R4 s_t[kNeedT ? kPairThreads : 1];
+  __shared__ int s_item;
+  int item = blockIdx.x;
+  while (true) {
+    if (PERSIST) {
+      __syncthreads();  // previous item finished with s_item and the tiles
+      if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+      __syncthreads();
+      item = s_item;
+    }
+    if (item >= nitems) break;
+    const PairItem it = items[item];
+    if (KIND == 2) {
+      if (it.wall)
+        pair_item<R, true, BLK, TPL>(it, pos, F, T, U, W, slab, t0, rt0, r0, s_pos, s_f, s_t);
+      else
+        pair_item<R, false, BLK, TPL>(it, pos, F, T, U, W, slab, t0, rt0, r0, s_pos, s_f, s_t);
+    } else {
+      pair_item<R, KIND == 1, BLK, TPL>(it, pos, F, T, U, W, slab, t0, rt0, r0, s_pos, s_f, s_t);
+    }
+    if (!PERSIST) break;
+  }
+}
+
+template <class R, int KIND, int BLK, int TPL>
 static void launch_pair_t(const PairArgs& a, hipStream_t s) {
   using R4 = typename Rec<R>::T;
-  hipLaunchKernelGGL((k_pair<R, WALL, BLK, TPL>), dim3(a.nitems), dim3(kPairThreads), 0, s,
-                     a.items, static_cast<const R4*>(a.pos), a.F, a.T, a.U, a.W, a.slab, a.t0,
-                     a.rt0, a.r0);
+  if (a.persistent) {
+    if (!a.counter) throw Error("[libmobility_amd] persistent launch without a work counter");
+    LM_HIP(hipMemsetAsync(a.counter, 0, sizeof(unsigned int), s));
+    hipLaunchKernelGGL((k_pair<R, KIND, BLK, TPL, true>), dim3(a.grid), dim3(kPairThreads), 0, s,
+                       a.items, a.nitems, a.counter, static_cast<const R4*>(a.pos), a.F, a.T, a.U,
+                       a.W, a.slab, a.t0, a.rt0, a.r0);
+  } else {
+    hipLaunchKernelGGL((k_pair<R, KIND, BLK, TPL, false>), dim3(a.nitems), dim3(kPairThreads), 0,
+                       s, a.items, a.nitems, a.counter, static_cast<const R4*>(a.pos), a.F, a.T,
+                       a.U, a.W, a.slab, a.t0, a.rt0, a.r0);
+  }
   LM_HIP(hipGetLastError());
 }
 
-template <class R, bool WALL, int BLK>
+template <class R, int KIND, int BLK>
 static void launch_pair_tpl(const PairArgs& a, hipStream_t s) {
   switch (a.tpl) {
-    case 1: launch_pair_t<R, WALL, BLK, 1>(a, s); break;
-    case 2: launch_pair_t<R, WALL, BLK, 2>(a, s); break;
+    case 1: launch_pair_t<R, KIND, BLK, 1>(a, s); break;
+    case 2: launch_pair_t<R, KIND, BLK, 2>(a, s); break;
     case 4:
-      if constexpr (!WALL && BLK == kUF) {
-        launch_pair_t<R, WALL, BLK, 4>(a, s);
+      if constexpr (KIND == 0 && BLK == kUF) {
+        launch_pair_t<R, KIND, BLK, 4>(a, s);
         break;
       }
       [[fallthrough]];
@@ -162,26 +205,33 @@ static void launch_pair_tpl(const PairArgs& a, hipStream_t s) {
   }
 }
 
-template <class R, bool WALL>
+template <class R, int KIND>
 static void launch_pair_blk(const PairArgs& a, hipStream_t s) {
   switch (a.blocks) {
-    case kUF: launch_pair_tpl<R, WALL, kUF>(a, s); break;
-    case kUF | kWF: launch_pair_tpl<R, WALL, kUF | kWF>(a, s); break;
-    case kUT | kWT: launch_pair_tpl<R, WALL, kUT | kWT>(a, s); break;
-    case kUF | kUT | kWT | kWF: launch_pair_tpl<R, WALL, kUF | kUT | kWT | kWF>(a, s); break;
+    case kUF: launch_pair_tpl<R, KIND, kUF>(a, s); break;
+    case kUF | kWF: launch_pair_tpl<R, KIND, kUF | kWF>(a, s); break;
+    case kUT | kWT: launch_pair_tpl<R, KIND, kUT | kWT>(a, s); break;
+    case kUF | kUT | kWT | kWF: launch_pair_tpl<R, KIND, kUF | kUT | kWT | kWF>(a, s); break;
     default: throw Error("[libmobility_amd] unsupported block combination");
+  }
+}
+
+template <class R>
+static void launch_pair_kind(const PairArgs& a, hipStream_t s) {
+  switch (a.wall) {
+    case 0: launch_pair_blk<R, 0>(a, s); break;
+    case 1: launch_pair_blk<R, 1>(a, s); break;
+    case 2: launch_pair_blk<R, 2>(a, s); break;
+    default: throw Error("[libmobility_amd] invalid kernel kind");
   }
 }
 
 void launch_pair(const PairArgs& a, hipStream_t s) {
   if (a.nitems <= 0) return;
-  if (a.fp32) {
-    if (a.wall) launch_pair_blk<float, true>(a, s);
-    else launch_pair_blk<float, false>(a, s);
-  } else {
-    if (a.wall) launch_pair_blk<double, true>(a, s);
-    else launch_pair_blk<double, false>(a, s);
-  }
+  if (a.fp32)
+    launch_pair_kind<float>(a, s);
+  else
+    launch_pair_kind<double>(a, s);
 }
 
 __global__ void k_chunk_reduce(const double* __restrict__ part, int nch, int64_t n,
@@ -205,10 +255,12 @@ void launch_chunk_reduce(const double* part, int nch, int64_t n, double* out, hi
 
 __global__ __launch_bounds__(256) void k_system_bbox(const double* __restrict__ pos,
                                                      const int64_t* __restrict__ off,
-                                                     double wall_shift,
+                                                     double wall_shift_in,
+                                                     const int* __restrict__ kinds, int wall,
                                                      double* __restrict__ bbox) {
   const int s = blockIdx.x;
   const int64_t b = off[s], e = off[s + 1];
+  const double wall_shift = (kinds ? kinds[s] : wall) ? wall_shift_in : 0.0;
   double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
   for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
     for (int c = 0; c < 3; ++c) {
@@ -240,16 +292,18 @@ __global__ __launch_bounds__(256) void k_system_bbox(const double* __restrict__ 
 }
 
 void launch_system_bbox(const double* pos, const int64_t* sys_off, int nsys, double wall_shift,
-                        double* bbox, hipStream_t s) {
+                        const int* kinds, int wall, double* bbox, hipStream_t s) {
   if (nsys <= 0) return;
-  hipLaunchKernelGGL(k_system_bbox, dim3(nsys), dim3(256), 0, s, pos, sys_off, wall_shift, bbox);
+  hipLaunchKernelGGL(k_system_bbox, dim3(nsys), dim3(256), 0, s, pos, sys_off, wall_shift, kinds,
+                     wall, bbox);
   LM_HIP(hipGetLastError());
 }
 
 __global__ void k_make_records(const double* __restrict__ pos, int64_t N,
                                const int64_t* __restrict__ off, int nsys,
-                               const double* __restrict__ bbox, int wall, double wall_shift,
-                               double a, double* __restrict__ shifted, double4* __restrict__ rec64,
+                               const double* __restrict__ bbox, const int* __restrict__ kinds,
+                               int wall_all, double wall_shift, double a,
+                               double* __restrict__ shifted, double4* __restrict__ rec64,
                                float4* __restrict__ rec32) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -259,6 +313,7 @@ __global__ void k_make_records(const double* __restrict__ pos, int64_t N,
     if (off[mid] <= i) lo = mid; else hi = mid;
   }
   const double* bb = bbox + 6 * lo;
+  const int wall = kinds ? kinds[lo] : wall_all;
   const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2] - (wall ? wall_shift : 0.0);
   if (shifted) {
     shifted[3 * i] = x;
@@ -279,13 +334,37 @@ __global__ void k_make_records(const double* __restrict__ pos, int64_t N,
 }
 
 void launch_make_records(const double* pos, int64_t N, const int64_t* sys_off, int nsys,
-                         const double* bbox, int wall, double wall_shift, double a,
-                         double* shifted, void* rec64, void* rec32, hipStream_t s) {
+                         const double* bbox, const int* kinds, int wall, double wall_shift,
+                         double a, double* shifted, void* rec64, void* rec32, hipStream_t s) {
   if (N <= 0) return;
   const int nt = 256;
   hipLaunchKernelGGL(k_make_records, dim3((unsigned)((N + nt - 1) / nt)), dim3(nt), 0, s, pos, N,
-                     sys_off, nsys, bbox, wall, wall_shift, a, shifted,
+                     sys_off, nsys, bbox, kinds, wall, wall_shift, a, shifted,
                      static_cast<double4*>(rec64), static_cast<float4*>(rec32));
+  LM_HIP(hipGetLastError());
+}
+
+__global__ void k_zero_open(double* __restrict__ v, int64_t N, const int64_t* __restrict__ off,
+                            int nsys, const int* __restrict__ kinds) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int lo = 0, hi = nsys;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid;
+  }
+  if (!kinds[lo]) {
+    v[3 * i] = 0.0;
+    v[3 * i + 1] = 0.0;
+    v[3 * i + 2] = 0.0;
+  }
+}
+
+void launch_zero_open_systems(double* v, int64_t N, const int64_t* sys_off, int nsys,
+                              const int* kinds, hipStream_t s) {
+  if (N <= 0 || !kinds) return;
+  hipLaunchKernelGGL(k_zero_open, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, v, N, sys_off,
+                     nsys, kinds);
   LM_HIP(hipGetLastError());
 }
 

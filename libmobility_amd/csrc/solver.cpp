@@ -20,7 +20,6 @@
 
 namespace lm {
 
-static constexpr int kIterationHardLimit = 200;  // LanczosAlgorithm.h:49
 
 NBodySolver::NBodySolver(int px, int py, int pz) {
   if (px < 0 || px > 4 || py < 0 || py > 4 || pz < 0 || pz > 4)
@@ -122,12 +121,18 @@ void NBodySolver::build_systems(int64_t N, std::vector<int64_t>& off, bool& vali
   for (int64_t b = 0; b <= nb; ++b) off[b] = b * npb;
 }
 
-void NBodySolver::setSystems(const int64_t* offsets, int nsys) {
+void NBodySolver::setSystems(const int64_t* offsets, int nsys, const int* kinds) {
   if (nsys < 1) throw Error("[libmobility_amd] need at least one system");
   if (offsets[0] != 0) throw Error("[libmobility_amd] system offsets must start at 0");
   for (int s = 0; s < nsys; ++s)
     if (offsets[s + 1] < offsets[s]) throw Error("[libmobility_amd] system offsets must be non-decreasing");
   sys_off_host_.assign(offsets, offsets + nsys + 1);
+  explicit_kinds_.clear();
+  if (kinds) {
+    for (int s = 0; s < nsys; ++s)
+      if (kinds[s] != 0 && kinds[s] != 1) throw Error("[libmobility_amd] system kind must be 0 (open) or 1 (wall)");
+    explicit_kinds_.assign(kinds, kinds + nsys);
+  }
   explicit_systems_ = true;
 }
 
@@ -143,6 +148,19 @@ void NBodySolver::setPositions(const double* pos, int64_t n3, bool dev) {
   if (explicit_systems_ && off.back() != N)
     throw Error("[libmobility_amd] system offsets do not cover the positions");
   const int nsys = (int)off.size() - 1;
+  std::vector<int> kinds(nsys, wall_ ? 1 : 0);
+  const bool use_kinds = explicit_systems_ && !explicit_kinds_.empty() &&
+                         (int)explicit_kinds_.size() == nsys && valid;
+  if (use_kinds) kinds = explicit_kinds_;
+  const int* kinds_d = nullptr;
+  if (N > 0 && use_kinds) {
+    sys_kind_.reserve(sizeof(int) * nsys);
+    LM_HIP(hipMemcpyAsync(sys_kind_.as<int>(), kinds.data(), sizeof(int) * nsys,
+                          hipMemcpyHostToDevice, stream_));
+    kinds_d = sys_kind_.as<int>();
+  }
+  bool any_wall = false, any_open = false;
+  for (int k : kinds) (k ? any_wall : any_open) = true;
   if (N > 0) {
     const double* src = pos;
     if (!dev) {
@@ -155,16 +173,17 @@ void NBodySolver::setPositions(const double* pos, int64_t n3, bool dev) {
     LM_HIP(hipMemcpyAsync(sys_off_.as<int64_t>(), off.data(), sizeof(int64_t) * off.size(),
                           hipMemcpyHostToDevice, stream_));
     bbox_.reserve(sizeof(double) * 6 * nsys);
-    const double shift = wall_ ? wall_height_ : 0.0;
-    launch_system_bbox(src, sys_off_.as<int64_t>(), nsys, shift, bbox_.as<double>(), stream_);
-    if (wall_) {
+    const double shift = wall_height_;
+    launch_system_bbox(src, sys_off_.as<int64_t>(), nsys, shift, kinds_d, wall_ ? 1 : 0,
+                       bbox_.as<double>(), stream_);
+    if (any_wall) {
       std::vector<double> bb(6 * (size_t)nsys);
       LM_HIP(hipMemcpyAsync(bb.data(), bbox_.as<double>(), sizeof(double) * bb.size(),
                             hipMemcpyDeviceToHost, stream_));
       LM_HIP(hipStreamSynchronize(stream_));
       double minz = 1e300;
       for (int s = 0; s < nsys; ++s)
-        if (off[s + 1] > off[s]) minz = std::min(minz, bb[6 * s + 2]);
+        if (off[s + 1] > off[s] && kinds[s]) minz = std::min(minz, bb[6 * s + 2]);
       if (minz < 0)
         throw Error(
             "[Mobility] The center of a particle has fallen below the wall. Check your "
@@ -173,12 +192,16 @@ void NBodySolver::setPositions(const double* pos, int64_t n3, bool dev) {
     shifted_.reserve(sizeof(double) * n3);
     rec64_.reserve(sizeof(double) * 4 * N);
     rec32_.reserve(sizeof(float) * 4 * N);
-    launch_make_records(src, N, sys_off_.as<int64_t>(), nsys, bbox_.as<double>(), wall_ ? 1 : 0,
-                        shift, a_, shifted_.as<double>(), rec64_.as<void>(), rec32_.as<void>(),
-                        stream_);
+    launch_make_records(src, N, sys_off_.as<int64_t>(), nsys, bbox_.as<double>(), kinds_d,
+                        wall_ ? 1 : 0, shift, a_, shifted_.as<double>(), rec64_.as<void>(),
+                        rec32_.as<void>(), stream_);
     rec_a_ = a_;
   }
   N_ = N;
+  sys_kind_host_ = kinds;
+  kinds_dev_valid_ = kinds_d != nullptr;
+  any_wall_ = any_wall;
+  any_open_ = any_open;
   if (!explicit_systems_) sys_off_host_ = off;
   ++layout_id_;
   items_tpl_ = -1;
@@ -206,13 +229,15 @@ void NBodySolver::ensure_items(int tpl, int nch_hint) {
   }
   const int maxch = (int)std::max<int64_t>(1, (maxn + kPairThreads - 1) / kPairThreads);
   nch = std::max(1, std::min({nch, 32, maxch}));
-  if (items_tpl_ == tpl && items_nch_ == nch && items_N_ == N_ && items_layout_ == layout_id_)
+  if (items_tpl_ == tpl && items_nch_ == nch && items_N_ == N_ && items_layout_ == layout_id_ &&
+      items_persist_ == use_persistent())
     return;
   std::vector<PairItem> items;
   items.reserve((size_t)tiles * nch);
   for (int s = 0; s < nsys; ++s) {
     const int64_t b = off[s], e = off[s + 1], ns = e - b;
     if (ns <= 0) continue;
+    const int kind = s < (int)sys_kind_host_.size() ? sys_kind_host_[s] : (wall_ ? 1 : 0);
     int64_t cs = (ns + nch - 1) / nch;
     cs = ((cs + kPairThreads - 1) / kPairThreads) * kPairThreads;
     for (int64_t t0 = b; t0 < e; t0 += bt) {
@@ -223,9 +248,19 @@ void NBodySolver::ensure_items(int tpl, int nch_hint) {
         it.src0 = std::min<int64_t>(b + (int64_t)c * cs, e);
         it.src1 = std::min<int64_t>(b + (int64_t)(c + 1) * cs, e);
         it.chunk = c;
+        it.wall = kind;
+        it.pad = 0;
         items.push_back(it);
       }
     }
+  }
+  if (use_persistent()) {
+    // work queue: longest items first (LPT order) for a balanced drain
+    auto cost = [](const PairItem& q) {
+      return (double)q.ntgt * (double)(q.src1 - q.src0) * (q.wall ? 5.0 : 1.0);
+    };
+    std::stable_sort(items.begin(), items.end(),
+                     [&](const PairItem& x, const PairItem& y) { return cost(x) > cost(y); });
   }
   items_.reserve(sizeof(PairItem) * std::max<size_t>(items.size(), 1));
   LM_HIP(hipMemcpyAsync(items_.as<PairItem>(), items.data(), sizeof(PairItem) * items.size(),
@@ -235,6 +270,7 @@ void NBodySolver::ensure_items(int tpl, int nch_hint) {
   items_nch_ = nch;
   items_N_ = N_;
   items_layout_ = layout_id_;
+  items_persist_ = use_persistent();
   items_count_ = (int)items.size();
 }
 
@@ -252,7 +288,13 @@ void NBodySolver::launch_pair_items(int blocks, bool fp32, const void* rec, cons
                                     const double* T, double* U, double* W) {
   const int64_t n3 = 3 * N_;
   PairArgs a{};
-  a.wall = wall_ ? 1 : 0;
+  a.wall = kind_code();
+  a.persistent = use_persistent() ? 1 : 0;
+  if (a.persistent) {
+    work_counter_.reserve(sizeof(unsigned int) * 64);
+    a.counter = work_counter_.as<unsigned int>();
+    a.grid = std::min(items_count_, 1024);  // 4 workgroups per CU drain the queue
+  }
   a.fp32 = fp32 ? 1 : 0;
   a.blocks = blocks;
   a.tpl = items_tpl_;
@@ -285,7 +327,7 @@ void NBodySolver::launch_pair_items(int blocks, bool fp32, const void* rec, cons
 // Tiling: user override, else the autotuned choice, else the default rule.
 void NBodySolver::choose_tiling(int blocks, bool fp32, const double* F, const double* T,
                                 double* U, double* W, const void* rec, int& tpl, int& nch) {
-  const bool tpl4_ok = !wall_ && blocks == kUF;
+  const bool tpl4_ok = kind_code() == 0 && blocks == kUF;
   if (tile_tpl_ != 0 || tile_nch_ != 0) {
     tpl = tile_tpl_ != 0 ? tile_tpl_ : 1;
     if (tpl == 4 && !tpl4_ok) tpl = 2;
@@ -310,7 +352,8 @@ void NBodySolver::choose_tiling(int blocks, bool fp32, const double* F, const do
   if (maxn >= 16384) tpl = 2;
   if (!autotune_) return;
   const int nsys = (int)sys_off_host_.size() - 1;
-  const auto key = std::make_tuple(N_, nsys, wall_ ? 1 : 0, fp32 ? 1 : 0, blocks);
+  const auto key = std::make_tuple(N_, nsys, kind_code() + 4 * (use_persistent() ? 1 : 0),
+                                   fp32 ? 1 : 0, blocks);
   auto it = tuned_.find(key);
   if (it != tuned_.end()) {
     tpl = it->second.first;
@@ -385,9 +428,11 @@ void NBodySolver::pair_product(const void* rec64, const void* rec32, const doubl
 void NBodySolver::ensure_records() {
   if (N_ <= 0 || rec_a_ == a_) return;
   const int nsys = (int)sys_off_host_.size() - 1;
-  launch_system_bbox(shifted_.as<double>(), sys_off_.as<int64_t>(), nsys, 0.0, bbox_.as<double>(), stream_);
+  launch_system_bbox(shifted_.as<double>(), sys_off_.as<int64_t>(), nsys, 0.0, kinds_dev(),
+                     wall_ ? 1 : 0, bbox_.as<double>(), stream_);
   launch_make_records(shifted_.as<double>(), N_, sys_off_.as<int64_t>(), nsys, bbox_.as<double>(),
-                      wall_ ? 1 : 0, 0.0, a_, nullptr, rec64_.as<void>(), rec32_.as<void>(), stream_);
+                      kinds_dev(), wall_ ? 1 : 0, 0.0, a_, nullptr, rec64_.as<void>(),
+                      rec32_.as<void>(), stream_);
   rec_a_ = a_;
   ++layout_id_;  // features are in units of a
 }
@@ -438,6 +483,13 @@ void NBodySolver::Mdot(const double* F, int64_t nF, const double* T, int64_t nT,
 
 // ---- Lanczos ---------------------------------------------------------------------
 
+void NBodySolver::setLanczosIterationLimit(int limit) {
+  if (limit < 3 || limit > kLanczosMaxIterations)
+    throw Error("[libmobility_amd] Lanczos iteration limit must be in [3, " +
+                std::to_string(kLanczosMaxIterations) + "]");
+  iter_limit_ = limit;
+}
+
 int NBodySolver::lanczos_run(const double* z, double** result) {
   const int64_t n = (angular_ ? 6 : 3) * N_;
   const int64_t N3 = 3 * N_;
@@ -449,8 +501,8 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     LM_HIP(hipMemsetAsync(red_counter_.as<void>(), 0, sizeof(unsigned int) * 64, stream_));
   }
   scalars_.reserve(sizeof(LanczosScalars));
-  tsq_work_.reserve(sizeof(double) * kTsqrtScratch);
-  coef_.reserve(sizeof(double) * 256);
+  tsq_work_.reserve(sizeof(double) * tsqrt_scratch(iter_limit_));
+  coef_.reserve(sizeof(double) * iter_limit_);
   w_.reserve(sizeof(double) * n);
   ReduceScratch rs{red_partial_.as<double>(), red_counter_.as<unsigned int>()};
   LanczosScalars* sc = scalars_.as<LanczosScalars>();
@@ -486,7 +538,7 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
       V_n_ = n;
     }
     if (cols <= V_cols_) return;
-    int64_t want = std::max<int64_t>(cols, std::min<int64_t>(kIterationHardLimit + 1,
+    int64_t want = std::max<int64_t>(cols, std::min<int64_t>(iter_limit_ + 1,
                                                             std::max<int64_t>(32, 2 * V_cols_)));
     V_.grow_keep(sizeof(double) * n * want, sizeof(double) * n * V_cols_, stream_);
     V_cols_ = want;
@@ -500,8 +552,8 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
   double* V = V_.as<double>();
   double* w = w_.as<double>();
   launch_lanczos_start(z, V, n, rs, sc, stream_);  // LanczosAlgorithm.cu:110-118
-  const int check = std::min(check_steps_, kIterationHardLimit - 2);
-  for (int i = 0; i < kIterationHardLimit; ++i) {
+  const int check = std::min(check_steps_, iter_limit_ - 2);
+  for (int i = 0; i < iter_limit_; ++i) {
     ensure_cols(i + 2);
     V = V_.as<double>();
     const double* vi = V + (int64_t)i * n;
@@ -695,18 +747,26 @@ void NBodySolver::divM_dev(double* U, double* W, double prefactor) {
   double* noise = rfd_noise_.as<double>();
   launch_normal_fill(noise, n3, seed, 1, stream_);
   const int nsys = (int)sys_off_host_.size() - 1;
+  // open systems of a mixed ensemble have no drift (mobility.h:189-190):
+  // no displacement and no force there
+  if (kind_code() == 2)
+    launch_zero_open_systems(noise, N, sys_off_.as<int64_t>(), nsys, kinds_dev(), stream_);
   // q + (delta/2) W, then M(q + delta/2 W) W  (random_finite_differences.h:47-49)
   launch_axpy_to(rfd_pos_.as<double>(), shifted_.as<double>(), noise, 0.5 * delta, n3, stream_);
-  launch_system_bbox(rfd_pos_.as<double>(), sys_off_.as<int64_t>(), nsys, 0.0, bbox_.as<double>(), stream_);
+  launch_system_bbox(rfd_pos_.as<double>(), sys_off_.as<int64_t>(), nsys, 0.0, kinds_dev(),
+                     wall_ ? 1 : 0, bbox_.as<double>(), stream_);
   launch_make_records(rfd_pos_.as<double>(), N, sys_off_.as<int64_t>(), nsys, bbox_.as<double>(),
-                      wall_ ? 1 : 0, 0.0, a_, nullptr, rfd_rec64_.as<void>(), rfd_rec32_.as<void>(), stream_);
+                      kinds_dev(), wall_ ? 1 : 0, 0.0, a_, nullptr, rfd_rec64_.as<void>(),
+                      rfd_rec32_.as<void>(), stream_);
   pair_product(rfd_rec64_.as<void>(), rfd_rec32_.as<void>(), noise, nullptr, rfd_mp_.as<double>(),
                angular_ ? rfd_wp_.as<double>() : nullptr);
   // q - (delta/2) W  (:50-52)
   launch_axpy_to(rfd_pos_.as<double>(), shifted_.as<double>(), noise, -0.5 * delta, n3, stream_);
-  launch_system_bbox(rfd_pos_.as<double>(), sys_off_.as<int64_t>(), nsys, 0.0, bbox_.as<double>(), stream_);
+  launch_system_bbox(rfd_pos_.as<double>(), sys_off_.as<int64_t>(), nsys, 0.0, kinds_dev(),
+                     wall_ ? 1 : 0, bbox_.as<double>(), stream_);
   launch_make_records(rfd_pos_.as<double>(), N, sys_off_.as<int64_t>(), nsys, bbox_.as<double>(),
-                      wall_ ? 1 : 0, 0.0, a_, nullptr, rfd_rec64_.as<void>(), rfd_rec32_.as<void>(), stream_);
+                      kinds_dev(), wall_ ? 1 : 0, 0.0, a_, nullptr, rfd_rec64_.as<void>(),
+                      rfd_rec32_.as<void>(), stream_);
   pair_product(rfd_rec64_.as<void>(), rfd_rec32_.as<void>(), noise, nullptr, rfd_mm_.as<double>(),
                angular_ ? rfd_wm_.as<double>() : nullptr);
   // out += (prefactor/delta) (M+ W - M- W)  (:54-62, mobility.h:227-235)
@@ -717,7 +777,7 @@ void NBodySolver::divM_dev(double* U, double* W, double prefactor) {
 
 void NBodySolver::divM(double* U, double* W, double prefactor, bool dev) {
   if (prefactor == 0) return;  // mobility.h:186-188
-  if (!wall_) return;          // open: no thermal drift (mobility.h:189-190)
+  if (!any_wall_) return;      // open: no thermal drift (mobility.h:189-190)
   require_positions(
       "[libMobility] The number of particles is not set. Did you forget to call setPositions?");
   if (!U) throw Error("[libMobility] The number of forces does not match the number of particles");
@@ -781,7 +841,7 @@ void NBodySolver::langevin(double dt, double kbt, const double* F, int64_t nF, c
   if (have_ft) mdot_dev(dF, dT, dU, dW);
   if (kbt != 0) {
     sqrtMdotW_dev(dU, dW, std::sqrt(2 * kbt / dt), nullptr);
-    if (wall_) divM_dev(dU, dW, kbt);
+    if (any_wall_) divM_dev(dU, dW, kbt);
   }
   if (!dev) {
     LM_HIP(hipMemcpyAsync(U, dU, sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));

@@ -41,8 +41,15 @@ class NBodySolver {
                   uint64_t seed, LanczosCallback cb, void* cb_user);
 
   void setPositions(const double* pos, int64_t n3, bool dev);
-  // Ragged systems (extension): offsets[0..nsys], offsets[nsys] = N.
-  void setSystems(const int64_t* offsets, int nsys);
+  // Ragged systems (extension): offsets[0..nsys], offsets[nsys] = N; kinds[s]
+  // (0 open, 1 wall) per system, nullptr = the solver's periodicity.
+  void setSystems(const int64_t* offsets, int nsys, const int* kinds);
+  // Persistent pair launches (a fixed grid draining a work queue).
+  void setPersistent(int mode) { persistent_mode_ = mode; }
+  // Lanczos iterations before "[Lanczos] Could not converge" (the public
+  // iterationHardLimit member of lanczos::Solver, LanczosAlgorithm.h:49).
+  void setLanczosIterationLimit(int limit);
+  int lanczosIterationLimit() const { return iter_limit_; }
   int64_t numParticles() const { return N_; }
   bool includeAngular() const { return angular_; }
 
@@ -116,12 +123,23 @@ class NBodySolver {
   // positions
   int64_t N_ = 0;
   std::vector<int64_t> sys_off_host_;
+  std::vector<int> sys_kind_host_;  // per system of the current layout
+  std::vector<int> explicit_kinds_;
   bool explicit_systems_ = false;
+  bool any_wall_ = false, any_open_ = true, kinds_dev_valid_ = false;
+  DevBuf sys_kind_, work_counter_;
+  int persistent_mode_ = -1;  // -1 auto (mixed ensembles), 0 off, 1 on
+  int kind_code() const { return (any_wall_ && any_open_) ? 2 : (any_wall_ ? 1 : 0); }
+  const int* kinds_dev() const { return kinds_dev_valid_ ? sys_kind_.as<int>() : nullptr; }
   DevBuf sys_off_, bbox_, shifted_, rec64_, rec32_;
   double rec_a_ = 0.0;  // radius the records were scaled with
   // pair work lists
   DevBuf items_;
   int items_tpl_ = -1, items_nch_ = 0, items_count_ = 0;
+  bool items_persist_ = false;
+  bool use_persistent() const {
+    return persistent_mode_ == 1 || (persistent_mode_ == -1 && kind_code() == 2);
+  }
   int64_t items_N_ = -1;
   uint64_t items_layout_ = 0, layout_id_ = 0;
   DevBuf part_u_, part_w_;
@@ -137,6 +155,7 @@ class NBodySolver {
   double* old_buf_ = nullptr;
   int64_t old_n_ = 0;
   int check_steps_ = 3;          // LanczosAlgorithm.cu:171
+  int iter_limit_ = 200;         // LanczosAlgorithm.h:49
   int last_iters_ = 0;
   // learned preconditioner
   void ensure_knn();

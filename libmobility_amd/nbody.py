@@ -175,13 +175,34 @@ class NBody:
             raise RuntimeError("[libmobility_amd] pair precision must be 'fp64' or 'fp32'")
         _ffi.check(self._lib.lm_nbody_set_pair_precision(self._h, _ffi.PAIR_PRECISION[precision]))
 
-    def setSystems(self, sizes):
-        """Extension: independent systems of the given (ragged) sizes."""
+    def setSystems(self, sizes, kinds=None):
+        """Extension: independent systems of the given (ragged) sizes.
+
+        ``kinds``: per system "open"/"single_wall" (or 0/1); default: the
+        solver's periodicity.  Mixed ensembles run in one persistent launch.
+        """
         sizes = np.asarray(sizes, dtype=np.int64)
         off = np.zeros(sizes.size + 1, dtype=np.int64)
         off[1:] = np.cumsum(sizes)
         self._sys_off = off
-        _ffi.check(self._lib.lm_nbody_set_systems(self._h, off.ctypes.data, int(sizes.size)))
+        kp = None
+        if kinds is not None:
+            k = np.array([{"open": 0, "single_wall": 1}.get(x, x) for x in kinds], dtype=np.int32)
+            if k.size != sizes.size:
+                raise RuntimeError("[libmobility_amd] one kind per system is required")
+            self._sys_kinds = k
+            kp = k.ctypes.data
+        _ffi.check(self._lib.lm_nbody_set_systems(self._h, off.ctypes.data, int(sizes.size), kp))
+
+    def setPersistent(self, mode=1):
+        """Extension: persistent pair launches (-1 auto, 0 off, 1 on)."""
+        _ffi.check(self._lib.lm_nbody_set_persistent(self._h, int(mode)))
+
+    def setLanczosIterationLimit(self, limit=200):
+        """Iterations before "[Lanczos] Could not converge" (the public
+        ``iterationHardLimit`` of the reference's lanczos::Solver,
+        LanczosAlgorithm.h:49; default 200, at most 1024)."""
+        _ffi.check(self._lib.lm_nbody_set_lanczos_iteration_limit(self._h, int(limit)))
 
     def setPreconditioner(self, preconditioner="learned"):
         """Extension: attach a learned block preconditioner to sqrtMdotW.

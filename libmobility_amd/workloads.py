@@ -6,6 +6,8 @@ tests to build inputs of the documented shapes:
   * ``wall_slab``         x, y uniform in [0, Lxy), z uniform in [zmin, zmax]
   * ``rsa_cube``          random sequential addition of non-overlapping spheres
                           of radius a in a cube sized for volume fraction phi
+  * ``rsa_box``           the same in a rectangular box
+  * ``ensemble_c4``       512 ragged systems, half open, half above a wall
 """
 import math
 
@@ -30,17 +32,24 @@ def rsa_box_side(N, phi, a=1.0):
 
 
 def rsa_cube(N, phi, seed, a=1.0, batch=None):
-    """Non-overlapping spheres (centre distance >= 2a) in [0, L)^3.
+    """Non-overlapping spheres in a cube [0, L)^3 sized for volume fraction phi;
+    returns (positions, L)."""
+    L = rsa_box_side(N, phi, a)
+    return rsa_box(N, (L, L, L), seed, a=a, batch=batch), L
+
+
+def rsa_box(N, dims, seed, a=1.0, batch=None, origin=(0.0, 0.0, 0.0)):
+    """Non-overlapping spheres (centre distance >= 2a) in origin + [0, dims).
 
     Candidates are drawn in batches; a candidate is kept when it is at least
     2a from every accepted sphere and from every lower-indexed candidate of its
     batch.  Uses a cell grid of side 2a/sqrt(3) (at most one centre per cell).
     """
     rng = np.random.default_rng(seed)
-    L = rsa_box_side(N, phi, a)
+    dims = np.asarray(dims, dtype=np.float64)
     cell = 2.0 * a / math.sqrt(3.0)
-    nc = max(1, int(math.ceil(L / cell)))
-    grid = -np.ones((nc, nc, nc), dtype=np.int64)
+    nc = np.maximum(1, np.ceil(dims / cell).astype(np.int64))
+    grid = -np.ones(tuple(nc), dtype=np.int64)
     pos = np.empty((N, 3))
     count = 0
     offs = np.array([(i, j, k) for i in range(-2, 3) for j in range(-2, 3) for k in range(-2, 3)])
@@ -50,9 +59,9 @@ def rsa_cube(N, phi, seed, a=1.0, batch=None):
     while count < N:
         guard += 1
         if guard > 10000:
-            raise RuntimeError("rsa_cube: volume fraction too high for sequential addition")
+            raise RuntimeError("rsa_box: volume fraction too high for sequential addition")
         B = min(batch, 2 * (N - count) + 64)
-        cand = rng.uniform(0.0, L, (B, 3))
+        cand = rng.uniform(0.0, 1.0, (B, 3)) * dims
         ci = np.minimum((cand / cell).astype(np.int64), nc - 1)
         # against accepted spheres
         nb = ci[:, None, :] + offs[None, :, :]
@@ -71,14 +80,14 @@ def rsa_cube(N, phi, seed, a=1.0, batch=None):
         # Within the batch: drop every candidate that conflicts with a
         # lower-indexed one (two centres in one cell always conflict).
         nbat = cand.shape[0]
-        key = (ci[:, 0] * nc + ci[:, 1]) * nc + ci[:, 2]
-        first = np.full(nc * nc * nc, nbat, dtype=np.int64)
+        key = (ci[:, 0] * nc[1] + ci[:, 1]) * nc[2] + ci[:, 2]
+        first = np.full(int(nc.prod()), nbat, dtype=np.int64)
         np.minimum.at(first, key, np.arange(nbat))
         keep = first[key] == np.arange(nbat)
         nb = ci[:, None, :] + offs[None, :, :]
         valid = np.all((nb >= 0) & (nb < nc), axis=2)
         nbc = np.clip(nb, 0, nc - 1)
-        nkey = (nbc[..., 0] * nc + nbc[..., 1]) * nc + nbc[..., 2]
+        nkey = (nbc[..., 0] * nc[1] + nbc[..., 1]) * nc[2] + nbc[..., 2]
         other = np.where(valid, first[nkey], nbat)
         lower = other < np.arange(nbat)[:, None]
         d = cand[:, None, :] - cand[np.minimum(other, nbat - 1)]
@@ -92,4 +101,25 @@ def rsa_cube(N, phi, seed, a=1.0, batch=None):
         pos[count:count + take] = cand
         grid[ci[:, 0], ci[:, 1], ci[:, 2]] = np.arange(count, count + take)
         count += take
-    return pos, L
+    return pos + np.asarray(origin, dtype=np.float64)
+
+
+def ensemble_c4(nsys=512, nmin=64, nmax=8192, seed=2024, a=1.0):
+    """Ragged ensemble: N_s log-uniform in [nmin, nmax], even systems open
+    (RSA cube), odd systems above a wall at z = 0 (RSA slab, z in [1.5a, 10a]),
+    volume fraction uniform in [0.05, 0.25].  Returns (sizes, kinds, positions)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(math.log(nmin), math.log(nmax), nsys))).astype(np.int64)
+    phis = rng.uniform(0.05, 0.25, nsys)
+    kinds = np.arange(nsys) % 2
+    pos = []
+    for s, (n, phi, k) in enumerate(zip(sizes, phis, kinds)):
+        vol = n * 4.0 * math.pi * a ** 3 / 3.0 / phi
+        if k:
+            h = 8.5 * a
+            lxy = math.sqrt(vol / h)
+            pos.append(rsa_box(int(n), (lxy, lxy, h), seed * 7 + s, a=a, origin=(0.0, 0.0, 1.5 * a)))
+        else:
+            side = vol ** (1.0 / 3.0)
+            pos.append(rsa_box(int(n), (side, side, side), seed * 7 + s, a=a))
+    return sizes, kinds, np.concatenate(pos)

@@ -58,6 +58,10 @@ def _load():
     lib.oracle_mdot.argtypes = [ctypes.c_int64, dp, i64p, ctypes.c_int64, ip, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_int, dp, dp, dp, dp]
     lib.oracle_mdot.restype = None
+    lib.oracle_mdot_rows.argtypes = [ctypes.c_int64, dp, i64p, ctypes.c_int64, ip, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, dp, dp, ctypes.c_int64, i64p,
+                                     dp, dp]
+    lib.oracle_mdot_rows.restype = None
     lib.oracle_mdot_open_uf.argtypes = [ctypes.c_int64, dp, ctypes.c_double, ctypes.c_double,
                                         dp, dp]
     lib.oracle_mdot_open_uf.restype = None
@@ -140,6 +144,27 @@ def mdot(pos, eta, a, F=None, T=None, angular=False, wall=False, sys_sizes=None)
                         _dptr(Tp) if Tp is not None else None,
                         _dptr(U), _dptr(W) if W is not None else None)
     return U, W
+
+
+def mdot_rows(pos, eta, a, rows, F=None, T=None, angular=False, wall=False, sys_sizes=None):
+    """Selected target rows of the matrix-free FP64 product: (U, W or None),
+    each of shape (len(rows), 3)."""
+    pos = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(-1))
+    N = pos.size // 3
+    off, wf = _systems(N, sys_sizes, wall)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    Fp = None if F is None else np.ascontiguousarray(np.asarray(F, np.float64).reshape(-1))
+    Tp = None if T is None else np.ascontiguousarray(np.asarray(T, np.float64).reshape(-1))
+    U = np.zeros(3 * rows.size)
+    W = np.zeros(3 * rows.size) if angular else None
+    _load().oracle_mdot_rows(N, _dptr(pos), off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                             len(off) - 1, wf.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                             float(eta), float(a), int(angular),
+                             _dptr(Fp) if Fp is not None else None,
+                             _dptr(Tp) if Tp is not None else None, rows.size,
+                             rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                             _dptr(U), _dptr(W) if W is not None else None)
+    return U.reshape(-1, 3), (W.reshape(-1, 3) if W is not None else None)
 
 
 def mdot_open_uf(pos, eta, a, F):

@@ -316,20 +316,17 @@ void oracle_dense_matrix(int64_t N, const double *pos, const int64_t *sys_off, i
   }
 }
 
-/*
- * Matrix-free O(N^2) product, FP64, OpenMP over targets.  F or T may be NULL
- * (absent).  Block selection follows NbodyRPY.cu:264-275: UF iff F, WF iff F
- * and angular, UT and WT iff T and angular.  U (3N) is overwritten; W (3N) is
- * overwritten when angular (NbodyRPY.cu:91-96).
- */
-void oracle_mdot(int64_t N, const double *pos, const int64_t *sys_off, int64_t nsys,
-                 const int *wall_flags, double eta, double a, int angular, const double *F,
-                 const double *T, double *U, double *W) {
+/* Rows `rows[0..nrows)` (all rows when rows == NULL, nrows = N) of the
+ * matrix-free product below; U/W receive 3*nrows values. */
+void oracle_mdot_rows(int64_t N, const double *pos, const int64_t *sys_off, int64_t nsys,
+                      const int *wall_flags, double eta, double a, int angular, const double *F,
+                      const double *T, int64_t nrows, const int64_t *rows, double *U, double *W) {
   double t0, rt0, r0;
   mobility_constants(eta, a, &t0, &rt0, &r0);
   int useT = (T != 0) && angular;
 #pragma omp parallel for schedule(dynamic, 16)
-  for (int64_t i = 0; i < N; i++) {
+  for (int64_t q = 0; q < nrows; q++) {
+    const int64_t i = rows ? rows[q] : q;
     int64_t s = sys_of(sys_off, nsys, i);
     double u[3] = {0, 0, 0}, w[3] = {0, 0, 0};
     double blk[36];
@@ -348,10 +345,22 @@ void oracle_mdot(int64_t N, const double *pos, const int64_t *sys_off, int64_t n
         }
     }
     for (int x = 0; x < 3; x++) {
-      U[3 * i + x] = u[x];
-      if (angular && W) W[3 * i + x] = w[x];
+      U[3 * q + x] = u[x];
+      if (angular && W) W[3 * q + x] = w[x];
     }
   }
+}
+
+/*
+ * Matrix-free O(N^2) product, FP64, OpenMP over targets.  F or T may be NULL
+ * (absent).  Block selection follows NbodyRPY.cu:264-275: UF iff F, WF iff F
+ * and angular, UT and WT iff T and angular.  U (3N) is overwritten; W (3N) is
+ * overwritten when angular (NbodyRPY.cu:91-96).
+ */
+void oracle_mdot(int64_t N, const double *pos, const int64_t *sys_off, int64_t nsys,
+                 const int *wall_flags, double eta, double a, int angular, const double *F,
+                 const double *T, double *U, double *W) {
+  oracle_mdot_rows(N, pos, sys_off, nsys, wall_flags, eta, a, angular, F, T, N, 0, U, W);
 }
 
 /* Open-boundary UF-only product specialised to the far/overlap branches

@@ -189,3 +189,48 @@ def test_hydrodynamic_velocities_decomposition():
     q, _ = s2.sqrtMdotW(1.0)
     d, _ = s2.divM(1.0)
     assert np.allclose(total, m + q + d, atol=1e-12, rtol=1e-12)
+
+
+def test_iteration_limit_beyond_default():
+    """lanczos::Solver::iterationHardLimit (LanczosAlgorithm.h:49) raised to
+    300: a tolerance no estimate can meet runs every iteration, with the
+    estimate and error computed at each one (callback set, cu:183), and ends in
+    "[Lanczos] Could not converge" (cu:200) as the restated solver does.  The
+    first errors follow the oracle's; the later ones stay at rounding level."""
+    rng = np.random.default_rng(3)
+    N = 200
+    pos = rng.uniform(0, 16.0, (N, 3))
+    eta = 1 / (6 * np.pi)
+    W = np.random.default_rng(5).normal(size=3 * N)
+    M = oracle.dense_matrix(pos, eta, 1.0)
+    errs_ref = []
+    with pytest.raises(oracle.LanczosError, match="Could not converge"):
+        oracle.LanczosOracle(iteration_hard_limit=300).run(
+            lambda v: M @ v, W, -1.0, callback=lambda i, e: errs_ref.append(e))
+    from libmobility_amd import NBody
+    errs = []
+    s = NBody("open", "open", "open")
+    s.setParameters()
+    s.initialize(viscosity=eta, hydrodynamicRadius=1.0, tolerance=-1.0,
+                 lanczos_callback=lambda i, e: errs.append((i, e)))
+    s.setLanczosIterationLimit(300)
+    s.setPositions(pos)
+    with pytest.raises(RuntimeError, match="Could not converge"):
+        s.sqrtMdotW_with_noise(W)
+    assert [i for i, _ in errs] == list(range(1, 300))
+    e = np.array([x for _, x in errs])
+    assert np.all(np.isfinite(e))
+    assert np.allclose(e[:10], errs_ref[:10], rtol=1e-6)
+    assert e[250:].max() < 1e-12
+    # limits outside [3, 1024] are refused; the default stays 200
+    with pytest.raises(RuntimeError, match="iteration limit"):
+        s.setLanczosIterationLimit(1025)
+    calls = []
+    s2 = NBody("open", "open", "open")
+    s2.setParameters()
+    s2.initialize(viscosity=eta, hydrodynamicRadius=1.0, tolerance=-1.0,
+                  lanczos_callback=lambda i, e: calls.append(i))
+    s2.setPositions(pos)
+    with pytest.raises(RuntimeError, match="Could not converge"):
+        s2.sqrtMdotW_with_noise(W)
+    assert calls == list(range(1, 200))

@@ -135,13 +135,13 @@ def test_preconditioned_fluctuation_dissipation():
 
 def test_ablation_trained_network_vs_plain():
     """What the trained network gains over plain Lanczos (and over the same
-    network untrained, and over the ideal blocks it is trained to predict) on
-    an RSA packing (N = 2048, phi = 0.2, tol 1e-6).  The blocks act on the
+    network untrained) on an RSA packing (N = 2048, phi = 0.2, tol 1e-6).  The blocks act on the
     local (near-field) part of the spectrum; the iteration count of this
     operator is set by its collective long-wavelength modes, so the expected
     gain is ~1x.  Asserted: the trained blocks are closer to their targets
-    than the untrained ones, and every variant still converges to a valid
-    square root without a slowdown beyond 20 %."""
+    than the untrained ones, and the trained preconditioner costs at most 20 %
+    more iterations than plain Lanczos (that the preconditioned result is a
+    square root of M is tested separately)."""
     import os
     from libmobility_amd import _ffi
     from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
@@ -149,27 +149,23 @@ def test_ablation_trained_network_vs_plain():
     if not os.path.exists(DEFAULT_WEIGHTS):
         pytest.skip("trained weights not packaged")
     pos, _ = rsa_cube(2048, 0.2, seed=21)
-    eta = 1.0 / (6 * np.pi)
     rng = np.random.default_rng(0)
     Ws = [rng.normal(size=3 * 2048) for _ in range(3)]
-    M = oracle.dense_matrix(pos, eta, 1.0)
-    exact = [oracle.dense_sqrt(M) @ w for w in Ws]
 
     def run(net):
         s = _open_solver(pos, tol=1e-6)
         if net is not None:
             s.setPreconditioner(net)
-        its, errs = [], []
-        for w, ex in zip(Ws, exact):
-            u, _ = s.sqrtMdotW_with_noise(w)
+        its = []
+        for w in Ws:
+            s.sqrtMdotW_with_noise(w)
             its.append(s.lastLanczosIterations())
-            errs.append(np.linalg.norm(u.ravel() @ u.ravel() - ex @ ex) / (ex @ ex))
         return its, s
 
     plain, _ = run(None)
     trained_net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
     trained, st = run(trained_net)
-    untrained, su = run(LearnedPreconditioner(seed=123))
+    untrained, _ = run(LearnedPreconditioner(seed=123))
     # block error against the FP64 targets on this configuration
     X = np.zeros((512, 48), dtype=np.float32)
     T = np.zeros((512, 9), dtype=np.float32)

@@ -175,3 +175,15 @@ def test_matrix_free_equals_dense_with_batches():
     Uo = oracle.mdot_open_uf(pos, 0.8, 1.1, F)
     U, _ = oracle.mdot(pos, 0.8, 1.1, F=F)
     assert np.allclose(Uo, U, rtol=1e-14, atol=1e-16)
+
+
+def test_row_sampled_product_equals_full():
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(0, 7, (60, 3))
+    pos[:, 2] += 0.4
+    F = rng.normal(size=180)
+    T = rng.normal(size=180)
+    U, W = oracle.mdot(pos, 1.0, 1.0, F=F, T=T, angular=True, wall=True)
+    rows = np.array([3, 17, 59, 0])
+    Ur, Wr = oracle.mdot_rows(pos, 1.0, 1.0, rows, F=F, T=T, angular=True, wall=True)
+    assert np.array_equal(Ur, U.reshape(-1, 3)[rows]) and np.array_equal(Wr, W.reshape(-1, 3)[rows])
