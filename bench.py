@@ -1,22 +1,37 @@
 """Benchmark: NBody Mdot throughput and Lanczos sqrtMdotW time on one MI355X.
 
-Workload (one GPU): open-boundary RPY, N = 131072 particles of radius a = 1,
-random sequential addition at volume fraction 0.20 in a cube of side 140.0a,
-viscosity 1/(6 pi), forces and Brownian noise W i.i.d. N(0,1).
+Default workload (one GPU): open-boundary RPY, N = 131072 particles of radius
+a = 1, random sequential addition at volume fraction 0.20 in a cube of side
+140.0a, viscosity 1/(6 pi), forces and Brownian noise W i.i.d. N(0,1).
 
 A "step" is one Mdot over the whole system (N^2 ordered pairs, self pair
 included) in FP32-pair / FP64-accumulate mode, inputs and outputs resident in
 HBM.  K steps are timed with HIP events on the solver's stream.  The Lanczos
 square root (tol 1e-6) is timed separately (median of a few calls) and
 reported with its iteration count, for the plain FP32-pair and FP64-pair
-operators.  The CPU baseline is the oracle's OpenMP FP64 matrix-free product
-on a bounded sample of target rows, run on this host's cores.
+operators and the learned-preconditioned FP32-pair operator.  The CPU baseline
+is the oracle's OpenMP FP64 matrix-free product on a bounded sample of target
+rows, run on this host's cores.
+
+Other workloads (``--workload``), each printing one JSON line of its own:
+  c1  open RPY, N = 1024 uniform in a cube of side 24a (overlaps), FP64 Mdot
+      and sqrtMdotW (tol 1e-6) against the dense FP64 matrix and its
+      eigen-square-root; CPU baseline = dense build + matvec + eigh square root
+  c2  single wall, N = 65536, x,y in [0,256a), z in [1.5a,20a], forces and
+      torques: FP64 vs FP32-pair Mdot, error per block (UF, UT, WF, WT)
+  c4  512 ragged systems (N log-uniform in [64, 8192]), half open, half
+      single wall: one persistent launch vs per-system solves
+  c5  single wall, N = 1048576, x,y in [0,1024a), z in [1.5a,30a], forces and
+      torques: FP32/FP64 Mdot and hydrodynamicVelocities (Mdot + sqrtMdotW at
+      tol 1e-6 + RFD drift with delta 1e-4 a), FP32 pairs, learned
+      preconditioner
 
 Prints exactly one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import ctypes
 import json
+import math
 import os
 import statistics
 import sys
@@ -28,34 +43,695 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mdot Gpair/s + sqrtMdotW ms at N=131072 tol 1e-6: learned-prec vs plain"
-PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector/matrix peak
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
 PEAK_FP64_TFLOPS = 78.6
+ETA = 1.0 / (6.0 * np.pi)
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pos, F, eta, target_seconds=12.0):
+def flops_per_pair():
+    with open(os.path.join(ROOT, "libmobility_amd", "flop_model.json")) as f:
+        return json.load(f)["flops_per_pair"]
+
+
+class Gpu:
+    """Device buffers, the torch stream and C-ABI solver handles."""
+
+    def __init__(self):
+        import torch
+        from libmobility_amd import _ffi
+        self.torch = torch
+        self.ffi = _ffi
+        self.lib = _ffi.load()
+        self.dev = torch.device("cuda:0")
+        self.stream = torch.cuda.current_stream(self.dev)
+
+    def put(self, a):
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+        return self.torch.from_numpy(a.copy()).to(self.dev)
+
+    def zeros(self, n):
+        return self.torch.zeros(n, dtype=self.torch.float64, device=self.dev)
+
+    def sync(self):
+        self.torch.cuda.synchronize()
+
+    def solver(self, pos_d, prec, wall=False, angular=False, tol=1e-6, delta=1e-3, autotune=True,
+               tiling=None, sizes=None, kinds=None, persistent=-1, iteration_limit=None,
+               seed=12345, callback=None):
+        """callback(i, err): the Lanczos callback of lanczos::Solver::run; with it
+        set, the estimate and its error are computed at every iteration."""
+        ffi, lib = self.ffi, self.lib
+        cb = ffi.LANCZOS_CALLBACK()
+        if callback is not None:
+            cb = ffi.LANCZOS_CALLBACK(lambda i, e, _u: callback(int(i), float(e)))
+            self._callbacks = getattr(self, "_callbacks", []) + [cb]  # keep alive
+        h = ctypes.c_void_p()
+        ffi.check(lib.lm_nbody_create(2, 2, 0 if wall else 2, ctypes.byref(h)))
+        ffi.check(lib.lm_nbody_set_parameters(h, 3, -1, -1, int(wall), 0.0, delta))
+        ffi.check(lib.lm_nbody_set_pair_precision(h, ffi.PAIR_PRECISION[prec]))
+        ffi.check(lib.lm_nbody_set_stream(h, ctypes.c_void_p(self.stream.cuda_stream)))
+        ffi.check(lib.lm_nbody_initialize(h, ETA, 1.0, int(angular), tol, seed, cb, None))
+        if sizes is not None:
+            off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+            k = np.ascontiguousarray(kinds, dtype=np.int32)
+            ffi.check(lib.lm_nbody_set_systems(h, off.ctypes.data, len(sizes), k.ctypes.data))
+            ffi.check(lib.lm_nbody_set_persistent(h, persistent))
+        if iteration_limit is not None:
+            ffi.check(lib.lm_nbody_set_lanczos_iteration_limit(h, iteration_limit))
+        ffi.check(lib.lm_nbody_set_positions(h, pos_d.data_ptr(), pos_d.numel(), ffi.LM_MEM_DEVICE))
+        if tiling is not None:
+            ffi.check(lib.lm_nbody_set_pair_tiling(h, tiling[0], tiling[1]))
+        else:
+            ffi.check(lib.lm_nbody_set_autotune(h, int(autotune)))
+        return h
+
+    def tiling(self, h):
+        t, c = ctypes.c_int(), ctypes.c_int()
+        self.ffi.check(self.lib.lm_nbody_get_pair_tiling(h, ctypes.byref(t), ctypes.byref(c)))
+        return {"targets_per_lane": t.value, "source_chunks": c.value}
+
+    def mdot(self, h, F=None, T=None, U=None, W=None):
+        p = lambda x: None if x is None else x.data_ptr()
+        self.ffi.check(self.lib.lm_nbody_mdot(h, p(F), 0 if F is None else F.numel(), p(T),
+                                              0 if T is None else T.numel(), p(U), p(W),
+                                              self.ffi.LM_MEM_DEVICE))
+
+    def time_events(self, fn, steps, warmup):
+        """ms per call of fn, HIP events on the solver stream."""
+        for _ in range(warmup):
+            fn()
+        self.sync()
+        e0 = self.torch.cuda.Event(enable_timing=True)
+        e1 = self.torch.cuda.Event(enable_timing=True)
+        e0.record(self.stream)
+        for _ in range(steps):
+            fn()
+        e1.record(self.stream)
+        self.sync()
+        return e0.elapsed_time(e1) / max(steps, 1)
+
+    def sqrt_noise(self, h, Wn, U, W=None, prefactor=1.0):
+        self.ffi.check(self.lib.lm_nbody_sqrt_mdot_w_noise(
+            h, Wn.data_ptr(), U.data_ptr(), None if W is None else W.data_ptr(), prefactor,
+            self.ffi.LM_MEM_DEVICE))
+
+    def time_lanczos(self, h, Wn, n_out, calls, angular=False):
+        U = self.zeros(n_out)
+        W = self.zeros(n_out) if angular else None
+        times, iters = [], []
+        for c in range(calls + 1):
+            self.sync()
+            t = time.perf_counter()
+            self.sqrt_noise(h, Wn, U, W)
+            self.sync()
+            dt = (time.perf_counter() - t) * 1e3
+            it = self.lib.lm_nbody_last_lanczos_iterations(h)
+            if c > 0 or calls == 0:
+                times.append(dt)
+                iters.append(it)
+            log(f"  sqrtMdotW call {c}: {dt:.1f} ms, iterations {it}")
+        return statistics.median(times), iters
+
+    def destroy(self, *hs):
+        for h in hs:
+            self.lib.lm_nbody_destroy(h)
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def result_line(metric, value, unit, steps, warmup, ms, higher, dtype, config, roofline, extra,
+                cpu):
+    return {
+        "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config,
+        "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
+    }
+
+
+def compute_roofline(flops_per_launch, ms, peak_tflops):
+    achieved = flops_per_launch / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": peak_tflops, "unit": "TFLOP/s",
+            "frac": achieved / peak_tflops, "traffic": None}
+
+
+# ---------------------------------------------------------------------------
+# C3 (default): open RPY, N = 131072, RSA phi = 0.2
+
+
+def cpu_baseline_rows(pos, F, target_seconds=12.0, first=256):
+    """CPU baseline of C3: the oracle's OpenMP FP64 open-RPY UF product over a
+    bounded sample of target rows (rows 0..R-1, R sized for ~12 s).  Returns
+    the baseline record and the sampled rows' velocities (the checker)."""
     import oracle
     N = pos.shape[0]
-    rows = 256
     t = time.perf_counter()
-    oracle.mdot_open_uf_rows(pos, eta, 1.0, F, 0, rows)
+    oracle.mdot_open_uf_rows(pos, ETA, 1.0, F, 0, first)
     dt = time.perf_counter() - t
-    rows = int(min(N, max(rows, rows * target_seconds / max(dt, 1e-6))))
+    rows = int(min(N, max(first, first * target_seconds / max(dt, 1e-6))))
     t = time.perf_counter()
-    oracle.mdot_open_uf_rows(pos, eta, 1.0, F, 0, rows)
+    U = oracle.mdot_open_uf_rows(pos, ETA, 1.0, F, 0, rows)
     dt = time.perf_counter() - t
     pairs = rows * N
-    return {
-        "value": pairs / dt / 1e9,
-        "unit": "Gpair/s",
-        "cores": oracle.num_threads(),
+    cpu = {
+        "value": pairs / dt / 1e9, "unit": "Gpair/s", "cores": oracle.num_threads(),
         "kind": "port",
         "sample": f"FP64 open-RPY UF product, target rows 0..{rows - 1} x all {N} sources "
                   f"({pairs:.3e} pairs, {dt:.1f} s), OpenMP C oracle",
     }
+    return cpu, U
+
+
+def bbt_errors(g, net, N=512, phi=0.20):
+    """||B B^T - M||_F / ||M||_F of the FP32-pair Lanczos square root, plain and
+    with the learned preconditioner, at tol 1e-4 and 1e-6, on an RSA packing of
+    N particles.  Column k of B is sqrtMdotW applied to e_k; M is the solver's
+    own FP64-pair matrix (Mdot of e_k), which the tests hold to 1e-12 of the
+    oracle's dense matrix."""
+    from libmobility_amd.workloads import rsa_cube
+    pos, _ = rsa_cube(N, phi, seed=5)
+    n = 3 * N
+    pos_d = g.put(pos)
+    E = g.torch.eye(n, dtype=g.torch.float64, device=g.dev)
+    h64 = g.solver(pos_d, "fp64", autotune=False)
+    Mcols = g.zeros(n * n).view(n, n)
+    for k in range(n):
+        g.mdot(h64, E[k], None, Mcols[k])
+    g.sync()
+    g.destroy(h64)
+    M = Mcols.cpu().numpy().T
+    out = {"N": N, "phi": phi}
+    for label, use_net in (("plain", False), ("learned", True)):
+        for tol in (1e-4, 1e-6):
+            h = g.solver(pos_d, "fp32", tol=tol, autotune=False)
+            if use_net:
+                g.ffi.check(g.lib.lm_nbody_set_preconditioner(h, net._h))
+            Bcols = g.zeros(n * n).view(n, n)
+            its = []
+            for k in range(n):
+                g.sqrt_noise(h, E[k], Bcols[k])
+                its.append(g.lib.lm_nbody_last_lanczos_iterations(h))
+            g.sync()
+            g.destroy(h)
+            B = Bcols.cpu().numpy().T
+            err = float(np.linalg.norm(B @ B.T - M) / np.linalg.norm(M))
+            out[f"{label}_tol{tol:g}"] = {"rel_err": err, "iterations_mean": float(np.mean(its))}
+            log(f"  ||BB^T - M||/||M|| {label} tol {tol:g}: {err:.2e} "
+                f"({np.mean(its):.1f} iterations)")
+    return out
+
+
+def bench_c3(args):
+    from libmobility_amd.workloads import rsa_cube
+    g = Gpu()
+    N = args.N or 131072
+    phi = 0.20
+    log(f"generating RSA packing N={N} phi={phi}")
+    pos, L = rsa_cube(N, phi, seed=2026)
+    rng = np.random.default_rng(7)
+    F = rng.normal(size=(N, 3))
+    Wn = rng.normal(size=3 * N)
+    pos_d, F_d, W_d, U_d = g.put(pos), g.put(F), g.put(Wn), g.zeros(3 * N)
+    fpp = flops_per_pair()["open_UF"]
+    pairs = float(N) * float(N)
+
+    log("FP32-pair Mdot")
+    h32 = g.solver(pos_d, "fp32", tiling=args.tiling)
+    ms32 = g.time_events(lambda: g.mdot(h32, F_d, None, U_d), args.steps, args.warmup)
+    U32 = U_d.cpu().numpy()
+    gpair32 = pairs / (ms32 * 1e-3) / 1e9
+    log(f"  {ms32:.3f} ms/Mdot, {gpair32:.1f} Gpair/s")
+    log("FP64-pair Mdot")
+    h64 = g.solver(pos_d, "fp64", tiling=args.tiling)
+    ms64 = g.time_events(lambda: g.mdot(h64, F_d, None, U_d), max(3, args.steps // 4), 1)
+    U64 = U_d.cpu().numpy()
+    gpair64 = pairs / (ms64 * 1e-3) / 1e9
+    log(f"  {ms64:.3f} ms/Mdot, {gpair64:.1f} Gpair/s")
+
+    extra = {
+        "tiling_fp32pair": g.tiling(h32),
+        "tiling_fp64pair": g.tiling(h64),
+        "mdot_fp32pair_ms": ms32,
+        "mdot_fp64pair_ms": ms64,
+        "mdot_fp64pair_gpair_s": gpair64,
+        "mdot_fp64pair_valu_frac_of_fp64_peak": gpair64 * 1e9 * fpp / (PEAK_FP64_TFLOPS * 1e12),
+        "flops_per_pair": fpp,
+        "mdot_rel_err_fp32pair_vs_fp64pair": rel(U32, U64),
+    }
+    if not args.no_lanczos:
+        log("plain Lanczos sqrtMdotW, FP32 pairs, tol 1e-6")
+        ms_l32, it32 = g.time_lanczos(h32, W_d, 3 * N, args.lanczos_calls)
+        extra["sqrtMdotW_plain_fp32pair_ms"] = ms_l32
+        extra["sqrtMdotW_plain_fp32pair_iterations"] = it32
+        log("plain Lanczos sqrtMdotW, FP64 pairs, tol 1e-6")
+        ms_l64, it64 = g.time_lanczos(h64, W_d, 3 * N, max(1, args.lanczos_calls - 2))
+        extra["sqrtMdotW_plain_fp64pair_ms"] = ms_l64
+        extra["sqrtMdotW_plain_fp64pair_iterations"] = it64
+        y32, y64 = g.zeros(3 * N), g.zeros(3 * N)
+        g.sqrt_noise(h32, W_d, y32)
+        g.sqrt_noise(h64, W_d, y64)
+        g.sync()
+        extra["sqrtMdotW_rel_diff_fp32pair_vs_fp64pair"] = rel(y32.cpu().numpy(), y64.cpu().numpy())
+        log("learned-preconditioned Lanczos sqrtMdotW, FP32 pairs, tol 1e-6")
+        from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
+        net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+        hl = g.solver(pos_d, "fp32", tiling=args.tiling)
+        g.ffi.check(g.lib.lm_nbody_set_preconditioner(hl, net._h))
+        ms_ll, itl = g.time_lanczos(hl, W_d, 3 * N, args.lanczos_calls)
+        g.destroy(hl)
+        extra["sqrtMdotW_learned_fp32pair_ms"] = ms_ll
+        extra["sqrtMdotW_learned_fp32pair_iterations"] = itl
+        extra["sqrtMdotW_learned_speedup_vs_plain"] = ms_l32 / ms_ll
+        if not args.no_bbt:
+            log("square-root check at small N (columns of B from unit vectors)")
+            extra["bbt_rel_err_small_N"] = bbt_errors(g, net)
+    g.destroy(h32, h64)
+
+    config = {
+        "workload": f"open-boundary RPY Mdot (UF), N={N}, RSA packing phi={phi} in a cube of "
+                    f"side {L:.1f}a, a=1, eta=1/(6 pi), F~N(0,1); FP32 pair terms with FP64 "
+                    f"per-tile accumulation; sqrtMdotW tol 1e-6",
+        "N": N,
+        "pairs_per_step": pairs,
+    }
+    res = result_line(METRIC, gpair32, "Gpair/s", args.steps, args.warmup, ms32, True, "f32",
+                      config, compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS), extra, None)
+    if not args.no_cpu:
+        log("CPU baseline (oracle, OpenMP FP64)")
+        cpu, Uo = cpu_baseline_rows(pos, F)
+        res["cpu_baseline"] = cpu
+        k = Uo.size
+        extra["mdot_rel_err_fp32pair_vs_oracle_rows"] = rel(U32[:k], Uo)
+        extra["mdot_rel_err_fp64pair_vs_oracle_rows"] = rel(U64[:k], Uo)
+        extra["oracle_rows"] = k // 3
+    return res
+
+
+# ---------------------------------------------------------------------------
+# C1: open RPY, N = 1024, uniform in a cube of side 24a, FP64
+
+
+def cpu_leg_c1(pos, F, Wn):
+    """CPU baseline of C1, the reference's CPU path restated: dense FP64 matrix
+    (OpenMP C oracle), matvec and eigen-square-root applied to W.  Its outputs
+    are also the checker for the GPU results."""
+    import oracle
+    log("CPU baseline: dense FP64 matrix, matvec, eigen-square-root")
+    t = time.perf_counter()
+    M = oracle.dense_matrix(pos, ETA, 1.0)
+    t_build = time.perf_counter() - t
+    t = time.perf_counter()
+    Uref = M @ F.ravel()
+    t_mv = time.perf_counter() - t
+    t = time.perf_counter()
+    exact = oracle.dense_sqrt(M) @ Wn
+    t_sqrt = time.perf_counter() - t
+    ref, it_ref = oracle.LanczosOracle().run(lambda v: M @ v, Wn, 1e-6)
+    try:
+        from threadpoolctl import threadpool_info
+        blas = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
+    except Exception:
+        blas = None
+    cpu = {
+        "value": (t_build + t_mv + t_sqrt) * 1e3, "unit": "ms", "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": f"whole workload: dense 3N x 3N FP64 matrix by the OpenMP C oracle "
+                  f"({t_build * 1e3:.1f} ms), matvec ({t_mv * 1e3:.2f} ms), eigh square root "
+                  f"applied to W ({t_sqrt * 1e3:.1f} ms; numpy/LAPACK with {blas} BLAS threads)",
+    }
+    return cpu, Uref, exact, ref, it_ref
+
+
+def bench_c1(args):
+    g = Gpu()
+    N = args.N or 1024
+    rng = np.random.default_rng(0)
+    pos = rng.uniform(0, 24.0, (N, 3))
+    F = rng.normal(size=(N, 3))
+    Wn = rng.normal(size=3 * N)
+    pos_d, F_d, W_d, U_d = g.put(pos), g.put(F), g.put(Wn), g.zeros(3 * N)
+    h = g.solver(pos_d, "fp64", tiling=args.tiling)
+    ms_mdot = g.time_events(lambda: g.mdot(h, F_d, None, U_d), args.steps, args.warmup)
+    U = U_d.cpu().numpy()
+    ms_sqrt, its = g.time_lanczos(h, W_d, 3 * N, args.lanczos_calls)
+    Uq = g.zeros(3 * N)
+    g.sqrt_noise(h, W_d, Uq)
+    g.sync()
+    sq = Uq.cpu().numpy()
+    tiling = g.tiling(h)
+    g.destroy(h)
+    pairs = float(N) ** 2
+    fpp = flops_per_pair()["open_UF"]
+    extra = {
+        "mdot_ms": ms_mdot,
+        "sqrtMdotW_ms": ms_sqrt,
+        "sqrtMdotW_iterations": its,
+        "tiling": tiling,
+    }
+    cpu = None
+    if not args.no_cpu:
+        cpu, Uref, exact, ref, it_ref = cpu_leg_c1(pos, F, Wn)
+        extra.update({
+            "oracle_lanczos_iterations": it_ref,
+            "mdot_rel_err_vs_dense_fp64": rel(U, Uref),
+            "sqrtMdotW_rel_err_vs_eigh": rel(sq, exact),
+            "oracle_lanczos_rel_err_vs_eigh": rel(ref, exact),
+            "sqrtMdotW_rel_diff_vs_oracle_lanczos": rel(sq, ref),
+        })
+    config = {"workload": f"open-boundary RPY, N={N} uniform in a cube of side 24a (overlaps), "
+                          f"a=1, eta=1/(6 pi), F,W~N(0,1); FP64 Mdot + sqrtMdotW tol 1e-6",
+              "N": N}
+    total = ms_mdot + ms_sqrt
+    return result_line("Mdot + sqrtMdotW ms at N=1024 open FP64 tol 1e-6", total, "ms",
+                       args.steps, args.warmup, total, False, "f64", config,
+                       compute_roofline(pairs * fpp, ms_mdot, PEAK_FP64_TFLOPS), extra, cpu)
+
+
+# ---------------------------------------------------------------------------
+# C2: single wall with torques, N = 65536
+
+
+def wall_positions(N, Lxy, zlo, zhi, seed):
+    rng = np.random.default_rng(seed)
+    pos = np.empty((N, 3))
+    pos[:, :2] = rng.uniform(0.0, Lxy, (N, 2))
+    pos[:, 2] = rng.uniform(zlo, zhi, N)
+    return pos
+
+
+def bench_c2(args):
+    g = Gpu()
+    N = args.N or 65536
+    pos = wall_positions(N, 256.0, 1.5, 20.0, seed=65536)
+    rng = np.random.default_rng(8)
+    F = rng.normal(size=(N, 3))
+    T = rng.normal(size=(N, 3))
+    pos_d, F_d, T_d = g.put(pos), g.put(F), g.put(T)
+    Wn_d = g.put(rng.normal(size=6 * N))
+    U_d, W_d = g.zeros(3 * N), g.zeros(3 * N)
+    fpp = flops_per_pair()["wall_6x6"]
+    pairs = float(N) ** 2
+    out = {}
+    hs = {}
+    for prec in ("fp32", "fp64"):
+        log(f"{prec}-pair wall Mdot (F, T)")
+        h = g.solver(pos_d, prec, wall=True, angular=True, tiling=args.tiling)
+        hs[prec] = h
+        steps = args.steps if prec == "fp32" else max(3, args.steps // 4)
+        ms = g.time_events(lambda: g.mdot(h, F_d, T_d, U_d, W_d), steps, args.warmup)
+        log(f"  {ms:.3f} ms/Mdot, {pairs / ms / 1e6:.1f} Gpair/s")
+        blocks = {}
+        for name, (f, t) in {"F": (F_d, None), "T": (None, T_d), "FT": (F_d, T_d)}.items():
+            g.mdot(h, f, t, U_d, W_d)
+            g.sync()
+            blocks[name] = (U_d.cpu().numpy().reshape(-1, 3), W_d.cpu().numpy().reshape(-1, 3))
+        rec = {"ms": ms, "blocks": blocks, "tiling": g.tiling(h)}
+        # thermal drift (RFD, delta 1e-3 a): both handles share the seed, so
+        # they draw the same random displacement
+        du, dw = g.zeros(3 * N), g.zeros(3 * N)
+        g.ffi.check(g.lib.lm_nbody_div_m(h, du.data_ptr(), dw.data_ptr(), 1.0, g.ffi.LM_MEM_DEVICE))
+        g.sync()
+        rec["drift"] = np.concatenate([du.cpu().numpy(), dw.cpu().numpy()])
+        if not args.no_lanczos:
+            log(f"  {prec}-pair sqrtMdotW (6N, tol 1e-6)")
+            ms_l, it_l = g.time_lanczos(h, Wn_d, 3 * N, 0, angular=True)
+            su, sw = g.zeros(3 * N), g.zeros(3 * N)
+            g.sqrt_noise(h, Wn_d, su, sw)
+            g.sync()
+            rec.update(sqrt_ms=ms_l, sqrt_it=it_l,
+                       sqrt=np.concatenate([su.cpu().numpy(), sw.cpu().numpy()]))
+        out[prec] = rec
+    g.destroy(*hs.values())
+
+    # blocks: U from F = UF F, W from F = WF F, U from T = UT T, W from T = WT T
+    def split(b):
+        return {"UF": b["F"][0], "WF": b["F"][1], "UT": b["T"][0], "WT": b["T"][1],
+                "U": b["FT"][0], "W": b["FT"][1]}
+
+    b32, b64 = split(out["fp32"]["blocks"]), split(out["fp64"]["blocks"])
+    err32_vs_64 = {k: rel(b32[k], b64[k]) for k in b32}
+    ms32, ms64 = out["fp32"]["ms"], out["fp64"]["ms"]
+    extra = {
+        "mdot_fp32pair_ms": ms32,
+        "mdot_fp64pair_ms": ms64,
+        "mdot_fp64pair_gpair_s": pairs / (ms64 * 1e-3) / 1e9,
+        "mdot_fp64pair_valu_frac_of_fp64_peak": pairs * fpp / (ms64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+        "tiling_fp32pair": out["fp32"]["tiling"],
+        "tiling_fp64pair": out["fp64"]["tiling"],
+        "flops_per_pair": fpp,
+        "rel_err_fp32pair_vs_fp64pair_all_rows": err32_vs_64,
+        "divM_rel_diff_fp32pair_vs_fp64pair": rel(out["fp32"]["drift"], out["fp64"]["drift"]),
+    }
+    if not args.no_lanczos:
+        extra.update({
+            "sqrtMdotW_fp32pair_ms": out["fp32"]["sqrt_ms"],
+            "sqrtMdotW_fp64pair_ms": out["fp64"]["sqrt_ms"],
+            "sqrtMdotW_fp32pair_iterations": out["fp32"]["sqrt_it"],
+            "sqrtMdotW_fp64pair_iterations": out["fp64"]["sqrt_it"],
+            "sqrtMdotW_rel_diff_fp32pair_vs_fp64pair": rel(out["fp32"]["sqrt"], out["fp64"]["sqrt"]),
+        })
+    config = {"workload": f"single-wall RPB/Swan-Brady, N={N}, x,y uniform in [0,256a), z uniform "
+                          f"in [1.5a,20a], a=1, eta=1/(6 pi), F,T~N(0,1); 6x6 Mdot (UF,UT,WF,WT), "
+                          f"FP32 pair terms with FP64 per-tile accumulation", "N": N}
+    res = result_line("Wall 6x6 Mdot Gpair/s at N=65536 (FP32 pairs)", pairs / (ms32 * 1e-3) / 1e9,
+                      "Gpair/s", args.steps, args.warmup, ms32, True, "f32", config,
+                      compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS), extra, None)
+    if not args.no_cpu:
+        rows = np.sort(np.random.default_rng(3).choice(N, size=min(N, args.oracle_rows),
+                                                      replace=False))
+        cpu, o = cpu_leg_c2(pos, F, T, rows)
+        res["cpu_baseline"] = cpu
+        extra["rel_err_fp64pair_vs_oracle_rows"] = {k: rel(b64[k][rows], o[k]) for k in o}
+        extra["rel_err_fp32pair_vs_oracle_rows"] = {k: rel(b32[k][rows], o[k]) for k in o}
+        extra["oracle_rows"] = int(rows.size)
+    return res
+
+
+def cpu_leg_c2(pos, F, T, rows):
+    """CPU baseline of C2: the OpenMP FP64 oracle's wall 6x6 product on a
+    sample of target rows, once for F alone and once for T alone (the blocks
+    UF, WF and UT, WT).  Its outputs are also the checker for those rows."""
+    import oracle
+    N = pos.shape[0]
+    log(f"CPU baseline: oracle rows, {rows.size} target rows x {N} sources, per block")
+    t = time.perf_counter()
+    uf, wf = oracle.mdot_rows(pos, ETA, 1.0, rows, F=F, angular=True, wall=True)
+    ut, wt = oracle.mdot_rows(pos, ETA, 1.0, rows, T=T, angular=True, wall=True)
+    dt = time.perf_counter() - t
+    pairs = 2.0 * rows.size * N
+    cpu = {
+        "value": pairs / dt / 1e9, "unit": "Gpair/s", "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": f"FP64 single-wall 6x6 product (F alone, then T alone), {rows.size} random "
+                  f"target rows x all {N} sources ({pairs:.3e} pair evaluations, {dt:.1f} s), "
+                  f"OpenMP C oracle",
+    }
+    return cpu, dict(UF=uf, WF=wf, UT=ut, WT=wt, U=uf + ut, W=wf + wt)
+
+
+# ---------------------------------------------------------------------------
+# C4: 512 ragged systems, half open, half single wall
+
+
+def bench_c4(args):
+    from libmobility_amd.workloads import ensemble_c4
+    g = Gpu()
+    log("generating the 512-system ensemble")
+    sizes, kinds, pos = ensemble_c4()
+    N = int(sizes.sum())
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    rng = np.random.default_rng(9)
+    F = rng.normal(size=(N, 3))
+    Wn = rng.normal(size=3 * N)
+    pos_d, F_d, W_d = g.put(pos), g.put(F), g.put(Wn)
+    U_d = g.zeros(3 * N)
+    fp = flops_per_pair()
+    sq = sizes.astype(np.float64) ** 2
+    pairs = float(sq.sum())
+    flops = float(sq[kinds == 0].sum() * fp["open_UF"] + sq[kinds == 1].sum() * fp["wall_UF"])
+
+    log(f"ensemble: {len(sizes)} systems, N={N}, sum N^2={pairs:.3e}; one persistent launch")
+    he = g.solver(pos_d, "fp32", wall=True, tol=1e-6, sizes=sizes, kinds=kinds, persistent=1,
+                  tiling=args.tiling)
+    ms_e = g.time_events(lambda: g.mdot(he, F_d, None, U_d), args.steps, args.warmup)
+    Ue = U_d.cpu().numpy()
+    log(f"  {ms_e:.3f} ms per ensemble Mdot, {pairs / ms_e / 1e6:.1f} Gpair/s")
+    hn = g.solver(pos_d, "fp32", wall=True, tol=1e-6, sizes=sizes, kinds=kinds, persistent=0,
+                  tiling=args.tiling)
+    ms_n = g.time_events(lambda: g.mdot(hn, F_d, None, U_d), args.steps, args.warmup)
+    Un = U_d.cpu().numpy()
+    g.destroy(hn)
+    log(f"  {ms_n:.3f} ms with the non-persistent grid")
+
+    log("per-system solves (one handle and one launch per system)")
+    hs = [g.solver(pos_d[3 * a:3 * b], "fp32", wall=bool(k), tol=1e-6)
+          for a, b, k in zip(off[:-1], off[1:], kinds)]
+    Us = [U_d[3 * a:3 * b] for a, b in zip(off[:-1], off[1:])]
+    Fs = [F_d[3 * a:3 * b] for a, b in zip(off[:-1], off[1:])]
+
+    def per_system():
+        for h, f, u in zip(hs, Fs, Us):
+            g.mdot(h, f, None, u)
+
+    ms_p = g.time_events(per_system, max(1, args.steps // 4), 1)
+    Up = U_d.cpu().numpy()
+    log(f"  {ms_p:.3f} ms for 512 per-system Mdots")
+    extra = {
+        "systems": int(len(sizes)), "N_total": N, "sum_N2": pairs,
+        "mdot_ensemble_persistent_ms": ms_e,
+        "mdot_ensemble_plain_grid_ms": ms_n,
+        "mdot_per_system_ms": ms_p,
+        "mdot_speedup_vs_per_system": ms_p / ms_e,
+        "mdot_rel_diff_ensemble_vs_per_system": rel(Ue, Up),
+        "mdot_persistent_equals_plain_grid_bitwise": bool(np.array_equal(Ue, Un)),
+        "tiling_ensemble": g.tiling(he),
+        "flops_per_launch": flops,
+    }
+    if not args.no_lanczos:
+        log("sqrtMdotW (tol 1e-6): one Lanczos over the ensemble")
+        ms_le, it_e = g.time_lanczos(he, W_d, 3 * N, args.lanczos_calls)
+        Ule = g.zeros(3 * N)
+        g.sqrt_noise(he, W_d, Ule)
+        log("sqrtMdotW per system")
+        Wns = [W_d[3 * a:3 * b] for a, b in zip(off[:-1], off[1:])]
+        Ulp = g.zeros(3 * N)
+        its = []
+        for h, w, a, b in zip(hs, Wns, off[:-1], off[1:]):  # warm-up: workspaces, tiling
+            g.sqrt_noise(h, w, Ulp[3 * a:3 * b])
+        g.sync()
+        Ulp.zero_()
+        t = time.perf_counter()
+        for h, w, a, b in zip(hs, Wns, off[:-1], off[1:]):
+            g.sqrt_noise(h, w, Ulp[3 * a:3 * b])
+            its.append(g.lib.lm_nbody_last_lanczos_iterations(h))
+        g.sync()
+        ms_lp = (time.perf_counter() - t) * 1e3
+        log("sqrtMdotW: one recurrence per system, all systems in one launch per iteration")
+        g.ffi.check(g.lib.lm_nbody_set_lanczos_per_system(he, 1))
+        ms_lr, it_r = g.time_lanczos(he, W_d, 3 * N, args.lanczos_calls)
+        Ulr = g.zeros(3 * N)
+        g.sqrt_noise(he, W_d, Ulr)
+        its_r = np.zeros(len(sizes), dtype=np.int32)
+        g.ffi.check(g.lib.lm_nbody_last_lanczos_iterations_per_system(he, its_r.ctypes.data,
+                                                                      len(sizes)))
+        up = Ulp.cpu().numpy()
+        extra.update({
+            "sqrtMdotW_ensemble_ms": ms_le,
+            "sqrtMdotW_ensemble_iterations": it_e,
+            "sqrtMdotW_per_system_ms": ms_lp,
+            "sqrtMdotW_per_system_iterations_mean": float(np.mean(its)),
+            "sqrtMdotW_per_system_iterations_max": int(np.max(its)),
+            "sqrtMdotW_speedup_vs_per_system": ms_lp / ms_le,
+            "sqrtMdotW_rel_diff_ensemble_vs_per_system": rel(Ule.cpu().numpy(), up),
+            "sqrtMdotW_per_system_recurrences_ms": ms_lr,
+            "sqrtMdotW_per_system_recurrences_iterations_max": it_r,
+            "sqrtMdotW_per_system_recurrences_iterations_mean": float(np.mean(its_r)),
+            "sqrtMdotW_per_system_recurrences_speedup_vs_per_system": ms_lp / ms_lr,
+            "sqrtMdotW_per_system_recurrences_rel_diff_vs_per_system": rel(Ulr.cpu().numpy(), up),
+        })
+    g.destroy(he, *hs)
+    config = {"workload": "512 independent suspensions, N_s log-uniform in [64, 8192] (seed 2024), "
+                          "even systems open (RSA cube), odd systems single wall (RSA slab, z in "
+                          "[1.5a, 10a]), phi uniform in [0.05, 0.25], a=1, eta=1/(6 pi), F~N(0,1); "
+                          "FP32 pairs, FP64 accumulation; one persistent launch", "N": N}
+    return result_line("Ensemble Mdot Gpair/s, 512 ragged systems (FP32 pairs)",
+                       pairs / (ms_e * 1e-3) / 1e9, "Gpair/s", args.steps, args.warmup, ms_e, True,
+                       "f32", config, compute_roofline(flops, ms_e, PEAK_FP32_TFLOPS), extra, None)
+
+
+# ---------------------------------------------------------------------------
+# C5: single wall with torques, N = 1048576, hydrodynamicVelocities
+
+
+def bench_c5(args):
+    g = Gpu()
+    N = args.N or 1048576
+    Lxy = 1024.0 * math.sqrt(N / 1048576.0)
+    pos = wall_positions(N, Lxy, 1.5, 30.0, seed=1048576)
+    rng = np.random.default_rng(10)
+    F = rng.normal(size=(N, 3))
+    T = rng.normal(size=(N, 3))
+    pos_d, F_d, T_d = g.put(pos), g.put(F), g.put(T)
+    U_d, W_d = g.zeros(3 * N), g.zeros(3 * N)
+    del pos, F, T
+    fpp = flops_per_pair()["wall_6x6"]
+    pairs = float(N) ** 2
+    tiling = args.tiling or (2, 0)
+    t_start = time.perf_counter()
+
+    def progress(i, err):
+        # one line per Lanczos iteration (a pair product of the 6N x 6N operator each)
+        log(f"  Lanczos iteration {i}: error estimate {err:.3e}, "
+            f"{time.perf_counter() - t_start:.0f} s")
+
+    h32 = g.solver(pos_d, "fp32", wall=True, angular=True, delta=1e-4, tiling=tiling,
+                   iteration_limit=1024, callback=progress)
+    log("FP32-pair Mdot")
+    ms32 = g.time_events(lambda: g.mdot(h32, F_d, T_d, U_d, W_d), 1, 0)
+    log(f"  {ms32:.1f} ms")
+    h64 = g.solver(pos_d, "fp64", wall=True, angular=True, delta=1e-4, tiling=tiling)
+    log("FP64-pair Mdot")
+    ms64 = g.time_events(lambda: g.mdot(h64, F_d, T_d, U_d, W_d), 1, 0)
+    log(f"  {ms64:.1f} ms")
+    g.destroy(h64)
+    extra = {"mdot_fp32pair_ms": ms32, "mdot_fp64pair_ms": ms64, "flops_per_pair": fpp,
+             "tiling": {"targets_per_lane": tiling[0], "source_chunks": tiling[1]},
+             "lanczos_iteration_limit": 1024}
+    step_ms = None
+    if not args.no_lanczos:
+        from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
+        net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+        g.ffi.check(g.lib.lm_nbody_set_preconditioner(h32, net._h))
+        log("hydrodynamicVelocities (Mdot + sqrtMdotW tol 1e-6 + RFD drift), FP32 pairs, learned "
+            "preconditioner")
+        g.sync()
+        t = t_start = time.perf_counter()
+        g.ffi.check(g.lib.lm_nbody_hydrodynamic_velocities(
+            h32, F_d.data_ptr(), 3 * N, T_d.data_ptr(), 3 * N, U_d.data_ptr(), W_d.data_ptr(),
+            g.ffi.LM_MEM_DEVICE))
+        g.sync()
+        step_ms = (time.perf_counter() - t) * 1e3
+        it = g.lib.lm_nbody_last_lanczos_iterations(h32)
+        log(f"  {step_ms / 1e3:.1f} s, Lanczos iterations {it}")
+        u = U_d.cpu().numpy()
+        extra.update({
+            "hydrodynamicVelocities_fp32pair_learned_ms": step_ms,
+            "lanczos_iterations": it,
+            "output_finite": bool(np.all(np.isfinite(u))),
+            "fp64pair_step_projected_ms": (it + 4) * ms64,
+            "fp64pair_step_projection": "(iterations + 4 pair products: Mdot, 2 RFD products, "
+                                        "1 estimate slack) x measured FP64 Mdot time; not run",
+        })
+    g.destroy(h32)
+    config = {"workload": f"single-wall RPB/Swan-Brady, N={N}, x,y uniform in [0,{Lxy:.0f}a), z "
+                          f"uniform in [1.5a,30a], a=1, eta=1/(6 pi), F,T~N(0,1); 6x6 operator; "
+                          f"hydrodynamicVelocities = Mdot + sqrtMdotW (prefactor 1, tol 1e-6) + "
+                          f"RFD drift (delta 1e-4 a); FP32 pairs, FP64 accumulation, learned "
+                          f"preconditioner", "N": N}
+    value = step_ms if step_ms is not None else ms32
+    return result_line("hydrodynamicVelocities ms at N=1048576 single wall (FP32 pairs, learned "
+                       "preconditioner)", value, "ms", 1, 1, value, False, "f32", config,
+                       compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS), extra, None)
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 28  # doubles: 2 GiB read + 2 GiB written per copy
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}), flush=True)
 
 
 def main():
@@ -63,10 +739,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("c3", "c1", "c2", "c4", "c5"), default="c3")
     ap.add_argument("--lanczos-calls", type=int, default=3)
-    ap.add_argument("--N", type=int, default=131072)
+    ap.add_argument("--N", type=int, default=None, help="override the particle count (c1, c2, c3, c5)")
+    ap.add_argument("--tiling", default=None,
+                    help="fix the pair tiling as TPL,CHUNKS instead of the autotuner")
+    ap.add_argument("--oracle-rows", type=int, default=4096,
+                    help="c2: target rows of the CPU baseline / checker")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--no-bbt", action="store_true",
+                    help="c3: skip the small-N ||BB^T - M|| check (thousands of small solves)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device-to-device copy (reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -74,169 +757,14 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(sharding is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
-
-    import torch
-    from libmobility_amd import _ffi
-    from libmobility_amd.workloads import rsa_cube
-
+    if args.tiling:
+        args.tiling = tuple(int(x) for x in args.tiling.split(","))
     if args.copy_calibration:
-        n = 1 << 28  # doubles: 2 GiB read + 2 GiB written per copy
-        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}), flush=True)
+        copy_calibration()
         return
-
-    lib = _ffi.load()
-    dev = torch.device("cuda:0")
-    N = args.N
-    phi = 0.20
-    eta = 1.0 / (6.0 * np.pi)
-    log(f"generating RSA packing N={N} phi={phi}")
-    pos, L = rsa_cube(N, phi, seed=2026)
-    rng = np.random.default_rng(7)
-    F = rng.normal(size=(N, 3))
-    Wn = rng.normal(size=3 * N)
-
-    pos_d = torch.from_numpy(pos.reshape(-1).copy()).to(dev)
-    F_d = torch.from_numpy(F.reshape(-1).copy()).to(dev)
-    W_d = torch.from_numpy(Wn).to(dev)
-    U_d = torch.zeros(3 * N, dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream(dev)
-
-    def make_solver(prec, tol=1e-6):
-        h = ctypes.c_void_p()
-        _ffi.check(lib.lm_nbody_create(2, 2, 2, ctypes.byref(h)))
-        _ffi.check(lib.lm_nbody_set_parameters(h, 3, -1, -1, 0, 0.0, 1e-3))
-        _ffi.check(lib.lm_nbody_set_pair_precision(h, _ffi.PAIR_PRECISION[prec]))
-        _ffi.check(lib.lm_nbody_set_stream(h, ctypes.c_void_p(stream.cuda_stream)))
-        _ffi.check(lib.lm_nbody_initialize(h, eta, 1.0, 0, tol, 12345, _ffi.LANCZOS_CALLBACK(), None))
-        _ffi.check(lib.lm_nbody_set_positions(h, pos_d.data_ptr(), 3 * N, _ffi.LM_MEM_DEVICE))
-        _ffi.check(lib.lm_nbody_set_autotune(h, 1))
-        return h
-
-    def tiling(h):
-        t, c = ctypes.c_int(), ctypes.c_int()
-        _ffi.check(lib.lm_nbody_get_pair_tiling(h, ctypes.byref(t), ctypes.byref(c)))
-        return {"targets_per_lane": t.value, "source_chunks": c.value}
-
-    def time_mdot(h, steps, warmup):
-        for _ in range(warmup):
-            _ffi.check(lib.lm_nbody_mdot(h, F_d.data_ptr(), 3 * N, None, 0, U_d.data_ptr(), None,
-                                         _ffi.LM_MEM_DEVICE))
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(steps):
-            _ffi.check(lib.lm_nbody_mdot(h, F_d.data_ptr(), 3 * N, None, 0, U_d.data_ptr(), None,
-                                         _ffi.LM_MEM_DEVICE))
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / steps  # ms per Mdot
-
-    def time_lanczos(h, calls):
-        out = torch.zeros(3 * N, dtype=torch.float64, device=dev)
-        times, iters = [], []
-        for c in range(calls + 1):
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            _ffi.check(lib.lm_nbody_sqrt_mdot_w_noise(h, W_d.data_ptr(), out.data_ptr(), None, 1.0,
-                                                      _ffi.LM_MEM_DEVICE))
-            torch.cuda.synchronize()
-            dt = (time.perf_counter() - t) * 1e3
-            if c > 0:
-                times.append(dt)
-                iters.append(lib.lm_nbody_last_lanczos_iterations(h))
-            log(f"  sqrtMdotW call {c}: {dt:.1f} ms, iterations {lib.lm_nbody_last_lanczos_iterations(h)}")
-        return statistics.median(times), iters
-
-    import json as _json
-    flops = _json.load(open(os.path.join(ROOT, "libmobility_amd", "flop_model.json")))["flops_per_pair"]
-    fpp = flops["open_UF"]
-    pairs = float(N) * float(N)
-
-    log("FP32-pair Mdot")
-    h32 = make_solver("fp32")
-    ms32 = time_mdot(h32, args.steps, args.warmup)
-    gpair32 = pairs / (ms32 * 1e-3) / 1e9
-    log(f"  {ms32:.3f} ms/Mdot, {gpair32:.1f} Gpair/s")
-    log("FP64-pair Mdot")
-    h64 = make_solver("fp64")
-    ms64 = time_mdot(h64, max(3, args.steps // 4), 1)
-    gpair64 = pairs / (ms64 * 1e-3) / 1e9
-    log(f"  {ms64:.3f} ms/Mdot, {gpair64:.1f} Gpair/s")
-
-    extra = {
-        "tiling_fp32pair": tiling(h32),
-        "tiling_fp64pair": tiling(h64),
-        "mdot_fp32pair_ms": ms32,
-        "mdot_fp64pair_ms": ms64,
-        "mdot_fp64pair_gpair_s": gpair64,
-        "mdot_fp64pair_valu_frac_of_fp64_peak": gpair64 * 1e9 * fpp / (PEAK_FP64_TFLOPS * 1e12),
-        "flops_per_pair": fpp,
-    }
-    if not args.no_lanczos:
-        log("plain Lanczos sqrtMdotW, FP32 pairs, tol 1e-6")
-        ms_l32, it32 = time_lanczos(h32, args.lanczos_calls)
-        extra["sqrtMdotW_plain_fp32pair_ms"] = ms_l32
-        extra["sqrtMdotW_plain_fp32pair_iterations"] = it32
-        log("plain Lanczos sqrtMdotW, FP64 pairs, tol 1e-6")
-        ms_l64, it64 = time_lanczos(h64, max(1, args.lanczos_calls - 2))
-        extra["sqrtMdotW_plain_fp64pair_ms"] = ms_l64
-        extra["sqrtMdotW_plain_fp64pair_iterations"] = it64
-        log("learned-preconditioned Lanczos sqrtMdotW, FP32 pairs, tol 1e-6")
-        from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
-        net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
-        hl = make_solver("fp32")
-        _ffi.check(lib.lm_nbody_set_preconditioner(hl, net._h))
-        ms_ll, itl = time_lanczos(hl, args.lanczos_calls)
-        lib.lm_nbody_destroy(hl)
-        extra["sqrtMdotW_learned_fp32pair_ms"] = ms_ll
-        extra["sqrtMdotW_learned_fp32pair_iterations"] = itl
-        extra["sqrtMdotW_learned_speedup_vs_plain"] = ms_l32 / ms_ll
-    for h in (h32, h64):
-        lib.lm_nbody_destroy(h)
-
-    achieved = gpair32 * 1e9 * fpp / 1e12
-    result = {
-        "metric": METRIC,
-        "value": gpair32,
-        "unit": "Gpair/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms32,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"open-boundary RPY Mdot (UF), N={N}, RSA packing phi={phi} in a cube of "
-                        f"side {L:.1f}a, a=1, eta=1/(6 pi), F~N(0,1); FP32 pair terms with FP64 "
-                        f"per-tile accumulation; sqrtMdotW tol 1e-6",
-            "N": N,
-            "pairs_per_step": pairs,
-        },
-        "roofline": {
-            "bound": "valu",
-            "achieved": achieved,
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
-        },
-        "extra": extra,
-    }
-    if not args.no_cpu:
-        log("CPU baseline (oracle, OpenMP FP64)")
-        result["cpu_baseline"] = cpu_baseline(pos, F, eta)
-    else:
-        result["cpu_baseline"] = None
-    print(json.dumps(result), flush=True)
+    res = {"c3": bench_c3, "c1": bench_c1, "c2": bench_c2, "c4": bench_c4,
+           "c5": bench_c5}[args.workload](args)
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

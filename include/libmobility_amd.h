@@ -124,6 +124,22 @@ int lm_nbody_set_persistent(lm_nbody* h, int mode);
  * Range [3, 1024]; the default stays 200. */
 int lm_nbody_set_lanczos_iteration_limit(lm_nbody* h, int limit);
 
+/* Ragged ensembles: one lanczos::Solver per system (LanczosAlgorithm.cu:171-232
+ * applied to each diagonal block, as separate per-system NBody solvers would
+ * run it) instead of one recurrence over the whole block-diagonal operator
+ * (the reference's batched behaviour, the default).  Each system keeps its own
+ * check schedule and previous estimate across calls; all systems advance in
+ * one pair launch per iteration and converged systems drop out of it.  The
+ * Lanczos callback receives the largest error of the systems tested at that
+ * step.  A system whose noise is exactly zero gets a zero result without
+ * iterating. */
+int lm_nbody_set_lanczos_per_system(lm_nbody* h, int on);
+/* Number of independent systems of the current positions (Nbatch of
+ * mobility.h:149-160, or the ragged systems of lm_nbody_set_systems). */
+int lm_nbody_num_systems(lm_nbody* h);
+/* Iterations each system took in the last per-system solve (-1 otherwise). */
+int lm_nbody_last_lanczos_iterations_per_system(lm_nbody* h, int* out, int nsys);
+
 /* NBody::getNumberParticles, mobility.h:164. */
 int64_t lm_nbody_num_particles(lm_nbody* h);
 

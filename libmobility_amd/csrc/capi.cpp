@@ -122,6 +122,20 @@ int lm_nbody_set_lanczos_iteration_limit(lm_nbody* h, int limit) {
   return guarded([&] { S(h).setLanczosIterationLimit(limit); });
 }
 
+int lm_nbody_set_lanczos_per_system(lm_nbody* h, int on) {
+  return guarded([&] { S(h).setLanczosPerSystem(on != 0); });
+}
+
+int lm_nbody_last_lanczos_iterations_per_system(lm_nbody* h, int* out, int nsys) {
+  return guarded([&] {
+    if (!out || nsys != S(h).numSystems())
+      throw lm::Error("[libmobility_amd] output must hold one entry per system");
+    S(h).lastLanczosIterationsPerSystem(out, nsys);
+  });
+}
+
+int lm_nbody_num_systems(lm_nbody* h) { return h ? h->solver.numSystems() : -1; }
+
 int64_t lm_nbody_num_particles(lm_nbody* h) { return h ? h->solver.numParticles() : -1; }
 
 int lm_nbody_mdot(lm_nbody* h, const double* F, int64_t nF, const double* T, int64_t nT,

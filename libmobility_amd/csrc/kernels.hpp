@@ -14,7 +14,7 @@ struct PairItem {
   int32_t ntgt;
   int32_t chunk;
   int32_t wall;  // kernel of the item's system: 0 open RPY, 1 bottom wall
-  int32_t pad;
+  int32_t sys;   // system index (items of inactive systems are skipped)
 };
 
 struct PairArgs {
@@ -33,6 +33,7 @@ struct PairArgs {
   int64_t slab;         // elements between slabs (3N)
   const PairItem* items;
   int nitems;
+  const int* active;    // per-system flags; items of systems with 0 are skipped (nullptr: all)
   double t0, rt0, r0;
 };
 
@@ -107,5 +108,63 @@ void launch_axpy_to(double* dst, const double* src, const double* w, double c, i
 void launch_diff_accumulate(double* out, const double* a, const double* b, double pref,
                             int64_t n, hipStream_t s);
 void launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
+// zero fills by kernel (no memset nodes in code a caller may capture in a graph)
+void launch_zero(double* p, int64_t n, hipStream_t s);
+void launch_zero_u32(unsigned int* p, int64_t n, hipStream_t s);
+
+// ---- independent Lanczos recurrences, one per system ---------------------------
+// A block of the segmented vector kernels: particles [p0, p1) of system `sys`,
+// i.e. vector entries [3 p0, 3 p1) and, for 6N vectors (torques after the
+// forces), [3N + 3 p0, 3N + 3 p1).
+struct SegBlock {
+  int64_t p0, p1;
+  int32_t sys;
+  int32_t pad;
+};
+constexpr int kSegParticles = 512;  // particles per block
+constexpr int kSegTsqrtGroups = 64; // concurrent tridiagonal solves (scratch slices)
+
+struct SegLayout {
+  const SegBlock* blocks;
+  int nblocks;
+  const int* first;       // [nsys + 1]: blocks of system s are first[s] .. first[s+1]-1
+  int nsys;
+  int64_t N3;             // 3N
+  int angular;            // 1: vectors hold 6N entries
+  double* partial;        // [nblocks * 2]
+};
+
+// Per-system state; alpha/beta rows have L entries (iteration capacity).
+struct SegScalars {
+  double* normz;   // [nsys]
+  double* alpha;   // [nsys * L]
+  double* beta;    // [nsys * L]
+  double* err;     // [nsys]
+  const int* active;  // [nsys] 1 while the system iterates
+  const int* check;   // [nsys] 1: estimate this iteration, 2: store y as the previous
+                      // estimate, 3: zero result (the system's noise is zero)
+  int L;
+};
+
+void launch_seg_start(const double* z, double* v0, const SegLayout& g, const SegScalars& sc,
+                      hipStream_t s);
+void launch_seg_alpha(double* w, const double* vi, const double* vp, int i, const SegLayout& g,
+                      const SegScalars& sc, hipStream_t s);
+void launch_seg_beta(double* w, const double* vi, int i, const SegLayout& g, const SegScalars& sc,
+                     hipStream_t s);
+void launch_seg_next(const double* w, double* vn, int i, const SegLayout& g, const SegScalars& sc,
+                     hipStream_t s);
+// coef[s * L + k] = (T_m^{1/2} e1)_k for the listed systems; work holds
+// kSegTsqrtGroups * tsqrt_scratch(m) doubles
+void launch_seg_tridiag_sqrt(const int* list, int nlist, int m, const SegScalars& sc, double* coef,
+                             double* work, hipStream_t s);
+// for systems with check != 0: y = normz V[:, :m] coef_s ; old <- old - y ; err_s
+void launch_seg_estimate(const double* V, int64_t ldv, int m, const double* coef, double* y,
+                         double* old, const SegLayout& g, const SegScalars& sc, hipStream_t s);
+// old <- y for systems with check == 2
+void launch_seg_store(const double* y, double* old, const SegLayout& g, const SegScalars& sc,
+                      hipStream_t s);
+// y <- 0 for systems with check == 3
+void launch_seg_clear(double* y, const SegLayout& g, const SegScalars& sc, hipStream_t s);
 
 }  // namespace lm

@@ -245,9 +245,11 @@ __device__ inline int sturm_count(const double* a, const double* b2, int m, doub
   return cnt;
 }
 
-__global__ __launch_bounds__(kTS) void k_tridiag_sqrt(const LanczosScalars* __restrict__ sc,
-                                                      int m, double* __restrict__ s_out,
-                                                      double* __restrict__ work) {
+// s_out[0..m) = T^{1/2} e1 for T = tridiag(beta, alpha, beta); one workgroup of
+// kTS threads, `work` = tsqrt_scratch(m) doubles.
+__device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
+                                  const double* __restrict__ beta, int m,
+                                  double* __restrict__ s_out, double* __restrict__ work) {
   __shared__ double a[kLanczosMaxIterations], b[kLanczosMaxIterations],
       b2[kLanczosMaxIterations], xs[kLanczosMaxIterations];
   __shared__ double red[kTS / 64];
@@ -256,8 +258,8 @@ __global__ __launch_bounds__(kTS) void k_tridiag_sqrt(const LanczosScalars* __re
   __shared__ int s_bad;
   const int t = threadIdx.x;
   for (int i = t; i < m; i += kTS) {
-    a[i] = sc->alpha[i];
-    b[i] = (i < m - 1) ? sc->beta[i] : 0.0;
+    a[i] = alpha[i];
+    b[i] = (i < m - 1) ? beta[i] : 0.0;
     b2[i] = b[i] * b[i];
   }
   __syncthreads();
@@ -375,11 +377,43 @@ __global__ __launch_bounds__(kTS) void k_tridiag_sqrt(const LanczosScalars* __re
   }
 }
 
+__global__ __launch_bounds__(kTS) void k_tridiag_sqrt(const LanczosScalars* __restrict__ sc,
+                                                      int m, double* __restrict__ s_out,
+                                                      double* __restrict__ work) {
+  tridiag_sqrt_body(sc->alpha, sc->beta, m, s_out, work);
+}
+
 void launch_tridiag_sqrt(const LanczosScalars* sc, int m, double* s_out, double* work,
                          hipStream_t s) {
   if (m < 1 || m > kLanczosMaxIterations)
     throw Error("[libmobility_amd] tridiagonal size out of range");
   hipLaunchKernelGGL(k_tridiag_sqrt, dim3(1), dim3(kTS), 0, s, sc, m, s_out, work);
+  LM_HIP(hipGetLastError());
+}
+
+// One workgroup per listed system (at most kSegTsqrtGroups at a time, each
+// with its own scratch slice).
+__global__ __launch_bounds__(kTS) void k_seg_tridiag_sqrt(const int* __restrict__ list, int nlist,
+                                                          int m, SegScalars sc,
+                                                          double* __restrict__ coef,
+                                                          double* __restrict__ work) {
+  double* my_work = work + (int64_t)blockIdx.x * tsqrt_scratch(m);
+  for (int j = blockIdx.x; j < nlist; j += gridDim.x) {
+    const int sys = list[j];
+    const int64_t row = (int64_t)sys * sc.L;
+    tridiag_sqrt_body(sc.alpha + row, sc.beta + row, m, coef + row, my_work);
+    __syncthreads();  // shared arrays and the scratch slice are reused
+  }
+}
+
+void launch_seg_tridiag_sqrt(const int* list, int nlist, int m, const SegScalars& sc,
+                             double* coef, double* work, hipStream_t s) {
+  if (nlist <= 0) return;
+  if (m < 1 || m > sc.L || m > kLanczosMaxIterations)
+    throw Error("[libmobility_amd] tridiagonal size out of range");
+  const int grid = nlist < kSegTsqrtGroups ? nlist : kSegTsqrtGroups;
+  hipLaunchKernelGGL(k_seg_tridiag_sqrt, dim3(grid), dim3(kTS), 0, s, list, nlist, m, sc, coef,
+                     work);
   LM_HIP(hipGetLastError());
 }
 
@@ -426,6 +460,258 @@ void launch_lanczos_estimate(const double* V, int64_t ldv, int m, const double* 
                              hipStream_t s) {
   hipLaunchKernelGGL(k_estimate, dim3(reduce_blocks(n)), dim3(kRT), 0, s, V, ldv, m, s_coef, y,
                      const_cast<double*>(old), n, rs.partial, rs.counter, sc);
+  LM_HIP(hipGetLastError());
+}
+
+// ---- independent recurrences per system (segmented kernels) --------------------
+// Each workgroup owns one SegBlock: the vector entries of particles [p0, p1)
+// of one system.  Partial sums go to partial[2 * block + q]; a finalising
+// kernel (one wave per system) adds a system's partials in block order, so the
+// sums are reproducible.  Every kernel skips the blocks of inactive systems.
+
+// entry j of a block (0 <= j < 2c for 6N vectors, c = 3 (p1 - p0))
+__device__ inline int64_t seg_entry(const SegBlock& b, int64_t j, int64_t N3) {
+  const int64_t c = 3 * (b.p1 - b.p0);
+  return j < c ? 3 * b.p0 + j : N3 + 3 * b.p0 + (j - c);
+}
+
+__device__ inline int64_t seg_count(const SegBlock& b, int angular) {
+  return 3 * (b.p1 - b.p0) * (angular ? 2 : 1);
+}
+
+// partial[2*blk + 0] = sum z^2
+__global__ __launch_bounds__(kRT) void k_seg_norm(const double* __restrict__ z, SegLayout g,
+                                                 SegScalars sc) {
+  __shared__ double sh[kRT / 64];
+  const SegBlock b = g.blocks[blockIdx.x];
+  if (!sc.active[b.sys]) return;
+  const int64_t cnt = seg_count(b, g.angular);
+  double v = 0.0;
+  for (int64_t j = threadIdx.x; j < cnt; j += kRT) {
+    const double x = z[seg_entry(b, j, g.N3)];
+    v += x * x;
+  }
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) g.partial[2 * blockIdx.x] = v;
+}
+
+// Finalising: one wave per system; MODE 0 normz, 1 alpha[i], 2 beta[i] with
+// the breakdown rule, 3 err from (num, den).
+template <int MODE>
+__global__ __launch_bounds__(kRT) void k_seg_finalize(SegLayout g, SegScalars sc, int i) {
+  const int sys = blockIdx.x * (kRT / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (sys >= g.nsys) return;
+  if (MODE == 3 ? sc.check[sys] == 0 : sc.active[sys] == 0) return;
+  const int b0 = g.first[sys], b1 = g.first[sys + 1];
+  double v0 = 0.0, v1 = 0.0;
+  for (int b = b0 + lane; b < b1; b += 64) {
+    v0 += g.partial[2 * b];
+    if (MODE == 3) v1 += g.partial[2 * b + 1];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    v0 += __shfl_xor(v0, o);
+    if (MODE == 3) v1 += __shfl_xor(v1, o);
+  }
+  if (lane != 0) return;
+  const int64_t row = (int64_t)sys * sc.L;
+  if (MODE == 0) {
+    sc.normz[sys] = sqrt(v0);
+  } else if (MODE == 1) {
+    sc.alpha[row + i] = v0;
+  } else if (MODE == 2) {
+    double be = sqrt(v0);
+    if (be < 1e-3 * sc.alpha[row + i] / sc.normz[sys]) be = 0.0;  // LanczosAlgorithm.cu:139-143
+    sc.beta[row + i] = be;
+  } else {
+    sc.err[sys] = fabs(sqrt(v0) / sqrt(v1));  // LanczosAlgorithm.cu:215
+  }
+}
+
+template <int MODE>
+static void seg_finalize(const SegLayout& g, const SegScalars& sc, int i, hipStream_t s) {
+  const int per = kRT / 64;
+  hipLaunchKernelGGL(k_seg_finalize<MODE>, dim3((g.nsys + per - 1) / per), dim3(kRT), 0, s, g, sc,
+                     i);
+  LM_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kRT) void k_seg_scale(const double* __restrict__ z,
+                                                  double* __restrict__ v0, SegLayout g,
+                                                  SegScalars sc) {
+  const SegBlock b = g.blocks[blockIdx.x];
+  if (!sc.active[b.sys]) return;
+  const double inv = 1.0 / sc.normz[b.sys];
+  const int64_t cnt = seg_count(b, g.angular);
+  for (int64_t j = threadIdx.x; j < cnt; j += kRT) {
+    const int64_t e = seg_entry(b, j, g.N3);
+    v0[e] = z[e] * inv;
+  }
+}
+
+void launch_seg_start(const double* z, double* v0, const SegLayout& g, const SegScalars& sc,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_norm, dim3(g.nblocks), dim3(kRT), 0, s, z, g, sc);
+  LM_HIP(hipGetLastError());
+  seg_finalize<0>(g, sc, 0, s);
+  hipLaunchKernelGGL(k_seg_scale, dim3(g.nblocks), dim3(kRT), 0, s, z, v0, g, sc);
+  LM_HIP(hipGetLastError());
+}
+
+// w -= beta_s[i-1] v_prev ; partial w . v_i
+__global__ __launch_bounds__(kRT) void k_seg_alpha(double* __restrict__ w,
+                                                  const double* __restrict__ vi,
+                                                  const double* __restrict__ vp, int i,
+                                                  SegLayout g, SegScalars sc) {
+  __shared__ double sh[kRT / 64];
+  const SegBlock b = g.blocks[blockIdx.x];
+  if (!sc.active[b.sys]) return;
+  const double be = (i > 0) ? sc.beta[(int64_t)b.sys * sc.L + i - 1] : 0.0;
+  const int64_t cnt = seg_count(b, g.angular);
+  double v = 0.0;
+  for (int64_t j = threadIdx.x; j < cnt; j += kRT) {
+    const int64_t e = seg_entry(b, j, g.N3);
+    double wk = w[e];
+    if (i > 0) {
+      wk -= be * vp[e];
+      w[e] = wk;
+    }
+    v += wk * vi[e];
+  }
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) g.partial[2 * blockIdx.x] = v;
+}
+
+void launch_seg_alpha(double* w, const double* vi, const double* vp, int i, const SegLayout& g,
+                      const SegScalars& sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_alpha, dim3(g.nblocks), dim3(kRT), 0, s, w, vi, vp, i, g, sc);
+  LM_HIP(hipGetLastError());
+  seg_finalize<1>(g, sc, i, s);
+}
+
+// w -= alpha_s[i] v_i ; partial ||w||^2
+__global__ __launch_bounds__(kRT) void k_seg_beta(double* __restrict__ w,
+                                                 const double* __restrict__ vi, int i, SegLayout g,
+                                                 SegScalars sc) {
+  __shared__ double sh[kRT / 64];
+  const SegBlock b = g.blocks[blockIdx.x];
+  if (!sc.active[b.sys]) return;
+  const double al = sc.alpha[(int64_t)b.sys * sc.L + i];
+  const int64_t cnt = seg_count(b, g.angular);
+  double v = 0.0;
+  for (int64_t j = threadIdx.x; j < cnt; j += kRT) {
+    const int64_t e = seg_entry(b, j, g.N3);
+    const double wk = w[e] - al * vi[e];
+    w[e] = wk;
+    v += wk * wk;
+  }
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) g.partial[2 * blockIdx.x] = v;
+}
+
+void launch_seg_beta(double* w, const double* vi, int i, const SegLayout& g, const SegScalars& sc,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_beta, dim3(g.nblocks), dim3(kRT), 0, s, w, vi, i, g, sc);
+  LM_HIP(hipGetLastError());
+  seg_finalize<2>(g, sc, i, s);
+}
+
+// v_{i+1} = w / beta_s[i], or the system's e1 after a breakdown
+__global__ __launch_bounds__(kRT) void k_seg_next(const double* __restrict__ w,
+                                                 double* __restrict__ vn, int i, SegLayout g,
+                                                 SegScalars sc, const int* __restrict__ first) {
+  const SegBlock b = g.blocks[blockIdx.x];
+  if (!sc.active[b.sys]) return;
+  const double be = sc.beta[(int64_t)b.sys * sc.L + i];
+  const double inv = be > 0.0 ? 1.0 / be : 0.0;
+  // e1 of a system is its first entry: entry 0 of the system's first block
+  const bool head = (int)blockIdx.x == first[b.sys];
+  const int64_t cnt = seg_count(b, g.angular);
+  for (int64_t j = threadIdx.x; j < cnt; j += kRT) {
+    const int64_t e = seg_entry(b, j, g.N3);
+    vn[e] = be > 0.0 ? w[e] * inv : ((head && j == 0) ? 1.0 : 0.0);
+  }
+}
+
+void launch_seg_next(const double* w, double* vn, int i, const SegLayout& g, const SegScalars& sc,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_next, dim3(g.nblocks), dim3(kRT), 0, s, w, vn, i, g, sc, g.first);
+  LM_HIP(hipGetLastError());
+}
+
+// y = normz_s V[:, :m] coef_s ; old <- old - y ; partials (||old - y||^2, ||old||^2)
+__global__ __launch_bounds__(kRT) void k_seg_estimate(const double* __restrict__ V, int64_t ldv,
+                                                     int m, const double* __restrict__ coef,
+                                                     double* __restrict__ y,
+                                                     double* __restrict__ old, SegLayout g,
+                                                     SegScalars sc) {
+  __shared__ double sc_s[kLanczosMaxIterations];
+  __shared__ double sh[kRT / 64];
+  const SegBlock b = g.blocks[blockIdx.x];
+  if (!sc.check[b.sys]) return;
+  const double nz = sc.normz[b.sys];
+  const double* cs = coef + (int64_t)b.sys * sc.L;
+  for (int k = threadIdx.x; k < m; k += kRT) sc_s[k] = nz * cs[k];
+  __syncthreads();
+  const int64_t cnt = seg_count(b, g.angular);
+  double num = 0.0, den = 0.0;
+  for (int64_t j = threadIdx.x; j < cnt; j += kRT) {
+    const int64_t e = seg_entry(b, j, g.N3);
+    double v = 0.0;
+    for (int k = 0; k < m; ++k) v += V[(int64_t)k * ldv + e] * sc_s[k];
+    y[e] = v;
+    const double o = old[e];
+    const double d = o - v;
+    old[e] = d;
+    num += d * d;
+    den += o * o;
+  }
+  num = block_sum(num, sh);
+  den = block_sum(den, sh);
+  if (threadIdx.x == 0) {
+    g.partial[2 * blockIdx.x] = num;
+    g.partial[2 * blockIdx.x + 1] = den;
+  }
+}
+
+void launch_seg_estimate(const double* V, int64_t ldv, int m, const double* coef, double* y,
+                         double* old, const SegLayout& g, const SegScalars& sc, hipStream_t s) {
+  if (m < 1 || m > sc.L || m > kLanczosMaxIterations)
+    throw Error("[libmobility_amd] Krylov dimension out of range");
+  hipLaunchKernelGGL(k_seg_estimate, dim3(g.nblocks), dim3(kRT), 0, s, V, ldv, m, coef, y, old, g,
+                     sc);
+  LM_HIP(hipGetLastError());
+  seg_finalize<3>(g, sc, 0, s);
+}
+
+__global__ __launch_bounds__(kRT) void k_seg_store(const double* __restrict__ y,
+                                                  double* __restrict__ old, SegLayout g,
+                                                  SegScalars sc) {
+  const SegBlock b = g.blocks[blockIdx.x];
+  if (sc.check[b.sys] != 2) return;
+  const int64_t cnt = seg_count(b, g.angular);
+  for (int64_t j = threadIdx.x; j < cnt; j += kRT) {
+    const int64_t e = seg_entry(b, j, g.N3);
+    old[e] = y[e];
+  }
+}
+
+void launch_seg_store(const double* y, double* old, const SegLayout& g, const SegScalars& sc,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_store, dim3(g.nblocks), dim3(kRT), 0, s, y, old, g, sc);
+  LM_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kRT) void k_seg_clear(double* __restrict__ y, SegLayout g,
+                                                  SegScalars sc) {
+  const SegBlock b = g.blocks[blockIdx.x];
+  if (sc.check[b.sys] != 3) return;
+  const int64_t cnt = seg_count(b, g.angular);
+  for (int64_t j = threadIdx.x; j < cnt; j += kRT) y[seg_entry(b, j, g.N3)] = 0.0;
+}
+
+void launch_seg_clear(double* y, const SegLayout& g, const SegScalars& sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_clear, dim3(g.nblocks), dim3(kRT), 0, s, y, g, sc);
   LM_HIP(hipGetLastError());
 }
 
@@ -476,6 +762,28 @@ __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src,
 void launch_copy(double* dst, const double* src, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_copy, dim3(reduce_blocks(n) * 4), dim3(kRT), 0, s, dst, src, n);
+  LM_HIP(hipGetLastError());
+}
+
+__global__ void k_zero(double* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+void launch_zero(double* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero, dim3(reduce_blocks(n) * 4), dim3(kRT), 0, s, p, n);
+  LM_HIP(hipGetLastError());
+}
+
+__global__ void k_zero_u32(unsigned int* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+void launch_zero_u32(unsigned int* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kRT), 0, s, p, n);
   LM_HIP(hipGetLastError());
 }
 

@@ -69,4 +69,36 @@ class DevBuf {
   size_t cap_ = 0;
 };
 
+// Pinned host allocation that only grows (small device<->host exchanges of
+// per-system flags and errors during a solve).
+class HostBuf {
+ public:
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  ~HostBuf() { release(); }
+  void* reserve(size_t bytes) {
+    if (bytes > cap_) {
+      release();
+      size_t want = bytes < 256 ? 256 : bytes;
+      LM_HIP(hipHostMalloc(&p_, want, hipHostMallocDefault));
+      cap_ = want;
+    }
+    return p_;
+  }
+  void release() {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    cap_ = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p_);
+  }
+
+ private:
+  void* p_ = nullptr;
+  size_t cap_ = 0;
+};
+
 }  // namespace lm

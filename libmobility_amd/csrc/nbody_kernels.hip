@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(
     const PairItem* __restrict__ items, int nitems, unsigned int* __restrict__ counter,
     const typename Rec<R>::T* __restrict__ pos, const double* __restrict__ F,
     const double* __restrict__ T, double* __restrict__ U, double* __restrict__ W, int64_t slab,
-    double t0, double rt0, double r0) {
+    const int* __restrict__ active, double t0, double rt0, double r0) {
   using R4 = typename Rec<R>::T;
   constexpr bool kNeedF = (BLK & (kUF | kWF)) != 0;
   constexpr bool kNeedT = (BLK & (kUT | kWT)) != 0;
@@ -161,6 +161,10 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(
     }
     if (item >= nitems) break;
     const PairItem it = items[item];
+    if (active != nullptr && active[it.sys] == 0) {  // system already converged
+      if (!PERSIST) break;
+      continue;
+    }
     if (KIND == 2) {
       if (it.wall)
         pair_item<R, true, BLK, TPL>(it, pos, F, T, U, W, slab, t0, rt0, r0, s_pos, s_f, s_t);
@@ -178,14 +182,14 @@ static void launch_pair_t(const PairArgs& a, hipStream_t s) {
   using R4 = typename Rec<R>::T;
   if (a.persistent) {
     if (!a.counter) throw Error("[libmobility_amd] persistent launch without a work counter");
-    LM_HIP(hipMemsetAsync(a.counter, 0, sizeof(unsigned int), s));
+    launch_zero_u32(a.counter, 1, s);
     hipLaunchKernelGGL((k_pair<R, KIND, BLK, TPL, true>), dim3(a.grid), dim3(kPairThreads), 0, s,
                        a.items, a.nitems, a.counter, static_cast<const R4*>(a.pos), a.F, a.T, a.U,
-                       a.W, a.slab, a.t0, a.rt0, a.r0);
+                       a.W, a.slab, a.active, a.t0, a.rt0, a.r0);
   } else {
     hipLaunchKernelGGL((k_pair<R, KIND, BLK, TPL, false>), dim3(a.nitems), dim3(kPairThreads), 0,
                        s, a.items, a.nitems, a.counter, static_cast<const R4*>(a.pos), a.F, a.T,
-                       a.U, a.W, a.slab, a.t0, a.rt0, a.r0);
+                       a.U, a.W, a.slab, a.active, a.t0, a.rt0, a.r0);
   }
   LM_HIP(hipGetLastError());
 }

@@ -249,7 +249,7 @@ void NBodySolver::ensure_items(int tpl, int nch_hint) {
         it.src1 = std::min<int64_t>(b + (int64_t)(c + 1) * cs, e);
         it.chunk = c;
         it.wall = kind;
-        it.pad = 0;
+        it.sys = s;
         items.push_back(it);
       }
     }
@@ -303,6 +303,7 @@ void NBodySolver::launch_pair_items(int blocks, bool fp32, const void* rec, cons
   a.T = (blocks & (kUT | kWT)) ? T : nullptr;
   a.items = items_.as<PairItem>();
   a.nitems = items_count_;
+  a.active = pair_active_;
   a.t0 = t0_;
   a.rt0 = rt0_;
   a.r0 = r0_;
@@ -406,8 +407,8 @@ void NBodySolver::pair_product(const void* rec64, const void* rec32, const doubl
   const int64_t n3 = 3 * N_;
   if (!U) throw Error("[libmobility_amd] Mdot needs a linear output array");
   if (blocks == 0) {
-    LM_HIP(hipMemsetAsync(U, 0, sizeof(double) * n3, stream_));
-    if (angular_ && W) LM_HIP(hipMemsetAsync(W, 0, sizeof(double) * n3, stream_));
+    launch_zero(U, n3, stream_);
+    if (angular_ && W) launch_zero(W, n3, stream_);
     return;
   }
   const bool ang = (blocks & (kWT | kWF)) != 0;
@@ -420,7 +421,7 @@ void NBodySolver::pair_product(const void* rec64, const void* rec32, const doubl
   ensure_items(tpl, nch);
   if (items_count_ <= 0) throw Error("[libmobility_amd] empty pair work list");
   launch_pair_items(blocks, fp32, rec, F, T, U, W);
-  if (angular_ && W && !ang) LM_HIP(hipMemsetAsync(W, 0, sizeof(double) * n3, stream_));
+  if (angular_ && W && !ang) launch_zero(W, n3, stream_);
 }
 
 // Records are scaled by 1/a; rebuild them if initialize() changed a after
@@ -483,6 +484,21 @@ void NBodySolver::Mdot(const double* F, int64_t nF, const double* T, int64_t nT,
 
 // ---- Lanczos ---------------------------------------------------------------------
 
+// Krylov basis, column-major n x cols, grown geometrically up to the
+// iteration limit + 1 columns, keeping the columns already computed.
+void NBodySolver::ensure_basis(int64_t n, int64_t cols) {
+  if (V_n_ != n) {
+    V_.release();
+    V_cols_ = 0;
+    V_n_ = n;
+  }
+  if (cols <= V_cols_) return;
+  int64_t want = std::max<int64_t>(cols, std::min<int64_t>(iter_limit_ + 1,
+                                                          std::max<int64_t>(32, 2 * V_cols_)));
+  V_.grow_keep(sizeof(double) * n * want, sizeof(double) * n * V_cols_, stream_);
+  V_cols_ = want;
+}
+
 void NBodySolver::setLanczosIterationLimit(int limit) {
   if (limit < 3 || limit > kLanczosMaxIterations)
     throw Error("[libmobility_amd] Lanczos iteration limit must be in [3, " +
@@ -498,7 +514,7 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
   red_partial_.reserve(sizeof(double) * 4 * kReduceBlocks);
   if (red_counter_.capacity() == 0) {
     red_counter_.reserve(sizeof(unsigned int) * 64);
-    LM_HIP(hipMemsetAsync(red_counter_.as<void>(), 0, sizeof(unsigned int) * 64, stream_));
+    launch_zero_u32(red_counter_.as<unsigned int>(), 64, stream_);
   }
   scalars_.reserve(sizeof(LanczosScalars));
   tsq_work_.reserve(sizeof(double) * tsqrt_scratch(iter_limit_));
@@ -519,8 +535,8 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     bufB_.release();
     bufA_.reserve(sizeof(double) * (n + 1));
     bufB_.reserve(sizeof(double) * (n + 1));
-    LM_HIP(hipMemsetAsync(bufA_.as<void>(), 0, sizeof(double) * (n + 1), stream_));
-    LM_HIP(hipMemsetAsync(bufB_.as<void>(), 0, sizeof(double) * (n + 1), stream_));
+    launch_zero(bufA_.as<double>(), n + 1, stream_);
+    launch_zero(bufB_.as<double>(), n + 1, stream_);
     old_buf_ = bufA_.as<double>();
     y_buf_ = bufB_.as<double>();
     if (keep > 0) {
@@ -530,19 +546,7 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     }
     old_n_ = n;
   }
-  // Krylov basis, column-major n x cols
-  auto ensure_cols = [&](int64_t cols) {
-    if (V_n_ != n) {
-      V_.release();
-      V_cols_ = 0;
-      V_n_ = n;
-    }
-    if (cols <= V_cols_) return;
-    int64_t want = std::max<int64_t>(cols, std::min<int64_t>(iter_limit_ + 1,
-                                                            std::max<int64_t>(32, 2 * V_cols_)));
-    V_.grow_keep(sizeof(double) * n * want, sizeof(double) * n * V_cols_, stream_);
-    V_cols_ = want;
-  };
+  auto ensure_cols = [&](int64_t cols) { ensure_basis(n, cols); };
   if (precond_) {
     ensure_precond_blocks();
     pc_t_.reserve(sizeof(double) * n);
@@ -557,19 +561,7 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     ensure_cols(i + 2);
     V = V_.as<double>();
     const double* vi = V + (int64_t)i * n;
-    // w = M v_i ; torques are stored after the forces (MobilityInterface.h:164-176)
-    if (precond_) {
-      // w = G M G v_i with the learned block-diagonal G
-      double* t = pc_t_.as<double>();
-      double* u = pc_u_.as<double>();
-      launch_apply_blocks(pc_G_.as<double>(), vi, N_, n, t, stream_);
-      pair_product(rec64_.as<void>(), rec32_.as<void>(), t, angular_ ? t + N3 : nullptr, u,
-                   angular_ ? u + N3 : nullptr);
-      launch_apply_blocks(pc_G_.as<double>(), u, N_, n, w, stream_);
-    } else {
-      pair_product(rec64_.as<void>(), rec32_.as<void>(), vi, angular_ ? vi + N3 : nullptr, w,
-                   angular_ ? w + N3 : nullptr);
-    }
+    lanczos_product(vi, w);
     launch_lanczos_alpha(w, vi, i > 0 ? V + (int64_t)(i - 1) * n : nullptr, n, i, rs, sc, stream_);
     launch_lanczos_beta(w, vi, n, i, rs, sc, stream_);
     launch_lanczos_next(w, V + (int64_t)(i + 1) * n, n, i, sc, stream_);
@@ -600,6 +592,221 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
   throw Error("[Lanczos] Could not converge");
 }
 
+// w = M v ; torques are stored after the forces (MobilityInterface.h:164-176).
+// With the learned preconditioner: w = G M G v, G block-diagonal.
+void NBodySolver::lanczos_product(const double* v, double* w) {
+  const int64_t n = (angular_ ? 6 : 3) * N_;
+  const int64_t N3 = 3 * N_;
+  if (precond_) {
+    double* t = pc_t_.as<double>();
+    double* u = pc_u_.as<double>();
+    launch_apply_blocks(pc_G_.as<double>(), v, N_, n, t, stream_);
+    pair_product(rec64_.as<void>(), rec32_.as<void>(), t, angular_ ? t + N3 : nullptr, u,
+                 angular_ ? u + N3 : nullptr);
+    launch_apply_blocks(pc_G_.as<double>(), u, N_, n, w, stream_);
+  } else {
+    pair_product(rec64_.as<void>(), rec32_.as<void>(), v, angular_ ? v + N3 : nullptr, w,
+                 angular_ ? w + N3 : nullptr);
+  }
+}
+
+// ---- one Lanczos recurrence per system ------------------------------------------
+// Each system runs lanczos::Solver::run (LanczosAlgorithm.cu:171-201) on its
+// own block of the block-diagonal operator: its own ||z||, alpha/beta,
+// breakdown test, check schedule (check_convergence_steps, adapted by
+// registerRequiredStepsForConverge, cu:224-232), previous estimate (oldBz)
+// and convergence test.  All systems advance together; a converged system
+// leaves the pair work list (its items are skipped) and the vector kernels.
+
+void NBodySolver::ensure_seg_layout(int64_t n) {
+  const int nsys = numSystems();
+  if (seg_off_ != sys_off_host_) {
+    std::vector<SegBlock> blocks;
+    std::vector<int> first(nsys + 1, 0);
+    for (int s = 0; s < nsys; ++s) {
+      first[s] = (int)blocks.size();
+      for (int64_t p0 = sys_off_host_[s]; p0 < sys_off_host_[s + 1]; p0 += kSegParticles) {
+        SegBlock b;
+        b.p0 = p0;
+        b.p1 = std::min<int64_t>(p0 + kSegParticles, sys_off_host_[s + 1]);
+        b.sys = s;
+        b.pad = 0;
+        blocks.push_back(b);
+      }
+    }
+    first[nsys] = (int)blocks.size();
+    if (blocks.empty()) throw Error("[libmobility_amd] no particles in any system");
+    seg_blocks_.reserve(sizeof(SegBlock) * blocks.size());
+    seg_first_.reserve(sizeof(int) * first.size());
+    LM_HIP(hipMemcpyAsync(seg_blocks_.as<void>(), blocks.data(), sizeof(SegBlock) * blocks.size(),
+                          hipMemcpyHostToDevice, stream_));
+    LM_HIP(hipMemcpyAsync(seg_first_.as<void>(), first.data(), sizeof(int) * first.size(),
+                          hipMemcpyHostToDevice, stream_));
+    LM_HIP(hipStreamSynchronize(stream_));  // host vectors go out of scope
+    seg_nblocks_ = (int)blocks.size();
+    seg_partial_.reserve(sizeof(double) * 2 * blocks.size());
+    seg_flags_.reserve(sizeof(int) * 3 * nsys);
+    seg_host_.reserve(sizeof(int) * 4 * nsys + sizeof(double) * nsys);
+    seg_check_steps_.assign(nsys, 3);  // LanczosAlgorithm.cu:171, one solver per system
+    seg_iters_.assign(nsys, 0);
+    seg_off_ = sys_off_host_;
+    seg_n_ = 0;
+    seg_L_ = 0;
+  }
+  if (seg_n_ != n) {
+    // previous estimates of every system start at zero (cu:175)
+    seg_old_.reserve(sizeof(double) * n);
+    seg_y_.reserve(sizeof(double) * n);
+    launch_zero(seg_old_.as<double>(), n, stream_);
+    launch_zero(seg_y_.as<double>(), n, stream_);
+    seg_n_ = n;
+  }
+  if (seg_L_ < iter_limit_) {
+    seg_state_.reserve(sizeof(double) * ((size_t)2 * nsys + (size_t)2 * nsys * iter_limit_));
+    seg_coef_.reserve(sizeof(double) * (size_t)nsys * iter_limit_);
+    seg_L_ = iter_limit_;
+  }
+}
+
+void NBodySolver::lastLanczosIterationsPerSystem(int* out, int nsys) const {
+  const int have = (int)seg_iters_.size();
+  for (int s = 0; s < nsys; ++s) out[s] = (seg_mode_ && s < have) ? seg_iters_[s] : -1;
+}
+
+int NBodySolver::lanczos_run_seg(const double* z, double** result) {
+  const int64_t n = (angular_ ? 6 : 3) * N_;
+  const int nsys = numSystems();
+  ensure_records();
+  ensure_seg_layout(n);
+  const int L = seg_L_;
+  w_.reserve(sizeof(double) * n);
+  if (precond_) {
+    ensure_precond_blocks();
+    pc_t_.reserve(sizeof(double) * n);
+    pc_u_.reserve(sizeof(double) * n);
+  }
+  double* st = seg_state_.as<double>();
+  int* d_active = seg_flags_.as<int>();
+  int* d_check = d_active + nsys;
+  int* d_list = d_active + 2 * nsys;
+  SegScalars sc{st, st + 2 * nsys, st + 2 * nsys + (int64_t)nsys * L, st + nsys, d_active,
+                d_check, L};
+  SegLayout g{seg_blocks_.as<SegBlock>(), seg_nblocks_, seg_first_.as<int>(), nsys, 3 * N_,
+              angular_ ? 1 : 0, seg_partial_.as<double>()};
+  // pinned host mirrors: the "check" flags alternate between two slots so a
+  // slot is only rewritten after the stream has consumed its last upload
+  int* h_active = seg_host_.as<int>();
+  int* h_check_est = h_active + nsys;
+  int* h_check_store = h_active + 2 * nsys;
+  int* h_list = h_active + 3 * nsys;
+  double* h_err = reinterpret_cast<double*>(h_active + 4 * nsys);
+  double* old = seg_old_.as<double>();
+  double* y = seg_y_.as<double>();
+  double* coef = seg_coef_.as<double>();
+
+  int nact = 0;
+  std::vector<int> check_at(nsys);
+  for (int s = 0; s < nsys; ++s) {
+    h_active[s] = sys_off_host_[s + 1] > sys_off_host_[s] ? 1 : 0;
+    nact += h_active[s];
+    seg_iters_[s] = 0;
+    check_at[s] = std::min(seg_check_steps_[s], iter_limit_ - 2);  // cu:178-179
+  }
+  LM_HIP(hipMemcpyAsync(d_active, h_active, sizeof(int) * nsys, hipMemcpyHostToDevice, stream_));
+  ensure_basis(n, 2);
+  launch_seg_start(z, V_.as<double>(), g, sc, stream_);  // cu:110-118 per system
+  // A system whose noise is exactly zero has the result zero; it does not
+  // iterate (its ||z|| = 0 would make every estimate NaN).
+  LM_HIP(hipMemcpyAsync(h_err, sc.normz, sizeof(double) * nsys, hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));
+  {
+    bool any_zero = false;
+    for (int s = 0; s < nsys; ++s) {
+      h_check_store[s] = 0;
+      if (h_active[s] && h_err[s] == 0.0) {
+        h_active[s] = 0;
+        h_check_store[s] = 3;
+        --nact;
+        any_zero = true;
+      }
+    }
+    if (any_zero) {
+      LM_HIP(hipMemcpyAsync(d_check, h_check_store, sizeof(int) * nsys, hipMemcpyHostToDevice,
+                            stream_));
+      launch_seg_clear(y, g, sc, stream_);
+      LM_HIP(hipMemcpyAsync(d_active, h_active, sizeof(int) * nsys, hipMemcpyHostToDevice, stream_));
+    }
+  }
+  struct ActiveGuard {
+    const int*& p;
+    ~ActiveGuard() { p = nullptr; }
+  } guard{pair_active_};
+  pair_active_ = d_active;
+  int last = 0;
+  for (int i = 0; i < iter_limit_ && nact > 0; ++i) {
+    ensure_basis(n, i + 2);
+    double* V = V_.as<double>();
+    const double* vi = V + (int64_t)i * n;
+    double* w = w_.as<double>();
+    lanczos_product(vi, w);
+    launch_seg_alpha(w, vi, i > 0 ? V + (int64_t)(i - 1) * n : nullptr, i, g, sc, stream_);
+    launch_seg_beta(w, vi, i, g, sc, stream_);
+    launch_seg_next(w, V + (int64_t)(i + 1) * n, i, g, sc, stream_);
+    int ncheck = 0;
+    for (int s = 0; s < nsys; ++s) {
+      const bool c = h_active[s] && (i >= check_at[s] || cb_ != nullptr);
+      h_check_est[s] = c ? 1 : 0;
+      if (c) h_list[ncheck++] = s;
+    }
+    if (ncheck == 0) continue;
+    const int m = i + 1;
+    if (seg_work_cols_ < (size_t)m) {
+      seg_work_.reserve(sizeof(double) * kSegTsqrtGroups * tsqrt_scratch(L));
+      seg_work_cols_ = (size_t)L;
+    }
+    LM_HIP(hipMemcpyAsync(d_check, h_check_est, sizeof(int) * nsys, hipMemcpyHostToDevice, stream_));
+    LM_HIP(hipMemcpyAsync(d_list, h_list, sizeof(int) * ncheck, hipMemcpyHostToDevice, stream_));
+    launch_seg_tridiag_sqrt(d_list, ncheck, m, sc, coef, seg_work_.as<double>(), stream_);
+    launch_seg_estimate(V, n, m, coef, y, old, g, sc, stream_);
+    LM_HIP(hipMemcpyAsync(h_err, sc.err, sizeof(double) * nsys, hipMemcpyDeviceToHost, stream_));
+    LM_HIP(hipStreamSynchronize(stream_));
+    bool changed = false;
+    double worst = 0.0;
+    for (int s = 0; s < nsys; ++s) h_check_store[s] = 0;
+    for (int j = 0; j < ncheck; ++j) {
+      const int s = h_list[j];
+      h_check_store[s] = 2;  // oldBz <- Bz unless converged (cu:196)
+      if (i == 0) continue;  // the error is only tested from the second step (cu:185)
+      const double err = h_err[s];
+      if (std::isnan(err)) throw Error("[Lanczos] Unknown error (found NaN in result guess)");
+      worst = std::max(worst, err);
+      if (err <= tol_) {
+        // registerRequiredStepsForConverge, cu:224-232
+        if (i - 2 > seg_check_steps_[s])
+          seg_check_steps_[s] += 1;
+        else
+          seg_check_steps_[s] = std::max(1, seg_check_steps_[s] - 2);
+        seg_iters_[s] = i;
+        last = std::max(last, i);
+        h_active[s] = 0;
+        h_check_store[s] = 0;
+        --nact;
+        changed = true;
+      }
+    }
+    if (cb_ && i > 0) cb_(i, worst, cb_user_);
+    LM_HIP(hipMemcpyAsync(d_check, h_check_store, sizeof(int) * nsys, hipMemcpyHostToDevice,
+                          stream_));
+    launch_seg_store(y, old, g, sc, stream_);
+    if (changed)
+      LM_HIP(hipMemcpyAsync(d_active, h_active, sizeof(int) * nsys, hipMemcpyHostToDevice, stream_));
+  }
+  if (nact > 0) throw Error("[Lanczos] Could not converge");
+  last_iters_ = last;
+  *result = y;
+  return last;
+}
+
 void NBodySolver::sqrtMdotW_dev(double* U, double* W, double prefactor, const double* noise) {
   const int64_t n = (angular_ ? 6 : 3) * N_;
   const double* z = noise;
@@ -616,7 +823,10 @@ void NBodySolver::sqrtMdotW_dev(double* U, double* W, double prefactor, const do
     z = noise_.as<double>();
   }
   double* y = nullptr;
-  lanczos_run(z, &y);
+  if (seg_mode_ && numSystems() > 1)
+    lanczos_run_seg(z, &y);
+  else
+    lanczos_run(z, &y);
   if (precond_) {
     // B W = G^{-1} (G M G)^{1/2} W
     launch_apply_blocks(pc_Gi_.as<double>(), y, N_, n, pc_t_.as<double>(), stream_);

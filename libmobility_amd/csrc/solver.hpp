@@ -50,6 +50,15 @@ class NBodySolver {
   // iterationHardLimit member of lanczos::Solver, LanczosAlgorithm.h:49).
   void setLanczosIterationLimit(int limit);
   int lanczosIterationLimit() const { return iter_limit_; }
+  // Ensembles: one Lanczos recurrence per system (each with its own tolerance
+  // test, check schedule, previous estimate and iteration count, as separate
+  // solvers per system would have) instead of one over the whole
+  // block-diagonal operator.
+  void setLanczosPerSystem(bool on) { seg_mode_ = on; }
+  bool lanczosPerSystem() const { return seg_mode_; }
+  int numSystems() const { return (int)sys_off_host_.size() - 1; }
+  // iterations of the last solve per system (per-system mode), else -1s
+  void lastLanczosIterationsPerSystem(int* out, int nsys) const;
   int64_t numParticles() const { return N_; }
   bool includeAngular() const { return angular_; }
 
@@ -97,6 +106,11 @@ class NBodySolver {
   void pair_product(const void* rec64, const void* rec32, const double* F, const double* T,
                     double* U, double* W);
   int lanczos_run(const double* z, double** result);
+  int lanczos_run_seg(const double* z, double** result);
+  void ensure_seg_layout(int64_t n);
+  void ensure_basis(int64_t n, int64_t cols);
+  // w = M v (or G M G v with the preconditioner) for the Lanczos recurrences
+  void lanczos_product(const double* v, double* w);
 
   // configuration
   bool wall_ = false;
@@ -157,6 +171,18 @@ class NBodySolver {
   int check_steps_ = 3;          // LanczosAlgorithm.cu:171
   int iter_limit_ = 200;         // LanczosAlgorithm.h:49
   int last_iters_ = 0;
+  // per-system Lanczos
+  bool seg_mode_ = false;
+  const int* pair_active_ = nullptr;  // per-system flags seen by the pair kernel
+  DevBuf seg_blocks_, seg_first_, seg_partial_, seg_state_, seg_flags_, seg_coef_, seg_work_,
+      seg_old_, seg_y_;
+  HostBuf seg_host_;  // active[nsys], check[nsys], list[nsys] (int), err[nsys] (double)
+  int seg_nblocks_ = 0, seg_L_ = 0;
+  uint64_t seg_layout_ = 0;
+  int64_t seg_n_ = 0;
+  size_t seg_work_cols_ = 0;
+  std::vector<int> seg_check_steps_, seg_iters_;
+  std::vector<int64_t> seg_off_;  // systems the per-system state belongs to
   // learned preconditioner
   void ensure_knn();
   void ensure_precond_blocks();

@@ -198,6 +198,19 @@ class NBody:
         """Extension: persistent pair launches (-1 auto, 0 off, 1 on)."""
         _ffi.check(self._lib.lm_nbody_set_persistent(self._h, int(mode)))
 
+    def setLanczosPerSystem(self, on=True):
+        """Ragged ensembles: one Lanczos recurrence per system (each converging
+        on its own, as separate per-system solvers would) instead of one over
+        the whole block-diagonal operator (the reference's batched mode)."""
+        _ffi.check(self._lib.lm_nbody_set_lanczos_per_system(self._h, int(bool(on))))
+
+    def lastLanczosIterationsPerSystem(self):
+        nsys = int(self._lib.lm_nbody_num_systems(self._h))
+        out = np.zeros(max(nsys, 0), dtype=np.int32)
+        _ffi.check(self._lib.lm_nbody_last_lanczos_iterations_per_system(
+            self._h, out.ctypes.data, nsys))
+        return out
+
     def setLanczosIterationLimit(self, limit=200):
         """Iterations before "[Lanczos] Could not converge" (the public
         ``iterationHardLimit`` of the reference's lanczos::Solver,
@@ -312,9 +325,18 @@ class NBody:
                                            _ptr(mt), mem))
         return mf, mt
 
+    def _own_shape(self):
+        """The module-global last shape (pythonify.h:34-36) in form -- (3N,) or
+        (N, 3) -- but sized for this solver's N: another solver's positions
+        set since must not size this solver's outputs (they receive 3N
+        values)."""
+        N = self.getNumberParticles()
+        return (3 * N,) if len(_last_shape) == 1 else (N, 3)
+
     def _outputs_like_positions(self):
-        linear = _zeros(_last_shape, _last_framework, _last_device)
-        angular = _zeros(_last_shape, _last_framework, _last_device) if self._include_angular else None
+        shape = self._own_shape()
+        linear = _zeros(shape, _last_framework, _last_device)
+        angular = _zeros(shape, _last_framework, _last_device) if self._include_angular else None
         return linear, angular, _mem_of(_last_framework, _last_device)
 
     def sqrtMdotW(self, prefactor=1.0):
@@ -353,11 +375,11 @@ class NBody:
         """pythonify.h:273-294 -> Mobility::LangevinVelocities (MobilityInterface.h:191-206)."""
         f, t, mf, mt, mem, device = self._setup_arrays(forces, torques)
         if f.size == 0:
-            mf = _zeros(_last_shape, _last_framework, _last_device)
+            mf = _zeros(self._own_shape(), _last_framework, _last_device)
             mem = _mem_of(_last_framework, _last_device)
             device = _last_device
         if self._include_angular and t.size == 0:
-            mt = _zeros(_last_shape, _last_framework, _last_device)
+            mt = _zeros(self._own_shape(), _last_framework, _last_device)
         f = f.moved_to(mem, device)
         t = t.moved_to(mem, device)
         self._bind_stream(mem, device)

@@ -125,3 +125,18 @@ def test_python_errors_before_device_work():
     with pytest.raises(RuntimeError, match="Invalid algorithm"):
         s.setParameters("fmm")
     assert NBody.precision == "double"
+
+
+def test_outputs_are_sized_by_the_solvers_own_particle_count(monkeypatch):
+    """sqrtMdotW/divM outputs follow the module-global last positions shape in
+    form ((N,3) or (3N,), pythonify.h:34-36) but always hold this solver's 3N
+    values: positions set on another solver since must not size them (the
+    library writes 3N entries)."""
+    from libmobility_amd import nbody
+    s = nbody.NBody("open", "open", "open")
+    monkeypatch.setattr(s, "getNumberParticles", lambda: 5)
+    monkeypatch.setattr(nbody, "_last_shape", (7, 3))
+    lin, ang, _ = s._outputs_like_positions()
+    assert lin.shape == (5, 3) and ang is None
+    monkeypatch.setattr(nbody, "_last_shape", (21,))
+    assert s._outputs_like_positions()[0].shape == (15,)

@@ -87,3 +87,96 @@ def test_ensemble_sqrt_and_drift():
             assert np.any(d[a:b] != 0)
         else:
             assert np.all(d[a:b] == 0)
+
+
+def _single(pos, kind, angular, tol):
+    from libmobility_amd import NBody
+    s = NBody("open", "open", "single_wall" if kind else "open")
+    s.setParameters(wallHeight=0.0) if kind else s.setParameters()
+    s.initialize(viscosity=0.9, hydrodynamicRadius=1.1, includeAngular=angular, tolerance=tol, seed=3)
+    s.setPositions(pos)
+    return s
+
+
+def _split(v, off, N, angular):
+    """Per-system vectors [forces_s, torques_s] of a global [forces, torques] vector."""
+    v = np.asarray(v).ravel()
+    out = []
+    for a, b in zip(off[:-1], off[1:]):
+        part = v[3 * a:3 * b]
+        if angular:
+            part = np.concatenate([part, v[3 * N + 3 * a:3 * N + 3 * b]])
+        out.append(part)
+    return out
+
+
+@pytest.mark.parametrize("angular", [False, True])
+def test_per_system_lanczos_matches_separate_solvers(angular):
+    """One recurrence per system (LanczosAlgorithm.cu:171-232 on each diagonal
+    block) against separate per-system solvers and the oracle's restatement,
+    over three calls so the per-system check schedule and previous estimate
+    carry over as in separate solvers."""
+    sizes, kinds, pos = _ensemble(seed=11, sizes=(40, 25, 60, 33, 7))
+    N = pos.shape[0]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    tol = 1e-6
+    ens = _solver(sizes, kinds, pos, angular=angular, tol=tol)
+    ens.setLanczosPerSystem(True)
+    singles = [_single(pos[a:b], k, angular, tol) for a, b, k in zip(off[:-1], off[1:], kinds)]
+    oracles = [oracle.LanczosOracle() for _ in sizes]
+    Ms = [oracle.dense_matrix(pos[a:b], 0.9, 1.1, angular=angular, wall=bool(k))
+          for a, b, k in zip(off[:-1], off[1:], kinds)]
+    n = (6 if angular else 3) * N
+    rng = np.random.default_rng(4)
+    for call in range(3):
+        W = rng.normal(size=n)
+        u, w = ens.sqrtMdotW_with_noise(W)
+        got = _split(np.concatenate([u.ravel(), w.ravel()]) if angular else u, off, N, angular)
+        its = ens.lastLanczosIterationsPerSystem()
+        assert its.shape == (len(sizes),)
+        for s, (Ws, g) in enumerate(zip(_split(W, off, N, angular), got)):
+            us, ws = singles[s].sqrtMdotW_with_noise(Ws)
+            one = np.concatenate([us.ravel(), ws.ravel()]) if angular else us.ravel()
+            ref, it_ref = oracles[s].run(lambda v, M=Ms[s]: M @ v, Ws, tol)
+            # the stopping rule reacts to rounding (see test_gpu_lanczos.py)
+            assert abs(its[s] - singles[s].lastLanczosIterations()) <= 6, (call, s)
+            assert abs(its[s] - it_ref) <= 6, (call, s)
+            assert np.linalg.norm(g - one) <= 10 * tol * np.linalg.norm(one), (call, s)
+            exact = oracle.dense_sqrt(Ms[s]) @ Ws
+            err_ref = np.linalg.norm(ref - exact)
+            assert np.linalg.norm(g - exact) <= 3 * err_ref + 5 * tol * np.linalg.norm(exact), (call, s)
+    assert ens.lastLanczosIterations() == max(its)
+
+
+def test_per_system_lanczos_with_preconditioner_is_a_square_root():
+    from libmobility_amd.precond import LearnedPreconditioner
+    sizes, kinds, pos = _ensemble(seed=12, sizes=(20, 14, 9))
+    s = _solver(sizes, kinds, pos, tol=1e-9)
+    s.setLanczosPerSystem(True)
+    s.setPreconditioner(LearnedPreconditioner(seed=9))
+    n = 3 * pos.shape[0]
+    B = np.zeros((n, n))
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1.0
+        B[:, k] = s.sqrtMdotW_with_noise(e)[0].ravel()
+    M = oracle.dense_matrix(pos, 0.9, 1.1, wall=kinds, sys_sizes=sizes)
+    assert np.linalg.norm(B @ B.T - M) <= 1e-5 * np.linalg.norm(M)
+
+
+def test_per_system_lanczos_limit_and_callback():
+    sizes, kinds, pos = _ensemble(seed=13, sizes=(30, 12))
+    from libmobility_amd import NBody
+    seen = []
+    s = NBody("open", "open", "single_wall")
+    s.setParameters(wallHeight=0.0)
+    s.initialize(viscosity=0.9, hydrodynamicRadius=1.1, tolerance=-1.0,
+                 lanczos_callback=lambda i, e: seen.append((i, e)))
+    s.setSystems(sizes, ["single_wall" if k else "open" for k in kinds])
+    s.setLanczosPerSystem(True)
+    s.setLanczosIterationLimit(40)
+    s.setPositions(pos)
+    with pytest.raises(RuntimeError, match="Could not converge"):
+        s.sqrtMdotW_with_noise(np.random.default_rng(0).normal(size=3 * pos.shape[0]))
+    assert [i for i, _ in seen] == list(range(1, 40))
+    assert all(np.isfinite(e) for _, e in seen)
