@@ -178,10 +178,18 @@ def result_line(metric, value, unit, steps, warmup, ms, higher, dtype, config, r
     }
 
 
-def compute_roofline(flops_per_launch, ms, peak_tflops):
+# HBM bytes per FP32 k_pair launch, from rocprofv3 --pmc FETCH_SIZE (x 2.00) and
+# WRITE_SIZE (x 1.00); the scales come from a 2 GiB device copy.  Keyed by
+# (workload, N, targets per lane, source chunks) of the profiled configuration.
+MEASURED_TRAFFIC_BYTES = {
+    ("c3", 131072, 2, 32): (10454.1 * 2.0 + 98304.0) * 1024.0,
+}
+
+
+def compute_roofline(flops_per_launch, ms, peak_tflops, traffic=None):
     achieved = flops_per_launch / (ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": peak_tflops, "unit": "TFLOP/s",
-            "frac": achieved / peak_tflops, "traffic": None}
+            "frac": achieved / peak_tflops, "traffic": traffic}
 
 
 # ---------------------------------------------------------------------------
@@ -323,8 +331,11 @@ def bench_c3(args):
         "N": N,
         "pairs_per_step": pairs,
     }
+    t32 = extra["tiling_fp32pair"]
+    traffic = MEASURED_TRAFFIC_BYTES.get(("c3", N, t32["targets_per_lane"], t32["source_chunks"]))
     res = result_line(METRIC, gpair32, "Gpair/s", args.steps, args.warmup, ms32, True, "f32",
-                      config, compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS), extra, None)
+                      config, compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS, traffic),
+                      extra, None)
     if not args.no_cpu:
         log("CPU baseline (oracle, OpenMP FP64)")
         cpu, Uo = cpu_baseline_rows(pos, F)

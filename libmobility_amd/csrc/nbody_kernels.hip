@@ -84,20 +84,60 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
       if (kNeedT) s_t[tid] = R4{R(0), R(0), R(0), R(0)};
     }
     __syncthreads();
+    if constexpr (WALL) {
 #pragma unroll (TPL >= 4 ? 1 : 4 / TPL)
-    for (int s = 0; s < kPairThreads; ++s) {
-      const R4 pj = s_pos[s];
-      R4 fj, tj;
-      if (kNeedF) fj = s_f[s]; else fj = R4{R(0), R(0), R(0), R(0)};
-      if (kNeedT) tj = s_t[s]; else tj = R4{R(0), R(0), R(0), R(0)};
+      for (int s = 0; s < kPairThreads; ++s) {
+        const R4 pj = s_pos[s];
+        R4 fj, tj;
+        if (kNeedF) fj = s_f[s]; else fj = R4{R(0), R(0), R(0), R(0)};
+        if (kNeedT) tj = s_t[s]; else tj = R4{R(0), R(0), R(0), R(0)};
 #pragma unroll
-      for (int k = 0; k < TPL; ++k) {
-        if (WALL)
+        for (int k = 0; k < TPL; ++k)
           wall_pair<R, BLK>(pi[k].x, pi[k].y, pi[k].z, pi[k].w, pj.x, pj.y, pj.z, pj.w, fj.x,
                             fj.y, fj.z, tj.x, tj.y, tj.z, acc[k]);
-        else
-          open_pair<R, BLK>(pi[k].x - pj.x, pi[k].y - pj.y, pi[k].z - pj.z, fj.x, fj.y, fj.z,
-                            tj.x, tj.y, tj.z, acc[k]);
+      }
+    } else {
+      // Open boundaries: a group of SG sources x TPL targets (4 pairs) gets
+      // its far-field coefficients first and one overlap test for the whole
+      // group; the overlap values (rare: overlapping spheres and the self
+      // pair) are patched in by selects inside that single branch.
+      constexpr int SG = TPL >= 4 ? 1 : 4 / TPL;
+      for (int s = 0; s < kPairThreads; s += SG) {
+        R dx[SG][TPL], dy[SG][TPL], dz[SG][TPL], r2[SG][TPL], ir[SG][TPL];
+        RpyCoef<R> cf[SG][TPL];
+        R4 fj[SG], tj[SG];
+        bool near = false;
+#pragma unroll
+        for (int g = 0; g < SG; ++g) {
+          const R4 pj = s_pos[s + g];
+          if (kNeedF) fj[g] = s_f[s + g]; else fj[g] = R4{R(0), R(0), R(0), R(0)};
+          if (kNeedT) tj[g] = s_t[s + g]; else tj[g] = R4{R(0), R(0), R(0), R(0)};
+#pragma unroll
+          for (int k = 0; k < TPL; ++k) {
+            dx[g][k] = pi[k].x - pj.x;
+            dy[g][k] = pi[k].y - pj.y;
+            dz[g][k] = pi[k].z - pj.z;
+            r2[g][k] = dx[g][k] * dx[g][k] + dy[g][k] * dy[g][k] + dz[g][k] * dz[g][k];
+            ir[g][k] = lm_rsqrt(r2[g][k]);
+            cf[g][k] = RpyCoef<R>{};
+            rpy_far<R, BLK>(ir[g][k], cf[g][k]);
+            near |= r2[g][k] < R(4);
+          }
+        }
+        if (__builtin_expect(near, 0)) {
+          // opaque statement: keep this a branch (not speculated onto every group)
+          __asm__ __volatile__("");
+#pragma unroll
+          for (int g = 0; g < SG; ++g)
+#pragma unroll
+            for (int k = 0; k < TPL; ++k) rpy_near_select<R, BLK>(r2[g][k], ir[g][k], cf[g][k]);
+        }
+#pragma unroll
+        for (int g = 0; g < SG; ++g)
+#pragma unroll
+          for (int k = 0; k < TPL; ++k)
+            open_accumulate<R, BLK>(dx[g][k], dy[g][k], dz[g][k], fj[g].x, fj[g].y, fj[g].z,
+                                    tj[g].x, tj[g].y, tj[g].z, cf[g][k], acc[k]);
       }
     }
     if constexpr (std::is_same<R, float>::value) {

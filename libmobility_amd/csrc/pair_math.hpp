@@ -86,49 +86,88 @@ __host__ __device__ inline double lm_sqrt(double x) { return __builtin_sqrt(x); 
 // :73-84, :97-100.  It is a divergent branch so that waves with no near pair
 // skip it.  For r = 0 the far-field values are infinite; they are overwritten
 // (never combined) by the overlap branch.
+template <class R>
+struct RpyCoef {
+  R fuf, guf, fwt, gwt, c;
+};
+
+// Far-field (r >= 2) coefficients from ir = 1/r.
 template <class R, int BLK>
-__host__ __device__ inline void rpy_coeffs(R r2, R& fuf, R& guf, R& fwt, R& gwt, R& c) {
-  const R ir = lm_rsqrt(r2);
+__host__ __device__ inline void rpy_far(R ir, RpyCoef<R>& k) {
   const R ir2 = ir * ir;
   const R ir3 = ir * ir2;
   if (BLK & kUF) {
-    fuf = ir * (R(0.75) + R(0.5) * ir2);
-    guf = ir3 * (R(0.75) - R(1.5) * ir2);
+    k.fuf = ir * (R(0.75) + R(0.5) * ir2);
+    k.guf = ir3 * (R(0.75) - R(1.5) * ir2);
   }
   if (BLK & kWT) {
-    fwt = R(-0.5) * ir3;
-    gwt = R(1.5) * ir3 * ir2;
+    k.fwt = R(-0.5) * ir3;
+    k.gwt = R(1.5) * ir3 * ir2;
   }
-  if (BLK & (kUT | kWF)) c = ir3;
+  if (BLK & (kUT | kWF)) k.c = ir3;
+}
+
+// Overlap (r < 2, including the self pair r = 0) coefficients; r and 1/r come
+// from the rsqrt already computed (no sqrt, no division).
+template <class R, int BLK>
+__host__ __device__ inline void rpy_near(R r2, R ir, RpyCoef<R>& k) {
+  const R r = r2 > R(0) ? r2 * ir : R(0);
+  const R ir_n = r2 > R(0) ? ir : R(0);
+  if (BLK & kUF) {
+    k.fuf = R(1) - R(0.28125) * r;
+    k.guf = R(0.09375) * ir_n;
+  }
+  if (BLK & kWT) {
+    k.fwt = R(1) - R(0.84375) * r + R(0.078125) * r * r * r;
+    k.gwt = r > R(0) ? (R(0.28125) * ir_n - R(0.046875) * r) : R(0);
+  }
+  if (BLK & (kUT | kWF)) k.c = R(0.5) * (R(1) - R(0.375) * r);
+}
+
+// Replace k by the overlap coefficients when r2 < 4, by selects (no branch).
+template <class R, int BLK>
+__host__ __device__ inline void rpy_near_select(R r2, R ir, RpyCoef<R>& k) {
+  RpyCoef<R> n = k;
+  rpy_near<R, BLK>(r2, ir, n);
+  const bool is_near = r2 < R(4);
+  if (BLK & kUF) {
+    k.fuf = is_near ? n.fuf : k.fuf;
+    k.guf = is_near ? n.guf : k.guf;
+  }
+  if (BLK & kWT) {
+    k.fwt = is_near ? n.fwt : k.fwt;
+    k.gwt = is_near ? n.gwt : k.gwt;
+  }
+  if (BLK & (kUT | kWF)) k.c = is_near ? n.c : k.c;
+}
+
+template <class R, int BLK>
+__host__ __device__ inline void rpy_coeffs(R r2, R& fuf, R& guf, R& fwt, R& gwt, R& c) {
+  const R ir = lm_rsqrt(r2);
+  RpyCoef<R> k{};
+  rpy_far<R, BLK>(ir, k);
   if (__builtin_expect(r2 < R(4), 0)) {
 #if defined(__HIP_DEVICE_COMPILE__)
     // An opaque statement keeps this a branch: speculating it (if-conversion)
     // would put the overlap arithmetic on every pair.
     __asm__ __volatile__("");
 #endif
-    // r and 1/r from the rsqrt already computed (no sqrt, no division)
-    const R r = r2 > R(0) ? r2 * ir : R(0);
-    const R ir_n = r2 > R(0) ? ir : R(0);
-    if (BLK & kUF) {
-      fuf = R(1) - R(0.28125) * r;
-      guf = R(0.09375) * ir_n;
-    }
-    if (BLK & kWT) {
-      fwt = R(1) - R(0.84375) * r + R(0.078125) * r * r * r;
-      gwt = r > R(0) ? (R(0.28125) * ir_n - R(0.046875) * r) : R(0);
-    }
-    if (BLK & (kUT | kWF)) c = R(0.5) * (R(1) - R(0.375) * r);
+    rpy_near<R, BLK>(r2, ir, k);
   }
+  fuf = k.fuf;
+  guf = k.guf;
+  fwt = k.fwt;
+  gwt = k.gwt;
+  c = k.c;
 }
 
 // Open-boundary contribution of one source (f = force, t = torque) to one
-// target, added into a (dimensionless; the caller applies t0, rt0, r0).
+// target with given coefficients, added into a (dimensionless; the caller
+// applies t0, rt0, r0).
 template <class R, int BLK>
-__host__ __device__ inline void open_pair(R dx, R dy, R dz, R fx, R fy, R fz, R tx, R ty, R tz,
-                                          PairAcc<R>& a) {
-  const R r2 = dx * dx + dy * dy + dz * dz;
-  R fuf, guf, fwt, gwt, c;
-  rpy_coeffs<R, BLK>(r2, fuf, guf, fwt, gwt, c);
+__host__ __device__ inline void open_accumulate(R dx, R dy, R dz, R fx, R fy, R fz, R tx, R ty,
+                                                R tz, const RpyCoef<R>& k, PairAcc<R>& a) {
+  const R fuf = k.fuf, guf = k.guf, fwt = k.fwt, gwt = k.gwt, c = k.c;
   // Accumulate as two FMAs per component: acc = fma(f, v, acc); acc = fma(s, d, acc).
   if (BLK & kUF) {
     const R s = guf * (dx * fx + dy * fy + dz * fz);
@@ -152,6 +191,15 @@ __host__ __device__ inline void open_pair(R dx, R dy, R dz, R fx, R fy, R fz, R 
     a.wf.y = lm_fma(c, fz * dx - fx * dz, a.wf.y);
     a.wf.z = lm_fma(c, fx * dy - fy * dx, a.wf.z);
   }
+}
+
+template <class R, int BLK>
+__host__ __device__ inline void open_pair(R dx, R dy, R dz, R fx, R fy, R fz, R tx, R ty, R tz,
+                                          PairAcc<R>& a) {
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  RpyCoef<R> k{};
+  rpy_coeffs<R, BLK>(r2, k.fuf, k.guf, k.fwt, k.gwt, k.c);
+  open_accumulate<R, BLK>(dx, dy, dz, fx, fy, fz, tx, ty, tz, k, a);
 }
 
 // Bottom-wall contribution (open RPY + Rotne-Prager-Blake/Swan-Brady image
