@@ -82,11 +82,23 @@ SIGNATURES = {
 _lib = None
 
 
+def _share_torch_hip_runtime():
+    """PyTorch-ROCm wheels carry their own libamdhip64 (same SONAME as the
+    system one).  Loading torch first lets this library bind to that runtime;
+    loading the system runtime first would put a second HIP runtime in the
+    process, after which torch finds no GPU."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load():
     """Load the HIP library (raises ImportError when it has not been built)."""
     global _lib
     if _lib is not None:
         return _lib
+    _share_torch_hip_runtime()
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"libmobility_amd: {LIB_PATH} is missing; build it with "
