@@ -258,6 +258,51 @@ def bbt_errors(g, net, N=512, phi=0.20):
     return out
 
 
+def multi_vector_leg(g, h, F_d, W_d, N, ms_single, extra, args, nrhs=4):
+    """Several vectors per pass: the FP32 product of nrhs force vectors and
+    nrhs square roots at tol 1e-6 sharing each pass, against the single-vector
+    figures of the same run."""
+    n3 = 3 * N
+    F4 = F_d.repeat(nrhs).view(nrhs, n3).clone()
+    F4 += 0.01 * g.torch.arange(nrhs, dtype=g.torch.float64, device=g.dev).view(nrhs, 1)
+    U4 = g.zeros(nrhs * n3)
+    log(f"{nrhs}-vector FP32 product")
+
+    def mdot4():
+        g.ffi.check(g.lib.lm_nbody_mdot_multi(h, F4.data_ptr(), nrhs, U4.data_ptr(),
+                                              g.ffi.LM_MEM_DEVICE))
+
+    ms4 = g.time_events(mdot4, max(3, args.steps // 2), 1)
+    out = {
+        "mdot_multi4_fp32pair_ms": ms4,
+        "mdot_multi4_fp32pair_gpair_vectors_s": nrhs * float(N) ** 2 / (ms4 * 1e-3) / 1e9,
+        "mdot_multi4_speedup_vs_single": nrhs * ms_single / ms4,
+    }
+    log(f"  {ms4:.3f} ms for {nrhs} vectors ({nrhs * ms_single / ms4:.2f}x the single-vector rate)")
+    if not args.no_lanczos and "sqrtMdotW_plain_fp32pair_ms" in extra:
+        W4 = W_d.repeat(nrhs).view(nrhs, n3).clone()
+        W4[1:] = g.torch.randn(nrhs - 1, n3, dtype=g.torch.float64, device=g.dev,
+                               generator=g.torch.Generator(device=g.dev).manual_seed(3))
+        Y4 = g.zeros(nrhs * n3)
+        times = []
+        for c in range(2):
+            g.sync()
+            t = time.perf_counter()
+            g.ffi.check(g.lib.lm_nbody_sqrt_mdot_w_multi(h, W4.data_ptr(), nrhs, Y4.data_ptr(), 1.0,
+                                                         g.ffi.LM_MEM_DEVICE))
+            g.sync()
+            times.append((time.perf_counter() - t) * 1e3)
+            log(f"  {nrhs}-vector sqrtMdotW call {c}: {times[-1]:.1f} ms")
+        its = np.zeros(nrhs, dtype=np.int32)
+        g.ffi.check(g.lib.lm_nbody_last_lanczos_iterations_multi(h, its.ctypes.data, nrhs))
+        out.update({
+            "sqrtMdotW_multi4_fp32pair_ms": times[-1],
+            "sqrtMdotW_multi4_iterations": its.tolist(),
+            "sqrtMdotW_multi4_speedup_vs_4_single": nrhs * extra["sqrtMdotW_plain_fp32pair_ms"] / times[-1],
+        })
+    return out
+
+
 def bench_c3(args):
     from libmobility_amd.workloads import rsa_cube
     g = Gpu()
@@ -322,6 +367,8 @@ def bench_c3(args):
         if not args.no_bbt:
             log("square-root check at small N (columns of B from unit vectors)")
             extra["bbt_rel_err_small_N"] = bbt_errors(g, net)
+    if not args.no_multi:
+        extra.update(multi_vector_leg(g, h32, F_d, W_d, N, ms32, extra, args))
     g.destroy(h32, h64)
 
     config = {
@@ -759,6 +806,8 @@ def main():
                     help="c2: target rows of the CPU baseline / checker")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--no-multi", action="store_true",
+                    help="c3: skip the several-vectors-per-pass leg")
     ap.add_argument("--no-bbt", action="store_true",
                     help="c3: skip the small-N ||BB^T - M|| check (thousands of small solves)")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -134,6 +134,25 @@ int lm_nbody_set_lanczos_iteration_limit(lm_nbody* h, int limit);
  * step.  A system whose noise is exactly zero gets a zero result without
  * iterating. */
 int lm_nbody_set_lanczos_per_system(lm_nbody* h, int on);
+/* Several force vectors in one pass over the pairs (no single counterpart in
+ * the reference; it equals nrhs calls of NBody::Mdot, mobility.h:166-182, for
+ * the translational block): linear_r = M forces_r, r < nrhs.  forces and
+ * linear hold nrhs consecutive 3N vectors.  The pair geometry and
+ * coefficients are computed once per pair for all vectors.  Solvers
+ * initialised with includeAngular are refused. */
+int lm_nbody_mdot_multi(lm_nbody* h, const double* forces, int nrhs, double* linear, int mem);
+/* nrhs independent square roots, one lanczos::Solver each
+ * (LanczosAlgorithm.cu:171-232; one per (vector, system) in per-system mode),
+ * sharing every pass over the pairs: linear_r += prefactor M^{1/2} W_r
+ * (Mobility::sqrtMdotW, MobilityInterface.h:132-188).  noise: nrhs
+ * consecutive 3N vectors, or NULL to draw the W_r that nrhs successive
+ * sqrtMdotW calls would draw.  Translational solvers only. */
+int lm_nbody_sqrt_mdot_w_multi(lm_nbody* h, const double* noise, int nrhs, double* linear,
+                               double prefactor, int mem);
+/* Iterations of each vector of the last multi-vector solve (largest over its
+ * systems in per-system mode). */
+int lm_nbody_last_lanczos_iterations_multi(lm_nbody* h, int* out, int nrhs);
+
 /* Number of independent systems of the current positions (Nbatch of
  * mobility.h:149-160, or the ragged systems of lm_nbody_set_systems). */
 int lm_nbody_num_systems(lm_nbody* h);

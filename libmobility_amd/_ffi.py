@@ -48,6 +48,10 @@ SIGNATURES = {
     "lm_nbody_set_lanczos_per_system": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lm_nbody_last_lanczos_iterations_per_system": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
     "lm_nbody_num_systems": (ctypes.c_int, [_vp]),
+    "lm_nbody_mdot_multi": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int]),
+    "lm_nbody_sqrt_mdot_w_multi": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_double,
+                                                  ctypes.c_int]),
+    "lm_nbody_last_lanczos_iterations_multi": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
     "lm_nbody_num_particles": (_i64, [_vp]),
     "lm_nbody_mdot": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, ctypes.c_int]),
     "lm_nbody_sqrt_mdot_w": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),

@@ -134,6 +134,22 @@ int lm_nbody_last_lanczos_iterations_per_system(lm_nbody* h, int* out, int nsys)
   });
 }
 
+int lm_nbody_mdot_multi(lm_nbody* h, const double* forces, int nrhs, double* linear, int mem) {
+  return guarded([&] { S(h).MdotMulti(forces, nrhs, linear, is_dev(mem)); });
+}
+
+int lm_nbody_sqrt_mdot_w_multi(lm_nbody* h, const double* noise, int nrhs, double* linear,
+                               double prefactor, int mem) {
+  return guarded([&] { S(h).sqrtMdotWMulti(linear, nrhs, prefactor, is_dev(mem), noise); });
+}
+
+int lm_nbody_last_lanczos_iterations_multi(lm_nbody* h, int* out, int nrhs) {
+  return guarded([&] {
+    if (!out || nrhs < 1) throw lm::Error("[libmobility_amd] invalid output for iteration counts");
+    S(h).lastLanczosIterationsMulti(out, nrhs);
+  });
+}
+
 int lm_nbody_num_systems(lm_nbody* h) { return h ? h->solver.numSystems() : -1; }
 
 int64_t lm_nbody_num_particles(lm_nbody* h) { return h ? h->solver.numParticles() : -1; }

@@ -40,6 +40,25 @@ struct PairArgs {
 constexpr int kPairThreads = 256;  // threads per workgroup = sources per LDS tile
 
 void launch_pair(const PairArgs& a, hipStream_t s);
+
+// Translational product with several force vectors in one pass over the pairs:
+// U_r = t0 * M F_r, r < nrhs.  F holds nrhs consecutive 3N vectors; the
+// partial of vector r and chunk c goes to U + (r * nch + c) * n3.
+struct PairMultiArgs {
+  int kind;   // 0 open RPY, 1 bottom wall, 2 per item
+  int fp32;   // 1: FP32 pair terms, FP64 per-tile accumulation
+  int nrhs;   // vectors per launch: 2 (two targets per lane) or 4 (one)
+  const void* pos;
+  const double* F;
+  double* U;
+  int64_t n3;
+  int nch;
+  const PairItem* items;
+  int nitems;
+  double t0;
+};
+constexpr int multi_tpl(int nrhs) { return nrhs >= 4 ? 1 : 2; }
+void launch_pair_multi(const PairMultiArgs& a, hipStream_t s);
 void launch_chunk_reduce(const double* part, int nchunks, int64_t n, double* out, hipStream_t s);
 
 // setPositions: per-system bounding boxes (6 doubles each: min xyz, max xyz)

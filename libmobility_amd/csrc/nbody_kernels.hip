@@ -217,6 +217,154 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(
   }
 }
 
+// ---- several force vectors per pass (translational block) ----------------------
+// Same tiling as pair_item; each pair's geometry and coefficients are computed
+// once and applied to the NR force vectors staged side by side in LDS.
+template <class R, bool WALL, int NR, int TPL>
+__device__ inline void pair_item_multi(const PairItem& it, const typename Rec<R>::T* __restrict__ pos,
+                                       const double* __restrict__ F, double* __restrict__ U,
+                                       int64_t n3, int nch, double t0, typename Rec<R>::T* s_pos,
+                                       typename Rec<R>::T* s_f) {
+  using R4 = typename Rec<R>::T;
+  const int tid = threadIdx.x;
+  R4 pi[TPL];
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+    const int loc = tid + k * kPairThreads;
+    pi[k] = (loc < it.ntgt) ? pos[it.tgt0 + loc] : far_record<R4>();
+  }
+  Vec3<R> acc[TPL][NR];
+  Vec3<double> acc64[TPL][NR];
+#pragma unroll
+  for (int k = 0; k < TPL; ++k)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      acc[k][r] = {R(0), R(0), R(0)};
+      acc64[k][r] = {0.0, 0.0, 0.0};
+    }
+  for (int64_t base = it.src0; base < it.src1; base += kPairThreads) {
+    const int64_t j = base + tid;
+    __syncthreads();
+    if (j < it.src1) {
+      s_pos[tid] = pos[j];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const double* f = F + (int64_t)r * n3 + 3 * j;
+        s_f[r * kPairThreads + tid] = R4{R(f[0]), R(f[1]), R(f[2]), R(0)};
+      }
+    } else {
+      s_pos[tid] = far_record<R4>();
+#pragma unroll
+      for (int r = 0; r < NR; ++r) s_f[r * kPairThreads + tid] = R4{R(0), R(0), R(0), R(0)};
+    }
+    __syncthreads();
+    for (int s = 0; s < kPairThreads; ++s) {
+      const R4 pj = s_pos[s];
+      R4 fj[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) fj[r] = s_f[r * kPairThreads + s];
+#pragma unroll
+      for (int k = 0; k < TPL; ++k) {
+        UFCoef<R> c;
+        if constexpr (WALL) {
+          uf_coef_wall<R>(pi[k].x, pi[k].y, pi[k].z, pi[k].w, pj.x, pj.y, pj.z, pj.w, c);
+#pragma unroll
+          for (int r = 0; r < NR; ++r) uf_apply_wall<R>(c, fj[r].x, fj[r].y, fj[r].z, acc[k][r]);
+        } else {
+          c.dx = pi[k].x - pj.x;
+          c.dy = pi[k].y - pj.y;
+          c.dz = pi[k].z - pj.z;
+          const R r2 = c.dx * c.dx + c.dy * c.dy + c.dz * c.dz;
+          R fwt, gwt, cc;
+          rpy_coeffs<R, kUF>(r2, c.a0, c.cd, fwt, gwt, cc);
+#pragma unroll
+          for (int r = 0; r < NR; ++r) uf_apply_open<R>(c, fj[r].x, fj[r].y, fj[r].z, acc[k][r]);
+        }
+      }
+    }
+    if constexpr (std::is_same<R, float>::value) {
+#pragma unroll
+      for (int k = 0; k < TPL; ++k)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          acc64[k][r].x += acc[k][r].x;
+          acc64[k][r].y += acc[k][r].y;
+          acc64[k][r].z += acc[k][r].z;
+          acc[k][r] = {R(0), R(0), R(0)};
+        }
+    }
+  }
+  if constexpr (std::is_same<R, double>::value) {
+#pragma unroll
+    for (int k = 0; k < TPL; ++k)
+#pragma unroll
+      for (int r = 0; r < NR; ++r) acc64[k][r] = acc[k][r];
+  }
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+    const int loc = tid + k * kPairThreads;
+    if (loc < it.ntgt) {
+      const int64_t i = it.tgt0 + loc;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        double* o = U + ((int64_t)r * nch + it.chunk) * n3;
+        o[3 * i + 0] = t0 * acc64[k][r].x;
+        o[3 * i + 1] = t0 * acc64[k][r].y;
+        o[3 * i + 2] = t0 * acc64[k][r].z;
+      }
+    }
+  }
+}
+
+template <class R, int KIND, int NR>
+__global__ __launch_bounds__(kPairThreads) void k_pair_multi(
+    const PairItem* __restrict__ items, const typename Rec<R>::T* __restrict__ pos,
+    const double* __restrict__ F, double* __restrict__ U, int64_t n3, int nch, double t0) {
+  using R4 = typename Rec<R>::T;
+  constexpr int TPL = multi_tpl(NR);
+  __shared__ R4 s_pos[kPairThreads];
+  __shared__ R4 s_f[NR * kPairThreads];
+  const PairItem it = items[blockIdx.x];
+  if constexpr (KIND == 0) {
+    pair_item_multi<R, false, NR, TPL>(it, pos, F, U, n3, nch, t0, s_pos, s_f);
+  } else if constexpr (KIND == 1) {
+    pair_item_multi<R, true, NR, TPL>(it, pos, F, U, n3, nch, t0, s_pos, s_f);
+  } else {
+    if (it.wall)
+      pair_item_multi<R, true, NR, TPL>(it, pos, F, U, n3, nch, t0, s_pos, s_f);
+    else
+      pair_item_multi<R, false, NR, TPL>(it, pos, F, U, n3, nch, t0, s_pos, s_f);
+  }
+}
+
+template <class R, int KIND>
+static void launch_pair_multi_k(const PairMultiArgs& a, hipStream_t s) {
+  using R4 = typename Rec<R>::T;
+  const R4* pos = static_cast<const R4*>(a.pos);
+  if (a.nrhs == 4)
+    hipLaunchKernelGGL((k_pair_multi<R, KIND, 4>), dim3(a.nitems), dim3(kPairThreads), 0, s,
+                       a.items, pos, a.F, a.U, a.n3, a.nch, a.t0);
+  else if (a.nrhs == 2)
+    hipLaunchKernelGGL((k_pair_multi<R, KIND, 2>), dim3(a.nitems), dim3(kPairThreads), 0, s,
+                       a.items, pos, a.F, a.U, a.n3, a.nch, a.t0);
+  else
+    throw Error("[libmobility_amd] multi-vector launch handles 2 or 4 vectors");
+  LM_HIP(hipGetLastError());
+}
+
+void launch_pair_multi(const PairMultiArgs& a, hipStream_t s) {
+  if (a.nitems <= 0) return;
+  if (a.fp32) {
+    if (a.kind == 0) launch_pair_multi_k<float, 0>(a, s);
+    else if (a.kind == 1) launch_pair_multi_k<float, 1>(a, s);
+    else launch_pair_multi_k<float, 2>(a, s);
+  } else {
+    if (a.kind == 0) launch_pair_multi_k<double, 0>(a, s);
+    else if (a.kind == 1) launch_pair_multi_k<double, 1>(a, s);
+    else launch_pair_multi_k<double, 2>(a, s);
+  }
+}
+
 template <class R, int KIND, int BLK, int TPL>
 static void launch_pair_t(const PairArgs& a, hipStream_t s) {
   using R4 = typename Rec<R>::T;

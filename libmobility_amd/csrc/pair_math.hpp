@@ -202,6 +202,95 @@ __host__ __device__ inline void open_pair(R dx, R dy, R dz, R fx, R fy, R fz, R 
   open_accumulate<R, BLK>(dx, dy, dz, fx, fy, fz, tx, ty, tz, k, a);
 }
 
+// ---- translational block (UF) split into a per-pair part and a per-vector part
+// for products with several force vectors at once.  For one pair
+//   U += a0 F + cd (d.F) d + (e.F) g + F_z h
+// holds the open RPY part (a0 = b fuf, cd = b guf) and, for the wall, the
+// image terms regrouped as above: a0 += b f1, g = b (f2 e + f4 z^),
+// h = b (f3 e + f5 z^); self pairs (r = 0) have e = g = 0 and
+// a0 = b (fuf + cp), h = (0, 0, b (cz - cp)).  b is the damping b_i b_j.
+template <class R>
+struct UFCoef {
+  R a0, cd, dx, dy, dz, ex, ey, ez, gx, gy, gz, hx, hy, hz;
+};
+
+// Open boundaries: coefficients from d and the RPY coefficients (e, g, h unused).
+template <class R>
+__host__ __device__ inline void uf_apply_open(const UFCoef<R>& k, R fx, R fy, R fz, Vec3<R>& u) {
+  const R s = k.cd * (k.dx * fx + k.dy * fy + k.dz * fz);
+  u.x = lm_fma(s, k.dx, lm_fma(k.a0, fx, u.x));
+  u.y = lm_fma(s, k.dy, lm_fma(k.a0, fy, u.y));
+  u.z = lm_fma(s, k.dz, lm_fma(k.a0, fz, u.z));
+}
+
+template <class R>
+__host__ __device__ inline void uf_apply_wall(const UFCoef<R>& k, R fx, R fy, R fz, Vec3<R>& u) {
+  const R s = k.cd * (k.dx * fx + k.dy * fy + k.dz * fz);
+  const R ev = k.ex * fx + k.ey * fy + k.ez * fz;
+  u.x = lm_fma(fz, k.hx, lm_fma(ev, k.gx, lm_fma(s, k.dx, lm_fma(k.a0, fx, u.x))));
+  u.y = lm_fma(fz, k.hy, lm_fma(ev, k.gy, lm_fma(s, k.dy, lm_fma(k.a0, fy, u.y))));
+  u.z = lm_fma(fz, k.hz, lm_fma(ev, k.gz, lm_fma(s, k.dz, lm_fma(k.a0, fz, u.z))));
+}
+
+// Wall UF coefficients of target record (xi, yi, hi, bi) and source (xj, yj, hj, bj).
+template <class R>
+__host__ __device__ inline void uf_coef_wall(R xi, R yi, R hi, R bi, R xj, R yj, R hj, R bj,
+                                             UFCoef<R>& k) {
+  const R dx = xi - xj, dy = yi - yj, dz = hi - hj;
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  const R b = bi * bj;
+  R fuf, guf, fwt, gwt, c;
+  rpy_coeffs<R, kUF>(r2, fuf, guf, fwt, gwt, c);
+  k.dx = dx;
+  k.dy = dy;
+  k.dz = dz;
+  k.cd = b * guf;
+  if (r2 != R(0)) {
+    const R Rz = hi + hj;
+    const R iR = lm_rsqrt(dx * dx + dy * dy + Rz * Rz);
+    const R iR2 = iR * iR;
+    const R iR3 = iR2 * iR;
+    const R iR5 = iR3 * iR2;
+    const R ex = dx * iR, ey = dy * iR, ez = Rz * iR;
+    const R ez2 = ez * ez;
+    const R hh = hj * lm_rcp(Rz);
+    const R t1 = R(5) * ez2 - R(1);
+    const R hq = hh * (R(1) - hh) * ez2;
+    const R f1 = R(-0.25) * (R(3) * (R(1) + R(2) * hq) * iR + R(2) * (R(1) - R(3) * ez2) * iR3 +
+                             R(2) * t1 * iR5);
+    const R f2 = R(-0.25) * (R(3) * (R(1) - R(6) * hq) * iR + R(6) * t1 * iR3 +
+                             R(10) * (R(1) - R(7) * ez2) * iR5);
+    const R f3 = R(0.5) * ez *
+                 (R(3) * hh * (R(1) - R(6) * (R(1) - hh) * ez2) * iR + R(6) * t1 * iR3 +
+                  R(10) * (R(2) - R(7) * ez2) * iR5);
+    const R f4 = R(0.5) * ez * (R(3) * hh * iR - R(10) * iR5);
+    const R f5 = -(R(3) * hh * hh * ez2 * iR + R(3) * ez2 * iR3 + (R(2) - R(15) * ez2) * iR5);
+    k.a0 = b * (fuf + f1);
+    k.ex = ex;
+    k.ey = ey;
+    k.ez = ez;
+    k.gx = b * f2 * ex;
+    k.gy = b * f2 * ey;
+    k.gz = b * (f2 * ez + f4);
+    k.hx = b * f3 * ex;
+    k.hy = b * f3 * ey;
+    k.hz = b * (f3 * ez + f5);
+  } else {
+    // self (Swan & Brady B1), as in wall_pair
+    const R ih = R(1) / hj;
+    const R ih2 = ih * ih;
+    const R ih3 = ih2 * ih;
+    const R ih5 = ih3 * ih2;
+    const R cp = R(-0.0625) * (R(9) * ih - R(2) * ih3 + ih5);
+    const R cz = R(-0.125) * (R(9) * ih - R(4) * ih3 + ih5);
+    k.a0 = b * (fuf + cp);
+    k.ex = k.ey = k.ez = R(0);
+    k.gx = k.gy = k.gz = R(0);
+    k.hx = k.hy = R(0);
+    k.hz = b * (cz - cp);
+  }
+}
+
 // Bottom-wall contribution (open RPY + Rotne-Prager-Blake/Swan-Brady image
 // terms, damped by b_i b_j).  (xi, yi, hi, bi) and (xj, yj, hj, bj) are
 // position records as described at the top of this file.

@@ -212,7 +212,10 @@ void NBodySolver::setPositions(const double* pos, int64_t n3, bool dev) {
 }
 
 // Work decomposition: target tiles of 256*tpl particles x source chunks.
-void NBodySolver::ensure_items(int tpl, int nch_hint) {
+// Source chunks per target block: the hint (auto: about 8192 items, ~32
+// workgroups per CU for a short tail on every XCD), clamped to [1, 32] and to
+// one chunk per 256 sources of the largest system.
+int NBodySolver::resolve_nch(int tpl, int nch_hint) const {
   const std::vector<int64_t>& off = sys_off_host_;
   const int nsys = (int)off.size() - 1;
   const int64_t bt = (int64_t)kPairThreads * tpl;
@@ -224,16 +227,19 @@ void NBodySolver::ensure_items(int tpl, int nch_hint) {
   }
   int nch = nch_hint;
   if (nch <= 0) {
-    const int64_t want = 8192;  // ~32 workgroups per CU: short tail, balanced XCDs
+    const int64_t want = 8192;
     nch = (int)std::max<int64_t>(1, (want + std::max<int64_t>(tiles, 1) - 1) / std::max<int64_t>(tiles, 1));
   }
   const int maxch = (int)std::max<int64_t>(1, (maxn + kPairThreads - 1) / kPairThreads);
-  nch = std::max(1, std::min({nch, 32, maxch}));
-  if (items_tpl_ == tpl && items_nch_ == nch && items_N_ == N_ && items_layout_ == layout_id_ &&
-      items_persist_ == use_persistent())
-    return;
-  std::vector<PairItem> items;
-  items.reserve((size_t)tiles * nch);
+  return std::max(1, std::min({nch, 32, maxch}));
+}
+
+// Work items: per system, target blocks of 256 * tpl targets x nch source chunks.
+void NBodySolver::build_items(int tpl, int nch, bool persist, std::vector<PairItem>& items) const {
+  const std::vector<int64_t>& off = sys_off_host_;
+  const int nsys = (int)off.size() - 1;
+  const int64_t bt = (int64_t)kPairThreads * tpl;
+  items.clear();
   for (int s = 0; s < nsys; ++s) {
     const int64_t b = off[s], e = off[s + 1], ns = e - b;
     if (ns <= 0) continue;
@@ -254,7 +260,7 @@ void NBodySolver::ensure_items(int tpl, int nch_hint) {
       }
     }
   }
-  if (use_persistent()) {
+  if (persist) {
     // work queue: longest items first (LPT order) for a balanced drain
     auto cost = [](const PairItem& q) {
       return (double)q.ntgt * (double)(q.src1 - q.src0) * (q.wall ? 5.0 : 1.0);
@@ -262,6 +268,15 @@ void NBodySolver::ensure_items(int tpl, int nch_hint) {
     std::stable_sort(items.begin(), items.end(),
                      [&](const PairItem& x, const PairItem& y) { return cost(x) > cost(y); });
   }
+}
+
+void NBodySolver::ensure_items(int tpl, int nch_hint) {
+  const int nch = resolve_nch(tpl, nch_hint);
+  if (items_tpl_ == tpl && items_nch_ == nch && items_N_ == N_ && items_layout_ == layout_id_ &&
+      items_persist_ == use_persistent())
+    return;
+  std::vector<PairItem> items;
+  build_items(tpl, nch, use_persistent(), items);
   items_.reserve(sizeof(PairItem) * std::max<size_t>(items.size(), 1));
   LM_HIP(hipMemcpyAsync(items_.as<PairItem>(), items.data(), sizeof(PairItem) * items.size(),
                         hipMemcpyHostToDevice, stream_));
@@ -272,6 +287,22 @@ void NBodySolver::ensure_items(int tpl, int nch_hint) {
   items_layout_ = layout_id_;
   items_persist_ = use_persistent();
   items_count_ = (int)items.size();
+}
+
+void NBodySolver::ensure_multi_items(ItemList& L, int tpl) {
+  if (L.tpl == tpl && L.N == N_ && L.layout == layout_id_) return;
+  std::vector<PairItem> items;
+  const int nch = resolve_nch(tpl, 0);
+  build_items(tpl, nch, false, items);
+  L.buf.reserve(sizeof(PairItem) * std::max<size_t>(items.size(), 1));
+  LM_HIP(hipMemcpyAsync(L.buf.as<PairItem>(), items.data(), sizeof(PairItem) * items.size(),
+                        hipMemcpyHostToDevice, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));
+  L.tpl = tpl;
+  L.nch = nch;
+  L.count = (int)items.size();
+  L.N = N_;
+  L.layout = layout_id_;
 }
 
 void NBodySolver::setPairTiling(int tpl, int nch) {
@@ -618,110 +649,112 @@ void NBodySolver::lanczos_product(const double* v, double* w) {
 // and convergence test.  All systems advance together; a converged system
 // leaves the pair work list (its items are skipped) and the vector kernels.
 
-void NBodySolver::ensure_seg_layout(int64_t n) {
-  const int nsys = numSystems();
-  if (seg_off_ != sys_off_host_) {
+void NBodySolver::ensure_seg_layout(SegSolve& S, const std::vector<int64_t>& off, int angular,
+                                    int64_t n) {
+  const int nrec = (int)off.size() - 1;
+  if (S.off != off) {
     std::vector<SegBlock> blocks;
-    std::vector<int> first(nsys + 1, 0);
-    for (int s = 0; s < nsys; ++s) {
+    std::vector<int> first(nrec + 1, 0);
+    for (int s = 0; s < nrec; ++s) {
       first[s] = (int)blocks.size();
-      for (int64_t p0 = sys_off_host_[s]; p0 < sys_off_host_[s + 1]; p0 += kSegParticles) {
+      for (int64_t p0 = off[s]; p0 < off[s + 1]; p0 += kSegParticles) {
         SegBlock b;
         b.p0 = p0;
-        b.p1 = std::min<int64_t>(p0 + kSegParticles, sys_off_host_[s + 1]);
+        b.p1 = std::min<int64_t>(p0 + kSegParticles, off[s + 1]);
         b.sys = s;
         b.pad = 0;
         blocks.push_back(b);
       }
     }
-    first[nsys] = (int)blocks.size();
+    first[nrec] = (int)blocks.size();
     if (blocks.empty()) throw Error("[libmobility_amd] no particles in any system");
-    seg_blocks_.reserve(sizeof(SegBlock) * blocks.size());
-    seg_first_.reserve(sizeof(int) * first.size());
-    LM_HIP(hipMemcpyAsync(seg_blocks_.as<void>(), blocks.data(), sizeof(SegBlock) * blocks.size(),
+    S.blocks.reserve(sizeof(SegBlock) * blocks.size());
+    S.first.reserve(sizeof(int) * first.size());
+    LM_HIP(hipMemcpyAsync(S.blocks.as<void>(), blocks.data(), sizeof(SegBlock) * blocks.size(),
                           hipMemcpyHostToDevice, stream_));
-    LM_HIP(hipMemcpyAsync(seg_first_.as<void>(), first.data(), sizeof(int) * first.size(),
+    LM_HIP(hipMemcpyAsync(S.first.as<void>(), first.data(), sizeof(int) * first.size(),
                           hipMemcpyHostToDevice, stream_));
     LM_HIP(hipStreamSynchronize(stream_));  // host vectors go out of scope
-    seg_nblocks_ = (int)blocks.size();
-    seg_partial_.reserve(sizeof(double) * 2 * blocks.size());
-    seg_flags_.reserve(sizeof(int) * 3 * nsys);
-    seg_host_.reserve(sizeof(int) * 4 * nsys + sizeof(double) * nsys);
-    seg_check_steps_.assign(nsys, 3);  // LanczosAlgorithm.cu:171, one solver per system
-    seg_iters_.assign(nsys, 0);
-    seg_off_ = sys_off_host_;
-    seg_n_ = 0;
-    seg_L_ = 0;
+    S.nblocks = (int)blocks.size();
+    S.partial.reserve(sizeof(double) * 2 * blocks.size());
+    S.flags.reserve(sizeof(int) * 3 * nrec);
+    S.host.reserve(sizeof(int) * 4 * nrec + sizeof(double) * nrec);
+    S.check_steps.assign(nrec, 3);  // LanczosAlgorithm.cu:171, one solver per recurrence
+    S.iters.assign(nrec, 0);
+    S.off = off;
+    S.angular = angular;
+    S.n = 0;
+    S.L = 0;
   }
-  if (seg_n_ != n) {
-    // previous estimates of every system start at zero (cu:175)
-    seg_old_.reserve(sizeof(double) * n);
-    seg_y_.reserve(sizeof(double) * n);
-    launch_zero(seg_old_.as<double>(), n, stream_);
-    launch_zero(seg_y_.as<double>(), n, stream_);
-    seg_n_ = n;
+  if (S.n != n || S.angular != angular) {
+    // previous estimates of every recurrence start at zero (cu:175)
+    S.old.reserve(sizeof(double) * n);
+    S.y.reserve(sizeof(double) * n);
+    launch_zero(S.old.as<double>(), n, stream_);
+    launch_zero(S.y.as<double>(), n, stream_);
+    S.n = n;
+    S.angular = angular;
   }
-  if (seg_L_ < iter_limit_) {
-    seg_state_.reserve(sizeof(double) * ((size_t)2 * nsys + (size_t)2 * nsys * iter_limit_));
-    seg_coef_.reserve(sizeof(double) * (size_t)nsys * iter_limit_);
-    seg_L_ = iter_limit_;
+  if (S.L < iter_limit_) {
+    S.state.reserve(sizeof(double) * ((size_t)2 * nrec + (size_t)2 * nrec * iter_limit_));
+    S.coef.reserve(sizeof(double) * (size_t)nrec * iter_limit_);
+    S.L = iter_limit_;
   }
 }
 
 void NBodySolver::lastLanczosIterationsPerSystem(int* out, int nsys) const {
-  const int have = (int)seg_iters_.size();
-  for (int s = 0; s < nsys; ++s) out[s] = (seg_mode_ && s < have) ? seg_iters_[s] : -1;
+  const int have = (int)seg_sys_.iters.size();
+  for (int s = 0; s < nsys; ++s) out[s] = (seg_mode_ && s < have) ? seg_sys_.iters[s] : -1;
 }
 
-int NBodySolver::lanczos_run_seg(const double* z, double** result) {
-  const int64_t n = (angular_ ? 6 : 3) * N_;
-  const int nsys = numSystems();
+int NBodySolver::lanczos_run_seg(SegSolve& S, const std::vector<int64_t>& off, bool angular,
+                                 bool skip_converged_pairs,
+                                 const std::function<void(const double*, double*)>& product,
+                                 const double* z, double** result) {
+  const int nrec = (int)off.size() - 1;
+  const int64_t nvirt = off.back();              // particles of the stacked layout
+  const int64_t n = (angular ? 6 : 3) * nvirt;  // entries of the stacked vector
   ensure_records();
-  ensure_seg_layout(n);
-  const int L = seg_L_;
-  w_.reserve(sizeof(double) * n);
-  if (precond_) {
-    ensure_precond_blocks();
-    pc_t_.reserve(sizeof(double) * n);
-    pc_u_.reserve(sizeof(double) * n);
-  }
-  double* st = seg_state_.as<double>();
-  int* d_active = seg_flags_.as<int>();
-  int* d_check = d_active + nsys;
-  int* d_list = d_active + 2 * nsys;
-  SegScalars sc{st, st + 2 * nsys, st + 2 * nsys + (int64_t)nsys * L, st + nsys, d_active,
+  ensure_seg_layout(S, off, angular ? 1 : 0, n);
+  const int L = S.L;
+  seg_w_.reserve(sizeof(double) * n);
+  double* st = S.state.as<double>();
+  int* d_active = S.flags.as<int>();
+  int* d_check = d_active + nrec;
+  int* d_list = d_active + 2 * nrec;
+  SegScalars sc{st, st + 2 * nrec, st + 2 * nrec + (int64_t)nrec * L, st + nrec, d_active,
                 d_check, L};
-  SegLayout g{seg_blocks_.as<SegBlock>(), seg_nblocks_, seg_first_.as<int>(), nsys, 3 * N_,
-              angular_ ? 1 : 0, seg_partial_.as<double>()};
+  SegLayout g{S.blocks.as<SegBlock>(), S.nblocks, S.first.as<int>(), nrec, 3 * nvirt,
+              angular ? 1 : 0, S.partial.as<double>()};
   // pinned host mirrors: the "check" flags alternate between two slots so a
   // slot is only rewritten after the stream has consumed its last upload
-  int* h_active = seg_host_.as<int>();
-  int* h_check_est = h_active + nsys;
-  int* h_check_store = h_active + 2 * nsys;
-  int* h_list = h_active + 3 * nsys;
-  double* h_err = reinterpret_cast<double*>(h_active + 4 * nsys);
-  double* old = seg_old_.as<double>();
-  double* y = seg_y_.as<double>();
-  double* coef = seg_coef_.as<double>();
+  int* h_active = S.host.as<int>();
+  int* h_check_est = h_active + nrec;
+  int* h_check_store = h_active + 2 * nrec;
+  int* h_list = h_active + 3 * nrec;
+  double* h_err = reinterpret_cast<double*>(h_active + 4 * nrec);
+  double* old = S.old.as<double>();
+  double* y = S.y.as<double>();
+  double* coef = S.coef.as<double>();
 
   int nact = 0;
-  std::vector<int> check_at(nsys);
-  for (int s = 0; s < nsys; ++s) {
-    h_active[s] = sys_off_host_[s + 1] > sys_off_host_[s] ? 1 : 0;
+  std::vector<int> check_at(nrec);
+  for (int s = 0; s < nrec; ++s) {
+    h_active[s] = off[s + 1] > off[s] ? 1 : 0;
     nact += h_active[s];
-    seg_iters_[s] = 0;
-    check_at[s] = std::min(seg_check_steps_[s], iter_limit_ - 2);  // cu:178-179
+    S.iters[s] = 0;
+    check_at[s] = std::min(S.check_steps[s], iter_limit_ - 2);  // cu:178-179
   }
-  LM_HIP(hipMemcpyAsync(d_active, h_active, sizeof(int) * nsys, hipMemcpyHostToDevice, stream_));
+  LM_HIP(hipMemcpyAsync(d_active, h_active, sizeof(int) * nrec, hipMemcpyHostToDevice, stream_));
   ensure_basis(n, 2);
-  launch_seg_start(z, V_.as<double>(), g, sc, stream_);  // cu:110-118 per system
-  // A system whose noise is exactly zero has the result zero; it does not
+  launch_seg_start(z, V_.as<double>(), g, sc, stream_);  // cu:110-118 per recurrence
+  // A recurrence whose noise is exactly zero has the result zero; it does not
   // iterate (its ||z|| = 0 would make every estimate NaN).
-  LM_HIP(hipMemcpyAsync(h_err, sc.normz, sizeof(double) * nsys, hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipMemcpyAsync(h_err, sc.normz, sizeof(double) * nrec, hipMemcpyDeviceToHost, stream_));
   LM_HIP(hipStreamSynchronize(stream_));
   {
     bool any_zero = false;
-    for (int s = 0; s < nsys; ++s) {
+    for (int s = 0; s < nrec; ++s) {
       h_check_store[s] = 0;
       if (h_active[s] && h_err[s] == 0.0) {
         h_active[s] = 0;
@@ -731,48 +764,48 @@ int NBodySolver::lanczos_run_seg(const double* z, double** result) {
       }
     }
     if (any_zero) {
-      LM_HIP(hipMemcpyAsync(d_check, h_check_store, sizeof(int) * nsys, hipMemcpyHostToDevice,
+      LM_HIP(hipMemcpyAsync(d_check, h_check_store, sizeof(int) * nrec, hipMemcpyHostToDevice,
                             stream_));
       launch_seg_clear(y, g, sc, stream_);
-      LM_HIP(hipMemcpyAsync(d_active, h_active, sizeof(int) * nsys, hipMemcpyHostToDevice, stream_));
+      LM_HIP(hipMemcpyAsync(d_active, h_active, sizeof(int) * nrec, hipMemcpyHostToDevice, stream_));
     }
   }
   struct ActiveGuard {
     const int*& p;
     ~ActiveGuard() { p = nullptr; }
   } guard{pair_active_};
-  pair_active_ = d_active;
+  if (skip_converged_pairs) pair_active_ = d_active;
   int last = 0;
   for (int i = 0; i < iter_limit_ && nact > 0; ++i) {
     ensure_basis(n, i + 2);
     double* V = V_.as<double>();
     const double* vi = V + (int64_t)i * n;
-    double* w = w_.as<double>();
-    lanczos_product(vi, w);
+    double* w = seg_w_.as<double>();
+    product(vi, w);
     launch_seg_alpha(w, vi, i > 0 ? V + (int64_t)(i - 1) * n : nullptr, i, g, sc, stream_);
     launch_seg_beta(w, vi, i, g, sc, stream_);
     launch_seg_next(w, V + (int64_t)(i + 1) * n, i, g, sc, stream_);
     int ncheck = 0;
-    for (int s = 0; s < nsys; ++s) {
+    for (int s = 0; s < nrec; ++s) {
       const bool c = h_active[s] && (i >= check_at[s] || cb_ != nullptr);
       h_check_est[s] = c ? 1 : 0;
       if (c) h_list[ncheck++] = s;
     }
     if (ncheck == 0) continue;
     const int m = i + 1;
-    if (seg_work_cols_ < (size_t)m) {
-      seg_work_.reserve(sizeof(double) * kSegTsqrtGroups * tsqrt_scratch(L));
-      seg_work_cols_ = (size_t)L;
+    if (S.work_cols < (size_t)m) {
+      S.work.reserve(sizeof(double) * kSegTsqrtGroups * tsqrt_scratch(L));
+      S.work_cols = (size_t)L;
     }
-    LM_HIP(hipMemcpyAsync(d_check, h_check_est, sizeof(int) * nsys, hipMemcpyHostToDevice, stream_));
+    LM_HIP(hipMemcpyAsync(d_check, h_check_est, sizeof(int) * nrec, hipMemcpyHostToDevice, stream_));
     LM_HIP(hipMemcpyAsync(d_list, h_list, sizeof(int) * ncheck, hipMemcpyHostToDevice, stream_));
-    launch_seg_tridiag_sqrt(d_list, ncheck, m, sc, coef, seg_work_.as<double>(), stream_);
+    launch_seg_tridiag_sqrt(d_list, ncheck, m, sc, coef, S.work.as<double>(), stream_);
     launch_seg_estimate(V, n, m, coef, y, old, g, sc, stream_);
-    LM_HIP(hipMemcpyAsync(h_err, sc.err, sizeof(double) * nsys, hipMemcpyDeviceToHost, stream_));
+    LM_HIP(hipMemcpyAsync(h_err, sc.err, sizeof(double) * nrec, hipMemcpyDeviceToHost, stream_));
     LM_HIP(hipStreamSynchronize(stream_));
     bool changed = false;
     double worst = 0.0;
-    for (int s = 0; s < nsys; ++s) h_check_store[s] = 0;
+    for (int s = 0; s < nrec; ++s) h_check_store[s] = 0;
     for (int j = 0; j < ncheck; ++j) {
       const int s = h_list[j];
       h_check_store[s] = 2;  // oldBz <- Bz unless converged (cu:196)
@@ -782,11 +815,11 @@ int NBodySolver::lanczos_run_seg(const double* z, double** result) {
       worst = std::max(worst, err);
       if (err <= tol_) {
         // registerRequiredStepsForConverge, cu:224-232
-        if (i - 2 > seg_check_steps_[s])
-          seg_check_steps_[s] += 1;
+        if (i - 2 > S.check_steps[s])
+          S.check_steps[s] += 1;
         else
-          seg_check_steps_[s] = std::max(1, seg_check_steps_[s] - 2);
-        seg_iters_[s] = i;
+          S.check_steps[s] = std::max(1, S.check_steps[s] - 2);
+        S.iters[s] = i;
         last = std::max(last, i);
         h_active[s] = 0;
         h_check_store[s] = 0;
@@ -795,11 +828,11 @@ int NBodySolver::lanczos_run_seg(const double* z, double** result) {
       }
     }
     if (cb_ && i > 0) cb_(i, worst, cb_user_);
-    LM_HIP(hipMemcpyAsync(d_check, h_check_store, sizeof(int) * nsys, hipMemcpyHostToDevice,
+    LM_HIP(hipMemcpyAsync(d_check, h_check_store, sizeof(int) * nrec, hipMemcpyHostToDevice,
                           stream_));
     launch_seg_store(y, old, g, sc, stream_);
     if (changed)
-      LM_HIP(hipMemcpyAsync(d_active, h_active, sizeof(int) * nsys, hipMemcpyHostToDevice, stream_));
+      LM_HIP(hipMemcpyAsync(d_active, h_active, sizeof(int) * nrec, hipMemcpyHostToDevice, stream_));
   }
   if (nact > 0) throw Error("[Lanczos] Could not converge");
   last_iters_ = last;
@@ -823,10 +856,17 @@ void NBodySolver::sqrtMdotW_dev(double* U, double* W, double prefactor, const do
     z = noise_.as<double>();
   }
   double* y = nullptr;
-  if (seg_mode_ && numSystems() > 1)
-    lanczos_run_seg(z, &y);
-  else
+  if (seg_mode_ && numSystems() > 1) {
+    if (precond_) {
+      ensure_precond_blocks();
+      pc_t_.reserve(sizeof(double) * n);
+      pc_u_.reserve(sizeof(double) * n);
+    }
+    lanczos_run_seg(seg_sys_, sys_off_host_, angular_, true,
+                    [this](const double* v, double* w) { lanczos_product(v, w); }, z, &y);
+  } else {
     lanczos_run(z, &y);
+  }
   if (precond_) {
     // B W = G^{-1} (G M G)^{1/2} W
     launch_apply_blocks(pc_Gi_.as<double>(), y, N_, n, pc_t_.as<double>(), stream_);
@@ -870,6 +910,181 @@ void NBodySolver::sqrtMdotW(double* U, double* W, double prefactor, bool dev,
   if (angular_ && W)
     LM_HIP(hipMemcpyAsync(W, st_w_.as<double>(), sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
   LM_HIP(hipStreamSynchronize(stream_));
+}
+
+// ---- several vectors per pass over the pairs ---------------------------------------
+
+void NBodySolver::mdot_multi_dev(const double* F, int nrhs, double* U) {
+  require_positions("[Mobility] Positions have 0 particles. Did you call setPositions?");
+  if (angular_)
+    throw Error(
+        "[libmobility_amd] the multi-vector product is translational only: initialize the solver "
+        "without includeAngular");
+  ensure_records();
+  const int64_t n3 = 3 * N_;
+  const bool fp32 = pair_prec_ == kPairFP32;
+  const void* rec = fp32 ? rec32_.as<void>() : rec64_.as<void>();
+  int r0 = 0;
+  while (r0 < nrhs) {
+    const int left = nrhs - r0;
+    const int nr = left >= 4 ? 4 : (left >= 2 ? 2 : 1);
+    if (nr == 1) {
+      pair_product(rec64_.as<void>(), rec32_.as<void>(), F + (int64_t)r0 * n3, nullptr,
+                   U + (int64_t)r0 * n3, nullptr);
+      r0 += 1;
+      continue;
+    }
+    const int tpl = multi_tpl(nr);
+    ItemList& L = multi_items_[tpl == 1 ? 0 : 1];
+    ensure_multi_items(L, tpl);
+    PairMultiArgs a{};
+    a.kind = kind_code();
+    a.fp32 = fp32 ? 1 : 0;
+    a.nrhs = nr;
+    a.pos = rec;
+    a.F = F + (int64_t)r0 * n3;
+    a.n3 = n3;
+    a.nch = L.nch;
+    a.items = L.buf.as<PairItem>();
+    a.nitems = L.count;
+    a.t0 = t0_;
+    if (L.nch == 1) {
+      a.U = U + (int64_t)r0 * n3;
+      launch_pair_multi(a, stream_);
+    } else {
+      part_multi_.reserve(sizeof(double) * (size_t)nr * L.nch * n3);
+      a.U = part_multi_.as<double>();
+      launch_pair_multi(a, stream_);
+      for (int r = 0; r < nr; ++r)
+        launch_chunk_reduce(part_multi_.as<double>() + (int64_t)r * L.nch * n3, L.nch, n3,
+                            U + (int64_t)(r0 + r) * n3, stream_);
+    }
+    r0 += nr;
+  }
+}
+
+void NBodySolver::MdotMulti(const double* F, int nrhs, double* U, bool dev) {
+  if (nrhs < 1) throw Error("[libmobility_amd] at least one force vector is required");
+  if (!F || !U) throw Error("[libmobility_amd] null force or output array");
+  require_positions("[Mobility] Positions have 0 particles. Did you call setPositions?");
+  const int64_t total = (int64_t)nrhs * 3 * N_;
+  if (dev) {
+    mdot_multi_dev(F, nrhs, U);
+    LM_HIP(hipStreamSynchronize(stream_));
+    return;
+  }
+  multi_f_.reserve(sizeof(double) * total);
+  multi_u_.reserve(sizeof(double) * total);
+  LM_HIP(hipMemcpyAsync(multi_f_.as<double>(), F, sizeof(double) * total, hipMemcpyHostToDevice,
+                        stream_));
+  mdot_multi_dev(multi_f_.as<double>(), nrhs, multi_u_.as<double>());
+  LM_HIP(hipMemcpyAsync(U, multi_u_.as<double>(), sizeof(double) * total, hipMemcpyDeviceToHost,
+                        stream_));
+  LM_HIP(hipStreamSynchronize(stream_));
+}
+
+// w = M v (or G M G v) for nrhs stacked vectors
+void NBodySolver::multi_product(const double* v, int nrhs, double* w) {
+  const int64_t n3 = 3 * N_;
+  if (precond_) {
+    double* t = multi_t_.as<double>();
+    double* u = multi_w_.as<double>();
+    for (int r = 0; r < nrhs; ++r)
+      launch_apply_blocks(pc_G_.as<double>(), v + r * n3, N_, n3, t + r * n3, stream_);
+    mdot_multi_dev(t, nrhs, u);
+    for (int r = 0; r < nrhs; ++r)
+      launch_apply_blocks(pc_G_.as<double>(), u + r * n3, N_, n3, w + r * n3, stream_);
+  } else {
+    mdot_multi_dev(v, nrhs, w);
+  }
+}
+
+void NBodySolver::sqrtMdotWMulti(double* U, int nrhs, double prefactor, bool dev,
+                                 const double* noise) {
+  if (nrhs < 1) throw Error("[libmobility_amd] at least one noise vector is required");
+  if (prefactor == 0) return;  // MobilityInterface.h:134-135
+  if (!initialized_)
+    throw Error(
+        "[libMobility] You must initialize the base class in order to use the default "
+        "stochastic displacement computation");
+  require_positions(
+      "[libMobility] The number of particles is not set. Did you forget to call setPositions?");
+  if (!U) throw Error("[libMobility] This solver requires linear velocities");
+  if (angular_)
+    throw Error(
+        "[libmobility_amd] the multi-vector square root is translational only: initialize the "
+        "solver without includeAngular");
+  const int64_t n3 = 3 * N_;
+  const int64_t total = (int64_t)nrhs * n3;
+  double* Ud = U;
+  const double* z = noise;
+  if (!dev) {
+    multi_u_.reserve(sizeof(double) * total);
+    LM_HIP(hipMemcpyAsync(multi_u_.as<double>(), U, sizeof(double) * total, hipMemcpyHostToDevice,
+                          stream_));
+    Ud = multi_u_.as<double>();
+    if (noise) {
+      multi_f_.reserve(sizeof(double) * total);
+      LM_HIP(hipMemcpyAsync(multi_f_.as<double>(), noise, sizeof(double) * total,
+                            hipMemcpyHostToDevice, stream_));
+      z = multi_f_.as<double>();
+    }
+  }
+  if (!z) {
+    // the seeds nrhs successive sqrtMdotW calls would draw (lanczos.h:35-52)
+    if (!lanczos_engine_live_) {
+      lanczos_engine_ = std::mt19937(lanczos_seed_);
+      lanczos_engine_live_ = true;
+    }
+    multi_noise_.reserve(sizeof(double) * total);
+    for (int r = 0; r < nrhs; ++r) {
+      const uint32_t s1 = std::uniform_int_distribution<uint32_t>(0, UINT32_MAX)(lanczos_engine_);
+      const uint32_t s2 = std::uniform_int_distribution<uint32_t>(0, UINT32_MAX)(lanczos_engine_);
+      launch_normal_fill(multi_noise_.as<double>() + r * n3, n3, ((uint64_t)s2 << 32) | s1, 0,
+                         stream_);
+    }
+    z = multi_noise_.as<double>();
+  }
+  // recurrences: one per vector, or one per (vector, system) in per-system mode
+  const int nsys = numSystems();
+  const bool per_sys = seg_mode_ && nsys > 1;
+  std::vector<int64_t> off(1, 0);
+  for (int r = 0; r < nrhs; ++r) {
+    if (per_sys)
+      for (int s = 1; s <= nsys; ++s) off.push_back((int64_t)r * N_ + sys_off_host_[s]);
+    else
+      off.push_back((int64_t)(r + 1) * N_);
+  }
+  if (precond_) {
+    ensure_precond_blocks();
+    multi_t_.reserve(sizeof(double) * total);
+    multi_w_.reserve(sizeof(double) * total);
+  }
+  double* y = nullptr;
+  lanczos_run_seg(seg_multi_, off, false, false,
+                  [this, nrhs](const double* v, double* w) { multi_product(v, nrhs, w); }, z, &y);
+  multi_nrhs_ = nrhs;
+  multi_per_ = per_sys ? nsys : 1;
+  if (precond_) {
+    double* t = multi_t_.as<double>();
+    for (int r = 0; r < nrhs; ++r)
+      launch_apply_blocks(pc_Gi_.as<double>(), y + r * n3, N_, n3, t + r * n3, stream_);
+    y = t;
+  }
+  launch_axpy_accumulate(Ud, y, prefactor, total, stream_);  // out += prefactor * y
+  if (!dev)
+    LM_HIP(hipMemcpyAsync(U, Ud, sizeof(double) * total, hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));
+}
+
+void NBodySolver::lastLanczosIterationsMulti(int* out, int nrhs) const {
+  for (int r = 0; r < nrhs; ++r) {
+    int best = -1;
+    if (r < multi_nrhs_)
+      for (int s = 0; s < multi_per_; ++s)
+        best = std::max(best, seg_multi_.iters[(size_t)r * multi_per_ + s]);
+    out[r] = best;
+  }
 }
 
 // ---- learned preconditioner ------------------------------------------------------

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <random>
 #include <tuple>
@@ -59,6 +60,17 @@ class NBodySolver {
   int numSystems() const { return (int)sys_off_host_.size() - 1; }
   // iterations of the last solve per system (per-system mode), else -1s
   void lastLanczosIterationsPerSystem(int* out, int nsys) const;
+
+  // Several force vectors in one pass over the pairs (translational block):
+  // U_r = M F_r, r < nrhs; F and U hold nrhs consecutive 3N vectors.
+  void MdotMulti(const double* F, int nrhs, double* U, bool dev);
+  void mdot_multi_dev(const double* F, int nrhs, double* U);
+  // nrhs independent Lanczos square roots sharing every pass over the pairs:
+  // U_r += prefactor M^{1/2} W_r.  noise: nrhs consecutive 3N vectors, or
+  // nullptr to draw them as nrhs successive sqrtMdotW calls would.
+  void sqrtMdotWMulti(double* U, int nrhs, double prefactor, bool dev, const double* noise);
+  // iterations of the last multi-vector solve per vector (largest over systems)
+  void lastLanczosIterationsMulti(int* out, int nrhs) const;
   int64_t numParticles() const { return N_; }
   bool includeAngular() const { return angular_; }
 
@@ -95,6 +107,24 @@ class NBodySolver {
   void divM_dev(double* U, double* W, double prefactor);
 
  private:
+  // State of a set of independent recurrences over a stacked layout: the
+  // systems of an ensemble, or (vector, system) pairs of a multi-vector solve.
+  struct SegSolve {
+    DevBuf blocks, first, partial, state, flags, coef, work, old, y;
+    HostBuf host;  // active, check (2 slots), list (int), err (double): one each per recurrence
+    int nblocks = 0, L = 0, angular = -1;
+    int64_t n = 0;
+    size_t work_cols = 0;
+    std::vector<int> check_steps, iters;
+    std::vector<int64_t> off;  // particle offsets of the recurrences in the stacked layout
+  };
+  // A pair work list with the key it was built for.
+  struct ItemList {
+    DevBuf buf;
+    int tpl = -1, nch = 0, count = 0;
+    uint64_t layout = 0;
+    int64_t N = -1;
+  };
   void require_positions(const char* msg) const;
   void ensure_records();
   void build_systems(int64_t N, std::vector<int64_t>& off, bool& valid) const;
@@ -106,8 +136,11 @@ class NBodySolver {
   void pair_product(const void* rec64, const void* rec32, const double* F, const double* T,
                     double* U, double* W);
   int lanczos_run(const double* z, double** result);
-  int lanczos_run_seg(const double* z, double** result);
-  void ensure_seg_layout(int64_t n);
+  int lanczos_run_seg(SegSolve& S, const std::vector<int64_t>& off, bool angular,
+                      bool skip_converged_pairs,
+                      const std::function<void(const double*, double*)>& product, const double* z,
+                      double** result);
+  void ensure_seg_layout(SegSolve& S, const std::vector<int64_t>& off, int angular, int64_t n);
   void ensure_basis(int64_t n, int64_t cols);
   // w = M v (or G M G v with the preconditioner) for the Lanczos recurrences
   void lanczos_product(const double* v, double* w);
@@ -174,15 +207,16 @@ class NBodySolver {
   // per-system Lanczos
   bool seg_mode_ = false;
   const int* pair_active_ = nullptr;  // per-system flags seen by the pair kernel
-  DevBuf seg_blocks_, seg_first_, seg_partial_, seg_state_, seg_flags_, seg_coef_, seg_work_,
-      seg_old_, seg_y_;
-  HostBuf seg_host_;  // active[nsys], check[nsys], list[nsys] (int), err[nsys] (double)
-  int seg_nblocks_ = 0, seg_L_ = 0;
-  uint64_t seg_layout_ = 0;
-  int64_t seg_n_ = 0;
-  size_t seg_work_cols_ = 0;
-  std::vector<int> seg_check_steps_, seg_iters_;
-  std::vector<int64_t> seg_off_;  // systems the per-system state belongs to
+  SegSolve seg_sys_, seg_multi_;
+  DevBuf seg_w_;
+  // multi-vector products
+  ItemList multi_items_[2];  // 4 vectors (one target per lane), 2 vectors (two)
+  DevBuf part_multi_, multi_f_, multi_u_, multi_noise_, multi_t_, multi_w_;
+  int multi_nrhs_ = 0, multi_per_ = 1;
+  int resolve_nch(int tpl, int nch_hint) const;
+  void build_items(int tpl, int nch, bool persist, std::vector<PairItem>& items) const;
+  void ensure_multi_items(ItemList& L, int tpl);
+  void multi_product(const double* v, int nrhs, double* w);
   // learned preconditioner
   void ensure_knn();
   void ensure_precond_blocks();

@@ -359,6 +359,51 @@ class NBody:
                                                         float(prefactor), mem))
         return linear, angular
 
+    # -- several vectors per pass (extension) ----------------------------------------
+    def _stacked(self, x, what):
+        a = _Arr(x)
+        n3 = 3 * self.getNumberParticles()
+        if n3 == 0 or a.size == 0 or a.size % n3 != 0 or a.ndim < 2 or a.shape[0] * n3 != a.size:
+            raise RuntimeError(f"[libmobility_amd] {what} must have shape (nrhs, N, 3) or (nrhs, 3N)")
+        return a, a.size // n3
+
+    def MdotMulti(self, forces):
+        """Extension: the translational product for several force vectors in
+        one pass over the pairs.  ``forces`` has shape (nrhs, N, 3) or
+        (nrhs, 3N); the result has the same shape and framework."""
+        f, nrhs = self._stacked(forces, "forces")
+        out = _zeros(f.shape, f.framework, f.device)
+        self._bind_stream(f.mem, f.device)
+        _ffi.check(self._lib.lm_nbody_mdot_multi(self._h, f.ptr, nrhs, _ptr(out), f.mem))
+        return out
+
+    def sqrtMdotWMulti(self, nrhs, prefactor=1.0):
+        """Extension: ``nrhs`` independent square roots (the noise that ``nrhs``
+        successive sqrtMdotW calls would draw) sharing every pass over the
+        pairs; shape (nrhs,) + the positions' form."""
+        shape = (int(nrhs),) + self._own_shape()
+        out = _zeros(shape, _last_framework, _last_device)
+        mem = _mem_of(_last_framework, _last_device)
+        self._bind_stream(mem, _last_device)
+        _ffi.check(self._lib.lm_nbody_sqrt_mdot_w_multi(self._h, None, int(nrhs), _ptr(out),
+                                                        float(prefactor), mem))
+        return out
+
+    def sqrtMdotWMulti_with_noise(self, noise, prefactor=1.0):
+        """Extension: prefactor * M^{1/2} W_r for given W of shape (nrhs, 3N) or (nrhs, N, 3)."""
+        w, nrhs = self._stacked(noise, "noise")
+        out = _zeros(w.shape, w.framework, w.device)
+        self._bind_stream(w.mem, w.device)
+        _ffi.check(self._lib.lm_nbody_sqrt_mdot_w_multi(self._h, w.ptr, nrhs, _ptr(out),
+                                                        float(prefactor), w.mem))
+        return out
+
+    def lastLanczosIterationsMulti(self, nrhs):
+        out = np.zeros(int(nrhs), dtype=np.int32)
+        _ffi.check(self._lib.lm_nbody_last_lanczos_iterations_multi(self._h, out.ctypes.data,
+                                                                    int(nrhs)))
+        return out
+
     def divM(self, prefactor=1):
         """pythonify.h:329-345 -> NBody::divM (mobility.h:184-236)."""
         if self.getNumberParticles() <= 0:
