@@ -79,7 +79,9 @@ void launch_zero_open_systems(double* v, int64_t N, const int64_t* sys_off, int 
 void launch_normal_fill(double* out, int64_t n, uint64_t seed, uint64_t stream, hipStream_t s);
 
 // Scratch for the single-pass reductions: partial sums + a self-resetting
-// arrival counter.  kReduceBlocks partials per reduction slot.
+// arrival counter.  kReduceBlocks partials per reduction slot (measured: a
+// wider grid costs more in contention on the arrival counter than it gains
+// in memory parallelism at n ~ 4e5).
 constexpr int kReduceBlocks = 256;
 struct ReduceScratch {
   double* partial;       // kReduceBlocks * 4 doubles
@@ -111,7 +113,7 @@ void launch_lanczos_next(const double* w, double* v_next, int64_t n, int i,
                          const LanczosScalars* sc, hipStream_t s);
 // s[0..m) = T_m^{1/2} e1 (FP64, one workgroup); work has tsqrt_scratch(m) doubles
 constexpr int kTsqrtNodes = 512;
-constexpr int64_t tsqrt_scratch(int m) { return 2 * (int64_t)m * kTsqrtNodes; }
+constexpr int64_t tsqrt_scratch(int m) { return (int64_t)m * kTsqrtNodes; }
 void launch_tridiag_sqrt(const LanczosScalars* sc, int m, double* s_out, double* work,
                          hipStream_t s);
 // y = normz * V[:, 0:m] s ; err = ||y - old|| / ||old||
@@ -141,7 +143,7 @@ struct SegBlock {
   int32_t pad;
 };
 constexpr int kSegParticles = 512;  // particles per block
-constexpr int kSegTsqrtGroups = 64; // concurrent tridiagonal solves (scratch slices)
+constexpr int kSegTsqrtGroups = 256; // concurrent tridiagonal solves (scratch slices)
 
 struct SegLayout {
   const SegBlock* blocks;

@@ -11,13 +11,14 @@
 // Tridiagonal square root (replaces host LAPACKE ?steqr + cblas ?gemv,
 // LanczosAlgorithm.cu:48-85): for the SPD tridiagonal T,
 //     T^{1/2} e1 = T T^{-1/2} e1,   T^{-1/2} = (2/pi) Int_R e^u (e^{2u} I + T)^{-1} du,
-// evaluated with the trapezoidal rule in u (spacing 0.3, integrand analytic in
-// the strip |Im u| < pi/2, so the error is ~e^{-pi^2/0.3}), truncated 33 units
-// beyond [log sqrt(lmin), log sqrt(lmax)].  Each quadrature node is one
-// shifted tridiagonal solve (Thomas algorithm), one node per thread; the
-// spectrum bounds come from Gershgorin (upper) and Sturm-count multisection
-// (lower).  A negative eigenvalue (non-SPD operator) yields NaN, which the
-// convergence test reports as the reference does (LanczosAlgorithm.cu:217-220).
+// evaluated with the trapezoidal rule after the substitution u = c + s sinh(t),
+// which turns the exponential (sech) tails of the integrand into doubly
+// exponential ones, so ~70-130 nodes reach ~1e-14.  Each node is one shifted
+// tridiagonal solve by UL elimination (a division chain from the bottom, then
+// a multiplication chain from the top), one node per thread; the spectrum
+// bounds come from Gershgorin (upper) and Sturm-count multisection (lower).  A
+// negative eigenvalue (non-SPD operator) yields NaN, which the convergence
+// test reports as the reference does (LanczosAlgorithm.cu:217-220).
 #include <cmath>
 
 #include "kernels.hpp"
@@ -303,9 +304,10 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
     for (int i = t; i < m; i += kTS) s_out[i] = NAN;
     return;
   }
-  // Multisection for the smallest eigenvalue: 4 rounds over kTS points.
+  // Multisection for the smallest eigenvalue: 3 rounds over kTS points bracket
+  // it to (range) / kTS^3; the lower end L is a lower bound.
   double L = s_lo, H = s_hi;
-  for (int round = 0; round < 4; ++round) {
+  for (int round = 0; round < 3; ++round) {
     const double x = L + (H - L) * (double)(t + 1) / (double)kTS;
     const int c = sturm_count(a, b2, m, x, pivmin);
     int first = (c >= 1) ? t : kTS;
@@ -323,40 +325,40 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
   }
   const double lmin = fmax(L, 1e-300 + 1e-16 * s_hi);
   const double lmax = s_hi;
-  // Trapezoidal nodes u_k = u0 + k h
-  double h = 0.3;
-  const double u0 = 0.5 * log(lmin) - 33.0;
-  const double u1 = 0.5 * log(lmax) + 33.0;
-  int K = (int)ceil((u1 - u0) / h) + 1;
-  if (K > kTS) {
-    K = kTS;
-    h = (u1 - u0) / (double)(kTS - 1);
+  // Nodes u_k = c + s sinh(t_k), t_k = (k - half) h: the map makes the sech
+  // tails of the integrand decay double-exponentially, so K ~ 70-130 nodes
+  // give ~1e-14 over condition numbers 1..1e9 (rule fitted and checked
+  // against eigh; s and h grow and shrink with log10 of the condition number).
+  const double lk = log10(fmax(lmax / lmin, 1.0));
+  const double sc_s = 1.2 + 0.2 * lk;
+  double h = 0.18 / (1.0 + 0.3 * lk);
+  int half = (int)floor(4.0 / h);
+  if (2 * half + 1 > kTS) {
+    half = (kTS - 1) / 2;
+    h = 4.0 / (double)half;
   }
-  double* cp = work;                      // [m][kTS]
-  double* dp = work + (int64_t)m * kTS;   // [m][kTS]
+  const int K = 2 * half + 1;
+  const double c0 = 0.25 * (log(lmin) + log(lmax));
+  double* q = work;  // [m][kTS]: UL multipliers, then weighted solution entries
   if (t < K) {
-    const double u = u0 + h * (double)t;
+    const double tk = (double)(t - half) * h;
+    const double u = c0 + sc_s * sinh(tk);
     const double sig = exp(2.0 * u);
-    const double wgt = (2.0 * h / M_PI) * exp(u);
-    // Thomas: (sig I + T) y = e1
-    double den = sig + a[0];
-    double c_prev = (m > 1) ? b[0] / den : 0.0;
-    double d_prev = 1.0 / den;
-    cp[t] = c_prev;
-    dp[t] = d_prev;
-    for (int i = 1; i < m; ++i) {
-      den = sig + a[i] - b[i - 1] * c_prev;
-      const double inv = 1.0 / den;
-      c_prev = (i < m - 1) ? b[i] * inv : 0.0;
-      d_prev = -b[i - 1] * d_prev * inv;
-      cp[i * kTS + t] = c_prev;
-      dp[i * kTS + t] = d_prev;
-    }
-    double y = d_prev;
-    dp[(m - 1) * kTS + t] = wgt * y;
+    const double wgt = (2.0 * h / M_PI) * exp(u) * sc_s * cosh(tk);
+    // (sig I + T) x = e1 by UL elimination: g_{m-1} = sig + a_{m-1},
+    // g_i = sig + a_i - b_i^2 / g_{i+1}; x_0 = 1/g_0, x_i = q_i x_{i-1} with
+    // q_i = -b_{i-1} / g_i.  The second pass only multiplies.
+    double rg = 1.0 / (sig + a[m - 1]);
+    if (m > 1) q[(int64_t)(m - 1) * kTS + t] = -b[m - 2] * rg;
     for (int i = m - 2; i >= 0; --i) {
-      y = dp[i * kTS + t] - cp[i * kTS + t] * y;
-      dp[i * kTS + t] = wgt * y;
+      rg = 1.0 / (sig + a[i] - b2[i] * rg);
+      if (i >= 1) q[(int64_t)i * kTS + t] = -b[i - 1] * rg;
+    }
+    double x = rg;
+    q[t] = wgt * x;
+    for (int i = 1; i < m; ++i) {
+      x *= q[(int64_t)i * kTS + t];
+      q[(int64_t)i * kTS + t] = wgt * x;
     }
   }
   __syncthreads();
@@ -364,7 +366,7 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
   const int lane = t & 63, wid = t >> 6;
   for (int j = wid; j < m; j += kTS / 64) {
     double v = 0.0;
-    for (int k = lane; k < K; k += 64) v += dp[j * kTS + k];
+    for (int k = lane; k < K; k += 64) v += q[(int64_t)j * kTS + k];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     if (lane == 0) xs[j] = v;
   }

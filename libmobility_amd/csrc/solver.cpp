@@ -794,8 +794,10 @@ int NBodySolver::lanczos_run_seg(SegSolve& S, const std::vector<int64_t>& off, b
     if (ncheck == 0) continue;
     const int m = i + 1;
     if (S.work_cols < (size_t)m) {
-      S.work.reserve(sizeof(double) * kSegTsqrtGroups * tsqrt_scratch(L));
-      S.work_cols = (size_t)L;
+      // grown geometrically with the Krylov dimension (slices of m x nodes)
+      const int cols = std::min(L, std::max({m, 2 * (int)S.work_cols, 16}));
+      S.work.reserve(sizeof(double) * kSegTsqrtGroups * tsqrt_scratch(cols));
+      S.work_cols = (size_t)cols;
     }
     LM_HIP(hipMemcpyAsync(d_check, h_check_est, sizeof(int) * nrec, hipMemcpyHostToDevice, stream_));
     LM_HIP(hipMemcpyAsync(d_list, h_list, sizeof(int) * ncheck, hipMemcpyHostToDevice, stream_));
