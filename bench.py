@@ -781,6 +781,29 @@ def bench_c5(args):
                        compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS), extra, None)
 
 
+def valu_ceiling():
+    """Measured VALU issue ceiling (independent FMA chains) and the cost of a
+    rsqrt in FMA slots; one JSON line."""
+    import torch  # noqa: F401  (one HIP runtime with torch)
+    from libmobility_amd import _ffi
+    lib = _ffi.load()
+    blocks, iters = 256 * 32, 4096
+    waves = blocks * 4
+    res = {}
+    for name, kind in (("f32_fma", 0), ("f32_fma_rsqrt", 1), ("f64_fma", 2)):
+        ms = ctypes.c_float()
+        for _ in range(2):  # first launch warms up clocks and code
+            _ffi.check(lib.lm_diag_valu_throughput(kind, blocks, iters, ctypes.byref(ms)))
+        res[name + "_ms"] = ms.value
+    per_s = lambda ms, n: waves * iters * n / (ms * 1e-3)
+    res["f32_fma_wave_instr_per_s"] = per_s(res["f32_fma_ms"], 24)
+    res["f32_fma_tflops"] = 2 * 64 * res["f32_fma_wave_instr_per_s"] / 1e12
+    res["f64_fma_tflops"] = 2 * 64 * per_s(res["f64_fma_ms"], 24) / 1e12
+    # 24 FMA + 8 add + 8 rsqrt per iteration: rsqrt cost in FMA slots
+    res["rsqrt_fma_slots"] = (res["f32_fma_rsqrt_ms"] / res["f32_fma_ms"] * 24 - 32) / 8
+    print(json.dumps(res), flush=True)
+
+
 def copy_calibration():
     import torch
     n = 1 << 28  # doubles: 2 GiB read + 2 GiB written per copy
@@ -810,6 +833,8 @@ def main():
                     help="c3: skip the several-vectors-per-pass leg")
     ap.add_argument("--no-bbt", action="store_true",
                     help="c3: skip the small-N ||BB^T - M|| check (thousands of small solves)")
+    ap.add_argument("--valu-ceiling", action="store_true",
+                    help="only measure the VALU issue ceiling (FMA chains, rsqrt cost)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device-to-device copy (reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -821,6 +846,9 @@ def main():
         args.tiling = tuple(int(x) for x in args.tiling.split(","))
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.valu_ceiling:
+        valu_ceiling()
         return
     res = {"c3": bench_c3, "c1": bench_c1, "c2": bench_c2, "c4": bench_c4,
            "c5": bench_c5}[args.workload](args)

@@ -251,6 +251,13 @@ int lm_nbody_precond_blocks(lm_nbody* h, double* G9);
 int lm_selftest_pair_block_host(int wall, int fp32, const double* pi, const double* pj, double a,
                                 double* block36);
 
+/* Diagnostic (no counterpart in the reference): time `blocks` workgroups of
+ * 256 threads running `iters` iterations of 8 independent chains of 3 FMAs
+ * (kind 0: FP32; 1: FP32 with one rsqrt per chain step; 2: FP64) on the null
+ * stream; *ms receives the elapsed milliseconds.  Gives the VALU issue
+ * ceiling the pair kernel is compared with. */
+int lm_diag_valu_throughput(int kind, int blocks, int iters, float* ms);
+
 /* Philox4x32-10 of the noise kernels: out = philox(ctr, key). */
 int lm_selftest_philox_host(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4);
 

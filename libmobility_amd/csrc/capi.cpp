@@ -323,6 +323,13 @@ int lm_selftest_pair_block_host(int wall, int fp32, const double* pi, const doub
   });
 }
 
+int lm_diag_valu_throughput(int kind, int blocks, int iters, float* ms) {
+  return guarded([&] {
+    if (!ms) throw lm::Error("[libmobility_amd] null output");
+    *ms = lm::diag_valu_ms(kind, blocks, iters, nullptr);
+  });
+}
+
 int lm_selftest_philox_host(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4) {
   return guarded([&] {
     lm::U32x4 c;

@@ -188,4 +188,7 @@ void launch_seg_store(const double* y, double* old, const SegLayout& g, const Se
 // y <- 0 for systems with check == 3
 void launch_seg_clear(double* y, const SegLayout& g, const SegScalars& sc, hipStream_t s);
 
+// Diagnostic VALU throughput probe (see diag_kernels.hip): elapsed ms.
+float diag_valu_ms(int kind, int blocks, int iters, hipStream_t s);
+
 }  // namespace lm
