@@ -187,6 +187,11 @@ void launch_seg_store(const double* y, double* old, const SegLayout& g, const Se
                       hipStream_t s);
 // y <- 0 for systems with check == 3
 void launch_seg_clear(double* y, const SegLayout& g, const SegScalars& sc, hipStream_t s);
+// launch_chunk_reduce over the particle blocks of systems with active != 0
+// only (same summation order); rows of other systems are left as they are.
+void launch_seg_chunk_reduce(const double* part, int nchunks, int64_t n, double* out,
+                             const SegBlock* blocks, int nblocks, const int* active,
+                             hipStream_t s);
 
 // Diagnostic VALU throughput probe (see diag_kernels.hip): elapsed ms.
 float diag_valu_ms(int kind, int blocks, int iters, hipStream_t s);

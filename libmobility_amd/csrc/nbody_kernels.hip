@@ -443,6 +443,30 @@ void launch_chunk_reduce(const double* part, int nch, int64_t n, double* out, hi
   LM_HIP(hipGetLastError());
 }
 
+// One workgroup per particle block; blocks of converged systems return at once.
+__global__ __launch_bounds__(256) void k_seg_chunk_reduce(const double* __restrict__ part,
+                                                          int nch, int64_t n,
+                                                          double* __restrict__ out,
+                                                          const SegBlock* __restrict__ blocks,
+                                                          const int* __restrict__ active) {
+  const SegBlock b = blocks[blockIdx.x];
+  if (active[b.sys] == 0) return;
+  for (int64_t i = 3 * b.p0 + threadIdx.x; i < 3 * b.p1; i += blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) s += part[(int64_t)c * n + i];
+    out[i] = s;
+  }
+}
+
+void launch_seg_chunk_reduce(const double* part, int nch, int64_t n, double* out,
+                             const SegBlock* blocks, int nblocks, const int* active,
+                             hipStream_t s) {
+  if (n <= 0 || nblocks <= 0) return;
+  hipLaunchKernelGGL(k_seg_chunk_reduce, dim3((unsigned)nblocks), dim3(256), 0, s, part, nch, n,
+                     out, blocks, active);
+  LM_HIP(hipGetLastError());
+}
+
 // ---- setPositions -----------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_system_bbox(const double* __restrict__ pos,

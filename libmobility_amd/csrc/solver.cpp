@@ -215,12 +215,13 @@ void NBodySolver::setPositions(const double* pos, int64_t n3, bool dev) {
 // Source chunks per target block: the hint (auto: about 8192 items, ~32
 // workgroups per CU for a short tail on every XCD), clamped to [1, 32] and to
 // one chunk per 256 sources of the largest system.
-int NBodySolver::resolve_nch(int tpl, int nch_hint) const {
+int NBodySolver::resolve_nch(int tpl, int nch_hint, const int* act) const {
   const std::vector<int64_t>& off = sys_off_host_;
   const int nsys = (int)off.size() - 1;
   const int64_t bt = (int64_t)kPairThreads * tpl;
   int64_t tiles = 0, maxn = 0;
   for (int s = 0; s < nsys; ++s) {
+    if (act && !act[s]) continue;
     const int64_t ns = off[s + 1] - off[s];
     tiles += (ns + bt - 1) / bt;
     maxn = std::max(maxn, ns);
@@ -235,12 +236,14 @@ int NBodySolver::resolve_nch(int tpl, int nch_hint) const {
 }
 
 // Work items: per system, target blocks of 256 * tpl targets x nch source chunks.
-void NBodySolver::build_items(int tpl, int nch, bool persist, std::vector<PairItem>& items) const {
+void NBodySolver::build_items(int tpl, int nch, bool persist, std::vector<PairItem>& items,
+                              const int* act) const {
   const std::vector<int64_t>& off = sys_off_host_;
   const int nsys = (int)off.size() - 1;
   const int64_t bt = (int64_t)kPairThreads * tpl;
   items.clear();
   for (int s = 0; s < nsys; ++s) {
+    if (act && !act[s]) continue;
     const int64_t b = off[s], e = off[s + 1], ns = e - b;
     if (ns <= 0) continue;
     const int kind = s < (int)sys_kind_host_.size() ? sys_kind_host_[s] : (wall_ ? 1 : 0);
@@ -314,9 +317,11 @@ void NBodySolver::setPairTiling(int tpl, int nch) {
   items_tpl_ = -1;
 }
 
-// Launch the pair kernel over the current work list (items_).
-void NBodySolver::launch_pair_items(int blocks, bool fp32, const void* rec, const double* F,
-                                    const double* T, double* U, double* W) {
+// Launch the pair kernel over a work list of `count` items (tpl targets per
+// lane, nch source chunks).
+void NBodySolver::launch_pair_items(const PairItem* list, int count, int tpl, int nch, int blocks,
+                                    bool fp32, const void* rec, const double* F, const double* T,
+                                    double* U, double* W) {
   const int64_t n3 = 3 * N_;
   PairArgs a{};
   a.wall = kind_code();
@@ -324,36 +329,92 @@ void NBodySolver::launch_pair_items(int blocks, bool fp32, const void* rec, cons
   if (a.persistent) {
     work_counter_.reserve(sizeof(unsigned int) * 64);
     a.counter = work_counter_.as<unsigned int>();
-    a.grid = std::min(items_count_, 1024);  // 4 workgroups per CU drain the queue
+    a.grid = std::min(count, 1024);  // 4 workgroups per CU drain the queue
   }
   a.fp32 = fp32 ? 1 : 0;
   a.blocks = blocks;
-  a.tpl = items_tpl_;
+  a.tpl = tpl;
   a.pos = rec;
   a.F = F;
   a.T = (blocks & (kUT | kWT)) ? T : nullptr;
-  a.items = items_.as<PairItem>();
-  a.nitems = items_count_;
+  a.items = list;
+  a.nitems = count;
   a.active = pair_active_;
   a.t0 = t0_;
   a.rt0 = rt0_;
   a.r0 = r0_;
   const bool ang = (blocks & (kWT | kWF)) != 0;
-  if (items_nch_ == 1) {
+  if (nch == 1) {
     a.U = U;
     a.W = ang ? W : nullptr;
     a.slab = 0;
     launch_pair(a, stream_);
   } else {
-    part_u_.reserve(sizeof(double) * n3 * items_nch_);
-    if (ang) part_w_.reserve(sizeof(double) * n3 * items_nch_);
+    part_u_.reserve(sizeof(double) * n3 * nch);
+    if (ang) part_w_.reserve(sizeof(double) * n3 * nch);
     a.U = part_u_.as<double>();
     a.W = ang ? part_w_.as<double>() : nullptr;
     a.slab = n3;
     launch_pair(a, stream_);
-    launch_chunk_reduce(part_u_.as<double>(), items_nch_, n3, U, stream_);
-    if (ang) launch_chunk_reduce(part_w_.as<double>(), items_nch_, n3, W, stream_);
+    auto reduce = [&](const double* part, double* out) {
+      if (seg_blocks_ && pair_active_)
+        launch_seg_chunk_reduce(part, nch, n3, out, seg_blocks_, seg_nblocks_, pair_active_,
+                                stream_);
+      else
+        launch_chunk_reduce(part, nch, n3, out, stream_);
+    };
+    reduce(part_u_.as<double>(), U);
+    if (ang) reduce(part_w_.as<double>(), W);
   }
+}
+
+// Work list of the systems with act[s] != 0: the full list's tiling while it
+// still gives every CU several items (filtered from the complete host list,
+// cached per tiling and already in work-queue order, so a rebuild is one pass
+// over the items), else one target per lane and enough source chunks for
+// about 8192 items, built for the few systems left.
+const std::vector<PairItem>& NBodySolver::host_items(int tpl, int nch) {
+  if (host_items_layout_ != layout_id_ || host_items_N_ != N_) {
+    host_items_.clear();
+    host_items_layout_ = layout_id_;
+    host_items_N_ = N_;
+  }
+  auto& v = host_items_[{tpl, nch, use_persistent() ? 1 : 0}];
+  if (v.empty()) build_items(tpl, nch, use_persistent(), v);
+  return v;
+}
+
+void NBodySolver::build_active_items(const int* act) {
+  int tpl = items_tpl_ > 0 ? items_tpl_ : 1;
+  int nch = items_nch_ > 0 ? items_nch_ : resolve_nch(tpl, 0, act);
+  auto count_active = [&](const std::vector<PairItem>& all) {
+    size_t c = 0;
+    for (const PairItem& it : all) c += act[it.sys] ? 1 : 0;
+    return c;
+  };
+  const std::vector<PairItem>* all = &host_items(tpl, nch);
+  std::vector<PairItem> fine;
+  if (count_active(*all) < 2048) {
+    tpl = 1;
+    nch = resolve_nch(1, 0, act);
+    build_items(tpl, nch, use_persistent(), fine, act);
+    all = &fine;
+  }
+  // the previous upload from the pinned buffer has completed: the caller
+  // synchronised the stream after it
+  seg_items_host_.reserve(sizeof(PairItem) * std::max<size_t>(all->size(), 1));
+  PairItem* dst = seg_items_host_.as<PairItem>();
+  int count = 0;
+  for (const PairItem& it : *all)
+    if (act[it.sys]) dst[count++] = it;
+  seg_items_.count = 0;
+  if (count == 0) return;
+  const size_t bytes = sizeof(PairItem) * (size_t)count;
+  seg_items_.buf.reserve(bytes);
+  LM_HIP(hipMemcpyAsync(seg_items_.buf.as<void>(), dst, bytes, hipMemcpyHostToDevice, stream_));
+  seg_items_.tpl = tpl;
+  seg_items_.nch = nch;
+  seg_items_.count = count;
 }
 
 // Tiling: user override, else the autotuned choice, else the default rule.
@@ -406,10 +467,10 @@ void NBodySolver::choose_tiling(int blocks, bool fp32, const double* F, const do
     for (int ci = 0; ci < 6; ++ci) {
       ensure_items(tpls[ti], nchs[ci]);
       if (nchs[ci] != 0 && items_nch_ != nchs[ci]) continue;  // clamped duplicate
-      launch_pair_items(blocks, fp32, rec, F, T, U, W);
+      launch_main_items(blocks, fp32, rec, F, T, U, W);
       LM_HIP(hipEventRecord(e0, stream_));
-      launch_pair_items(blocks, fp32, rec, F, T, U, W);
-      launch_pair_items(blocks, fp32, rec, F, T, U, W);
+      launch_main_items(blocks, fp32, rec, F, T, U, W);
+      launch_main_items(blocks, fp32, rec, F, T, U, W);
       LM_HIP(hipEventRecord(e1, stream_));
       LM_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -447,11 +508,17 @@ void NBodySolver::pair_product(const void* rec64, const void* rec32, const doubl
     throw Error("[libmobility_amd] the solver includes angular velocities: an angular output array is required");
   const bool fp32 = pair_prec_ == kPairFP32;
   const void* rec = fp32 ? rec32 : rec64;
-  int tpl = 1, nch = 0;
-  choose_tiling(blocks, fp32, F, T, U, W, rec, tpl, nch);
-  ensure_items(tpl, nch);
-  if (items_count_ <= 0) throw Error("[libmobility_amd] empty pair work list");
-  launch_pair_items(blocks, fp32, rec, F, T, U, W);
+  if (seg_items_.count > 0) {
+    // per-system Lanczos after some systems converged: their work is dropped
+    launch_pair_items(seg_items_.buf.as<PairItem>(), seg_items_.count, seg_items_.tpl,
+                      seg_items_.nch, blocks, fp32, rec, F, T, U, W);
+  } else {
+    int tpl = 1, nch = 0;
+    choose_tiling(blocks, fp32, F, T, U, W, rec, tpl, nch);
+    ensure_items(tpl, nch);
+    if (items_count_ <= 0) throw Error("[libmobility_amd] empty pair work list");
+    launch_main_items(blocks, fp32, rec, F, T, U, W);
+  }
   if (angular_ && W && !ang) launch_zero(W, n3, stream_);
 }
 
@@ -771,10 +838,18 @@ int NBodySolver::lanczos_run_seg(SegSolve& S, const std::vector<int64_t>& off, b
     }
   }
   struct ActiveGuard {
-    const int*& p;
-    ~ActiveGuard() { p = nullptr; }
-  } guard{pair_active_};
-  if (skip_converged_pairs) pair_active_ = d_active;
+    NBodySolver& s;
+    ~ActiveGuard() {
+      s.pair_active_ = nullptr;
+      s.seg_blocks_ = nullptr;
+      s.seg_items_.count = 0;
+    }
+  } guard{*this};
+  if (skip_converged_pairs) {
+    pair_active_ = d_active;
+    seg_blocks_ = g.blocks;
+    seg_nblocks_ = g.nblocks;
+  }
   int last = 0;
   for (int i = 0; i < iter_limit_ && nact > 0; ++i) {
     ensure_basis(n, i + 2);
@@ -833,8 +908,10 @@ int NBodySolver::lanczos_run_seg(SegSolve& S, const std::vector<int64_t>& off, b
     LM_HIP(hipMemcpyAsync(d_check, h_check_store, sizeof(int) * nrec, hipMemcpyHostToDevice,
                           stream_));
     launch_seg_store(y, old, g, sc, stream_);
-    if (changed)
+    if (changed) {
       LM_HIP(hipMemcpyAsync(d_active, h_active, sizeof(int) * nrec, hipMemcpyHostToDevice, stream_));
+      if (skip_converged_pairs && nact > 0) build_active_items(h_active);
+    }
   }
   if (nact > 0) throw Error("[Lanczos] Could not converge");
   last_iters_ = last;

@@ -131,8 +131,13 @@ class NBodySolver {
   void ensure_items(int tpl, int nch_hint);
   void choose_tiling(int blocks, bool fp32, const double* F, const double* T, double* U,
                      double* W, const void* rec, int& tpl, int& nch);
-  void launch_pair_items(int blocks, bool fp32, const void* rec, const double* F,
-                         const double* T, double* U, double* W);
+  void launch_pair_items(const PairItem* list, int count, int tpl, int nch, int blocks, bool fp32,
+                         const void* rec, const double* F, const double* T, double* U, double* W);
+  void launch_main_items(int blocks, bool fp32, const void* rec, const double* F, const double* T,
+                         double* U, double* W) {
+    launch_pair_items(items_.as<PairItem>(), items_count_, items_tpl_, items_nch_, blocks, fp32,
+                      rec, F, T, U, W);
+  }
   void pair_product(const void* rec64, const void* rec32, const double* F, const double* T,
                     double* U, double* W);
   int lanczos_run(const double* z, double** result);
@@ -207,14 +212,28 @@ class NBodySolver {
   // per-system Lanczos
   bool seg_mode_ = false;
   const int* pair_active_ = nullptr;  // per-system flags seen by the pair kernel
+  // Pair work of the systems still iterating (rebuilt when a system
+  // converges; count 0: the full list) and the particle blocks the chunk sum
+  // is restricted to.
+  ItemList seg_items_;
+  HostBuf seg_items_host_;
+  const SegBlock* seg_blocks_ = nullptr;
+  int seg_nblocks_ = 0;
+  void build_active_items(const int* act);
+  std::map<std::tuple<int, int, int>, std::vector<PairItem>> host_items_;
+  uint64_t host_items_layout_ = 0;
+  int64_t host_items_N_ = -1;
+  const std::vector<PairItem>& host_items(int tpl, int nch);
   SegSolve seg_sys_, seg_multi_;
   DevBuf seg_w_;
   // multi-vector products
   ItemList multi_items_[2];  // 4 vectors (one target per lane), 2 vectors (two)
   DevBuf part_multi_, multi_f_, multi_u_, multi_noise_, multi_t_, multi_w_;
   int multi_nrhs_ = 0, multi_per_ = 1;
-  int resolve_nch(int tpl, int nch_hint) const;
-  void build_items(int tpl, int nch, bool persist, std::vector<PairItem>& items) const;
+  // act (host, per system; nullptr: all): only systems with act[s] != 0
+  int resolve_nch(int tpl, int nch_hint, const int* act = nullptr) const;
+  void build_items(int tpl, int nch, bool persist, std::vector<PairItem>& items,
+                   const int* act = nullptr) const;
   void ensure_multi_items(ItemList& L, int tpl);
   void multi_product(const double* v, int nrhs, double* w);
   // learned preconditioner

@@ -148,6 +148,56 @@ def test_per_system_lanczos_matches_separate_solvers(angular):
     assert ens.lastLanczosIterations() == max(its)
 
 
+def _rsa_ensemble(seed, sizes, a=1.1, phi=0.15):
+    """Non-overlapping spheres (volume fraction phi) per system; wall systems in
+    a slab 1.2 <= z < 9.2 above the wall at z = 0.  Overlapping random clouds
+    make M so ill-conditioned that the Lanczos stopping step reacts to
+    rounding; these systems converge in 30-45 steps whatever the rounding."""
+    from libmobility_amd.workloads import rsa_box, rsa_box_side
+    kinds = [s % 2 for s in range(len(sizes))]
+    pos = []
+    for k, (sz, kind) in enumerate(zip(sizes, kinds)):
+        if kind:
+            lxy = np.sqrt(sz * 4 * np.pi * a ** 3 / 3 / phi / 8.0)
+            pos.append(rsa_box(sz, (lxy, lxy, 8.0), seed + k, a=a, origin=(0.0, 0.0, 1.2)))
+        else:
+            L = rsa_box_side(sz, phi, a)
+            pos.append(rsa_box(sz, (L, L, L), seed + k, a=a))
+    return list(sizes), kinds, np.concatenate(pos)
+
+
+@pytest.mark.parametrize("angular,persistent", [(False, 1), (False, 0), (True, 1)])
+def test_per_system_lanczos_large_systems_match_separate_solvers(angular, persistent):
+    """Systems of several source chunks: once a system converges the pair work
+    list is rebuilt for the systems still iterating (finer tiles, fewer
+    chunks) and the chunk sum covers only their particles; every system still
+    equals its own solver (LanczosAlgorithm.cu:171-232)."""
+    sizes, kinds, pos = _rsa_ensemble(seed=21, sizes=(2100, 40, 900, 1300, 5))
+    N = pos.shape[0]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    tol = 1e-6
+    ens = _solver(sizes, kinds, pos, angular=angular, tol=tol, persistent=persistent)
+    ens.setLanczosPerSystem(True)
+    twin = _solver(sizes, kinds, pos, angular=angular, tol=tol, persistent=persistent)
+    twin.setLanczosPerSystem(True)
+    singles = [_single(pos[a:b], k, angular, tol) for a, b, k in zip(off[:-1], off[1:], kinds)]
+    n = (6 if angular else 3) * N
+    rng = np.random.default_rng(5)
+    for call in range(2):
+        W = rng.normal(size=n)
+        u, w = ens.sqrtMdotW_with_noise(W)
+        u2, w2 = twin.sqrtMdotW_with_noise(W)
+        assert np.array_equal(u, u2)  # the rebuilt work lists are deterministic
+        got = _split(np.concatenate([u.ravel(), w.ravel()]) if angular else u, off, N, angular)
+        its = ens.lastLanczosIterationsPerSystem()
+        assert len(set(its.tolist())) > 1  # systems left the recurrence at different steps
+        for s, (Ws, g) in enumerate(zip(_split(W, off, N, angular), got)):
+            us, ws = singles[s].sqrtMdotW_with_noise(Ws)
+            one = np.concatenate([us.ravel(), ws.ravel()]) if angular else us.ravel()
+            assert abs(its[s] - singles[s].lastLanczosIterations()) <= 6, (call, s)
+            assert np.linalg.norm(g - one) <= 10 * tol * np.linalg.norm(one), (call, s)
+
+
 def test_per_system_lanczos_with_preconditioner_is_a_square_root():
     from libmobility_amd.precond import LearnedPreconditioner
     sizes, kinds, pos = _ensemble(seed=12, sizes=(20, 14, 9))
