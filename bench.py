@@ -109,6 +109,12 @@ class Gpu:
             ffi.check(lib.lm_nbody_set_autotune(h, int(autotune)))
         return h
 
+    def evaluation(self, h):
+        """'symmetric' if the last product evaluated each unordered pair once."""
+        used = ctypes.c_int()
+        self.ffi.check(self.lib.lm_nbody_last_pair_symmetric(h, ctypes.byref(used)))
+        return "symmetric" if used.value else "ordered"
+
     def tiling(self, h):
         t, c = ctypes.c_int(), ctypes.c_int()
         self.ffi.check(self.lib.lm_nbody_get_pair_tiling(h, ctypes.byref(t), ctypes.byref(c)))
@@ -331,6 +337,8 @@ def bench_c3(args):
     log(f"  {ms64:.3f} ms/Mdot, {gpair64:.1f} Gpair/s")
 
     extra = {
+        "pair_evaluation_fp32pair": g.evaluation(h32),
+        "pair_evaluation_fp64pair": g.evaluation(h64),
         "tiling_fp32pair": g.tiling(h32),
         "tiling_fp64pair": g.tiling(h64),
         "mdot_fp32pair_ms": ms32,
@@ -340,6 +348,15 @@ def bench_c3(args):
         "flops_per_pair": fpp,
         "mdot_rel_err_fp32pair_vs_fp64pair": rel(U32, U64),
     }
+    if extra["pair_evaluation_fp32pair"] == "symmetric":
+        log("FP32-pair Mdot, ordered evaluation (every ordered pair, as the reference)")
+        ho = g.solver(pos_d, "fp32", tiling=args.tiling)
+        g.ffi.check(g.lib.lm_nbody_set_pair_symmetric(ho, 0))
+        ms_o = g.time_events(lambda: g.mdot(ho, F_d, None, U_d), max(3, args.steps // 4), 1)
+        g.destroy(ho)
+        extra["mdot_fp32pair_ordered_ms"] = ms_o
+        extra["mdot_fp32pair_symmetric_speedup"] = ms_o / ms32
+        log(f"  {ms_o:.3f} ms/Mdot ordered; symmetric {ms_o / ms32:.2f}x faster")
     if not args.no_lanczos:
         log("plain Lanczos sqrtMdotW, FP32 pairs, tol 1e-6")
         ms_l32, it32 = g.time_lanczos(h32, W_d, 3 * N, args.lanczos_calls)

@@ -86,6 +86,17 @@ int lm_nbody_set_pair_precision(lm_nbody* h, int precision);
  * reference's blockDim choice, NbodyRPY.cu:105-108). */
 int lm_nbody_set_pair_tiling(lm_nbody* h, int targets_per_lane, int source_chunks);
 
+/* Extension: symmetric evaluation of the open-boundary translational block
+ * of a single system (-1 auto, 0 off, 1 on).  M_ij = M_ji for open RPY, so
+ * each unordered pair is evaluated once and applied to both particles; the
+ * reference evaluates every ordered pair (NbodyRPY.cu:27-97).  Auto uses it
+ * from N = 32768 while its partial-sum slabs fit in 4 GiB.  Results agree
+ * with the ordered evaluation to rounding and are bitwise reproducible. */
+int lm_nbody_set_pair_symmetric(lm_nbody* h, int mode);
+
+/* Extension: 1 if the last pair product used the symmetric evaluation. */
+int lm_nbody_last_pair_symmetric(lm_nbody* h, int* used);
+
 /* Extension: when enabled, the first product for a new (N, systems, kernel,
  * precision, blocks) combination times every tiling candidate on the device
  * and keeps the fastest for later calls. */

@@ -78,6 +78,17 @@ int lm_nbody_set_pair_tiling(lm_nbody* h, int tpl, int nch) {
   return guarded([&] { S(h).setPairTiling(tpl, nch); });
 }
 
+int lm_nbody_set_pair_symmetric(lm_nbody* h, int mode) {
+  return guarded([&] { S(h).setPairSymmetric(mode); });
+}
+
+int lm_nbody_last_pair_symmetric(lm_nbody* h, int* used) {
+  return guarded([&] {
+    if (!used) throw lm::Error("[libmobility_amd] null output");
+    *used = S(h).lastPairSymmetric() ? 1 : 0;
+  });
+}
+
 int lm_nbody_set_autotune(lm_nbody* h, int enable) {
   return guarded([&] { S(h).setAutotune(enable != 0); });
 }

@@ -58,6 +58,39 @@ struct PairMultiArgs {
   double t0;
 };
 constexpr int multi_tpl(int nrhs) { return nrhs >= 4 ? 1 : 2; }
+
+// Symmetric evaluation of the open-boundary translational block of one
+// system: M_ji = M_ij, so each unordered pair is evaluated once and applied to
+// both particles.  Tiles of 256 * tpl particles; tile I meets tiles I + o
+// (mod ntiles) for o = 1 .. hmax, hmax = ntiles / 2 (offset ntiles / 2 of an
+// even count only from I < ntiles / 2), so every unordered tile pair is met
+// once.  Target-side sums of an item go to target slab `chunk`, source-side
+// sums of offset o to source slab o - 1 (each row of it written by exactly one
+// item); the diagonal tile (o0 = 0) is an ordinary item into target slab
+// `nchunks`.
+struct SymItem {
+  int32_t tile;    // target tile I
+  int32_t o0, o1;  // partner offsets [o0, o1); o0 = 0: the diagonal tile
+  int32_t chunk;   // target slab
+};
+struct SymArgs {
+  int fp32;
+  int tpl;
+  const void* pos;   // float4 / double4 records
+  const double* F;   // 3N AoS
+  double* slab_t;    // nchunks + 1 target slabs of n3
+  double* slab_s;    // hmax source slabs of n3
+  int64_t n3, N;
+  int ntiles, hmax;
+  const SymItem* items;
+  int nitems;
+  double t0;
+};
+void launch_pair_sym(const SymArgs& a, hipStream_t s);
+// out = sum of the nt target slabs, then of the hmax source slabs (rows a
+// source slab does not hold are skipped), in that fixed order.
+void launch_sym_reduce(const double* slab_t, int nt, const double* slab_s, int hmax, int64_t n3,
+                       int tpl, int ntiles, double* out, hipStream_t s);
 void launch_pair_multi(const PairMultiArgs& a, hipStream_t s);
 void launch_chunk_reduce(const double* part, int nchunks, int64_t n, double* out, hipStream_t s);
 

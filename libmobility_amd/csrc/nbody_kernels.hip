@@ -217,6 +217,223 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(
   }
 }
 
+// ---- symmetric evaluation (open boundaries, translational block) --------------
+// Open RPY depends on r only and is even in d, so M_ji F_i uses the pair's
+// coefficients unchanged: fuf F_i + guf (d.F_i) d.  An item holds the 256*TPL
+// targets of tile I in the lanes and streams each partner tile J through LDS
+// in sub-tiles of 256 sources (four blocks of 64, each stored twice).  Lane l
+// meets source (l + k) mod 64 of a block at step k (address l + k) and carries
+// that source's running sum, which moves one lane down after every step
+// (DPP wave_rol:1), so after 64 steps every source has met all targets of the
+// wave and its sum sits back in lane (source index).  The four waves' sums are
+// added in a fixed order and written once per source to the offset's slab.
+
+// Lane l receives lane (l + 1) mod 64.
+template <class R>
+__device__ inline R lane_rol1(R v);
+template <>
+__device__ inline float lane_rol1<float>(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xf, 0xf, true));
+}
+template <>
+__device__ inline double lane_rol1<double>(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, 0x134, 0xf, 0xf, true);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x134, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+template <class R, int TPL>
+__device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Rec<R>::T* s_pos,
+                                typename Rec<R>::T* s_f, R* s_red) {
+  using R4 = typename Rec<R>::T;
+  constexpr int kTile = kPairThreads * TPL;
+  constexpr int SG = TPL >= 4 ? 1 : 4 / TPL;  // steps per overlap test (4 pairs)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const R4* __restrict__ pos = static_cast<const R4*>(a.pos);
+  const double* __restrict__ F = a.F;
+  const int64_t N = a.N;
+  const int64_t tgt0 = (int64_t)it.tile * kTile;
+
+  R4 pi[TPL];
+  R fix[TPL], fiy[TPL], fiz[TPL];
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int64_t i = tgt0 + tid + t * kPairThreads;
+    if (i < N) {
+      pi[t] = pos[i];
+      fix[t] = R(F[3 * i]);
+      fiy[t] = R(F[3 * i + 1]);
+      fiz[t] = R(F[3 * i + 2]);
+    } else {
+      pi[t] = far_record<R4>();
+      fix[t] = fiy[t] = fiz[t] = R(0);
+    }
+  }
+  Vec3<R> acc[TPL];
+  Vec3<double> acc64[TPL];
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    acc[t] = {R(0), R(0), R(0)};
+    acc64[t] = {0.0, 0.0, 0.0};
+  }
+
+  for (int o = it.o0; o < it.o1; ++o) {
+    if (2 * o == a.ntiles && 2 * it.tile >= a.ntiles) continue;  // met from the other side
+    int J = it.tile + o;
+    if (J >= a.ntiles) J -= a.ntiles;
+    double* __restrict__ So = a.slab_s + (int64_t)(o - 1) * a.n3;
+    for (int sub = 0; sub < TPL; ++sub) {
+      const int64_t base = (int64_t)J * kTile + sub * kPairThreads;
+      if (base >= N) break;
+      const int64_t j = base + tid;
+      __syncthreads();  // the previous sub-tile is done with s_pos, s_f and s_red
+      R4 pj = far_record<R4>();
+      R4 fj = R4{R(0), R(0), R(0), R(0)};
+      if (j < N) {
+        pj = pos[j];
+        fj = R4{R(F[3 * j]), R(F[3 * j + 1]), R(F[3 * j + 2]), R(0)};
+      }
+      s_pos[wave * 128 + lane] = pj;
+      s_pos[wave * 128 + 64 + lane] = pj;
+      s_f[wave * 128 + lane] = fj;
+      s_f[wave * 128 + 64 + lane] = fj;
+      __syncthreads();
+      for (int blk = 0; blk < 4; ++blk) {
+        const R4* sp = s_pos + blk * 128 + lane;
+        const R4* sf = s_f + blk * 128 + lane;
+        Vec3<R> sa = {R(0), R(0), R(0)};
+#pragma unroll 2
+        for (int k = 0; k < 64; k += SG) {
+          R dx[SG][TPL], dy[SG][TPL], dz[SG][TPL], r2[SG][TPL], ir[SG][TPL];
+          RpyCoef<R> cf[SG][TPL];
+          R4 pg[SG], fg[SG];
+          bool near = false;
+#pragma unroll
+          for (int g = 0; g < SG; ++g) {
+            pg[g] = sp[k + g];
+            fg[g] = sf[k + g];
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) {
+              dx[g][t] = pi[t].x - pg[g].x;
+              dy[g][t] = pi[t].y - pg[g].y;
+              dz[g][t] = pi[t].z - pg[g].z;
+              r2[g][t] = dx[g][t] * dx[g][t] + dy[g][t] * dy[g][t] + dz[g][t] * dz[g][t];
+              ir[g][t] = lm_rsqrt(r2[g][t]);
+              cf[g][t] = RpyCoef<R>{};
+              rpy_far<R, kUF>(ir[g][t], cf[g][t]);
+              near |= r2[g][t] < R(4);
+            }
+          }
+          if (__builtin_expect(near, 0)) {
+            __asm__ __volatile__("");  // keep this a branch
+#pragma unroll
+            for (int g = 0; g < SG; ++g)
+#pragma unroll
+              for (int t = 0; t < TPL; ++t) rpy_near_select<R, kUF>(r2[g][t], ir[g][t], cf[g][t]);
+          }
+#pragma unroll
+          for (int g = 0; g < SG; ++g) {
+            // this step's source contribution first; only the final add (with
+            // the move) depends on the previous step
+            Vec3<R> c;
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) {
+              const R fuf = cf[g][t].fuf, guf = cf[g][t].guf;
+              const R st = guf * (dx[g][t] * fg[g].x + dy[g][t] * fg[g].y + dz[g][t] * fg[g].z);
+              acc[t].x = lm_fma(st, dx[g][t], lm_fma(fuf, fg[g].x, acc[t].x));
+              acc[t].y = lm_fma(st, dy[g][t], lm_fma(fuf, fg[g].y, acc[t].y));
+              acc[t].z = lm_fma(st, dz[g][t], lm_fma(fuf, fg[g].z, acc[t].z));
+              const R ss = guf * (dx[g][t] * fix[t] + dy[g][t] * fiy[t] + dz[g][t] * fiz[t]);
+              if (t == 0) {
+                c.x = lm_fma(ss, dx[g][t], fuf * fix[t]);
+                c.y = lm_fma(ss, dy[g][t], fuf * fiy[t]);
+                c.z = lm_fma(ss, dz[g][t], fuf * fiz[t]);
+              } else {
+                c.x = lm_fma(ss, dx[g][t], lm_fma(fuf, fix[t], c.x));
+                c.y = lm_fma(ss, dy[g][t], lm_fma(fuf, fiy[t], c.y));
+                c.z = lm_fma(ss, dz[g][t], lm_fma(fuf, fiz[t], c.z));
+              }
+            }
+            sa.x = lane_rol1(sa.x) + c.x;
+            sa.y = lane_rol1(sa.y) + c.y;
+            sa.z = lane_rol1(sa.z) + c.z;
+          }
+        }
+        // after 64 steps lane l holds source l - 1: one more move puts source l in lane l
+        s_red[(wave * 3 + 0) * kPairThreads + blk * 64 + lane] = lane_rol1(sa.x);
+        s_red[(wave * 3 + 1) * kPairThreads + blk * 64 + lane] = lane_rol1(sa.y);
+        s_red[(wave * 3 + 2) * kPairThreads + blk * 64 + lane] = lane_rol1(sa.z);
+      }
+      __syncthreads();
+      if (j < N) {
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          vx += (double)s_red[(w * 3 + 0) * kPairThreads + tid];
+          vy += (double)s_red[(w * 3 + 1) * kPairThreads + tid];
+          vz += (double)s_red[(w * 3 + 2) * kPairThreads + tid];
+        }
+        So[3 * j + 0] = a.t0 * vx;
+        So[3 * j + 1] = a.t0 * vy;
+        So[3 * j + 2] = a.t0 * vz;
+      }
+      if constexpr (std::is_same<R, float>::value) {
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+          acc64[t].x += acc[t].x;
+          acc64[t].y += acc[t].y;
+          acc64[t].z += acc[t].z;
+          acc[t] = {R(0), R(0), R(0)};
+        }
+      }
+    }
+  }
+  if constexpr (std::is_same<R, double>::value) {
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) acc64[t] = acc[t];
+  }
+  double* __restrict__ Uo = a.slab_t + (int64_t)it.chunk * a.n3;
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int64_t i = tgt0 + tid + t * kPairThreads;
+    if (i < N) {
+      Uo[3 * i + 0] = a.t0 * acc64[t].x;
+      Uo[3 * i + 1] = a.t0 * acc64[t].y;
+      Uo[3 * i + 2] = a.t0 * acc64[t].z;
+    }
+  }
+}
+
+// One item per workgroup: a diagonal tile (all ordered pairs of tile I, the
+// self pairs included) or a range of partner offsets.
+template <class R, int TPL>
+__global__ __launch_bounds__(kPairThreads) void k_pair_sym(SymArgs a) {
+  using R4 = typename Rec<R>::T;
+  __shared__ R4 s_pos[4 * 128];
+  __shared__ R4 s_f[4 * 128];
+  __shared__ R s_red[4 * 3 * kPairThreads];
+  const SymItem it = a.items[blockIdx.x];
+  if (it.o0 == 0) {
+    constexpr int kTile = kPairThreads * TPL;
+    PairItem p;
+    p.tgt0 = (int64_t)it.tile * kTile;
+    p.ntgt = (int32_t)(a.N - p.tgt0 < kTile ? a.N - p.tgt0 : kTile);
+    p.src0 = p.tgt0;
+    p.src1 = p.tgt0 + p.ntgt;
+    p.chunk = it.chunk;
+    p.wall = 0;
+    p.sys = 0;
+    pair_item<R, false, kUF, TPL>(p, static_cast<const R4*>(a.pos), a.F, nullptr, a.slab_t, nullptr,
+                                  a.n3, a.t0, 0.0, 0.0, s_pos, s_f, s_f);
+    return;
+  }
+  sym_item<R, TPL>(it, a, s_pos, s_f, s_red);
+}
+
 // ---- several force vectors per pass (translational block) ----------------------
 // Same tiling as pair_item; each pair's geometry and coefficients are computed
 // once and applied to the NR force vectors staged side by side in LDS.
@@ -424,6 +641,51 @@ void launch_pair(const PairArgs& a, hipStream_t s) {
     launch_pair_kind<float>(a, s);
   else
     launch_pair_kind<double>(a, s);
+}
+
+template <class R>
+static void launch_pair_sym_r(const SymArgs& a, hipStream_t s) {
+  const dim3 grid((unsigned)a.nitems), block(kPairThreads);
+  switch (a.tpl) {
+    case 1: hipLaunchKernelGGL((k_pair_sym<R, 1>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_pair_sym<R, 2>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_pair_sym<R, 4>), grid, block, 0, s, a); break;
+    default: throw Error("[libmobility_amd] unsupported targets-per-lane for the symmetric kernel");
+  }
+  LM_HIP(hipGetLastError());
+}
+
+void launch_pair_sym(const SymArgs& a, hipStream_t s) {
+  if (a.nitems <= 0) return;
+  if (a.fp32)
+    launch_pair_sym_r<float>(a, s);
+  else
+    launch_pair_sym_r<double>(a, s);
+}
+
+__global__ void k_sym_reduce(const double* __restrict__ slab_t, int nt,
+                             const double* __restrict__ slab_s, int hmax, int64_t n, int64_t tile3,
+                             int ntiles, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int c = 0; c < nt; ++c) s += slab_t[(int64_t)c * n + i];
+  // an even tile count meets offset ntiles/2 only from tiles I < ntiles/2, so
+  // that slab holds no rows of tiles J < ntiles/2
+  const int tile = (int)(i / tile3);
+  const int skip = (ntiles % 2 == 0 && 2 * tile < ntiles) ? ntiles / 2 : 0;
+  for (int o = 1; o <= hmax; ++o)
+    if (o != skip) s += slab_s[(int64_t)(o - 1) * n + i];
+  out[i] = s;
+}
+
+void launch_sym_reduce(const double* slab_t, int nt, const double* slab_s, int hmax, int64_t n3,
+                       int tpl, int ntiles, double* out, hipStream_t s) {
+  if (n3 <= 0) return;
+  const int nt_ = 256;
+  hipLaunchKernelGGL(k_sym_reduce, dim3((unsigned)((n3 + nt_ - 1) / nt_)), dim3(nt_), 0, s, slab_t,
+                     nt, slab_s, hmax, n3, (int64_t)3 * kPairThreads * tpl, ntiles, out);
+  LM_HIP(hipGetLastError());
 }
 
 __global__ void k_chunk_reduce(const double* __restrict__ part, int nch, int64_t n,

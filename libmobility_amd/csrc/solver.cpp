@@ -308,6 +308,141 @@ void NBodySolver::ensure_multi_items(ItemList& L, int tpl) {
   L.layout = layout_id_;
 }
 
+void NBodySolver::setPairSymmetric(int mode) {
+  if (mode < -1 || mode > 1)
+    throw Error("[libmobility_amd] symmetric pair mode must be -1 (auto), 0 (off) or 1 (on)");
+  sym_mode_ = mode;
+}
+
+// Shape of a symmetric tiling request (0 = default rule): targets per lane
+// (2 from N = 65536, else 1) and offset chunks per tile (about 8192 items),
+// at most hmax.
+void NBodySolver::sym_shape(int tpl_req, int nch_req, int& tpl, int& nch, int& ntiles,
+                            int& hmax) const {
+  tpl = tpl_req != 0 ? tpl_req : (N_ >= 65536 ? 2 : 1);
+  const int64_t bt = (int64_t)kPairThreads * tpl;
+  ntiles = (int)((N_ + bt - 1) / bt);
+  hmax = ntiles / 2;
+  nch = nch_req != 0 ? nch_req : (8192 + ntiles - 1) / ntiles;
+  nch = std::min(nch, hmax);
+  if (hmax > 0) nch = std::max(nch, 1);
+}
+
+// The user's tiling, else the autotuned one for this N and precision, else
+// the default rule.
+void NBodySolver::sym_tiling(bool fp32, int& tpl, int& nch, int& ntiles, int& hmax) const {
+  int tr = tile_tpl_, cr = tile_nch_;
+  if (tr == 0 && cr == 0 && sym_tuned_N_[fp32 ? 1 : 0] == N_) {
+    tr = sym_tuned_[fp32 ? 1 : 0].first;
+    cr = sym_tuned_[fp32 ? 1 : 0].second;
+  }
+  sym_shape(tr, cr, tpl, nch, ntiles, hmax);
+}
+
+bool NBodySolver::use_sym(int blocks) const {
+  if (sym_mode_ == 0 || blocks != kUF || kind_code() != 0 || sys_off_host_.size() != 2 || N_ <= 0)
+    return false;
+  if (sym_mode_ == 1) return true;
+  if (N_ < 32768) return false;
+  // the largest slab set the tilings (default, autotuned or user) can ask for
+  int tpl, nch, nt, h;
+  sym_shape(tile_tpl_ != 0 ? tile_tpl_ : 1, 64, tpl, nch, nt, h);
+  const double bytes = 8.0 * 3.0 * (double)N_ * (double)(nch + 1 + h);
+  return bytes <= 4.0 * 1024.0 * 1024.0 * 1024.0;
+}
+
+// Items: for every tile, nch ranges of partner offsets, then the diagonal
+// tiles (lighter, they fill the tail).  Slabs: nch + 1 target slabs, then
+// hmax source slabs, summed per row in that order.
+void NBodySolver::run_sym(int tpl, int nch, int nt, int h, bool fp32, const void* rec,
+                          const double* F, double* U) {
+  if (sym_.N != N_ || sym_.tpl != tpl || sym_.nchunks != nch) {
+    std::vector<SymItem> items;
+    items.reserve((size_t)nt * (nch + 1));
+    for (int I = 0; I < nt; ++I)
+      for (int c = 0; c < nch; ++c) {
+        SymItem it;
+        it.tile = I;
+        it.o0 = 1 + (int)((int64_t)c * h / nch);
+        it.o1 = 1 + (int)((int64_t)(c + 1) * h / nch);
+        it.chunk = c;
+        items.push_back(it);
+      }
+    for (int I = 0; I < nt; ++I) items.push_back(SymItem{I, 0, 0, nch});
+    sym_.items.reserve(sizeof(SymItem) * items.size());
+    LM_HIP(hipMemcpyAsync(sym_.items.as<void>(), items.data(), sizeof(SymItem) * items.size(),
+                          hipMemcpyHostToDevice, stream_));
+    LM_HIP(hipStreamSynchronize(stream_));  // the host vector goes out of scope
+    sym_.N = N_;
+    sym_.tpl = tpl;
+    sym_.nchunks = nch;
+    sym_.ntiles = nt;
+    sym_.hmax = h;
+    sym_.nitems = (int)items.size();
+  }
+  const int64_t n3 = 3 * N_;
+  part_u_.reserve(sizeof(double) * n3 * (size_t)(nch + 1 + h));
+  double* slab_t = part_u_.as<double>();
+  double* slab_s = slab_t + (int64_t)(nch + 1) * n3;
+  SymArgs a{};
+  a.fp32 = fp32 ? 1 : 0;
+  a.tpl = tpl;
+  a.pos = rec;
+  a.F = F;
+  a.slab_t = slab_t;
+  a.slab_s = slab_s;
+  a.n3 = n3;
+  a.N = N_;
+  a.ntiles = nt;
+  a.hmax = h;
+  a.items = sym_.items.as<SymItem>();
+  a.nitems = sym_.nitems;
+  a.t0 = t0_;
+  launch_pair_sym(a, stream_);
+  launch_sym_reduce(slab_t, nch + 1, slab_s, h, n3, tpl, nt, U, stream_);
+}
+
+void NBodySolver::launch_sym(bool fp32, const void* rec, const double* F, double* U) {
+  const int p = fp32 ? 1 : 0;
+  if (autotune_ && tile_tpl_ == 0 && tile_nch_ == 0 && sym_tuned_N_[p] != N_) {
+    // time every candidate on the live inputs (U is overwritten by the real
+    // launch that follows)
+    hipEvent_t e0, e1;
+    LM_HIP(hipEventCreate(&e0));
+    LM_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    std::pair<int, int> pick{0, 0};
+    std::vector<std::tuple<int, int, int>> seen;
+    for (int tr : {1, 2, 4})
+      for (int cr : {0, 16, 32, 64}) {
+        int tpl, nch, nt, h;
+        sym_shape(tr, cr, tpl, nch, nt, h);
+        if (std::find(seen.begin(), seen.end(), std::make_tuple(tpl, nch, nt)) != seen.end())
+          continue;
+        seen.emplace_back(tpl, nch, nt);
+        run_sym(tpl, nch, nt, h, fp32, rec, F, U);
+        LM_HIP(hipEventRecord(e0, stream_));
+        run_sym(tpl, nch, nt, h, fp32, rec, F, U);
+        run_sym(tpl, nch, nt, h, fp32, rec, F, U);
+        LM_HIP(hipEventRecord(e1, stream_));
+        LM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        LM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          pick = {tpl, nch};
+        }
+      }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    sym_tuned_[p] = pick;
+    sym_tuned_N_[p] = N_;
+  }
+  int tpl, nch, nt, h;
+  sym_tiling(fp32, tpl, nch, nt, h);
+  run_sym(tpl, nch, nt, h, fp32, rec, F, U);
+}
+
 void NBodySolver::setPairTiling(int tpl, int nch) {
   if (tpl != 0 && tpl != 1 && tpl != 2 && tpl != 4)
     throw Error("[libmobility_amd] targets per lane must be 0 (auto), 1, 2 or 4");
@@ -508,10 +643,14 @@ void NBodySolver::pair_product(const void* rec64, const void* rec32, const doubl
     throw Error("[libmobility_amd] the solver includes angular velocities: an angular output array is required");
   const bool fp32 = pair_prec_ == kPairFP32;
   const void* rec = fp32 ? rec32 : rec64;
+  last_sym_ = false;
   if (seg_items_.count > 0) {
     // per-system Lanczos after some systems converged: their work is dropped
     launch_pair_items(seg_items_.buf.as<PairItem>(), seg_items_.count, seg_items_.tpl,
                       seg_items_.nch, blocks, fp32, rec, F, T, U, W);
+  } else if (use_sym(blocks)) {
+    launch_sym(fp32, rec, F, U);
+    last_sym_ = true;
   } else {
     int tpl = 1, nch = 0;
     choose_tiling(blocks, fp32, F, T, U, W, rec, tpl, nch);

@@ -85,10 +85,15 @@ class NBodySolver {
   // Pair-kernel tiling: targets per lane (1, 2, 4) and source chunks; 0 = auto.
   void setPairTiling(int tpl, int nch);
   void setAutotune(bool on) { autotune_ = on; }
+  // The symmetric path reports its targets per lane and offset chunks.
   void lastPairTiling(int* tpl, int* nch) const {
-    *tpl = items_tpl_;
-    *nch = items_nch_;
+    *tpl = last_sym_ ? sym_.tpl : items_tpl_;
+    *nch = last_sym_ ? sym_.nchunks : items_nch_;
   }
+  // Symmetric evaluation of the open translational block of one system
+  // (-1 auto: from N = 32768 while its slabs fit in 4 GiB; 0 off; 1 on).
+  void setPairSymmetric(int mode);
+  bool lastPairSymmetric() const { return last_sym_; }
   int checkConvergenceSteps() const { return check_steps_; }
 
   // Learned block preconditioner for sqrtMdotW (nullptr: plain Lanczos).  The
@@ -220,6 +225,23 @@ class NBodySolver {
   const SegBlock* seg_blocks_ = nullptr;
   int seg_nblocks_ = 0;
   void build_active_items(const int* act);
+  // symmetric evaluation (open boundaries, translational block, one system)
+  struct SymPlan {
+    DevBuf items;
+    int tpl = 0, nchunks = 0, ntiles = 0, hmax = 0, nitems = 0;
+    int64_t N = -1;
+  };
+  SymPlan sym_;
+  int sym_mode_ = -1;
+  bool last_sym_ = false;
+  std::pair<int, int> sym_tuned_[2];  // (tpl, nch) per precision (FP64, FP32)
+  int64_t sym_tuned_N_[2] = {-1, -1};
+  void sym_shape(int tpl_req, int nch_req, int& tpl, int& nch, int& ntiles, int& hmax) const;
+  void sym_tiling(bool fp32, int& tpl, int& nch, int& ntiles, int& hmax) const;
+  bool use_sym(int blocks) const;
+  void run_sym(int tpl, int nch, int nt, int h, bool fp32, const void* rec, const double* F,
+               double* U);
+  void launch_sym(bool fp32, const void* rec, const double* F, double* U);
   std::map<std::tuple<int, int, int>, std::vector<PairItem>> host_items_;
   uint64_t host_items_layout_ = 0;
   int64_t host_items_N_ = -1;

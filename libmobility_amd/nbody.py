@@ -194,6 +194,27 @@ class NBody:
             kp = k.ctypes.data
         _ffi.check(self._lib.lm_nbody_set_systems(self._h, off.ctypes.data, int(sizes.size), kp))
 
+    def setPairTiling(self, targetsPerLane=0, sourceChunks=0):
+        """Extension: fix the pair-kernel tiling (0 = automatic)."""
+        _ffi.check(self._lib.lm_nbody_set_pair_tiling(self._h, int(targetsPerLane), int(sourceChunks)))
+
+    def lastPairTiling(self):
+        """Extension: (targets per lane, source chunks) of the last pair launch."""
+        tpl, nch = ctypes.c_int(0), ctypes.c_int(0)
+        _ffi.check(self._lib.lm_nbody_get_pair_tiling(self._h, ctypes.byref(tpl), ctypes.byref(nch)))
+        return tpl.value, nch.value
+
+    def setPairSymmetric(self, mode=-1):
+        """Extension: evaluate each unordered pair once for the open
+        translational block of one system (-1 auto, 0 off, 1 on)."""
+        _ffi.check(self._lib.lm_nbody_set_pair_symmetric(self._h, int(mode)))
+
+    def lastPairSymmetric(self):
+        """Extension: whether the last pair product used the symmetric path."""
+        used = ctypes.c_int(0)
+        _ffi.check(self._lib.lm_nbody_last_pair_symmetric(self._h, ctypes.byref(used)))
+        return bool(used.value)
+
     def setPersistent(self, mode=1):
         """Extension: persistent pair launches (-1 auto, 0 off, 1 on)."""
         _ffi.check(self._lib.lm_nbody_set_persistent(self._h, int(mode)))

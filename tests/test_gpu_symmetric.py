@@ -1,0 +1,173 @@
+"""Symmetric evaluation of the open-boundary translational block (extension):
+each unordered pair is evaluated once and applied to both particles.  The
+product must equal the reference's ordered evaluation (NbodyRPY.cu:27-97,
+the kernel of hydrodynamicKernels.cuh:51-84) as restated by the oracle, for
+every tile count (one tile, even and odd counts, a partial last tile) and
+every tiling, including overlapping and coincident particles in different
+tiles; FP64 1e-12, FP32 pair terms 1e-6 relative in norm."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+ETA, A = 0.7, 1.3
+
+
+def _solver(precision="fp64", sym=1, tpl=0, nch=0, tol=1e-6):
+    from libmobility_amd import NBody
+    s = NBody("open", "open", "open")
+    s.setParameters()
+    s.setPairPrecision(precision)
+    s.setPairSymmetric(sym)
+    s.setPairTiling(tpl, nch)
+    s.initialize(viscosity=ETA, hydrodynamicRadius=A, tolerance=tol)
+    return s
+
+
+def _cloud(N, seed, phi=0.15):
+    rng = np.random.default_rng(seed)
+    L = (N * 4 * np.pi * A ** 3 / 3 / phi) ** (1 / 3)
+    return rng.uniform(0, L, (N, 3))
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b))
+
+
+# (N, targets per lane, offset chunks): tile counts 1 .. 9, odd and even, and
+# partial last tiles
+CASES = [
+    (100, 1, 0), (256, 1, 0), (300, 1, 0), (700, 1, 0), (1000, 1, 0), (1025, 1, 1),
+    (1200, 1, 2), (2300, 1, 4), (2300, 2, 0), (2300, 4, 0), (5000, 2, 3), (5000, 4, 1),
+]
+
+
+@pytest.mark.parametrize("N,tpl,nch", CASES)
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-6)])
+def test_symmetric_product_vs_oracle(N, tpl, nch, precision, tol):
+    pos = _cloud(N, N)
+    F = np.random.default_rng(N + 1).normal(size=(N, 3))
+    s = _solver(precision, 1, tpl, nch)
+    s.setPositions(pos)
+    U, _ = s.Mdot(F)
+    assert s.lastPairSymmetric()
+    assert s.lastPairTiling()[0] == (tpl or 1)
+    Uo, _ = oracle.mdot(pos, ETA, A, F=F)
+    assert _rel(U, Uo) <= tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-14), ("fp32", 1e-6)])
+def test_symmetric_equals_ordered_evaluation(precision, tol):
+    N = 9000
+    pos = _cloud(N, 3)
+    F = np.random.default_rng(4).normal(size=(N, 3))
+    ordered = _solver(precision, 0)
+    ordered.setPositions(pos)
+    U0, _ = ordered.Mdot(F)
+    assert not ordered.lastPairSymmetric()
+    s = _solver(precision, 1)
+    s.setPositions(pos)
+    U1, _ = s.Mdot(F)
+    assert _rel(U1, U0) <= tol
+    U2, _ = s.Mdot(F)
+    assert np.array_equal(U1, U2)  # fixed summation order: bitwise reproducible
+
+
+def test_overlapping_and_coincident_particles_in_different_tiles():
+    """Copies of particles of tile 0 placed in the last tile (r = 0 between
+    distinct particles: the overlap branch's value at r = 0) and a dense
+    cluster of overlapping spheres spread over all tiles."""
+    N = 1500  # 6 tiles of 256
+    pos = _cloud(N, 5)
+    pos[1400:1420] = pos[0:20]
+    pos[300:340] = pos[300] + np.random.default_rng(6).normal(scale=0.8, size=(40, 3))
+    pos[900:940] = pos[300] + np.random.default_rng(7).normal(scale=0.8, size=(40, 3))
+    F = np.random.default_rng(8).normal(size=(N, 3))
+    for tpl in (1, 2):
+        s = _solver("fp64", 1, tpl)
+        s.setPositions(pos)
+        U, _ = s.Mdot(F)
+        Uo, _ = oracle.mdot(pos, ETA, A, F=F)
+        assert _rel(U, Uo) <= 1e-12, tpl
+
+
+def test_symmetric_matrix_is_symmetric_and_matches_dense():
+    N = 600  # 3 tiles of 256 (odd count)
+    pos = _cloud(N, 9, phi=0.3)
+    s = _solver("fp64", 1, 1)
+    s.setPositions(pos)
+    n = 3 * N
+    cols = np.random.default_rng(10).choice(n, 40, replace=False)
+    M = oracle.dense_matrix(pos, ETA, A)
+    for c in cols:
+        e = np.zeros(n)
+        e[c] = 1.0
+        u, _ = s.Mdot(e)
+        assert np.abs(u.ravel() - M[:, c]).max() <= 1e-12 * np.abs(M).max(), c
+
+
+def test_auto_mode_picks_the_symmetric_path_for_large_open_systems():
+    N = 40000
+    pos = _cloud(N, 11, phi=0.2)
+    F = np.random.default_rng(12).normal(size=(N, 3))
+    s = _solver("fp32", -1)
+    s.setPositions(pos)
+    U, _ = s.Mdot(F)
+    assert s.lastPairSymmetric()
+    rows = np.random.default_rng(13).choice(N, 2000, replace=False)
+    Uo, _ = oracle.mdot_rows(pos, ETA, A, rows, F=F)
+    assert _rel(U[rows], Uo) <= 1e-6
+    small = _solver("fp32", -1)
+    small.setPositions(pos[:1000])
+    small.Mdot(F[:1000])
+    assert not small.lastPairSymmetric()
+
+
+def test_symmetric_path_is_not_used_for_other_blocks_or_walls():
+    from libmobility_amd import NBody
+    pos = _cloud(600, 14)
+    ang = NBody("open", "open", "open")
+    ang.setParameters()
+    ang.setPairSymmetric(1)
+    ang.initialize(viscosity=ETA, hydrodynamicRadius=A, includeAngular=True)
+    ang.setPositions(pos)
+    ang.Mdot(np.ones((600, 3)), np.ones((600, 3)))
+    assert not ang.lastPairSymmetric()
+    wall = NBody("open", "open", "single_wall")
+    wall.setParameters(wallHeight=0.0)
+    wall.setPairSymmetric(1)
+    wall.initialize(viscosity=ETA, hydrodynamicRadius=A)
+    p = pos.copy()
+    p[:, 2] = np.abs(p[:, 2]) + 1.5
+    wall.setPositions(p)
+    wall.Mdot(np.ones((600, 3)))
+    assert not wall.lastPairSymmetric()
+
+
+def test_lanczos_on_the_symmetric_product():
+    """sqrtMdotW through the symmetric product: a square root of M as
+    accurate as the oracle's own Lanczos run (LanczosAlgorithm.cu:171-232)."""
+    from libmobility_amd.workloads import rsa_box, rsa_box_side
+    N = 700
+    L = rsa_box_side(N, 0.2, A)
+    # non-overlapping spheres at tol 1e-6: the oracle's stopping step (38)
+    # does not move under 1e-15 perturbations of M v
+    pos = rsa_box(N, (L, L, L), 15, a=A)
+    tol = 1e-6
+    s = _solver("fp64", 1, 1, tol=tol)
+    s.setPositions(pos)
+    W = np.random.default_rng(16).normal(size=3 * N)
+    y, _ = s.sqrtMdotW_with_noise(W)
+    M = oracle.dense_matrix(pos, ETA, A)
+    ref, it_ref = oracle.LanczosOracle().run(lambda v: M @ v, W, tol)
+    exact = oracle.dense_sqrt(M) @ W
+    assert abs(s.lastLanczosIterations() - it_ref) <= 6
+    assert np.linalg.norm(y.ravel() - exact) <= 3 * np.linalg.norm(ref - exact) + 5 * tol * np.linalg.norm(exact)
+
+
+def test_mode_is_validated():
+    s = _solver("fp64", -1)
+    with pytest.raises(RuntimeError):
+        s.setPairSymmetric(2)
