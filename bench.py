@@ -184,11 +184,13 @@ def result_line(metric, value, unit, steps, warmup, ms, higher, dtype, config, r
     }
 
 
-# HBM bytes per FP32 k_pair launch, from rocprofv3 --pmc FETCH_SIZE (x 2.00) and
-# WRITE_SIZE (x 1.00); the scales come from a 2 GiB device copy.  Keyed by
-# (workload, N, targets per lane, source chunks) of the profiled configuration.
+# HBM bytes per FP32 pair-kernel launch, from rocprofv3 --pmc FETCH_SIZE (x 2.00)
+# and WRITE_SIZE (x 1.00); the scales come from a 2 GiB device copy.  Keyed by
+# (workload, N, pair evaluation, targets per lane, source chunks) of the
+# profiled configuration.
 MEASURED_TRAFFIC_BYTES = {
-    ("c3", 131072, 2, 32): (10454.1 * 2.0 + 98304.0) * 1024.0,
+    ("c3", 131072, "ordered", 2, 32): (10454.1 * 2.0 + 98304.0) * 1024.0,
+    ("c3", 131072, "symmetric", 4, 64): (152410.0 * 2.0 + 394750.0) * 1024.0,
 }
 
 
@@ -396,7 +398,8 @@ def bench_c3(args):
         "pairs_per_step": pairs,
     }
     t32 = extra["tiling_fp32pair"]
-    traffic = MEASURED_TRAFFIC_BYTES.get(("c3", N, t32["targets_per_lane"], t32["source_chunks"]))
+    traffic = MEASURED_TRAFFIC_BYTES.get((
+        "c3", N, extra["pair_evaluation_fp32pair"], t32["targets_per_lane"], t32["source_chunks"]))
     res = result_line(METRIC, gpair32, "Gpair/s", args.steps, args.warmup, ms32, True, "f32",
                       config, compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS, traffic),
                       extra, None)
