@@ -86,13 +86,18 @@ int lm_nbody_set_pair_precision(lm_nbody* h, int precision);
  * reference's blockDim choice, NbodyRPY.cu:105-108). */
 int lm_nbody_set_pair_tiling(lm_nbody* h, int targets_per_lane, int source_chunks);
 
-/* Extension: symmetric evaluation of the open-boundary translational block
- * of a single system (-1 auto, 0 off, 1 on).  M_ij = M_ji for open RPY, so
- * each unordered pair is evaluated once and applied to both particles; the
- * reference evaluates every ordered pair (NbodyRPY.cu:27-97).  Auto uses it
- * from N = 32768 while its partial-sum slabs fit in 4 GiB.  Results agree
- * with the ordered evaluation to rounding and are bitwise reproducible. */
+/* Extension: symmetric evaluation of the product of a single open or
+ * single-wall system, every block set (-1 auto, 0 off, 1 on).  The grand
+ * mobility matrix is symmetric (M_ji = M_ij^T blockwise), so each unordered
+ * pair is evaluated once and applied to both particles; the reference
+ * evaluates every ordered pair (NbodyRPY.cu:27-97, hydrodynamicKernels.cuh).
+ * Auto uses it from N = 32768.  Results agree with the ordered evaluation to
+ * rounding and are bitwise reproducible. */
 int lm_nbody_set_pair_symmetric(lm_nbody* h, int mode);
+
+/* Extension: device memory for the symmetric evaluation's partial sums
+ * (0 = 4 GiB).  The ring of tile offsets is split into passes that fit. */
+int lm_nbody_set_pair_symmetric_memory(lm_nbody* h, int64_t bytes);
 
 /* Extension: 1 if the last pair product used the symmetric evaluation. */
 int lm_nbody_last_pair_symmetric(lm_nbody* h, int* used);

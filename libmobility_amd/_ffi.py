@@ -37,6 +37,7 @@ SIGNATURES = {
     "lm_nbody_set_stream": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_set_pair_tiling": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "lm_nbody_set_pair_symmetric": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lm_nbody_set_pair_symmetric_memory": (ctypes.c_int, [_vp, ctypes.c_int64]),
     "lm_nbody_last_pair_symmetric": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "lm_nbody_set_autotune": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lm_nbody_get_pair_tiling": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),

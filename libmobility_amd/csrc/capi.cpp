@@ -82,6 +82,10 @@ int lm_nbody_set_pair_symmetric(lm_nbody* h, int mode) {
   return guarded([&] { S(h).setPairSymmetric(mode); });
 }
 
+int lm_nbody_set_pair_symmetric_memory(lm_nbody* h, int64_t bytes) {
+  return guarded([&] { S(h).setPairSymmetricMemory(bytes); });
+}
+
 int lm_nbody_last_pair_symmetric(lm_nbody* h, int* used) {
   return guarded([&] {
     if (!used) throw lm::Error("[libmobility_amd] null output");

@@ -59,9 +59,9 @@ struct PairMultiArgs {
 };
 constexpr int multi_tpl(int nrhs) { return nrhs >= 4 ? 1 : 2; }
 
-// Symmetric evaluation of the open-boundary translational block of one
-// system: M_ji = M_ij, so each unordered pair is evaluated once and applied to
-// both particles.  Tiles of 256 * tpl particles; tile I meets tiles I + o
+// Symmetric evaluation of one system (open or bottom wall, any block set):
+// the grand mobility matrix is symmetric (M_ji = M_ij^T blockwise), so each
+// unordered pair is evaluated once and applied to both particles.  Tiles of 256 * tpl particles; tile I meets tiles I + o
 // (mod ntiles) for o = 1 .. hmax, hmax = ntiles / 2 (offset ntiles / 2 of an
 // even count only from I < ntiles / 2), so every unordered tile pair is met
 // once.  Target-side sums of an item go to target slab `chunk`, source-side
@@ -76,21 +76,29 @@ struct SymItem {
 struct SymArgs {
   int fp32;
   int tpl;
+  int wall;          // 0 open RPY, 1 bottom wall
+  int blocks;        // BlockBits mask
   const void* pos;   // float4 / double4 records
-  const double* F;   // 3N AoS
-  double* slab_t;    // nchunks + 1 target slabs of n3
-  double* slab_s;    // hmax source slabs of n3
+  const double* F;   // 3N AoS (nullptr when absent)
+  const double* T;   // 3N AoS (nullptr when absent)
+  double* slab_t;    // nchunks + 1 target slabs of n3 (linear velocities)
+  double* slab_s;    // hmax source slabs of n3 (linear velocities)
+  double* slab_tw;   // as slab_t for angular velocities (nullptr without them)
+  double* slab_sw;   // as slab_s for angular velocities
   int64_t n3, N;
   int ntiles, hmax;
+  int obase;         // first ring offset of this pass: source slab index = o - obase
   const SymItem* items;
   int nitems;
-  double t0;
+  double t0, rt0, r0;
 };
 void launch_pair_sym(const SymArgs& a, hipStream_t s);
-// out = sum of the nt target slabs, then of the hmax source slabs (rows a
-// source slab does not hold are skipped), in that fixed order.
-void launch_sym_reduce(const double* slab_t, int nt, const double* slab_s, int hmax, int64_t n3,
-                       int tpl, int ntiles, double* out, hipStream_t s);
+// out (= or +=, accumulate) the sum of the nt target slabs, then of the
+// source slabs of ring offsets o0 .. o1-1 (rows a source slab does not hold
+// are skipped), in that fixed order.
+void launch_sym_reduce(const double* slab_t, int nt, const double* slab_s, int o0, int o1,
+                       int64_t n3, int tpl, int ntiles, int accumulate, double* out,
+                       hipStream_t s);
 void launch_pair_multi(const PairMultiArgs& a, hipStream_t s);
 void launch_chunk_reduce(const double* part, int nchunks, int64_t n, double* out, hipStream_t s);
 

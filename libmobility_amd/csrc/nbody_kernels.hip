@@ -285,7 +285,7 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
     if (2 * o == a.ntiles && 2 * it.tile >= a.ntiles) continue;  // met from the other side
     int J = it.tile + o;
     if (J >= a.ntiles) J -= a.ntiles;
-    double* __restrict__ So = a.slab_s + (int64_t)(o - 1) * a.n3;
+    double* __restrict__ So = a.slab_s + (int64_t)(o - a.obase) * a.n3;
     for (int sub = 0; sub < TPL; ++sub) {
       const int64_t base = (int64_t)J * kTile + sub * kPairThreads;
       if (base >= N) break;
@@ -408,14 +408,178 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
   }
 }
 
+// General symmetric item (bottom wall, or open boundaries with angular
+// blocks): the same ring, lanes and rotation as sym_item, with pair_both
+// giving both directions of every block from one set of coefficients.  The
+// source sums carry U (units of t0) and, with angular blocks, W (units of r0).
+template <class R, bool WALL, int BLK, int TPL>
+__device__ inline void sym_item_gen(const SymItem& it, const SymArgs& a,
+                                    typename Rec<R>::T* s_pos, typename Rec<R>::T* s_f,
+                                    typename Rec<R>::T* s_t, R* s_red) {
+  using R4 = typename Rec<R>::T;
+  constexpr int kTile = kPairThreads * TPL;
+  constexpr bool kNeedF = (BLK & (kUF | kWF)) != 0;
+  constexpr bool kNeedT = (BLK & (kUT | kWT)) != 0;
+  constexpr bool kAngular = (BLK & (kWT | kWF)) != 0;
+  constexpr int kComp = kAngular ? 6 : 3;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const R4* __restrict__ pos = static_cast<const R4*>(a.pos);
+  const int64_t N = a.N;
+  const int64_t tgt0 = (int64_t)it.tile * kTile;
+  const R kut = R(a.rt0 / a.t0), kwf = R(a.rt0 / a.r0);
+  const Vec3<R> zero = {R(0), R(0), R(0)};
+
+  R4 pi[TPL];
+  Vec3<R> fi[TPL], ti[TPL];
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int64_t i = tgt0 + tid + t * kPairThreads;
+    const bool ok = i < N;
+    pi[t] = ok ? pos[i] : far_record<R4>();
+    fi[t] = (kNeedF && ok) ? Vec3<R>{R(a.F[3 * i]), R(a.F[3 * i + 1]), R(a.F[3 * i + 2])} : zero;
+    ti[t] = (kNeedT && ok) ? Vec3<R>{R(a.T[3 * i]), R(a.T[3 * i + 1]), R(a.T[3 * i + 2])} : zero;
+  }
+  PairAcc<R> acc[TPL];
+  PairAcc<double> acc64[TPL];
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    acc_zero(acc[t]);
+    acc_zero(acc64[t]);
+  }
+
+  for (int o = it.o0; o < it.o1; ++o) {
+    if (2 * o == a.ntiles && 2 * it.tile >= a.ntiles) continue;  // met from the other side
+    int J = it.tile + o;
+    if (J >= a.ntiles) J -= a.ntiles;
+    for (int sub = 0; sub < TPL; ++sub) {
+      const int64_t base = (int64_t)J * kTile + sub * kPairThreads;
+      if (base >= N) break;
+      const int64_t j = base + tid;
+      __syncthreads();  // the previous sub-tile is done with the staged sources and s_red
+      const bool ok = j < N;
+      const R4 pj = ok ? pos[j] : far_record<R4>();
+      s_pos[wave * 128 + lane] = pj;
+      s_pos[wave * 128 + 64 + lane] = pj;
+      if (kNeedF) {
+        const R4 fj = ok ? R4{R(a.F[3 * j]), R(a.F[3 * j + 1]), R(a.F[3 * j + 2]), R(0)}
+                         : R4{R(0), R(0), R(0), R(0)};
+        s_f[wave * 128 + lane] = fj;
+        s_f[wave * 128 + 64 + lane] = fj;
+      }
+      if (kNeedT) {
+        const R4 tj = ok ? R4{R(a.T[3 * j]), R(a.T[3 * j + 1]), R(a.T[3 * j + 2]), R(0)}
+                         : R4{R(0), R(0), R(0), R(0)};
+        s_t[wave * 128 + lane] = tj;
+        s_t[wave * 128 + 64 + lane] = tj;
+      }
+      __syncthreads();
+      for (int blk = 0; blk < 4; ++blk) {
+        const int b0 = blk * 128 + lane;
+        Vec3<R> su = zero, sw = zero;
+        for (int k = 0; k < 64; ++k) {
+          const R4 pj4 = s_pos[b0 + k];
+          Vec3<R> fj = zero, tj = zero;
+          if (kNeedF) {
+            const R4 v = s_f[b0 + k];
+            fj = {v.x, v.y, v.z};
+          }
+          if (kNeedT) {
+            const R4 v = s_t[b0 + k];
+            tj = {v.x, v.y, v.z};
+          }
+          Vec3<R> cu = zero, cw = zero;
+#pragma unroll
+          for (int t = 0; t < TPL; ++t)
+            pair_both<R, WALL, BLK>(pi[t].x, pi[t].y, pi[t].z, pi[t].w, pj4.x, pj4.y, pj4.z, pj4.w,
+                                    fj, tj, fi[t], ti[t], kut, kwf, acc[t], cu, cw);
+          su.x = lane_rol1(su.x) + cu.x;
+          su.y = lane_rol1(su.y) + cu.y;
+          su.z = lane_rol1(su.z) + cu.z;
+          if (kAngular) {
+            sw.x = lane_rol1(sw.x) + cw.x;
+            sw.y = lane_rol1(sw.y) + cw.y;
+            sw.z = lane_rol1(sw.z) + cw.z;
+          }
+        }
+        // lane l holds source l - 1 after 64 steps: one more move
+        R* red = s_red + wave * kComp * kPairThreads + blk * 64 + lane;
+        red[0 * kPairThreads] = lane_rol1(su.x);
+        red[1 * kPairThreads] = lane_rol1(su.y);
+        red[2 * kPairThreads] = lane_rol1(su.z);
+        if (kAngular) {
+          red[3 * kPairThreads] = lane_rol1(sw.x);
+          red[4 * kPairThreads] = lane_rol1(sw.y);
+          red[5 * kPairThreads] = lane_rol1(sw.z);
+        }
+      }
+      __syncthreads();
+      if (ok) {
+        double v[kComp];
+#pragma unroll
+        for (int q = 0; q < kComp; ++q) v[q] = 0.0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w)
+#pragma unroll
+          for (int q = 0; q < kComp; ++q) v[q] += (double)s_red[(w * kComp + q) * kPairThreads + tid];
+        double* So = a.slab_s + (int64_t)(o - a.obase) * a.n3;
+        So[3 * j + 0] = a.t0 * v[0];
+        So[3 * j + 1] = a.t0 * v[1];
+        So[3 * j + 2] = a.t0 * v[2];
+        if (kAngular) {
+          double* Sw = a.slab_sw + (int64_t)(o - a.obase) * a.n3;
+          Sw[3 * j + 0] = a.r0 * v[kComp - 3];
+          Sw[3 * j + 1] = a.r0 * v[kComp - 2];
+          Sw[3 * j + 2] = a.r0 * v[kComp - 1];
+        }
+      }
+      if constexpr (std::is_same<R, float>::value) {
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+          acc64[t].uf.x += acc[t].uf.x; acc64[t].uf.y += acc[t].uf.y; acc64[t].uf.z += acc[t].uf.z;
+          acc64[t].ut.x += acc[t].ut.x; acc64[t].ut.y += acc[t].ut.y; acc64[t].ut.z += acc[t].ut.z;
+          acc64[t].wt.x += acc[t].wt.x; acc64[t].wt.y += acc[t].wt.y; acc64[t].wt.z += acc[t].wt.z;
+          acc64[t].wf.x += acc[t].wf.x; acc64[t].wf.y += acc[t].wf.y; acc64[t].wf.z += acc[t].wf.z;
+          acc_zero(acc[t]);
+        }
+      }
+    }
+  }
+  if constexpr (std::is_same<R, double>::value) {
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) acc64[t] = acc[t];
+  }
+  double* Uo = a.slab_t + (int64_t)it.chunk * a.n3;
+  double* Wo = kAngular ? a.slab_tw + (int64_t)it.chunk * a.n3 : nullptr;
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int64_t i = tgt0 + tid + t * kPairThreads;
+    if (i < N) {
+      const PairAcc<double>& c = acc64[t];
+      Uo[3 * i + 0] = a.t0 * c.uf.x + a.rt0 * c.ut.x;
+      Uo[3 * i + 1] = a.t0 * c.uf.y + a.rt0 * c.ut.y;
+      Uo[3 * i + 2] = a.t0 * c.uf.z + a.rt0 * c.ut.z;
+      if (kAngular) {
+        Wo[3 * i + 0] = a.r0 * c.wt.x + a.rt0 * c.wf.x;
+        Wo[3 * i + 1] = a.r0 * c.wt.y + a.rt0 * c.wf.y;
+        Wo[3 * i + 2] = a.r0 * c.wt.z + a.rt0 * c.wf.z;
+      }
+    }
+  }
+}
+
 // One item per workgroup: a diagonal tile (all ordered pairs of tile I, the
 // self pairs included) or a range of partner offsets.
-template <class R, int TPL>
+template <class R, bool WALL, int BLK, int TPL>
 __global__ __launch_bounds__(kPairThreads) void k_pair_sym(SymArgs a) {
   using R4 = typename Rec<R>::T;
+  constexpr bool kNeedF = (BLK & (kUF | kWF)) != 0;
+  constexpr bool kNeedT = (BLK & (kUT | kWT)) != 0;
+  constexpr bool kSpecial = !WALL && BLK == kUF;  // the open translational fast path
+  constexpr int kComp = (BLK & (kWT | kWF)) ? 6 : 3;
   __shared__ R4 s_pos[4 * 128];
-  __shared__ R4 s_f[4 * 128];
-  __shared__ R s_red[4 * 3 * kPairThreads];
+  __shared__ R4 s_f[kNeedF ? 4 * 128 : 1];
+  __shared__ R4 s_t[kNeedT ? 4 * 128 : 1];
+  __shared__ R s_red[4 * kComp * kPairThreads];
   const SymItem it = a.items[blockIdx.x];
   if (it.o0 == 0) {
     constexpr int kTile = kPairThreads * TPL;
@@ -425,13 +589,17 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_sym(SymArgs a) {
     p.src0 = p.tgt0;
     p.src1 = p.tgt0 + p.ntgt;
     p.chunk = it.chunk;
-    p.wall = 0;
+    p.wall = WALL ? 1 : 0;
     p.sys = 0;
-    pair_item<R, false, kUF, TPL>(p, static_cast<const R4*>(a.pos), a.F, nullptr, a.slab_t, nullptr,
-                                  a.n3, a.t0, 0.0, 0.0, s_pos, s_f, s_f);
+    pair_item<R, WALL, BLK, TPL>(p, static_cast<const R4*>(a.pos), a.F, a.T, a.slab_t, a.slab_tw,
+                                 a.n3, a.t0, a.rt0, a.r0, s_pos, kNeedF ? s_f : s_pos,
+                                 kNeedT ? s_t : s_pos);
     return;
   }
-  sym_item<R, TPL>(it, a, s_pos, s_f, s_red);
+  if constexpr (kSpecial)
+    sym_item<R, TPL>(it, a, s_pos, s_f, s_red);
+  else
+    sym_item_gen<R, WALL, BLK, TPL>(it, a, s_pos, s_f, s_t, s_red);
 }
 
 // ---- several force vectors per pass (translational block) ----------------------
@@ -643,16 +811,40 @@ void launch_pair(const PairArgs& a, hipStream_t s) {
     launch_pair_kind<double>(a, s);
 }
 
-template <class R>
-static void launch_pair_sym_r(const SymArgs& a, hipStream_t s) {
+template <class R, bool WALL, int BLK>
+static void launch_pair_sym_tpl(const SymArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)a.nitems), block(kPairThreads);
   switch (a.tpl) {
-    case 1: hipLaunchKernelGGL((k_pair_sym<R, 1>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_pair_sym<R, 2>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_pair_sym<R, 4>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_pair_sym<R, WALL, BLK, 1>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_pair_sym<R, WALL, BLK, 2>), grid, block, 0, s, a); break;
+    case 4:
+      if constexpr (!WALL && BLK == kUF) {
+        hipLaunchKernelGGL((k_pair_sym<R, WALL, BLK, 4>), grid, block, 0, s, a);
+        break;
+      }
+      [[fallthrough]];
     default: throw Error("[libmobility_amd] unsupported targets-per-lane for the symmetric kernel");
   }
   LM_HIP(hipGetLastError());
+}
+
+template <class R, bool WALL>
+static void launch_pair_sym_blk(const SymArgs& a, hipStream_t s) {
+  switch (a.blocks) {
+    case kUF: launch_pair_sym_tpl<R, WALL, kUF>(a, s); break;
+    case kUF | kWF: launch_pair_sym_tpl<R, WALL, kUF | kWF>(a, s); break;
+    case kUT | kWT: launch_pair_sym_tpl<R, WALL, kUT | kWT>(a, s); break;
+    case kUF | kUT | kWT | kWF: launch_pair_sym_tpl<R, WALL, kUF | kUT | kWT | kWF>(a, s); break;
+    default: throw Error("[libmobility_amd] unsupported block combination");
+  }
+}
+
+template <class R>
+static void launch_pair_sym_r(const SymArgs& a, hipStream_t s) {
+  if (a.wall)
+    launch_pair_sym_blk<R, true>(a, s);
+  else
+    launch_pair_sym_blk<R, false>(a, s);
 }
 
 void launch_pair_sym(const SymArgs& a, hipStream_t s) {
@@ -664,27 +856,29 @@ void launch_pair_sym(const SymArgs& a, hipStream_t s) {
 }
 
 __global__ void k_sym_reduce(const double* __restrict__ slab_t, int nt,
-                             const double* __restrict__ slab_s, int hmax, int64_t n, int64_t tile3,
-                             int ntiles, double* __restrict__ out) {
+                             const double* __restrict__ slab_s, int o0, int o1, int64_t n,
+                             int64_t tile3, int ntiles, int accumulate, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double s = 0.0;
+  double s = accumulate ? out[i] : 0.0;
   for (int c = 0; c < nt; ++c) s += slab_t[(int64_t)c * n + i];
   // an even tile count meets offset ntiles/2 only from tiles I < ntiles/2, so
   // that slab holds no rows of tiles J < ntiles/2
   const int tile = (int)(i / tile3);
   const int skip = (ntiles % 2 == 0 && 2 * tile < ntiles) ? ntiles / 2 : 0;
-  for (int o = 1; o <= hmax; ++o)
-    if (o != skip) s += slab_s[(int64_t)(o - 1) * n + i];
+  for (int o = o0; o < o1; ++o)
+    if (o != skip) s += slab_s[(int64_t)(o - o0) * n + i];
   out[i] = s;
 }
 
-void launch_sym_reduce(const double* slab_t, int nt, const double* slab_s, int hmax, int64_t n3,
-                       int tpl, int ntiles, double* out, hipStream_t s) {
+void launch_sym_reduce(const double* slab_t, int nt, const double* slab_s, int o0, int o1,
+                       int64_t n3, int tpl, int ntiles, int accumulate, double* out,
+                       hipStream_t s) {
   if (n3 <= 0) return;
   const int nt_ = 256;
   hipLaunchKernelGGL(k_sym_reduce, dim3((unsigned)((n3 + nt_ - 1) / nt_)), dim3(nt_), 0, s, slab_t,
-                     nt, slab_s, hmax, n3, (int64_t)3 * kPairThreads * tpl, ntiles, out);
+                     nt, slab_s, o0, o1, n3, (int64_t)3 * kPairThreads * tpl, ntiles, accumulate,
+                     out);
   LM_HIP(hipGetLastError());
 }
 

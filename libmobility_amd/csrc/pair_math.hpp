@@ -425,4 +425,183 @@ __host__ __device__ inline void wall_pair(R xi, R yi, R hi, R bi, R xj, R yj, R 
   }
 }
 
+// ---- one pair, both directions (symmetric evaluation) ----------------------
+// The grand mobility matrix is symmetric: blockwise M_ji = M_ij^T, with UF and
+// WT transposing into themselves and the cross blocks swapping,
+// M_UT(ji) = M_WF(ij)^T and M_WF(ji) = M_UT(ij)^T.  So the coefficients of the
+// pair (i <- j) also give what j receives from i.  In the vector forms above,
+// with q = (e_y, -e_x, 0):
+//   UF^T v = f1 v + (f2 (e.v) + f4 v_z) e + z^ (f3 (e.v) + f5 v_z)   (f3, f4 swap)
+//   WT^T v = g1 v + (g2 (e.v) + g3 v_z) e + g4 (q.v) q
+//   WF:  B v = q (k3 (e.v) + (k1 + k2) v_z) + kz (z^ x v) - k1 (q.v) z^
+//        B^T v = (k3 e + (k1 + k2) z^) (q.v) - kz (z^ x v) - k1 v_z q
+//   UT:  C v = k3 (q.v) e - kz (z^ x v) + k1 v_z q + gam (q.v) z^,
+//        gam = -(k1 + k2 + 2 k3 e_z)  (coefficients with h = h_i)
+//        C^T v = (k3 (e.v) + gam v_z) q + kz (z^ x v) + k1 (q.v) z^
+// and the open parts: UF, WT symmetric; c (v x d) transposes to -c (v x d).
+// The target accumulates M_ij [F_j; T_j] into a (as wall_pair / open_pair);
+// the source sums are kept in units of t0 (su: UF part + kut * UT part,
+// kut = rt0/t0) and r0 (sw: WT part + kwf * WF part, kwf = rt0/r0).
+template <class R, bool WALL, int BLK>
+__host__ __device__ inline void pair_both(R xi, R yi, R hi, R bi, R xj, R yj, R hj, R bj,
+                                          const Vec3<R>& Fj, const Vec3<R>& Tj,
+                                          const Vec3<R>& Fi, const Vec3<R>& Ti, R kut, R kwf,
+                                          PairAcc<R>& a, Vec3<R>& su, Vec3<R>& sw) {
+  constexpr bool kCross = (BLK & (kUT | kWF)) != 0;
+  const R dx = xi - xj, dy = yi - yj, dz = hi - hj;
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  const R bij = WALL ? bi * bj : R(1);
+  R fuf, guf, fwt, gwt, c;
+  rpy_coeffs<R, BLK | (kCross ? (kUT | kWF) : 0)>(r2, fuf, guf, fwt, gwt, c);
+  const Vec3<R> zero = {R(0), R(0), R(0)};
+  Vec3<R> uf = zero, ut = zero, wt = zero, wf = zero;      // image part, to i
+  Vec3<R> suf = zero, sut = zero, swt = zero, swf = zero;  // image part, to j
+  if (WALL) {
+    if (r2 != R(0)) {
+      const R Rz = hi + hj;
+      const R iR = lm_rsqrt(dx * dx + dy * dy + Rz * Rz);
+      const R iR2 = iR * iR;
+      const R iR3 = iR2 * iR;
+      const R ex = dx * iR, ey = dy * iR, ez = Rz * iR;
+      const R ez2 = ez * ez;
+      const R iRz = lm_rcp(Rz);
+      if (BLK & kUF) {
+        const R hh = hj * iRz;
+        const R iR5 = iR3 * iR2;
+        const R t1 = R(5) * ez2 - R(1);
+        const R hq = hh * (R(1) - hh) * ez2;
+        const R f1 = R(-0.25) * (R(3) * (R(1) + R(2) * hq) * iR + R(2) * (R(1) - R(3) * ez2) * iR3 +
+                                 R(2) * t1 * iR5);
+        const R f2 = R(-0.25) * (R(3) * (R(1) - R(6) * hq) * iR + R(6) * t1 * iR3 +
+                                 R(10) * (R(1) - R(7) * ez2) * iR5);
+        const R f3 = R(0.5) * ez *
+                     (R(3) * hh * (R(1) - R(6) * (R(1) - hh) * ez2) * iR + R(6) * t1 * iR3 +
+                      R(10) * (R(2) - R(7) * ez2) * iR5);
+        const R f4 = R(0.5) * ez * (R(3) * hh * iR - R(10) * iR5);
+        const R f5 = -(R(3) * hh * hh * ez2 * iR + R(3) * ez2 * iR3 + (R(2) - R(15) * ez2) * iR5);
+        const R ev = ex * Fj.x + ey * Fj.y + ez * Fj.z;
+        const R q = f2 * ev + f3 * Fj.z;
+        uf = {f1 * Fj.x + q * ex, f1 * Fj.y + q * ey, f1 * Fj.z + q * ez + f4 * ev + f5 * Fj.z};
+        const R evs = ex * Fi.x + ey * Fi.y + ez * Fi.z;
+        const R qs = f2 * evs + f4 * Fi.z;
+        suf = {f1 * Fi.x + qs * ex, f1 * Fi.y + qs * ey, f1 * Fi.z + qs * ez + f3 * evs + f5 * Fi.z};
+      }
+      if (BLK & kWT) {
+        const R g1 = R(0.5) * (R(1) - R(6) * ez2) * iR3;
+        const R g2 = R(-1.5) * iR3;
+        const R g3 = R(3) * iR3 * ez;
+        const R g4 = R(3) * iR3;
+        const R ev = ex * Tj.x + ey * Tj.y + ez * Tj.z;
+        const R p = g4 * (ey * Tj.x - ex * Tj.y);
+        wt = {g1 * Tj.x + g2 * ev * ex + p * ey, g1 * Tj.y + g2 * ev * ey - p * ex,
+              g1 * Tj.z + (g2 * ez + g3) * ev};
+        const R evs = ex * Ti.x + ey * Ti.y + ez * Ti.z;
+        const R ps = g4 * (ey * Ti.x - ex * Ti.y);
+        const R ss = g2 * evs + g3 * Ti.z;
+        swt = {g1 * Ti.x + ss * ex + ps * ey, g1 * Ti.y + ss * ey - ps * ex, g1 * Ti.z + ss * ez};
+      }
+      if (kCross) {
+        const R iR4 = iR2 * iR2;
+        const R k1 = iR2;
+        // WF factors (h = h_j) and UT factors (h = h_i)
+        const R hw = hj * iRz, hu = hi * iRz;
+        const R k2w = R(2) * (R(6) * hw * ez2 * iR2 + (R(1) - R(10) * ez2) * iR4);
+        const R k3w = R(-2) * ez * (R(3) * hw * iR2 - R(5) * iR4);
+        const R kzw = k1 * ez + R(-2) * ez * (hw * iR2 - iR4);
+        const R k2u = R(2) * (R(6) * hu * ez2 * iR2 + (R(1) - R(10) * ez2) * iR4);
+        const R k3u = R(-2) * ez * (R(3) * hu * iR2 - R(5) * iR4);
+        const R kzu = k1 * ez + R(-2) * ez * (hu * iR2 - iR4);
+        if (BLK & kWF) {
+          // to i: B F_j
+          const R ev = ex * Fj.x + ey * Fj.y + ez * Fj.z;
+          const R p = ey * Fj.x - ex * Fj.y;
+          const R k12 = k1 + k2w;
+          wf = {k3w * ey * ev - kzw * Fj.y + k12 * ey * Fj.z,
+                kzw * Fj.x - k3w * ex * ev - k12 * ex * Fj.z, -k1 * p};
+          // to j: C^T F_i (W_j from F_i is M_UT(ij)^T)
+          const R evs = ex * Fi.x + ey * Fi.y + ez * Fi.z;
+          const R gam = -(k1 + k2u + R(2) * k3u * ez);
+          const R s = k3u * evs + gam * Fi.z;
+          swf = {s * ey - kzu * Fi.y, -s * ex + kzu * Fi.x, k1 * (ey * Fi.x - ex * Fi.y)};
+        }
+        if (BLK & kUT) {
+          // to i: C T_j
+          const R p = ey * Tj.x - ex * Tj.y;
+          ut = {k3u * ex * p + kzu * Tj.y + k1 * ey * Tj.z,
+                k3u * ey * p - kzu * Tj.x - k1 * ex * Tj.z, -(k1 + k2u + k3u * ez) * p};
+          // to j: B^T T_i (U_j from T_i is M_WF(ij)^T)
+          const R ps = ey * Ti.x - ex * Ti.y;
+          const R k12 = k1 + k2w;
+          sut = {k3w * ex * ps + kzw * Ti.y - k1 * ey * Ti.z,
+                 k3w * ey * ps - kzw * Ti.x + k1 * ex * Ti.z, (k3w * ez + k12) * ps};
+        }
+      }
+    } else {
+      // Self terms (r == 0, h_i == h_j), Swan & Brady B1-B3 as in wall_pair;
+      // each block equals its transpose's counterpart, so j gets the same forms.
+      const R ih = R(1) / hj;
+      const R ih2 = ih * ih;
+      const R ih3 = ih2 * ih;
+      if (BLK & kUF) {
+        const R ih5 = ih3 * ih2;
+        const R cp = R(-0.0625) * (R(9) * ih - R(2) * ih3 + ih5);
+        const R cz = R(-0.125) * (R(9) * ih - R(4) * ih3 + ih5);
+        uf = {cp * Fj.x, cp * Fj.y, cz * Fj.z};
+        suf = {cp * Fi.x, cp * Fi.y, cz * Fi.z};
+      }
+      if (BLK & kWT) {
+        wt = {R(-0.3125) * ih3 * Tj.x, R(-0.3125) * ih3 * Tj.y, R(-0.125) * ih3 * Tj.z};
+        swt = {R(-0.3125) * ih3 * Ti.x, R(-0.3125) * ih3 * Ti.y, R(-0.125) * ih3 * Ti.z};
+      }
+      const R c4 = R(0.125) * ih3 * ih;
+      if (BLK & kWF) {
+        wf = {-c4 * Fj.y, c4 * Fj.x, R(0)};
+        swf = {-c4 * Fi.y, c4 * Fi.x, R(0)};
+      }
+      if (BLK & kUT) {
+        ut = {c4 * Tj.y, -c4 * Tj.x, R(0)};
+        sut = {c4 * Ti.y, -c4 * Ti.x, R(0)};
+      }
+    }
+  }
+  if (BLK & kUF) {
+    const R s = guf * (dx * Fj.x + dy * Fj.y + dz * Fj.z);
+    a.uf.x += bij * (fuf * Fj.x + s * dx + uf.x);
+    a.uf.y += bij * (fuf * Fj.y + s * dy + uf.y);
+    a.uf.z += bij * (fuf * Fj.z + s * dz + uf.z);
+    const R ss = guf * (dx * Fi.x + dy * Fi.y + dz * Fi.z);
+    su.x += bij * (fuf * Fi.x + ss * dx + suf.x);
+    su.y += bij * (fuf * Fi.y + ss * dy + suf.y);
+    su.z += bij * (fuf * Fi.z + ss * dz + suf.z);
+  }
+  if (BLK & kWT) {
+    const R s = gwt * (dx * Tj.x + dy * Tj.y + dz * Tj.z);
+    a.wt.x += bij * (fwt * Tj.x + s * dx + wt.x);
+    a.wt.y += bij * (fwt * Tj.y + s * dy + wt.y);
+    a.wt.z += bij * (fwt * Tj.z + s * dz + wt.z);
+    const R ss = gwt * (dx * Ti.x + dy * Ti.y + dz * Ti.z);
+    sw.x += bij * (fwt * Ti.x + ss * dx + swt.x);
+    sw.y += bij * (fwt * Ti.y + ss * dy + swt.y);
+    sw.z += bij * (fwt * Ti.z + ss * dz + swt.z);
+  }
+  if (BLK & kUT) {
+    a.ut.x += bij * (c * (Tj.y * dz - Tj.z * dy) + ut.x);
+    a.ut.y += bij * (c * (Tj.z * dx - Tj.x * dz) + ut.y);
+    a.ut.z += bij * (c * (Tj.x * dy - Tj.y * dx) + ut.z);
+    const R b2 = kut * bij;
+    su.x += b2 * (sut.x - c * (Ti.y * dz - Ti.z * dy));
+    su.y += b2 * (sut.y - c * (Ti.z * dx - Ti.x * dz));
+    su.z += b2 * (sut.z - c * (Ti.x * dy - Ti.y * dx));
+  }
+  if (BLK & kWF) {
+    a.wf.x += bij * (c * (Fj.y * dz - Fj.z * dy) + wf.x);
+    a.wf.y += bij * (c * (Fj.z * dx - Fj.x * dz) + wf.y);
+    a.wf.z += bij * (c * (Fj.x * dy - Fj.y * dx) + wf.z);
+    const R b2 = kwf * bij;
+    sw.x += b2 * (swf.x - c * (Fi.y * dz - Fi.z * dy));
+    sw.y += b2 * (swf.y - c * (Fi.z * dx - Fi.x * dz));
+    sw.z += b2 * (swf.z - c * (Fi.x * dy - Fi.y * dx));
+  }
+}
+
 }  // namespace lm

@@ -314,12 +314,18 @@ void NBodySolver::setPairSymmetric(int mode) {
   sym_mode_ = mode;
 }
 
+void NBodySolver::setPairSymmetricMemory(int64_t bytes) {
+  if (bytes < 0) throw Error("[libmobility_amd] the symmetric slab budget must be >= 0");
+  sym_budget_ = bytes == 0 ? kDefaultSymBudget : bytes;
+}
+
 // Shape of a symmetric tiling request (0 = default rule): targets per lane
-// (2 from N = 65536, else 1) and offset chunks per tile (about 8192 items),
-// at most hmax.
-void NBodySolver::sym_shape(int tpl_req, int nch_req, int& tpl, int& nch, int& ntiles,
-                            int& hmax) const {
-  tpl = tpl_req != 0 ? tpl_req : (N_ >= 65536 ? 2 : 1);
+// (open translational block: 2 from N = 65536, else 1; other block sets 1,
+// at most 2) and offset chunks per tile (about 8192 items), at most hmax.
+void NBodySolver::sym_shape(bool special, int tpl_req, int nch_req, int& tpl, int& nch,
+                            int& ntiles, int& hmax) const {
+  tpl = tpl_req != 0 ? tpl_req : (special && N_ >= 65536 ? 2 : 1);
+  if (!special) tpl = std::min(tpl, 2);
   const int64_t bt = (int64_t)kPairThreads * tpl;
   ntiles = (int)((N_ + bt - 1) / bt);
   hmax = ntiles / 2;
@@ -328,85 +334,126 @@ void NBodySolver::sym_shape(int tpl_req, int nch_req, int& tpl, int& nch, int& n
   if (hmax > 0) nch = std::max(nch, 1);
 }
 
-// The user's tiling, else the autotuned one for this N and precision, else
-// the default rule.
-void NBodySolver::sym_tiling(bool fp32, int& tpl, int& nch, int& ntiles, int& hmax) const {
-  int tr = tile_tpl_, cr = tile_nch_;
-  if (tr == 0 && cr == 0 && sym_tuned_N_[fp32 ? 1 : 0] == N_) {
-    tr = sym_tuned_[fp32 ? 1 : 0].first;
-    cr = sym_tuned_[fp32 ? 1 : 0].second;
-  }
-  sym_shape(tr, cr, tpl, nch, ntiles, hmax);
-}
-
 bool NBodySolver::use_sym(int blocks) const {
-  if (sym_mode_ == 0 || blocks != kUF || kind_code() != 0 || sys_off_host_.size() != 2 || N_ <= 0)
+  if (sym_mode_ == 0 || N_ <= 0 || sys_off_host_.size() != 2) return false;
+  const int kind = kind_code();
+  if (kind != 0 && kind != 1) return false;
+  if (blocks != kUF && blocks != (kUF | kWF) && blocks != (kUT | kWT) &&
+      blocks != (kUF | kUT | kWT | kWF))
     return false;
-  if (sym_mode_ == 1) return true;
-  if (N_ < 32768) return false;
-  // the largest slab set the tilings (default, autotuned or user) can ask for
-  int tpl, nch, nt, h;
-  sym_shape(tile_tpl_ != 0 ? tile_tpl_ : 1, 64, tpl, nch, nt, h);
-  const double bytes = 8.0 * 3.0 * (double)N_ * (double)(nch + 1 + h);
-  return bytes <= 4.0 * 1024.0 * 1024.0 * 1024.0;
+  return sym_mode_ == 1 || N_ >= 32768;
 }
 
-// Items: for every tile, nch ranges of partner offsets, then the diagonal
-// tiles (lighter, they fill the tail).  Slabs: nch + 1 target slabs, then
-// hmax source slabs, summed per row in that order.
-void NBodySolver::run_sym(int tpl, int nch, int nt, int h, bool fp32, const void* rec,
-                          const double* F, double* U) {
-  if (sym_.N != N_ || sym_.tpl != tpl || sym_.nchunks != nch) {
-    std::vector<SymItem> items;
-    items.reserve((size_t)nt * (nch + 1));
+// Items: per pass, for every tile, the pass's offset chunks; the diagonal
+// tiles (lighter, they fill the tail) in the first pass.  Ring offsets are
+// split into passes whose partial-sum slabs fit kSymSlabBudget: per pass the
+// target slabs (one per chunk, plus the diagonal in the first) and one source
+// slab per offset, reduced into the output in that order, the first pass
+// assigning and later ones adding.
+void NBodySolver::ensure_sym_plan(int tpl, int nch, int nt, int h, bool ang) {
+  if (sym_.N == N_ && sym_.tpl == tpl && sym_.nchunks == nch && sym_.ang == (ang ? 1 : 0) &&
+      sym_.budget == sym_budget_)
+    return;
+  const double kSymSlabBudget = (double)sym_budget_;
+  const double unit = 8.0 * 3.0 * (double)N_ * (ang ? 2.0 : 1.0);  // bytes per slab (U and W)
+  const double per_offset = 1.0 + (h > 0 ? (double)nch / h : 0.0);
+  const int span = std::max(1, (int)((kSymSlabBudget / unit - 2.0) / per_offset));
+  std::vector<SymItem> items;
+  sym_.passes.clear();
+  sym_.max_slabs = 0;
+  int oa = 1;
+  do {
+    const int ob = std::min(h + 1, oa + span);
+    const bool first = oa == 1;
+    SymPass ps;
+    ps.item0 = (int)items.size();
+    ps.oa = oa;
+    ps.ob = ob;
+    const int width = ob - oa;
+    const int nchp = width > 0 ? std::max(1, std::min(width, (int)(((int64_t)nch * width + h - 1) / h))) : 0;
     for (int I = 0; I < nt; ++I)
-      for (int c = 0; c < nch; ++c) {
+      for (int c = 0; c < nchp; ++c) {
         SymItem it;
         it.tile = I;
-        it.o0 = 1 + (int)((int64_t)c * h / nch);
-        it.o1 = 1 + (int)((int64_t)(c + 1) * h / nch);
+        it.o0 = oa + (int)((int64_t)c * width / nchp);
+        it.o1 = oa + (int)((int64_t)(c + 1) * width / nchp);
         it.chunk = c;
         items.push_back(it);
       }
-    for (int I = 0; I < nt; ++I) items.push_back(SymItem{I, 0, 0, nch});
-    sym_.items.reserve(sizeof(SymItem) * items.size());
-    LM_HIP(hipMemcpyAsync(sym_.items.as<void>(), items.data(), sizeof(SymItem) * items.size(),
-                          hipMemcpyHostToDevice, stream_));
-    LM_HIP(hipStreamSynchronize(stream_));  // the host vector goes out of scope
-    sym_.N = N_;
-    sym_.tpl = tpl;
-    sym_.nchunks = nch;
-    sym_.ntiles = nt;
-    sym_.hmax = h;
-    sym_.nitems = (int)items.size();
-  }
-  const int64_t n3 = 3 * N_;
-  part_u_.reserve(sizeof(double) * n3 * (size_t)(nch + 1 + h));
-  double* slab_t = part_u_.as<double>();
-  double* slab_s = slab_t + (int64_t)(nch + 1) * n3;
-  SymArgs a{};
-  a.fp32 = fp32 ? 1 : 0;
-  a.tpl = tpl;
-  a.pos = rec;
-  a.F = F;
-  a.slab_t = slab_t;
-  a.slab_s = slab_s;
-  a.n3 = n3;
-  a.N = N_;
-  a.ntiles = nt;
-  a.hmax = h;
-  a.items = sym_.items.as<SymItem>();
-  a.nitems = sym_.nitems;
-  a.t0 = t0_;
-  launch_pair_sym(a, stream_);
-  launch_sym_reduce(slab_t, nch + 1, slab_s, h, n3, tpl, nt, U, stream_);
+    if (first)
+      for (int I = 0; I < nt; ++I) items.push_back(SymItem{I, 0, 0, nchp});
+    ps.nitems = (int)items.size() - ps.item0;
+    ps.nslab_t = nchp + (first ? 1 : 0);
+    sym_.max_slabs = std::max(sym_.max_slabs, (size_t)(ps.nslab_t + width));
+    sym_.passes.push_back(ps);
+    oa = ob;
+  } while (oa <= h);
+  sym_.items.reserve(sizeof(SymItem) * items.size());
+  LM_HIP(hipMemcpyAsync(sym_.items.as<void>(), items.data(), sizeof(SymItem) * items.size(),
+                        hipMemcpyHostToDevice, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));  // the host vector goes out of scope
+  sym_.N = N_;
+  sym_.tpl = tpl;
+  sym_.nchunks = nch;
+  sym_.ntiles = nt;
+  sym_.hmax = h;
+  sym_.ang = ang ? 1 : 0;
+  sym_.budget = sym_budget_;
+  sym_.nitems = (int)items.size();
 }
 
-void NBodySolver::launch_sym(bool fp32, const void* rec, const double* F, double* U) {
-  const int p = fp32 ? 1 : 0;
-  if (autotune_ && tile_tpl_ == 0 && tile_nch_ == 0 && sym_tuned_N_[p] != N_) {
-    // time every candidate on the live inputs (U is overwritten by the real
-    // launch that follows)
+void NBodySolver::run_sym(int tpl, int nch, int nt, int h, int blocks, bool fp32, const void* rec,
+                          const double* F, const double* T, double* U, double* W) {
+  const bool ang = (blocks & (kWT | kWF)) != 0;
+  ensure_sym_plan(tpl, nch, nt, h, ang);
+  const int64_t n3 = 3 * N_;
+  part_u_.reserve(sizeof(double) * n3 * sym_.max_slabs);
+  if (ang) part_w_.reserve(sizeof(double) * n3 * sym_.max_slabs);
+  for (size_t p = 0; p < sym_.passes.size(); ++p) {
+    const SymPass& ps = sym_.passes[p];
+    double* slab_t = part_u_.as<double>();
+    double* slab_s = slab_t + (int64_t)ps.nslab_t * n3;
+    double* slab_tw = ang ? part_w_.as<double>() : nullptr;
+    double* slab_sw = ang ? slab_tw + (int64_t)ps.nslab_t * n3 : nullptr;
+    SymArgs a{};
+    a.fp32 = fp32 ? 1 : 0;
+    a.tpl = tpl;
+    a.wall = kind_code();
+    a.blocks = blocks;
+    a.pos = rec;
+    a.F = F;
+    a.T = T;
+    a.slab_t = slab_t;
+    a.slab_s = slab_s;
+    a.slab_tw = slab_tw;
+    a.slab_sw = slab_sw;
+    a.n3 = n3;
+    a.N = N_;
+    a.ntiles = nt;
+    a.hmax = h;
+    a.obase = ps.oa;
+    a.items = sym_.items.as<SymItem>() + ps.item0;
+    a.nitems = ps.nitems;
+    a.t0 = t0_;
+    a.rt0 = rt0_;
+    a.r0 = r0_;
+    launch_pair_sym(a, stream_);
+    const int acc = p > 0 ? 1 : 0;
+    launch_sym_reduce(slab_t, ps.nslab_t, slab_s, ps.oa, ps.ob, n3, tpl, nt, acc, U, stream_);
+    if (ang)
+      launch_sym_reduce(slab_tw, ps.nslab_t, slab_sw, ps.oa, ps.ob, n3, tpl, nt, acc, W, stream_);
+  }
+}
+
+void NBodySolver::launch_sym(int blocks, bool fp32, const void* rec, const double* F,
+                             const double* T, double* U, double* W) {
+  const bool special = kind_code() == 0 && blocks == kUF;
+  const auto key = std::make_tuple(N_, fp32 ? 1 : 0, kind_code(), blocks);
+  // time every candidate on the live inputs (outputs are overwritten by the
+  // real launch that follows); not for very large N, where one product takes
+  // seconds
+  if (autotune_ && tile_tpl_ == 0 && tile_nch_ == 0 && N_ <= 262144 &&
+      sym_tuned_.find(key) == sym_tuned_.end()) {
     hipEvent_t e0, e1;
     LM_HIP(hipEventCreate(&e0));
     LM_HIP(hipEventCreate(&e1));
@@ -415,15 +462,16 @@ void NBodySolver::launch_sym(bool fp32, const void* rec, const double* F, double
     std::vector<std::tuple<int, int, int>> seen;
     for (int tr : {1, 2, 4})
       for (int cr : {0, 16, 32, 64}) {
+        if (tr == 4 && !special) continue;
         int tpl, nch, nt, h;
-        sym_shape(tr, cr, tpl, nch, nt, h);
+        sym_shape(special, tr, cr, tpl, nch, nt, h);
         if (std::find(seen.begin(), seen.end(), std::make_tuple(tpl, nch, nt)) != seen.end())
           continue;
         seen.emplace_back(tpl, nch, nt);
-        run_sym(tpl, nch, nt, h, fp32, rec, F, U);
+        run_sym(tpl, nch, nt, h, blocks, fp32, rec, F, T, U, W);
         LM_HIP(hipEventRecord(e0, stream_));
-        run_sym(tpl, nch, nt, h, fp32, rec, F, U);
-        run_sym(tpl, nch, nt, h, fp32, rec, F, U);
+        run_sym(tpl, nch, nt, h, blocks, fp32, rec, F, T, U, W);
+        run_sym(tpl, nch, nt, h, blocks, fp32, rec, F, T, U, W);
         LM_HIP(hipEventRecord(e1, stream_));
         LM_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -435,12 +483,19 @@ void NBodySolver::launch_sym(bool fp32, const void* rec, const double* F, double
       }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    sym_tuned_[p] = pick;
-    sym_tuned_N_[p] = N_;
+    sym_tuned_[key] = pick;
+  }
+  int tr = tile_tpl_, cr = tile_nch_;
+  if (tr == 0 && cr == 0) {
+    auto it = sym_tuned_.find(key);
+    if (it != sym_tuned_.end()) {
+      tr = it->second.first;
+      cr = it->second.second;
+    }
   }
   int tpl, nch, nt, h;
-  sym_tiling(fp32, tpl, nch, nt, h);
-  run_sym(tpl, nch, nt, h, fp32, rec, F, U);
+  sym_shape(special, tr, cr, tpl, nch, nt, h);
+  run_sym(tpl, nch, nt, h, blocks, fp32, rec, F, T, U, W);
 }
 
 void NBodySolver::setPairTiling(int tpl, int nch) {
@@ -649,7 +704,7 @@ void NBodySolver::pair_product(const void* rec64, const void* rec32, const doubl
     launch_pair_items(seg_items_.buf.as<PairItem>(), seg_items_.count, seg_items_.tpl,
                       seg_items_.nch, blocks, fp32, rec, F, T, U, W);
   } else if (use_sym(blocks)) {
-    launch_sym(fp32, rec, F, U);
+    launch_sym(blocks, fp32, rec, F, T, U, W);
     last_sym_ = true;
   } else {
     int tpl = 1, nch = 0;

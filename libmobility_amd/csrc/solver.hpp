@@ -90,9 +90,12 @@ class NBodySolver {
     *tpl = last_sym_ ? sym_.tpl : items_tpl_;
     *nch = last_sym_ ? sym_.nchunks : items_nch_;
   }
-  // Symmetric evaluation of the open translational block of one system
-  // (-1 auto: from N = 32768 while its slabs fit in 4 GiB; 0 off; 1 on).
+  // Symmetric evaluation of one open or single-wall system, any block set
+  // (-1 auto: from N = 32768; 0 off; 1 on).
   void setPairSymmetric(int mode);
+  // Memory for the symmetric path's partial sums (0: the default, 4 GiB);
+  // ring offsets are split into passes that fit it.
+  void setPairSymmetricMemory(int64_t bytes);
   bool lastPairSymmetric() const { return last_sym_; }
   int checkConvergenceSteps() const { return check_steps_; }
 
@@ -225,23 +228,32 @@ class NBodySolver {
   const SegBlock* seg_blocks_ = nullptr;
   int seg_nblocks_ = 0;
   void build_active_items(const int* act);
-  // symmetric evaluation (open boundaries, translational block, one system)
+  // symmetric evaluation of one system (open or wall, any block set)
+  struct SymPass {
+    int item0 = 0, nitems = 0, oa = 1, ob = 1, nslab_t = 0;
+  };
   struct SymPlan {
     DevBuf items;
-    int tpl = 0, nchunks = 0, ntiles = 0, hmax = 0, nitems = 0;
-    int64_t N = -1;
+    int tpl = 0, nchunks = 0, ntiles = 0, hmax = 0, nitems = 0, ang = -1;
+    int64_t N = -1, budget = 0;
+    std::vector<SymPass> passes;
+    size_t max_slabs = 0;
   };
+  static constexpr int64_t kDefaultSymBudget = (int64_t)4 << 30;
+  int64_t sym_budget_ = kDefaultSymBudget;
   SymPlan sym_;
   int sym_mode_ = -1;
   bool last_sym_ = false;
-  std::pair<int, int> sym_tuned_[2];  // (tpl, nch) per precision (FP64, FP32)
-  int64_t sym_tuned_N_[2] = {-1, -1};
-  void sym_shape(int tpl_req, int nch_req, int& tpl, int& nch, int& ntiles, int& hmax) const;
-  void sym_tiling(bool fp32, int& tpl, int& nch, int& ntiles, int& hmax) const;
+  // (N, fp32, kind, blocks) -> (targets per lane, offset chunks)
+  std::map<std::tuple<int64_t, int, int, int>, std::pair<int, int>> sym_tuned_;
+  void sym_shape(bool special, int tpl_req, int nch_req, int& tpl, int& nch, int& ntiles,
+                 int& hmax) const;
   bool use_sym(int blocks) const;
-  void run_sym(int tpl, int nch, int nt, int h, bool fp32, const void* rec, const double* F,
-               double* U);
-  void launch_sym(bool fp32, const void* rec, const double* F, double* U);
+  void ensure_sym_plan(int tpl, int nch, int nt, int h, bool ang);
+  void run_sym(int tpl, int nch, int nt, int h, int blocks, bool fp32, const void* rec,
+               const double* F, const double* T, double* U, double* W);
+  void launch_sym(int blocks, bool fp32, const void* rec, const double* F, const double* T,
+                  double* U, double* W);
   std::map<std::tuple<int, int, int>, std::vector<PairItem>> host_items_;
   uint64_t host_items_layout_ = 0;
   int64_t host_items_N_ = -1;

@@ -204,10 +204,12 @@ class NBody:
         _ffi.check(self._lib.lm_nbody_get_pair_tiling(self._h, ctypes.byref(tpl), ctypes.byref(nch)))
         return tpl.value, nch.value
 
-    def setPairSymmetric(self, mode=-1):
-        """Extension: evaluate each unordered pair once for the open
-        translational block of one system (-1 auto, 0 off, 1 on)."""
+    def setPairSymmetric(self, mode=-1, memoryBytes=0):
+        """Extension: evaluate each unordered pair once for a single open or
+        single-wall system (-1 auto, 0 off, 1 on); memoryBytes bounds its
+        partial sums (0 = 4 GiB; the work is split into passes that fit)."""
         _ffi.check(self._lib.lm_nbody_set_pair_symmetric(self._h, int(mode)))
+        _ffi.check(self._lib.lm_nbody_set_pair_symmetric_memory(self._h, int(memoryBytes)))
 
     def lastPairSymmetric(self):
         """Extension: whether the last pair product used the symmetric path."""

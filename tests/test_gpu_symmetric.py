@@ -1,10 +1,13 @@
-"""Symmetric evaluation of the open-boundary translational block (extension):
-each unordered pair is evaluated once and applied to both particles.  The
-product must equal the reference's ordered evaluation (NbodyRPY.cu:27-97,
-the kernel of hydrodynamicKernels.cuh:51-84) as restated by the oracle, for
-every tile count (one tile, even and odd counts, a partial last tile) and
-every tiling, including overlapping and coincident particles in different
-tiles; FP64 1e-12, FP32 pair terms 1e-6 relative in norm."""
+"""Symmetric evaluation (extension): each unordered pair is evaluated once and
+applied to both particles, M_ji = M_ij^T blockwise.  The product must equal
+the reference's ordered evaluation (NbodyRPY.cu:27-97 with the blocks of
+hydrodynamicKernels.cuh: open RPY :41-173, wall terms :187-431) as restated
+by the oracle.  Covered: every tile count (one tile, even and odd counts, a
+partial last tile), every tiling, open and single-wall systems, every block
+set (F, F with angular output, T, F and T), heights below the radius,
+overlapping and coincident particles in different tiles, and the ring split
+into passes; FP64 1e-12, FP32 pair terms 1e-6 (translational) / 2e-6
+(angular blocks) relative in norm."""
 import numpy as np
 import pytest
 
@@ -125,25 +128,115 @@ def test_auto_mode_picks_the_symmetric_path_for_large_open_systems():
     assert not small.lastPairSymmetric()
 
 
-def test_symmetric_path_is_not_used_for_other_blocks_or_walls():
+def _wall_solver(precision="fp64", sym=1, tpl=0, nch=0, angular=False, wall=True, memory=0):
+    from libmobility_amd import NBody
+    s = NBody("open", "open", "single_wall" if wall else "open")
+    s.setParameters(wallHeight=0.0) if wall else s.setParameters()
+    s.setPairPrecision(precision)
+    s.setPairSymmetric(sym, memory)
+    s.setPairTiling(tpl, nch)
+    s.initialize(viscosity=ETA, hydrodynamicRadius=A, includeAngular=angular)
+    return s
+
+
+def _wall_cloud(N, seed, wall=True):
+    pos = _cloud(N, seed)
+    if wall:
+        pos[:, 2] = np.random.default_rng(seed + 99).uniform(0.4, 9.0, N)  # some below the radius
+    return pos
+
+
+# (name, wall, forces, torques, angular)
+BLOCK_SETS = [
+    ("wall_F", True, True, False, False),
+    ("wall_F_angular", True, True, False, True),
+    ("wall_T", True, False, True, True),
+    ("wall_FT", True, True, True, True),
+    ("open_F_angular", False, True, False, True),
+    ("open_T", False, False, True, True),
+    ("open_FT", False, True, True, True),
+]
+
+
+@pytest.mark.parametrize("name,wall,hasF,hasT,angular", BLOCK_SETS)
+@pytest.mark.parametrize("N,tpl,nch", [(300, 1, 0), (1100, 1, 2), (2300, 2, 3)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_symmetric_blocks_vs_oracle(name, wall, hasF, hasT, angular, N, tpl, nch, precision):
+    pos = _wall_cloud(N, N + 7, wall)
+    pos[N - 9:N] = pos[0:9]  # coincident with particles of the first tile
+    rng = np.random.default_rng(N)
+    F = rng.normal(size=(N, 3)) if hasF else None
+    T = rng.normal(size=(N, 3)) if hasT else None
+    s = _wall_solver(precision, 1, tpl, nch, angular, wall)
+    s.setPositions(pos)
+    U, W = s.Mdot(F, T)
+    assert s.lastPairSymmetric()
+    Uo, Wo = oracle.mdot(pos, ETA, A, F=F, T=T, angular=angular, wall=wall)
+    tol_u = 1e-12 if precision == "fp64" else 1e-6
+    tol_w = 1e-12 if precision == "fp64" else 2e-6
+    assert _rel(U, Uo) <= tol_u
+    if angular:
+        assert _rel(W, Wo) <= tol_w
+
+
+def test_symmetric_wall_matrix_matches_dense():
+    N = 200  # one tile of 256: diagonal item only; then 600 (3 tiles)
+    for N in (200, 600):
+        pos = _wall_cloud(N, 21 + N)
+        s = _wall_solver("fp64", 1, 1, 0, True)
+        s.setPositions(pos)
+        M = oracle.dense_matrix(pos, ETA, A, angular=True, wall=True)
+        n = 3 * N
+        for c in np.random.default_rng(N).choice(6 * N, 30, replace=False):
+            e = np.zeros(6 * N)
+            e[c] = 1.0
+            u, w = s.Mdot(e[:n], e[n:])
+            col = np.concatenate([u.ravel(), w.ravel()])
+            assert np.abs(col - M[:, c]).max() <= 1e-12 * np.abs(M).max(), (N, c)
+
+
+@pytest.mark.parametrize("wall,angular", [(False, False), (True, True)])
+def test_symmetric_passes_under_a_small_memory_budget(wall, angular):
+    """A budget of a few slabs splits the ring of offsets into many passes;
+    the sum is the same product (to rounding) and still deterministic."""
+    N = 3000  # 12 tiles of 256: offsets 1..6
+    pos = _wall_cloud(N, 31, wall)
+    rng = np.random.default_rng(32)
+    F = rng.normal(size=(N, 3))
+    T = rng.normal(size=(N, 3)) if angular else None
+    one = _wall_solver("fp64", 1, 1, 0, angular, wall)
+    one.setPositions(pos)
+    U1, W1 = one.Mdot(F, T)
+    slab = 8 * 3 * N * (2 if angular else 1)
+    many = _wall_solver("fp64", 1, 1, 0, angular, wall, memory=4 * slab)
+    many.setPositions(pos)
+    U2, W2 = many.Mdot(F, T)
+    assert _rel(U2, U1) <= 1e-14
+    if angular:
+        assert _rel(W2, W1) <= 1e-14
+    U3, _ = many.Mdot(F, T)
+    assert np.array_equal(U2, U3)
+    Uo, _ = oracle.mdot(pos, ETA, A, F=F, T=T, angular=angular, wall=wall)
+    assert _rel(U2, Uo) <= 1e-12
+
+
+def test_symmetric_path_is_not_used_for_ensembles_or_when_off():
     from libmobility_amd import NBody
     pos = _cloud(600, 14)
-    ang = NBody("open", "open", "open")
-    ang.setParameters()
-    ang.setPairSymmetric(1)
-    ang.initialize(viscosity=ETA, hydrodynamicRadius=A, includeAngular=True)
-    ang.setPositions(pos)
-    ang.Mdot(np.ones((600, 3)), np.ones((600, 3)))
-    assert not ang.lastPairSymmetric()
-    wall = NBody("open", "open", "single_wall")
-    wall.setParameters(wallHeight=0.0)
-    wall.setPairSymmetric(1)
-    wall.initialize(viscosity=ETA, hydrodynamicRadius=A)
+    ens = NBody("open", "open", "open")
+    ens.setParameters()
+    ens.setPairSymmetric(1)
+    ens.initialize(viscosity=ETA, hydrodynamicRadius=A)
+    ens.setSystems([300, 300])
+    ens.setPositions(pos)
+    ens.Mdot(np.ones((600, 3)))
+    assert not ens.lastPairSymmetric()
+    off = _wall_solver("fp64", 0, 0, 0, True, True)
     p = pos.copy()
     p[:, 2] = np.abs(p[:, 2]) + 1.5
-    wall.setPositions(p)
-    wall.Mdot(np.ones((600, 3)))
-    assert not wall.lastPairSymmetric()
+    off.setPositions(p)
+    off.Mdot(np.ones((600, 3)), np.ones((600, 3)))
+    assert not off.lastPairSymmetric()
 
 
 def test_lanczos_on_the_symmetric_product():
