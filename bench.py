@@ -532,7 +532,7 @@ def bench_c2(args):
             g.mdot(h, f, t, U_d, W_d)
             g.sync()
             blocks[name] = (U_d.cpu().numpy().reshape(-1, 3), W_d.cpu().numpy().reshape(-1, 3))
-        rec = {"ms": ms, "blocks": blocks, "tiling": g.tiling(h)}
+        rec = {"ms": ms, "blocks": blocks, "tiling": g.tiling(h), "evaluation": g.evaluation(h)}
         # thermal drift (RFD, delta 1e-3 a): both handles share the seed, so
         # they draw the same random displacement
         du, dw = g.zeros(3 * N), g.zeros(3 * N)
@@ -565,6 +565,8 @@ def bench_c2(args):
         "mdot_fp64pair_valu_frac_of_fp64_peak": pairs * fpp / (ms64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
         "tiling_fp32pair": out["fp32"]["tiling"],
         "tiling_fp64pair": out["fp64"]["tiling"],
+        "pair_evaluation_fp32pair": out["fp32"]["evaluation"],
+        "pair_evaluation_fp64pair": out["fp64"]["evaluation"],
         "flops_per_pair": fpp,
         "rel_err_fp32pair_vs_fp64pair_all_rows": err32_vs_64,
         "divM_rel_diff_fp32pair_vs_fp64pair": rel(out["fp32"]["drift"], out["fp64"]["drift"]),
@@ -755,13 +757,16 @@ def bench_c5(args):
                    iteration_limit=1024, callback=progress)
     log("FP32-pair Mdot")
     ms32 = g.time_events(lambda: g.mdot(h32, F_d, T_d, U_d, W_d), 1, 0)
-    log(f"  {ms32:.1f} ms")
+    ev32 = g.evaluation(h32)
+    log(f"  {ms32:.1f} ms ({ev32} evaluation)")
     h64 = g.solver(pos_d, "fp64", wall=True, angular=True, delta=1e-4, tiling=tiling)
     log("FP64-pair Mdot")
     ms64 = g.time_events(lambda: g.mdot(h64, F_d, T_d, U_d, W_d), 1, 0)
-    log(f"  {ms64:.1f} ms")
+    ev64 = g.evaluation(h64)
+    log(f"  {ms64:.1f} ms ({ev64} evaluation)")
     g.destroy(h64)
     extra = {"mdot_fp32pair_ms": ms32, "mdot_fp64pair_ms": ms64, "flops_per_pair": fpp,
+             "pair_evaluation_fp32pair": ev32, "pair_evaluation_fp64pair": ev64,
              "tiling": {"targets_per_lane": tiling[0], "source_chunks": tiling[1]},
              "lanczos_iteration_limit": 1024}
     step_ms = None
