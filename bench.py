@@ -639,18 +639,27 @@ def bench_c4(args):
     pairs = float(sq.sum())
     flops = float(sq[kinds == 0].sum() * fp["open_UF"] + sq[kinds == 1].sum() * fp["wall_UF"])
 
-    log(f"ensemble: {len(sizes)} systems, N={N}, sum N^2={pairs:.3e}; one persistent launch")
+    log(f"ensemble: {len(sizes)} systems, N={N}, sum N^2={pairs:.3e}; one launch")
     he = g.solver(pos_d, "fp32", wall=True, tol=1e-6, sizes=sizes, kinds=kinds, persistent=1,
                   tiling=args.tiling)
     ms_e = g.time_events(lambda: g.mdot(he, F_d, None, U_d), args.steps, args.warmup)
     Ue = U_d.cpu().numpy()
-    log(f"  {ms_e:.3f} ms per ensemble Mdot, {pairs / ms_e / 1e6:.1f} Gpair/s")
+    ev = g.evaluation(he)
+    log(f"  {ms_e:.3f} ms per ensemble Mdot ({ev} evaluation), {pairs / ms_e / 1e6:.1f} Gpair/s")
+    ho = g.solver(pos_d, "fp32", wall=True, tol=1e-6, sizes=sizes, kinds=kinds, persistent=1,
+                  tiling=args.tiling)
+    g.ffi.check(g.lib.lm_nbody_set_pair_symmetric(ho, 0))
+    ms_o = g.time_events(lambda: g.mdot(ho, F_d, None, U_d), args.steps, args.warmup)
+    Uo = U_d.cpu().numpy()
+    g.destroy(ho)
+    log(f"  {ms_o:.3f} ms with the ordered evaluation (persistent grid)")
     hn = g.solver(pos_d, "fp32", wall=True, tol=1e-6, sizes=sizes, kinds=kinds, persistent=0,
                   tiling=args.tiling)
+    g.ffi.check(g.lib.lm_nbody_set_pair_symmetric(hn, 0))
     ms_n = g.time_events(lambda: g.mdot(hn, F_d, None, U_d), args.steps, args.warmup)
     Un = U_d.cpu().numpy()
     g.destroy(hn)
-    log(f"  {ms_n:.3f} ms with the non-persistent grid")
+    log(f"  {ms_n:.3f} ms with the ordered evaluation, non-persistent grid")
 
     log("per-system solves (one handle and one launch per system)")
     hs = [g.solver(pos_d[3 * a:3 * b], "fp32", wall=bool(k), tol=1e-6)
@@ -667,12 +676,16 @@ def bench_c4(args):
     log(f"  {ms_p:.3f} ms for 512 per-system Mdots")
     extra = {
         "systems": int(len(sizes)), "N_total": N, "sum_N2": pairs,
-        "mdot_ensemble_persistent_ms": ms_e,
-        "mdot_ensemble_plain_grid_ms": ms_n,
+        "pair_evaluation_ensemble": ev,
+        "mdot_ensemble_ms": ms_e,
+        "mdot_ensemble_ordered_persistent_ms": ms_o,
+        "mdot_ensemble_ordered_plain_grid_ms": ms_n,
+        "mdot_ensemble_symmetric_speedup_vs_ordered": ms_o / ms_e,
         "mdot_per_system_ms": ms_p,
         "mdot_speedup_vs_per_system": ms_p / ms_e,
         "mdot_rel_diff_ensemble_vs_per_system": rel(Ue, Up),
-        "mdot_persistent_equals_plain_grid_bitwise": bool(np.array_equal(Ue, Un)),
+        "mdot_rel_diff_ensemble_vs_ordered": rel(Ue, Uo),
+        "mdot_ordered_persistent_equals_plain_grid_bitwise": bool(np.array_equal(Uo, Un)),
         "tiling_ensemble": g.tiling(he),
         "flops_per_launch": flops,
     }

@@ -59,46 +59,74 @@ struct PairMultiArgs {
 };
 constexpr int multi_tpl(int nrhs) { return nrhs >= 4 ? 1 : 2; }
 
-// Symmetric evaluation of one system (open or bottom wall, any block set):
-// the grand mobility matrix is symmetric (M_ji = M_ij^T blockwise), so each
-// unordered pair is evaluated once and applied to both particles.  Tiles of 256 * tpl particles; tile I meets tiles I + o
-// (mod ntiles) for o = 1 .. hmax, hmax = ntiles / 2 (offset ntiles / 2 of an
-// even count only from I < ntiles / 2), so every unordered tile pair is met
-// once.  Target-side sums of an item go to target slab `chunk`, source-side
-// sums of offset o to source slab o - 1 (each row of it written by exactly one
-// item); the diagonal tile (o0 = 0) is an ordinary item into target slab
-// `nchunks`.
+// Symmetric evaluation (any block set, open or bottom-wall systems, one system
+// or an ensemble): the grand mobility matrix is symmetric (M_ji = M_ij^T
+// blockwise), so each unordered pair is evaluated once and applied to both
+// particles.  Each system is cut into tiles of 256 * tpl particles; tile I
+// meets tiles I + o (mod ntiles) for o = 1 .. hmax, hmax = ntiles / 2 (offset
+// ntiles / 2 of an even count only from I < ntiles / 2), so every unordered
+// tile pair of the system is met once.  Target-side sums of an item go to
+// target slab `chunk`, source-side sums of offset o to source slab o - obase
+// (each row of it written by exactly one item); the diagonal tile (o0 = 0) is
+// an ordinary item into target slab `nchunks`.
+struct SymSys {
+  int64_t off;     // first particle
+  int32_t n;       // particles
+  int32_t ntiles;  // tiles of 256 * tpl
+  int32_t hmax;    // ring offsets, ntiles / 2
+  int32_t wall;    // 0 open RPY, 1 bottom wall
+};
 struct SymItem {
-  int32_t tile;    // target tile I
+  int32_t sys;     // system
+  int32_t tile;    // target tile I within the system
   int32_t o0, o1;  // partner offsets [o0, o1); o0 = 0: the diagonal tile
   int32_t chunk;   // target slab
+};
+// At most 256 particles of one tile of one system, for the reduction.
+struct SymRows {
+  int64_t p0;
+  int32_t np;
+  int32_t sys;
 };
 struct SymArgs {
   int fp32;
   int tpl;
-  int wall;          // 0 open RPY, 1 bottom wall
+  int wall;          // 0 open RPY, 1 bottom wall, 2 per system
   int blocks;        // BlockBits mask
   const void* pos;   // float4 / double4 records
   const double* F;   // 3N AoS (nullptr when absent)
   const double* T;   // 3N AoS (nullptr when absent)
   double* slab_t;    // nchunks + 1 target slabs of n3 (linear velocities)
-  double* slab_s;    // hmax source slabs of n3 (linear velocities)
+  double* slab_s;    // source slabs of n3, one per ring offset of the pass
   double* slab_tw;   // as slab_t for angular velocities (nullptr without them)
   double* slab_sw;   // as slab_s for angular velocities
-  int64_t n3, N;
-  int ntiles, hmax;
+  int64_t n3;
   int obase;         // first ring offset of this pass: source slab index = o - obase
+  const SymSys* sys;
   const SymItem* items;
   int nitems;
+  const int* active;  // per-system flags; items of systems with 0 return at once (nullptr: all)
   double t0, rt0, r0;
 };
 void launch_pair_sym(const SymArgs& a, hipStream_t s);
-// out (= or +=, accumulate) the sum of the nt target slabs, then of the
-// source slabs of ring offsets o0 .. o1-1 (rows a source slab does not hold
-// are skipped), in that fixed order.
-void launch_sym_reduce(const double* slab_t, int nt, const double* slab_s, int o0, int o1,
-                       int64_t n3, int tpl, int ntiles, int accumulate, double* out,
-                       hipStream_t s);
+// Per row block: out (= or +=, accumulate) the system's target slabs of the
+// pass (its chunks of offsets [o0, o1), at most nch of them, then the
+// diagonal slab nch when `diag`), then its source slabs of offsets o0 ..
+// min(o1, hmax + 1) - 1 (offset ntiles/2 of an even count skipped for tiles
+// below ntiles/2), in that fixed order.  Blocks of systems with active == 0
+// are skipped (active nullptr: none).
+struct SymReduceArgs {
+  const double* slab_t;
+  const double* slab_s;
+  int64_t n3;
+  int nch, o0, o1, diag, accumulate, tpl;
+  const SymSys* sys;
+  const SymRows* rows;
+  int nrows;
+  const int* active;
+  double* out;
+};
+void launch_sym_reduce(const SymReduceArgs& r, hipStream_t s);
 void launch_pair_multi(const PairMultiArgs& a, hipStream_t s);
 void launch_chunk_reduce(const double* part, int nchunks, int64_t n, double* out, hipStream_t s);
 

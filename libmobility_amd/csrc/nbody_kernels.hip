@@ -255,8 +255,9 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const R4* __restrict__ pos = static_cast<const R4*>(a.pos);
   const double* __restrict__ F = a.F;
-  const int64_t N = a.N;
-  const int64_t tgt0 = (int64_t)it.tile * kTile;
+  const SymSys S = a.sys[it.sys];
+  const int64_t N = S.off + S.n;  // end of the system's particles
+  const int64_t tgt0 = S.off + (int64_t)it.tile * kTile;
 
   R4 pi[TPL];
   R fix[TPL], fiy[TPL], fiz[TPL];
@@ -282,12 +283,12 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
   }
 
   for (int o = it.o0; o < it.o1; ++o) {
-    if (2 * o == a.ntiles && 2 * it.tile >= a.ntiles) continue;  // met from the other side
+    if (2 * o == S.ntiles && 2 * it.tile >= S.ntiles) continue;  // met from the other side
     int J = it.tile + o;
-    if (J >= a.ntiles) J -= a.ntiles;
+    if (J >= S.ntiles) J -= S.ntiles;
     double* __restrict__ So = a.slab_s + (int64_t)(o - a.obase) * a.n3;
     for (int sub = 0; sub < TPL; ++sub) {
-      const int64_t base = (int64_t)J * kTile + sub * kPairThreads;
+      const int64_t base = S.off + (int64_t)J * kTile + sub * kPairThreads;
       if (base >= N) break;
       const int64_t j = base + tid;
       __syncthreads();  // the previous sub-tile is done with s_pos, s_f and s_red
@@ -424,8 +425,9 @@ __device__ inline void sym_item_gen(const SymItem& it, const SymArgs& a,
   constexpr int kComp = kAngular ? 6 : 3;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const R4* __restrict__ pos = static_cast<const R4*>(a.pos);
-  const int64_t N = a.N;
-  const int64_t tgt0 = (int64_t)it.tile * kTile;
+  const SymSys S = a.sys[it.sys];
+  const int64_t N = S.off + S.n;  // end of the system's particles
+  const int64_t tgt0 = S.off + (int64_t)it.tile * kTile;
   const R kut = R(a.rt0 / a.t0), kwf = R(a.rt0 / a.r0);
   const Vec3<R> zero = {R(0), R(0), R(0)};
 
@@ -448,11 +450,11 @@ __device__ inline void sym_item_gen(const SymItem& it, const SymArgs& a,
   }
 
   for (int o = it.o0; o < it.o1; ++o) {
-    if (2 * o == a.ntiles && 2 * it.tile >= a.ntiles) continue;  // met from the other side
+    if (2 * o == S.ntiles && 2 * it.tile >= S.ntiles) continue;  // met from the other side
     int J = it.tile + o;
-    if (J >= a.ntiles) J -= a.ntiles;
+    if (J >= S.ntiles) J -= S.ntiles;
     for (int sub = 0; sub < TPL; ++sub) {
-      const int64_t base = (int64_t)J * kTile + sub * kPairThreads;
+      const int64_t base = S.off + (int64_t)J * kTile + sub * kPairThreads;
       if (base >= N) break;
       const int64_t j = base + tid;
       __syncthreads();  // the previous sub-tile is done with the staged sources and s_red
@@ -568,38 +570,60 @@ __device__ inline void sym_item_gen(const SymItem& it, const SymArgs& a,
 }
 
 // One item per workgroup: a diagonal tile (all ordered pairs of tile I, the
-// self pairs included) or a range of partner offsets.
-template <class R, bool WALL, int BLK, int TPL>
+// self pairs included) or a range of partner offsets.  KIND: 0 open, 1 wall,
+// 2 per system (mixed ensembles).
+template <class R, int KIND, int BLK, int TPL>
 __global__ __launch_bounds__(kPairThreads) void k_pair_sym(SymArgs a) {
   using R4 = typename Rec<R>::T;
   constexpr bool kNeedF = (BLK & (kUF | kWF)) != 0;
   constexpr bool kNeedT = (BLK & (kUT | kWT)) != 0;
-  constexpr bool kSpecial = !WALL && BLK == kUF;  // the open translational fast path
   constexpr int kComp = (BLK & (kWT | kWF)) ? 6 : 3;
   __shared__ R4 s_pos[4 * 128];
   __shared__ R4 s_f[kNeedF ? 4 * 128 : 1];
   __shared__ R4 s_t[kNeedT ? 4 * 128 : 1];
   __shared__ R s_red[4 * kComp * kPairThreads];
   const SymItem it = a.items[blockIdx.x];
+  if (a.active != nullptr && a.active[it.sys] == 0) return;  // system already converged
+  const SymSys S = a.sys[it.sys];
+  const bool wall = KIND == 2 ? S.wall != 0 : KIND == 1;
   if (it.o0 == 0) {
     constexpr int kTile = kPairThreads * TPL;
     PairItem p;
-    p.tgt0 = (int64_t)it.tile * kTile;
-    p.ntgt = (int32_t)(a.N - p.tgt0 < kTile ? a.N - p.tgt0 : kTile);
+    p.tgt0 = S.off + (int64_t)it.tile * kTile;
+    const int64_t left = S.off + S.n - p.tgt0;
+    p.ntgt = (int32_t)(left < kTile ? left : kTile);
     p.src0 = p.tgt0;
     p.src1 = p.tgt0 + p.ntgt;
     p.chunk = it.chunk;
-    p.wall = WALL ? 1 : 0;
-    p.sys = 0;
-    pair_item<R, WALL, BLK, TPL>(p, static_cast<const R4*>(a.pos), a.F, a.T, a.slab_t, a.slab_tw,
-                                 a.n3, a.t0, a.rt0, a.r0, s_pos, kNeedF ? s_f : s_pos,
-                                 kNeedT ? s_t : s_pos);
+    p.wall = wall ? 1 : 0;
+    p.sys = it.sys;
+    const R4* pos = static_cast<const R4*>(a.pos);
+    R4* sf = kNeedF ? s_f : s_pos;
+    R4* st = kNeedT ? s_t : s_pos;
+    if constexpr (KIND != 0) {
+      if (wall) {
+        pair_item<R, true, BLK, TPL>(p, pos, a.F, a.T, a.slab_t, a.slab_tw, a.n3, a.t0, a.rt0, a.r0,
+                                     s_pos, sf, st);
+        return;
+      }
+    }
+    if constexpr (KIND != 1)
+      pair_item<R, false, BLK, TPL>(p, pos, a.F, a.T, a.slab_t, a.slab_tw, a.n3, a.t0, a.rt0, a.r0,
+                                    s_pos, sf, st);
     return;
   }
-  if constexpr (kSpecial)
-    sym_item<R, TPL>(it, a, s_pos, s_f, s_red);
-  else
-    sym_item_gen<R, WALL, BLK, TPL>(it, a, s_pos, s_f, s_t, s_red);
+  if constexpr (KIND != 0) {
+    if (wall) {
+      sym_item_gen<R, true, BLK, TPL>(it, a, s_pos, s_f, s_t, s_red);
+      return;
+    }
+  }
+  if constexpr (KIND != 1) {
+    if constexpr (BLK == kUF)
+      sym_item<R, TPL>(it, a, s_pos, s_f, s_red);  // the open translational fast path
+    else
+      sym_item_gen<R, false, BLK, TPL>(it, a, s_pos, s_f, s_t, s_red);
+  }
 }
 
 // ---- several force vectors per pass (translational block) ----------------------
@@ -811,15 +835,15 @@ void launch_pair(const PairArgs& a, hipStream_t s) {
     launch_pair_kind<double>(a, s);
 }
 
-template <class R, bool WALL, int BLK>
+template <class R, int KIND, int BLK>
 static void launch_pair_sym_tpl(const SymArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)a.nitems), block(kPairThreads);
   switch (a.tpl) {
-    case 1: hipLaunchKernelGGL((k_pair_sym<R, WALL, BLK, 1>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_pair_sym<R, WALL, BLK, 2>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_pair_sym<R, KIND, BLK, 1>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_pair_sym<R, KIND, BLK, 2>), grid, block, 0, s, a); break;
     case 4:
-      if constexpr (!WALL && BLK == kUF) {
-        hipLaunchKernelGGL((k_pair_sym<R, WALL, BLK, 4>), grid, block, 0, s, a);
+      if constexpr (KIND == 0 && BLK == kUF) {
+        hipLaunchKernelGGL((k_pair_sym<R, KIND, BLK, 4>), grid, block, 0, s, a);
         break;
       }
       [[fallthrough]];
@@ -828,23 +852,25 @@ static void launch_pair_sym_tpl(const SymArgs& a, hipStream_t s) {
   LM_HIP(hipGetLastError());
 }
 
-template <class R, bool WALL>
+template <class R, int KIND>
 static void launch_pair_sym_blk(const SymArgs& a, hipStream_t s) {
   switch (a.blocks) {
-    case kUF: launch_pair_sym_tpl<R, WALL, kUF>(a, s); break;
-    case kUF | kWF: launch_pair_sym_tpl<R, WALL, kUF | kWF>(a, s); break;
-    case kUT | kWT: launch_pair_sym_tpl<R, WALL, kUT | kWT>(a, s); break;
-    case kUF | kUT | kWT | kWF: launch_pair_sym_tpl<R, WALL, kUF | kUT | kWT | kWF>(a, s); break;
+    case kUF: launch_pair_sym_tpl<R, KIND, kUF>(a, s); break;
+    case kUF | kWF: launch_pair_sym_tpl<R, KIND, kUF | kWF>(a, s); break;
+    case kUT | kWT: launch_pair_sym_tpl<R, KIND, kUT | kWT>(a, s); break;
+    case kUF | kUT | kWT | kWF: launch_pair_sym_tpl<R, KIND, kUF | kUT | kWT | kWF>(a, s); break;
     default: throw Error("[libmobility_amd] unsupported block combination");
   }
 }
 
 template <class R>
 static void launch_pair_sym_r(const SymArgs& a, hipStream_t s) {
-  if (a.wall)
-    launch_pair_sym_blk<R, true>(a, s);
-  else
-    launch_pair_sym_blk<R, false>(a, s);
+  switch (a.wall) {
+    case 0: launch_pair_sym_blk<R, 0>(a, s); break;
+    case 1: launch_pair_sym_blk<R, 1>(a, s); break;
+    case 2: launch_pair_sym_blk<R, 2>(a, s); break;
+    default: throw Error("[libmobility_amd] invalid kernel kind");
+  }
 }
 
 void launch_pair_sym(const SymArgs& a, hipStream_t s) {
@@ -855,30 +881,33 @@ void launch_pair_sym(const SymArgs& a, hipStream_t s) {
     launch_pair_sym_r<double>(a, s);
 }
 
-__global__ void k_sym_reduce(const double* __restrict__ slab_t, int nt,
-                             const double* __restrict__ slab_s, int o0, int o1, int64_t n,
-                             int64_t tile3, int ntiles, int accumulate, double* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = accumulate ? out[i] : 0.0;
-  for (int c = 0; c < nt; ++c) s += slab_t[(int64_t)c * n + i];
+// One workgroup per row block (at most 256 particles of one tile of one system).
+__global__ __launch_bounds__(256) void k_sym_reduce(SymReduceArgs r) {
+  const SymRows rb = r.rows[blockIdx.x];
+  if (r.active != nullptr && r.active[rb.sys] == 0) return;
+  const SymSys S = r.sys[rb.sys];
+  const int64_t kTile = (int64_t)kPairThreads * r.tpl;
+  const int tile = (int)((rb.p0 - S.off) / kTile);
+  const int hi = min(r.o1, S.hmax + 1);  // this system's offsets in the pass: [o0, hi)
+  const int width = hi - r.o0;
+  const int nchp = width > 0 ? min(r.nch, width) : 0;
   // an even tile count meets offset ntiles/2 only from tiles I < ntiles/2, so
   // that slab holds no rows of tiles J < ntiles/2
-  const int tile = (int)(i / tile3);
-  const int skip = (ntiles % 2 == 0 && 2 * tile < ntiles) ? ntiles / 2 : 0;
-  for (int o = o0; o < o1; ++o)
-    if (o != skip) s += slab_s[(int64_t)(o - o0) * n + i];
-  out[i] = s;
+  const int skip = (S.ntiles % 2 == 0 && 2 * tile < S.ntiles) ? S.ntiles / 2 : 0;
+  const int64_t n = r.n3;
+  for (int64_t e = 3 * rb.p0 + threadIdx.x; e < 3 * (rb.p0 + rb.np); e += blockDim.x) {
+    double s = r.accumulate ? r.out[e] : 0.0;
+    for (int c = 0; c < nchp; ++c) s += r.slab_t[(int64_t)c * n + e];
+    if (r.diag) s += r.slab_t[(int64_t)r.nch * n + e];
+    for (int o = r.o0; o < hi; ++o)
+      if (o != skip) s += r.slab_s[(int64_t)(o - r.o0) * n + e];
+    r.out[e] = s;
+  }
 }
 
-void launch_sym_reduce(const double* slab_t, int nt, const double* slab_s, int o0, int o1,
-                       int64_t n3, int tpl, int ntiles, int accumulate, double* out,
-                       hipStream_t s) {
-  if (n3 <= 0) return;
-  const int nt_ = 256;
-  hipLaunchKernelGGL(k_sym_reduce, dim3((unsigned)((n3 + nt_ - 1) / nt_)), dim3(nt_), 0, s, slab_t,
-                     nt, slab_s, o0, o1, n3, (int64_t)3 * kPairThreads * tpl, ntiles, accumulate,
-                     out);
+void launch_sym_reduce(const SymReduceArgs& r, hipStream_t s) {
+  if (r.nrows <= 0) return;
+  hipLaunchKernelGGL(k_sym_reduce, dim3((unsigned)r.nrows), dim3(256), 0, s, r);
   LM_HIP(hipGetLastError());
 }
 

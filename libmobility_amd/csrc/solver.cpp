@@ -320,44 +320,64 @@ void NBodySolver::setPairSymmetricMemory(int64_t bytes) {
 }
 
 // Shape of a symmetric tiling request (0 = default rule): targets per lane
-// (open translational block: 2 from N = 65536, else 1; other block sets 1,
-// at most 2) and offset chunks per tile (about 8192 items), at most hmax.
+// (open translational block of one system: 2 from N = 65536, else 1; other
+// cases 1, at most 2) and offset chunks per tile (about 8192 items), at most
+// the longest ring.  ntiles is the total over systems, hmax the longest ring.
 void NBodySolver::sym_shape(bool special, int tpl_req, int nch_req, int& tpl, int& nch,
                             int& ntiles, int& hmax) const {
-  tpl = tpl_req != 0 ? tpl_req : (special && N_ >= 65536 ? 2 : 1);
+  const int nsys = (int)sys_off_host_.size() - 1;
+  tpl = tpl_req != 0 ? tpl_req : (special && nsys == 1 && N_ >= 65536 ? 2 : 1);
   if (!special) tpl = std::min(tpl, 2);
   const int64_t bt = (int64_t)kPairThreads * tpl;
-  ntiles = (int)((N_ + bt - 1) / bt);
-  hmax = ntiles / 2;
-  nch = nch_req != 0 ? nch_req : (8192 + ntiles - 1) / ntiles;
+  ntiles = 0;
+  hmax = 0;
+  for (int s = 0; s < nsys; ++s) {
+    const int nt = (int)((sys_off_host_[s + 1] - sys_off_host_[s] + bt - 1) / bt);
+    ntiles += nt;
+    hmax = std::max(hmax, nt / 2);
+  }
+  nch = nch_req != 0 ? nch_req : (8192 + std::max(ntiles, 1) - 1) / std::max(ntiles, 1);
   nch = std::min(nch, hmax);
   if (hmax > 0) nch = std::max(nch, 1);
 }
 
 bool NBodySolver::use_sym(int blocks) const {
-  if (sym_mode_ == 0 || N_ <= 0 || sys_off_host_.size() != 2) return false;
-  const int kind = kind_code();
-  if (kind != 0 && kind != 1) return false;
+  if (sym_mode_ == 0 || N_ <= 0) return false;
   if (blocks != kUF && blocks != (kUF | kWF) && blocks != (kUT | kWT) &&
       blocks != (kUF | kUT | kWT | kWF))
     return false;
   return sym_mode_ == 1 || N_ >= 32768;
 }
 
-// Items: per pass, for every tile, the pass's offset chunks; the diagonal
-// tiles (lighter, they fill the tail) in the first pass.  Ring offsets are
-// split into passes whose partial-sum slabs fit kSymSlabBudget: per pass the
-// target slabs (one per chunk, plus the diagonal in the first) and one source
-// slab per offset, reduced into the output in that order, the first pass
+// Items: per pass, for every system and tile, the system's chunks of the
+// pass's offsets; the diagonal tiles in the first pass.  Ensembles order each
+// pass longest first (wall pairs weigh 5).  Ring offsets are split into
+// passes whose partial-sum slabs fit the memory budget: per pass the target
+// slabs (one per chunk, plus the diagonal in the first) and one source slab
+// per offset, reduced into the output in that order, the first pass
 // assigning and later ones adding.
-void NBodySolver::ensure_sym_plan(int tpl, int nch, int nt, int h, bool ang) {
-  if (sym_.N == N_ && sym_.tpl == tpl && sym_.nchunks == nch && sym_.ang == (ang ? 1 : 0) &&
-      sym_.budget == sym_budget_)
+void NBodySolver::ensure_sym_plan(int tpl, int nch, int h, bool ang) {
+  if (sym_.N == N_ && sym_.layout == layout_id_ && sym_.tpl == tpl && sym_.nchunks == nch &&
+      sym_.ang == (ang ? 1 : 0) && sym_.budget == sym_budget_)
     return;
-  const double kSymSlabBudget = (double)sym_budget_;
+  const int nsys = (int)sys_off_host_.size() - 1;
+  const int64_t bt = (int64_t)kPairThreads * tpl;
+  std::vector<SymSys> systems(nsys);
+  std::vector<SymRows> rows;
+  for (int s = 0; s < nsys; ++s) {
+    const int64_t b = sys_off_host_[s], e = sys_off_host_[s + 1];
+    SymSys& S = systems[s];
+    S.off = b;
+    S.n = (int32_t)(e - b);
+    S.ntiles = (int32_t)((e - b + bt - 1) / bt);
+    S.hmax = S.ntiles / 2;
+    S.wall = s < (int)sys_kind_host_.size() ? sys_kind_host_[s] : (wall_ ? 1 : 0);
+    for (int64_t p0 = b; p0 < e; p0 += kPairThreads)
+      rows.push_back(SymRows{p0, (int32_t)std::min<int64_t>(kPairThreads, e - p0), s});
+  }
   const double unit = 8.0 * 3.0 * (double)N_ * (ang ? 2.0 : 1.0);  // bytes per slab (U and W)
   const double per_offset = 1.0 + (h > 0 ? (double)nch / h : 0.0);
-  const int span = std::max(1, (int)((kSymSlabBudget / unit - 2.0) / per_offset));
+  const int span = std::max(1, (int)(((double)sym_budget_ / unit - 2.0) / per_offset));
   std::vector<SymItem> items;
   sym_.passes.clear();
   sym_.max_slabs = 0;
@@ -365,60 +385,116 @@ void NBodySolver::ensure_sym_plan(int tpl, int nch, int nt, int h, bool ang) {
   do {
     const int ob = std::min(h + 1, oa + span);
     const bool first = oa == 1;
+    const int width = ob - oa;
     SymPass ps;
     ps.item0 = (int)items.size();
     ps.oa = oa;
     ps.ob = ob;
-    const int width = ob - oa;
-    const int nchp = width > 0 ? std::max(1, std::min(width, (int)(((int64_t)nch * width + h - 1) / h))) : 0;
-    for (int I = 0; I < nt; ++I)
-      for (int c = 0; c < nchp; ++c) {
-        SymItem it;
-        it.tile = I;
-        it.o0 = oa + (int)((int64_t)c * width / nchp);
-        it.o1 = oa + (int)((int64_t)(c + 1) * width / nchp);
-        it.chunk = c;
-        items.push_back(it);
-      }
-    if (first)
-      for (int I = 0; I < nt; ++I) items.push_back(SymItem{I, 0, 0, nchp});
+    ps.nch = width > 0 ? std::max(1, std::min(width, (int)(((int64_t)nch * width + h - 1) / h))) : 0;
+    ps.diag = first ? 1 : 0;
+    std::vector<std::pair<double, SymItem>> pass_items;
+    for (int s = 0; s < nsys; ++s) {
+      const SymSys& S = systems[s];
+      const double wgt = S.wall ? 5.0 : 1.0;
+      const int ws = std::min(ob, S.hmax + 1) - oa;  // this system's offsets in the pass
+      const int cs = ws > 0 ? std::min(ps.nch, ws) : 0;
+      for (int I = 0; I < S.ntiles; ++I)
+        for (int c = 0; c < cs; ++c) {
+          SymItem it;
+          it.sys = s;
+          it.tile = I;
+          it.o0 = oa + (int)((int64_t)c * ws / cs);
+          it.o1 = oa + (int)((int64_t)(c + 1) * ws / cs);
+          it.chunk = c;
+          pass_items.emplace_back(wgt * (it.o1 - it.o0), it);
+        }
+      if (first)
+        for (int I = 0; I < S.ntiles; ++I) {
+          const double nt = (double)std::min<int64_t>(bt, S.n - (int64_t)I * bt) / bt;
+          pass_items.emplace_back(0.5 * wgt * nt * nt, SymItem{s, I, 0, 0, ps.nch});
+        }
+    }
+    if (nsys > 1)
+      std::stable_sort(pass_items.begin(), pass_items.end(),
+                       [](const std::pair<double, SymItem>& x, const std::pair<double, SymItem>& y) {
+                         return x.first > y.first;
+                       });
+    for (const auto& pi : pass_items) items.push_back(pi.second);
     ps.nitems = (int)items.size() - ps.item0;
-    ps.nslab_t = nchp + (first ? 1 : 0);
-    sym_.max_slabs = std::max(sym_.max_slabs, (size_t)(ps.nslab_t + width));
+    sym_.max_slabs = std::max(sym_.max_slabs, (size_t)(ps.nch + ps.diag + width));
     sym_.passes.push_back(ps);
     oa = ob;
   } while (oa <= h);
-  sym_.items.reserve(sizeof(SymItem) * items.size());
+  sym_.items.reserve(sizeof(SymItem) * std::max<size_t>(items.size(), 1));
+  sym_.sys.reserve(sizeof(SymSys) * systems.size());
+  sym_.rows.reserve(sizeof(SymRows) * rows.size());
   LM_HIP(hipMemcpyAsync(sym_.items.as<void>(), items.data(), sizeof(SymItem) * items.size(),
                         hipMemcpyHostToDevice, stream_));
-  LM_HIP(hipStreamSynchronize(stream_));  // the host vector goes out of scope
+  LM_HIP(hipMemcpyAsync(sym_.sys.as<void>(), systems.data(), sizeof(SymSys) * systems.size(),
+                        hipMemcpyHostToDevice, stream_));
+  LM_HIP(hipMemcpyAsync(sym_.rows.as<void>(), rows.data(), sizeof(SymRows) * rows.size(),
+                        hipMemcpyHostToDevice, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));  // the host vectors go out of scope (items are kept)
+  sym_.host_items = std::move(items);
   sym_.N = N_;
+  sym_.layout = layout_id_;
   sym_.tpl = tpl;
   sym_.nchunks = nch;
-  sym_.ntiles = nt;
   sym_.hmax = h;
   sym_.ang = ang ? 1 : 0;
   sym_.budget = sym_budget_;
-  sym_.nitems = (int)items.size();
+  sym_.nitems = (int)sym_.host_items.size();
+  sym_.nrows = (int)rows.size();
+  sym_.active_valid = false;
 }
 
-void NBodySolver::run_sym(int tpl, int nch, int nt, int h, int blocks, bool fp32, const void* rec,
+// Per-system Lanczos: the items of the systems still iterating, pass by pass
+// (filtered from the plan's host list, already in launch order).
+void NBodySolver::build_sym_active(const int* act) {
+  sym_.active_valid = false;
+  if (sym_.N != N_ || sym_.layout != layout_id_ || sym_.host_items.empty()) return;
+  sym_.active_passes = sym_.passes;
+  sym_active_host_.reserve(sizeof(SymItem) * sym_.host_items.size());
+  SymItem* dst = sym_active_host_.as<SymItem>();
+  int count = 0;
+  for (size_t p = 0; p < sym_.passes.size(); ++p) {
+    const SymPass& ps = sym_.passes[p];
+    sym_.active_passes[p].item0 = count;
+    for (int k = ps.item0; k < ps.item0 + ps.nitems; ++k)
+      if (act[sym_.host_items[k].sys]) dst[count++] = sym_.host_items[k];
+    sym_.active_passes[p].nitems = count - sym_.active_passes[p].item0;
+  }
+  // the previous upload from the pinned buffer has completed: the caller
+  // synchronised the stream after it
+  sym_.active_items.reserve(sizeof(SymItem) * std::max(count, 1));
+  if (count > 0)
+    LM_HIP(hipMemcpyAsync(sym_.active_items.as<void>(), dst, sizeof(SymItem) * count,
+                          hipMemcpyHostToDevice, stream_));
+  sym_.active_valid = true;
+}
+
+void NBodySolver::run_sym(int tpl, int nch, int h, int blocks, bool fp32, const void* rec,
                           const double* F, const double* T, double* U, double* W) {
   const bool ang = (blocks & (kWT | kWF)) != 0;
-  ensure_sym_plan(tpl, nch, nt, h, ang);
+  ensure_sym_plan(tpl, nch, h, ang);
+  const bool compact = pair_active_ != nullptr && sym_.active_valid;
+  const std::vector<SymPass>& passes = compact ? sym_.active_passes : sym_.passes;
+  const SymItem* items = compact ? sym_.active_items.as<SymItem>() : sym_.items.as<SymItem>();
   const int64_t n3 = 3 * N_;
   part_u_.reserve(sizeof(double) * n3 * sym_.max_slabs);
   if (ang) part_w_.reserve(sizeof(double) * n3 * sym_.max_slabs);
-  for (size_t p = 0; p < sym_.passes.size(); ++p) {
-    const SymPass& ps = sym_.passes[p];
+  const int nsys = (int)sys_off_host_.size() - 1;
+  for (size_t p = 0; p < passes.size(); ++p) {
+    const SymPass& ps = passes[p];
+    const int nslab_t = ps.nch + ps.diag;
     double* slab_t = part_u_.as<double>();
-    double* slab_s = slab_t + (int64_t)ps.nslab_t * n3;
+    double* slab_s = slab_t + (int64_t)nslab_t * n3;
     double* slab_tw = ang ? part_w_.as<double>() : nullptr;
-    double* slab_sw = ang ? slab_tw + (int64_t)ps.nslab_t * n3 : nullptr;
+    double* slab_sw = ang ? slab_tw + (int64_t)nslab_t * n3 : nullptr;
     SymArgs a{};
     a.fp32 = fp32 ? 1 : 0;
     a.tpl = tpl;
-    a.wall = kind_code();
+    a.wall = nsys > 1 ? 2 : kind_code();
     a.blocks = blocks;
     a.pos = rec;
     a.F = F;
@@ -428,31 +504,53 @@ void NBodySolver::run_sym(int tpl, int nch, int nt, int h, int blocks, bool fp32
     a.slab_tw = slab_tw;
     a.slab_sw = slab_sw;
     a.n3 = n3;
-    a.N = N_;
-    a.ntiles = nt;
-    a.hmax = h;
     a.obase = ps.oa;
-    a.items = sym_.items.as<SymItem>() + ps.item0;
+    a.sys = sym_.sys.as<SymSys>();
+    a.items = items + ps.item0;
     a.nitems = ps.nitems;
+    a.active = pair_active_;
     a.t0 = t0_;
     a.rt0 = rt0_;
     a.r0 = r0_;
     launch_pair_sym(a, stream_);
-    const int acc = p > 0 ? 1 : 0;
-    launch_sym_reduce(slab_t, ps.nslab_t, slab_s, ps.oa, ps.ob, n3, tpl, nt, acc, U, stream_);
-    if (ang)
-      launch_sym_reduce(slab_tw, ps.nslab_t, slab_sw, ps.oa, ps.ob, n3, tpl, nt, acc, W, stream_);
+    SymReduceArgs r{};
+    r.slab_t = slab_t;
+    r.slab_s = slab_s;
+    r.n3 = n3;
+    r.nch = ps.nch;
+    r.o0 = ps.oa;
+    r.o1 = ps.ob;
+    r.diag = ps.diag;
+    r.accumulate = p > 0 ? 1 : 0;
+    r.tpl = tpl;
+    r.sys = sym_.sys.as<SymSys>();
+    r.rows = sym_.rows.as<SymRows>();
+    r.nrows = sym_.nrows;
+    r.active = pair_active_;
+    r.out = U;
+    launch_sym_reduce(r, stream_);
+    if (ang) {
+      r.slab_t = slab_tw;
+      r.slab_s = slab_sw;
+      r.out = W;
+      launch_sym_reduce(r, stream_);
+    }
   }
 }
 
 void NBodySolver::launch_sym(int blocks, bool fp32, const void* rec, const double* F,
                              const double* T, double* U, double* W) {
+  const int nsys = (int)sys_off_host_.size() - 1;
   const bool special = kind_code() == 0 && blocks == kUF;
-  const auto key = std::make_tuple(N_, fp32 ? 1 : 0, kind_code(), blocks);
+  const auto key = std::make_tuple(N_, fp32 ? 1 : 0, kind_code() + 4 * nsys, blocks);
+  double pairs = 0.0;
+  for (int s = 0; s < nsys; ++s) {
+    const double n = (double)(sys_off_host_[s + 1] - sys_off_host_[s]);
+    pairs += n * n;
+  }
   // time every candidate on the live inputs (outputs are overwritten by the
-  // real launch that follows); not for very large N, where one product takes
-  // seconds
-  if (autotune_ && tile_tpl_ == 0 && tile_nch_ == 0 && N_ <= 262144 &&
+  // real launch that follows); not where one product takes seconds
+  if (autotune_ && tile_tpl_ == 0 && tile_nch_ == 0 && pairs <= 1e11 &&
       sym_tuned_.find(key) == sym_tuned_.end()) {
     hipEvent_t e0, e1;
     LM_HIP(hipEventCreate(&e0));
@@ -462,16 +560,16 @@ void NBodySolver::launch_sym(int blocks, bool fp32, const void* rec, const doubl
     std::vector<std::tuple<int, int, int>> seen;
     for (int tr : {1, 2, 4})
       for (int cr : {0, 16, 32, 64}) {
-        if (tr == 4 && !special) continue;
+        if (tr == 4 && !(special && nsys == 1)) continue;
         int tpl, nch, nt, h;
         sym_shape(special, tr, cr, tpl, nch, nt, h);
         if (std::find(seen.begin(), seen.end(), std::make_tuple(tpl, nch, nt)) != seen.end())
           continue;
         seen.emplace_back(tpl, nch, nt);
-        run_sym(tpl, nch, nt, h, blocks, fp32, rec, F, T, U, W);
+        run_sym(tpl, nch, h, blocks, fp32, rec, F, T, U, W);
         LM_HIP(hipEventRecord(e0, stream_));
-        run_sym(tpl, nch, nt, h, blocks, fp32, rec, F, T, U, W);
-        run_sym(tpl, nch, nt, h, blocks, fp32, rec, F, T, U, W);
+        run_sym(tpl, nch, h, blocks, fp32, rec, F, T, U, W);
+        run_sym(tpl, nch, h, blocks, fp32, rec, F, T, U, W);
         LM_HIP(hipEventRecord(e1, stream_));
         LM_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -495,7 +593,7 @@ void NBodySolver::launch_sym(int blocks, bool fp32, const void* rec, const doubl
   }
   int tpl, nch, nt, h;
   sym_shape(special, tr, cr, tpl, nch, nt, h);
-  run_sym(tpl, nch, nt, h, blocks, fp32, rec, F, T, U, W);
+  run_sym(tpl, nch, h, blocks, fp32, rec, F, T, U, W);
 }
 
 void NBodySolver::setPairTiling(int tpl, int nch) {
@@ -699,13 +797,13 @@ void NBodySolver::pair_product(const void* rec64, const void* rec32, const doubl
   const bool fp32 = pair_prec_ == kPairFP32;
   const void* rec = fp32 ? rec32 : rec64;
   last_sym_ = false;
-  if (seg_items_.count > 0) {
+  if (use_sym(blocks)) {
+    launch_sym(blocks, fp32, rec, F, T, U, W);
+    last_sym_ = true;
+  } else if (seg_items_.count > 0) {
     // per-system Lanczos after some systems converged: their work is dropped
     launch_pair_items(seg_items_.buf.as<PairItem>(), seg_items_.count, seg_items_.tpl,
                       seg_items_.nch, blocks, fp32, rec, F, T, U, W);
-  } else if (use_sym(blocks)) {
-    launch_sym(blocks, fp32, rec, F, T, U, W);
-    last_sym_ = true;
   } else {
     int tpl = 1, nch = 0;
     choose_tiling(blocks, fp32, F, T, U, W, rec, tpl, nch);
@@ -1037,6 +1135,7 @@ int NBodySolver::lanczos_run_seg(SegSolve& S, const std::vector<int64_t>& off, b
       s.pair_active_ = nullptr;
       s.seg_blocks_ = nullptr;
       s.seg_items_.count = 0;
+      s.sym_.active_valid = false;
     }
   } guard{*this};
   if (skip_converged_pairs) {
@@ -1104,7 +1203,10 @@ int NBodySolver::lanczos_run_seg(SegSolve& S, const std::vector<int64_t>& off, b
     launch_seg_store(y, old, g, sc, stream_);
     if (changed) {
       LM_HIP(hipMemcpyAsync(d_active, h_active, sizeof(int) * nrec, hipMemcpyHostToDevice, stream_));
-      if (skip_converged_pairs && nact > 0) build_active_items(h_active);
+      if (skip_converged_pairs && nact > 0) {
+        build_active_items(h_active);
+        build_sym_active(h_active);
+      }
     }
   }
   if (nact > 0) throw Error("[Lanczos] Could not converge");

@@ -228,30 +228,35 @@ class NBodySolver {
   const SegBlock* seg_blocks_ = nullptr;
   int seg_nblocks_ = 0;
   void build_active_items(const int* act);
-  // symmetric evaluation of one system (open or wall, any block set)
+  // symmetric evaluation (open or wall systems, any block set, ensembles)
   struct SymPass {
-    int item0 = 0, nitems = 0, oa = 1, ob = 1, nslab_t = 0;
+    int item0 = 0, nitems = 0, oa = 1, ob = 1, nch = 0, diag = 0;
   };
   struct SymPlan {
-    DevBuf items;
-    int tpl = 0, nchunks = 0, ntiles = 0, hmax = 0, nitems = 0, ang = -1;
+    DevBuf items, sys, rows, active_items;
+    std::vector<SymItem> host_items;  // all passes, in launch order
+    int tpl = 0, nchunks = 0, hmax = 0, nitems = 0, nrows = 0, ang = -1;
     int64_t N = -1, budget = 0;
-    std::vector<SymPass> passes;
+    uint64_t layout = 0;
+    std::vector<SymPass> passes, active_passes;
     size_t max_slabs = 0;
+    bool active_valid = false;  // active_items / active_passes hold the iterating systems
   };
   static constexpr int64_t kDefaultSymBudget = (int64_t)4 << 30;
   int64_t sym_budget_ = kDefaultSymBudget;
   SymPlan sym_;
+  HostBuf sym_active_host_;
   int sym_mode_ = -1;
   bool last_sym_ = false;
-  // (N, fp32, kind, blocks) -> (targets per lane, offset chunks)
+  // (N, fp32, kind + 4 * systems, blocks) -> (targets per lane, offset chunks)
   std::map<std::tuple<int64_t, int, int, int>, std::pair<int, int>> sym_tuned_;
   void sym_shape(bool special, int tpl_req, int nch_req, int& tpl, int& nch, int& ntiles,
                  int& hmax) const;
   bool use_sym(int blocks) const;
-  void ensure_sym_plan(int tpl, int nch, int nt, int h, bool ang);
-  void run_sym(int tpl, int nch, int nt, int h, int blocks, bool fp32, const void* rec,
-               const double* F, const double* T, double* U, double* W);
+  void ensure_sym_plan(int tpl, int nch, int h, bool ang);
+  void build_sym_active(const int* act);
+  void run_sym(int tpl, int nch, int h, int blocks, bool fp32, const void* rec, const double* F,
+               const double* T, double* U, double* W);
   void launch_sym(int blocks, bool fp32, const void* rec, const double* F, const double* T,
                   double* U, double* W);
   std::map<std::tuple<int, int, int>, std::vector<PairItem>> host_items_;

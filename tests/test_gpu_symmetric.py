@@ -220,23 +220,108 @@ def test_symmetric_passes_under_a_small_memory_budget(wall, angular):
     assert _rel(U2, Uo) <= 1e-12
 
 
-def test_symmetric_path_is_not_used_for_ensembles_or_when_off():
-    from libmobility_amd import NBody
-    pos = _cloud(600, 14)
-    ens = NBody("open", "open", "open")
-    ens.setParameters()
-    ens.setPairSymmetric(1)
-    ens.initialize(viscosity=ETA, hydrodynamicRadius=A)
-    ens.setSystems([300, 300])
-    ens.setPositions(pos)
-    ens.Mdot(np.ones((600, 3)))
-    assert not ens.lastPairSymmetric()
+def test_symmetric_path_is_not_used_when_off():
+    pos = _wall_cloud(600, 14)
     off = _wall_solver("fp64", 0, 0, 0, True, True)
-    p = pos.copy()
-    p[:, 2] = np.abs(p[:, 2]) + 1.5
-    off.setPositions(p)
+    off.setPositions(pos)
     off.Mdot(np.ones((600, 3)), np.ones((600, 3)))
     assert not off.lastPairSymmetric()
+
+
+# ragged ensembles: tile counts 1 (tiny and one-tile systems), 2, 3 and more,
+# partial last tiles, open and wall systems mixed
+ENSEMBLE_SIZES = [5, 256, 300, 700, 1, 1500, 60, 2300]
+
+
+def _ensemble_solver(precision, tpl=0, nch=0, angular=False, memory=0, kinds=None):
+    from libmobility_amd import NBody
+    s = NBody("open", "open", "single_wall")
+    s.setParameters(wallHeight=0.0)
+    s.setPairPrecision(precision)
+    s.setPairSymmetric(1, memory)
+    s.setPairTiling(tpl, nch)
+    s.initialize(viscosity=ETA, hydrodynamicRadius=A, includeAngular=angular, tolerance=1e-6)
+    k = kinds if kinds is not None else [i % 2 for i in range(len(ENSEMBLE_SIZES))]
+    s.setSystems(ENSEMBLE_SIZES, ["single_wall" if x else "open" for x in k])
+    return s, k
+
+
+def _ensemble_positions(seed, kinds):
+    parts = []
+    for i, (n, k) in enumerate(zip(ENSEMBLE_SIZES, kinds)):
+        parts.append(_wall_cloud(n, seed + i, wall=bool(k)))
+    return np.concatenate(parts)
+
+
+@pytest.mark.parametrize("hasF,hasT,angular", [(True, False, False), (True, False, True),
+                                                (False, True, True), (True, True, True)])
+@pytest.mark.parametrize("tpl,nch", [(1, 0), (2, 2)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_symmetric_ensemble_vs_oracle(hasF, hasT, angular, tpl, nch, precision):
+    s, kinds = _ensemble_solver(precision, tpl, nch, angular)
+    pos = _ensemble_positions(40, kinds)
+    N = pos.shape[0]
+    pos[N - 4:N] = pos[N - 2300:N - 2296]  # coincident particles inside the last system
+    rng = np.random.default_rng(41)
+    F = rng.normal(size=(N, 3)) if hasF else None
+    T = rng.normal(size=(N, 3)) if hasT else None
+    s.setPositions(pos)
+    U, W = s.Mdot(F, T)
+    assert s.lastPairSymmetric()
+    Uo, Wo = oracle.mdot(pos, ETA, A, F=F, T=T, angular=angular, wall=kinds,
+                         sys_sizes=ENSEMBLE_SIZES)
+    tol_u = 1e-12 if precision == "fp64" else 1e-6
+    tol_w = 1e-12 if precision == "fp64" else 2e-6
+    assert _rel(U, Uo) <= tol_u
+    if angular:
+        assert _rel(W, Wo) <= tol_w
+
+
+def test_symmetric_ensemble_passes_and_determinism():
+    s, kinds = _ensemble_solver("fp64", 1, 0, True)
+    pos = _ensemble_positions(50, kinds)
+    N = pos.shape[0]
+    s.setPositions(pos)
+    rng = np.random.default_rng(51)
+    F, T = rng.normal(size=(N, 3)), rng.normal(size=(N, 3))
+    U1, W1 = s.Mdot(F, T)
+    small, _ = _ensemble_solver("fp64", 1, 0, True, memory=4 * 8 * 3 * N * 2)
+    small.setPositions(pos)
+    U2, W2 = small.Mdot(F, T)
+    assert _rel(U2, U1) <= 1e-14 and _rel(W2, W1) <= 1e-14
+    U3, W3 = small.Mdot(F, T)
+    assert np.array_equal(U2, U3) and np.array_equal(W2, W3)
+
+
+def test_symmetric_per_system_lanczos_matches_separate_solvers():
+    """Per-system recurrences (LanczosAlgorithm.cu:171-232 per system) over the
+    symmetric product: as systems converge their items leave the launch and
+    their rows the reduction; every system still equals its own solver."""
+    from tests.test_gpu_ensemble import _rsa_ensemble, _single
+    sizes, kinds, pos = _rsa_ensemble(seed=61, sizes=(2100, 40, 900, 1300, 5))
+    N = pos.shape[0]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    tol = 1e-6
+    from libmobility_amd import NBody
+    ens = NBody("open", "open", "single_wall")
+    ens.setParameters(wallHeight=0.0)
+    ens.setPairSymmetric(1)
+    ens.initialize(viscosity=0.9, hydrodynamicRadius=1.1, tolerance=tol, seed=3)
+    ens.setSystems(sizes, ["single_wall" if k else "open" for k in kinds])
+    ens.setLanczosPerSystem(True)
+    ens.setPositions(pos)
+    singles = [_single(pos[a:b], k, False, tol) for a, b, k in zip(off[:-1], off[1:], kinds)]
+    rng = np.random.default_rng(62)
+    for call in range(2):
+        W = rng.normal(size=3 * N)
+        u, _ = ens.sqrtMdotW_with_noise(W)
+        assert ens.lastPairSymmetric()
+        its = ens.lastLanczosIterationsPerSystem()
+        assert len(set(its.tolist())) > 1
+        for s, (a, b) in enumerate(zip(off[:-1], off[1:])):
+            us, _ = singles[s].sqrtMdotW_with_noise(W[3 * a:3 * b])
+            assert abs(its[s] - singles[s].lastLanczosIterations()) <= 6, (call, s)
+            assert np.linalg.norm(u.ravel()[3 * a:3 * b] - us.ravel()) <= 10 * tol * np.linalg.norm(us), (call, s)
 
 
 def test_lanczos_on_the_symmetric_product():
