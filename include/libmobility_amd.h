@@ -267,6 +267,14 @@ int lm_nbody_precond_blocks(lm_nbody* h, double* G9);
 int lm_selftest_pair_block_host(int wall, int fp32, const double* pi, const double* pj, double a,
                                 double* block36);
 
+/* Host self-check of the symmetric evaluation's pair arithmetic (FP64, all
+ * blocks): fwd = the 6x6 block of target i from unit forces/torques on j, rev
+ * = the block the same pair evaluation gives j from unit forces/torques on i
+ * (the transposed blocks).  They must equal the reference blocks for (i, j)
+ * and (j, i) (hydrodynamicKernels.cuh:20-442). */
+int lm_selftest_pair_both_host(int wall, const double* pi, const double* pj, double a,
+                               double* fwd36, double* rev36);
+
 /* Diagnostic (no counterpart in the reference): time `blocks` workgroups of
  * 256 threads running `iters` iterations of 8 independent chains of 3 FMAs
  * (kind 0: FP32; 1: FP32 with one rsqrt per chain step; 2: FP64) on the null

@@ -81,6 +81,8 @@ SIGNATURES = {
     "lm_nbody_precond_blocks": (ctypes.c_int, [_vp, _vp]),
     "lm_selftest_pair_block_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp,
                                                    ctypes.c_double, _dp]),
+    "lm_selftest_pair_both_host": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_double, _dp,
+                                                  _dp]),
     "lm_diag_valu_throughput": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                 ctypes.POINTER(ctypes.c_float)]),
     "lm_selftest_philox_host": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32),

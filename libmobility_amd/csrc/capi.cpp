@@ -338,6 +338,56 @@ int lm_selftest_pair_block_host(int wall, int fp32, const double* pi, const doub
   });
 }
 
+int lm_selftest_pair_both_host(int wall, const double* pi, const double* pj, double a,
+                               double* fwd, double* rev) {
+  return guarded([&] {
+    if (!pi || !pj || !fwd || !rev) throw lm::Error("[libmobility_amd] null argument");
+    double r[2][4];
+    const double* p[2] = {pi, pj};
+    for (int q = 0; q < 2; ++q) {
+      const double z = p[q][2];
+      const double b = z / a;
+      r[q][0] = p[q][0] / a;
+      r[q][1] = p[q][1] / a;
+      r[q][2] = wall ? (z > a ? z : a) / a : z / a;
+      r[q][3] = wall ? (b < 0 ? 0 : (b > 1 ? 1 : b)) : 1.0;
+    }
+    constexpr int all = lm::kUF | lm::kUT | lm::kWT | lm::kWF;
+    const lm::Vec3<double> zero = {0, 0, 0};
+    for (int col = 0; col < 6; ++col) {
+      double v[6] = {0, 0, 0, 0, 0, 0};
+      v[col] = 1.0;
+      const lm::Vec3<double> f = {v[0], v[1], v[2]}, t = {v[3], v[4], v[5]};
+      // forward: unit force/torque on j, velocities of i; reverse: unit
+      // force/torque on i, what the same pair gives j (transposed blocks)
+      lm::PairAcc<double> acc;
+      lm::acc_zero(acc);
+      lm::Vec3<double> su = zero, sw = zero, du = zero, dw = zero;
+      lm::PairAcc<double> dummy;
+      lm::acc_zero(dummy);
+      if (wall) {
+        lm::pair_both<double, true, all>(r[0][0], r[0][1], r[0][2], r[0][3], r[1][0], r[1][1],
+                                         r[1][2], r[1][3], f, t, zero, zero, 1.0, 1.0, acc, du, dw);
+        lm::pair_both<double, true, all>(r[0][0], r[0][1], r[0][2], r[0][3], r[1][0], r[1][1],
+                                         r[1][2], r[1][3], zero, zero, f, t, 1.0, 1.0, dummy, su, sw);
+      } else {
+        lm::pair_both<double, false, all>(r[0][0], r[0][1], r[0][2], r[0][3], r[1][0], r[1][1],
+                                          r[1][2], r[1][3], f, t, zero, zero, 1.0, 1.0, acc, du, dw);
+        lm::pair_both<double, false, all>(r[0][0], r[0][1], r[0][2], r[0][3], r[1][0], r[1][1],
+                                          r[1][2], r[1][3], zero, zero, f, t, 1.0, 1.0, dummy, su,
+                                          sw);
+      }
+      const double u[6] = {acc.uf.x + acc.ut.x, acc.uf.y + acc.ut.y, acc.uf.z + acc.ut.z,
+                           acc.wt.x + acc.wf.x, acc.wt.y + acc.wf.y, acc.wt.z + acc.wf.z};
+      const double s6[6] = {su.x, su.y, su.z, sw.x, sw.y, sw.z};
+      for (int row = 0; row < 6; ++row) {
+        fwd[row * 6 + col] = u[row];
+        rev[row * 6 + col] = s6[row];
+      }
+    }
+  });
+}
+
 int lm_diag_valu_throughput(int kind, int blocks, int iters, float* ms) {
   return guarded([&] {
     if (!ms) throw lm::Error("[libmobility_amd] null output");

@@ -75,6 +75,36 @@ def test_kernel_pair_arithmetic_fp64(wall):
 
 
 @pytest.mark.parametrize("wall", [0, 1])
+def test_symmetric_pair_arithmetic_both_directions(wall):
+    """One evaluation of a pair (pair_both in pair_math.hpp) gives target i the
+    reference block M_ij and source j the block M_ji, i.e. the transposed
+    forms of every block are right (incl. self pairs, overlaps and z < a)."""
+    lib = _lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(30 + wall)
+    for t in range(2000):
+        a = rng.uniform(0.5, 2.0)
+        pi = rng.uniform(-4, 4, 3) * a
+        pj = rng.uniform(-4, 4, 3) * a
+        if wall:
+            pi[2] = rng.uniform(-0.2, 4) * a
+            pj[2] = rng.uniform(-0.2, 4) * a
+        if t % 40 == 0:
+            pj = pi.copy()
+        if t % 40 == 1:
+            pj = pi + rng.uniform(-0.5, 0.5, 3) * a  # overlapping
+        fwd, rev = np.zeros(36), np.zeros(36)
+        assert lib.lm_selftest_pair_both_host(
+            wall, pi.ctypes.data_as(dp), pj.ctypes.data_as(dp), a, fwd.ctypes.data_as(dp),
+            rev.ctypes.data_as(dp)) == 0
+        Oij = oracle.pair_block(wall, pi, pj, a)
+        Oji = oracle.pair_block(wall, pj, pi, a)
+        scale = max(np.abs(Oij).max(), np.abs(Oji).max(), 1e-300)
+        assert np.abs(fwd.reshape(6, 6) - Oij).max() <= 1e-13 * scale, t
+        assert np.abs(rev.reshape(6, 6) - Oji).max() <= 1e-13 * scale, t
+
+
+@pytest.mark.parametrize("wall", [0, 1])
 def test_kernel_pair_arithmetic_fp32(wall):
     rng = np.random.default_rng(20 + wall)
     worst = 0.0
