@@ -267,6 +267,39 @@ int lm_nbody_precond_blocks(lm_nbody* h, double* G9);
 int lm_selftest_pair_block_host(int wall, int fp32, const double* pi, const double* pj, double a,
                                 double* block36);
 
+/* ---- SelfMobility (solvers/SelfMobility/mobility.h:17-111) --------------------
+ * No hydrodynamic interactions: M = 1/(6 pi eta a) I for linear and
+ * 1/(8 pi eta a^3) I for angular velocities.  Memory modes as for NBody. */
+typedef struct lm_self lm_self;
+
+/* SelfMobility(Configuration), mobility.h:31-36: open boundaries only
+ * ("[Mobility] This is an open boundary solver"). */
+int lm_self_create(int periodicity_x, int periodicity_y, int periodicity_z, lm_self** out);
+int lm_self_destroy(lm_self* h);
+int lm_self_set_stream(lm_self* h, void* stream);
+/* SelfMobility::initialize, mobility.h:38-50 (the tolerance is unused). */
+int lm_self_initialize(lm_self* h, double viscosity, double radius, int include_angular,
+                       double tolerance, uint64_t seed);
+/* SelfMobility::setPositions, mobility.h:56-59: only the particle count. */
+int lm_self_set_positions(lm_self* h, const double* positions, int64_t n, int mem);
+int64_t lm_self_num_particles(lm_self* h);
+/* SelfMobility::Mdot, mobility.h:63-86: linear = F/(6 pi eta a) when forces
+ * are given, angular = T/(8 pi eta a^3) when torques are given. */
+int lm_self_mdot(lm_self* h, const double* forces, int64_t n_forces, const double* torques,
+                 int64_t n_torques, double* linear, double* angular, int mem);
+/* SelfMobility::sqrtMdotW, mobility.h:90-110: linear (and angular) +=
+ * prefactor sqrt(mobility) N(0,1), noise drawn on the device. */
+int lm_self_sqrt_mdot_w(lm_self* h, double* linear, double* angular, double prefactor, int mem);
+/* Mobility::divM default, MobilityInterface.h:211-216: the drift is zero
+ * (outputs are left as they are). */
+int lm_self_div_m(lm_self* h, double* linear, double* angular, double prefactor, int mem);
+/* Mobility::LangevinVelocities, MobilityInterface.h:191-206. */
+int lm_self_langevin_velocities(lm_self* h, double dt, double kbt, const double* forces,
+                                int64_t n_forces, const double* torques, int64_t n_torques,
+                                double* linear, double* angular, int mem);
+/* Mobility::clean, MobilityInterface.h:220-223. */
+int lm_self_clean(lm_self* h);
+
 /* Host self-check of the symmetric evaluation's pair arithmetic (FP64, all
  * blocks): fwd = the 6x6 block of target i from unit forces/torques on j, rev
  * = the block the same pair evaluation gives j from unit forces/torques on i

@@ -1,4 +1,5 @@
-"""libmobility_amd: libMobility's NBody solver and Lanczos noise on MI355X (gfx950).
+"""libmobility_amd: libMobility's NBody solver (with its Lanczos noise) and
+SelfMobility on MI355X (gfx950).
 
     from libmobility_amd import NBody
     solver = NBody("open", "open", "open")
@@ -8,6 +9,7 @@
     U, _ = solver.Mdot(forces)
 """
 from .nbody import NBody
+from .self_mobility import SelfMobility
 
-__all__ = ["NBody"]
+__all__ = ["NBody", "SelfMobility"]
 __version__ = "0.1.0"

@@ -81,6 +81,21 @@ SIGNATURES = {
     "lm_nbody_precond_blocks": (ctypes.c_int, [_vp, _vp]),
     "lm_selftest_pair_block_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp,
                                                    ctypes.c_double, _dp]),
+    "lm_self_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "lm_self_destroy": (ctypes.c_int, [_vp]),
+    "lm_self_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "lm_self_initialize": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_uint64]),
+    "lm_self_set_positions": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int]),
+    "lm_self_num_particles": (ctypes.c_int64, [_vp]),
+    "lm_self_mdot": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64, _vp, _vp,
+                                    ctypes.c_int]),
+    "lm_self_sqrt_mdot_w": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
+    "lm_self_div_m": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
+    "lm_self_langevin_velocities": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp,
+                                                   ctypes.c_int64, _vp, ctypes.c_int64, _vp, _vp,
+                                                   ctypes.c_int]),
+    "lm_self_clean": (ctypes.c_int, [_vp]),
     "lm_selftest_pair_both_host": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_double, _dp,
                                                   _dp]),
     "lm_diag_valu_throughput": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,

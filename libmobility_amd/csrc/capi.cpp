@@ -6,7 +6,13 @@
 
 #include "pair_math.hpp"
 #include "philox.hpp"
+#include "self_solver.hpp"
 #include "solver.hpp"
+
+struct lm_self {
+  lm::SelfSolver solver;
+  lm_self(int px, int py, int pz) : solver(px, py, pz) {}
+};
 
 struct lm_nbody {
   lm::NBodySolver solver;
@@ -32,6 +38,11 @@ int guarded(F&& f) {
     g_last_error = "[libmobility_amd] unknown error";
   }
   return LM_ERROR;
+}
+
+lm::SelfSolver& SS(lm_self* h) {
+  if (!h) throw lm::Error("[libmobility_amd] null solver handle");
+  return h->solver;
 }
 
 lm::NBodySolver& S(lm_nbody* h) {
@@ -386,6 +397,68 @@ int lm_selftest_pair_both_host(int wall, const double* pi, const double* pj, dou
       }
     }
   });
+}
+
+// ---- SelfMobility ------------------------------------------------------------
+
+int lm_self_create(int px, int py, int pz, lm_self** out) {
+  return guarded([&] {
+    if (!out) throw lm::Error("[libmobility_amd] null output");
+    *out = nullptr;
+    *out = new lm_self(px, py, pz);
+  });
+}
+
+int lm_self_destroy(lm_self* h) {
+  return guarded([&] { delete h; });
+}
+
+int lm_self_set_stream(lm_self* h, void* stream) {
+  return guarded([&] { SS(h).setStream(static_cast<hipStream_t>(stream)); });
+}
+
+int lm_self_initialize(lm_self* h, double viscosity, double radius, int include_angular,
+                       double tolerance, uint64_t seed) {
+  return guarded(
+      [&] { SS(h).initialize(viscosity, radius, include_angular != 0, tolerance, seed); });
+}
+
+int lm_self_set_positions(lm_self* h, const double* positions, int64_t n, int mem) {
+  (void)positions;
+  (void)mem;
+  return guarded([&] { SS(h).setPositions(n); });
+}
+
+int64_t lm_self_num_particles(lm_self* h) { return h ? h->solver.numParticles() : 0; }
+
+int lm_self_mdot(lm_self* h, const double* F, int64_t nF, const double* T, int64_t nT, double* U,
+                 double* W, int mem) {
+  return guarded([&] { SS(h).Mdot(F, F ? nF : 0, T, T ? nT : 0, U, W, is_dev(mem)); });
+}
+
+int lm_self_sqrt_mdot_w(lm_self* h, double* U, double* W, double prefactor, int mem) {
+  return guarded([&] {
+    if (prefactor == 0) return;
+    SS(h).sqrtMdotW(U, W, prefactor, is_dev(mem));
+  });
+}
+
+int lm_self_div_m(lm_self* h, double* U, double* W, double prefactor, int mem) {
+  (void)U;
+  (void)W;
+  (void)prefactor;
+  (void)mem;
+  return guarded([&] { SS(h); });
+}
+
+int lm_self_langevin_velocities(lm_self* h, double dt, double kbt, const double* F, int64_t nF,
+                                const double* T, int64_t nT, double* U, double* W, int mem) {
+  return guarded(
+      [&] { SS(h).langevin(dt, kbt, F, F ? nF : 0, T, T ? nT : 0, U, W, is_dev(mem)); });
+}
+
+int lm_self_clean(lm_self* h) {
+  return guarded([&] { SS(h).clean(); });
 }
 
 int lm_diag_valu_throughput(int kind, int blocks, int iters, float* ms) {

@@ -122,7 +122,140 @@ def _mem_of(framework, device):
     return _ffi.LM_MEM_DEVICE if (framework == "torch" and device.startswith("cuda")) else _ffi.LM_MEM_HOST
 
 
-class NBody:
+class _Solver:
+    """The part of the pythonify.h class every solver shares (positions,
+    array rules, Mdot, sqrtMdotW, divM, LangevinVelocities, clean), over the
+    C-ABI functions lm_<prefix>_*."""
+
+    _prefix = ""
+
+    def _f(self, name):
+        return getattr(self._lib, "lm_" + self._prefix + "_" + name)
+
+    # -- helpers ------------------------------------------------------------------
+    def _bind_stream(self, mem, device):
+        if mem != _ffi.LM_MEM_DEVICE:
+            return
+        import torch
+
+        st = torch.cuda.current_stream(torch.device(device)).cuda_stream
+        if st != self._stream_set:
+            _ffi.check(self._f("set_stream")(self._h, ctypes.c_void_p(st)))
+            self._stream_set = st
+
+    def getNumberParticles(self):
+        return int(self._f("num_particles")(self._h))
+
+    # -- interface ----------------------------------------------------------------
+    def setPositions(self, positions):
+        """pythonify.h:262-271 -> NBody::setPositions (mobility.h:116-162)."""
+        global _last_shape, _last_framework, _last_device
+        arr = _Arr(positions)
+        shape = _check_and_get_shape(arr)
+        _last_shape = shape
+        _last_framework = arr.framework
+        _last_device = arr.device
+        self._bind_stream(arr.mem, arr.device)
+        _ffi.check(self._f("set_positions")(self._h, arr.ptr, arr.size, arr.mem))
+        _last_shape = arr.shape
+
+    def _setup_arrays(self, forces, torques):
+        """pythonify.h:109-149."""
+        global _last_framework, _last_device
+        N = self.getNumberParticles()
+        f = _Arr(forces)
+        t = _Arr(torques)
+        if f.size > 0 and f.size != 3 * N:
+            raise RuntimeError("The forces array must have size 3*N.")
+        if t.size > 0 and t.size != 3 * N:
+            raise RuntimeError("The torques array must have size 3*N.")
+        lead = f if f.size > 0 else (t if t.size > 0 else f)
+        framework, device, mem = lead.framework, lead.device, lead.mem
+        _last_framework = framework
+        _last_device = device
+        f = f.moved_to(mem, device)
+        t = t.moved_to(mem, device)
+        f_shape = _check_and_get_shape(f) if f.size > 0 else _check_and_get_shape(t)
+        mf = _zeros(f_shape, framework, device)
+        mt = None
+        if not self._include_angular:
+            if t.size > 0:
+                raise RuntimeError(
+                    "The solver was configured without including angular velocities. Set "
+                    "includeAngular to true when initializing if you want to use torques")
+        else:
+            t_shape = f_shape if t.size == 0 else _check_and_get_shape(t)
+            mt = _zeros(t_shape, framework, device)
+        return f, t, mf, mt, mem, device
+
+    def Mdot(self, forces=None, torques=None):
+        """pythonify.h:208-216 -> NBody::Mdot (mobility.h:166-182)."""
+        f, t, mf, mt, mem, device = self._setup_arrays(forces, torques)
+        self._bind_stream(mem, device)
+        _ffi.check(self._f("mdot")(self._h, f.ptr, f.size, t.ptr, t.size, _ptr(mf),
+                                           _ptr(mt), mem))
+        return mf, mt
+
+    def _own_shape(self):
+        """The module-global last shape (pythonify.h:34-36) in form -- (3N,) or
+        (N, 3) -- but sized for this solver's N: another solver's positions
+        set since must not size this solver's outputs (they receive 3N
+        values)."""
+        N = self.getNumberParticles()
+        return (3 * N,) if len(_last_shape) == 1 else (N, 3)
+
+    def _outputs_like_positions(self):
+        shape = self._own_shape()
+        linear = _zeros(shape, _last_framework, _last_device)
+        angular = _zeros(shape, _last_framework, _last_device) if self._include_angular else None
+        return linear, angular, _mem_of(_last_framework, _last_device)
+
+    def sqrtMdotW(self, prefactor=1.0):
+        """pythonify.h:170-185 -> Mobility::sqrtMdotW (MobilityInterface.h:132-188)."""
+        linear, angular, mem = self._outputs_like_positions()
+        self._bind_stream(mem, _last_device)
+        _ffi.check(self._f("sqrt_mdot_w")(self._h, _ptr(linear), _ptr(angular),
+                                                  float(prefactor), mem))
+        return linear, angular
+
+    def divM(self, prefactor=1):
+        """pythonify.h:329-345 -> NBody::divM (mobility.h:184-236)."""
+        if self.getNumberParticles() <= 0:
+            raise RuntimeError(
+                "[libMobility] The number of particles is not set. Did you forget to call "
+                "setPositions?")
+        linear, angular, mem = self._outputs_like_positions()
+        self._bind_stream(mem, _last_device)
+        _ffi.check(self._f("div_m")(self._h, _ptr(linear), _ptr(angular), float(prefactor),
+                                            mem))
+        return linear, angular
+
+    def LangevinVelocities(self, dt, kbt, forces=None, torques=None):
+        """pythonify.h:273-294 -> Mobility::LangevinVelocities (MobilityInterface.h:191-206)."""
+        f, t, mf, mt, mem, device = self._setup_arrays(forces, torques)
+        if f.size == 0:
+            mf = _zeros(self._own_shape(), _last_framework, _last_device)
+            mem = _mem_of(_last_framework, _last_device)
+            device = _last_device
+        if self._include_angular and t.size == 0:
+            mt = _zeros(self._own_shape(), _last_framework, _last_device)
+        f = f.moved_to(mem, device)
+        t = t.moved_to(mem, device)
+        self._bind_stream(mem, device)
+        _ffi.check(self._f("langevin_velocities")(
+            self._h, float(dt), float(kbt), f.ptr, f.size, t.ptr, t.size, _ptr(mf), _ptr(mt), mem))
+        return mf, mt
+
+    def hydrodynamicVelocities(self, forces=None, torques=None):
+        """Mdot + M^{1/2} W + div M with unit prefactors (LangevinVelocities, dt=2, kbt=1)."""
+        return self.LangevinVelocities(2.0, 1.0, forces, torques)
+
+    def clean(self):
+        """Mobility::clean (MobilityInterface.h:220-223)."""
+        _ffi.check(self._f("clean")(self._h))
+
+
+class NBody(_Solver):
     """O(N^2) hydrodynamic interactions on one MI355X.
 
     Open boundaries in x and y; z is ``open`` (Rotne-Prager-Yamakawa) or
@@ -132,6 +265,7 @@ class NBody:
 
     precision = "double"
     version = "1.1"
+    _prefix = "nbody"
 
     def __init__(self, periodicityX, periodicityY, periodicityZ):
         # string2Periodicity, pythonify.h:38-52
@@ -284,92 +418,6 @@ class NBody:
             float(tolerance), ctypes.c_uint64(s & 0xFFFFFFFFFFFFFFFF), cb, None))
         self._include_angular = bool(includeAngular)
 
-    # -- helpers ------------------------------------------------------------------
-    def _bind_stream(self, mem, device):
-        if mem != _ffi.LM_MEM_DEVICE:
-            return
-        import torch
-
-        st = torch.cuda.current_stream(torch.device(device)).cuda_stream
-        if st != self._stream_set:
-            _ffi.check(self._lib.lm_nbody_set_stream(self._h, ctypes.c_void_p(st)))
-            self._stream_set = st
-
-    def getNumberParticles(self):
-        return int(self._lib.lm_nbody_num_particles(self._h))
-
-    # -- interface ----------------------------------------------------------------
-    def setPositions(self, positions):
-        """pythonify.h:262-271 -> NBody::setPositions (mobility.h:116-162)."""
-        global _last_shape, _last_framework, _last_device
-        arr = _Arr(positions)
-        shape = _check_and_get_shape(arr)
-        _last_shape = shape
-        _last_framework = arr.framework
-        _last_device = arr.device
-        self._bind_stream(arr.mem, arr.device)
-        _ffi.check(self._lib.lm_nbody_set_positions(self._h, arr.ptr, arr.size, arr.mem))
-        _last_shape = arr.shape
-
-    def _setup_arrays(self, forces, torques):
-        """pythonify.h:109-149."""
-        global _last_framework, _last_device
-        N = self.getNumberParticles()
-        f = _Arr(forces)
-        t = _Arr(torques)
-        if f.size > 0 and f.size != 3 * N:
-            raise RuntimeError("The forces array must have size 3*N.")
-        if t.size > 0 and t.size != 3 * N:
-            raise RuntimeError("The torques array must have size 3*N.")
-        lead = f if f.size > 0 else (t if t.size > 0 else f)
-        framework, device, mem = lead.framework, lead.device, lead.mem
-        _last_framework = framework
-        _last_device = device
-        f = f.moved_to(mem, device)
-        t = t.moved_to(mem, device)
-        f_shape = _check_and_get_shape(f) if f.size > 0 else _check_and_get_shape(t)
-        mf = _zeros(f_shape, framework, device)
-        mt = None
-        if not self._include_angular:
-            if t.size > 0:
-                raise RuntimeError(
-                    "The solver was configured without including angular velocities. Set "
-                    "includeAngular to true when initializing if you want to use torques")
-        else:
-            t_shape = f_shape if t.size == 0 else _check_and_get_shape(t)
-            mt = _zeros(t_shape, framework, device)
-        return f, t, mf, mt, mem, device
-
-    def Mdot(self, forces=None, torques=None):
-        """pythonify.h:208-216 -> NBody::Mdot (mobility.h:166-182)."""
-        f, t, mf, mt, mem, device = self._setup_arrays(forces, torques)
-        self._bind_stream(mem, device)
-        _ffi.check(self._lib.lm_nbody_mdot(self._h, f.ptr, f.size, t.ptr, t.size, _ptr(mf),
-                                           _ptr(mt), mem))
-        return mf, mt
-
-    def _own_shape(self):
-        """The module-global last shape (pythonify.h:34-36) in form -- (3N,) or
-        (N, 3) -- but sized for this solver's N: another solver's positions
-        set since must not size this solver's outputs (they receive 3N
-        values)."""
-        N = self.getNumberParticles()
-        return (3 * N,) if len(_last_shape) == 1 else (N, 3)
-
-    def _outputs_like_positions(self):
-        shape = self._own_shape()
-        linear = _zeros(shape, _last_framework, _last_device)
-        angular = _zeros(shape, _last_framework, _last_device) if self._include_angular else None
-        return linear, angular, _mem_of(_last_framework, _last_device)
-
-    def sqrtMdotW(self, prefactor=1.0):
-        """pythonify.h:170-185 -> Mobility::sqrtMdotW (MobilityInterface.h:132-188)."""
-        linear, angular, mem = self._outputs_like_positions()
-        self._bind_stream(mem, _last_device)
-        _ffi.check(self._lib.lm_nbody_sqrt_mdot_w(self._h, _ptr(linear), _ptr(angular),
-                                                  float(prefactor), mem))
-        return linear, angular
-
     def sqrtMdotW_with_noise(self, noise, prefactor=1.0):
         """Extension: prefactor * M^{1/2} W for a given W (3N, or 6N with angular)."""
         linear, angular, mem = self._outputs_like_positions()
@@ -426,42 +474,6 @@ class NBody:
         _ffi.check(self._lib.lm_nbody_last_lanczos_iterations_multi(self._h, out.ctypes.data,
                                                                     int(nrhs)))
         return out
-
-    def divM(self, prefactor=1):
-        """pythonify.h:329-345 -> NBody::divM (mobility.h:184-236)."""
-        if self.getNumberParticles() <= 0:
-            raise RuntimeError(
-                "[libMobility] The number of particles is not set. Did you forget to call "
-                "setPositions?")
-        linear, angular, mem = self._outputs_like_positions()
-        self._bind_stream(mem, _last_device)
-        _ffi.check(self._lib.lm_nbody_div_m(self._h, _ptr(linear), _ptr(angular), float(prefactor),
-                                            mem))
-        return linear, angular
-
-    def LangevinVelocities(self, dt, kbt, forces=None, torques=None):
-        """pythonify.h:273-294 -> Mobility::LangevinVelocities (MobilityInterface.h:191-206)."""
-        f, t, mf, mt, mem, device = self._setup_arrays(forces, torques)
-        if f.size == 0:
-            mf = _zeros(self._own_shape(), _last_framework, _last_device)
-            mem = _mem_of(_last_framework, _last_device)
-            device = _last_device
-        if self._include_angular and t.size == 0:
-            mt = _zeros(self._own_shape(), _last_framework, _last_device)
-        f = f.moved_to(mem, device)
-        t = t.moved_to(mem, device)
-        self._bind_stream(mem, device)
-        _ffi.check(self._lib.lm_nbody_langevin_velocities(
-            self._h, float(dt), float(kbt), f.ptr, f.size, t.ptr, t.size, _ptr(mf), _ptr(mt), mem))
-        return mf, mt
-
-    def hydrodynamicVelocities(self, forces=None, torques=None):
-        """Mdot + M^{1/2} W + div M with unit prefactors (LangevinVelocities, dt=2, kbt=1)."""
-        return self.LangevinVelocities(2.0, 1.0, forces, torques)
-
-    def clean(self):
-        """Mobility::clean (MobilityInterface.h:220-223)."""
-        _ffi.check(self._lib.lm_nbody_clean(self._h))
 
     def lastLanczosIterations(self):
         """Extension: iteration index at which the last sqrtMdotW converged."""

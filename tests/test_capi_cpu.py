@@ -170,3 +170,21 @@ def test_outputs_are_sized_by_the_solvers_own_particle_count(monkeypatch):
     assert lin.shape == (5, 3) and ang is None
     monkeypatch.setattr(nbody, "_last_shape", (21,))
     assert s._outputs_like_positions()[0].shape == (15,)
+
+
+def test_libmobility_surface_and_selfmobility_configuration():
+    """solvers/__init__.py:1-6 and SelfMobility's constructor
+    (solvers/SelfMobility/mobility.h:31-36): open boundaries only; PSE and
+    DPStokes are not part of this library and raise."""
+    import libMobility
+    assert set(libMobility.__all__) == {"PSE", "SelfMobility", "NBody", "DPStokes"}
+    for cls, args in ((libMobility.PSE, ("periodic",) * 3),
+                      (libMobility.DPStokes, ("periodic", "periodic", "open"))):
+        with pytest.raises(RuntimeError, match="not part of this library"):
+            cls(*args)
+    with pytest.raises(RuntimeError, match="open boundary solver"):
+        libMobility.SelfMobility("open", "open", "single_wall")
+    with pytest.raises(RuntimeError, match="Invalid periodicity"):
+        libMobility.SelfMobility("open", "open", "nope")
+    s = libMobility.SelfMobility("open", "open", "open")
+    s.setParameters(parameter=5.0)

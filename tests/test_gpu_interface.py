@@ -1,5 +1,6 @@
-"""Drop-in behaviour of the Python NBody class on the GPU, following the
-reference's own interface tests for NBody (tests/test_interface.py,
+"""Drop-in behaviour of the Python NBody class (and, for the generic tests,
+SelfMobility) on the GPU, following the reference's own interface tests
+(tests/test_interface.py,
 test_initialization.py, test_precision.py, test_array_origins.py,
 test_thermal_drift.py:77-96): shapes and types of outputs, error cases,
 parameter handling; plus edge cases of the pair kernel (coincident distinct
@@ -13,10 +14,19 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 GEOMS = ["open", "single_wall"]
+# the generic interface tests also run on SelfMobility (solver_configs_all,
+# tests/utils.py:32-41, with its sane parameter of utils.py:17)
+ALL = GEOMS + ["self"]
 
 
 def _solver(geom, includeAngular=False, seed=None, **params):
-    from libmobility_amd import NBody
+    from libmobility_amd import NBody, SelfMobility
+    if geom == "self":
+        s = SelfMobility("open", "open", "open")
+        s.setParameters(parameter=5.0)
+        s.initialize(viscosity=1.0, hydrodynamicRadius=1.0, includeAngular=includeAngular,
+                     seed=seed)
+        return s
     s = NBody("open", "open", geom)
     if geom == "single_wall":
         params.setdefault("wallHeight", 0.0)
@@ -32,7 +42,7 @@ def _pos(n, rng):
     return p
 
 
-@pytest.mark.parametrize("geom", GEOMS)
+@pytest.mark.parametrize("geom", ALL)
 def test_non_contiguous_positions(geom):
     """test_interface.py:15-30"""
     rng = np.random.default_rng(0)
@@ -45,7 +55,7 @@ def test_non_contiguous_positions(geom):
     assert u.shape == (3,) and np.all(np.isfinite(u))
 
 
-@pytest.mark.parametrize("geom", GEOMS)
+@pytest.mark.parametrize("geom", ALL)
 def test_linear_only_shapes(geom):
     """test_interface.py:33-57: without includeAngular only U, shaped like F."""
     rng = np.random.default_rng(1)
@@ -62,7 +72,7 @@ def test_linear_only_shapes(geom):
     assert u.shape == (6,) and w is None
 
 
-@pytest.mark.parametrize("geom", GEOMS)
+@pytest.mark.parametrize("geom", ALL)
 @pytest.mark.parametrize("which", ["torques", "forces"])
 def test_angular_returns_both(geom, which):
     """test_interface.py:60-123: a torque (or force) alone gives both U and W,
@@ -75,10 +85,16 @@ def test_angular_returns_both(geom, which):
         s.setPositions(p)
         u, w = s.Mdot(**{which: x})
         assert u.shape == x.shape and w.shape == x.shape
-        assert np.linalg.norm(u) > 0 and np.linalg.norm(w) > 0
+        # the driven block is nonzero; the coupled one too except for
+        # SelfMobility, which has no hydrodynamic coupling
+        # (test_interface.py:66-67, :100-101)
+        driven, coupled = (w, u) if which == "torques" else (u, w)
+        assert np.linalg.norm(driven) > 0
+        if geom != "self":
+            assert np.linalg.norm(coupled) > 0
 
 
-@pytest.mark.parametrize("geom", GEOMS)
+@pytest.mark.parametrize("geom", ALL)
 def test_mixed_input_shapes(geom):
     """test_interface.py:126-154: U follows the forces' shape, W the torques'."""
     rng = np.random.default_rng(3)
@@ -93,7 +109,7 @@ def test_mixed_input_shapes(geom):
             assert np.linalg.norm(u) > 0 and np.linalg.norm(w) > 0
 
 
-@pytest.mark.parametrize("geom", GEOMS)
+@pytest.mark.parametrize("geom", ALL)
 def test_noise_outputs_follow_positions_shape(geom):
     """test_interface.py:159-251: sqrtMdotW, divM and LangevinVelocities
     without forces take the shape of the positions; with forces, the forces'."""
@@ -115,7 +131,7 @@ def test_noise_outputs_follow_positions_shape(geom):
         assert v.shape == fs and w.shape == ts
 
 
-@pytest.mark.parametrize("geom", GEOMS)
+@pytest.mark.parametrize("geom", ALL)
 def test_error_cases(geom):
     """test_interface.py:254-333: torques without includeAngular, calls before
     setPositions, malformed positions, forces/torques of the wrong size."""
@@ -143,7 +159,7 @@ def test_error_cases(geom):
             sa.Mdot(torques=rng.random(3 * n_wrong))
 
 
-@pytest.mark.parametrize("geom", GEOMS)
+@pytest.mark.parametrize("geom", ALL)
 @pytest.mark.parametrize("includeAngular", [False, True])
 def test_changing_number_of_particles(geom, includeAngular):
     """test_interface.py:360-383"""
@@ -159,7 +175,7 @@ def test_changing_number_of_particles(geom, includeAngular):
             assert x.shape == (n, 3)
 
 
-@pytest.mark.parametrize("geom", GEOMS)
+@pytest.mark.parametrize("geom", ALL)
 @pytest.mark.parametrize("includeAngular", [False, True])
 def test_zero_temperature_langevin_is_mdot(geom, includeAngular):
     """test_interface.py:336-357"""
@@ -218,7 +234,7 @@ def test_bad_batch_parameters(nb, npb, n):
         s.setPositions(np.random.default_rng(10).random((n, 3)))
 
 
-@pytest.mark.parametrize("geom", GEOMS)
+@pytest.mark.parametrize("geom", ALL)
 def test_single_precision_inputs(geom):
     """test_precision.py:9-62: float32 inputs are converted; the product of
     ones is nonzero in every entry."""
@@ -231,7 +247,7 @@ def test_single_precision_inputs(geom):
     assert np.all(u != 0)
 
 
-@pytest.mark.parametrize("geom", GEOMS)
+@pytest.mark.parametrize("geom", ALL)
 @pytest.mark.parametrize("use_torques", [False, True])
 @pytest.mark.parametrize("method", ["Mdot", "divM", "sqrtMdotW", "LangevinVelocities"])
 @pytest.mark.parametrize("framework", ["numpy", "torch_cpu", "torch_gpu"])
