@@ -881,8 +881,9 @@ void launch_pair_sym(const SymArgs& a, hipStream_t s) {
     launch_pair_sym_r<double>(a, s);
 }
 
-// One workgroup per row block (at most 256 particles of one tile of one system).
-__global__ __launch_bounds__(256) void k_sym_reduce(SymReduceArgs r) {
+// One workgroup per row block (at most 256 particles of one tile of one
+// system), one thread per entry (3 x 256).
+__global__ __launch_bounds__(3 * kPairThreads) void k_sym_reduce(SymReduceArgs r) {
   const SymRows rb = r.rows[blockIdx.x];
   if (r.active != nullptr && r.active[rb.sys] == 0) return;
   const SymSys S = r.sys[rb.sys];
@@ -907,7 +908,7 @@ __global__ __launch_bounds__(256) void k_sym_reduce(SymReduceArgs r) {
 
 void launch_sym_reduce(const SymReduceArgs& r, hipStream_t s) {
   if (r.nrows <= 0) return;
-  hipLaunchKernelGGL(k_sym_reduce, dim3((unsigned)r.nrows), dim3(256), 0, s, r);
+  hipLaunchKernelGGL(k_sym_reduce, dim3((unsigned)r.nrows), dim3(3 * kPairThreads), 0, s, r);
   LM_HIP(hipGetLastError());
 }
 

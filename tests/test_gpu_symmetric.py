@@ -349,3 +349,32 @@ def test_mode_is_validated():
     s = _solver("fp64", -1)
     with pytest.raises(RuntimeError):
         s.setPairSymmetric(2)
+
+
+@pytest.mark.parametrize("wall", [False, True])
+def test_symmetric_smallest_systems(wall):
+    """One particle (a diagonal item and no ring), two coincident particles,
+    and an ensemble with an empty system, all against the oracle."""
+    rng = np.random.default_rng(71)
+    for pos in (np.array([[1.0, 2.0, 3.0]]), np.array([[1.0, 2.0, 3.0], [1.0, 2.0, 3.0]])):
+        s = _wall_solver("fp64", 1, 0, 0, True, wall)
+        s.setPositions(pos)
+        F, T = rng.normal(size=pos.shape), rng.normal(size=pos.shape)
+        U, W = s.Mdot(F, T)
+        assert s.lastPairSymmetric()
+        Uo, Wo = oracle.mdot(pos, ETA, A, F=F, T=T, angular=True, wall=wall)
+        assert _rel(U, Uo) <= 1e-13 and _rel(W, Wo) <= 1e-13
+    from libmobility_amd import NBody
+    e = NBody("open", "open", "single_wall" if wall else "open")
+    e.setParameters(wallHeight=0.0) if wall else e.setParameters()
+    e.setPairSymmetric(1)
+    e.initialize(viscosity=ETA, hydrodynamicRadius=A)
+    sizes = [300, 0, 1, 520]
+    e.setSystems(sizes)
+    pos = _wall_cloud(sum(sizes), 72, wall)
+    e.setPositions(pos)
+    F = rng.normal(size=(sum(sizes), 3))
+    U, _ = e.Mdot(F)
+    assert e.lastPairSymmetric()
+    Uo, _ = oracle.mdot(pos, ETA, A, F=F, wall=wall, sys_sizes=sizes)
+    assert _rel(U, Uo) <= 1e-12
