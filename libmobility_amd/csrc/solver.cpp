@@ -134,6 +134,8 @@ void NBodySolver::setSystems(const int64_t* offsets, int nsys, const int* kinds)
     explicit_kinds_.assign(kinds, kinds + nsys);
   }
   explicit_systems_ = true;
+  ++struct_id_;
+  items_tpl_ = -1;
 }
 
 void NBodySolver::setPositions(const double* pos, int64_t n3, bool dev) {
@@ -197,6 +199,10 @@ void NBodySolver::setPositions(const double* pos, int64_t n3, bool dev) {
                         rec32_.as<void>(), stream_);
     rec_a_ = a_;
   }
+  // Work lists depend on N, the system offsets and kinds only: a new
+  // configuration of the same systems (every step of a time loop) keeps them.
+  const bool same_structure =
+      N == N_ && kinds == sys_kind_host_ && (explicit_systems_ || off == sys_off_host_);
   N_ = N;
   sys_kind_host_ = kinds;
   kinds_dev_valid_ = kinds_d != nullptr;
@@ -204,7 +210,10 @@ void NBodySolver::setPositions(const double* pos, int64_t n3, bool dev) {
   any_open_ = any_open;
   if (!explicit_systems_) sys_off_host_ = off;
   ++layout_id_;
-  items_tpl_ = -1;
+  if (!same_structure) {
+    ++struct_id_;
+    items_tpl_ = -1;
+  }
   if (!valid)
     throw Error(
         "[Mobility] Invalid batch parameters for NBody. If in doubt, use the defaults.");
@@ -275,7 +284,7 @@ void NBodySolver::build_items(int tpl, int nch, bool persist, std::vector<PairIt
 
 void NBodySolver::ensure_items(int tpl, int nch_hint) {
   const int nch = resolve_nch(tpl, nch_hint);
-  if (items_tpl_ == tpl && items_nch_ == nch && items_N_ == N_ && items_layout_ == layout_id_ &&
+  if (items_tpl_ == tpl && items_nch_ == nch && items_N_ == N_ && items_struct_ == struct_id_ &&
       items_persist_ == use_persistent())
     return;
   std::vector<PairItem> items;
@@ -287,13 +296,13 @@ void NBodySolver::ensure_items(int tpl, int nch_hint) {
   items_tpl_ = tpl;
   items_nch_ = nch;
   items_N_ = N_;
-  items_layout_ = layout_id_;
+  items_struct_ = struct_id_;
   items_persist_ = use_persistent();
   items_count_ = (int)items.size();
 }
 
 void NBodySolver::ensure_multi_items(ItemList& L, int tpl) {
-  if (L.tpl == tpl && L.N == N_ && L.layout == layout_id_) return;
+  if (L.tpl == tpl && L.N == N_ && L.structure == struct_id_) return;
   std::vector<PairItem> items;
   const int nch = resolve_nch(tpl, 0);
   build_items(tpl, nch, false, items);
@@ -305,7 +314,7 @@ void NBodySolver::ensure_multi_items(ItemList& L, int tpl) {
   L.nch = nch;
   L.count = (int)items.size();
   L.N = N_;
-  L.layout = layout_id_;
+  L.structure = struct_id_;
 }
 
 void NBodySolver::setPairSymmetric(int mode) {
@@ -327,7 +336,7 @@ void NBodySolver::sym_shape(bool special, int tpl_req, int nch_req, int& tpl, in
                             int& ntiles, int& hmax) const {
   const int nsys = (int)sys_off_host_.size() - 1;
   tpl = tpl_req != 0 ? tpl_req : (special && nsys == 1 && N_ >= 65536 ? 2 : 1);
-  if (!special) tpl = std::min(tpl, 2);
+  if (!(special && nsys == 1)) tpl = std::min(tpl, 2);  // 4 only for one open system
   const int64_t bt = (int64_t)kPairThreads * tpl;
   ntiles = 0;
   hmax = 0;
@@ -357,7 +366,7 @@ bool NBodySolver::use_sym(int blocks) const {
 // per offset, reduced into the output in that order, the first pass
 // assigning and later ones adding.
 void NBodySolver::ensure_sym_plan(int tpl, int nch, int h, bool ang) {
-  if (sym_.N == N_ && sym_.layout == layout_id_ && sym_.tpl == tpl && sym_.nchunks == nch &&
+  if (sym_.N == N_ && sym_.structure == struct_id_ && sym_.tpl == tpl && sym_.nchunks == nch &&
       sym_.ang == (ang ? 1 : 0) && sym_.budget == sym_budget_)
     return;
   const int nsys = (int)sys_off_host_.size() - 1;
@@ -437,7 +446,7 @@ void NBodySolver::ensure_sym_plan(int tpl, int nch, int h, bool ang) {
   LM_HIP(hipStreamSynchronize(stream_));  // the host vectors go out of scope (items are kept)
   sym_.host_items = std::move(items);
   sym_.N = N_;
-  sym_.layout = layout_id_;
+  sym_.structure = struct_id_;
   sym_.tpl = tpl;
   sym_.nchunks = nch;
   sym_.hmax = h;
@@ -452,7 +461,7 @@ void NBodySolver::ensure_sym_plan(int tpl, int nch, int h, bool ang) {
 // (filtered from the plan's host list, already in launch order).
 void NBodySolver::build_sym_active(const int* act) {
   sym_.active_valid = false;
-  if (sym_.N != N_ || sym_.layout != layout_id_ || sym_.host_items.empty()) return;
+  if (sym_.N != N_ || sym_.structure != struct_id_ || sym_.host_items.empty()) return;
   sym_.active_passes = sym_.passes;
   sym_active_host_.reserve(sizeof(SymItem) * sym_.host_items.size());
   SymItem* dst = sym_active_host_.as<SymItem>();
@@ -662,9 +671,9 @@ void NBodySolver::launch_pair_items(const PairItem* list, int count, int tpl, in
 // over the items), else one target per lane and enough source chunks for
 // about 8192 items, built for the few systems left.
 const std::vector<PairItem>& NBodySolver::host_items(int tpl, int nch) {
-  if (host_items_layout_ != layout_id_ || host_items_N_ != N_) {
+  if (host_items_struct_ != struct_id_ || host_items_N_ != N_) {
     host_items_.clear();
-    host_items_layout_ = layout_id_;
+    host_items_struct_ = struct_id_;
     host_items_N_ = N_;
   }
   auto& v = host_items_[{tpl, nch, use_persistent() ? 1 : 0}];

@@ -130,7 +130,7 @@ class NBodySolver {
   struct ItemList {
     DevBuf buf;
     int tpl = -1, nch = 0, count = 0;
-    uint64_t layout = 0;
+    uint64_t structure = 0;
     int64_t N = -1;
   };
   void require_positions(const char* msg) const;
@@ -201,7 +201,9 @@ class NBodySolver {
     return persistent_mode_ == 1 || (persistent_mode_ == -1 && kind_code() == 2);
   }
   int64_t items_N_ = -1;
-  uint64_t items_layout_ = 0, layout_id_ = 0;
+  // layout_id_: positions (and radius) changed; struct_id_: N, system
+  // offsets or kinds changed (work lists depend on these only)
+  uint64_t items_struct_ = 0, layout_id_ = 0, struct_id_ = 0;
   DevBuf part_u_, part_w_;
   int tile_tpl_ = 0, tile_nch_ = 0;  // user override (0 = auto)
   bool autotune_ = false;
@@ -237,7 +239,7 @@ class NBodySolver {
     std::vector<SymItem> host_items;  // all passes, in launch order
     int tpl = 0, nchunks = 0, hmax = 0, nitems = 0, nrows = 0, ang = -1;
     int64_t N = -1, budget = 0;
-    uint64_t layout = 0;
+    uint64_t structure = 0;
     std::vector<SymPass> passes, active_passes;
     size_t max_slabs = 0;
     bool active_valid = false;  // active_items / active_passes hold the iterating systems
@@ -260,7 +262,7 @@ class NBodySolver {
   void launch_sym(int blocks, bool fp32, const void* rec, const double* F, const double* T,
                   double* U, double* W);
   std::map<std::tuple<int, int, int>, std::vector<PairItem>> host_items_;
-  uint64_t host_items_layout_ = 0;
+  uint64_t host_items_struct_ = 0;
   int64_t host_items_N_ = -1;
   const std::vector<PairItem>& host_items(int tpl, int nch);
   SegSolve seg_sys_, seg_multi_;

@@ -378,3 +378,47 @@ def test_symmetric_smallest_systems(wall):
     assert e.lastPairSymmetric()
     Uo, _ = oracle.mdot(pos, ETA, A, F=F, wall=wall, sys_sizes=sizes)
     assert _rel(U, Uo) <= 1e-12
+
+
+def test_symmetric_open_ensemble_with_four_targets_per_lane_request():
+    """A tiling request of 4 targets per lane is served with 2 for ensembles
+    (4 exists for one open system only); the product is unchanged."""
+    from libmobility_amd import NBody
+    sizes = [1500, 700, 2100]
+    pos = _cloud(sum(sizes), 81)
+    F = np.random.default_rng(82).normal(size=(sum(sizes), 3))
+    s = NBody("open", "open", "open")
+    s.setParameters()
+    s.setPairSymmetric(1)
+    s.setPairTiling(4, 0)
+    s.initialize(viscosity=ETA, hydrodynamicRadius=A)
+    s.setSystems(sizes)
+    s.setPositions(pos)
+    U, _ = s.Mdot(F)
+    assert s.lastPairSymmetric() and s.lastPairTiling()[0] == 2
+    Uo, _ = oracle.mdot(pos, ETA, A, F=F, sys_sizes=sizes)
+    assert _rel(U, Uo) <= 1e-12
+
+
+@pytest.mark.parametrize("sym", [0, 1])
+@pytest.mark.parametrize("sizes", [None, [700, 0, 1300]])
+def test_new_positions_of_the_same_systems_keep_work_lists_but_move_the_result(sym, sizes):
+    """setPositions with the same N and systems (a time loop): the work lists
+    are kept, the product follows the new positions."""
+    from libmobility_amd import NBody
+    s = NBody("open", "open", "single_wall")
+    s.setParameters(wallHeight=0.0)
+    s.setPairSymmetric(sym)
+    s.initialize(viscosity=ETA, hydrodynamicRadius=A)
+    N = 2000
+    if sizes is not None:
+        s.setSystems(sizes)
+    rng = np.random.default_rng(91)
+    F = rng.normal(size=(N, 3))
+    for step in range(3):
+        pos = _wall_cloud(N, 92 + step)
+        s.setPositions(pos)
+        U, _ = s.Mdot(F)
+        assert s.lastPairSymmetric() == bool(sym)
+        Uo, _ = oracle.mdot(pos, ETA, A, F=F, wall=True, sys_sizes=sizes)
+        assert _rel(U, Uo) <= 1e-12, step
