@@ -257,6 +257,23 @@ int lm_nbody_set_preconditioner(lm_nbody* h, lm_precond* p);
 /* The blocks G (9N doubles, row-major 3x3 per particle) for the positions. */
 int lm_nbody_precond_blocks(lm_nbody* h, double* G9);
 
+/* Extension, no counterpart in the reference: coarse level of the sqrtMdotW
+ * preconditioner for one system.  Particles are grouped into spatial cells of
+ * about particles_per_cell particles (coarse dimension 3C <= 16384); with Q
+ * the orthonormal cell columns and Q^T M Q = L L^T, Lanczos runs on
+ * G M G^T with G = D (I + Q (L^{-1} - I) Q^T) (D the learned blocks when a
+ * network is attached, else I) and the result is G^{-1} (G M G^T)^{1/2} W.
+ * 0 switches it off.  Built on the first sqrtMdotW after each setPositions. */
+int lm_nbody_set_coarse_level(lm_nbody* h, int particles_per_cell);
+/* Cells of the last coarse build: *ncells, grid[3] (cells per axis before
+ * empty ones are dropped) and, if cell_of != NULL, the cell of each particle
+ * (N ints). */
+int lm_nbody_coarse_cells(lm_nbody* h, int* ncells, int* grid, int* cell_of);
+/* L and L^{-1} of the last coarse build, 3C x 3C row-major doubles each. */
+int lm_nbody_coarse_factors(lm_nbody* h, double* L, double* Linv);
+/* Wall-clock milliseconds of the last coarse build (cells, Q^T M Q, factors). */
+int lm_nbody_coarse_build_ms(lm_nbody* h, double* ms);
+
 /* ---- self-checks of the kernel arithmetic, run on the host CPU -----------
  * Not used by any solver path.  They evaluate the same pair and generator
  * code that the kernels are compiled from, so CPU tests can check it
@@ -317,6 +334,12 @@ int lm_diag_valu_throughput(int kind, int blocks, int iters, float* ms);
 
 /* Philox4x32-10 of the noise kernels: out = philox(ctr, key). */
 int lm_selftest_philox_host(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4);
+
+/* Cell assignment of the coarse level for N positions xyz (3N doubles, units
+ * of a): *ncells, grid[3] and cell_of (N ints), as lm_nbody_coarse_cells
+ * reports after a build. */
+int lm_selftest_coarse_cells_host(const double* xyz, int64_t N, int particles_per_cell,
+                                  int* ncells, int* grid, int* cell_of);
 
 #ifdef __cplusplus
 }

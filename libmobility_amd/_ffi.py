@@ -79,6 +79,10 @@ SIGNATURES = {
     "lm_nbody_precond_dataset": (ctypes.c_int, [_vp, _i64, ctypes.c_uint64, _vp, _vp]),
     "lm_nbody_set_preconditioner": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_precond_blocks": (ctypes.c_int, [_vp, _vp]),
+    "lm_nbody_set_coarse_level": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lm_nbody_coarse_cells": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "lm_nbody_coarse_factors": (ctypes.c_int, [_vp, _vp, _vp]),
+    "lm_nbody_coarse_build_ms": (ctypes.c_int, [_vp, _vp]),
     "lm_selftest_pair_block_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp,
                                                    ctypes.c_double, _dp]),
     "lm_self_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -103,6 +107,8 @@ SIGNATURES = {
     "lm_selftest_philox_host": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32),
                                                ctypes.POINTER(ctypes.c_uint32),
                                                ctypes.POINTER(ctypes.c_uint32)]),
+    "lm_selftest_coarse_cells_host": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int, _vp, _vp,
+                                                     _vp]),
 }
 
 _lib = None

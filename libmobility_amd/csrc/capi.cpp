@@ -1,9 +1,11 @@
 // C ABI (include/libmobility_amd.h) over lm::NBodySolver.
 #include "../../include/libmobility_amd.h"
 
+#include <algorithm>
 #include <exception>
 #include <string>
 
+#include "coarse.hpp"
 #include "pair_math.hpp"
 #include "philox.hpp"
 #include "self_solver.hpp"
@@ -301,6 +303,31 @@ int lm_nbody_precond_blocks(lm_nbody* h, double* G9) {
   });
 }
 
+int lm_nbody_set_coarse_level(lm_nbody* h, int particles_per_cell) {
+  return guarded([&] { S(h).setCoarseLevel(particles_per_cell); });
+}
+
+int lm_nbody_coarse_cells(lm_nbody* h, int* ncells, int* grid, int* cell_of) {
+  return guarded([&] {
+    if (!ncells || !grid) throw lm::Error("[libmobility_amd] null output");
+    *ncells = S(h).coarseCells(cell_of, grid);
+  });
+}
+
+int lm_nbody_coarse_factors(lm_nbody* h, double* L, double* Linv) {
+  return guarded([&] {
+    if (!L || !Linv) throw lm::Error("[libmobility_amd] null output");
+    S(h).coarseFactors(L, Linv);
+  });
+}
+
+int lm_nbody_coarse_build_ms(lm_nbody* h, double* ms) {
+  return guarded([&] {
+    if (!ms) throw lm::Error("[libmobility_amd] null output");
+    *ms = S(h).coarseBuildMs();
+  });
+}
+
 int lm_selftest_pair_block_host(int wall, int fp32, const double* pi, const double* pj, double a,
                                 double* blk) {
   return guarded([&] {
@@ -474,6 +501,18 @@ int lm_selftest_philox_host(const uint32_t* ctr4, const uint32_t* key2, uint32_t
     for (int i = 0; i < 4; ++i) c.v[i] = ctr4[i];
     const lm::U32x4 r = lm::philox4x32_10(c, key2[0], key2[1]);
     for (int i = 0; i < 4; ++i) out4[i] = r.v[i];
+  });
+}
+
+int lm_selftest_coarse_cells_host(const double* xyz, int64_t N, int particles_per_cell,
+                                  int* ncells, int* grid, int* cell_of) {
+  return guarded([&] {
+    if (!xyz || !ncells || !grid || !cell_of) throw lm::Error("[libmobility_amd] null argument");
+    lm::CoarseCells c;
+    lm::coarse_cells(xyz, 3, N, particles_per_cell, c);
+    *ncells = c.C;
+    for (int k = 0; k < 3; ++k) grid[k] = c.g[k];
+    std::copy(c.cell_of.begin(), c.cell_of.end(), cell_of);
   });
 }
 

@@ -946,8 +946,8 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     old_n_ = n;
   }
   auto ensure_cols = [&](int64_t cols) { ensure_basis(n, cols); };
-  if (precond_) {
-    ensure_precond_blocks();
+  if (pre_on()) {
+    ensure_pre();
     pc_t_.reserve(sizeof(double) * n);
     pc_u_.reserve(sizeof(double) * n);
   }
@@ -992,17 +992,17 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
 }
 
 // w = M v ; torques are stored after the forces (MobilityInterface.h:164-176).
-// With the learned preconditioner: w = G M G v, G block-diagonal.
+// With the preconditioner: w = G M G^T v, G = D Gc.
 void NBodySolver::lanczos_product(const double* v, double* w) {
   const int64_t n = (angular_ ? 6 : 3) * N_;
   const int64_t N3 = 3 * N_;
-  if (precond_) {
+  if (pre_on()) {
     double* t = pc_t_.as<double>();
     double* u = pc_u_.as<double>();
-    launch_apply_blocks(pc_G_.as<double>(), v, N_, n, t, stream_);
+    pre_forward(v, t, n);
     pair_product(rec64_.as<void>(), rec32_.as<void>(), t, angular_ ? t + N3 : nullptr, u,
                  angular_ ? u + N3 : nullptr);
-    launch_apply_blocks(pc_G_.as<double>(), u, N_, n, w, stream_);
+    pre_back(u, w, n);
   } else {
     pair_product(rec64_.as<void>(), rec32_.as<void>(), v, angular_ ? v + N3 : nullptr, w,
                  angular_ ? w + N3 : nullptr);
@@ -1241,8 +1241,8 @@ void NBodySolver::sqrtMdotW_dev(double* U, double* W, double prefactor, const do
   }
   double* y = nullptr;
   if (seg_mode_ && numSystems() > 1) {
-    if (precond_) {
-      ensure_precond_blocks();
+    if (pre_on()) {
+      ensure_pre();
       pc_t_.reserve(sizeof(double) * n);
       pc_u_.reserve(sizeof(double) * n);
     }
@@ -1251,9 +1251,9 @@ void NBodySolver::sqrtMdotW_dev(double* U, double* W, double prefactor, const do
   } else {
     lanczos_run(z, &y);
   }
-  if (precond_) {
-    // B W = G^{-1} (G M G)^{1/2} W
-    launch_apply_blocks(pc_Gi_.as<double>(), y, N_, n, pc_t_.as<double>(), stream_);
+  if (pre_on()) {
+    // B W = G^{-1} (G M G^T)^{1/2} W
+    pre_inverse(y, pc_t_.as<double>(), n);
     y = pc_t_.as<double>();
   }
   // out += prefactor * y (MobilityInterface.h:179-187)
@@ -1367,17 +1367,15 @@ void NBodySolver::MdotMulti(const double* F, int nrhs, double* U, bool dev) {
   LM_HIP(hipStreamSynchronize(stream_));
 }
 
-// w = M v (or G M G v) for nrhs stacked vectors
+// w = M v (or G M G^T v) for nrhs stacked vectors
 void NBodySolver::multi_product(const double* v, int nrhs, double* w) {
   const int64_t n3 = 3 * N_;
-  if (precond_) {
+  if (pre_on()) {
     double* t = multi_t_.as<double>();
     double* u = multi_w_.as<double>();
-    for (int r = 0; r < nrhs; ++r)
-      launch_apply_blocks(pc_G_.as<double>(), v + r * n3, N_, n3, t + r * n3, stream_);
+    for (int r = 0; r < nrhs; ++r) pre_forward(v + r * n3, t + r * n3, n3);
     mdot_multi_dev(t, nrhs, u);
-    for (int r = 0; r < nrhs; ++r)
-      launch_apply_blocks(pc_G_.as<double>(), u + r * n3, N_, n3, w + r * n3, stream_);
+    for (int r = 0; r < nrhs; ++r) pre_back(u + r * n3, w + r * n3, n3);
   } else {
     mdot_multi_dev(v, nrhs, w);
   }
@@ -1439,8 +1437,8 @@ void NBodySolver::sqrtMdotWMulti(double* U, int nrhs, double prefactor, bool dev
     else
       off.push_back((int64_t)(r + 1) * N_);
   }
-  if (precond_) {
-    ensure_precond_blocks();
+  if (pre_on()) {
+    ensure_pre();
     multi_t_.reserve(sizeof(double) * total);
     multi_w_.reserve(sizeof(double) * total);
   }
@@ -1449,10 +1447,9 @@ void NBodySolver::sqrtMdotWMulti(double* U, int nrhs, double prefactor, bool dev
                   [this, nrhs](const double* v, double* w) { multi_product(v, nrhs, w); }, z, &y);
   multi_nrhs_ = nrhs;
   multi_per_ = per_sys ? nsys : 1;
-  if (precond_) {
+  if (pre_on()) {
     double* t = multi_t_.as<double>();
-    for (int r = 0; r < nrhs; ++r)
-      launch_apply_blocks(pc_Gi_.as<double>(), y + r * n3, N_, n3, t + r * n3, stream_);
+    for (int r = 0; r < nrhs; ++r) pre_inverse(y + r * n3, t + r * n3, n3);
     y = t;
   }
   launch_axpy_accumulate(Ud, y, prefactor, total, stream_);  // out += prefactor * y
@@ -1498,6 +1495,70 @@ void NBodySolver::ensure_precond_blocks() {
   pc_layout_ = layout_id_;
   pc_version_ = precond_->version();
 }
+
+void NBodySolver::setCoarseLevel(int particles_per_cell) {
+  if (particles_per_cell < 0)
+    throw Error("[libmobility_amd] particles per cell must be >= 0 (0: no coarse level)");
+  coarse_ppc_ = particles_per_cell;
+}
+
+void NBodySolver::ensure_coarse() {
+  if (coarse_ppc_ <= 0) return;
+  if (numSystems() != 1)
+    throw Error("[libmobility_amd] the coarse level needs a single system (setSystems with one system or none)");
+  ensure_records();
+  if (coarse_.layout == layout_id_ && coarse_.ppc == coarse_ppc_ && coarse_.cells() > 0) return;
+  const bool wall = sys_kind_host_.empty() ? wall_ : sys_kind_host_[0] != 0;
+  coarse_.build(rec64_.as<void>(), rec32_.as<void>(), N_, wall, coarse_ppc_, stream_);
+  coarse_.layout = layout_id_;
+  coarse_.ppc = coarse_ppc_;
+}
+
+void NBodySolver::ensure_pre() {
+  if (precond_) ensure_precond_blocks();
+  ensure_coarse();
+}
+
+// G = D Gc: t = G^T v = Gc^T D v
+void NBodySolver::pre_forward(const double* v, double* t, int64_t n) {
+  if (precond_) {
+    launch_apply_blocks(pc_G_.as<double>(), v, N_, n, t, stream_);
+    if (coarse_ppc_ > 0) coarse_.apply(CoarseLevel::kGt, t, t, n, stream_);
+  } else {
+    coarse_.apply(CoarseLevel::kGt, v, t, n, stream_);
+  }
+}
+
+// w = G u = D Gc u
+void NBodySolver::pre_back(double* u, double* w, int64_t n) {
+  if (precond_) {
+    if (coarse_ppc_ > 0) coarse_.apply(CoarseLevel::kG, u, u, n, stream_);
+    launch_apply_blocks(pc_G_.as<double>(), u, N_, n, w, stream_);
+  } else {
+    coarse_.apply(CoarseLevel::kG, u, w, n, stream_);
+  }
+}
+
+// out = G^{-1} y = Gc^{-1} D^{-1} y
+void NBodySolver::pre_inverse(const double* y, double* out, int64_t n) {
+  if (precond_) {
+    launch_apply_blocks(pc_Gi_.as<double>(), y, N_, n, out, stream_);
+    if (coarse_ppc_ > 0) coarse_.apply(CoarseLevel::kGinv, out, out, n, stream_);
+  } else {
+    coarse_.apply(CoarseLevel::kGinv, y, out, n, stream_);
+  }
+}
+
+int NBodySolver::coarseCells(int* cell_of, int* grid) const {
+  const CoarseCells& c = coarse_.host_cells();
+  if (c.C == 0) throw Error("[libmobility_amd] no coarse level has been built");
+  if (cell_of) std::copy(c.cell_of.begin(), c.cell_of.end(), cell_of);
+  if (grid)
+    for (int k = 0; k < 3; ++k) grid[k] = c.g[k];
+  return c.C;
+}
+
+void NBodySolver::coarseFactors(double* L, double* X) const { coarse_.factors(L, X, stream_); }
 
 void NBodySolver::precondDataset(int64_t ncenters, uint64_t seed, float* X, float* T9) {
   ensure_knn();

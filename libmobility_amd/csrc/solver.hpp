@@ -13,6 +13,7 @@
 #include <tuple>
 #include <vector>
 
+#include "coarse.hpp"
 #include "kernels.hpp"
 #include "lm_common.hpp"
 #include "pair_math.hpp"
@@ -107,6 +108,18 @@ class NBodySolver {
   void precondDataset(int64_t ncenters, uint64_t seed, float* X, float* T9);
   // The 3x3 blocks G (9N, host) the attached network gives for the positions.
   void precondBlocks(double* G9);
+  // Coarse level of the sqrtMdotW preconditioner (one system): cells of about
+  // `particles_per_cell` particles, Lanczos on G M G^T with G = D Gc (D the
+  // learned blocks when attached, Gc the coarse whitening), result
+  // G^{-1} (G M G^T)^{1/2} W.  0 switches it off.
+  void setCoarseLevel(int particles_per_cell);
+  int coarseLevel() const { return coarse_ppc_; }
+  // Cells of the last coarse build (cell_of: N entries, host) and the grid;
+  // returns the number of cells.
+  int coarseCells(int* cell_of, int* grid) const;
+  // L and L^{-1} (3C x 3C each, host) of the last coarse build.
+  void coarseFactors(double* L, double* X) const;
+  double coarseBuildMs() const { return coarse_.build_ms; }
 
   // Device-pointer cores (no staging, no synchronisation beyond what the
   // algorithm needs).
@@ -155,8 +168,15 @@ class NBodySolver {
                       double** result);
   void ensure_seg_layout(SegSolve& S, const std::vector<int64_t>& off, int angular, int64_t n);
   void ensure_basis(int64_t n, int64_t cols);
-  // w = M v (or G M G v with the preconditioner) for the Lanczos recurrences
+  // w = M v (or G M G^T v with the preconditioner) for the Lanczos recurrences
   void lanczos_product(const double* v, double* w);
+  // G = D Gc, either factor optional: t = G^T v; w = G u (u overwritten);
+  // out = G^{-1} y.  Vectors of length n >= 3N.
+  bool pre_on() const { return precond_ != nullptr || coarse_ppc_ > 0; }
+  void ensure_pre();
+  void pre_forward(const double* v, double* t, int64_t n);
+  void pre_back(double* u, double* w, int64_t n);
+  void pre_inverse(const double* y, double* out, int64_t n);
 
   // configuration
   bool wall_ = false;
@@ -283,6 +303,10 @@ class NBodySolver {
   LearnedPreconditioner* precond_ = nullptr;
   DevBuf pc_feat_, pc_nbr_, pc_out_, pc_G_, pc_Gi_, pc_t_, pc_u_;
   uint64_t knn_layout_ = 0, pc_layout_ = 0, pc_version_ = 0;
+  // coarse level
+  void ensure_coarse();
+  CoarseLevel coarse_;
+  int coarse_ppc_ = 0;
   // RFD
   DevBuf rfd_noise_, rfd_pos_, rfd_rec64_, rfd_rec32_, rfd_mp_, rfd_mm_, rfd_wp_, rfd_wm_;
 };

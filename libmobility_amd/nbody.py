@@ -400,6 +400,43 @@ class NBody(_Solver):
         _ffi.check(self._lib.lm_nbody_precond_blocks(self._h, G.ctypes.data))
         return G
 
+    def setCoarseLevel(self, particlesPerCell=64):
+        """Extension: coarse level of the sqrtMdotW preconditioner (one system).
+
+        Particles are grouped into spatial cells of about ``particlesPerCell``
+        particles.  With Q the orthonormal cell columns and ``Q^T M Q = L L^T``,
+        Lanczos runs on ``G M G^T`` with ``G = D (I + Q (L^{-1} - I) Q^T)`` (D the
+        learned blocks when a network is attached) and the result is
+        ``G^{-1} (G M G^T)^{1/2} W``.  ``0`` or ``None`` switches it off.
+        """
+        ppc = 0 if particlesPerCell is None else int(particlesPerCell)
+        _ffi.check(self._lib.lm_nbody_set_coarse_level(self._h, ppc))
+
+    def coarseCells(self):
+        """Extension: (cell of each particle (N,), cells per axis (3,), number of
+        non-empty cells) of the last coarse build."""
+        N = self.getNumberParticles()
+        cell_of = np.zeros(N, dtype=np.int32)
+        grid = np.zeros(3, dtype=np.int32)
+        nc = ctypes.c_int(0)
+        _ffi.check(self._lib.lm_nbody_coarse_cells(self._h, ctypes.addressof(nc),
+                                                   grid.ctypes.data, cell_of.ctypes.data))
+        return cell_of, grid, nc.value
+
+    def coarseFactors(self):
+        """Extension: (L, L^{-1}) of the last coarse build, (3C, 3C) each."""
+        _, _, nc = self.coarseCells()
+        L = np.zeros((3 * nc, 3 * nc))
+        X = np.zeros((3 * nc, 3 * nc))
+        _ffi.check(self._lib.lm_nbody_coarse_factors(self._h, L.ctypes.data, X.ctypes.data))
+        return L, X
+
+    def coarseBuildMs(self):
+        """Extension: wall-clock milliseconds of the last coarse build."""
+        ms = ctypes.c_double(0.0)
+        _ffi.check(self._lib.lm_nbody_coarse_build_ms(self._h, ctypes.addressof(ms)))
+        return ms.value
+
     def initialize(self, viscosity, hydrodynamicRadius, includeAngular=False, tolerance=1e-4,
                    seed=None, lanczos_callback=None):
         """pythonify.h:241-260 -> NBody::initialize (mobility.h:99-114)."""

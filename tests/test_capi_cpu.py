@@ -188,3 +188,38 @@ def test_libmobility_surface_and_selfmobility_configuration():
         libMobility.SelfMobility("open", "open", "nope")
     s = libMobility.SelfMobility("open", "open", "open")
     s.setParameters(parameter=5.0)
+
+
+@pytest.mark.parametrize("shape", ["cube", "slab", "plane", "one", "capped"])
+def test_coarse_cells_match_the_oracle_rule(shape):
+    """The library's cell assignment of the coarse level is the oracle's rule,
+    integer for integer (oracle/coarse.py)."""
+    from oracle import coarse
+    from libmobility_amd.workloads import rsa_cube
+    lib = _lib()
+    rng = np.random.default_rng(11)
+    ppc = 32
+    if shape == "cube":
+        xyz, _ = rsa_cube(3000, 0.2, seed=3)
+    elif shape == "slab":
+        xyz = rng.uniform(size=(5000, 3)) * np.array([256.0, 256.0, 18.5]) + np.array([0, 0, 1.5])
+    elif shape == "plane":
+        xyz = rng.uniform(size=(700, 3)) * np.array([30.0, 50.0, 0.0])
+    elif shape == "one":
+        xyz = np.array([[1.0, 2.0, 3.0]])
+    else:
+        xyz = rng.uniform(size=(30000, 3)) * 100.0
+        ppc = 1
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    N = xyz.shape[0]
+    nc = ctypes.c_int(0)
+    grid = np.zeros(3, dtype=np.int32)
+    cell_of = np.zeros(N, dtype=np.int32)
+    st = lib.lm_selftest_coarse_cells_host(xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           N, ppc, ctypes.addressof(nc), grid.ctypes.data,
+                                           cell_of.ctypes.data)
+    assert st == 0
+    ref, g, C = coarse.coarse_cells(xyz, ppc)
+    assert nc.value == C and list(grid) == list(g)
+    assert np.array_equal(cell_of, ref)
+    assert 3 * C <= 16384

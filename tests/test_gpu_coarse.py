@@ -1,0 +1,163 @@
+"""GPU checks of the coarse level of the sqrtMdotW preconditioner (no
+reference counterpart; the oracle is oracle/coarse.py, pinned in
+tests/test_oracle_coarse.py).
+
+  * the cells of a build are the oracle's cells for the same positions;
+  * L L^T equals the oracle's Q^T M Q (FP32 pair terms: 1e-5 relative) and
+    L^{-1} L = I to FP64 rounding;
+  * the result is a square root of M: ||B B^T - M|| / ||M|| <= 1e-5 from
+    dense columns, open / wall / wall with torques, with and without the
+    learned blocks;
+  * sqrtMdotW equals the oracle's two-level Lanczos run with the build's L
+    on the same W;
+  * fewer Lanczos iterations than plain Lanczos on an RSA packing;
+  * the multi-vector square root agrees with the single-vector one;
+  * errors: ensembles, negative cell targets, queries before a build.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import coarse
+
+pytestmark = pytest.mark.gpu
+
+ETA = 1.0 / (6 * np.pi)
+
+
+def _solver(pos, wall=False, angular=False, tol=1e-6, ppc=8):
+    from libmobility_amd import NBody
+    s = NBody("open", "open", "single_wall" if wall else "open")
+    if wall:
+        s.setParameters(wallHeight=0.0)
+    else:
+        s.setParameters()
+    s.initialize(viscosity=ETA, hydrodynamicRadius=1.0, includeAngular=angular, tolerance=tol)
+    s.setPositions(pos)
+    s.setCoarseLevel(ppc)
+    return s
+
+
+def _wall_positions(N, seed):
+    rng = np.random.default_rng(seed)
+    L = (N / 0.05 / 8.0) ** 0.5
+    return np.column_stack([rng.uniform(0, L, N), rng.uniform(0, L, N), rng.uniform(1.5, 9.5, N)])
+
+
+def _records_xyz(pos, wall):
+    """Positions as the solver's records hold them (a = 1): wall heights are
+    max(z, a)."""
+    xyz = np.array(pos, dtype=np.float64).reshape(-1, 3).copy()
+    if wall:
+        xyz[:, 2] = np.maximum(xyz[:, 2], 1.0)
+    return xyz
+
+
+@pytest.mark.parametrize("wall", [False, True])
+def test_cells_and_factors_match_the_oracle(wall):
+    from libmobility_amd.workloads import rsa_cube
+    N = 1500
+    pos = _wall_positions(N, 3) if wall else rsa_cube(N, 0.2, seed=3)[0]
+    s = _solver(pos, wall=wall, ppc=24)
+    s.sqrtMdotW_with_noise(np.random.default_rng(0).normal(size=3 * N))
+    cell_of, grid, C = s.coarseCells()
+    ref, g, Cr = coarse.coarse_cells(_records_xyz(pos, wall), 24)
+    assert C == Cr and list(grid) == list(g)
+    assert np.array_equal(cell_of, ref)
+    L, X = s.coarseFactors()
+    M = oracle.dense_matrix(pos, ETA, 1.0, wall=wall) / (1.0 / (6 * np.pi * ETA))
+    Mc = coarse.coarse_matrix(M, cell_of, C)
+    assert np.allclose(L, np.tril(L))
+    rel = np.linalg.norm(L @ L.T - Mc) / np.linalg.norm(Mc)
+    print(f"[coarse] wall={wall} C={C} |LL^T - Mc|/|Mc| = {rel:.2e}")
+    assert rel <= 1e-5
+    assert np.abs(X @ L - np.eye(3 * C)).max() <= 1e-11
+    assert s.coarseBuildMs() > 0.0
+
+
+@pytest.mark.parametrize("case", ["open", "open+net", "wall", "wall+angular", "wall+angular+net"])
+def test_two_level_result_is_a_square_root(case):
+    from libmobility_amd.precond import LearnedPreconditioner
+    from libmobility_amd.workloads import rsa_cube
+    wall = case.startswith("wall")
+    angular = "angular" in case
+    N = 36
+    pos = _wall_positions(N, 5) if wall else rsa_cube(N, 0.2, seed=4)[0]
+    s = _solver(pos, wall=wall, angular=angular, tol=1e-8, ppc=4)
+    if "net" in case:
+        net = LearnedPreconditioner(seed=9)
+        s.setPreconditioner(net)
+    n = (6 if angular else 3) * N
+    B = np.zeros((n, n))
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1.0
+        u, w = s.sqrtMdotW_with_noise(e)
+        B[:, k] = np.concatenate([u.ravel(), w.ravel()]) if angular else u.ravel()
+    M = oracle.dense_matrix(pos, ETA, 1.0, angular=angular, wall=wall)
+    rel = np.linalg.norm(B @ B.T - M) / np.linalg.norm(M)
+    print(f"[coarse] {case}: cells {s.coarseCells()[2]}, |BB^T - M|/|M| = {rel:.2e}")
+    assert rel <= 1e-5
+
+
+def test_sqrt_matches_the_oracle_two_level_lanczos():
+    from libmobility_amd.workloads import rsa_cube
+    N = 600
+    pos, _ = rsa_cube(N, 0.2, seed=8)
+    s = _solver(pos, tol=1e-8, ppc=16)
+    W = np.random.default_rng(5).normal(size=3 * N)
+    u, _ = s.sqrtMdotW_with_noise(W)
+    cell_of, _, C = s.coarseCells()
+    L, X = s.coarseFactors()
+    M = oracle.dense_matrix(pos, ETA, 1.0)
+    ref, it = coarse.TwoLevel(coarse.coarse_Q(cell_of, C), L, X).sqrt(M, W, 1e-8)
+    rel = np.linalg.norm(u.ravel() - ref) / np.linalg.norm(ref)
+    print(f"[coarse] N={N}: GPU {s.lastLanczosIterations()} / oracle {it} iterations, rel {rel:.2e}")
+    assert rel <= 1e-6
+    assert abs(s.lastLanczosIterations() - it) <= 2
+
+
+def test_coarse_level_lowers_iterations():
+    from libmobility_amd.workloads import rsa_cube
+    N = 8192
+    pos, _ = rsa_cube(N, 0.2, seed=12)
+    W = np.random.default_rng(6).normal(size=3 * N)
+    s = _solver(pos, ppc=0)
+    s.setPairPrecision("fp32")
+    s.sqrtMdotW_with_noise(W)
+    plain = s.lastLanczosIterations()
+    s.setCoarseLevel(32)
+    s.sqrtMdotW_with_noise(W)
+    two = s.lastLanczosIterations()
+    print(f"[coarse] N={N}: plain {plain}, coarse(32) {two} iterations")
+    assert two <= 0.6 * plain
+
+
+def test_multi_vector_sqrt_with_coarse_level():
+    from libmobility_amd.workloads import rsa_cube
+    N = 900
+    pos, _ = rsa_cube(N, 0.2, seed=13)
+    s = _solver(pos, tol=1e-8, ppc=16)
+    rng = np.random.default_rng(7)
+    Ws = rng.normal(size=(3, 3 * N))
+    U = s.sqrtMdotWMulti_with_noise(Ws)
+    for r in range(3):
+        u, _ = s.sqrtMdotW_with_noise(Ws[r])
+        assert np.linalg.norm(U[r].ravel() - u.ravel()) <= 1e-6 * np.linalg.norm(u)
+
+
+def test_coarse_level_errors():
+    from libmobility_amd.workloads import rsa_cube
+    pos, _ = rsa_cube(200, 0.2, seed=14)
+    s = _solver(pos, ppc=0)
+    with pytest.raises(RuntimeError, match="particles per cell"):
+        s.setCoarseLevel(-1)
+    with pytest.raises(RuntimeError, match="no coarse level"):
+        s.coarseCells()
+    s.setSystems([100, 100])
+    s.setPositions(pos)
+    s.setCoarseLevel(8)
+    with pytest.raises(RuntimeError, match="single system"):
+        s.sqrtMdotW()
+    s.setCoarseLevel(0)
+    s.sqrtMdotW()
