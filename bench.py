@@ -43,6 +43,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mdot Gpair/s + sqrtMdotW ms at N=131072 tol 1e-6: learned-prec vs plain"
+# coarse level of the preconditioned sqrtMdotW: target particles per cell
+COARSE_PPC = 64
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
 PEAK_FP64_TFLOPS = 78.6
 ETA = 1.0 / (6.0 * np.pi)
@@ -140,18 +142,36 @@ class Gpu:
         self.sync()
         return e0.elapsed_time(e1) / max(steps, 1)
 
+    def coarse_cells(self, h):
+        nc = ctypes.c_int()
+        grid = (ctypes.c_int * 3)()
+        self.ffi.check(self.lib.lm_nbody_coarse_cells(h, ctypes.byref(nc), grid, None))
+        return nc.value
+
+    def coarse_build_ms(self, h):
+        ms = ctypes.c_double()
+        self.ffi.check(self.lib.lm_nbody_coarse_build_ms(h, ctypes.byref(ms)))
+        return ms.value
+
     def sqrt_noise(self, h, Wn, U, W=None, prefactor=1.0):
         self.ffi.check(self.lib.lm_nbody_sqrt_mdot_w_noise(
             h, Wn.data_ptr(), U.data_ptr(), None if W is None else W.data_ptr(), prefactor,
             self.ffi.LM_MEM_DEVICE))
 
-    def time_lanczos(self, h, Wn, n_out, calls, angular=False):
+    def time_lanczos(self, h, Wn, n_out, calls, angular=False, fresh_pos=None):
+        """Median ms per sqrtMdotW call after one untimed call.  With fresh_pos,
+        each timed call starts with setPositions(fresh_pos), so it includes
+        what new positions cost (records, and with a preconditioner the kNN
+        graph, the network and the coarse level)."""
         U = self.zeros(n_out)
         W = self.zeros(n_out) if angular else None
         times, iters = [], []
         for c in range(calls + 1):
             self.sync()
             t = time.perf_counter()
+            if fresh_pos is not None:
+                self.ffi.check(self.lib.lm_nbody_set_positions(
+                    h, fresh_pos.data_ptr(), fresh_pos.numel(), self.ffi.LM_MEM_DEVICE))
             self.sqrt_noise(h, Wn, U, W)
             self.sync()
             dt = (time.perf_counter() - t) * 1e3
@@ -229,7 +249,8 @@ def cpu_baseline_rows(pos, F, target_seconds=12.0, first=256):
 
 def bbt_errors(g, net, N=512, phi=0.20):
     """||B B^T - M||_F / ||M||_F of the FP32-pair Lanczos square root, plain and
-    with the learned preconditioner, at tol 1e-4 and 1e-6, on an RSA packing of
+    with the two-level preconditioner (learned blocks + coarse level), at tol
+    1e-4 and 1e-6, on an RSA packing of
     N particles.  Column k of B is sqrtMdotW applied to e_k; M is the solver's
     own FP64-pair matrix (Mdot of e_k), which the tests hold to 1e-12 of the
     oracle's dense matrix."""
@@ -251,6 +272,7 @@ def bbt_errors(g, net, N=512, phi=0.20):
             h = g.solver(pos_d, "fp32", tol=tol, autotune=False)
             if use_net:
                 g.ffi.check(g.lib.lm_nbody_set_preconditioner(h, net._h))
+                g.ffi.check(g.lib.lm_nbody_set_coarse_level(h, COARSE_PPC))
             Bcols = g.zeros(n * n).view(n, n)
             its = []
             for k in range(n):
@@ -373,16 +395,45 @@ def bench_c3(args):
         g.sqrt_noise(h64, W_d, y64)
         g.sync()
         extra["sqrtMdotW_rel_diff_fp32pair_vs_fp64pair"] = rel(y32.cpu().numpy(), y64.cpu().numpy())
-        log("learned-preconditioned Lanczos sqrtMdotW, FP32 pairs, tol 1e-6")
+        log("plain Lanczos sqrtMdotW, FP32 pairs, tol 1e-6, new positions before each call")
+        ms_l32p, _ = g.time_lanczos(h32, W_d, 3 * N, args.lanczos_calls, fresh_pos=pos_d)
+        extra["sqrtMdotW_plain_fp32pair_with_setPositions_ms"] = ms_l32p
         from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
         net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+        log(f"two-level preconditioned Lanczos sqrtMdotW (learned 3x3 blocks + coarse level of "
+            f"{COARSE_PPC} particles per cell), FP32 pairs, tol 1e-6")
         hl = g.solver(pos_d, "fp32", tiling=args.tiling)
         g.ffi.check(g.lib.lm_nbody_set_preconditioner(hl, net._h))
+        g.ffi.check(g.lib.lm_nbody_set_coarse_level(hl, COARSE_PPC))
         ms_ll, itl = g.time_lanczos(hl, W_d, 3 * N, args.lanczos_calls)
+        log("  the same with new positions before each call (kNN, network and coarse build "
+            "included)")
+        ms_llp, _ = g.time_lanczos(hl, W_d, 3 * N, args.lanczos_calls, fresh_pos=pos_d)
+        cells = g.coarse_cells(hl)
+        build = g.coarse_build_ms(hl)
         g.destroy(hl)
+        extra["sqrtMdotW_learned_preconditioner"] = (
+            f"learned 3x3 blocks (16-neighbour MLP, BF16 MFMA) + coarse level "
+            f"({COARSE_PPC} particles per cell target, {cells} cells)")
         extra["sqrtMdotW_learned_fp32pair_ms"] = ms_ll
         extra["sqrtMdotW_learned_fp32pair_iterations"] = itl
         extra["sqrtMdotW_learned_speedup_vs_plain"] = ms_l32 / ms_ll
+        extra["sqrtMdotW_learned_with_setPositions_ms"] = ms_llp
+        extra["sqrtMdotW_learned_speedup_vs_plain_with_setPositions"] = ms_l32p / ms_llp
+        extra["coarse_build_ms"] = build
+        log("ablation: coarse level only (no network)")
+        hc = g.solver(pos_d, "fp32", tiling=args.tiling)
+        g.ffi.check(g.lib.lm_nbody_set_coarse_level(hc, COARSE_PPC))
+        ms_c, itc = g.time_lanczos(hc, W_d, 3 * N, max(1, args.lanczos_calls - 1))
+        g.destroy(hc)
+        log("ablation: learned blocks only (no coarse level)")
+        hb = g.solver(pos_d, "fp32", tiling=args.tiling)
+        g.ffi.check(g.lib.lm_nbody_set_preconditioner(hb, net._h))
+        ms_b, itb = g.time_lanczos(hb, W_d, 3 * N, max(1, args.lanczos_calls - 2))
+        g.destroy(hb)
+        extra["sqrtMdotW_ablation"] = {
+            "coarse_only_ms": ms_c, "coarse_only_iterations": itc,
+            "learned_blocks_only_ms": ms_b, "learned_blocks_only_iterations": itb}
         if not args.no_bbt:
             log("square-root check at small N (columns of B from unit vectors)")
             extra["bbt_rel_err_small_N"] = bbt_errors(g, net)
@@ -787,8 +838,9 @@ def bench_c5(args):
         from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
         net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
         g.ffi.check(g.lib.lm_nbody_set_preconditioner(h32, net._h))
-        log("hydrodynamicVelocities (Mdot + sqrtMdotW tol 1e-6 + RFD drift), FP32 pairs, learned "
-            "preconditioner")
+        g.ffi.check(g.lib.lm_nbody_set_coarse_level(h32, COARSE_PPC))
+        log("hydrodynamicVelocities (Mdot + sqrtMdotW tol 1e-6 + RFD drift), FP32 pairs, "
+            "two-level preconditioner (learned blocks + coarse level)")
         g.sync()
         t = t_start = time.perf_counter()
         g.ffi.check(g.lib.lm_nbody_hydrodynamic_velocities(
@@ -801,6 +853,8 @@ def bench_c5(args):
         u = U_d.cpu().numpy()
         extra.update({
             "hydrodynamicVelocities_fp32pair_learned_ms": step_ms,
+            "preconditioner": f"learned 3x3 blocks + coarse level ({g.coarse_cells(h32)} cells, "
+                              f"built in {g.coarse_build_ms(h32):.0f} ms)",
             "lanczos_iterations": it,
             "output_finite": bool(np.all(np.isfinite(u))),
             "fp64pair_step_projected_ms": (it + 4) * ms64,
