@@ -51,7 +51,7 @@ void coarse_cells(const double* xyz, int64_t stride, int64_t N, int ppc, CoarseC
     int C = 0;
     for (int64_t c = 0; c < total; ++c) C += count[c] > 0;
     if (3 * (int64_t)C <= CoarseLevel::kMaxDim) break;
-    side *= 1.1;  // too many cells: coarser grid
+    side *= 1.03;  // too many cells: coarser grid
   }
   // compact the non-empty cells in raw order, then a stable counting sort
   std::vector<int> compact(count.size(), -1);

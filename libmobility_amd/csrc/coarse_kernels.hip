@@ -153,54 +153,63 @@ __device__ inline void tile_mm(const double (*A)[kP], const double (*B)[kP], dou
 
 // ---- Cholesky Mc = L L^T (lower, right-looking, 64 x 64 tiles) -----------------
 
-// Factor the diagonal tile k in LDS, write L_kk (upper part zeroed) and its
-// inverse Dinv_k.  info[0] = 1 when a pivot is not positive.
-__global__ __launch_bounds__(256) void k_chol_diag(double* __restrict__ A, int64_t ld, int k,
-                                                   double* __restrict__ Dinv,
-                                                   int* __restrict__ info) {
-  __shared__ double L[kT][kP];
-  __shared__ double X[kT][kP];
-  const int t = threadIdx.x;
+// value of v in lane `lane` (uniform) of the wave
+__device__ inline double lane_value(double v, int lane) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Factor the diagonal tile k, write L_kk (upper part zeroed) and its inverse
+// Dinv_k.  One wave: lane r holds row r in registers; column j of the
+// right-looking factorisation reaches the other lanes by readlane, so no
+// barrier is needed.  The inverse: lane c forms column c of L_kk^{-1} by
+// forward substitution, L_kk rows read from LDS (same address in every lane).
+// info[0] = 1 when a pivot is not positive.
+__global__ __launch_bounds__(64) void k_chol_diag(double* __restrict__ A, int64_t ld, int k,
+                                                  double* __restrict__ Dinv,
+                                                  int* __restrict__ info) {
+  __shared__ double Ls[kT][kP];
+  const int r = threadIdx.x;
   double* Akk = A + (int64_t)k * kT * ld + (int64_t)k * kT;
-  tile_load(L, Akk, ld);
-  __syncthreads();
+  double a[kT];
+#pragma unroll
+  for (int c = 0; c < kT; ++c) a[c] = Akk[(int64_t)r * ld + c];
+  bool bad = false;
+#pragma unroll
   for (int j = 0; j < kT; ++j) {
-    if (t == 0) {
-      double dd = L[j][j];
-      if (!(dd > 0.0)) {
-        info[0] = 1;
-        dd = 1.0;
-      }
-      L[j][j] = sqrt(dd);
+    double d = lane_value(a[j], j);
+    if (!(d > 0.0)) {
+      bad = true;
+      d = 1.0;
     }
-    __syncthreads();
-    if (t > j && t < kT) L[t][j] /= L[j][j];
-    __syncthreads();
-    for (int e = t; e < kT * kT; e += 256) {
-      const int r = e >> 6, c = e & 63;
-      if (c > j && r >= c) L[r][c] -= L[r][j] * L[c][j];
-    }
-    __syncthreads();
+    const double piv = sqrt(d);
+    const double ipiv = 1.0 / piv;
+    a[j] = r == j ? piv : (r > j ? a[j] * ipiv : a[j]);
+    // rows r >= c: A_rc -= L_rj L_cj (rows r < c hold the unused upper part)
+#pragma unroll
+    for (int c = j + 1; c < kT; ++c) a[c] = __builtin_fma(-a[j], lane_value(a[j], c), a[c]);
   }
-  // column t of L^{-1} by forward substitution
-  if (t < kT) {
-    for (int i = 0; i < kT; ++i) {
-      if (i < t) {
-        X[i][t] = 0.0;
-        continue;
-      }
-      double s = (i == t) ? 1.0 : 0.0;
-      for (int m = t; m < i; ++m) s -= L[i][m] * X[m][t];
-      X[i][t] = s / L[i][i];
-    }
+#pragma unroll
+  for (int c = 0; c < kT; ++c) {
+    const double v = c <= r ? a[c] : 0.0;
+    Ls[r][c] = v;
+    Akk[(int64_t)r * ld + c] = v;
   }
+  if (bad && r == 0) info[0] = 1;
   __syncthreads();
-  double* D = Dinv + (int64_t)k * kT * kT;
-  for (int e = t; e < kT * kT; e += 256) {
-    const int r = e >> 6, c = e & 63;
-    Akk[r * ld + c] = c <= r ? L[r][c] : 0.0;
-    D[e] = X[r][c];
+  double x[kT];
+#pragma unroll
+  for (int i = 0; i < kT; ++i) {
+    double s[4] = {i == r ? 1.0 : 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < i; ++m) s[m & 3] = __builtin_fma(-Ls[i][m], x[m], s[m & 3]);
+    x[i] = ((s[0] + s[1]) + (s[2] + s[3])) * (1.0 / Ls[i][i]);
   }
+  double* D = Dinv + (int64_t)k * kT * kT;
+#pragma unroll
+  for (int i = 0; i < kT; ++i) D[i * kT + r] = x[i];
 }
 
 // L_ik = A_ik L_kk^{-T} for the tiles below the diagonal tile k
@@ -303,16 +312,34 @@ __global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ X,
 
 // ---- per-application kernels ------------------------------------------------------
 
-// cvec[3c+a] = scale[c] * sum over the particles of cell c (sorted order) of in[3i+a]
-__global__ void k_coarse_restrict(const double* __restrict__ in, const int* __restrict__ perm,
-                                  const int* __restrict__ start, const double* __restrict__ scale,
-                                  int C, double* __restrict__ cvec) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 3 * C) return;
-  const int c = e / 3, a = e - 3 * c;
-  double s = 0.0;
-  for (int k = start[c]; k < start[c + 1]; ++k) s += in[3 * (int64_t)perm[k] + a];
-  cvec[e] = scale[c] * s;
+// cvec[3c+a] = scale[c] * sum over the particles of cell c of in[3i+a]; one
+// wave per cell, lanes stride the cell's sorted list, fixed-order butterfly.
+__global__ __launch_bounds__(256) void k_coarse_restrict(const double* __restrict__ in,
+                                                         const int* __restrict__ perm,
+                                                         const int* __restrict__ start,
+                                                         const double* __restrict__ scale, int C,
+                                                         double* __restrict__ cvec) {
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= C) return;  // uniform per wave
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int k = start[c] + lane; k < start[c + 1]; k += 64) {
+    const int64_t p = 3 * (int64_t)perm[k];
+    s0 += in[p];
+    s1 += in[p + 1];
+    s2 += in[p + 2];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    s0 += __shfl_xor(s0, o);
+    s1 += __shfl_xor(s1, o);
+    s2 += __shfl_xor(s2, o);
+  }
+  if (lane == 0) {
+    cvec[3 * c] = scale[c] * s0;
+    cvec[3 * c + 1] = scale[c] * s1;
+    cvec[3 * c + 2] = scale[c] * s2;
+  }
 }
 
 // y[r] = sum_m T[r][m] x[m] - x[r] over m <= r (lower) or m >= r (upper), m < n3c;
@@ -389,7 +416,7 @@ void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* i
                           int n, hipStream_t s) {
   const int nb = n / kT;
   for (int k = 0; k < nb; ++k) {
-    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, A, ld, k, Dinv, info);
+    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(64), 0, s, A, ld, k, Dinv, info);
     const int below = nb - k - 1;
     if (below > 0) {
       hipLaunchKernelGGL(k_chol_panel, dim3(below), dim3(256), 0, s, A, ld, k, Dinv);
@@ -415,8 +442,8 @@ void launch_coarse_apply(const double* T, int64_t ld, int n3c, bool lower, const
                          double* out, int64_t N, int64_t n, const int* perm, const int* start,
                          const int* cell_of, const double* scale, int C, double* cvec,
                          double* yvec, hipStream_t s) {
-  hipLaunchKernelGGL(k_coarse_restrict, dim3(blocks_for(3 * (int64_t)C, 128)), dim3(128), 0, s, in,
-                     perm, start, scale, C, cvec);
+  hipLaunchKernelGGL(k_coarse_restrict, dim3(blocks_for(C, 4)), dim3(256), 0, s, in, perm, start,
+                     scale, C, cvec);
   hipLaunchKernelGGL(k_coarse_gemv, dim3(n3c), dim3(256), 0, s, T, ld, n3c, lower ? 1 : 0, cvec,
                      yvec);
   hipLaunchKernelGGL(k_coarse_prolong, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, cell_of,

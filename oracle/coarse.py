@@ -8,7 +8,7 @@ Only tests/ may import this module.
 
 Definitions (one system, positions in units of a):
   * cells: g_k = max(1, round(max(ext_k, 1) / s)) cells along axis k, with
-    s = cbrt(prod_k max(ext_k, 1) / max(1, N / ppc)) grown by 10 % until the
+    s = cbrt(prod_k max(ext_k, 1) / max(1, N / ppc)) grown by 3 % until the
     non-empty cells C satisfy 3C <= max_dim; particle i is in cell
     q_k = min(g_k - 1, max(0, floor((x_k - lo_k) / ext_k * g_k))); empty cells
     are dropped and the rest numbered in raw order (x slowest).
@@ -50,7 +50,7 @@ def coarse_cells(xyz, ppc, max_dim=16384):
         used = np.unique(raw)
         if 3 * used.size <= max_dim:
             break
-        side *= 1.1
+        side *= 1.03
     cell_of = np.searchsorted(used, raw)
     return cell_of.astype(np.int32), g, used.size
 
