@@ -161,3 +161,37 @@ def test_coarse_level_errors():
         s.sqrtMdotW()
     s.setCoarseLevel(0)
     s.sqrtMdotW()
+
+
+@pytest.mark.parametrize("case", ["one", "plane", "fewer_than_a_cell", "wall_below_a"])
+def test_coarse_level_edge_cases(case):
+    """Degenerate cell grids: one particle (one cell), coplanar particles (an
+    axis of zero extent), fewer particles than one cell target, and wall
+    particles below z = a (cells use the clamped heights).  The result is a
+    square root of M (dense columns)."""
+    rng = np.random.default_rng(21)
+    wall = case == "wall_below_a"
+    if case == "one":
+        pos = np.array([[0.3, -1.2, 2.0]])
+    elif case == "plane":
+        pos = np.column_stack([rng.uniform(0, 12, 20), rng.uniform(0, 9, 20), np.full(20, 4.0)])
+    elif case == "fewer_than_a_cell":
+        pos = rng.uniform(0, 10, size=(12, 3))
+    else:
+        pos = np.column_stack([rng.uniform(0, 10, 16), rng.uniform(0, 10, 16),
+                               rng.uniform(0.2, 3.0, 16)])
+    N = pos.shape[0]
+    s = _solver(pos, wall=wall, tol=1e-10, ppc=6 if case != "fewer_than_a_cell" else 64)
+    n = 3 * N
+    B = np.zeros((n, n))
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1.0
+        B[:, k] = s.sqrtMdotW_with_noise(e)[0].ravel()
+    cell_of, grid, C = s.coarseCells()
+    ref, g, Cr = coarse.coarse_cells(_records_xyz(pos, wall), 6 if case != "fewer_than_a_cell" else 64)
+    assert np.array_equal(cell_of, ref) and C == Cr
+    M = oracle.dense_matrix(pos, ETA, 1.0, wall=wall)
+    rel = np.linalg.norm(B @ B.T - M) / np.linalg.norm(M)
+    print(f"[coarse] {case}: N={N} cells {C} grid {list(grid)} |BB^T - M|/|M| = {rel:.2e}")
+    assert rel <= 1e-7
