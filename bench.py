@@ -411,7 +411,26 @@ def bench_c3(args):
         ms_llp, _ = g.time_lanczos(hl, W_d, 3 * N, args.lanczos_calls, fresh_pos=pos_d)
         cells = g.coarse_cells(hl)
         build = g.coarse_build_ms(hl)
+        # the last timed call rebuilt the preconditioner; kept for `refresh`
+        # setPositions calls, the next `refresh` calls hold exactly one build
+        refresh = 10
+        log(f"  the same, preconditioner kept for {refresh} setPositions calls (one build in "
+            f"{refresh} calls)")
+        g.ffi.check(g.lib.lm_nbody_set_preconditioner_refresh(hl, refresh))
+        nb0, nb = ctypes.c_int64(), ctypes.c_int64()
+        g.ffi.check(g.lib.lm_nbody_coarse_builds(hl, ctypes.byref(nb0)))
+        Ur = g.zeros(3 * N)
+        g.sync()
+        t = time.perf_counter()
+        for _ in range(refresh):
+            g.ffi.check(g.lib.lm_nbody_set_positions(hl, pos_d.data_ptr(), pos_d.numel(),
+                                                     g.ffi.LM_MEM_DEVICE))
+            g.sqrt_noise(hl, W_d, Ur)
+        g.sync()
+        ms_refresh = (time.perf_counter() - t) * 1e3 / refresh
+        g.ffi.check(g.lib.lm_nbody_coarse_builds(hl, ctypes.byref(nb)))
         g.destroy(hl)
+        log(f"  {ms_refresh:.1f} ms per call, {nb.value - nb0.value} coarse build(s)")
         extra["sqrtMdotW_learned_preconditioner"] = (
             f"learned 3x3 blocks (16-neighbour MLP, BF16 MFMA) + coarse level "
             f"({COARSE_PPC} particles per cell target, {cells} cells)")
@@ -420,6 +439,9 @@ def bench_c3(args):
         extra["sqrtMdotW_learned_speedup_vs_plain"] = ms_l32 / ms_ll
         extra["sqrtMdotW_learned_with_setPositions_ms"] = ms_llp
         extra["sqrtMdotW_learned_speedup_vs_plain_with_setPositions"] = ms_l32p / ms_llp
+        extra["sqrtMdotW_learned_with_setPositions_refresh10_ms"] = ms_refresh
+        extra["sqrtMdotW_learned_refresh10_coarse_builds"] = nb.value - nb0.value
+        extra["sqrtMdotW_learned_speedup_vs_plain_with_setPositions_refresh10"] = ms_l32p / ms_refresh
         extra["coarse_build_ms"] = build
         log("ablation: coarse level only (no network)")
         hc = g.solver(pos_d, "fp32", tiling=args.tiling)

@@ -273,6 +273,15 @@ int lm_nbody_coarse_cells(lm_nbody* h, int* ncells, int* grid, int* cell_of);
 int lm_nbody_coarse_factors(lm_nbody* h, double* L, double* Linv);
 /* Wall-clock milliseconds of the last coarse build (cells, Q^T M Q, factors). */
 int lm_nbody_coarse_build_ms(lm_nbody* h, double* ms);
+/* Number of coarse builds so far on this handle. */
+int lm_nbody_coarse_builds(lm_nbody* h, int64_t* count);
+/* Extension, no counterpart in the reference: keep the preconditioner (the
+ * learned blocks and the coarse level) for `interval` set_positions calls of
+ * the same systems (same N, offsets, kinds and radius) before rebuilding it;
+ * default 1 (rebuilt for every new positions).  G = D Gc from earlier
+ * positions keeps its exact inverse, so the result is still a square root of
+ * the current M; only the iteration count can drift.  interval >= 1. */
+int lm_nbody_set_preconditioner_refresh(lm_nbody* h, int interval);
 
 /* ---- self-checks of the kernel arithmetic, run on the host CPU -----------
  * Not used by any solver path.  They evaluate the same pair and generator

@@ -83,6 +83,8 @@ SIGNATURES = {
     "lm_nbody_coarse_cells": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "lm_nbody_coarse_factors": (ctypes.c_int, [_vp, _vp, _vp]),
     "lm_nbody_coarse_build_ms": (ctypes.c_int, [_vp, _vp]),
+    "lm_nbody_coarse_builds": (ctypes.c_int, [_vp, _vp]),
+    "lm_nbody_set_preconditioner_refresh": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lm_selftest_pair_block_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp,
                                                    ctypes.c_double, _dp]),
     "lm_self_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),

@@ -328,6 +328,17 @@ int lm_nbody_coarse_build_ms(lm_nbody* h, double* ms) {
   });
 }
 
+int lm_nbody_coarse_builds(lm_nbody* h, int64_t* count) {
+  return guarded([&] {
+    if (!count) throw lm::Error("[libmobility_amd] null output");
+    *count = S(h).coarseBuilds();
+  });
+}
+
+int lm_nbody_set_preconditioner_refresh(lm_nbody* h, int interval) {
+  return guarded([&] { S(h).setPreconditionerRefresh(interval); });
+}
+
 int lm_selftest_pair_block_host(int wall, int fp32, const double* pi, const double* pj, double a,
                                 double* blk) {
   return guarded([&] {

@@ -70,7 +70,10 @@ class CoarseLevel {
   // L and X = L^{-1} (3C x 3C row-major, host) of the last build
   void factors(double* L, double* X, hipStream_t s) const;
 
-  uint64_t layout = 0;  // the solver's layout id and cell target it was built for
+  // what it was built for: the solver's layout and structure ids, radius and
+  // cell target
+  uint64_t layout = 0, structure = 0;
+  double a = 0.0;
   int ppc = 0;
   double build_ms = 0.0;
 

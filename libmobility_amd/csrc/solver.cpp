@@ -1482,10 +1482,25 @@ void NBodySolver::ensure_knn() {
   knn_layout_ = layout_id_;
 }
 
+// A preconditioner part built for earlier positions of the same systems (same
+// N, offsets, kinds and radius) is still exact: G keeps its exact inverse, so
+// B B^T = M holds for the new positions and only the iteration count can
+// drift.  It is kept for pc_refresh_ setPositions calls.
+bool NBodySolver::pre_current(uint64_t layout, uint64_t structure, double a) const {
+  return layout != 0 && structure == struct_id_ && a == a_ && layout_id_ >= layout &&
+         layout_id_ - layout < (uint64_t)pc_refresh_;
+}
+
+void NBodySolver::setPreconditionerRefresh(int interval) {
+  if (interval < 1) throw Error("[libmobility_amd] the refresh interval must be >= 1");
+  pc_refresh_ = interval;
+}
+
 void NBodySolver::ensure_precond_blocks() {
   if (!precond_) throw Error("[libmobility_amd] no preconditioner attached");
+  ensure_records();
+  if (pc_version_ == precond_->version() && pre_current(pc_layout_, pc_struct_, pc_a_)) return;
   ensure_knn();
-  if (pc_layout_ == layout_id_ && pc_version_ == precond_->version()) return;
   pc_out_.reserve(sizeof(float) * N_ * kPrecondOut);
   pc_G_.reserve(sizeof(double) * 9 * N_);
   pc_Gi_.reserve(sizeof(double) * 9 * N_);
@@ -1493,6 +1508,8 @@ void NBodySolver::ensure_precond_blocks() {
   launch_make_blocks(pc_out_.as<float>(), pc_nbr_.as<int>(), N_, pc_G_.as<double>(),
                      pc_Gi_.as<double>(), stream_);
   pc_layout_ = layout_id_;
+  pc_struct_ = struct_id_;
+  pc_a_ = a_;
   pc_version_ = precond_->version();
 }
 
@@ -1507,11 +1524,16 @@ void NBodySolver::ensure_coarse() {
   if (numSystems() != 1)
     throw Error("[libmobility_amd] the coarse level needs a single system (setSystems with one system or none)");
   ensure_records();
-  if (coarse_.layout == layout_id_ && coarse_.ppc == coarse_ppc_ && coarse_.cells() > 0) return;
+  if (coarse_.ppc == coarse_ppc_ && coarse_.cells() > 0 &&
+      pre_current(coarse_.layout, coarse_.structure, coarse_.a))
+    return;
   const bool wall = sys_kind_host_.empty() ? wall_ : sys_kind_host_[0] != 0;
   coarse_.build(rec64_.as<void>(), rec32_.as<void>(), N_, wall, coarse_ppc_, stream_);
   coarse_.layout = layout_id_;
+  coarse_.structure = struct_id_;
+  coarse_.a = a_;
   coarse_.ppc = coarse_ppc_;
+  ++coarse_builds_;
 }
 
 void NBodySolver::ensure_pre() {

@@ -120,6 +120,12 @@ class NBodySolver {
   // L and L^{-1} (3C x 3C each, host) of the last coarse build.
   void coarseFactors(double* L, double* X) const;
   double coarseBuildMs() const { return coarse_.build_ms; }
+  int64_t coarseBuilds() const { return coarse_builds_; }
+  // Keep the preconditioner (learned blocks and coarse level) for `interval`
+  // setPositions calls of the same systems before rebuilding it (default 1:
+  // rebuilt for every new positions).  Results stay exact square roots of the
+  // current M; stale parts only cost iterations.
+  void setPreconditionerRefresh(int interval);
 
   // Device-pointer cores (no staging, no synchronisation beyond what the
   // algorithm needs).
@@ -174,6 +180,7 @@ class NBodySolver {
   // out = G^{-1} y.  Vectors of length n >= 3N.
   bool pre_on() const { return precond_ != nullptr || coarse_ppc_ > 0; }
   void ensure_pre();
+  bool pre_current(uint64_t layout, uint64_t structure, double a) const;
   void pre_forward(const double* v, double* t, int64_t n);
   void pre_back(double* u, double* w, int64_t n);
   void pre_inverse(const double* y, double* out, int64_t n);
@@ -302,11 +309,14 @@ class NBodySolver {
   void ensure_precond_blocks();
   LearnedPreconditioner* precond_ = nullptr;
   DevBuf pc_feat_, pc_nbr_, pc_out_, pc_G_, pc_Gi_, pc_t_, pc_u_;
-  uint64_t knn_layout_ = 0, pc_layout_ = 0, pc_version_ = 0;
+  uint64_t knn_layout_ = 0, pc_layout_ = 0, pc_struct_ = 0, pc_version_ = 0;
+  double pc_a_ = 0.0;
+  int pc_refresh_ = 1;
   // coarse level
   void ensure_coarse();
   CoarseLevel coarse_;
   int coarse_ppc_ = 0;
+  int64_t coarse_builds_ = 0;
   // RFD
   DevBuf rfd_noise_, rfd_pos_, rfd_rec64_, rfd_rec32_, rfd_mp_, rfd_mm_, rfd_wp_, rfd_wm_;
 };

@@ -437,6 +437,19 @@ class NBody(_Solver):
         _ffi.check(self._lib.lm_nbody_coarse_build_ms(self._h, ctypes.addressof(ms)))
         return ms.value
 
+    def coarseBuilds(self):
+        """Extension: number of coarse builds so far."""
+        n = ctypes.c_int64(0)
+        _ffi.check(self._lib.lm_nbody_coarse_builds(self._h, ctypes.addressof(n)))
+        return n.value
+
+    def setPreconditionerRefresh(self, interval=1):
+        """Extension: keep the preconditioner (learned blocks and coarse level)
+        for ``interval`` setPositions calls of the same systems before
+        rebuilding it.  The result stays an exact square root of the current M
+        (G keeps its exact inverse); only the iteration count can drift."""
+        _ffi.check(self._lib.lm_nbody_set_preconditioner_refresh(self._h, int(interval)))
+
     def initialize(self, viscosity, hydrodynamicRadius, includeAngular=False, tolerance=1e-4,
                    seed=None, lanczos_callback=None):
         """pythonify.h:241-260 -> NBody::initialize (mobility.h:99-114)."""

@@ -195,3 +195,50 @@ def test_coarse_level_edge_cases(case):
     rel = np.linalg.norm(B @ B.T - M) / np.linalg.norm(M)
     print(f"[coarse] {case}: N={N} cells {C} grid {list(grid)} |BB^T - M|/|M| = {rel:.2e}")
     assert rel <= 1e-7
+
+
+def test_refresh_interval_keeps_an_exact_square_root():
+    """With a refresh interval, the preconditioner built for earlier positions
+    is kept: the result is still a square root of the current M (dense
+    columns), the cells are those of the earlier positions, and the build
+    comes back after `interval` setPositions calls or when N changes."""
+    from libmobility_amd.precond import LearnedPreconditioner
+    from libmobility_amd.workloads import rsa_cube
+    N = 36
+    pos0, _ = rsa_cube(N, 0.2, seed=31)
+    pos1 = pos0[::-1] * np.array([1.0, 0.7, 1.3]) + 2.0  # other cells for the same particles
+    s = _solver(pos0, tol=1e-8, ppc=4)
+    net = LearnedPreconditioner(seed=9)
+    s.setPreconditioner(net)
+    s.setPreconditionerRefresh(3)
+    with pytest.raises(RuntimeError, match="refresh interval"):
+        s.setPreconditionerRefresh(0)
+    s.sqrtMdotW_with_noise(np.ones(3 * N))
+    assert s.coarseBuilds() == 1
+    cells0 = s.coarseCells()[0].copy()
+    s.setPositions(pos1)
+    n = 3 * N
+    B = np.zeros((n, n))
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1.0
+        B[:, k] = s.sqrtMdotW_with_noise(e)[0].ravel()
+    assert s.coarseBuilds() == 1
+    assert np.array_equal(s.coarseCells()[0], cells0)
+    M1 = oracle.dense_matrix(pos1, ETA, 1.0)
+    rel = np.linalg.norm(B @ B.T - M1) / np.linalg.norm(M1)
+    print(f"[coarse] stale preconditioner: |BB^T - M|/|M| = {rel:.2e}")
+    assert rel <= 1e-5
+    ref1, _, _ = coarse.coarse_cells(pos1, 4)
+    assert not np.array_equal(ref1, cells0)
+    s.setPositions(pos1)
+    s.sqrtMdotW_with_noise(np.ones(3 * N))
+    assert s.coarseBuilds() == 1
+    s.setPositions(pos1)
+    s.sqrtMdotW_with_noise(np.ones(3 * N))
+    assert s.coarseBuilds() == 2
+    assert np.array_equal(s.coarseCells()[0], ref1)
+    pos2, _ = rsa_cube(N + 4, 0.2, seed=32)
+    s.setPositions(pos2)
+    s.sqrtMdotW_with_noise(np.ones(3 * (N + 4)))
+    assert s.coarseBuilds() == 3
