@@ -129,26 +129,50 @@ __device__ inline void tile_load(double (*S)[kP], const double* __restrict__ g, 
   for (int e = threadIdx.x; e < kT * kT; e += blockDim.x) S[e >> 6][e & 63] = g[(e >> 6) * ld + (e & 63)];
 }
 
-// acc[a][b] = sum_m A[r0+a][m] * (TB ? B[c0+b][m] : B[m][c0+b]), 256 threads,
-// each a 4 x 4 block of the 64 x 64 result.
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// P = A * (TB ? B^T : B) for 64 x 64 tiles in LDS on the FP64 matrix cores
+// (256 threads): wave w computes the 32 x 32
+// quadrant (w >> 1, w & 1) as 2 x 2 tiles of v_mfma_f64_16x16x4f64.  Operand
+// lanes: A[i][k] from lane 16 k + i, B[k][j] from lane 16 k + j; result
+// register r of lane l holds row 4 r + (l >> 4) of column l & 15 (layout
+// measured on gfx950).  Every index stays in the 64 x 64 tile.
 template <bool TB>
-__device__ inline void tile_mm(const double (*A)[kP], const double (*B)[kP], double acc[4][4]) {
-  const int r0 = (threadIdx.x >> 4) * 4, c0 = (threadIdx.x & 15) * 4;
+__device__ inline void tile_mm_mfma(const double (*A)[kP], const double (*B)[kP], d4 acc[2][2]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int R0 = (w >> 1) * 32, C0 = (w & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-  for (int m = 0; m < kT; ++m) {
-    double x[4], y[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) x[a] = A[r0 + a][m];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) y[b] = TB ? B[c0 + b][m] : B[m][c0 + b];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_fma(x[a], y[b], acc[a][b]);
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int kk = 0; kk < kT; kk += 4) {
+    const double a0 = A[R0 + li][kk + lk], a1 = A[R0 + 16 + li][kk + lk];
+    const double b0 = TB ? B[C0 + li][kk + lk] : B[kk + lk][C0 + li];
+    const double b1 = TB ? B[C0 + 16 + li][kk + lk] : B[kk + lk][C0 + 16 + li];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
   }
+}
+
+// out[row][col] (= or -=) the quadrant result of tile_mm_mfma
+template <bool SUB>
+__device__ inline void tile_store_mfma(double* __restrict__ out, int64_t ld, const d4 acc[2][2]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int R0 = (w >> 1) * 32, C0 = (w & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double* o = out + (int64_t)(R0 + 16 * a + 4 * r + lk) * ld + C0 + 16 * b + li;
+        *o = SUB ? *o - acc[a][b][r] : acc[a][b][r];
+      }
 }
 
 // ---- Cholesky Mc = L L^T (lower, right-looking, 64 x 64 tiles) -----------------
@@ -222,13 +246,9 @@ __global__ __launch_bounds__(256) void k_chol_panel(double* __restrict__ A, int6
   tile_load(P, Aik, ld);
   tile_load(Dk, Dinv + (int64_t)k * kT * kT, kT);
   __syncthreads();
-  double acc[4][4];
-  tile_mm<true>(P, Dk, acc);
-  const int r0 = (threadIdx.x >> 4) * 4, c0 = (threadIdx.x & 15) * 4;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) Aik[(r0 + a) * ld + c0 + b] = acc[a][b];
+  d4 acc[2][2];
+  tile_mm_mfma<true>(P, Dk, acc);
+  tile_store_mfma<false>(Aik, ld, acc);
 }
 
 // A_ij -= L_ik L_jk^T for k < j <= i < nb
@@ -241,14 +261,9 @@ __global__ __launch_bounds__(256) void k_chol_update(double* __restrict__ A, int
   tile_load(Pi, A + (int64_t)i * kT * ld + (int64_t)k * kT, ld);
   tile_load(Pj, A + (int64_t)j * kT * ld + (int64_t)k * kT, ld);
   __syncthreads();
-  double acc[4][4];
-  tile_mm<true>(Pi, Pj, acc);
-  double* Aij = A + (int64_t)i * kT * ld + (int64_t)j * kT;
-  const int r0 = (threadIdx.x >> 4) * 4, c0 = (threadIdx.x & 15) * 4;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) Aij[(r0 + a) * ld + c0 + b] -= acc[a][b];
+  d4 acc[2][2];
+  tile_mm_mfma<true>(Pi, Pj, acc);
+  tile_store_mfma<true>(A + (int64_t)i * kT * ld + (int64_t)j * kT, ld, acc);
 }
 
 // ---- X = L^{-1}: block forward substitution on L X = I ----------------------------
@@ -264,13 +279,9 @@ __global__ __launch_bounds__(256) void k_trinv_scale(double* __restrict__ X, int
   tile_load(R, Xkm, ld);
   tile_load(Dk, Dinv + (int64_t)k * kT * kT, kT);
   __syncthreads();
-  double acc[4][4];
-  tile_mm<false>(Dk, R, acc);
-  const int r0 = (threadIdx.x >> 4) * 4, c0 = (threadIdx.x & 15) * 4;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) Xkm[(r0 + a) * ld + c0 + b] = acc[a][b];
+  d4 acc[2][2];
+  tile_mm_mfma<false>(Dk, R, acc);
+  tile_store_mfma<false>(Xkm, ld, acc);
 }
 
 __global__ __launch_bounds__(256) void k_trinv_update(double* __restrict__ X,
@@ -283,14 +294,9 @@ __global__ __launch_bounds__(256) void k_trinv_update(double* __restrict__ X,
   tile_load(Lik, L + (int64_t)i * kT * ld + (int64_t)k * kT, ld);
   tile_load(R, X + (int64_t)k * kT * ld + (int64_t)m * kT, ld);
   __syncthreads();
-  double acc[4][4];
-  tile_mm<false>(Lik, R, acc);
-  double* Xim = X + (int64_t)i * kT * ld + (int64_t)m * kT;
-  const int r0 = (threadIdx.x >> 4) * 4, c0 = (threadIdx.x & 15) * 4;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) Xim[(r0 + a) * ld + c0 + b] -= acc[a][b];
+  d4 acc[2][2];
+  tile_mm_mfma<false>(Lik, R, acc);
+  tile_store_mfma<true>(X + (int64_t)i * kT * ld + (int64_t)m * kT, ld, acc);
 }
 
 __global__ void k_identity(double* __restrict__ X, int64_t ld, int n) {
