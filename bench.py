@@ -44,7 +44,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mdot Gpair/s + sqrtMdotW ms at N=131072 tol 1e-6: learned-prec vs plain"
 # coarse level of the preconditioned sqrtMdotW: target particles per cell
-COARSE_PPC = 64
+COARSE_PPC = 48
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
 PEAK_FP64_TFLOPS = 78.6
 ETA = 1.0 / (6.0 * np.pi)

@@ -22,6 +22,7 @@ namespace {
 
 constexpr int kT = kCoarseTile;  // 64: dense tile edge
 constexpr int kP = kT + 1;       // LDS row pitch (doubles)
+constexpr int kKB = 4;           // tile columns per block of the factorisation updates
 
 // ---- cell-sorted records ---------------------------------------------------
 
@@ -131,7 +132,14 @@ __device__ inline void tile_load(double (*S)[kP], const double* __restrict__ g, 
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-// P = A * (TB ? B^T : B) for 64 x 64 tiles in LDS on the FP64 matrix cores
+__device__ inline void tile_zero(d4 acc[2][2]) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+// acc += A * (TB ? B^T : B) for 64 x 64 tiles in LDS on the FP64 matrix cores
 // (256 threads): wave w computes the 32 x 32
 // quadrant (w >> 1, w & 1) as 2 x 2 tiles of v_mfma_f64_16x16x4f64.  Operand
 // lanes: A[i][k] from lane 16 k + i, B[k][j] from lane 16 k + j; result
@@ -142,10 +150,6 @@ __device__ inline void tile_mm_mfma(const double (*A)[kP], const double (*B)[kP]
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int R0 = (w >> 1) * 32, C0 = (w & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int kk = 0; kk < kT; kk += 4) {
     const double a0 = A[R0 + li][kk + lk], a1 = A[R0 + 16 + li][kk + lk];
@@ -247,22 +251,50 @@ __global__ __launch_bounds__(256) void k_chol_panel(double* __restrict__ A, int6
   tile_load(Dk, Dinv + (int64_t)k * kT * kT, kT);
   __syncthreads();
   d4 acc[2][2];
+  tile_zero(acc);
   tile_mm_mfma<true>(P, Dk, acc);
   tile_store_mfma<false>(Aik, ld, acc);
 }
 
-// A_ij -= L_ik L_jk^T for k < j <= i < nb
-__global__ __launch_bounds__(256) void k_chol_update(double* __restrict__ A, int64_t ld, int k) {
+// The updates are blocked by kKB tile columns: inside a block of columns
+// [kb, kend) the step k updates only the block's own columns, and the tiles
+// right of the block receive the block's kKB products in one read-modify-write.
+
+// A_ij -= L_ik L_jk^T for the block's columns k < j < kend, rows j <= i < nb
+__global__ __launch_bounds__(256) void k_chol_local(double* __restrict__ A, int64_t ld, int k,
+                                                    int kend) {
   __shared__ double Pi[kT][kP];
   __shared__ double Pj[kT][kP];
-  int ii, jj;
-  tri_index(blockIdx.x, ii, jj);
-  const int i = k + 1 + ii, j = k + 1 + jj;
+  const int jcount = kend - k - 1;
+  const int j = k + 1 + (int)(blockIdx.x % jcount);
+  const int i = k + 1 + (int)(blockIdx.x / jcount);
+  if (i < j) return;  // uniform per workgroup
   tile_load(Pi, A + (int64_t)i * kT * ld + (int64_t)k * kT, ld);
   tile_load(Pj, A + (int64_t)j * kT * ld + (int64_t)k * kT, ld);
   __syncthreads();
   d4 acc[2][2];
+  tile_zero(acc);
   tile_mm_mfma<true>(Pi, Pj, acc);
+  tile_store_mfma<true>(A + (int64_t)i * kT * ld + (int64_t)j * kT, ld, acc);
+}
+
+// A_ij -= sum_{k in [kb, kend)} L_ik L_jk^T for kend <= j <= i < nb
+__global__ __launch_bounds__(256) void k_chol_trailing(double* __restrict__ A, int64_t ld, int kb,
+                                                       int kend) {
+  __shared__ double Pi[kT][kP];
+  __shared__ double Pj[kT][kP];
+  int ii, jj;
+  tri_index(blockIdx.x, ii, jj);
+  const int i = kend + ii, j = kend + jj;
+  d4 acc[2][2];
+  tile_zero(acc);
+  for (int k = kb; k < kend; ++k) {
+    if (k > kb) __syncthreads();  // the previous products have read the tiles
+    tile_load(Pi, A + (int64_t)i * kT * ld + (int64_t)k * kT, ld);
+    tile_load(Pj, A + (int64_t)j * kT * ld + (int64_t)k * kT, ld);
+    __syncthreads();
+    tile_mm_mfma<true>(Pi, Pj, acc);
+  }
   tile_store_mfma<true>(A + (int64_t)i * kT * ld + (int64_t)j * kT, ld, acc);
 }
 
@@ -280,13 +312,15 @@ __global__ __launch_bounds__(256) void k_trinv_scale(double* __restrict__ X, int
   tile_load(Dk, Dinv + (int64_t)k * kT * kT, kT);
   __syncthreads();
   d4 acc[2][2];
+  tile_zero(acc);
   tile_mm_mfma<false>(Dk, R, acc);
   tile_store_mfma<false>(Xkm, ld, acc);
 }
 
-__global__ __launch_bounds__(256) void k_trinv_update(double* __restrict__ X,
-                                                      const double* __restrict__ L, int64_t ld,
-                                                      int k) {
+// X_im -= L_ik X_km for the block's rows k < i < kend, m <= k
+__global__ __launch_bounds__(256) void k_trinv_local(double* __restrict__ X,
+                                                     const double* __restrict__ L, int64_t ld,
+                                                     int k) {
   __shared__ double Lik[kT][kP];
   __shared__ double R[kT][kP];
   const int m = blockIdx.x % (k + 1);
@@ -295,7 +329,30 @@ __global__ __launch_bounds__(256) void k_trinv_update(double* __restrict__ X,
   tile_load(R, X + (int64_t)k * kT * ld + (int64_t)m * kT, ld);
   __syncthreads();
   d4 acc[2][2];
+  tile_zero(acc);
   tile_mm_mfma<false>(Lik, R, acc);
+  tile_store_mfma<true>(X + (int64_t)i * kT * ld + (int64_t)m * kT, ld, acc);
+}
+
+// X_im -= sum_{k in [max(kb, m), kend)} L_ik X_km for kend <= i < nb, m < kend
+// (X_km = 0 for m > k)
+__global__ __launch_bounds__(256) void k_trinv_trailing(double* __restrict__ X,
+                                                        const double* __restrict__ L, int64_t ld,
+                                                        int kb, int kend) {
+  __shared__ double Lik[kT][kP];
+  __shared__ double R[kT][kP];
+  const int m = blockIdx.x % kend;
+  const int i = kend + blockIdx.x / kend;
+  d4 acc[2][2];
+  tile_zero(acc);
+  const int k0 = m > kb ? m : kb;
+  for (int k = k0; k < kend; ++k) {
+    if (k > k0) __syncthreads();
+    tile_load(Lik, L + (int64_t)i * kT * ld + (int64_t)k * kT, ld);
+    tile_load(R, X + (int64_t)k * kT * ld + (int64_t)m * kT, ld);
+    __syncthreads();
+    tile_mm_mfma<false>(Lik, R, acc);
+  }
   tile_store_mfma<true>(X + (int64_t)i * kT * ld + (int64_t)m * kT, ld, acc);
 }
 
@@ -421,23 +478,35 @@ void launch_coarse_matrix(const float4* rec, const int* start, const double* sca
 void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* info, int64_t ld,
                           int n, hipStream_t s) {
   const int nb = n / kT;
-  for (int k = 0; k < nb; ++k) {
-    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(64), 0, s, A, ld, k, Dinv, info);
-    const int below = nb - k - 1;
-    if (below > 0) {
-      hipLaunchKernelGGL(k_chol_panel, dim3(below), dim3(256), 0, s, A, ld, k, Dinv);
-      hipLaunchKernelGGL(k_chol_update, dim3((unsigned)((int64_t)below * (below + 1) / 2)),
-                         dim3(256), 0, s, A, ld, k);
+  for (int kb = 0; kb < nb; kb += kKB) {
+    const int kend = kb + kKB < nb ? kb + kKB : nb;
+    for (int k = kb; k < kend; ++k) {
+      hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(64), 0, s, A, ld, k, Dinv, info);
+      const int below = nb - k - 1;
+      if (below > 0) hipLaunchKernelGGL(k_chol_panel, dim3(below), dim3(256), 0, s, A, ld, k, Dinv);
+      if (kend - k - 1 > 0)
+        hipLaunchKernelGGL(k_chol_local, dim3((unsigned)((kend - k - 1) * below)), dim3(256), 0, s,
+                           A, ld, k, kend);
     }
+    const int T = nb - kend;
+    if (T > 0)
+      hipLaunchKernelGGL(k_chol_trailing, dim3((unsigned)((int64_t)T * (T + 1) / 2)), dim3(256), 0,
+                         s, A, ld, kb, kend);
   }
   LM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_identity, dim3(blocks_for((int64_t)n * n, 256)), dim3(256), 0, s, X, ld, n);
-  for (int k = 0; k < nb; ++k) {
-    hipLaunchKernelGGL(k_trinv_scale, dim3(k + 1), dim3(256), 0, s, X, ld, k, Dinv);
-    const int below = nb - k - 1;
-    if (below > 0)
-      hipLaunchKernelGGL(k_trinv_update, dim3((unsigned)((int64_t)below * (k + 1))), dim3(256), 0,
-                         s, X, A, ld, k);
+  for (int kb = 0; kb < nb; kb += kKB) {
+    const int kend = kb + kKB < nb ? kb + kKB : nb;
+    for (int k = kb; k < kend; ++k) {
+      hipLaunchKernelGGL(k_trinv_scale, dim3(k + 1), dim3(256), 0, s, X, ld, k, Dinv);
+      if (kend - k - 1 > 0)
+        hipLaunchKernelGGL(k_trinv_local, dim3((unsigned)((kend - k - 1) * (k + 1))), dim3(256), 0,
+                           s, X, A, ld, k);
+    }
+    const int T = nb - kend;
+    if (T > 0)
+      hipLaunchKernelGGL(k_trinv_trailing, dim3((unsigned)((int64_t)T * kend)), dim3(256), 0, s, X,
+                         A, ld, kb, kend);
   }
   LM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_transpose, dim3(nb, nb), dim3(256), 0, s, X, Xt, ld);
