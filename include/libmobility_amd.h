@@ -258,19 +258,22 @@ int lm_nbody_set_preconditioner(lm_nbody* h, lm_precond* p);
 int lm_nbody_precond_blocks(lm_nbody* h, double* G9);
 
 /* Extension, no counterpart in the reference: coarse level of the sqrtMdotW
- * preconditioner for one system.  Particles are grouped into spatial cells of
- * about particles_per_cell particles (coarse dimension 3C <= 16384); with Q
- * the orthonormal cell columns and Q^T M Q = L L^T, Lanczos runs on
- * G M G^T with G = D (I + Q (L^{-1} - I) Q^T) (D the learned blocks when a
- * network is attached, else I) and the result is G^{-1} (G M G^T)^{1/2} W.
- * 0 switches it off.  Built on the first sqrtMdotW after each setPositions. */
+ * preconditioner.  The particles of every system are grouped into spatial
+ * cells of about particles_per_cell particles (coarse dimension 3C_s <= 16384
+ * per system); with Q the orthonormal cell columns and Q^T M Q = L L^T (block
+ * diagonal over the systems), Lanczos runs on G M G^T with
+ * G = D (I + Q (L^{-1} - I) Q^T) (D the learned blocks when a network is
+ * attached, else I) and the result is G^{-1} (G M G^T)^{1/2} W.  0 switches
+ * it off.  Built on the first sqrtMdotW after each setPositions. */
 int lm_nbody_set_coarse_level(lm_nbody* h, int particles_per_cell);
-/* Cells of the last coarse build: *ncells, grid[3] (cells per axis before
- * empty ones are dropped) and, if cell_of != NULL, the cell of each particle
- * (N ints). */
+/* Cells of the last coarse build, over all systems: *ncells, grid[3] (cells
+ * per axis of the first system, before empty ones are dropped) and, if
+ * cell_of != NULL, the cell of each particle (N ints). */
 int lm_nbody_coarse_cells(lm_nbody* h, int* ncells, int* grid, int* cell_of);
-/* L and L^{-1} of the last coarse build, 3C x 3C row-major doubles each. */
-int lm_nbody_coarse_factors(lm_nbody* h, double* L, double* Linv);
+/* L and L^{-1} of one system's coarse block, 3C_s x 3C_s row-major doubles
+ * each (C_s from lm_nbody_coarse_system_cells). */
+int lm_nbody_coarse_factors(lm_nbody* h, int system, double* L, double* Linv);
+int lm_nbody_coarse_system_cells(lm_nbody* h, int system, int* ncells);
 /* Wall-clock milliseconds of the last coarse build (cells, Q^T M Q, factors). */
 int lm_nbody_coarse_build_ms(lm_nbody* h, double* ms);
 /* Number of coarse builds so far on this handle. */

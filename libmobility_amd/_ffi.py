@@ -81,7 +81,8 @@ SIGNATURES = {
     "lm_nbody_precond_blocks": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_set_coarse_level": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lm_nbody_coarse_cells": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
-    "lm_nbody_coarse_factors": (ctypes.c_int, [_vp, _vp, _vp]),
+    "lm_nbody_coarse_factors": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+    "lm_nbody_coarse_system_cells": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "lm_nbody_coarse_build_ms": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_coarse_builds": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_set_preconditioner_refresh": (ctypes.c_int, [_vp, ctypes.c_int]),

@@ -314,10 +314,17 @@ int lm_nbody_coarse_cells(lm_nbody* h, int* ncells, int* grid, int* cell_of) {
   });
 }
 
-int lm_nbody_coarse_factors(lm_nbody* h, double* L, double* Linv) {
+int lm_nbody_coarse_factors(lm_nbody* h, int system, double* L, double* Linv) {
   return guarded([&] {
     if (!L || !Linv) throw lm::Error("[libmobility_amd] null output");
-    S(h).coarseFactors(L, Linv);
+    S(h).coarseFactors(system, L, Linv);
+  });
+}
+
+int lm_nbody_coarse_system_cells(lm_nbody* h, int system, int* ncells) {
+  return guarded([&] {
+    if (!ncells) throw lm::Error("[libmobility_amd] null output");
+    *ncells = S(h).coarseSystemCells(system);
   });
 }
 

@@ -22,23 +22,39 @@ namespace lm {
 
 constexpr int kCoarseTile = 64;
 
+// One system's block of the coarse level: its cells [c0, c0 + C) of the
+// global cell list, and its n x n matrices (n = 3C rounded up to a multiple
+// of kCoarseTile, rows and columns past 3C are padding with unit diagonal)
+// at element moff of the packed buffers, its diagonal-tile inverses at doff.
+struct CoarseSys {
+  int64_t moff, doff;
+  int c0, C, n, wall;
+};
+
 void launch_coarse_gather(const float4* rec, const int* perm, int64_t N, float4* out,
                           hipStream_t s);
-// Mc (n x n, leading dimension ld, zero-filled here) from cell-sorted FP32
-// records; rows and columns [n3c, n) are padding with unit diagonal.
-void launch_coarse_matrix(const float4* rec, const int* start, const double* scale, int C,
-                          bool wall, double* Mc, int64_t ld, int n3c, int n, hipStream_t s);
-// In place A -> L (lower; tiles above the diagonal keep stale values), X = L^{-1},
-// Xt = X^T, Dinv = inverses of the diagonal tiles; info[0] = 1 if A is not
-// positive definite.  n is a multiple of kCoarseTile.
+// Mc (all systems' blocks, `total` elements, zero-filled here) from
+// cell-sorted FP32 records; pair_off[s] = cell pairs (c >= d) of the systems
+// before s, npairs = all of them.
+void launch_coarse_matrix(const float4* rec, const int* start, const double* scale,
+                          const CoarseSys* sys, const int64_t* pair_off, int nsys, int64_t npairs,
+                          double* Mc, int64_t total, hipStream_t s);
+// One matrix, one launch per step: in place A -> L (lower; tiles above the
+// diagonal keep stale values), X = L^{-1}, Xt = X^T, Dinv = inverses of the
+// diagonal tiles; info[0] = 1 if A is not positive definite.  n is a
+// multiple of kCoarseTile.
 void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* info, int64_t ld,
                           int n, hipStream_t s);
-// out = in + Q (T - I) Q^T in on entries [0, 3N), out = in on [3N, n);
-// T lower (m <= r) or upper (m >= r) triangular.  in may equal out.
-void launch_coarse_apply(const double* T, int64_t ld, int n3c, bool lower, const double* in,
-                         double* out, int64_t N, int64_t n, const int* perm, const int* start,
-                         const int* cell_of, const double* scale, int C, double* cvec,
-                         double* yvec, hipStream_t s);
+// The same for the systems list[0..count), one workgroup each (small blocks).
+void launch_coarse_factor_small(double* A, double* X, double* Xt, double* Dinv, int* info,
+                                const CoarseSys* sys, const int* list, int count, hipStream_t s);
+// out = in + Q (T - I) Q^T in on entries [0, 3N), out = in on [3N, n); T the
+// systems' lower (m <= r) or upper (m >= r) triangular blocks; row_sys[r] is
+// the system of coarse row r < rows.  in may equal out.
+void launch_coarse_apply(const double* T, const CoarseSys* sys, const int* row_sys, int rows,
+                         bool lower, const double* in, double* out, int64_t N, int64_t n,
+                         const int* perm, const int* start, const int* cell_of,
+                         const double* scale, int C, double* cvec, double* yvec, hipStream_t s);
 
 // Cell assignment of positions in units of a (x, y, z per particle, host):
 // g_k cells along axis k, cell edge ~ (volume / (N / ppc))^{1/3}; empty cells
@@ -58,17 +74,28 @@ class CoarseLevel {
   // Largest coarse dimension 3C; larger requests get bigger cells.
   static constexpr int kMaxDim = 16384;
 
-  // rec64 / rec32: the solver's position records of one system (N particles).
-  void build(const void* rec64, const void* rec32, int64_t N, bool wall, int ppc, hipStream_t s);
+  // Largest number of 64-tiles of a block factored by the one-workgroup path
+  // (used when at least kSmallBatch systems qualify).
+  static constexpr int kSmallTiles = 16, kSmallBatch = 8;
+
+  // rec64 / rec32: the solver's position records; off[0..nsys] the systems'
+  // particle offsets, wall[s] 1 for a single-wall system.  Every system gets
+  // its own cells and coarse block.
+  void build(const void* rec64, const void* rec32, const std::vector<int64_t>& off,
+             const std::vector<int>& wall, int ppc, hipStream_t s);
   // out = Gc in (kG), Gc^T in (kGt) or Gc^{-1} in (kGinv); vectors of length
   // n >= 3N whose entries past 3N pass through.  in may equal out.
   void apply(Op op, const double* in, double* out, int64_t n, hipStream_t s);
 
   int cells() const { return cells_.C; }
   int dim() const { return 3 * cells_.C; }
+  int systems() const { return (int)sys_.size(); }
+  // global cells (cell_of, perm, start over all systems; g of the first
+  // system with particles)
   const CoarseCells& host_cells() const { return cells_; }
-  // L and X = L^{-1} (3C x 3C row-major, host) of the last build
-  void factors(double* L, double* X, hipStream_t s) const;
+  // L and X = L^{-1} of system s's block (3C_s x 3C_s row-major, host)
+  void factors(int s, double* L, double* X, hipStream_t st) const;
+  int system_cells(int s) const { return sys_.at(s).C; }
 
   // what it was built for: the solver's layout and structure ids, radius and
   // cell target
@@ -79,9 +106,10 @@ class CoarseLevel {
 
  private:
   CoarseCells cells_;
-  int n_ = 0;  // padded dimension
+  std::vector<CoarseSys> sys_;
   int64_t N_ = 0;
   DevBuf perm_, start_, cell_of_, scale_, sorted_, A_, X_, Xt_, Dinv_, info_, cvec_, yvec_;
+  DevBuf sys_dev_, row_sys_, pair_off_, small_;
 };
 
 }  // namespace lm

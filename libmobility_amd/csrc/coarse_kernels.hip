@@ -1,8 +1,10 @@
 // Coarse level of the two-level sqrtMdotW preconditioner (no counterpart in the
-// reference): particles are grouped into spatial cells; Q (3N x 3C) holds one
-// orthonormal column per cell and direction, Q[3i+a, 3c+a] = 1/sqrt(n_c) for
-// particle i in cell c.  The kernels build Mc = Q^T M Q from the pair terms,
-// factor Mc = L L^T, form X = L^{-1}, and apply
+// reference): particles are grouped into spatial cells, separately for every
+// system of an ensemble; Q (3N x 3C) holds one orthonormal column per cell
+// and direction, Q[3i+a, 3c+a] = 1/sqrt(n_c) for particle i in cell c.  The
+// kernels build Mc = Q^T M Q from the pair terms (block-diagonal over the
+// systems, one dense block each), factor each block Mc_s = L_s L_s^T, form
+// X_s = L_s^{-1}, and apply
 //     out = in + Q (T - I) Q^T in,   T = X, X^T or L,
 // on the translational entries.  Everything is FP64 except the pair terms of
 // Mc (FP32 geometry and coefficients, FP32 sums per lane over one source cell,
@@ -43,24 +45,11 @@ __device__ inline void tri_index(int64_t p, int& c, int& d) {
   d = (int)(p - cc * (cc + 1) / 2);
 }
 
-// One wave per cell pair (c >= d): lanes take the targets of cell c, every
-// lane walks the sources of cell d, then a butterfly sums the lanes.  The
-// block and its transpose are written (Mc is symmetric because M_ji = M_ij^T).
+// s += sum over targets i in [i0, i1) (this lane's, stride 64) and sources
+// j in [j0, j1) of the 3x3 translational block M_ij / t0, row-major
 template <bool WALL>
-__global__ __launch_bounds__(256) void k_coarse_matrix(const float4* __restrict__ rec,
-                                                       const int* __restrict__ start,
-                                                       const double* __restrict__ scale,
-                                                       int64_t npairs, double* __restrict__ Mc,
-                                                       int64_t ld) {
-  const int lane = threadIdx.x & 63;
-  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (p >= npairs) return;  // uniform per wave
-  int c, d;
-  tri_index(p, c, d);
-  const int i0 = start[c], i1 = start[c + 1], j0 = start[d], j1 = start[d + 1];
-  float s[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) s[k] = 0.0f;
+__device__ inline void cell_pair_sum(const float4* __restrict__ rec, int i0, int i1, int j0,
+                                     int j1, int lane, float s[9]) {
   for (int ib = i0; ib < i1; ib += 64) {
     const int i = ib + lane;
     if (i >= i1) break;
@@ -99,6 +88,39 @@ __global__ __launch_bounds__(256) void k_coarse_matrix(const float4* __restrict_
     s[6] = s[2];
     s[7] = s[5];
   }
+}
+
+// One wave per cell pair (c >= d) of one system: lanes take the targets of
+// cell c, every lane walks the sources of cell d, then a butterfly sums the
+// lanes.  The block and its transpose go to the system's matrix (Mc is
+// symmetric because M_ji = M_ij^T).  pair_off[s] counts the cell pairs of the
+// systems before s.
+__global__ __launch_bounds__(256) void k_coarse_matrix(const float4* __restrict__ rec,
+                                                       const int* __restrict__ start,
+                                                       const double* __restrict__ scale,
+                                                       const CoarseSys* __restrict__ sys,
+                                                       const int64_t* __restrict__ pair_off,
+                                                       int nsys, int64_t npairs,
+                                                       double* __restrict__ Mc) {
+  const int lane = threadIdx.x & 63;
+  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= npairs) return;  // uniform per wave
+  int lo = 0, hi = nsys;  // the last system with pair_off <= p
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pair_off[mid] <= p) lo = mid; else hi = mid;
+  }
+  const CoarseSys S = sys[lo];
+  int c, d;
+  tri_index(p - pair_off[lo], c, d);
+  const int gc = S.c0 + c, gd = S.c0 + d;
+  float s[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) s[k] = 0.0f;
+  if (S.wall)
+    cell_pair_sum<true>(rec, start[gc], start[gc + 1], start[gd], start[gd + 1], lane, s);
+  else
+    cell_pair_sum<false>(rec, start[gc], start[gc + 1], start[gd], start[gd + 1], lane, s);
   double v[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
@@ -107,21 +129,23 @@ __global__ __launch_bounds__(256) void k_coarse_matrix(const float4* __restrict_
     for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
   }
   if (lane == 0) {
-    const double f = scale[c] * scale[d];
+    const double f = scale[gc] * scale[gd];
+    double* M = Mc + S.moff;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
-        Mc[(int64_t)(3 * c + a) * ld + 3 * d + b] = f * v[3 * a + b];
-        if (c != d) Mc[(int64_t)(3 * d + b) * ld + 3 * c + a] = f * v[3 * a + b];
+        M[(int64_t)(3 * c + a) * S.n + 3 * d + b] = f * v[3 * a + b];
+        if (c != d) M[(int64_t)(3 * d + b) * S.n + 3 * c + a] = f * v[3 * a + b];
       }
   }
 }
 
-// Mc[r][r] = 1 for the padding rows r in [n3c, n)
-__global__ void k_pad_identity(double* __restrict__ A, int64_t ld, int n3c, int n) {
-  const int r = n3c + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (r < n) A[(int64_t)r * ld + r] = 1.0;
+// unit diagonal on each system's padding rows [3C, n)
+__global__ void k_pad_identity(double* __restrict__ A, const CoarseSys* __restrict__ sys) {
+  const CoarseSys S = sys[blockIdx.x];
+  for (int r = 3 * S.C + (int)threadIdx.x; r < S.n; r += blockDim.x)
+    A[S.moff + (int64_t)r * S.n + r] = 1.0;
 }
 
 // ---- dense 64 x 64 tiles in LDS ------------------------------------------------
@@ -140,11 +164,11 @@ __device__ inline void tile_zero(d4 acc[2][2]) {
 }
 
 // acc += A * (TB ? B^T : B) for 64 x 64 tiles in LDS on the FP64 matrix cores
-// (256 threads): wave w computes the 32 x 32
-// quadrant (w >> 1, w & 1) as 2 x 2 tiles of v_mfma_f64_16x16x4f64.  Operand
-// lanes: A[i][k] from lane 16 k + i, B[k][j] from lane 16 k + j; result
-// register r of lane l holds row 4 r + (l >> 4) of column l & 15 (layout
-// measured on gfx950).  Every index stays in the 64 x 64 tile.
+// (256 threads): wave w computes the 32 x 32 quadrant (w >> 1, w & 1) as
+// 2 x 2 tiles of v_mfma_f64_16x16x4f64.  Operand lanes: A[i][k] from lane
+// 16 k + i, B[k][j] from lane 16 k + j; result register r of lane l holds
+// row 4 r + (l >> 4) of column l & 15 (layout measured on gfx950).  Every
+// index stays in the 64 x 64 tile.
 template <bool TB>
 __device__ inline void tile_mm_mfma(const double (*A)[kP], const double (*B)[kP], d4 acc[2][2]) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -179,7 +203,14 @@ __device__ inline void tile_store_mfma(double* __restrict__ out, int64_t ld, con
       }
 }
 
-// ---- Cholesky Mc = L L^T (lower, right-looking, 64 x 64 tiles) -----------------
+// ---- tile operations of the factorisation (256 threads each) -------------------
+// Every operation starts with a workgroup barrier, so that one workgroup can
+// run a sequence of them on the same LDS tiles and matrices.  The Cholesky
+// is right-looking, blocked by kKB tile columns: inside a block [kb, kend)
+// the step k updates only the block's own columns, and the tiles right of
+// the block receive the block's kKB products in one read-modify-write.
+// X = L^{-1} is the block forward substitution on L X = I, blocked the same
+// way.
 
 // value of v in lane `lane` (uniform) of the wave
 __device__ inline double lane_value(double v, int lane) {
@@ -190,16 +221,16 @@ __device__ inline double lane_value(double v, int lane) {
 }
 
 // Factor the diagonal tile k, write L_kk (upper part zeroed) and its inverse
-// Dinv_k.  One wave: lane r holds row r in registers; column j of the
-// right-looking factorisation reaches the other lanes by readlane, so no
-// barrier is needed.  The inverse: lane c forms column c of L_kk^{-1} by
-// forward substitution, L_kk rows read from LDS (same address in every lane).
-// info[0] = 1 when a pivot is not positive.
-__global__ __launch_bounds__(64) void k_chol_diag(double* __restrict__ A, int64_t ld, int k,
-                                                  double* __restrict__ Dinv,
-                                                  int* __restrict__ info) {
-  __shared__ double Ls[kT][kP];
-  const int r = threadIdx.x;
+// Dinv_k.  One wave (lanes 0..63 of the caller), registers only: lane r holds
+// row r, and column j of the right-looking factorisation reaches the other
+// lanes by readlane.  The inverse is right-looking too: at step m row m of
+// L_kk^{-1} is final (lane m scales it by 1 / L_mm) and reaches the rows below
+// by readlane.  Entry c of a row of the inverse comes to life at step c, when
+// column c of L dies, so about 65 doubles per lane are live.  info[0] = 1
+// when a pivot is not positive.
+__device__ __attribute__((always_inline)) inline void chol_diag_wave(double* __restrict__ A, int64_t ld, int k,
+                                      double* __restrict__ Dinv, int* __restrict__ info) {
+  const int r = threadIdx.x & 63;
   double* Akk = A + (int64_t)k * kT * ld + (int64_t)k * kT;
   double a[kT];
 #pragma unroll
@@ -221,117 +252,153 @@ __global__ __launch_bounds__(64) void k_chol_diag(double* __restrict__ A, int64_
   }
 #pragma unroll
   for (int c = 0; c < kT; ++c) {
-    const double v = c <= r ? a[c] : 0.0;
-    Ls[r][c] = v;
-    Akk[(int64_t)r * ld + c] = v;
+    a[c] = c <= r ? a[c] : 0.0;
+    Akk[(int64_t)r * ld + c] = a[c];
   }
   if (bad && r == 0) info[0] = 1;
-  __syncthreads();
-  double x[kT];
+  double x[kT];  // row r of L_kk^{-1}
 #pragma unroll
-  for (int i = 0; i < kT; ++i) {
-    double s[4] = {i == r ? 1.0 : 0.0, 0.0, 0.0, 0.0};
+  for (int m = 0; m < kT; ++m) {
+    x[m] = r == m ? 1.0 : 0.0;
+    const double ilmm = 1.0 / lane_value(a[m], m);
 #pragma unroll
-    for (int m = 0; m < i; ++m) s[m & 3] = __builtin_fma(-Ls[i][m], x[m], s[m & 3]);
-    x[i] = ((s[0] + s[1]) + (s[2] + s[3])) * (1.0 / Ls[i][i]);
+    for (int c = 0; c <= m; ++c) x[c] = r == m ? x[c] * ilmm : x[c];
+    // rows r > m: X_rc -= L_rm X_mc, c <= m
+#pragma unroll
+    for (int c = 0; c <= m; ++c) {
+      const double xmc = lane_value(x[c], m);
+      x[c] = r > m ? __builtin_fma(-a[m], xmc, x[c]) : x[c];
+    }
   }
   double* D = Dinv + (int64_t)k * kT * kT;
 #pragma unroll
-  for (int i = 0; i < kT; ++i) D[i * kT + r] = x[i];
+  for (int c = 0; c < kT; ++c) D[r * kT + c] = x[c];
 }
 
-// L_ik = A_ik L_kk^{-T} for the tiles below the diagonal tile k
-__global__ __launch_bounds__(256) void k_chol_panel(double* __restrict__ A, int64_t ld, int k,
-                                                    const double* __restrict__ Dinv) {
-  __shared__ double P[kT][kP];
-  __shared__ double Dk[kT][kP];
-  const int i = k + 1 + blockIdx.x;
+// L_ik = A_ik L_kk^{-T}
+__device__ inline void op_panel(double* __restrict__ A, int64_t ld, int k, int i,
+                                const double* __restrict__ Dinv, double (*P0)[kP],
+                                double (*P1)[kP]) {
+  __syncthreads();
   double* Aik = A + (int64_t)i * kT * ld + (int64_t)k * kT;
-  tile_load(P, Aik, ld);
-  tile_load(Dk, Dinv + (int64_t)k * kT * kT, kT);
+  tile_load(P0, Aik, ld);
+  tile_load(P1, Dinv + (int64_t)k * kT * kT, kT);
   __syncthreads();
   d4 acc[2][2];
   tile_zero(acc);
-  tile_mm_mfma<true>(P, Dk, acc);
+  tile_mm_mfma<true>(P0, P1, acc);
   tile_store_mfma<false>(Aik, ld, acc);
 }
 
-// The updates are blocked by kKB tile columns: inside a block of columns
-// [kb, kend) the step k updates only the block's own columns, and the tiles
-// right of the block receive the block's kKB products in one read-modify-write.
+// A_ij -= sum_{k in [k0, k1)} L_ik L_jk^T
+__device__ inline void op_chol_update(double* __restrict__ A, int64_t ld, int i, int j, int k0,
+                                      int k1, double (*P0)[kP], double (*P1)[kP]) {
+  d4 acc[2][2];
+  tile_zero(acc);
+  for (int k = k0; k < k1; ++k) {
+    __syncthreads();
+    tile_load(P0, A + (int64_t)i * kT * ld + (int64_t)k * kT, ld);
+    tile_load(P1, A + (int64_t)j * kT * ld + (int64_t)k * kT, ld);
+    __syncthreads();
+    tile_mm_mfma<true>(P0, P1, acc);
+  }
+  tile_store_mfma<true>(A + (int64_t)i * kT * ld + (int64_t)j * kT, ld, acc);
+}
+
+// X_km = Dinv_k X_km
+__device__ inline void op_trinv_scale(double* __restrict__ X, int64_t ld, int k, int m,
+                                      const double* __restrict__ Dinv, double (*P0)[kP],
+                                      double (*P1)[kP]) {
+  __syncthreads();
+  double* Xkm = X + (int64_t)k * kT * ld + (int64_t)m * kT;
+  tile_load(P0, Xkm, ld);
+  tile_load(P1, Dinv + (int64_t)k * kT * kT, kT);
+  __syncthreads();
+  d4 acc[2][2];
+  tile_zero(acc);
+  tile_mm_mfma<false>(P1, P0, acc);
+  tile_store_mfma<false>(Xkm, ld, acc);
+}
+
+// X_im -= sum_{k in [k0, k1)} L_ik X_km
+__device__ inline void op_trinv_update(double* __restrict__ X, const double* __restrict__ L,
+                                       int64_t ld, int i, int m, int k0, int k1,
+                                       double (*P0)[kP], double (*P1)[kP]) {
+  d4 acc[2][2];
+  tile_zero(acc);
+  for (int k = k0; k < k1; ++k) {
+    __syncthreads();
+    tile_load(P0, L + (int64_t)i * kT * ld + (int64_t)k * kT, ld);
+    tile_load(P1, X + (int64_t)k * kT * ld + (int64_t)m * kT, ld);
+    __syncthreads();
+    tile_mm_mfma<false>(P0, P1, acc);
+  }
+  tile_store_mfma<true>(X + (int64_t)i * kT * ld + (int64_t)m * kT, ld, acc);
+}
+
+// Xt tile (bj, bi) = (X tile (bi, bj))^T
+__device__ inline void op_transpose(const double* __restrict__ X, double* __restrict__ Xt,
+                                    int64_t ld, int bi, int bj, double (*P0)[kP]) {
+  __syncthreads();
+  tile_load(P0, X + (int64_t)bi * kT * ld + (int64_t)bj * kT, ld);
+  __syncthreads();
+  double* o = Xt + (int64_t)bj * kT * ld + (int64_t)bi * kT;
+  for (int e = threadIdx.x; e < kT * kT; e += blockDim.x) o[(e >> 6) * ld + (e & 63)] = P0[e & 63][e >> 6];
+}
+
+// ---- one matrix, one launch per step ---------------------------------------------
+
+__global__ __launch_bounds__(64) void k_chol_diag(double* __restrict__ A, int64_t ld, int k,
+                                                  double* __restrict__ Dinv,
+                                                  int* __restrict__ info) {
+  chol_diag_wave(A, ld, k, Dinv, info);
+}
+
+__global__ __launch_bounds__(256) void k_chol_panel(double* __restrict__ A, int64_t ld, int k,
+                                                    const double* __restrict__ Dinv) {
+  __shared__ double P0[kT][kP];
+  __shared__ double P1[kT][kP];
+  op_panel(A, ld, k, k + 1 + blockIdx.x, Dinv, P0, P1);
+}
 
 // A_ij -= L_ik L_jk^T for the block's columns k < j < kend, rows j <= i < nb
 __global__ __launch_bounds__(256) void k_chol_local(double* __restrict__ A, int64_t ld, int k,
                                                     int kend) {
-  __shared__ double Pi[kT][kP];
-  __shared__ double Pj[kT][kP];
+  __shared__ double P0[kT][kP];
+  __shared__ double P1[kT][kP];
   const int jcount = kend - k - 1;
   const int j = k + 1 + (int)(blockIdx.x % jcount);
   const int i = k + 1 + (int)(blockIdx.x / jcount);
   if (i < j) return;  // uniform per workgroup
-  tile_load(Pi, A + (int64_t)i * kT * ld + (int64_t)k * kT, ld);
-  tile_load(Pj, A + (int64_t)j * kT * ld + (int64_t)k * kT, ld);
-  __syncthreads();
-  d4 acc[2][2];
-  tile_zero(acc);
-  tile_mm_mfma<true>(Pi, Pj, acc);
-  tile_store_mfma<true>(A + (int64_t)i * kT * ld + (int64_t)j * kT, ld, acc);
+  op_chol_update(A, ld, i, j, k, k + 1, P0, P1);
 }
 
 // A_ij -= sum_{k in [kb, kend)} L_ik L_jk^T for kend <= j <= i < nb
 __global__ __launch_bounds__(256) void k_chol_trailing(double* __restrict__ A, int64_t ld, int kb,
                                                        int kend) {
-  __shared__ double Pi[kT][kP];
-  __shared__ double Pj[kT][kP];
+  __shared__ double P0[kT][kP];
+  __shared__ double P1[kT][kP];
   int ii, jj;
   tri_index(blockIdx.x, ii, jj);
-  const int i = kend + ii, j = kend + jj;
-  d4 acc[2][2];
-  tile_zero(acc);
-  for (int k = kb; k < kend; ++k) {
-    if (k > kb) __syncthreads();  // the previous products have read the tiles
-    tile_load(Pi, A + (int64_t)i * kT * ld + (int64_t)k * kT, ld);
-    tile_load(Pj, A + (int64_t)j * kT * ld + (int64_t)k * kT, ld);
-    __syncthreads();
-    tile_mm_mfma<true>(Pi, Pj, acc);
-  }
-  tile_store_mfma<true>(A + (int64_t)i * kT * ld + (int64_t)j * kT, ld, acc);
+  op_chol_update(A, ld, kend + ii, kend + jj, kb, kend, P0, P1);
 }
-
-// ---- X = L^{-1}: block forward substitution on L X = I ----------------------------
-// Step k: X_km <- Dinv_k X_km (m <= k), then X_im -= L_ik X_km (i > k, m <= k).
-// X starts as the identity.
 
 __global__ __launch_bounds__(256) void k_trinv_scale(double* __restrict__ X, int64_t ld, int k,
                                                      const double* __restrict__ Dinv) {
-  __shared__ double R[kT][kP];
-  __shared__ double Dk[kT][kP];
-  const int m = blockIdx.x;
-  double* Xkm = X + (int64_t)k * kT * ld + (int64_t)m * kT;
-  tile_load(R, Xkm, ld);
-  tile_load(Dk, Dinv + (int64_t)k * kT * kT, kT);
-  __syncthreads();
-  d4 acc[2][2];
-  tile_zero(acc);
-  tile_mm_mfma<false>(Dk, R, acc);
-  tile_store_mfma<false>(Xkm, ld, acc);
+  __shared__ double P0[kT][kP];
+  __shared__ double P1[kT][kP];
+  op_trinv_scale(X, ld, k, blockIdx.x, Dinv, P0, P1);
 }
 
 // X_im -= L_ik X_km for the block's rows k < i < kend, m <= k
 __global__ __launch_bounds__(256) void k_trinv_local(double* __restrict__ X,
                                                      const double* __restrict__ L, int64_t ld,
                                                      int k) {
-  __shared__ double Lik[kT][kP];
-  __shared__ double R[kT][kP];
+  __shared__ double P0[kT][kP];
+  __shared__ double P1[kT][kP];
   const int m = blockIdx.x % (k + 1);
   const int i = k + 1 + blockIdx.x / (k + 1);
-  tile_load(Lik, L + (int64_t)i * kT * ld + (int64_t)k * kT, ld);
-  tile_load(R, X + (int64_t)k * kT * ld + (int64_t)m * kT, ld);
-  __syncthreads();
-  d4 acc[2][2];
-  tile_zero(acc);
-  tile_mm_mfma<false>(Lik, R, acc);
-  tile_store_mfma<true>(X + (int64_t)i * kT * ld + (int64_t)m * kT, ld, acc);
+  op_trinv_update(X, L, ld, i, m, k, k + 1, P0, P1);
 }
 
 // X_im -= sum_{k in [max(kb, m), kend)} L_ik X_km for kend <= i < nb, m < kend
@@ -339,21 +406,11 @@ __global__ __launch_bounds__(256) void k_trinv_local(double* __restrict__ X,
 __global__ __launch_bounds__(256) void k_trinv_trailing(double* __restrict__ X,
                                                         const double* __restrict__ L, int64_t ld,
                                                         int kb, int kend) {
-  __shared__ double Lik[kT][kP];
-  __shared__ double R[kT][kP];
+  __shared__ double P0[kT][kP];
+  __shared__ double P1[kT][kP];
   const int m = blockIdx.x % kend;
   const int i = kend + blockIdx.x / kend;
-  d4 acc[2][2];
-  tile_zero(acc);
-  const int k0 = m > kb ? m : kb;
-  for (int k = k0; k < kend; ++k) {
-    if (k > k0) __syncthreads();
-    tile_load(Lik, L + (int64_t)i * kT * ld + (int64_t)k * kT, ld);
-    tile_load(R, X + (int64_t)k * kT * ld + (int64_t)m * kT, ld);
-    __syncthreads();
-    tile_mm_mfma<false>(Lik, R, acc);
-  }
-  tile_store_mfma<true>(X + (int64_t)i * kT * ld + (int64_t)m * kT, ld, acc);
+  op_trinv_update(X, L, ld, i, m, m > kb ? m : kb, kend, P0, P1);
 }
 
 __global__ void k_identity(double* __restrict__ X, int64_t ld, int n) {
@@ -365,12 +422,54 @@ __global__ void k_identity(double* __restrict__ X, int64_t ld, int n) {
 
 __global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ X,
                                                    double* __restrict__ Xt, int64_t ld) {
-  __shared__ double S[kT][kP];
-  const int bi = blockIdx.y, bj = blockIdx.x;
-  tile_load(S, X + (int64_t)bi * kT * ld + (int64_t)bj * kT, ld);
+  __shared__ double P0[kT][kP];
+  op_transpose(X, Xt, ld, blockIdx.y, blockIdx.x, P0);
+}
+
+// ---- many small matrices: one workgroup per system, the same schedule -----------
+
+__global__ __launch_bounds__(256) void k_factor_small(double* __restrict__ A,
+                                                      double* __restrict__ X,
+                                                      double* __restrict__ Xt,
+                                                      double* __restrict__ Dinv,
+                                                      int* __restrict__ info,
+                                                      const CoarseSys* __restrict__ sys,
+                                                      const int* __restrict__ list) {
+  __shared__ double P0[kT][kP];
+  __shared__ double P1[kT][kP];
+  const CoarseSys S = sys[list[blockIdx.x]];
+  const int n = S.n, nb = n / kT;
+  double* a = A + S.moff;
+  double* x = X + S.moff;
+  double* xt = Xt + S.moff;
+  double* dinv = Dinv + S.doff;
+  for (int kb = 0; kb < nb; kb += kKB) {
+    const int kend = kb + kKB < nb ? kb + kKB : nb;
+    for (int k = kb; k < kend; ++k) {
+      __syncthreads();
+      if (threadIdx.x < 64) chol_diag_wave(a, n, k, dinv, info);
+      for (int i = k + 1; i < nb; ++i) op_panel(a, n, k, i, dinv, P0, P1);
+      for (int j = k + 1; j < kend; ++j)
+        for (int i = j; i < nb; ++i) op_chol_update(a, n, i, j, k, k + 1, P0, P1);
+    }
+    for (int j = kend; j < nb; ++j)
+      for (int i = j; i < nb; ++i) op_chol_update(a, n, i, j, kb, kend, P0, P1);
+  }
   __syncthreads();
-  double* o = Xt + (int64_t)bj * kT * ld + (int64_t)bi * kT;
-  for (int e = threadIdx.x; e < kT * kT; e += 256) o[(e >> 6) * ld + (e & 63)] = S[e & 63][e >> 6];
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += blockDim.x)
+    x[e] = (e / n) == (e % n) ? 1.0 : 0.0;
+  for (int kb = 0; kb < nb; kb += kKB) {
+    const int kend = kb + kKB < nb ? kb + kKB : nb;
+    for (int k = kb; k < kend; ++k) {
+      for (int m = 0; m <= k; ++m) op_trinv_scale(x, n, k, m, dinv, P0, P1);
+      for (int i = k + 1; i < kend; ++i)
+        for (int m = 0; m <= k; ++m) op_trinv_update(x, a, n, i, m, k, k + 1, P0, P1);
+    }
+    for (int i = kend; i < nb; ++i)
+      for (int m = 0; m < kend; ++m) op_trinv_update(x, a, n, i, m, m > kb ? m : kb, kend, P0, P1);
+  }
+  for (int bi = 0; bi < nb; ++bi)
+    for (int bj = 0; bj < nb; ++bj) op_transpose(x, xt, n, bi, bj, P0);
 }
 
 // ---- per-application kernels ------------------------------------------------------
@@ -405,18 +504,23 @@ __global__ __launch_bounds__(256) void k_coarse_restrict(const double* __restric
   }
 }
 
-// y[r] = sum_m T[r][m] x[m] - x[r] over m <= r (lower) or m >= r (upper), m < n3c;
-// one workgroup per row, fixed-order reduction.
-__global__ __launch_bounds__(256) void k_coarse_gemv(const double* __restrict__ T, int64_t ld,
-                                                     int n3c, int lower,
+// y[r] = sum_m T_s[lr][m] x[r0 + m] - x[r] over the system's rows, m <= lr
+// (lower) or m >= lr (upper), m < 3 C_s; r = r0 + lr with r0 = 3 c0 of the
+// row's system s; one workgroup per row, fixed-order reduction.
+__global__ __launch_bounds__(256) void k_coarse_gemv(const double* __restrict__ T,
+                                                     const CoarseSys* __restrict__ sys,
+                                                     const int* __restrict__ row_sys, int lower,
                                                      const double* __restrict__ x,
                                                      double* __restrict__ y) {
   __shared__ double red[256];
   const int r = blockIdx.x;
-  const int m0 = lower ? 0 : r, m1 = lower ? r + 1 : n3c;
-  const double* row = T + (int64_t)r * ld;
+  const CoarseSys S = sys[row_sys[r]];
+  const int r0 = 3 * S.c0, lr = r - r0;
+  const int m0 = lower ? 0 : lr, m1 = lower ? lr + 1 : 3 * S.C;
+  const double* row = T + S.moff + (int64_t)lr * S.n;
+  const double* xs = x + r0;
   double s = 0.0;
-  for (int m = m0 + (int)threadIdx.x; m < m1; m += 256) s = __builtin_fma(row[m], x[m], s);
+  for (int m = m0 + (int)threadIdx.x; m < m1; m += 256) s = __builtin_fma(row[m], xs[m], s);
   red[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -427,8 +531,7 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(const double* __restrict__ 
 }
 
 // out[k] = in[k] + scale[c] * y[3c + a] for k = 3i + a < 3N (c = cell of i),
-// out[k] = in[k] beyond
-// (in may equal out)
+// out[k] = in[k] beyond (in may equal out)
 __global__ void k_coarse_prolong(const double* in, const int* __restrict__ cell_of,
                                  const double* __restrict__ scale, const double* __restrict__ y,
                                  int64_t n3, int64_t n, double* out) {
@@ -456,23 +559,17 @@ void launch_coarse_gather(const float4* rec, const int* perm, int64_t N, float4*
   LM_HIP(hipGetLastError());
 }
 
-void launch_coarse_matrix(const float4* rec, const int* start, const double* scale, int C,
-                          bool wall, double* Mc, int64_t ld, int n3c, int n, hipStream_t s) {
-  launch_zero(Mc, (int64_t)n * ld, s);
-  const int64_t npairs = (int64_t)C * (C + 1) / 2;
-  const dim3 grid(blocks_for(npairs, 4));
-  if (wall)
-    hipLaunchKernelGGL(k_coarse_matrix<true>, grid, dim3(256), 0, s, rec, start, scale, npairs, Mc,
-                       ld);
-  else
-    hipLaunchKernelGGL(k_coarse_matrix<false>, grid, dim3(256), 0, s, rec, start, scale, npairs,
-                       Mc, ld);
-  LM_HIP(hipGetLastError());
-  if (n > n3c) {
-    hipLaunchKernelGGL(k_pad_identity, dim3(blocks_for(n - n3c, 64)), dim3(64), 0, s, Mc, ld, n3c,
-                       n);
+void launch_coarse_matrix(const float4* rec, const int* start, const double* scale,
+                          const CoarseSys* sys, const int64_t* pair_off, int nsys, int64_t npairs,
+                          double* Mc, int64_t total, hipStream_t s) {
+  launch_zero(Mc, total, s);
+  if (npairs > 0) {
+    hipLaunchKernelGGL(k_coarse_matrix, dim3(blocks_for(npairs, 4)), dim3(256), 0, s, rec, start,
+                       scale, sys, pair_off, nsys, npairs, Mc);
     LM_HIP(hipGetLastError());
   }
+  hipLaunchKernelGGL(k_pad_identity, dim3(nsys), dim3(64), 0, s, Mc, sys);
+  LM_HIP(hipGetLastError());
 }
 
 void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* info, int64_t ld,
@@ -513,14 +610,21 @@ void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* i
   LM_HIP(hipGetLastError());
 }
 
-void launch_coarse_apply(const double* T, int64_t ld, int n3c, bool lower, const double* in,
-                         double* out, int64_t N, int64_t n, const int* perm, const int* start,
-                         const int* cell_of, const double* scale, int C, double* cvec,
-                         double* yvec, hipStream_t s) {
+void launch_coarse_factor_small(double* A, double* X, double* Xt, double* Dinv, int* info,
+                                const CoarseSys* sys, const int* list, int count, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_factor_small, dim3(count), dim3(256), 0, s, A, X, Xt, Dinv, info, sys, list);
+  LM_HIP(hipGetLastError());
+}
+
+void launch_coarse_apply(const double* T, const CoarseSys* sys, const int* row_sys, int rows,
+                         bool lower, const double* in, double* out, int64_t N, int64_t n,
+                         const int* perm, const int* start, const int* cell_of,
+                         const double* scale, int C, double* cvec, double* yvec, hipStream_t s) {
   hipLaunchKernelGGL(k_coarse_restrict, dim3(blocks_for(C, 4)), dim3(256), 0, s, in, perm, start,
                      scale, C, cvec);
-  hipLaunchKernelGGL(k_coarse_gemv, dim3(n3c), dim3(256), 0, s, T, ld, n3c, lower ? 1 : 0, cvec,
-                     yvec);
+  hipLaunchKernelGGL(k_coarse_gemv, dim3(rows), dim3(256), 0, s, T, sys, row_sys, lower ? 1 : 0,
+                     cvec, yvec);
   hipLaunchKernelGGL(k_coarse_prolong, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, cell_of,
                      scale, yvec, 3 * N, n, out);
   LM_HIP(hipGetLastError());

@@ -1521,14 +1521,15 @@ void NBodySolver::setCoarseLevel(int particles_per_cell) {
 
 void NBodySolver::ensure_coarse() {
   if (coarse_ppc_ <= 0) return;
-  if (numSystems() != 1)
-    throw Error("[libmobility_amd] the coarse level needs a single system (setSystems with one system or none)");
   ensure_records();
   if (coarse_.ppc == coarse_ppc_ && coarse_.cells() > 0 &&
       pre_current(coarse_.layout, coarse_.structure, coarse_.a))
     return;
-  const bool wall = sys_kind_host_.empty() ? wall_ : sys_kind_host_[0] != 0;
-  coarse_.build(rec64_.as<void>(), rec32_.as<void>(), N_, wall, coarse_ppc_, stream_);
+  const int nsys = numSystems();
+  std::vector<int> wall(nsys);
+  for (int q = 0; q < nsys; ++q)
+    wall[q] = q < (int)sys_kind_host_.size() ? (sys_kind_host_[q] != 0) : (wall_ ? 1 : 0);
+  coarse_.build(rec64_.as<void>(), rec32_.as<void>(), sys_off_host_, wall, coarse_ppc_, stream_);
   coarse_.layout = layout_id_;
   coarse_.structure = struct_id_;
   coarse_.a = a_;
@@ -1580,7 +1581,16 @@ int NBodySolver::coarseCells(int* cell_of, int* grid) const {
   return c.C;
 }
 
-void NBodySolver::coarseFactors(double* L, double* X) const { coarse_.factors(L, X, stream_); }
+void NBodySolver::coarseFactors(int system, double* L, double* X) const {
+  coarse_.factors(system, L, X, stream_);
+}
+
+int NBodySolver::coarseSystemCells(int system) const {
+  if (coarse_.cells() == 0) throw Error("[libmobility_amd] no coarse level has been built");
+  if (system < 0 || system >= coarse_.systems())
+    throw Error("[libmobility_amd] coarse level: no such system");
+  return coarse_.system_cells(system);
+}
 
 void NBodySolver::precondDataset(int64_t ncenters, uint64_t seed, float* X, float* T9) {
   ensure_knn();

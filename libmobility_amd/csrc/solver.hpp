@@ -108,7 +108,7 @@ class NBodySolver {
   void precondDataset(int64_t ncenters, uint64_t seed, float* X, float* T9);
   // The 3x3 blocks G (9N, host) the attached network gives for the positions.
   void precondBlocks(double* G9);
-  // Coarse level of the sqrtMdotW preconditioner (one system): cells of about
+  // Coarse level of the sqrtMdotW preconditioner (per system): cells of about
   // `particles_per_cell` particles, Lanczos on G M G^T with G = D Gc (D the
   // learned blocks when attached, Gc the coarse whitening), result
   // G^{-1} (G M G^T)^{1/2} W.  0 switches it off.
@@ -117,8 +117,9 @@ class NBodySolver {
   // Cells of the last coarse build (cell_of: N entries, host) and the grid;
   // returns the number of cells.
   int coarseCells(int* cell_of, int* grid) const;
-  // L and L^{-1} (3C x 3C each, host) of the last coarse build.
-  void coarseFactors(double* L, double* X) const;
+  // L and L^{-1} of one system's coarse block (3C_s x 3C_s each, host).
+  void coarseFactors(int system, double* L, double* X) const;
+  int coarseSystemCells(int system) const;
   double coarseBuildMs() const { return coarse_.build_ms; }
   int64_t coarseBuilds() const { return coarse_builds_; }
   // Keep the preconditioner (learned blocks and coarse level) for `interval`

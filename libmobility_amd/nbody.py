@@ -401,7 +401,8 @@ class NBody(_Solver):
         return G
 
     def setCoarseLevel(self, particlesPerCell=64):
-        """Extension: coarse level of the sqrtMdotW preconditioner (one system).
+        """Extension: coarse level of the sqrtMdotW preconditioner (every system
+        of an ensemble gets its own cells and coarse block).
 
         Particles are grouped into spatial cells of about ``particlesPerCell``
         particles.  With Q the orthonormal cell columns and ``Q^T M Q = L L^T``,
@@ -423,12 +424,15 @@ class NBody(_Solver):
                                                    grid.ctypes.data, cell_of.ctypes.data))
         return cell_of, grid, nc.value
 
-    def coarseFactors(self):
-        """Extension: (L, L^{-1}) of the last coarse build, (3C, 3C) each."""
-        _, _, nc = self.coarseCells()
-        L = np.zeros((3 * nc, 3 * nc))
-        X = np.zeros((3 * nc, 3 * nc))
-        _ffi.check(self._lib.lm_nbody_coarse_factors(self._h, L.ctypes.data, X.ctypes.data))
+    def coarseFactors(self, system=0):
+        """Extension: (L, L^{-1}) of one system's coarse block, (3C_s, 3C_s) each."""
+        nc = ctypes.c_int(0)
+        _ffi.check(self._lib.lm_nbody_coarse_system_cells(self._h, int(system),
+                                                          ctypes.addressof(nc)))
+        L = np.zeros((3 * nc.value, 3 * nc.value))
+        X = np.zeros((3 * nc.value, 3 * nc.value))
+        _ffi.check(self._lib.lm_nbody_coarse_factors(self._h, int(system), L.ctypes.data,
+                                                     X.ctypes.data))
         return L, X
 
     def coarseBuildMs(self):

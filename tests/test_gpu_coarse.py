@@ -12,7 +12,11 @@ tests/test_oracle_coarse.py).
     on the same W;
   * fewer Lanczos iterations than plain Lanczos on an RSA packing;
   * the multi-vector square root agrees with the single-vector one;
-  * errors: ensembles, negative cell targets, queries before a build.
+  * ensembles: per-system cells and factors (the one-workgroup batched path
+    and the per-system launches), B B^T = M over a mixed open / wall
+    ensemble, per-system square roots against the oracle;
+  * errors: negative cell targets, queries before a build or of a missing
+    system.
 """
 import numpy as np
 import pytest
@@ -154,13 +158,10 @@ def test_coarse_level_errors():
         s.setCoarseLevel(-1)
     with pytest.raises(RuntimeError, match="no coarse level"):
         s.coarseCells()
-    s.setSystems([100, 100])
-    s.setPositions(pos)
-    s.setCoarseLevel(8)
-    with pytest.raises(RuntimeError, match="single system"):
+    with pytest.raises(RuntimeError, match="no such system"):
+        s.setCoarseLevel(8)
         s.sqrtMdotW()
-    s.setCoarseLevel(0)
-    s.sqrtMdotW()
+        s.coarseFactors(5)
 
 
 @pytest.mark.parametrize("case", ["one", "plane", "fewer_than_a_cell", "wall_below_a"])
@@ -242,3 +243,140 @@ def test_refresh_interval_keeps_an_exact_square_root():
     s.setPositions(pos2)
     s.sqrtMdotW_with_noise(np.ones(3 * (N + 4)))
     assert s.coarseBuilds() == 3
+
+
+def test_two_level_fluctuation_dissipation():
+    """Samples of the two-level square root (learned blocks + coarse level,
+    fresh noise each call) have covariance M: whitened by the eigenvectors of
+    the oracle's dense M, the sample covariance is the identity within its
+    sampling error."""
+    from libmobility_amd.precond import LearnedPreconditioner
+    from libmobility_amd.workloads import rsa_cube
+    N = 20
+    pos, _ = rsa_cube(N, 0.2, seed=41)
+    s = _solver(pos, tol=1e-8, ppc=4)
+    s.initialize(viscosity=ETA, hydrodynamicRadius=1.0, tolerance=1e-8, seed=91)
+    s.setPositions(pos)
+    s.setCoarseLevel(4)
+    net = LearnedPreconditioner(seed=5)
+    s.setPreconditioner(net)
+    M = oracle.dense_matrix(pos, ETA, 1.0)
+    ns = 3000
+    X = np.array([s.sqrtMdotW()[0].ravel() for _ in range(ns)])
+    assert s.coarseCells()[2] > 1
+    lam, V = np.linalg.eigh(M)
+    Z = (X @ V) / np.sqrt(lam)
+    cov = np.cov(Z.T)
+    err = np.abs(cov - np.eye(3 * N)).max()
+    print(f"[coarse] FDT: max |cov - I| = {err:.3f} (bound {6.0 * np.sqrt(2.0 / ns):.3f})")
+    assert err < 6.0 * np.sqrt(2.0 / ns)
+
+
+def _ensemble(sizes, kinds, seed):
+    from libmobility_amd.workloads import rsa_cube
+    rng = np.random.default_rng(seed)
+    pos = []
+    for q, (sz, k) in enumerate(zip(sizes, kinds)):
+        if sz == 0:
+            continue
+        if k:
+            L = (sz / 0.05 / 5.0) ** 0.5
+            pos.append(np.column_stack([rng.uniform(0, L, sz), rng.uniform(0, L, sz),
+                                        rng.uniform(1.5, 6.5, sz)]))
+        else:
+            pos.append(rsa_cube(sz, 0.2, seed=seed * 100 + q)[0])
+    return np.concatenate(pos)
+
+
+def _ens_solver(sizes, kinds, pos, tol, ppc, per_system=False):
+    from libmobility_amd import NBody
+    s = NBody("open", "open", "single_wall")
+    s.setParameters(wallHeight=0.0)
+    s.initialize(viscosity=ETA, hydrodynamicRadius=1.0, tolerance=tol, seed=7)
+    s.setSystems(sizes, ["single_wall" if k else "open" for k in kinds])
+    s.setPositions(pos)
+    s.setCoarseLevel(ppc)
+    s.setLanczosPerSystem(per_system)
+    return s
+
+
+@pytest.mark.parametrize("path", ["batched", "per_system_launches"])
+def test_ensemble_cells_and_factors_match_the_oracle(path):
+    """Every system gets its own cells and coarse block: the cells are the
+    oracle's rule applied to the system alone, L_s L_s^T is the oracle's
+    Q_s^T M_s Q_s and X_s L_s = I.  Nine multi-tile blocks take the batched
+    one-workgroup path, two take the per-system launches."""
+    if path == "batched":
+        sizes = [70, 95, 0, 120, 60, 88, 101, 77, 64, 110]
+        kinds = [0, 1, 0, 1, 0, 0, 1, 0, 1, 0]
+        ppc = 2
+    else:
+        sizes = [300, 0, 220]
+        kinds = [1, 0, 0]
+        ppc = 2
+    pos = _ensemble(sizes, kinds, 3)
+    s = _ens_solver(sizes, kinds, pos, 1e-6, ppc)
+    s.sqrtMdotW_with_noise(np.random.default_rng(1).normal(size=pos.size))
+    cell_of, _, C = s.coarseCells()
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    c0 = 0
+    for q, (a, b) in enumerate(zip(off[:-1], off[1:])):
+        if a == b:
+            assert s.coarseFactors(q)[0].shape == (0, 0)
+            continue
+        ps = pos[a:b]
+        ref, _, Cq = coarse.coarse_cells(_records_xyz(ps, kinds[q]), ppc)
+        assert np.array_equal(cell_of[a:b], ref + c0), q
+        L, X = s.coarseFactors(q)
+        assert L.shape == (3 * Cq, 3 * Cq)
+        M = oracle.dense_matrix(ps, ETA, 1.0, wall=bool(kinds[q]))
+        Mc = coarse.coarse_matrix(M, ref, Cq)
+        rel = np.linalg.norm(L @ L.T - Mc) / np.linalg.norm(Mc)
+        assert rel <= 1e-5, (q, rel)
+        assert np.abs(X @ L - np.eye(3 * Cq)).max() <= 1e-10, q
+        c0 += Cq
+    assert c0 == C
+
+
+@pytest.mark.parametrize("per_system", [False, True])
+def test_ensemble_two_level_is_a_square_root(per_system):
+    sizes, kinds = [14, 0, 12, 10, 9, 11, 8, 13, 10, 12], [0, 0, 1, 1, 0, 1, 0, 1, 0, 0]
+    pos = _ensemble(sizes, kinds, 5)
+    s = _ens_solver(sizes, kinds, pos, 1e-9, 3, per_system=per_system)
+    n = pos.size
+    B = np.zeros((n, n))
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1.0
+        B[:, k] = s.sqrtMdotW_with_noise(e)[0].ravel()
+    M = oracle.dense_matrix(pos, ETA, 1.0, wall=kinds, sys_sizes=sizes)
+    rel = np.linalg.norm(B @ B.T - M) / np.linalg.norm(M)
+    print(f"[coarse] ensemble (per system {per_system}): cells {s.coarseCells()[2]}, "
+          f"|BB^T - M|/|M| = {rel:.2e}")
+    assert rel <= 1e-6
+
+
+def test_ensemble_per_system_sqrt_matches_the_oracle():
+    """Per-system recurrences with the coarse level: each system's result is
+    the oracle's two-level Lanczos on that system with the build's L_s."""
+    sizes, kinds = [300, 0, 260, 200], [0, 0, 1, 0]
+    pos = _ensemble(sizes, kinds, 9)
+    s = _ens_solver(sizes, kinds, pos, 1e-8, 12, per_system=True)
+    W = np.random.default_rng(3).normal(size=pos.size)
+    u, _ = s.sqrtMdotW_with_noise(W)
+    u = u.ravel()
+    cell_of, _, _ = s.coarseCells()
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    c0 = 0
+    for q, (a, b) in enumerate(zip(off[:-1], off[1:])):
+        if a == b:
+            continue
+        L, X = s.coarseFactors(q)
+        Cq = L.shape[0] // 3
+        M = oracle.dense_matrix(pos[a:b], ETA, 1.0, wall=bool(kinds[q]))
+        Q = coarse.coarse_Q(cell_of[a:b] - c0, Cq)
+        ref, _ = coarse.TwoLevel(Q, L, X).sqrt(M, W[3 * a:3 * b], 1e-8)
+        rel = np.linalg.norm(u[3 * a:3 * b] - ref) / np.linalg.norm(ref)
+        print(f"[coarse] system {q}: cells {Cq}, rel {rel:.2e}")
+        assert rel <= 1e-6, (q, rel)
+        c0 += Cq
