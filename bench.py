@@ -811,8 +811,27 @@ def bench_c4(args):
         its_r = np.zeros(len(sizes), dtype=np.int32)
         g.ffi.check(g.lib.lm_nbody_last_lanczos_iterations_per_system(he, its_r.ctypes.data,
                                                                       len(sizes)))
+        log(f"sqrtMdotW: one recurrence per system, two-level preconditioner (learned blocks + "
+            f"coarse level of {COARSE_PPC} particles per cell, per system)")
+        from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
+        net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+        g.ffi.check(g.lib.lm_nbody_set_preconditioner(he, net._h))
+        g.ffi.check(g.lib.lm_nbody_set_coarse_level(he, COARSE_PPC))
+        ms_lt, it_t = g.time_lanczos(he, W_d, 3 * N, args.lanczos_calls)
+        its_t = np.zeros(len(sizes), dtype=np.int32)
+        g.ffi.check(g.lib.lm_nbody_last_lanczos_iterations_per_system(he, its_t.ctypes.data,
+                                                                      len(sizes)))
+        cells_t, build_t = g.coarse_cells(he), g.coarse_build_ms(he)
+        g.ffi.check(g.lib.lm_nbody_set_coarse_level(he, 0))
+        g.ffi.check(g.lib.lm_nbody_set_preconditioner(he, None))
         up = Ulp.cpu().numpy()
         extra.update({
+            "sqrtMdotW_per_system_two_level_ms": ms_lt,
+            "sqrtMdotW_per_system_two_level_iterations_max": int(np.max(its_t)),
+            "sqrtMdotW_per_system_two_level_iterations_mean": float(np.mean(its_t)),
+            "sqrtMdotW_per_system_two_level_cells": cells_t,
+            "sqrtMdotW_per_system_two_level_coarse_build_ms": build_t,
+            "sqrtMdotW_per_system_two_level_speedup_vs_recurrences": ms_lr / ms_lt,
             "sqrtMdotW_ensemble_ms": ms_le,
             "sqrtMdotW_ensemble_iterations": it_e,
             "sqrtMdotW_per_system_ms": ms_lp,
