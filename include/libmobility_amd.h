@@ -259,7 +259,7 @@ int lm_nbody_precond_blocks(lm_nbody* h, double* G9);
 
 /* Extension, no counterpart in the reference: coarse level of the sqrtMdotW
  * preconditioner.  The particles of every system are grouped into spatial
- * cells of about particles_per_cell particles (coarse dimension 3C_s <= 16384
+ * cells of about particles_per_cell particles (coarse dimension 3C_s <= 24576
  * per system); with Q the orthonormal cell columns and Q^T M Q = L L^T (block
  * diagonal over the systems), Lanczos runs on G M G^T with
  * G = D (I + Q (L^{-1} - I) Q^T) (D the learned blocks when a network is

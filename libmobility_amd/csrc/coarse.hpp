@@ -72,7 +72,7 @@ class CoarseLevel {
  public:
   enum Op : int { kG = 0, kGt = 1, kGinv = 2 };
   // Largest coarse dimension 3C; larger requests get bigger cells.
-  static constexpr int kMaxDim = 16384;
+  static constexpr int kMaxDim = 24576;
 
   // Largest number of 64-tiles of a block factored by the one-workgroup path
   // (used when at least kSmallBatch systems qualify).

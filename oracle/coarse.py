@@ -27,7 +27,7 @@ import numpy as np
 from . import LanczosOracle
 
 
-def coarse_cells(xyz, ppc, max_dim=16384):
+def coarse_cells(xyz, ppc, max_dim=24576):
     """(cell_of (N,), grid (3,), C) by the rule in the module docstring."""
     xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
     N = xyz.shape[0]

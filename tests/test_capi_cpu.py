@@ -222,4 +222,4 @@ def test_coarse_cells_match_the_oracle_rule(shape):
     ref, g, C = coarse.coarse_cells(xyz, ppc)
     assert nc.value == C and list(grid) == list(g)
     assert np.array_equal(cell_of, ref)
-    assert 3 * C <= 16384
+    assert 3 * C <= 24576
