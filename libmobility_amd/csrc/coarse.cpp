@@ -116,6 +116,8 @@ void CoarseLevel::build(const void* rec64, const void* rec32, const std::vector<
   }
   cells_.start.push_back((int)N);
   cells_.C = C;
+  max_row_ = 0;
+  for (const CoarseSys& S : sys_) max_row_ = std::max(max_row_, 3 * S.C);
   N_ = N;
   std::vector<double> scale(C);
   for (int c = 0; c < C; ++c)
@@ -181,9 +183,10 @@ void CoarseLevel::apply(Op op, const double* in, double* out, int64_t n, hipStre
   if (cells_.C == 0) throw Error("[libmobility_amd] coarse level not built");
   if (n < 3 * N_) throw Error("[libmobility_amd] coarse level: vector too short");
   const double* T = op == kG ? X_.as<double>() : (op == kGt ? Xt_.as<double>() : A_.as<double>());
-  launch_coarse_apply(T, sys_dev_.as<CoarseSys>(), row_sys_.as<int>(), 3 * cells_.C, op != kGt, in,
-                      out, N_, n, perm_.as<int>(), start_.as<int>(), cell_of_.as<int>(),
-                      scale_.as<double>(), cells_.C, cvec_.as<double>(), yvec_.as<double>(), s);
+  launch_coarse_apply(T, sys_dev_.as<CoarseSys>(), row_sys_.as<int>(), 3 * cells_.C, max_row_,
+                      op != kGt, in, out, N_, n, perm_.as<int>(), start_.as<int>(),
+                      cell_of_.as<int>(), scale_.as<double>(), cells_.C, cvec_.as<double>(),
+                      yvec_.as<double>(), s);
 }
 
 void CoarseLevel::factors(int q, double* L, double* X, hipStream_t st) const {

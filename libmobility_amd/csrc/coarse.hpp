@@ -50,10 +50,11 @@ void launch_coarse_factor_small(double* A, double* X, double* Xt, double* Dinv, 
                                 const CoarseSys* sys, const int* list, int count, hipStream_t s);
 // out = in + Q (T - I) Q^T in on entries [0, 3N), out = in on [3N, n); T the
 // systems' lower (m <= r) or upper (m >= r) triangular blocks; row_sys[r] is
-// the system of coarse row r < rows.  in may equal out.
+// the system of coarse row r < rows; max_row = the largest 3 C_s (selects one
+// wave or one workgroup per row).  in may equal out.
 void launch_coarse_apply(const double* T, const CoarseSys* sys, const int* row_sys, int rows,
-                         bool lower, const double* in, double* out, int64_t N, int64_t n,
-                         const int* perm, const int* start, const int* cell_of,
+                         int max_row, bool lower, const double* in, double* out, int64_t N,
+                         int64_t n, const int* perm, const int* start, const int* cell_of,
                          const double* scale, int C, double* cvec, double* yvec, hipStream_t s);
 
 // Cell assignment of positions in units of a (x, y, z per particle, host):
@@ -108,6 +109,7 @@ class CoarseLevel {
   CoarseCells cells_;
   std::vector<CoarseSys> sys_;
   int64_t N_ = 0;
+  int max_row_ = 0;  // largest 3 C_s
   DevBuf perm_, start_, cell_of_, scale_, sorted_, A_, X_, Xt_, Dinv_, info_, cvec_, yvec_;
   DevBuf sys_dev_, row_sys_, pair_off_, small_;
 };

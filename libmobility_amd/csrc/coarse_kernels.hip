@@ -25,6 +25,7 @@ namespace {
 constexpr int kT = kCoarseTile;  // 64: dense tile edge
 constexpr int kP = kT + 1;       // LDS row pitch (doubles)
 constexpr int kKB = 4;           // tile columns per block of the factorisation updates
+constexpr int kWaveRowMax = 2048;  // longest coarse row of the one-wave-per-row matvec
 
 // ---- cell-sorted records ---------------------------------------------------
 
@@ -530,6 +531,29 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(const double* __restrict__ 
   if (threadIdx.x == 0) y[r] = red[0] - x[r];
 }
 
+// The same with one wave per row (short rows: many small systems);
+// fixed-order butterfly.
+__global__ __launch_bounds__(256) void k_coarse_gemv_wave(const double* __restrict__ T,
+                                                          const CoarseSys* __restrict__ sys,
+                                                          const int* __restrict__ row_sys,
+                                                          int rows, int lower,
+                                                          const double* __restrict__ x,
+                                                          double* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= rows) return;  // uniform per wave
+  const CoarseSys S = sys[row_sys[r]];
+  const int r0 = 3 * S.c0, lr = r - r0;
+  const int m0 = lower ? 0 : lr, m1 = lower ? lr + 1 : 3 * S.C;
+  const double* row = T + S.moff + (int64_t)lr * S.n;
+  const double* xs = x + r0;
+  double s = 0.0;
+  for (int m = m0 + lane; m < m1; m += 64) s = __builtin_fma(row[m], xs[m], s);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) y[r] = s - x[r];
+}
+
 // out[k] = in[k] + scale[c] * y[3c + a] for k = 3i + a < 3N (c = cell of i),
 // out[k] = in[k] beyond (in may equal out)
 __global__ void k_coarse_prolong(const double* in, const int* __restrict__ cell_of,
@@ -618,13 +642,17 @@ void launch_coarse_factor_small(double* A, double* X, double* Xt, double* Dinv, 
 }
 
 void launch_coarse_apply(const double* T, const CoarseSys* sys, const int* row_sys, int rows,
-                         bool lower, const double* in, double* out, int64_t N, int64_t n,
-                         const int* perm, const int* start, const int* cell_of,
+                         int max_row, bool lower, const double* in, double* out, int64_t N,
+                         int64_t n, const int* perm, const int* start, const int* cell_of,
                          const double* scale, int C, double* cvec, double* yvec, hipStream_t s) {
   hipLaunchKernelGGL(k_coarse_restrict, dim3(blocks_for(C, 4)), dim3(256), 0, s, in, perm, start,
                      scale, C, cvec);
-  hipLaunchKernelGGL(k_coarse_gemv, dim3(rows), dim3(256), 0, s, T, sys, row_sys, lower ? 1 : 0,
-                     cvec, yvec);
+  if (max_row <= kWaveRowMax)
+    hipLaunchKernelGGL(k_coarse_gemv_wave, dim3(blocks_for(rows, 4)), dim3(256), 0, s, T, sys,
+                       row_sys, rows, lower ? 1 : 0, cvec, yvec);
+  else
+    hipLaunchKernelGGL(k_coarse_gemv, dim3(rows), dim3(256), 0, s, T, sys, row_sys, lower ? 1 : 0,
+                       cvec, yvec);
   hipLaunchKernelGGL(k_coarse_prolong, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, cell_of,
                      scale, yvec, 3 * N, n, out);
   LM_HIP(hipGetLastError());

@@ -380,3 +380,34 @@ def test_ensemble_per_system_sqrt_matches_the_oracle():
         print(f"[coarse] system {q}: cells {Cq}, rel {rel:.2e}")
         assert rel <= 1e-6, (q, rel)
         c0 += Cq
+
+
+def test_sqrt_matches_the_oracle_with_long_coarse_rows():
+    """A coarse block wider than one wave's matvec rows (3C > 2048: one
+    workgroup per row): the result is the oracle's two-level Lanczos with the
+    build's L, the oracle's matrix-free FP64 product and a sparse Q."""
+    import scipy.sparse as sp
+    from libmobility_amd.workloads import rsa_cube
+    N = 3000
+    pos, _ = rsa_cube(N, 0.2, seed=17)
+    s = _solver(pos, tol=1e-8, ppc=4)
+    W = np.random.default_rng(8).normal(size=3 * N)
+    u, _ = s.sqrtMdotW_with_noise(W)
+    cell_of, _, C = s.coarseCells()
+    assert 3 * C > 2048
+    L, X = s.coarseFactors()
+    counts = np.bincount(cell_of, minlength=C)
+    rows = np.arange(3 * N)
+    cols = 3 * np.repeat(cell_of, 3) + np.tile(np.arange(3), N)
+    Q = sp.csr_matrix((1.0 / np.sqrt(np.repeat(counts[cell_of], 3)), (rows, cols)),
+                      shape=(3 * N, 3 * C))
+
+    class Op:
+        def __matmul__(self, v):
+            return oracle.mdot(pos, ETA, 1.0, F=v)[0]
+
+    ref, it = coarse.TwoLevel(Q, L, X).sqrt(Op(), W, 1e-8)
+    rel = np.linalg.norm(u.ravel() - ref) / np.linalg.norm(ref)
+    print(f"[coarse] N={N}, 3C={3 * C}: GPU {s.lastLanczosIterations()} / oracle {it} iterations, "
+          f"rel {rel:.2e}")
+    assert rel <= 1e-6
