@@ -443,6 +443,19 @@ def bench_c3(args):
         extra["sqrtMdotW_learned_refresh10_coarse_builds"] = nb.value - nb0.value
         extra["sqrtMdotW_learned_speedup_vs_plain_with_setPositions_refresh10"] = ms_l32p / ms_refresh
         extra["coarse_build_ms"] = build
+        log("tol 1e-4: plain and two-level sqrtMdotW, FP32 pairs")
+        hp4 = g.solver(pos_d, "fp32", tol=1e-4, tiling=args.tiling)
+        ms_p4, it_p4 = g.time_lanczos(hp4, W_d, 3 * N, max(1, args.lanczos_calls - 1))
+        g.destroy(hp4)
+        ht4 = g.solver(pos_d, "fp32", tol=1e-4, tiling=args.tiling)
+        g.ffi.check(g.lib.lm_nbody_set_preconditioner(ht4, net._h))
+        g.ffi.check(g.lib.lm_nbody_set_coarse_level(ht4, COARSE_PPC))
+        ms_t4, it_t4 = g.time_lanczos(ht4, W_d, 3 * N, max(1, args.lanczos_calls - 1))
+        g.destroy(ht4)
+        extra["sqrtMdotW_tol1e-4"] = {
+            "plain_fp32pair_ms": ms_p4, "plain_fp32pair_iterations": it_p4,
+            "learned_fp32pair_ms": ms_t4, "learned_fp32pair_iterations": it_t4,
+            "learned_speedup_vs_plain": ms_p4 / ms_t4}
         log("ablation: coarse level only (no network)")
         hc = g.solver(pos_d, "fp32", tiling=args.tiling)
         g.ffi.check(g.lib.lm_nbody_set_coarse_level(hc, COARSE_PPC))
