@@ -273,6 +273,8 @@ int lm_nbody_coarse_cells(lm_nbody* h, int* ncells, int* grid, int* cell_of);
 /* L and L^{-1} of one system's coarse block, 3C_s x 3C_s row-major doubles
  * each (C_s from lm_nbody_coarse_system_cells). */
 int lm_nbody_coarse_factors(lm_nbody* h, int system, double* L, double* Linv);
+/* Number of cells C_s of one system in the last coarse build (0 for an empty
+ * system). */
 int lm_nbody_coarse_system_cells(lm_nbody* h, int system, int* ncells);
 /* Wall-clock milliseconds of the last coarse build (cells, Q^T M Q, factors). */
 int lm_nbody_coarse_build_ms(lm_nbody* h, double* ms);
@@ -347,8 +349,8 @@ int lm_diag_valu_throughput(int kind, int blocks, int iters, float* ms);
 /* Philox4x32-10 of the noise kernels: out = philox(ctr, key). */
 int lm_selftest_philox_host(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4);
 
-/* Cell assignment of the coarse level for N positions xyz (3N doubles, units
- * of a): *ncells, grid[3] and cell_of (N ints), as lm_nbody_coarse_cells
+/* Cell assignment of the coarse level for the N positions of one system, xyz
+ * (3N doubles, units of a): *ncells, grid[3] and cell_of (N ints), as lm_nbody_coarse_cells
  * reports after a build. */
 int lm_selftest_coarse_cells_host(const double* xyz, int64_t N, int particles_per_cell,
                                   int* ncells, int* grid, int* cell_of);
