@@ -222,15 +222,16 @@ __device__ inline double lane_value(double v, int lane) {
 }
 
 // Factor the diagonal tile k, write L_kk (upper part zeroed) and its inverse
-// Dinv_k.  One wave (lanes 0..63 of the caller), registers only: lane r holds
-// row r, and column j of the right-looking factorisation reaches the other
-// lanes by readlane.  The inverse is right-looking too: at step m row m of
-// L_kk^{-1} is final (lane m scales it by 1 / L_mm) and reaches the rows below
-// by readlane.  Entry c of a row of the inverse comes to life at step c, when
-// column c of L dies, so about 65 doubles per lane are live.  info[0] = 1
-// when a pivot is not positive.
-__device__ __attribute__((always_inline)) inline void chol_diag_wave(double* __restrict__ A, int64_t ld, int k,
-                                      double* __restrict__ Dinv, int* __restrict__ info) {
+// Dinv_k.  One wave (lanes 0..63 of the caller): lane r holds row r in
+// registers, and column j of the right-looking factorisation reaches the
+// other lanes by readlane.  Then lane c forms column c of L_kk^{-1} by
+// forward substitution, the rows of L_kk read from Ls (same address in every
+// lane).  info[0] = 1 when a pivot is not positive.
+__device__ __attribute__((always_inline)) inline void chol_diag_wave(double* __restrict__ A,
+                                                                     int64_t ld, int k,
+                                                                     double* __restrict__ Dinv,
+                                                                     int* __restrict__ info,
+                                                                     double (*Ls)[kP]) {
   const int r = threadIdx.x & 63;
   double* Akk = A + (int64_t)k * kT * ld + (int64_t)k * kT;
   double a[kT];
@@ -253,27 +254,26 @@ __device__ __attribute__((always_inline)) inline void chol_diag_wave(double* __r
   }
 #pragma unroll
   for (int c = 0; c < kT; ++c) {
-    a[c] = c <= r ? a[c] : 0.0;
-    Akk[(int64_t)r * ld + c] = a[c];
+    const double v = c <= r ? a[c] : 0.0;
+    Ls[r][c] = v;
+    Akk[(int64_t)r * ld + c] = v;
   }
   if (bad && r == 0) info[0] = 1;
-  double x[kT];  // row r of L_kk^{-1}
+  // The wave's LDS writes complete before its reads.  No workgroup barrier:
+  // the caller may be one wave of a larger workgroup.
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  double x[kT];
 #pragma unroll
-  for (int m = 0; m < kT; ++m) {
-    x[m] = r == m ? 1.0 : 0.0;
-    const double ilmm = 1.0 / lane_value(a[m], m);
+  for (int i = 0; i < kT; ++i) {
+    double s[4] = {i == r ? 1.0 : 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c <= m; ++c) x[c] = r == m ? x[c] * ilmm : x[c];
-    // rows r > m: X_rc -= L_rm X_mc, c <= m
-#pragma unroll
-    for (int c = 0; c <= m; ++c) {
-      const double xmc = lane_value(x[c], m);
-      x[c] = r > m ? __builtin_fma(-a[m], xmc, x[c]) : x[c];
-    }
+    for (int m = 0; m < i; ++m) s[m & 3] = __builtin_fma(-Ls[i][m], x[m], s[m & 3]);
+    x[i] = ((s[0] + s[1]) + (s[2] + s[3])) * (1.0 / Ls[i][i]);
   }
   double* D = Dinv + (int64_t)k * kT * kT;
 #pragma unroll
-  for (int c = 0; c < kT; ++c) D[r * kT + c] = x[c];
+  for (int i = 0; i < kT; ++i) D[i * kT + r] = x[i];
 }
 
 // L_ik = A_ik L_kk^{-T}
@@ -352,7 +352,8 @@ __device__ inline void op_transpose(const double* __restrict__ X, double* __rest
 __global__ __launch_bounds__(64) void k_chol_diag(double* __restrict__ A, int64_t ld, int k,
                                                   double* __restrict__ Dinv,
                                                   int* __restrict__ info) {
-  chol_diag_wave(A, ld, k, Dinv, info);
+  __shared__ double Ls[kT][kP];
+  chol_diag_wave(A, ld, k, Dinv, info, Ls);
 }
 
 __global__ __launch_bounds__(256) void k_chol_panel(double* __restrict__ A, int64_t ld, int k,
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(256) void k_factor_small(double* __restrict__ A,
     const int kend = kb + kKB < nb ? kb + kKB : nb;
     for (int k = kb; k < kend; ++k) {
       __syncthreads();
-      if (threadIdx.x < 64) chol_diag_wave(a, n, k, dinv, info);
+      if (threadIdx.x < 64) chol_diag_wave(a, n, k, dinv, info, P0);
       for (int i = k + 1; i < nb; ++i) op_panel(a, n, k, i, dinv, P0, P1);
       for (int j = k + 1; j < kend; ++j)
         for (int i = j; i < nb; ++i) op_chol_update(a, n, i, j, k, k + 1, P0, P1);
