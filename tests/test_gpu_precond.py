@@ -176,3 +176,39 @@ def test_ablation_trained_network_vs_plain():
           f"block loss trained {err_t:.3e} untrained {err_u:.3e}")
     assert err_t < 0.5 * err_u
     assert np.mean(trained) <= 1.2 * np.mean(plain)
+
+
+def test_ablation_with_the_coarse_level():
+    """What the trained blocks add to the coarse level (RSA packing, N = 4096,
+    phi = 0.2, tol 1e-6, 48 particles per cell): the coarse level sets the
+    iteration count (well under plain Lanczos); the trained blocks on top
+    change it by at most 20 %, and both stay exact square roots (tested
+    separately)."""
+    import os
+    from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
+    from libmobility_amd.workloads import rsa_cube
+    if not os.path.exists(DEFAULT_WEIGHTS):
+        pytest.skip("trained weights not packaged")
+    N = 4096
+    pos, _ = rsa_cube(N, 0.2, seed=22)
+    rng = np.random.default_rng(1)
+    Ws = [rng.normal(size=3 * N) for _ in range(2)]
+
+    def run(net, ppc):
+        s = _open_solver(pos, tol=1e-6)
+        if net is not None:
+            s.setPreconditioner(net)
+        s.setCoarseLevel(ppc)
+        its = []
+        for w in Ws:
+            s.sqrtMdotW_with_noise(w)
+            its.append(s.lastLanczosIterations())
+        return its
+
+    plain = run(None, 0)
+    coarse_only = run(None, 48)
+    two_level = run(LearnedPreconditioner(weights=DEFAULT_WEIGHTS), 48)
+    print(f"[ablation] iterations plain {plain} coarse level {coarse_only} "
+          f"trained blocks + coarse level {two_level}")
+    assert np.mean(coarse_only) <= 0.75 * np.mean(plain)
+    assert np.mean(two_level) <= 1.2 * np.mean(coarse_only)
