@@ -246,6 +246,27 @@ __device__ inline double lane_rol1<double>(double v) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// Open-boundary UF coefficients times 4/3, so that the far field costs one
+// operation less: fuf = 1/r + (2/3)/r^3, guf = 1/r^3 - 2/r^5 (4/3 of
+// rpy_far's); the caller scales its sums by 3/4.
+template <class R>
+__device__ inline void rpy_far_uf34(R ir, RpyCoef<R>& k) {
+  const R ir2 = ir * ir;
+  const R ir3 = ir * ir2;
+  k.fuf = lm_fma(R(2.0 / 3.0), ir3, ir);
+  k.guf = lm_fma(R(-2) * ir2, ir3, ir3);
+}
+
+// rpy_near_select for the 4/3-scaled coefficients
+template <class R>
+__device__ inline void rpy_near_select_uf34(R r2, R ir, RpyCoef<R>& k) {
+  RpyCoef<R> n = k;
+  rpy_near<R, kUF>(r2, ir, n);
+  const bool is_near = r2 < R(4);
+  k.fuf = is_near ? n.fuf * R(4.0 / 3.0) : k.fuf;
+  k.guf = is_near ? n.guf * R(4.0 / 3.0) : k.guf;
+}
+
 template <class R, int TPL>
 __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Rec<R>::T* s_pos,
                                 typename Rec<R>::T* s_f, R* s_red) {
@@ -258,6 +279,7 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
   const SymSys S = a.sys[it.sys];
   const int64_t N = S.off + S.n;  // end of the system's particles
   const int64_t tgt0 = S.off + (int64_t)it.tile * kTile;
+  const double t0s = 0.75 * a.t0;  // the coefficients carry 4/3 (rpy_far_uf34)
 
   R4 pi[TPL];
   R fix[TPL], fiy[TPL], fiz[TPL];
@@ -325,7 +347,7 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
               r2[g][t] = dx[g][t] * dx[g][t] + dy[g][t] * dy[g][t] + dz[g][t] * dz[g][t];
               ir[g][t] = lm_rsqrt(r2[g][t]);
               cf[g][t] = RpyCoef<R>{};
-              rpy_far<R, kUF>(ir[g][t], cf[g][t]);
+              rpy_far_uf34<R>(ir[g][t], cf[g][t]);
               near |= r2[g][t] < R(4);
             }
           }
@@ -334,7 +356,7 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
 #pragma unroll
             for (int g = 0; g < SG; ++g)
 #pragma unroll
-              for (int t = 0; t < TPL; ++t) rpy_near_select<R, kUF>(r2[g][t], ir[g][t], cf[g][t]);
+              for (int t = 0; t < TPL; ++t) rpy_near_select_uf34<R>(r2[g][t], ir[g][t], cf[g][t]);
           }
 #pragma unroll
           for (int g = 0; g < SG; ++g) {
@@ -378,9 +400,9 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
           vy += (double)s_red[(w * 3 + 1) * kPairThreads + tid];
           vz += (double)s_red[(w * 3 + 2) * kPairThreads + tid];
         }
-        So[3 * j + 0] = a.t0 * vx;
-        So[3 * j + 1] = a.t0 * vy;
-        So[3 * j + 2] = a.t0 * vz;
+        So[3 * j + 0] = t0s * vx;
+        So[3 * j + 1] = t0s * vy;
+        So[3 * j + 2] = t0s * vz;
       }
       if constexpr (std::is_same<R, float>::value) {
 #pragma unroll
@@ -402,9 +424,9 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
   for (int t = 0; t < TPL; ++t) {
     const int64_t i = tgt0 + tid + t * kPairThreads;
     if (i < N) {
-      Uo[3 * i + 0] = a.t0 * acc64[t].x;
-      Uo[3 * i + 1] = a.t0 * acc64[t].y;
-      Uo[3 * i + 2] = a.t0 * acc64[t].z;
+      Uo[3 * i + 0] = t0s * acc64[t].x;
+      Uo[3 * i + 1] = t0s * acc64[t].y;
+      Uo[3 * i + 2] = t0s * acc64[t].z;
     }
   }
 }
