@@ -211,6 +211,7 @@ def result_line(metric, value, unit, steps, warmup, ms, higher, dtype, config, r
 MEASURED_TRAFFIC_BYTES = {
     ("c3", 131072, "ordered", 2, 32): (10454.1 * 2.0 + 98304.0) * 1024.0,
     ("c3", 131072, "symmetric", 4, 64): (152410.0 * 2.0 + 394750.0) * 1024.0,
+    ("c3", 131072, "symmetric", 4, 32): (104635.2 * 2.0 + 296448.0) * 1024.0,
 }
 
 
