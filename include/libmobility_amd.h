@@ -225,7 +225,8 @@ int lm_nbody_synchronize(lm_nbody* h);
  * nearest neighbours (48 floats, units of a) through 48-128-128-6 (GELU,
  * BF16 MFMA, FP32 accumulation) to the Cholesky factor of an SPD 3x3 block
  * G_i.  With a preconditioner attached, sqrtMdotW runs Lanczos on
- * G M G and returns G^{-1} (G M G)^{1/2} W, another square root of M.
+ * G M G and returns G^{-1} (G M G)^{1/2} W, another square root of M (with
+ * the coarse level too, G = D Gc; see lm_nbody_set_coarse_level).
  * All arrays of this group are host arrays. */
 typedef struct lm_precond lm_precond;
 
