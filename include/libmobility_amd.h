@@ -257,6 +257,10 @@ int lm_nbody_precond_dataset(lm_nbody* h, int64_t ncenters, uint64_t seed, float
 int lm_nbody_set_preconditioner(lm_nbody* h, lm_precond* p);
 /* The blocks G (9N doubles, row-major 3x3 per particle) for the positions. */
 int lm_nbody_precond_blocks(lm_nbody* h, double* G9);
+/* The 16 nearest neighbours of every particle within its system (16N ints,
+ * nearest first, ties by index; -1 past the system's other particles), the
+ * network's input geometry. */
+int lm_nbody_precond_neighbours(lm_nbody* h, int* nbr);
 
 /* Extension, no counterpart in the reference: coarse level of the sqrtMdotW
  * preconditioner.  The particles of every system are grouped into spatial

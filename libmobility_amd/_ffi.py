@@ -79,6 +79,7 @@ SIGNATURES = {
     "lm_nbody_precond_dataset": (ctypes.c_int, [_vp, _i64, ctypes.c_uint64, _vp, _vp]),
     "lm_nbody_set_preconditioner": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_precond_blocks": (ctypes.c_int, [_vp, _vp]),
+    "lm_nbody_precond_neighbours": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_set_coarse_level": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lm_nbody_coarse_cells": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "lm_nbody_coarse_factors": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),

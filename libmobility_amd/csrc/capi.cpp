@@ -303,6 +303,13 @@ int lm_nbody_precond_blocks(lm_nbody* h, double* G9) {
   });
 }
 
+int lm_nbody_precond_neighbours(lm_nbody* h, int* nbr) {
+  return guarded([&] {
+    if (!nbr) throw lm::Error("[libmobility_amd] null output");
+    S(h).precondNeighbours(nbr);
+  });
+}
+
 int lm_nbody_set_coarse_level(lm_nbody* h, int particles_per_cell) {
   return guarded([&] { S(h).setCoarseLevel(particles_per_cell); });
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "lm_common.hpp"
 
@@ -24,6 +25,27 @@ constexpr int64_t kPrecondNumWeights = kOffB3 + kPrecondOut;
 
 void launch_knn(const double4* rec, const int64_t* sys_off, int nsys, int64_t N, float* feat,
                 int* nbr, hipStream_t s);
+
+// Uniform grid of one system for the cell-list kNN: cell (qx, qy, qz) covers
+// [lo + q edge, lo + (q + 1) edge) per axis, local id (qx g1 + qy) g2 + qz,
+// global id cell0 + local id.
+struct KnnSys {
+  double lo[3], edge[3];
+  int g[3];
+  int64_t cell0;
+};
+// Host side: grids of all systems (about 8 particles per cell), each
+// particle's local cell id, the particles sorted by (system, cell, index)
+// and the cells' first positions in that order (total cells + 1 entries).
+// xyz: 4 doubles per particle (the FP64 records).
+void knn_grid(const double* xyz4, const int64_t* sys_off, int nsys, std::vector<KnnSys>& sys,
+              std::vector<int>& cell_of, std::vector<int>& perm, std::vector<int64_t>& start);
+// The same neighbours as launch_knn (K smallest FP32 squared distances, ties
+// by index), searching shells of grid cells until the K-th is closer than
+// every unsearched cell.
+void launch_knn_cells(const double4* rec, const int64_t* sys_off, int nsys, const KnnSys* sys,
+                      const int* cell_of, const int* perm, const int64_t* start, int64_t N,
+                      float* feat, int* nbr, hipStream_t s);
 void launch_local_targets(const double4* rec, const int* nbr, const int64_t* centers,
                           int64_t ncenters, float* target9, int* status, hipStream_t s);
 void launch_gemm_bf16(int M, int N, int K, const float* A, int64_t sam, int64_t sak, const float* B,

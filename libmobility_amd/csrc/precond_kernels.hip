@@ -12,7 +12,9 @@
 //              FP64 (coupled Newton-Schulz iteration in LDS).
 //   use      : Lanczos runs on A = G M G (G block diagonal, symmetric) and
 //              returns G^{-1} A^{1/2} W, whose covariance is M.
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "kernels.hpp"
 #include "lm_common.hpp"
@@ -98,6 +100,162 @@ __global__ __launch_bounds__(256) void k_knn(const double4* __restrict__ rec,
     feat[i * 3 * K + 3 * k + 1] = fy;
     feat[i * 3 * K + 3 * k + 2] = fz;
   }
+}
+
+// ---- k nearest neighbours over a cell grid ---------------------------------------
+// One thread per target.  Shell R holds the cells at Chebyshev distance R
+// from the target's cell; after each shell the K-th best squared distance is
+// compared with the distance to the nearest face of the searched block that
+// is not a face of the grid (a 1e-4 margin covers the rounding of the cell
+// assignment).  Candidates are ordered by (FP32 squared distance, index), so
+// the result equals the brute-force kernel's whatever the scan order.
+template <int K>
+__global__ __launch_bounds__(256) void k_knn_cells(
+    const double4* __restrict__ rec, const int64_t* __restrict__ sys_off, int nsys,
+    const KnnSys* __restrict__ sys, const int* __restrict__ cell_of, const int* __restrict__ perm,
+    const int64_t* __restrict__ start, int64_t N, float* __restrict__ feat,
+    int* __restrict__ nbr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int lo = 0, hi = nsys;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (sys_off[mid] <= i) lo = mid; else hi = mid;
+  }
+  const KnnSys S = sys[lo];
+  const int c = cell_of[i];
+  const int q[3] = {c / (S.g[1] * S.g[2]), (c / S.g[2]) % S.g[1], c % S.g[2]};
+  const double4 pi = rec[i];
+  const double x[3] = {pi.x, pi.y, pi.z};
+  float bd[K];
+  int bi[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    bd[k] = 3.0e38f;
+    bi[k] = -1;
+  }
+  for (int R = 0;; ++R) {
+    for (int cx = q[0] - R; cx <= q[0] + R; ++cx) {
+      if (cx < 0 || cx >= S.g[0]) continue;
+      for (int cy = q[1] - R; cy <= q[1] + R; ++cy) {
+        if (cy < 0 || cy >= S.g[1]) continue;
+        const bool edge_xy = cx == q[0] - R || cx == q[0] + R || cy == q[1] - R || cy == q[1] + R;
+        for (int cz = q[2] - R; cz <= q[2] + R; cz += (edge_xy || R == 0) ? 1 : 2 * R) {
+          if (cz < 0 || cz >= S.g[2]) continue;
+          const int64_t cell = S.cell0 + ((int64_t)cx * S.g[1] + cy) * S.g[2] + cz;
+          for (int64_t p = start[cell]; p < start[cell + 1]; ++p) {
+            const int jj = perm[p];
+            if (jj == i) continue;
+            const double4 pj = rec[jj];
+            const float dx = (float)(pj.x - pi.x), dy = (float)(pj.y - pi.y),
+                        dz = (float)(pj.z - pi.z);
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < bd[K - 1] || (d2 == bd[K - 1] && jj < bi[K - 1])) {
+              float cd = d2;
+              int ci = jj;
+#pragma unroll
+              for (int k = 0; k < K; ++k) {  // insert keeping (bd, bi) ascending
+                const bool sw = cd < bd[k] || (cd == bd[k] && (unsigned)ci < (unsigned)bi[k]);
+                const float td = bd[k];
+                const int ti = bi[k];
+                bd[k] = sw ? cd : td;
+                bi[k] = sw ? ci : ti;
+                cd = sw ? td : cd;
+                ci = sw ? ti : ci;
+              }
+            }
+          }
+        }
+      }
+    }
+    // distance to the nearest unsearched cell
+    double dmin = 1.0e300;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (q[k] - R > 0) dmin = fmin(dmin, x[k] - (S.lo[k] + (q[k] - R) * S.edge[k]));
+      if (q[k] + R + 1 < S.g[k]) dmin = fmin(dmin, S.lo[k] + (q[k] + R + 1) * S.edge[k] - x[k]);
+    }
+    if (dmin >= 1.0e299) break;  // the block covers the grid
+    if (dmin > 0.0 && (double)bd[K - 1] < dmin * dmin * (1.0 - 1e-4)) break;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    nbr[i * K + k] = bi[k];
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (bi[k] >= 0) {
+      const double4 pj = rec[bi[k]];
+      fx = (float)(pj.x - pi.x);
+      fy = (float)(pj.y - pi.y);
+      fz = (float)(pj.z - pi.z);
+    }
+    feat[i * 3 * K + 3 * k + 0] = fx;
+    feat[i * 3 * K + 3 * k + 1] = fy;
+    feat[i * 3 * K + 3 * k + 2] = fz;
+  }
+}
+
+void launch_knn_cells(const double4* rec, const int64_t* sys_off, int nsys, const KnnSys* sys,
+                      const int* cell_of, const int* perm, const int64_t* start, int64_t N,
+                      float* feat, int* nbr, hipStream_t s) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_knn_cells<kPrecondK>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s,
+                     rec, sys_off, nsys, sys, cell_of, perm, start, N, feat, nbr);
+  LM_HIP(hipGetLastError());
+}
+
+void knn_grid(const double* xyz4, const int64_t* sys_off, int nsys, std::vector<KnnSys>& sys,
+              std::vector<int>& cell_of, std::vector<int>& perm, std::vector<int64_t>& start) {
+  const int64_t N = sys_off[nsys];
+  sys.assign(nsys, KnnSys{});
+  cell_of.assign(N, 0);
+  perm.assign(N, 0);
+  start.clear();
+  int64_t cells = 0;
+  for (int q = 0; q < nsys; ++q) {
+    const int64_t a = sys_off[q], b = sys_off[q + 1], n = b - a;
+    KnnSys& S = sys[q];
+    S.cell0 = cells;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, ext[3];
+    if (n > 0)
+      for (int k = 0; k < 3; ++k) lo[k] = hi[k] = xyz4[4 * a + k];
+    for (int64_t i = a; i < b; ++i)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], xyz4[4 * i + k]);
+        hi[k] = std::max(hi[k], xyz4[4 * i + k]);
+      }
+    double vol = 1.0;
+    for (int k = 0; k < 3; ++k) {
+      ext[k] = hi[k] - lo[k];
+      vol *= std::max(ext[k], 1.0);
+    }
+    const double side = std::cbrt(vol / std::max(1.0, (double)n / 8.0));
+    int64_t ncell = 1;
+    for (int k = 0; k < 3; ++k) {
+      S.g[k] = (int)std::max<long long>(1, std::min<long long>(1024, std::llround(std::max(ext[k], 1.0) / side)));
+      S.lo[k] = lo[k];
+      S.edge[k] = ext[k] > 0.0 ? ext[k] / S.g[k] : 1.0;
+      ncell *= S.g[k];
+    }
+    std::vector<int64_t> count(ncell + 1, 0);
+    for (int64_t i = a; i < b; ++i) {
+      int qq[3];
+      for (int k = 0; k < 3; ++k) {
+        qq[k] = 0;
+        if (ext[k] > 0.0) {
+          const double f = std::floor((xyz4[4 * i + k] - lo[k]) / S.edge[k]);
+          qq[k] = (int)std::min<double>(S.g[k] - 1, std::max(0.0, f));
+        }
+      }
+      cell_of[i] = (qq[0] * S.g[1] + qq[1]) * S.g[2] + qq[2];
+      ++count[cell_of[i] + 1];
+    }
+    for (int64_t c = 0; c < ncell; ++c) count[c + 1] += count[c];
+    for (int64_t c = 0; c < ncell; ++c) start.push_back(a + count[c]);
+    std::vector<int64_t> fill(count.begin(), count.end() - 1);
+    for (int64_t i = a; i < b; ++i) perm[a + fill[cell_of[i]]++] = (int)i;
+    cells += ncell;
+  }
+  start.push_back(N);
 }
 
 void launch_knn(const double4* rec, const int64_t* sys_off, int nsys, int64_t N, float* feat,

@@ -1477,9 +1477,32 @@ void NBodySolver::ensure_knn() {
   pc_feat_.reserve(sizeof(float) * N_ * kPrecondIn);
   pc_nbr_.reserve(sizeof(int) * N_ * kPrecondK);
   const int nsys = (int)sys_off_host_.size() - 1;
-  launch_knn(rec64_.as<double4>(), sys_off_.as<int64_t>(), nsys, N_, pc_feat_.as<float>(),
-             pc_nbr_.as<int>(), stream_);
+  // cell grid on the host (deterministic), neighbour search on the device
+  knn_xyz_.resize((size_t)4 * N_);
+  LM_HIP(hipMemcpyAsync(knn_xyz_.data(), rec64_.as<void>(), sizeof(double) * 4 * N_,
+                        hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));
+  knn_grid(knn_xyz_.data(), sys_off_host_.data(), nsys, knn_sys_host_, knn_cell_host_,
+           knn_perm_host_, knn_start_host_);
+  auto upload = [&](DevBuf& b, const void* src, size_t bytes) {
+    b.reserve(bytes);
+    LM_HIP(hipMemcpyAsync(b.as<void>(), src, bytes, hipMemcpyHostToDevice, stream_));
+  };
+  upload(knn_sys_, knn_sys_host_.data(), sizeof(KnnSys) * knn_sys_host_.size());
+  upload(knn_cell_, knn_cell_host_.data(), sizeof(int) * N_);
+  upload(knn_perm_, knn_perm_host_.data(), sizeof(int) * N_);
+  upload(knn_start_, knn_start_host_.data(), sizeof(int64_t) * knn_start_host_.size());
+  launch_knn_cells(rec64_.as<double4>(), sys_off_.as<int64_t>(), nsys, knn_sys_.as<KnnSys>(),
+                   knn_cell_.as<int>(), knn_perm_.as<int>(), knn_start_.as<int64_t>(), N_,
+                   pc_feat_.as<float>(), pc_nbr_.as<int>(), stream_);
   knn_layout_ = layout_id_;
+}
+
+void NBodySolver::precondNeighbours(int* nbr) {
+  ensure_knn();
+  LM_HIP(hipMemcpyAsync(nbr, pc_nbr_.as<int>(), sizeof(int) * kPrecondK * N_,
+                        hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipStreamSynchronize(stream_));
 }
 
 // A preconditioner part built for earlier positions of the same systems (same

@@ -108,6 +108,9 @@ class NBodySolver {
   void precondDataset(int64_t ncenters, uint64_t seed, float* X, float* T9);
   // The 3x3 blocks G (9N, host) the attached network gives for the positions.
   void precondBlocks(double* G9);
+  // The K nearest neighbours of every particle within its system (N x K,
+  // host; -1 past the system's other particles), nearest first.
+  void precondNeighbours(int* nbr);
   // Coarse level of the sqrtMdotW preconditioner (per system): cells of about
   // `particles_per_cell` particles, Lanczos on G M G^T with G = D Gc (D the
   // learned blocks when attached, Gc the coarse whitening), result
@@ -310,6 +313,12 @@ class NBodySolver {
   void ensure_precond_blocks();
   LearnedPreconditioner* precond_ = nullptr;
   DevBuf pc_feat_, pc_nbr_, pc_out_, pc_G_, pc_Gi_, pc_t_, pc_u_;
+  // cell grid of the neighbour search
+  std::vector<double> knn_xyz_;
+  std::vector<KnnSys> knn_sys_host_;
+  std::vector<int> knn_cell_host_, knn_perm_host_;
+  std::vector<int64_t> knn_start_host_;
+  DevBuf knn_sys_, knn_cell_, knn_perm_, knn_start_;
   uint64_t knn_layout_ = 0, pc_layout_ = 0, pc_struct_ = 0, pc_version_ = 0;
   double pc_a_ = 0.0;
   int pc_refresh_ = 1;

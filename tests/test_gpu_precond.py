@@ -212,3 +212,55 @@ def test_ablation_with_the_coarse_level():
           f"trained blocks + coarse level {two_level}")
     assert np.mean(coarse_only) <= 0.75 * np.mean(plain)
     assert np.mean(two_level) <= 1.2 * np.mean(coarse_only)
+
+
+@pytest.mark.parametrize("geom", ["cube", "slab", "clusters", "ensemble"])
+def test_neighbours_are_the_nearest(geom):
+    """The cell-grid neighbour search returns, for every particle, the 16
+    nearest particles of its own system, nearest first (distances compared in
+    FP64 against a full sort, to FP32 rounding), and -1 past the system's
+    other particles."""
+    from libmobility_amd import NBody, _ffi
+    from libmobility_amd.workloads import rsa_cube
+    rng = np.random.default_rng(4)
+    sizes, kinds = None, None
+    if geom == "cube":
+        pos = rsa_cube(2500, 0.2, seed=2)[0]
+    elif geom == "slab":
+        pos = np.column_stack([rng.uniform(0, 120, (3000, 2)), rng.uniform(1.5, 9.0, 3000)])
+    elif geom == "clusters":
+        pos = np.vstack([rng.normal(0, 2, (1500, 3)), rng.normal(80, 6, (400, 3)),
+                         rng.uniform(-50, 150, (20, 3))])
+    else:
+        sizes, kinds = [5, 900, 0, 1700, 17, 1], [0, 1, 0, 0, 1, 0]
+        parts = []
+        for n, k in zip(sizes, kinds):
+            p = rng.uniform(0, (n / 0.02) ** (1 / 3) + 1, (n, 3))
+            if k:
+                p[:, 2] = rng.uniform(1.5, 6.0, n)
+            parts.append(p)
+        pos = np.concatenate(parts)
+    wall = geom == "slab"
+    s = NBody("open", "open", "single_wall" if (wall or sizes) else "open")
+    s.setParameters(wallHeight=0.0) if (wall or sizes) else s.setParameters()
+    s.initialize(viscosity=1.0 / (6 * np.pi), hydrodynamicRadius=1.0)
+    if sizes:
+        s.setSystems(sizes, ["single_wall" if k else "open" for k in kinds])
+    s.setPositions(pos)
+    N = pos.shape[0]
+    nbr = np.zeros((N, 16), dtype=np.int32)
+    _ffi.check(_ffi.load().lm_nbody_precond_neighbours(s._h, nbr.ctypes.data))
+    off = np.concatenate([[0], np.cumsum(sizes if sizes else [N])])
+    for a, b in zip(off[:-1], off[1:]):
+        x = pos[a:b]
+        n = b - a
+        for i in range(n):
+            d = np.sqrt(((x - x[i]) ** 2).sum(1))
+            d[i] = np.inf
+            ref = np.sort(d)[:min(16, n - 1)]
+            got = nbr[a + i]
+            m = min(16, n - 1)
+            assert np.all(got[m:] == -1), (a + i, got)
+            assert np.all((got[:m] >= a) & (got[:m] < b)), (a + i, got)
+            dg = np.sqrt(((pos[got[:m]] - x[i]) ** 2).sum(1))
+            assert np.allclose(dg, ref, rtol=1e-5, atol=1e-6), (a + i, dg, ref)
