@@ -23,9 +23,6 @@ constexpr int64_t kOffW3 = kOffB2 + kPrecondH;
 constexpr int64_t kOffB3 = kOffW3 + (int64_t)kPrecondH * kPrecondOut;
 constexpr int64_t kPrecondNumWeights = kOffB3 + kPrecondOut;
 
-void launch_knn(const double4* rec, const int64_t* sys_off, int nsys, int64_t N, float* feat,
-                int* nbr, hipStream_t s);
-
 // Uniform grid of one system for the cell-list kNN: cell (qx, qy, qz) covers
 // [lo + q edge, lo + (q + 1) edge) per axis, local id (qx g1 + qy) g2 + qz,
 // global id cell0 + local id.
@@ -40,8 +37,9 @@ struct KnnSys {
 // xyz: 4 doubles per particle (the FP64 records).
 void knn_grid(const double* xyz4, const int64_t* sys_off, int nsys, std::vector<KnnSys>& sys,
               std::vector<int>& cell_of, std::vector<int>& perm, std::vector<int64_t>& start);
-// The same neighbours as launch_knn (K smallest FP32 squared distances, ties
-// by index), searching shells of grid cells until the K-th is closer than
+// The K nearest neighbours of each particle within its system (K smallest
+// FP32 squared distances, ties by index; -1 past the system's other
+// particles), searching shells of grid cells until the K-th is closer than
 // every unsearched cell.
 void launch_knn_cells(const double4* rec, const int64_t* sys_off, int nsys, const KnnSys* sys,
                       const int* cell_of, const int* perm, const int64_t* start, int64_t N,

@@ -23,92 +23,13 @@
 
 namespace lm {
 
-// ---- k nearest neighbours (brute force over the particle's system) ----------
-// One thread per target; sources stream through LDS.  The K best (squared
-// distance, index) pairs are kept sorted in registers by a fully unrolled
-// insertion step, executed only when a candidate beats the current K-th.
-template <int K>
-__global__ __launch_bounds__(256) void k_knn(const double4* __restrict__ rec,
-                                             const int64_t* __restrict__ sys_off, int nsys,
-                                             int64_t N, float* __restrict__ feat,
-                                             int* __restrict__ nbr) {
-  __shared__ double4 s_rec[256];
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t i0 = (int64_t)blockIdx.x * 256;
-  const int64_t i1 = min(i0 + 256, N) - 1;
-  auto sys_of = [&](int64_t q) {
-    int lo = 0, hi = nsys;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (sys_off[mid] <= q) lo = mid; else hi = mid;
-    }
-    return lo;
-  };
-  // the block's targets may straddle systems: stream the union of their sources
-  const int64_t src0 = sys_off[sys_of(i0)];
-  const int64_t src1 = sys_off[sys_of(i1) + 1];
-  const bool active = i < N;
-  const int my_sys = active ? sys_of(i) : 0;
-  const int64_t my0 = sys_off[my_sys], my1 = sys_off[my_sys + 1];
-  const double4 pi = active ? rec[i] : double4{0, 0, 0, 0};
-  float bd[K];
-  int bi[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    bd[k] = 3.0e38f;
-    bi[k] = -1;
-  }
-  for (int64_t base = src0; base < src1; base += 256) {
-    __syncthreads();
-    const int64_t j = base + threadIdx.x;
-    if (j < src1) s_rec[threadIdx.x] = rec[j];
-    __syncthreads();
-    const int cnt = (int)min((int64_t)256, src1 - base);
-    for (int s = 0; s < cnt; ++s) {
-      const int64_t jj = base + s;
-      const double4 pj = s_rec[s];
-      const float dx = (float)(pj.x - pi.x), dy = (float)(pj.y - pi.y), dz = (float)(pj.z - pi.z);
-      const float d2 = dx * dx + dy * dy + dz * dz;
-      if (active && jj != i && jj >= my0 && jj < my1 && d2 < bd[K - 1]) {
-        float cd = d2;
-        int ci = (int)jj;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {  // insert keeping bd ascending
-          const bool sw = cd < bd[k];
-          const float td = bd[k];
-          const int ti = bi[k];
-          bd[k] = sw ? cd : td;
-          bi[k] = sw ? ci : ti;
-          cd = sw ? td : cd;
-          ci = sw ? ti : ci;
-        }
-      }
-    }
-  }
-  if (!active) return;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    nbr[i * K + k] = bi[k];
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-    if (bi[k] >= 0) {
-      const double4 pj = rec[bi[k]];
-      fx = (float)(pj.x - pi.x);
-      fy = (float)(pj.y - pi.y);
-      fz = (float)(pj.z - pi.z);
-    }
-    feat[i * 3 * K + 3 * k + 0] = fx;
-    feat[i * 3 * K + 3 * k + 1] = fy;
-    feat[i * 3 * K + 3 * k + 2] = fz;
-  }
-}
-
 // ---- k nearest neighbours over a cell grid ---------------------------------------
 // One thread per target.  Shell R holds the cells at Chebyshev distance R
 // from the target's cell; after each shell the K-th best squared distance is
 // compared with the distance to the nearest face of the searched block that
 // is not a face of the grid (a 1e-4 margin covers the rounding of the cell
 // assignment).  Candidates are ordered by (FP32 squared distance, index), so
-// the result equals the brute-force kernel's whatever the scan order.
+// the result is the same whatever the scan order.
 template <int K>
 __global__ __launch_bounds__(256) void k_knn_cells(
     const double4* __restrict__ rec, const int64_t* __restrict__ sys_off, int nsys,
@@ -258,13 +179,6 @@ void knn_grid(const double* xyz4, const int64_t* sys_off, int nsys, std::vector<
   start.push_back(N);
 }
 
-void launch_knn(const double4* rec, const int64_t* sys_off, int nsys, int64_t N, float* feat,
-                int* nbr, hipStream_t s) {
-  if (N <= 0) return;
-  hipLaunchKernelGGL(k_knn<kPrecondK>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, rec,
-                     sys_off, nsys, N, feat, nbr);
-  LM_HIP(hipGetLastError());
-}
 
 // ---- FP64 targets: center block of (M_local/t0)^{-1/2} -------------------------
 constexpr int kC = kPrecondK + 1;  // cluster size
