@@ -1,10 +1,11 @@
 // Coarse level of the two-level sqrtMdotW preconditioner (no counterpart in the
 // reference).
 //
-// Particles of one system are grouped into a grid of spatial cells.  With Q
-// the 3N x 3C matrix of orthonormal cell columns (Q[3i+a, 3c+a] = 1/sqrt(n_c)
-// for particle i in cell c) and Mc = Q^T M Q = L L^T (translational block, in
-// units of t0), the coarse operator
+// The particles of every system are grouped into a grid of spatial cells
+// (cells never span systems).  With Q the 3N x 3C matrix of orthonormal cell
+// columns (Q[3i+a, 3c+a] = 1/sqrt(n_c) for particle i in cell c) and
+// Mc = Q^T M Q = L L^T (translational block in units of t0, block-diagonal
+// over the systems), the coarse operator
 //     Gc = I + Q (L^{-1} - I) Q^T
 // whitens M on the span of Q (Q^T Gc M Gc^T Q = I) and leaves the rest alone.
 // The long-wavelength collective modes, which set the Lanczos iteration count
@@ -72,7 +73,8 @@ void coarse_cells(const double* xyz, int64_t stride, int64_t N, int ppc, CoarseC
 class CoarseLevel {
  public:
   enum Op : int { kG = 0, kGt = 1, kGinv = 2 };
-  // Largest coarse dimension 3C; larger requests get bigger cells.
+  // Largest coarse dimension 3C_s of one system; larger requests get bigger
+  // cells.
   static constexpr int kMaxDim = 24576;
 
   // Largest number of 64-tiles of a block factored by the one-workgroup path
