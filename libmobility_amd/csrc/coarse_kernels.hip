@@ -26,6 +26,8 @@ constexpr int kT = kCoarseTile;  // 64: dense tile edge
 constexpr int kP = kT + 1;       // LDS row pitch (doubles)
 constexpr int kKB = 4;           // tile columns per block of the factorisation updates
 constexpr int kWaveRowMax = 2048;  // longest coarse row of the one-wave-per-row matvec
+// trailing updates of at least this many tiles use the two-by-two-tile kernels
+constexpr int64_t kQuadTiles = 1024;
 
 // ---- cell-sorted records ---------------------------------------------------
 
@@ -201,6 +203,44 @@ __device__ inline void tile_store_mfma(double* __restrict__ out, int64_t ld, con
       for (int r = 0; r < 4; ++r) {
         double* o = out + (int64_t)(R0 + 16 * a + 4 * r + lk) * ld + C0 + 16 * b + li;
         *o = SUB ? *o - acc[a][b][r] : acc[a][b][r];
+      }
+}
+
+// acc += A * (TB ? B^T : B) for 64 x 64 tiles in LDS, the whole tile by one
+// wave: 4 x 4 tiles of v_mfma_f64_16x16x4f64, 16 products per 8 operand
+// reads (same lane layout as tile_mm_mfma).
+template <bool TB>
+__device__ inline void tile_mm_wave(const double (*A)[kP], const double (*B)[kP], d4 acc[4][4]) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 2
+  for (int kk = 0; kk < kT; kk += 4) {
+    double a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = A[16 * q + li][kk + lk];
+      b[q] = TB ? B[16 * q + li][kk + lk] : B[kk + lk][16 * q + li];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        acc[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p], b[q], acc[p][q], 0, 0, 0);
+  }
+}
+
+// out[row][col] -= the tile result of tile_mm_wave
+__device__ inline void tile_sub_wave(double* __restrict__ out, int64_t ld, const d4 acc[4][4]) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double* o = out + (int64_t)(16 * p + 4 * r + lk) * ld + 16 * q + li;
+        *o -= acc[p][q][r];
       }
 }
 
@@ -383,6 +423,120 @@ __global__ __launch_bounds__(256) void k_chol_trailing(double* __restrict__ A, i
   int ii, jj;
   tri_index(blockIdx.x, ii, jj);
   op_chol_update(A, ld, kend + ii, kend + jj, kb, kend, P0, P1);
+}
+
+// Two tile rows [r0, r0 + 2) by two tile columns [c0, c0 + 2) per workgroup
+// (wave w owns tile (r0 + (w >> 1), c0 + (w & 1))), so that every tile
+// staged in LDS feeds two output tiles: half the panel bytes per flop of the
+// one-tile kernels.  Tiles past the matrix (tile row nb; for X, tile column
+// kend) are neither read nor written.
+// The four tiles of one k (a0, a1 the row operands, b0, b1 the column
+// operands; null for a tile past the matrix) pass through registers: pf is
+// loaded for step k + 1 while the matrix cores work on step k.
+struct QuadTiles {
+  double v[4][kT * kT / 256];
+};
+
+__device__ inline void quad_fetch(QuadTiles& pf, const double* __restrict__ a0,
+                                  const double* __restrict__ a1, const double* __restrict__ b0,
+                                  const double* __restrict__ b1, int64_t ld) {
+  const double* src[4] = {a0, a1, b0, b1};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (src[t] == nullptr) continue;  // uniform
+#pragma unroll
+    for (int q = 0; q < kT * kT / 256; ++q) {
+      const int e = (int)threadIdx.x + 256 * q;
+      pf.v[t][q] = src[t][(e >> 6) * ld + (e & 63)];
+    }
+  }
+}
+
+__device__ inline void quad_stage(double (*S)[kT][kP], const QuadTiles& pf, bool a1, bool b1) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if ((t == 1 && !a1) || (t == 3 && !b1)) continue;  // uniform
+#pragma unroll
+    for (int q = 0; q < kT * kT / 256; ++q) {
+      const int e = (int)threadIdx.x + 256 * q;
+      S[t][e >> 6][e & 63] = pf.v[t][q];
+    }
+  }
+}
+
+// A_ij -= sum_{k in [kb, kend)} L_ik L_jk^T for kend <= j <= i < nb, a 2 x 2
+// block of tiles per workgroup (block pair (I, J), I >= J, from blockIdx.x)
+__global__ __launch_bounds__(256) void k_chol_trailing2(double* __restrict__ A, int64_t ld, int kb,
+                                                        int kend, int nb) {
+  __shared__ double S[4][kT][kP];
+  int I, J;
+  tri_index(blockIdx.x, I, J);
+  const int i0 = kend + 2 * I, j0 = kend + 2 * J;
+  const int w = threadIdx.x >> 6;
+  const int i = i0 + (w >> 1), j = j0 + (w & 1);
+  const bool live = i < nb && j <= i;  // uniform per wave
+  d4 acc[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[p][q] = d4{0.0, 0.0, 0.0, 0.0};
+  const bool a1 = i0 + 1 < nb, b1 = j0 + 1 < nb;
+  QuadTiles pf;
+  const double* ai = A + (int64_t)i0 * kT * ld;
+  const double* aj = A + (int64_t)j0 * kT * ld;
+  const int64_t row = (int64_t)kT * ld;
+  quad_fetch(pf, ai + (int64_t)kb * kT, a1 ? ai + row + (int64_t)kb * kT : nullptr,
+             aj + (int64_t)kb * kT, b1 ? aj + row + (int64_t)kb * kT : nullptr, ld);
+  for (int k = kb; k < kend; ++k) {
+    __syncthreads();  // the previous step is done with S
+    quad_stage(S, pf, a1, b1);
+    __syncthreads();
+    if (k + 1 < kend) {
+      const int64_t c = (int64_t)(k + 1) * kT;
+      quad_fetch(pf, ai + c, a1 ? ai + row + c : nullptr, aj + c, b1 ? aj + row + c : nullptr, ld);
+    }
+    if (live) tile_mm_wave<true>(S[w >> 1], S[2 + (w & 1)], acc);
+  }
+  if (live) tile_sub_wave(A + (int64_t)i * kT * ld + (int64_t)j * kT, ld, acc);
+}
+
+// X_im -= sum_{k in [max(kb, m0), kend)} L_ik X_km for kend <= i < nb,
+// m < kend, a 2 x 2 block of tiles per workgroup (rows i0, i0 + 1, columns
+// m0, m0 + 1).  X_km = 0 for m > k, so the shared k range adds exact zeros
+// to column m0 + 1.
+__global__ __launch_bounds__(256) void k_trinv_trailing2(double* __restrict__ X,
+                                                         const double* __restrict__ L, int64_t ld,
+                                                         int kb, int kend, int nb) {
+  __shared__ double S[4][kT][kP];
+  const int mcols = (kend + 1) / 2;
+  const int i0 = kend + 2 * (int)(blockIdx.x / mcols), m0 = 2 * (int)(blockIdx.x % mcols);
+  const int w = threadIdx.x >> 6;
+  const int i = i0 + (w >> 1), m = m0 + (w & 1);
+  const bool live = i < nb && m < kend;  // uniform per wave
+  d4 acc[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[p][q] = d4{0.0, 0.0, 0.0, 0.0};
+  const bool a1 = i0 + 1 < nb, b1 = m0 + 1 < kend;
+  const int k0 = m0 > kb ? m0 : kb;
+  QuadTiles pf;
+  const double* li = L + (int64_t)i0 * kT * ld;  // row operands L_ik, L_(i+1)k
+  const double* xm = X + (int64_t)m0 * kT;       // column operands X_km, X_k(m+1)
+  const int64_t row = (int64_t)kT * ld;
+  quad_fetch(pf, li + (int64_t)k0 * kT, a1 ? li + row + (int64_t)k0 * kT : nullptr,
+             xm + (int64_t)k0 * row, b1 ? xm + (int64_t)k0 * row + kT : nullptr, ld);
+  for (int k = k0; k < kend; ++k) {
+    __syncthreads();  // the previous step is done with S
+    quad_stage(S, pf, a1, b1);
+    __syncthreads();
+    if (k + 1 < kend) {
+      const int64_t c = (int64_t)(k + 1) * kT, r = (int64_t)(k + 1) * row;
+      quad_fetch(pf, li + c, a1 ? li + row + c : nullptr, xm + r, b1 ? xm + r + kT : nullptr, ld);
+    }
+    if (live) tile_mm_wave<false>(S[w >> 1], S[2 + (w & 1)], acc);
+  }
+  if (live) tile_sub_wave(X + (int64_t)i * kT * ld + (int64_t)m * kT, ld, acc);
 }
 
 __global__ __launch_bounds__(256) void k_trinv_scale(double* __restrict__ X, int64_t ld, int k,
@@ -611,9 +765,12 @@ void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* i
                            A, ld, k, kend);
     }
     const int T = nb - kend;
-    if (T > 0)
-      hipLaunchKernelGGL(k_chol_trailing, dim3((unsigned)((int64_t)T * (T + 1) / 2)), dim3(256), 0,
-                         s, A, ld, kb, kend);
+    const int64_t tiles = (int64_t)T * (T + 1) / 2, T2 = (T + 1) / 2;
+    if (tiles >= kQuadTiles)
+      hipLaunchKernelGGL(k_chol_trailing2, dim3((unsigned)(T2 * (T2 + 1) / 2)), dim3(256), 0, s, A,
+                         ld, kb, kend, nb);
+    else if (T > 0)
+      hipLaunchKernelGGL(k_chol_trailing, dim3((unsigned)tiles), dim3(256), 0, s, A, ld, kb, kend);
   }
   LM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_identity, dim3(blocks_for((int64_t)n * n, 256)), dim3(256), 0, s, X, ld, n);
@@ -626,9 +783,12 @@ void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* i
                            s, X, A, ld, k);
     }
     const int T = nb - kend;
-    if (T > 0)
-      hipLaunchKernelGGL(k_trinv_trailing, dim3((unsigned)((int64_t)T * kend)), dim3(256), 0, s, X,
-                         A, ld, kb, kend);
+    const int64_t tiles = (int64_t)T * kend;
+    if (tiles >= kQuadTiles)
+      hipLaunchKernelGGL(k_trinv_trailing2, dim3((unsigned)((int64_t)((T + 1) / 2) * ((kend + 1) / 2))),
+                         dim3(256), 0, s, X, A, ld, kb, kend, nb);
+    else if (T > 0)
+      hipLaunchKernelGGL(k_trinv_trailing, dim3((unsigned)tiles), dim3(256), 0, s, X, A, ld, kb, kend);
   }
   LM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_transpose, dim3(nb, nb), dim3(256), 0, s, X, Xt, ld);
