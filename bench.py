@@ -44,7 +44,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mdot Gpair/s + sqrtMdotW ms at N=131072 tol 1e-6: learned-prec vs plain"
 # coarse level of the preconditioned sqrtMdotW: target particles per cell
-COARSE_PPC = 48
+COARSE_PPC = 40
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
 PEAK_FP64_TFLOPS = 78.6
 ETA = 1.0 / (6.0 * np.pi)
@@ -212,6 +212,7 @@ MEASURED_TRAFFIC_BYTES = {
     ("c3", 131072, "ordered", 2, 32): (10454.1 * 2.0 + 98304.0) * 1024.0,
     ("c3", 131072, "symmetric", 4, 64): (152410.0 * 2.0 + 394750.0) * 1024.0,
     ("c3", 131072, "symmetric", 4, 32): (104635.2 * 2.0 + 296448.0) * 1024.0,
+    ("c3", 131072, "symmetric", 2, 64): (177902.1 * 2.0 + 591360.0) * 1024.0,
 }
 
 
@@ -383,6 +384,13 @@ def bench_c3(args):
         extra["mdot_fp32pair_symmetric_speedup"] = ms_o / ms32
         log(f"  {ms_o:.3f} ms/Mdot ordered; symmetric {ms_o / ms32:.2f}x faster")
     if not args.no_lanczos:
+        # Every FP32 Lanczos leg uses the plain solver's tiling, so that plain
+        # and preconditioned solves apply the same products.  Tilings of equal
+        # speed sum in different orders, and near tol 1e-6 the plain FP32
+        # iteration count moves with that rounding (133 or 151 at N=131072).
+        t = g.tiling(h32)
+        lt = (t["targets_per_lane"], t["source_chunks"])
+        extra["lanczos_tiling_fp32pair"] = t
         log("plain Lanczos sqrtMdotW, FP32 pairs, tol 1e-6")
         ms_l32, it32 = g.time_lanczos(h32, W_d, 3 * N, args.lanczos_calls)
         extra["sqrtMdotW_plain_fp32pair_ms"] = ms_l32
@@ -403,7 +411,7 @@ def bench_c3(args):
         net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
         log(f"two-level preconditioned Lanczos sqrtMdotW (learned 3x3 blocks + coarse level of "
             f"{COARSE_PPC} particles per cell), FP32 pairs, tol 1e-6")
-        hl = g.solver(pos_d, "fp32", tiling=args.tiling)
+        hl = g.solver(pos_d, "fp32", tiling=lt)
         g.ffi.check(g.lib.lm_nbody_set_preconditioner(hl, net._h))
         g.ffi.check(g.lib.lm_nbody_set_coarse_level(hl, COARSE_PPC))
         ms_ll, itl = g.time_lanczos(hl, W_d, 3 * N, args.lanczos_calls)
@@ -445,10 +453,10 @@ def bench_c3(args):
         extra["sqrtMdotW_learned_speedup_vs_plain_with_setPositions_refresh10"] = ms_l32p / ms_refresh
         extra["coarse_build_ms"] = build
         log("tol 1e-4: plain and two-level sqrtMdotW, FP32 pairs")
-        hp4 = g.solver(pos_d, "fp32", tol=1e-4, tiling=args.tiling)
+        hp4 = g.solver(pos_d, "fp32", tol=1e-4, tiling=lt)
         ms_p4, it_p4 = g.time_lanczos(hp4, W_d, 3 * N, max(1, args.lanczos_calls - 1))
         g.destroy(hp4)
-        ht4 = g.solver(pos_d, "fp32", tol=1e-4, tiling=args.tiling)
+        ht4 = g.solver(pos_d, "fp32", tol=1e-4, tiling=lt)
         g.ffi.check(g.lib.lm_nbody_set_preconditioner(ht4, net._h))
         g.ffi.check(g.lib.lm_nbody_set_coarse_level(ht4, COARSE_PPC))
         ms_t4, it_t4 = g.time_lanczos(ht4, W_d, 3 * N, max(1, args.lanczos_calls - 1))
@@ -458,12 +466,12 @@ def bench_c3(args):
             "learned_fp32pair_ms": ms_t4, "learned_fp32pair_iterations": it_t4,
             "learned_speedup_vs_plain": ms_p4 / ms_t4}
         log("ablation: coarse level only (no network)")
-        hc = g.solver(pos_d, "fp32", tiling=args.tiling)
+        hc = g.solver(pos_d, "fp32", tiling=lt)
         g.ffi.check(g.lib.lm_nbody_set_coarse_level(hc, COARSE_PPC))
         ms_c, itc = g.time_lanczos(hc, W_d, 3 * N, max(1, args.lanczos_calls - 1))
         g.destroy(hc)
         log("ablation: learned blocks only (no coarse level)")
-        hb = g.solver(pos_d, "fp32", tiling=args.tiling)
+        hb = g.solver(pos_d, "fp32", tiling=lt)
         g.ffi.check(g.lib.lm_nbody_set_preconditioner(hb, net._h))
         ms_b, itb = g.time_lanczos(hb, W_d, 3 * N, max(1, args.lanczos_calls - 2))
         g.destroy(hb)
@@ -1006,7 +1014,10 @@ def main():
                     help="only measure the VALU issue ceiling (FMA chains, rsqrt cost)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device-to-device copy (reference for the HBM byte counters)")
+    ap.add_argument("--coarse-ppc", type=int, default=COARSE_PPC,
+                    help="particles per cell of the preconditioner's coarse level")
     args = ap.parse_args()
+    globals()["COARSE_PPC"] = args.coarse_ppc
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(sharding is described in DESIGN.md).", file=sys.stderr)
