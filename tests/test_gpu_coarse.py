@@ -4,7 +4,8 @@ tests/test_oracle_coarse.py).
 
   * the cells of a build are the oracle's cells for the same positions;
   * L L^T equals the oracle's Q^T M Q (FP32 pair terms: 1e-5 relative) and
-    L^{-1} L = I to FP64 rounding;
+    L^{-1} L = I to FP64 rounding, also for a block large enough for the
+    two-by-two-tile trailing updates;
   * the result is a square root of M: ||B B^T - M|| / ||M|| <= 1e-5 from
     dense columns, open / wall / wall with torques, with and without the
     learned blocks;
@@ -411,3 +412,35 @@ def test_sqrt_matches_the_oracle_with_long_coarse_rows():
     print(f"[coarse] N={N}, 3C={3 * C}: GPU {s.lastLanczosIterations()} / oracle {it} iterations, "
           f"rel {rel:.2e}")
     assert rel <= 1e-6
+
+
+def test_factors_on_the_two_by_two_tile_path():
+    """A coarse block of about 94 tiles: the trailing updates of L and of
+    L^{-1} reach the two-by-two-tile kernels (used from 1024 trailing tiles
+    up), and L L^T is the oracle's Q^T M Q, X L = I."""
+    import scipy.sparse as sp
+    from libmobility_amd.workloads import rsa_cube
+    N = 4000
+    pos, _ = rsa_cube(N, 0.2, seed=21)
+    s = _solver(pos, ppc=2)
+    s.sqrtMdotW_with_noise(np.random.default_rng(1).normal(size=3 * N))
+    cell_of, _, C = s.coarseCells()
+    nb = -(-3 * C // 64)
+    ends = list(range(4, nb, 4))  # ends of the blocks of 4 tile columns
+    assert any((nb - e) * (nb - e + 1) // 2 >= 1024 for e in ends)  # L
+    assert any((nb - e) * e >= 1024 for e in ends)  # L^{-1}
+    L, X = s.coarseFactors()
+    M = oracle.dense_matrix(pos, ETA, 1.0) / (1.0 / (6 * np.pi * ETA))
+    counts = np.bincount(cell_of, minlength=C)
+    rows = np.arange(3 * N)
+    cols = 3 * np.repeat(cell_of, 3) + np.tile(np.arange(3), N)
+    Q = sp.csr_matrix((1.0 / np.sqrt(np.repeat(counts[cell_of], 3)), (rows, cols)),
+                      shape=(3 * N, 3 * C))
+    Mc = np.asarray((Q.T @ M) @ Q)
+    del M
+    assert np.allclose(L, np.tril(L))
+    rel = np.linalg.norm(L @ L.T - Mc) / np.linalg.norm(Mc)
+    err = np.abs(X @ L - np.eye(3 * C)).max()
+    print(f"[coarse] C={C}, {nb} tiles: |LL^T - Mc|/|Mc| = {rel:.2e}, max|XL - I| = {err:.2e}")
+    assert rel <= 1e-5
+    assert err <= 1e-11
