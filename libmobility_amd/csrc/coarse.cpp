@@ -70,6 +70,21 @@ void coarse_cells(const double* xyz, int64_t stride, int64_t N, int ppc, CoarseC
   for (int64_t i = 0; i < N; ++i) out.perm[fill[out.cell_of[i]]++] = (int)i;
 }
 
+SideStream::~SideStream() {
+  // Teardown: nothing can act on an error here.  The owner synchronised
+  // with every use of the stream.
+  if (block) (void)hipEventDestroy(block);
+  if (done) (void)hipEventDestroy(done);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void SideStream::ensure() {
+  if (stream) return;
+  LM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  LM_HIP(hipEventCreateWithFlags(&block, hipEventDisableTiming));
+  LM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+}
+
 void CoarseLevel::build(const void* rec64, const void* rec32, const std::vector<int64_t>& off,
                         const std::vector<int>& wall, int ppc_req, hipStream_t s) {
   const auto t_start = std::chrono::steady_clock::now();
@@ -162,11 +177,12 @@ void CoarseLevel::build(const void* rec64, const void* rec32, const std::vector<
   launch_coarse_matrix(sorted_.as<float4>(), start_.as<int>(), scale_.as<double>(),
                        sys_dev_.as<CoarseSys>(), pair_off_.as<int64_t>(), nsys, pair_off[nsys],
                        A_.as<double>(), moff, s);
+  if (!large.empty()) side_.ensure();
   for (int q : large) {
     const CoarseSys& S = sys_[q];
     launch_coarse_factor(A_.as<double>() + S.moff, X_.as<double>() + S.moff,
                          Xt_.as<double>() + S.moff, Dinv_.as<double>() + S.doff, info_.as<int>(),
-                         S.n, S.n, s);
+                         S.n, S.n, s, side_);
   }
   launch_coarse_factor_small(A_.as<double>(), X_.as<double>(), Xt_.as<double>(),
                              Dinv_.as<double>(), info_.as<int>(), sys_dev_.as<CoarseSys>(),

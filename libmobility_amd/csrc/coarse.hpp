@@ -40,12 +40,28 @@ void launch_coarse_gather(const float4* rec, const int* perm, int64_t N, float4*
 void launch_coarse_matrix(const float4* rec, const int* start, const double* scale,
                           const CoarseSys* sys, const int64_t* pair_off, int nsys, int64_t npairs,
                           double* Mc, int64_t total, hipStream_t s);
+// A second stream (default priority, non-blocking) and two events, created
+// on first use with the build's device current and released with their
+// owner.
+class SideStream {
+ public:
+  SideStream() = default;
+  SideStream(const SideStream&) = delete;
+  SideStream& operator=(const SideStream&) = delete;
+  ~SideStream();
+  void ensure();
+  hipStream_t stream = nullptr;
+  hipEvent_t block = nullptr, done = nullptr;
+};
+
 // One matrix, one launch per step: in place A -> L (lower; tiles above the
 // diagonal keep stale values), X = L^{-1}, Xt = X^T, Dinv = inverses of the
 // diagonal tiles; info[0] = 1 if A is not positive definite.  n is a
-// multiple of kCoarseTile.
+// multiple of kCoarseTile.  The steps of X run on side.stream, each block of
+// X behind an event on s that follows the block's columns of L; s waits for
+// the whole of X before it goes on.
 void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* info, int64_t ld,
-                          int n, hipStream_t s);
+                          int n, hipStream_t s, const SideStream& side);
 // The same for the systems list[0..count), one workgroup each (small blocks).
 void launch_coarse_factor_small(double* A, double* X, double* Xt, double* Dinv, int* info,
                                 const CoarseSys* sys, const int* list, int count, hipStream_t s);
@@ -114,6 +130,7 @@ class CoarseLevel {
   int max_row_ = 0;  // largest 3 C_s
   DevBuf perm_, start_, cell_of_, scale_, sorted_, A_, X_, Xt_, Dinv_, info_, cvec_, yvec_;
   DevBuf sys_dev_, row_sys_, pair_off_, small_;
+  SideStream side_;
 };
 
 }  // namespace lm

@@ -752,8 +752,13 @@ void launch_coarse_matrix(const float4* rec, const int* start, const double* sca
 }
 
 void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* info, int64_t ld,
-                          int n, hipStream_t s) {
+                          int n, hipStream_t s, const SideStream& side) {
   const int nb = n / kT;
+  hipStream_t x = side.stream;
+  // X may still be read by work queued on s before this call.
+  LM_HIP(hipEventRecord(side.block, s));
+  LM_HIP(hipStreamWaitEvent(x, side.block, 0));
+  hipLaunchKernelGGL(k_identity, dim3(blocks_for((int64_t)n * n, 256)), dim3(256), 0, x, X, ld, n);
   for (int kb = 0; kb < nb; kb += kKB) {
     const int kend = kb + kKB < nb ? kb + kKB : nb;
     for (int k = kb; k < kend; ++k) {
@@ -764,7 +769,24 @@ void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* i
         hipLaunchKernelGGL(k_chol_local, dim3((unsigned)((kend - k - 1) * below)), dim3(256), 0, s,
                            A, ld, k, kend);
     }
+    // Columns [kb, kend) of L and their diagonal inverses are final here.
+    // Block kb of X reads nothing else of L, and the trailing update of L
+    // below writes only columns >= kend.
+    LM_HIP(hipEventRecord(side.block, s));
+    LM_HIP(hipStreamWaitEvent(x, side.block, 0));
+    for (int k = kb; k < kend; ++k) {
+      hipLaunchKernelGGL(k_trinv_scale, dim3(k + 1), dim3(256), 0, x, X, ld, k, Dinv);
+      if (kend - k - 1 > 0)
+        hipLaunchKernelGGL(k_trinv_local, dim3((unsigned)((kend - k - 1) * (k + 1))), dim3(256), 0,
+                           x, X, A, ld, k);
+    }
     const int T = nb - kend;
+    const int64_t xtiles = (int64_t)T * kend;
+    if (xtiles >= kQuadTiles)
+      hipLaunchKernelGGL(k_trinv_trailing2, dim3((unsigned)((int64_t)((T + 1) / 2) * ((kend + 1) / 2))),
+                         dim3(256), 0, x, X, A, ld, kb, kend, nb);
+    else if (T > 0)
+      hipLaunchKernelGGL(k_trinv_trailing, dim3((unsigned)xtiles), dim3(256), 0, x, X, A, ld, kb, kend);
     const int64_t tiles = (int64_t)T * (T + 1) / 2, T2 = (T + 1) / 2;
     if (tiles >= kQuadTiles)
       hipLaunchKernelGGL(k_chol_trailing2, dim3((unsigned)(T2 * (T2 + 1) / 2)), dim3(256), 0, s, A,
@@ -773,26 +795,10 @@ void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* i
       hipLaunchKernelGGL(k_chol_trailing, dim3((unsigned)tiles), dim3(256), 0, s, A, ld, kb, kend);
   }
   LM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_identity, dim3(blocks_for((int64_t)n * n, 256)), dim3(256), 0, s, X, ld, n);
-  for (int kb = 0; kb < nb; kb += kKB) {
-    const int kend = kb + kKB < nb ? kb + kKB : nb;
-    for (int k = kb; k < kend; ++k) {
-      hipLaunchKernelGGL(k_trinv_scale, dim3(k + 1), dim3(256), 0, s, X, ld, k, Dinv);
-      if (kend - k - 1 > 0)
-        hipLaunchKernelGGL(k_trinv_local, dim3((unsigned)((kend - k - 1) * (k + 1))), dim3(256), 0,
-                           s, X, A, ld, k);
-    }
-    const int T = nb - kend;
-    const int64_t tiles = (int64_t)T * kend;
-    if (tiles >= kQuadTiles)
-      hipLaunchKernelGGL(k_trinv_trailing2, dim3((unsigned)((int64_t)((T + 1) / 2) * ((kend + 1) / 2))),
-                         dim3(256), 0, s, X, A, ld, kb, kend, nb);
-    else if (T > 0)
-      hipLaunchKernelGGL(k_trinv_trailing, dim3((unsigned)tiles), dim3(256), 0, s, X, A, ld, kb, kend);
-  }
+  hipLaunchKernelGGL(k_transpose, dim3(nb, nb), dim3(256), 0, x, X, Xt, ld);
   LM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_transpose, dim3(nb, nb), dim3(256), 0, s, X, Xt, ld);
-  LM_HIP(hipGetLastError());
+  LM_HIP(hipEventRecord(side.done, x));
+  LM_HIP(hipStreamWaitEvent(s, side.done, 0));
 }
 
 void launch_coarse_factor_small(double* A, double* X, double* Xt, double* Dinv, int* info,
