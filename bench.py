@@ -993,6 +993,7 @@ def copy_calibration():
 
 
 def main():
+    global COARSE_PPC
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -1017,10 +1018,9 @@ def main():
     ap.add_argument("--coarse-ppc", type=int, default=COARSE_PPC,
                     help="particles per cell of the preconditioner's coarse level")
     args = ap.parse_args()
-    globals()["COARSE_PPC"] = args.coarse_ppc
+    COARSE_PPC = args.coarse_ppc
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
     if args.tiling:
         args.tiling = tuple(int(x) for x in args.tiling.split(","))
