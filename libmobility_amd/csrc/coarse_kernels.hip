@@ -48,41 +48,65 @@ __device__ inline void tri_index(int64_t p, int& c, int& d) {
   d = (int)(p - cc * (cc + 1) / 2);
 }
 
-// s += sum over targets i in [i0, i1) (this lane's, stride 64) and sources
-// j in [j0, j1) of the 3x3 translational block M_ij / t0, row-major
+// s += the 3x3 translational block M_ij / t0 of the pair (ri, rj),
+// row-major (open: the six entries of the upper triangle)
+template <bool WALL>
+__device__ inline void pair_block_sum(const float4 ri, const float4 rj, float s[9]) {
+  if (WALL) {
+    UFCoef<float> k;
+    uf_coef_wall<float>(ri.x, ri.y, ri.z, ri.w, rj.x, rj.y, rj.z, rj.w, k);
+    const float dv[3] = {k.dx, k.dy, k.dz};
+    const float ev[3] = {k.ex, k.ey, k.ez};
+    const float gv[3] = {k.gx, k.gy, k.gz};
+    const float hv[3] = {k.hx, k.hy, k.hz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        s[3 * a + b] += (a == b ? k.a0 : 0.0f) + k.cd * dv[a] * dv[b] + gv[a] * ev[b] +
+                        (b == 2 ? hv[a] : 0.0f);
+  } else {
+    const float dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    float fuf, guf, fwt, gwt, cc;
+    rpy_coeffs<float, kUF>(r2, fuf, guf, fwt, gwt, cc);
+    s[0] += fuf + guf * dx * dx;
+    s[1] += guf * dx * dy;
+    s[2] += guf * dx * dz;
+    s[4] += fuf + guf * dy * dy;
+    s[5] += guf * dy * dz;
+    s[8] += fuf + guf * dz * dz;
+  }
+}
+
+// s += sum over targets i in [i0, i1) and sources j in [j0, j1) of the
+// pair blocks, this lane's share.  The two cells go through the wave's LDS
+// (sh: 128 records) in chunks of up to 64 x 64 particles, and the 64 lanes
+// walk a chunk's ni x nj pairs in row-major order, lane l taking pairs
+// l, l + 64, ...: every lane works whatever the cell sizes.
 template <bool WALL>
 __device__ inline void cell_pair_sum(const float4* __restrict__ rec, int i0, int i1, int j0,
-                                     int j1, int lane, float s[9]) {
+                                     int j1, int lane, float4* sh, float s[9]) {
   for (int ib = i0; ib < i1; ib += 64) {
-    const int i = ib + lane;
-    if (i >= i1) break;
-    const float4 ri = rec[i];
-    for (int j = j0; j < j1; ++j) {
-      const float4 rj = rec[j];
-      if (WALL) {
-        UFCoef<float> k;
-        uf_coef_wall<float>(ri.x, ri.y, ri.z, ri.w, rj.x, rj.y, rj.z, rj.w, k);
-        const float dv[3] = {k.dx, k.dy, k.dz};
-        const float ev[3] = {k.ex, k.ey, k.ez};
-        const float gv[3] = {k.gx, k.gy, k.gz};
-        const float hv[3] = {k.hx, k.hy, k.hz};
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-          for (int b = 0; b < 3; ++b)
-            s[3 * a + b] += (a == b ? k.a0 : 0.0f) + k.cd * dv[a] * dv[b] + gv[a] * ev[b] +
-                            (b == 2 ? hv[a] : 0.0f);
-      } else {
-        const float dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        float fuf, guf, fwt, gwt, cc;
-        rpy_coeffs<float, kUF>(r2, fuf, guf, fwt, gwt, cc);
-        s[0] += fuf + guf * dx * dx;
-        s[1] += guf * dx * dy;
-        s[2] += guf * dx * dz;
-        s[4] += fuf + guf * dy * dy;
-        s[5] += guf * dy * dz;
-        s[8] += fuf + guf * dz * dz;
+    const int ni = min(64, i1 - ib);
+    for (int jb = j0; jb < j1; jb += 64) {
+      const int nj = min(64, j1 - jb);
+      // The wave's reads of the previous chunk precede these writes in its
+      // LDS order.
+      if (lane < ni) sh[lane] = rec[ib + lane];
+      if (lane < nj) sh[64 + lane] = rec[jb + lane];
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+      __builtin_amdgcn_wave_barrier();
+      const int di = 64 / nj, dj = 64 - di * nj;  // a step of 64 pairs
+      int ii = lane / nj, jj = lane - ii * nj;
+      for (int p = lane; p < ni * nj; p += 64) {
+        pair_block_sum<WALL>(sh[ii], sh[64 + jj], s);
+        ii += di;
+        jj += dj;
+        if (jj >= nj) {
+          jj -= nj;
+          ++ii;
+        }
       }
     }
   }
@@ -93,11 +117,10 @@ __device__ inline void cell_pair_sum(const float4* __restrict__ rec, int i0, int
   }
 }
 
-// One wave per cell pair (c >= d) of one system: lanes take the targets of
-// cell c, every lane walks the sources of cell d, then a butterfly sums the
-// lanes.  The block and its transpose go to the system's matrix (Mc is
-// symmetric because M_ji = M_ij^T).  pair_off[s] counts the cell pairs of the
-// systems before s.
+// One wave per cell pair (c >= d) of one system: the lanes share the pairs
+// of the two cells, then a butterfly sums the lanes.  The block and its
+// transpose go to the system's matrix (Mc is symmetric because
+// M_ji = M_ij^T).  pair_off[s] counts the cell pairs of the systems before s.
 __global__ __launch_bounds__(256) void k_coarse_matrix(const float4* __restrict__ rec,
                                                        const int* __restrict__ start,
                                                        const double* __restrict__ scale,
@@ -105,6 +128,7 @@ __global__ __launch_bounds__(256) void k_coarse_matrix(const float4* __restrict_
                                                        const int64_t* __restrict__ pair_off,
                                                        int nsys, int64_t npairs,
                                                        double* __restrict__ Mc) {
+  __shared__ float4 sh[4][128];
   const int lane = threadIdx.x & 63;
   const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (p >= npairs) return;  // uniform per wave
@@ -120,10 +144,11 @@ __global__ __launch_bounds__(256) void k_coarse_matrix(const float4* __restrict_
   float s[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) s[k] = 0.0f;
+  float4* w = sh[threadIdx.x >> 6];
   if (S.wall)
-    cell_pair_sum<true>(rec, start[gc], start[gc + 1], start[gd], start[gd + 1], lane, s);
+    cell_pair_sum<true>(rec, start[gc], start[gc + 1], start[gd], start[gd + 1], lane, w, s);
   else
-    cell_pair_sum<false>(rec, start[gc], start[gc + 1], start[gd], start[gd + 1], lane, s);
+    cell_pair_sum<false>(rec, start[gc], start[gc + 1], start[gd], start[gd + 1], lane, w, s);
   double v[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
