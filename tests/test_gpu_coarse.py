@@ -59,14 +59,17 @@ def _records_xyz(pos, wall):
 
 
 @pytest.mark.parametrize("wall", [False, True])
-def test_cells_and_factors_match_the_oracle(wall):
+@pytest.mark.parametrize("ppc", [24, 100])
+def test_cells_and_factors_match_the_oracle(wall, ppc):
+    """ppc 100: cells of more than 64 particles, which the coarse matrix
+    kernel takes in several 64 x 64 chunks."""
     from libmobility_amd.workloads import rsa_cube
     N = 1500
     pos = _wall_positions(N, 3) if wall else rsa_cube(N, 0.2, seed=3)[0]
-    s = _solver(pos, wall=wall, ppc=24)
+    s = _solver(pos, wall=wall, ppc=ppc)
     s.sqrtMdotW_with_noise(np.random.default_rng(0).normal(size=3 * N))
     cell_of, grid, C = s.coarseCells()
-    ref, g, Cr = coarse.coarse_cells(_records_xyz(pos, wall), 24)
+    ref, g, Cr = coarse.coarse_cells(_records_xyz(pos, wall), ppc)
     assert C == Cr and list(grid) == list(g)
     assert np.array_equal(cell_of, ref)
     L, X = s.coarseFactors()
@@ -74,7 +77,9 @@ def test_cells_and_factors_match_the_oracle(wall):
     Mc = coarse.coarse_matrix(M, cell_of, C)
     assert np.allclose(L, np.tril(L))
     rel = np.linalg.norm(L @ L.T - Mc) / np.linalg.norm(Mc)
-    print(f"[coarse] wall={wall} C={C} |LL^T - Mc|/|Mc| = {rel:.2e}")
+    if ppc == 100:
+        assert np.bincount(cell_of).max() > 64
+    print(f"[coarse] wall={wall} ppc={ppc} C={C} |LL^T - Mc|/|Mc| = {rel:.2e}")
     assert rel <= 1e-5
     assert np.abs(X @ L - np.eye(3 * C)).max() <= 1e-11
     assert s.coarseBuildMs() > 0.0
