@@ -42,6 +42,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from libmobility_amd import workloads  # noqa: E402  (numpy generators only)
+
 METRIC = "Mdot Gpair/s + sqrtMdotW ms at N=131072 tol 1e-6: learned-prec vs plain"
 # coarse level of the preconditioned sqrtMdotW: target particles per cell
 COARSE_PPC = 40
@@ -336,15 +338,11 @@ def multi_vector_leg(g, h, F_d, W_d, N, ms_single, extra, args, nrhs=4):
 
 
 def bench_c3(args):
-    from libmobility_amd.workloads import rsa_cube
     g = Gpu()
     N = args.N or 131072
     phi = 0.20
     log(f"generating RSA packing N={N} phi={phi}")
-    pos, L = rsa_cube(N, phi, seed=2026)
-    rng = np.random.default_rng(7)
-    F = rng.normal(size=(N, 3))
-    Wn = rng.normal(size=3 * N)
+    pos, L, F, Wn = workloads.config_c3(N, phi)
     pos_d, F_d, W_d, U_d = g.put(pos), g.put(F), g.put(Wn), g.zeros(3 * N)
     fpp = flops_per_pair()["open_UF"]
     pairs = float(N) * float(N)
@@ -547,10 +545,7 @@ def cpu_leg_c1(pos, F, Wn):
 def bench_c1(args):
     g = Gpu()
     N = args.N or 1024
-    rng = np.random.default_rng(0)
-    pos = rng.uniform(0, 24.0, (N, 3))
-    F = rng.normal(size=(N, 3))
-    Wn = rng.normal(size=3 * N)
+    pos, F, Wn = workloads.config_c1(N)
     pos_d, F_d, W_d, U_d = g.put(pos), g.put(F), g.put(Wn), g.zeros(3 * N)
     h = g.solver(pos_d, "fp64", tiling=args.tiling)
     ms_mdot = g.time_events(lambda: g.mdot(h, F_d, None, U_d), args.steps, args.warmup)
@@ -593,23 +588,12 @@ def bench_c1(args):
 # C2: single wall with torques, N = 65536
 
 
-def wall_positions(N, Lxy, zlo, zhi, seed):
-    rng = np.random.default_rng(seed)
-    pos = np.empty((N, 3))
-    pos[:, :2] = rng.uniform(0.0, Lxy, (N, 2))
-    pos[:, 2] = rng.uniform(zlo, zhi, N)
-    return pos
-
-
 def bench_c2(args):
     g = Gpu()
     N = args.N or 65536
-    pos = wall_positions(N, 256.0, 1.5, 20.0, seed=65536)
-    rng = np.random.default_rng(8)
-    F = rng.normal(size=(N, 3))
-    T = rng.normal(size=(N, 3))
+    pos, F, T, Wn = workloads.config_c2(N)
     pos_d, F_d, T_d = g.put(pos), g.put(F), g.put(T)
-    Wn_d = g.put(rng.normal(size=6 * N))
+    Wn_d = g.put(Wn)
     U_d, W_d = g.zeros(3 * N), g.zeros(3 * N)
     fpp = flops_per_pair()["wall_6x6"]
     pairs = float(N) ** 2
@@ -740,15 +724,11 @@ def cpu_leg_c2(pos, F, T, rows):
 
 
 def bench_c4(args):
-    from libmobility_amd.workloads import ensemble_c4
     g = Gpu()
     log("generating the 512-system ensemble")
-    sizes, kinds, pos = ensemble_c4()
+    sizes, kinds, pos, F, Wn = workloads.config_c4()
     N = int(sizes.sum())
     off = np.concatenate([[0], np.cumsum(sizes)])
-    rng = np.random.default_rng(9)
-    F = rng.normal(size=(N, 3))
-    Wn = rng.normal(size=3 * N)
     pos_d, F_d, W_d = g.put(pos), g.put(F), g.put(Wn)
     U_d = g.zeros(3 * N)
     fp = flops_per_pair()
@@ -884,11 +864,7 @@ def bench_c4(args):
 def bench_c5(args):
     g = Gpu()
     N = args.N or 1048576
-    Lxy = 1024.0 * math.sqrt(N / 1048576.0)
-    pos = wall_positions(N, Lxy, 1.5, 30.0, seed=1048576)
-    rng = np.random.default_rng(10)
-    F = rng.normal(size=(N, 3))
-    T = rng.normal(size=(N, 3))
+    pos, F, T, Lxy = workloads.config_c5(N)
     pos_d, F_d, T_d = g.put(pos), g.put(F), g.put(T)
     U_d, W_d = g.zeros(3 * N), g.zeros(3 * N)
     del pos, F, T
@@ -958,29 +934,6 @@ def bench_c5(args):
                        compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS), extra, None)
 
 
-def valu_ceiling():
-    """Measured VALU issue ceiling (independent FMA chains) and the cost of a
-    rsqrt in FMA slots; one JSON line."""
-    import torch  # noqa: F401  (one HIP runtime with torch)
-    from libmobility_amd import _ffi
-    lib = _ffi.load()
-    blocks, iters = 256 * 32, 4096
-    waves = blocks * 4
-    res = {}
-    for name, kind in (("f32_fma", 0), ("f32_fma_rsqrt", 1), ("f64_fma", 2)):
-        ms = ctypes.c_float()
-        for _ in range(2):  # first launch warms up clocks and code
-            _ffi.check(lib.lm_diag_valu_throughput(kind, blocks, iters, ctypes.byref(ms)))
-        res[name + "_ms"] = ms.value
-    per_s = lambda ms, n: waves * iters * n / (ms * 1e-3)
-    res["f32_fma_wave_instr_per_s"] = per_s(res["f32_fma_ms"], 24)
-    res["f32_fma_tflops"] = 2 * 64 * res["f32_fma_wave_instr_per_s"] / 1e12
-    res["f64_fma_tflops"] = 2 * 64 * per_s(res["f64_fma_ms"], 24) / 1e12
-    # 24 FMA + 8 add + 8 rsqrt per iteration: rsqrt cost in FMA slots
-    res["rsqrt_fma_slots"] = (res["f32_fma_rsqrt_ms"] / res["f32_fma_ms"] * 24 - 32) / 8
-    print(json.dumps(res), flush=True)
-
-
 def copy_calibration():
     import torch
     n = 1 << 28  # doubles: 2 GiB read + 2 GiB written per copy
@@ -1011,8 +964,6 @@ def main():
                     help="c3: skip the several-vectors-per-pass leg")
     ap.add_argument("--no-bbt", action="store_true",
                     help="c3: skip the small-N ||BB^T - M|| check (thousands of small solves)")
-    ap.add_argument("--valu-ceiling", action="store_true",
-                    help="only measure the VALU issue ceiling (FMA chains, rsqrt cost)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device-to-device copy (reference for the HBM byte counters)")
     ap.add_argument("--coarse-ppc", type=int, default=COARSE_PPC,
@@ -1026,9 +977,6 @@ def main():
         args.tiling = tuple(int(x) for x in args.tiling.split(","))
     if args.copy_calibration:
         copy_calibration()
-        return
-    if args.valu_ceiling:
-        valu_ceiling()
         return
     res = {"c3": bench_c3, "c1": bench_c1, "c2": bench_c2, "c4": bench_c4,
            "c5": bench_c5}[args.workload](args)

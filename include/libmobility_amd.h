@@ -81,6 +81,15 @@ int lm_nbody_set_parameters(lm_nbody* h, int algorithm, int nbatch, int nperbatc
 /* Extension: pair arithmetic (LM_PAIR_FP64 default, LM_PAIR_FP32). */
 int lm_nbody_set_pair_precision(lm_nbody* h, int precision);
 
+/* Extension: pair arithmetic of the thermal drift's two products
+ * (NBody::divM, mobility.h:184-236).  Default LM_PAIR_FP64 whatever
+ * lm_nbody_set_pair_precision says: the random finite difference divides a
+ * difference of two products by delta*a (random_finite_differences.h:54-62),
+ * so FP32 pair terms amplify their ~1e-7 rounding by 1/delta, and FP32
+ * positions far from the system centre cannot represent a displacement of
+ * delta*a/2 at all. */
+int lm_nbody_set_drift_precision(lm_nbody* h, int precision);
+
 /* Extension: pair-kernel tiling.  targets_per_lane in {0 (auto), 1, 2, 4};
  * source_chunks >= 0 (0 = auto).  Fixes the work decomposition (the
  * reference's blockDim choice, NbodyRPY.cu:105-108). */
@@ -196,6 +205,13 @@ int lm_nbody_sqrt_mdot_w_noise(lm_nbody* h, const double* noise, double* linear,
 /* NBody::divM (random finite differences), mobility.h:184-236:
  * outputs += prefactor * div M; no-op for open boundaries. */
 int lm_nbody_div_m(lm_nbody* h, double* linear, double* angular, double prefactor, int mem);
+
+/* Extension for parity tests: as lm_nbody_div_m with the random displacement
+ * W (3N values, random_finite_differences.h:46-52) given instead of drawn;
+ * W is used both as displacement and as force, the entries of open systems
+ * of a mixed ensemble are ignored.  The solver's generator is not advanced. */
+int lm_nbody_div_m_noise(lm_nbody* h, const double* noise, double* linear, double* angular,
+                         double prefactor, int mem);
 
 /* Mobility::LangevinVelocities, MobilityInterface.h:191-206:
  * Mdot (when forces/torques given) + sqrt(2 kbt/dt) M^{1/2} W + kbt div M. */
@@ -343,13 +359,6 @@ int lm_self_clean(lm_self* h);
  * and (j, i) (hydrodynamicKernels.cuh:20-442). */
 int lm_selftest_pair_both_host(int wall, const double* pi, const double* pj, double a,
                                double* fwd36, double* rev36);
-
-/* Diagnostic (no counterpart in the reference): time `blocks` workgroups of
- * 256 threads running `iters` iterations of 8 independent chains of 3 FMAs
- * (kind 0: FP32; 1: FP32 with one rsqrt per chain step; 2: FP64) on the null
- * stream; *ms receives the elapsed milliseconds.  Gives the VALU issue
- * ceiling the pair kernel is compared with. */
-int lm_diag_valu_throughput(int kind, int blocks, int iters, float* ms);
 
 /* Philox4x32-10 of the noise kernels: out = philox(ctr, key). */
 int lm_selftest_philox_host(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4);

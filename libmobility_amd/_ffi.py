@@ -34,6 +34,7 @@ SIGNATURES = {
     "lm_nbody_set_parameters": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "lm_nbody_set_pair_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lm_nbody_set_drift_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lm_nbody_set_stream": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_set_pair_tiling": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "lm_nbody_set_pair_symmetric": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -60,6 +61,7 @@ SIGNATURES = {
     "lm_nbody_sqrt_mdot_w": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
     "lm_nbody_sqrt_mdot_w_noise": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
     "lm_nbody_div_m": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
+    "lm_nbody_div_m_noise": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
     "lm_nbody_langevin_velocities": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _i64,
                                                     _vp, _i64, _vp, _vp, ctypes.c_int]),
     "lm_nbody_hydrodynamic_velocities": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, ctypes.c_int]),
@@ -106,8 +108,6 @@ SIGNATURES = {
     "lm_self_clean": (ctypes.c_int, [_vp]),
     "lm_selftest_pair_both_host": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_double, _dp,
                                                   _dp]),
-    "lm_diag_valu_throughput": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                                ctypes.POINTER(ctypes.c_float)]),
     "lm_selftest_philox_host": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32),
                                                ctypes.POINTER(ctypes.c_uint32),
                                                ctypes.POINTER(ctypes.c_uint32)]),

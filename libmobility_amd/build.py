@@ -13,7 +13,7 @@ OUT_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT_DIR, "liblmamd.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["nbody_kernels.hip", "lanczos_kernels.hip", "precond_kernels.hip", "diag_kernels.hip",
+SOURCES = ["nbody_kernels.hip", "lanczos_kernels.hip", "precond_kernels.hip",
            "coarse_kernels.hip", "coarse.cpp", "precond.cpp", "solver.cpp", "self_solver.cpp", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -87,6 +87,10 @@ int lm_nbody_set_pair_precision(lm_nbody* h, int precision) {
   return guarded([&] { S(h).setPairPrecision(precision); });
 }
 
+int lm_nbody_set_drift_precision(lm_nbody* h, int precision) {
+  return guarded([&] { S(h).setDriftPrecision(precision); });
+}
+
 int lm_nbody_set_pair_tiling(lm_nbody* h, int tpl, int nch) {
   return guarded([&] { S(h).setPairTiling(tpl, nch); });
 }
@@ -201,6 +205,14 @@ int lm_nbody_sqrt_mdot_w_noise(lm_nbody* h, const double* noise, double* U, doub
 
 int lm_nbody_div_m(lm_nbody* h, double* U, double* W, double prefactor, int mem) {
   return guarded([&] { S(h).divM(U, W, prefactor, is_dev(mem)); });
+}
+
+int lm_nbody_div_m_noise(lm_nbody* h, const double* noise, double* U, double* W,
+                         double prefactor, int mem) {
+  return guarded([&] {
+    if (!noise) throw lm::Error("[libmobility_amd] null noise");
+    S(h).divM(U, W, prefactor, is_dev(mem), noise);
+  });
 }
 
 int lm_nbody_langevin_velocities(lm_nbody* h, double dt, double kbt, const double* F, int64_t nF,
@@ -511,13 +523,6 @@ int lm_self_langevin_velocities(lm_self* h, double dt, double kbt, const double*
 
 int lm_self_clean(lm_self* h) {
   return guarded([&] { SS(h).clean(); });
-}
-
-int lm_diag_valu_throughput(int kind, int blocks, int iters, float* ms) {
-  return guarded([&] {
-    if (!ms) throw lm::Error("[libmobility_amd] null output");
-    *ms = lm::diag_valu_ms(kind, blocks, iters, nullptr);
-  });
 }
 
 int lm_selftest_philox_host(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4) {

@@ -262,7 +262,5 @@ void launch_seg_chunk_reduce(const double* part, int nchunks, int64_t n, double*
                              const SegBlock* blocks, int nblocks, const int* active,
                              hipStream_t s);
 
-// Diagnostic VALU throughput probe (see diag_kernels.hip): elapsed ms.
-float diag_valu_ms(int kind, int blocks, int iters, hipStream_t s);
 
 }  // namespace lm

@@ -63,6 +63,11 @@ void NBodySolver::setPairPrecision(int p) {
   items_tpl_ = -1;
 }
 
+void NBodySolver::setDriftPrecision(int p) {
+  if (p != kPairFP64 && p != kPairFP32) throw Error("[libmobility_amd] Invalid pair precision");
+  drift_prec_ = p;
+}
+
 void NBodySolver::initialize(double viscosity, double radius, bool include_angular,
                              double tolerance, uint64_t seed, LanczosCallback cb, void* cb_user) {
   if (initialized_) clean();  // MobilityInterface.h:104-105
@@ -788,7 +793,7 @@ void NBodySolver::choose_tiling(int blocks, bool fp32, const double* F, const do
 
 // U (3N) and, when angular, W (3N) are overwritten (NbodyRPY.cu:91-96).
 void NBodySolver::pair_product(const void* rec64, const void* rec32, const double* F,
-                               const double* T, double* U, double* W) {
+                               const double* T, double* U, double* W, int prec) {
   int blocks = 0;
   // Block selection of NbodyRPY.cu:264-275.
   if (F) blocks |= kUF | (angular_ ? kWF : 0);
@@ -803,7 +808,7 @@ void NBodySolver::pair_product(const void* rec64, const void* rec32, const doubl
   const bool ang = (blocks & (kWT | kWF)) != 0;
   if (ang && !W)
     throw Error("[libmobility_amd] the solver includes angular velocities: an angular output array is required");
-  const bool fp32 = pair_prec_ == kPairFP32;
+  const bool fp32 = (prec < 0 ? pair_prec_ : prec) == kPairFP32;
   const void* rec = fp32 ? rec32 : rec64;
   last_sym_ = false;
   if (use_sym(blocks)) {
@@ -1654,11 +1659,10 @@ void NBodySolver::precondBlocks(double* G9) {
 
 // ---- random finite differences -------------------------------------------------------
 
-void NBodySolver::divM_dev(double* U, double* W, double prefactor) {
+void NBodySolver::divM_dev(double* U, double* W, double prefactor, const double* given) {
   const int64_t N = N_, n3 = 3 * N;
   ensure_records();
-  const uint32_t seed = rng_nbody_();  // mobility.h:218
-  const double delta = delta_ * a_;    // mobility.h:224
+  const double delta = delta_ * a_;  // mobility.h:224
   rfd_noise_.reserve(sizeof(double) * n3);
   rfd_pos_.reserve(sizeof(double) * n3);
   rfd_rec64_.reserve(sizeof(double) * 4 * N);
@@ -1670,7 +1674,12 @@ void NBodySolver::divM_dev(double* U, double* W, double prefactor) {
     rfd_wm_.reserve(sizeof(double) * n3);
   }
   double* noise = rfd_noise_.as<double>();
-  launch_normal_fill(noise, n3, seed, 1, stream_);
+  if (given) {
+    LM_HIP(hipMemcpyAsync(noise, given, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_));
+  } else {
+    const uint32_t seed = rng_nbody_();  // mobility.h:218
+    launch_normal_fill(noise, n3, seed, 1, stream_);
+  }
   const int nsys = (int)sys_off_host_.size() - 1;
   // open systems of a mixed ensemble have no drift (mobility.h:189-190):
   // no displacement and no force there
@@ -1684,7 +1693,7 @@ void NBodySolver::divM_dev(double* U, double* W, double prefactor) {
                       kinds_dev(), wall_ ? 1 : 0, 0.0, a_, nullptr, rfd_rec64_.as<void>(),
                       rfd_rec32_.as<void>(), stream_);
   pair_product(rfd_rec64_.as<void>(), rfd_rec32_.as<void>(), noise, nullptr, rfd_mp_.as<double>(),
-               angular_ ? rfd_wp_.as<double>() : nullptr);
+               angular_ ? rfd_wp_.as<double>() : nullptr, drift_prec_);
   // q - (delta/2) W  (:50-52)
   launch_axpy_to(rfd_pos_.as<double>(), shifted_.as<double>(), noise, -0.5 * delta, n3, stream_);
   launch_system_bbox(rfd_pos_.as<double>(), sys_off_.as<int64_t>(), nsys, 0.0, kinds_dev(),
@@ -1693,14 +1702,14 @@ void NBodySolver::divM_dev(double* U, double* W, double prefactor) {
                       kinds_dev(), wall_ ? 1 : 0, 0.0, a_, nullptr, rfd_rec64_.as<void>(),
                       rfd_rec32_.as<void>(), stream_);
   pair_product(rfd_rec64_.as<void>(), rfd_rec32_.as<void>(), noise, nullptr, rfd_mm_.as<double>(),
-               angular_ ? rfd_wm_.as<double>() : nullptr);
+               angular_ ? rfd_wm_.as<double>() : nullptr, drift_prec_);
   // out += (prefactor/delta) (M+ W - M- W)  (:54-62, mobility.h:227-235)
   launch_diff_accumulate(U, rfd_mp_.as<double>(), rfd_mm_.as<double>(), prefactor / delta, n3, stream_);
   if (angular_ && W)
     launch_diff_accumulate(W, rfd_wp_.as<double>(), rfd_wm_.as<double>(), prefactor / delta, n3, stream_);
 }
 
-void NBodySolver::divM(double* U, double* W, double prefactor, bool dev) {
+void NBodySolver::divM(double* U, double* W, double prefactor, bool dev, const double* noise) {
   if (prefactor == 0) return;  // mobility.h:186-188
   if (!any_wall_) return;      // open: no thermal drift (mobility.h:189-190)
   require_positions(
@@ -1708,7 +1717,7 @@ void NBodySolver::divM(double* U, double* W, double prefactor, bool dev) {
   if (!U) throw Error("[libMobility] The number of forces does not match the number of particles");
   const int64_t n3 = 3 * N_;
   if (dev) {
-    divM_dev(U, W, prefactor);
+    divM_dev(U, W, prefactor, noise);
     LM_HIP(hipStreamSynchronize(stream_));
     return;
   }
@@ -1717,7 +1726,13 @@ void NBodySolver::divM(double* U, double* W, double prefactor, bool dev) {
   LM_HIP(hipMemcpyAsync(st_u_.as<double>(), U, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
   if (angular_ && W)
     LM_HIP(hipMemcpyAsync(st_w_.as<double>(), W, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
-  divM_dev(st_u_.as<double>(), (angular_ && W) ? st_w_.as<double>() : nullptr, prefactor);
+  const double* dnoise = nullptr;
+  if (noise) {
+    st_f_.reserve(sizeof(double) * n3);
+    LM_HIP(hipMemcpyAsync(st_f_.as<double>(), noise, sizeof(double) * n3, hipMemcpyHostToDevice, stream_));
+    dnoise = st_f_.as<double>();
+  }
+  divM_dev(st_u_.as<double>(), (angular_ && W) ? st_w_.as<double>() : nullptr, prefactor, dnoise);
   LM_HIP(hipMemcpyAsync(U, st_u_.as<double>(), sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));
   if (angular_ && W)
     LM_HIP(hipMemcpyAsync(W, st_w_.as<double>(), sizeof(double) * n3, hipMemcpyDeviceToHost, stream_));

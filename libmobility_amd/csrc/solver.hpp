@@ -37,6 +37,11 @@ class NBodySolver {
   void setParameters(int algorithm, int nbatch, int nperbatch, bool has_wall_height,
                      double wall_height, double delta);
   void setPairPrecision(int p);
+  // Pair arithmetic of the thermal drift's two products (default FP64
+  // whatever the pair precision: the drift is a difference divided by the
+  // step, and FP32 positions far from the system centre cannot resolve a
+  // step of 1e-4 a).
+  void setDriftPrecision(int p);
   void setStream(hipStream_t s) { stream_ = s; }
   hipStream_t stream() const { return stream_; }
   void initialize(double viscosity, double radius, bool include_angular, double tolerance,
@@ -78,7 +83,8 @@ class NBodySolver {
   void Mdot(const double* F, int64_t nF, const double* T, int64_t nT, double* U, double* W,
             bool dev);
   void sqrtMdotW(double* U, double* W, double prefactor, bool dev, const double* noise);
-  void divM(double* U, double* W, double prefactor, bool dev);
+  // noise: the 3N random displacement W (nullptr: drawn as the reference does)
+  void divM(double* U, double* W, double prefactor, bool dev, const double* noise = nullptr);
   void langevin(double dt, double kbt, const double* F, int64_t nF, const double* T, int64_t nT,
                 double* U, double* W, bool dev);
   void clean();
@@ -135,7 +141,7 @@ class NBodySolver {
   // algorithm needs).
   void mdot_dev(const double* F, const double* T, double* U, double* W);
   void sqrtMdotW_dev(double* U, double* W, double prefactor, const double* noise);
-  void divM_dev(double* U, double* W, double prefactor);
+  void divM_dev(double* U, double* W, double prefactor, const double* noise = nullptr);
 
  private:
   // State of a set of independent recurrences over a stacked layout: the
@@ -169,8 +175,9 @@ class NBodySolver {
     launch_pair_items(items_.as<PairItem>(), items_count_, items_tpl_, items_nch_, blocks, fp32,
                       rec, F, T, U, W);
   }
+  // prec: kPairFP64 / kPairFP32, or -1 for the solver's pair precision
   void pair_product(const void* rec64, const void* rec32, const double* F, const double* T,
-                    double* U, double* W);
+                    double* U, double* W, int prec = -1);
   int lanczos_run(const double* z, double** result);
   int lanczos_run_seg(SegSolve& S, const std::vector<int64_t>& off, bool angular,
                       bool skip_converged_pairs,
@@ -196,6 +203,7 @@ class NBodySolver {
   double wall_height_ = 0.0;
   double delta_ = 1e-3;
   int pair_prec_ = kPairFP64;
+  int drift_prec_ = kPairFP64;
   hipStream_t stream_ = nullptr;
 
   // initialize()

@@ -133,6 +133,20 @@ class _Solver:
         return getattr(self._lib, "lm_" + self._prefix + "_" + name)
 
     # -- helpers ------------------------------------------------------------------
+    def _out(self, x, nvec=1):
+        """Pointer of an output array after checking that it holds exactly the
+        nvec * 3N values the library writes into it (a smaller buffer would be
+        overrun).  Mdot and LangevinVelocities need no such check: the library
+        refuses forces and torques of another size than 3N before writing."""
+        if x is None:
+            return None
+        n = nvec * 3 * self.getNumberParticles()
+        size = x.numel() if _is_torch(x) else x.size
+        if size != n:
+            raise RuntimeError(f"[libmobility_amd] output array holds {size} values, the solver "
+                               f"writes {n}")
+        return _ptr(x)
+
     def _bind_stream(self, mem, device):
         if mem != _ffi.LM_MEM_DEVICE:
             return
@@ -214,7 +228,7 @@ class _Solver:
         """pythonify.h:170-185 -> Mobility::sqrtMdotW (MobilityInterface.h:132-188)."""
         linear, angular, mem = self._outputs_like_positions()
         self._bind_stream(mem, _last_device)
-        _ffi.check(self._f("sqrt_mdot_w")(self._h, _ptr(linear), _ptr(angular),
+        _ffi.check(self._f("sqrt_mdot_w")(self._h, self._out(linear), self._out(angular),
                                                   float(prefactor), mem))
         return linear, angular
 
@@ -226,7 +240,7 @@ class _Solver:
                 "setPositions?")
         linear, angular, mem = self._outputs_like_positions()
         self._bind_stream(mem, _last_device)
-        _ffi.check(self._f("div_m")(self._h, _ptr(linear), _ptr(angular), float(prefactor),
+        _ffi.check(self._f("div_m")(self._h, self._out(linear), self._out(angular), float(prefactor),
                                             mem))
         return linear, angular
 
@@ -308,6 +322,13 @@ class NBody(_Solver):
         if precision not in _ffi.PAIR_PRECISION:
             raise RuntimeError("[libmobility_amd] pair precision must be 'fp64' or 'fp32'")
         _ffi.check(self._lib.lm_nbody_set_pair_precision(self._h, _ffi.PAIR_PRECISION[precision]))
+
+    def setDriftPrecision(self, precision="fp64"):
+        """Extension: pair arithmetic of divM's two products, 'fp64' (default,
+        whatever the pair precision) or 'fp32'."""
+        if precision not in _ffi.PAIR_PRECISION:
+            raise RuntimeError("[libmobility_amd] pair precision must be 'fp64' or 'fp32'")
+        _ffi.check(self._lib.lm_nbody_set_drift_precision(self._h, _ffi.PAIR_PRECISION[precision]))
 
     def setSystems(self, sizes, kinds=None):
         """Extension: independent systems of the given (ragged) sizes.
@@ -482,8 +503,25 @@ class NBody(_Solver):
         if w.size != n:
             raise RuntimeError(f"[libmobility_amd] noise must have {n} entries")
         self._bind_stream(mem, _last_device)
-        _ffi.check(self._lib.lm_nbody_sqrt_mdot_w_noise(self._h, w.ptr, _ptr(linear), _ptr(angular),
+        _ffi.check(self._lib.lm_nbody_sqrt_mdot_w_noise(self._h, w.ptr, self._out(linear), self._out(angular),
                                                         float(prefactor), mem))
+        return linear, angular
+
+    def divM_with_noise(self, noise, prefactor=1.0):
+        """Extension: the random finite difference of divM for a given
+        displacement W (3N values, random_finite_differences.h:46-62)."""
+        if self.getNumberParticles() <= 0:
+            raise RuntimeError(
+                "[libMobility] The number of particles is not set. Did you forget to call "
+                "setPositions?")
+        linear, angular, mem = self._outputs_like_positions()
+        w = _Arr(noise).moved_to(mem, _last_device)
+        n = 3 * self.getNumberParticles()
+        if w.size != n:
+            raise RuntimeError(f"[libmobility_amd] noise must have {n} entries")
+        self._bind_stream(mem, _last_device)
+        _ffi.check(self._lib.lm_nbody_div_m_noise(self._h, w.ptr, self._out(linear), self._out(angular),
+                                                  float(prefactor), mem))
         return linear, angular
 
     # -- several vectors per pass (extension) ----------------------------------------
@@ -501,7 +539,7 @@ class NBody(_Solver):
         f, nrhs = self._stacked(forces, "forces")
         out = _zeros(f.shape, f.framework, f.device)
         self._bind_stream(f.mem, f.device)
-        _ffi.check(self._lib.lm_nbody_mdot_multi(self._h, f.ptr, nrhs, _ptr(out), f.mem))
+        _ffi.check(self._lib.lm_nbody_mdot_multi(self._h, f.ptr, nrhs, self._out(out, nrhs), f.mem))
         return out
 
     def sqrtMdotWMulti(self, nrhs, prefactor=1.0):
@@ -512,7 +550,7 @@ class NBody(_Solver):
         out = _zeros(shape, _last_framework, _last_device)
         mem = _mem_of(_last_framework, _last_device)
         self._bind_stream(mem, _last_device)
-        _ffi.check(self._lib.lm_nbody_sqrt_mdot_w_multi(self._h, None, int(nrhs), _ptr(out),
+        _ffi.check(self._lib.lm_nbody_sqrt_mdot_w_multi(self._h, None, int(nrhs), self._out(out, int(nrhs)),
                                                         float(prefactor), mem))
         return out
 
@@ -521,7 +559,7 @@ class NBody(_Solver):
         w, nrhs = self._stacked(noise, "noise")
         out = _zeros(w.shape, w.framework, w.device)
         self._bind_stream(w.mem, w.device)
-        _ffi.check(self._lib.lm_nbody_sqrt_mdot_w_multi(self._h, w.ptr, nrhs, _ptr(out),
+        _ffi.check(self._lib.lm_nbody_sqrt_mdot_w_multi(self._h, w.ptr, nrhs, self._out(out, nrhs),
                                                         float(prefactor), w.mem))
         return out
 

@@ -8,6 +8,9 @@ tests to build inputs of the documented shapes:
                           of radius a in a cube sized for volume fraction phi
   * ``rsa_box``           the same in a rectangular box
   * ``ensemble_c4``       512 ragged systems, half open, half above a wall
+
+and the five benchmark configurations with their seeds (``config_c1`` ...
+``config_c5``), shared by bench.py and the full-size tests.
 """
 import math
 
@@ -21,8 +24,7 @@ def uniform_cube(N, L, seed):
 def wall_slab(N, Lxy, zmin, zmax, seed):
     rng = np.random.default_rng(seed)
     pos = np.empty((N, 3))
-    pos[:, 0] = rng.uniform(0.0, Lxy, N)
-    pos[:, 1] = rng.uniform(0.0, Lxy, N)
+    pos[:, :2] = rng.uniform(0.0, Lxy, (N, 2))
     pos[:, 2] = rng.uniform(zmin, zmax, N)
     return pos
 
@@ -123,3 +125,51 @@ def ensemble_c4(nsys=512, nmin=64, nmax=8192, seed=2024, a=1.0):
             side = vol ** (1.0 / 3.0)
             pos.append(rsa_box(int(n), (side, side, side), seed * 7 + s, a=a))
     return sizes, kinds, np.concatenate(pos)
+
+
+# ---- the five benchmark configurations (a = 1, seeds fixed) --------------------
+
+
+def config_c1(N=1024):
+    """Open RPY, N i.i.d. uniform in a cube of side 24a (overlaps kept); forces
+    and noise N(0,1).  Returns (positions, forces, noise)."""
+    rng = np.random.default_rng(0)
+    pos = rng.uniform(0, 24.0, (N, 3))
+    return pos, rng.normal(size=(N, 3)), rng.normal(size=3 * N)
+
+
+def config_c2(N=65536):
+    """Single wall, x, y uniform in [0, 256a), z uniform in [1.5a, 20a]; forces
+    and torques N(0,1).  Returns (positions, forces, torques, noise6N)."""
+    pos = wall_slab(N, 256.0, 1.5, 20.0, seed=65536)
+    rng = np.random.default_rng(8)
+    F = rng.normal(size=(N, 3))
+    T = rng.normal(size=(N, 3))
+    return pos, F, T, rng.normal(size=6 * N)
+
+
+def config_c3(N=131072, phi=0.20):
+    """Open RPY, RSA packing at volume fraction phi in a cube; forces and noise
+    N(0,1).  Returns (positions, side, forces, noise)."""
+    pos, L = rsa_cube(N, phi, seed=2026)
+    rng = np.random.default_rng(7)
+    return pos, L, rng.normal(size=(N, 3)), rng.normal(size=3 * N)
+
+
+def config_c4():
+    """512 ragged systems (``ensemble_c4``); forces and noise N(0,1).  Returns
+    (sizes, kinds, positions, forces, noise)."""
+    sizes, kinds, pos = ensemble_c4()
+    N = int(sizes.sum())
+    rng = np.random.default_rng(9)
+    return sizes, kinds, pos, rng.normal(size=(N, 3)), rng.normal(size=3 * N)
+
+
+def config_c5(N=1048576):
+    """Single wall, x, y uniform in [0, 1024a sqrt(N/2^20)), z uniform in
+    [1.5a, 30a]; forces and torques N(0,1).  Returns (positions, forces,
+    torques, Lxy)."""
+    Lxy = 1024.0 * math.sqrt(N / 1048576.0)
+    pos = wall_slab(N, Lxy, 1.5, 30.0, seed=1048576)
+    rng = np.random.default_rng(10)
+    return pos, rng.normal(size=(N, 3)), rng.normal(size=(N, 3)), Lxy

@@ -22,7 +22,11 @@ Pinning status (each checked in tests/test_oracle_*.py):
   LanczosOracle      converges to dense_sqrt on SPD matrices; tridiagonal
                      step against scipy.linalg.eigh_tridiagonal
   dense_sqrt         (dense_sqrt)^2 == M
-  rfd_drift          mean over many draws vs deterministic_divM
+  deterministic_divM closed-form d/dz of the Swan-Brady self mobility above
+                     the wall and of the damped self mobility below one
+                     radius; zero divergence of open RPY (test_oracle_drift.py)
+  rfd_drift          the same closed form for one particle; the sum over unit
+                     displacements equals deterministic_divM
 """
 import ctypes
 import os
@@ -280,14 +284,21 @@ class LanczosOracle:
         raise LanczosError("[Lanczos] Could not converge")
 
 
-def rfd_drift(pos, W, eta, a, delta, prefactor=1.0, angular=False, wall=True, sys_sizes=None):
+def rfd_drift(pos, W, eta, a, delta, prefactor=1.0, angular=False, wall=True, sys_sizes=None,
+              rows=None):
     """(prefactor/delta) [M(q + delta/2 W) W - M(q - delta/2 W) W] with W used
     as both displacement and force (random_finite_differences.h:46-62); the
-    step delta is already multiplied by a (mobility.h:224)."""
+    step delta is already multiplied by a (mobility.h:224).  With ``rows``,
+    only those target particles (every source moves), shaped (len(rows), 3)."""
     pos = np.asarray(pos, dtype=np.float64).reshape(-1)
     W = np.asarray(W, dtype=np.float64).reshape(-1)
-    Up, Wp = mdot(pos + 0.5 * delta * W, eta, a, F=W, angular=angular, wall=wall, sys_sizes=sys_sizes)
-    Um, Wm = mdot(pos - 0.5 * delta * W, eta, a, F=W, angular=angular, wall=wall, sys_sizes=sys_sizes)
+    kw = dict(F=W, angular=angular, wall=wall, sys_sizes=sys_sizes)
+    if rows is None:
+        Up, Wp = mdot(pos + 0.5 * delta * W, eta, a, **kw)
+        Um, Wm = mdot(pos - 0.5 * delta * W, eta, a, **kw)
+    else:
+        Up, Wp = mdot_rows(pos + 0.5 * delta * W, eta, a, rows, **kw)
+        Um, Wm = mdot_rows(pos - 0.5 * delta * W, eta, a, rows, **kw)
     lin = (prefactor / delta) * (Up - Um)
     ang = (prefactor / delta) * (Wp - Wm) if angular else None
     return lin, ang

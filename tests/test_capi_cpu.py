@@ -172,6 +172,22 @@ def test_outputs_are_sized_by_the_solvers_own_particle_count(monkeypatch):
     assert s._outputs_like_positions()[0].shape == (15,)
 
 
+def test_output_buffers_smaller_than_3N_are_refused(monkeypatch):
+    """Regression for a host heap overrun: an output sized from another
+    solver's positions (7 particles) handed to a solver of 165 particles would
+    receive 495 values in a 21-value buffer.  Every output pointer passed to
+    the library is checked against this solver's 3N first."""
+    from libmobility_amd import nbody
+    s = nbody.NBody("open", "open", "open")
+    monkeypatch.setattr(s, "getNumberParticles", lambda: 165)
+    with pytest.raises(RuntimeError, match="holds 21 values, the solver writes 495"):
+        s._out(np.zeros((7, 3)))
+    assert s._out(np.zeros((165, 3))) is not None
+    assert s._out(None) is None
+    with pytest.raises(RuntimeError, match="holds 495 values, the solver writes 990"):
+        s._out(np.zeros((165, 3)), nvec=2)
+
+
 def test_libmobility_surface_and_selfmobility_configuration():
     """solvers/__init__.py:1-6 and SelfMobility's constructor
     (solvers/SelfMobility/mobility.h:31-36): open boundaries only; PSE and

@@ -1,0 +1,249 @@
+"""Full-size parity of the benchmark configurations C2-C5 (C1 is covered in
+test_gpu_mdot.py and test_gpu_lanczos.py at its full N = 1024).
+
+Inputs come from the same generators and seeds as bench.py
+(libmobility_amd.workloads.config_c2 ... config_c5):
+  C2  single wall, N = 65,536, x,y in [0,256a), z in [1.5a,20a], F and T
+  C3  open, N = 131,072, RSA packing at phi = 0.2
+  C4  512 ragged systems, N_s log-uniform in [64, 8192], half open, half wall
+  C5  single wall, N = 1,048,576, x,y in [0,1024a), z in [1.5a,30a], F and T
+Every product is compared with the oracle's FP64 matrix-free rows
+(oracle.mdot_rows, every source) on a few hundred sampled targets.
+
+Tolerances (relative, in norm over the sampled rows):
+  FP64 pairs                  1e-12 for every block
+  FP32 pairs, open UF         1e-6
+  FP32 pairs, wall blocks     UF 1.5e-6, UT and WF 4e-6, WT 3e-6 (C2 errors
+                              measured 6.7e-7, 1.9e-6, 1.95e-6, 1.1e-6: the
+                              antisymmetric cross blocks cancel more)
+  FP32 pairs, C5 6x6          1e-5 (measured 4.1e-6 for W: FP32 records
+                              quantize positions to 2^-24 of the 1024a
+                              slab, 6e-5 a)
+Square roots at these sizes have no dense oracle; they are checked through
+identities of the exact square root: ||M^{1/2} W||^2 = W.(M W) (1e-5), and
+M^{1/2}(M^{1/2} W) = M W (1e-4: two solves, each stopped when successive
+estimates change by less than tol = 1e-6, which leaves a true error of about
+ten times tol -- 1.2e-5 at C1 against the dense eigen square root -- plus the
+FP32 operator's 1e-5 distance from the FP64 one).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from libmobility_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+ETA = 1.0 / (6 * np.pi)
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b))
+
+
+def _rows(N, k, seed):
+    return np.sort(np.random.default_rng(seed).choice(N, size=k, replace=False))
+
+
+def _nbody(wall, precision, angular=False, tol=1e-6, delta=1e-3, sizes=None, kinds=None):
+    from libmobility_amd import NBody
+    s = NBody("open", "open", "single_wall" if wall else "open")
+    s.setParameters(wallHeight=0.0, delta=delta) if wall else s.setParameters(delta=delta)
+    s.setPairPrecision(precision)
+    if sizes is not None:
+        s.setSystems(sizes, kinds)
+    s.initialize(viscosity=ETA, hydrodynamicRadius=1.0, includeAngular=angular, tolerance=tol,
+                 seed=11)
+    return s
+
+
+# ---- C2: single wall, N = 65536, forces and torques ----------------------------------
+
+@pytest.fixture(scope="module")
+def c2():
+    pos, F, T, _ = workloads.config_c2()
+    rows = _rows(pos.shape[0], 384, 2)
+    ref = {}
+    for name, f, t in (("F", F, None), ("T", None, T)):
+        ref[name] = oracle.mdot_rows(pos, ETA, 1.0, rows, F=f, T=t, angular=True, wall=True)
+    return pos, F, T, rows, ref
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_c2_wall_blocks_full_size(c2, precision):
+    pos, F, T, rows, ref = c2
+    assert pos.shape[0] == 65536
+    s = _nbody(True, precision, angular=True)
+    s.setPositions(pos)
+    uf, wf = s.Mdot(F)
+    assert s.lastPairSymmetric()
+    ut, wt = s.Mdot(None, T)
+    got = {"UF": uf[rows], "WF": wf[rows], "UT": ut[rows], "WT": wt[rows]}
+    want = {"UF": ref["F"][0], "WF": ref["F"][1], "UT": ref["T"][0], "WT": ref["T"][1]}
+    bound = ({"UF": 1e-12, "WF": 1e-12, "UT": 1e-12, "WT": 1e-12} if precision == "fp64" else
+             {"UF": 1.5e-6, "WF": 4e-6, "UT": 4e-6, "WT": 3e-6})
+    for b in got:
+        assert _rel(got[b], want[b]) <= bound[b], b
+    # F and T together are the sum of the blocks
+    u, w = s.Mdot(F, T)
+    assert _rel(u[rows], want["UF"] + want["UT"]) <= 4 * bound["UT"]
+    assert _rel(w[rows], want["WF"] + want["WT"]) <= 4 * bound["WF"]
+
+
+def test_c2_drift_full_size(c2):
+    """RFD drift at C2 (delta = 1e-3 a, FP64 drift products) on a given W."""
+    pos, _, _, rows, _ = c2
+    N = pos.shape[0]
+    W = np.random.default_rng(12).normal(size=3 * N)
+    s = _nbody(True, "fp32", angular=True)
+    s.setPositions(pos)
+    u, w = s.divM_with_noise(W)
+    lin, ang = oracle.rfd_drift(pos, W, ETA, 1.0, 1e-3, angular=True, rows=rows[:128])
+    assert _rel(u[rows[:128]], lin) <= 1e-9
+    assert _rel(w[rows[:128]], ang) <= 1e-9
+
+
+# ---- C3: open RSA packing, N = 131072 ------------------------------------------------
+
+@pytest.fixture(scope="module")
+def c3():
+    pos, L, F, Wn = workloads.config_c3()
+    rows = _rows(pos.shape[0], 512, 3)
+    Uref, _ = oracle.mdot_rows(pos, ETA, 1.0, rows, F=F)
+    return pos, F, Wn, rows, Uref
+
+
+@pytest.mark.parametrize("precision,bound", [("fp64", 1e-12), ("fp32", 1e-6)])
+@pytest.mark.parametrize("symmetric", [1, 0])
+def test_c3_open_uf_full_size(c3, precision, bound, symmetric):
+    pos, F, _, rows, Uref = c3
+    assert pos.shape[0] == 131072
+    s = _nbody(False, precision)
+    s.setPairSymmetric(symmetric)
+    s.setPositions(pos)
+    U, _ = s.Mdot(F)
+    assert s.lastPairSymmetric() == bool(symmetric)
+    assert _rel(U[rows], Uref) <= bound
+
+
+def test_c3_plain_square_root_identities(c3):
+    """Plain Lanczos at tol 1e-6 with FP32 pairs: y = M^{1/2} W satisfies
+    ||y||^2 = W.(M W) and M^{1/2} y = M W (M from the FP64 product)."""
+    pos, _, Wn, _, _ = c3
+    s = _nbody(False, "fp32", tol=1e-6)
+    s.setPositions(pos)
+    y, _ = s.sqrtMdotW_with_noise(Wn)
+    y = y.ravel()
+    s64 = _nbody(False, "fp64")
+    s64.setPositions(pos)
+    MW, _ = s64.Mdot(Wn)
+    MW = MW.ravel()
+    assert abs(y @ y - Wn @ MW) <= 1e-5 * (Wn @ MW)
+    yy, _ = s.sqrtMdotW_with_noise(y)
+    assert _rel(yy, MW) <= 1e-4
+
+
+def test_c3_two_level_square_root_shares_the_norm_identity(c3):
+    """Two-level preconditioned Lanczos (coarse level of 40 particles per cell,
+    learned blocks) returns B W with B B^T = M, another square root than
+    M^{1/2}, so no per-sample identity with M alone holds.  What can be checked
+    at this size: it takes fewer iterations than plain Lanczos, repeats
+    bitwise, and E||B W||^2 = tr(M) = E||M^{1/2} W||^2, both concentrated to
+    well under 1% for 3N = 393216 (within 5% asserted)."""
+    from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
+    pos, _, Wn, _, _ = c3
+    s = _nbody(False, "fp32", tol=1e-6)
+    net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+    s.setPreconditioner(net)
+    s.setCoarseLevel(40)
+    s.setPositions(pos)
+    yb, _ = s.sqrtMdotW_with_noise(Wn)
+    it_b = s.lastLanczosIterations()
+    yb2, _ = s.sqrtMdotW_with_noise(Wn)
+    assert np.array_equal(yb, yb2)
+    p = _nbody(False, "fp32", tol=1e-6)
+    p.setPositions(pos)
+    yp, _ = p.sqrtMdotW_with_noise(Wn)
+    assert it_b < p.lastLanczosIterations()
+    assert abs(np.linalg.norm(yb) / np.linalg.norm(yp) - 1) <= 0.05
+
+
+# ---- C4: 512 ragged systems ----------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def c4():
+    sizes, kinds, pos, F, Wn = workloads.config_c4()
+    rows = _rows(pos.shape[0], 512, 4)
+    Uref, _ = oracle.mdot_rows(pos, ETA, 1.0, rows, F=F, wall=list(kinds), sys_sizes=sizes)
+    return sizes, kinds, pos, F, Wn, rows, Uref
+
+
+@pytest.mark.parametrize("precision,bound", [("fp64", 1e-12), ("fp32", 1.5e-6)])
+def test_c4_ensemble_full_size(c4, precision, bound):
+    sizes, kinds, pos, F, _, rows, Uref = c4
+    assert len(sizes) == 512 and sizes.min() >= 64 and sizes.max() <= 8192
+    s = _nbody(True, precision, sizes=sizes, kinds=kinds)
+    s.setPositions(pos)
+    U, _ = s.Mdot(F)
+    assert _rel(U[rows], Uref) <= bound
+
+
+def test_c4_per_system_square_roots_norm_identity(c4):
+    """Per-system Lanczos (one recurrence per system, tol 1e-6, FP32 pairs):
+    ||y_s||^2 = W_s.(M_s W_s) for every system."""
+    sizes, kinds, pos, _, Wn, _, _ = c4
+    s = _nbody(True, "fp32", sizes=sizes, kinds=kinds)
+    s.setLanczosPerSystem(True)
+    s.setPositions(pos)
+    y, _ = s.sqrtMdotW_with_noise(Wn)
+    its = s.lastLanczosIterationsPerSystem()
+    assert its.min() >= 1
+    s64 = _nbody(True, "fp64", sizes=sizes, kinds=kinds)
+    s64.setPositions(pos)
+    MW, _ = s64.Mdot(Wn)
+    y, MW = y.ravel(), MW.ravel()
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    worst = 0.0
+    for a0, b0 in zip(off[:-1], off[1:]):
+        sl = slice(3 * a0, 3 * b0)
+        q = Wn[sl] @ MW[sl]
+        worst = max(worst, abs(y[sl] @ y[sl] - q) / q)
+    assert worst <= 2e-5
+
+
+# ---- C5: single wall, N = 1048576, forces and torques -------------------------------
+
+@pytest.fixture(scope="module")
+def c5():
+    pos, F, T, Lxy = workloads.config_c5()
+    rows = _rows(pos.shape[0], 96, 5)
+    U, W = oracle.mdot_rows(pos, ETA, 1.0, rows, F=F, T=T, angular=True, wall=True)
+    return pos, F, T, rows, U, W
+
+
+@pytest.mark.parametrize("precision,bound", [("fp32", 1e-5), ("fp64", 1e-12)])
+def test_c5_wall_6x6_full_size(c5, precision, bound):
+    """One Mdot of the 6N x 6N operator; the symmetric evaluation runs in
+    several passes under its default 4 GiB slab budget."""
+    pos, F, T, rows, Uref, Wref = c5
+    assert pos.shape[0] == 1048576
+    s = _nbody(True, precision, angular=True)
+    s.setPositions(pos)
+    U, W = s.Mdot(F, T)
+    assert s.lastPairSymmetric()
+    assert _rel(U[rows], Uref) <= bound
+    assert _rel(W[rows], Wref) <= bound
+
+
+def test_c5_drift_full_size(c5):
+    """RFD drift at C5 (delta = 1e-4 a) on a given W: FP64 drift products, as
+    the default is, against the oracle's rows."""
+    pos, _, _, rows, _, _ = c5
+    N = pos.shape[0]
+    W = np.random.default_rng(13).normal(size=3 * N)
+    s = _nbody(True, "fp32", angular=True, delta=1e-4)
+    s.setPositions(pos)
+    u, w = s.divM_with_noise(W)
+    lin, ang = oracle.rfd_drift(pos, W, ETA, 1.0, 1e-4, angular=True, rows=rows[:48])
+    assert _rel(u[rows[:48]], lin) <= 1e-8
+    assert _rel(w[rows[:48]], ang) <= 1e-8
