@@ -193,6 +193,21 @@ int64_t lm_nbody_num_particles(lm_nbody* h);
 int lm_nbody_mdot(lm_nbody* h, const double* forces, int64_t n_forces, const double* torques,
                   int64_t n_torques, double* linear, double* angular, int mem);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): lm_nbody_mdot on device pointers (forces, torques may
+ * be NULL) that calls no hipMalloc, hipFree or allocating routine.  It runs
+ * the ordered pair evaluation with its work items and chunk partial sums in
+ * a static device array of 64 MiB per GPU (held until the process ends), so
+ * the slabs are cut to fit (one chunk needs none) and it refuses a layout
+ * whose work items alone exceed 64 MiB (about 4e8 particles).  One system or
+ * equal batches only (not lm_nbody_set_systems).  The work is queued on the
+ * handle's stream and the call returns without waiting.  All calls on one GPU
+ * share that GPU's array: calls that overlap in time on one GPU race.  Call
+ * lm_nbody_set_positions (which allocates) beforehand. */
+int lm_nbody_mdot_noalloc(lm_nbody* h, const double* forces, const double* torques,
+                          double* linear, double* angular);
+
 /* Mobility::sqrtMdotW, MobilityInterface.h:132-188 (Lanczos, lanczos.h:43-58):
  * linear (and angular) += prefactor * M^{1/2} W, W ~ N(0,1) drawn on the device. */
 int lm_nbody_sqrt_mdot_w(lm_nbody* h, double* linear, double* angular, double prefactor, int mem);
@@ -359,6 +374,13 @@ int lm_self_clean(lm_self* h);
  * and (j, i) (hydrodynamicKernels.cuh:20-442). */
 int lm_selftest_pair_both_host(int wall, const double* pi, const double* pj, double a,
                                double* fwd36, double* rev36);
+
+/* Self-check of the Lanczos tridiagonal square root, run on the GPU (replaces
+ * LAPACKE ?steqr + cblas ?gemv of LanczosAlgorithm.cu:48-85): out = T^{1/2} e1
+ * for T = tridiag(beta, alpha, beta), alpha: m host values, beta: m - 1,
+ * 1 <= m <= 1024; the same device code as the solver's.  NaN if T has a
+ * negative eigenvalue. */
+int lm_selftest_tridiag_sqrt(const double* alpha, const double* beta, int m, double* out);
 
 /* Philox4x32-10 of the noise kernels: out = philox(ctr, key). */
 int lm_selftest_philox_host(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4);

@@ -60,6 +60,7 @@ SIGNATURES = {
     "lm_nbody_mdot": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, ctypes.c_int]),
     "lm_nbody_sqrt_mdot_w": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
     "lm_nbody_sqrt_mdot_w_noise": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
+    "lm_nbody_mdot_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "lm_nbody_div_m": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
     "lm_nbody_div_m_noise": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int]),
     "lm_nbody_langevin_velocities": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _i64,
@@ -108,6 +109,7 @@ SIGNATURES = {
     "lm_self_clean": (ctypes.c_int, [_vp]),
     "lm_selftest_pair_both_host": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_double, _dp,
                                                   _dp]),
+    "lm_selftest_tridiag_sqrt": (ctypes.c_int, [_dp, _dp, ctypes.c_int, _dp]),
     "lm_selftest_philox_host": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32),
                                                ctypes.POINTER(ctypes.c_uint32),
                                                ctypes.POINTER(ctypes.c_uint32)]),

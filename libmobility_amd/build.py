@@ -14,7 +14,7 @@ LIB = os.path.join(OUT_DIR, "liblmamd.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 
 SOURCES = ["nbody_kernels.hip", "lanczos_kernels.hip", "precond_kernels.hip",
-           "coarse_kernels.hip", "coarse.cpp", "precond.cpp", "solver.cpp", "self_solver.cpp", "capi.cpp"]
+           "coarse_kernels.hip", "noalloc.hip", "coarse.cpp", "precond.cpp", "solver.cpp", "self_solver.cpp", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LM_OFFLOAD_ARCH", "gfx950")

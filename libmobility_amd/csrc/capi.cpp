@@ -191,6 +191,10 @@ int lm_nbody_mdot(lm_nbody* h, const double* F, int64_t nF, const double* T, int
   return guarded([&] { S(h).Mdot(F, F ? nF : 0, T, T ? nT : 0, U, W, is_dev(mem)); });
 }
 
+int lm_nbody_mdot_noalloc(lm_nbody* h, const double* F, const double* T, double* U, double* W) {
+  return guarded([&] { S(h).mdotNoAlloc(F, T, U, W); });
+}
+
 int lm_nbody_sqrt_mdot_w(lm_nbody* h, double* U, double* W, double prefactor, int mem) {
   return guarded([&] { S(h).sqrtMdotW(U, W, prefactor, is_dev(mem), nullptr); });
 }
@@ -523,6 +527,13 @@ int lm_self_langevin_velocities(lm_self* h, double dt, double kbt, const double*
 
 int lm_self_clean(lm_self* h) {
   return guarded([&] { SS(h).clean(); });
+}
+
+int lm_selftest_tridiag_sqrt(const double* alpha, const double* beta, int m, double* out) {
+  return guarded([&] {
+    if (!alpha || !out || (m > 1 && !beta)) throw lm::Error("[libmobility_amd] null array");
+    lm::tridiag_sqrt_selftest(alpha, beta, m, out);
+  });
 }
 
 int lm_selftest_philox_host(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4) {

@@ -183,6 +183,9 @@ void launch_lanczos_next(const double* w, double* v_next, int64_t n, int i,
 // s[0..m) = T_m^{1/2} e1 (FP64, one workgroup); work has tsqrt_scratch(m) doubles
 constexpr int kTsqrtNodes = 512;
 constexpr int64_t tsqrt_scratch(int m) { return (int64_t)m * kTsqrtNodes; }
+// The same kernel on host arrays alpha (m) and beta (m - 1): out = T^{1/2} e1
+// (synchronous; temporary device buffers).  For tests.
+void tridiag_sqrt_selftest(const double* alpha, const double* beta, int m, double* out);
 void launch_tridiag_sqrt(const LanczosScalars* sc, int m, double* s_out, double* work,
                          hipStream_t s);
 // y = normz * V[:, 0:m] s ; err = ||y - old|| / ||old||

@@ -252,7 +252,7 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
                                   const double* __restrict__ beta, int m,
                                   double* __restrict__ s_out, double* __restrict__ work) {
   __shared__ double a[kLanczosMaxIterations], b[kLanczosMaxIterations],
-      b2[kLanczosMaxIterations], xs[kLanczosMaxIterations];
+      b2[kLanczosMaxIterations];
   __shared__ double red[kTS / 64];
   __shared__ int ired[kTS / 64];
   __shared__ double s_lo, s_hi;
@@ -304,10 +304,13 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
     for (int i = t; i < m; i += kTS) s_out[i] = NAN;
     return;
   }
-  // Multisection for the smallest eigenvalue: 3 rounds over kTS points bracket
-  // it to (range) / kTS^3; the lower end L is a lower bound.
+  // Multisection for the smallest eigenvalue: each round over kTS points
+  // shrinks the bracket [L, H] by kTS; rounds continue until L > 0 brackets it
+  // within a factor 2 (at most 8 rounds: (range) / kTS^8 is far below the
+  // 1e-16 floor).  The lower end L is a lower bound.
   double L = s_lo, H = s_hi;
-  for (int round = 0; round < 3; ++round) {
+  for (int round = 0; round < 8; ++round) {
+    if (round >= 2 && L > 0.0 && H - L <= 0.5 * L) break;  // uniform over the workgroup
     const double x = L + (H - L) * (double)(t + 1) / (double)kTS;
     const int c = sturm_count(a, b2, m, x, pivmin);
     int first = (c >= 1) ? t : kTS;
@@ -323,12 +326,15 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
     L = nL;
     H = nH;
   }
+  // Eigenvalues below 1e-16 of the largest (a numerically singular operator)
+  // are treated as lying on that floor.
   const double lmin = fmax(L, 1e-300 + 1e-16 * s_hi);
   const double lmax = s_hi;
   // Nodes u_k = c + s sinh(t_k), t_k = (k - half) h: the map makes the sech
-  // tails of the integrand decay double-exponentially, so K ~ 70-130 nodes
-  // give ~1e-14 over condition numbers 1..1e9 (rule fitted and checked
-  // against eigh; s and h grow and shrink with log10 of the condition number).
+  // tails of the integrand decay double-exponentially; s and h grow and
+  // shrink with log10 of the condition number, from 71 nodes at kappa 1e2 to
+  // 257 at 1e16, with a scalar quadrature error below 2e-13 over the whole
+  // spectrum.
   const double lk = log10(fmax(lmax / lmin, 1.0));
   const double sc_s = 1.2 + 0.2 * lk;
   double h = 0.18 / (1.0 + 0.3 * lk);
@@ -339,7 +345,13 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
   }
   const int K = 2 * half + 1;
   const double c0 = 0.25 * (log(lmin) + log(lmax));
-  double* q = work;  // [m][kTS]: UL multipliers, then weighted solution entries
+  // T^{1/2} e1 = sum_k w_k T x_k with (sig_k I + T) x_k = e1.  Each node forms
+  // its own T x_k: as e1 - sig_k x_k below the geometric middle of the
+  // spectrum and as the tridiagonal product above it.  (Summing the x_k first
+  // and multiplying by T once amplifies the solves' rounding by about
+  // sqrt(kappa): 2e-10 instead of 2e-13 at kappa 1e16.)
+  const double tau = sqrt(lmin * lmax);
+  double* q = work;  // [m][kTS]: UL multipliers, then weighted entries of T x_k
   if (t < K) {
     const double tk = (double)(t - half) * h;
     const double u = c0 + sc_s * sinh(tk);
@@ -354,28 +366,39 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
       rg = 1.0 / (sig + a[i] - b2[i] * rg);
       if (i >= 1) q[(int64_t)i * kTS + t] = -b[i - 1] * rg;
     }
-    double x = rg;
-    q[t] = wgt * x;
-    for (int i = 1; i < m; ++i) {
-      x *= q[(int64_t)i * kTS + t];
-      q[(int64_t)i * kTS + t] = wgt * x;
+    if (sig < tau) {
+      // T x = e1 - sig x
+      double x = rg;
+      q[t] = wgt * (1.0 - sig * x);
+      for (int i = 1; i < m; ++i) {
+        x *= q[(int64_t)i * kTS + t];
+        q[(int64_t)i * kTS + t] = -wgt * sig * x;
+      }
+    } else {
+      // (T x)_{i-1} = b_{i-2} x_{i-2} + a_{i-1} x_{i-1} + b_{i-1} x_i, written
+      // once x_i is known (over entry i-1 of q, whose multiplier is used)
+      double xm2 = 0.0, xm1 = rg;
+      for (int i = 1; i < m; ++i) {
+        const double x = xm1 * q[(int64_t)i * kTS + t];
+        double y = a[i - 1] * xm1 + b[i - 1] * x;
+        if (i >= 2) y += b[i - 2] * xm2;
+        q[(int64_t)(i - 1) * kTS + t] = wgt * y;
+        xm2 = xm1;
+        xm1 = x;
+      }
+      double y = a[m - 1] * xm1;
+      if (m >= 2) y += b[m - 2] * xm2;
+      q[(int64_t)(m - 1) * kTS + t] = wgt * y;
     }
   }
   __syncthreads();
-  // x_j = sum_k dp[j][k] : one wave per component, fixed order
+  // (T^{1/2} e1)_j = sum_k q[j][k] : one wave per component, fixed order
   const int lane = t & 63, wid = t >> 6;
   for (int j = wid; j < m; j += kTS / 64) {
     double v = 0.0;
     for (int k = lane; k < K; k += 64) v += q[(int64_t)j * kTS + k];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane == 0) xs[j] = v;
-  }
-  __syncthreads();
-  for (int j = t; j < m; j += kTS) {
-    double v = a[j] * xs[j];
-    if (j > 0) v += b[j - 1] * xs[j - 1];
-    if (j < m - 1) v += b[j] * xs[j + 1];
-    s_out[j] = v;
+    if (lane == 0) s_out[j] = v;
   }
 }
 
@@ -391,6 +414,30 @@ void launch_tridiag_sqrt(const LanczosScalars* sc, int m, double* s_out, double*
     throw Error("[libmobility_amd] tridiagonal size out of range");
   hipLaunchKernelGGL(k_tridiag_sqrt, dim3(1), dim3(kTS), 0, s, sc, m, s_out, work);
   LM_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kTS) void k_tridiag_sqrt_arrays(const double* __restrict__ alpha,
+                                                             const double* __restrict__ beta,
+                                                             int m, double* __restrict__ s_out,
+                                                             double* __restrict__ work) {
+  tridiag_sqrt_body(alpha, beta, m, s_out, work);
+}
+
+void tridiag_sqrt_selftest(const double* alpha, const double* beta, int m, double* out) {
+  if (m < 1 || m > kLanczosMaxIterations)
+    throw Error("[libmobility_amd] tridiagonal size out of range");
+  DevBuf da, db, dout, dwork;
+  da.reserve(sizeof(double) * m);
+  db.reserve(sizeof(double) * m);
+  dout.reserve(sizeof(double) * m);
+  dwork.reserve(sizeof(double) * tsqrt_scratch(m));
+  LM_HIP(hipMemcpy(da.as<double>(), alpha, sizeof(double) * m, hipMemcpyHostToDevice));
+  LM_HIP(hipMemcpy(db.as<double>(), beta, sizeof(double) * (m > 1 ? m - 1 : 1),
+                   hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_tridiag_sqrt_arrays, dim3(1), dim3(kTS), 0, nullptr, da.as<double>(),
+                     db.as<double>(), m, dout.as<double>(), dwork.as<double>());
+  LM_HIP(hipGetLastError());
+  LM_HIP(hipMemcpy(out, dout.as<double>(), sizeof(double) * m, hipMemcpyDeviceToHost));
 }
 
 // One workgroup per listed system (at most kSegTsqrtGroups at a time, each

@@ -29,6 +29,9 @@ enum PairPrecision : int { kPairFP64 = 0, kPairFP32 = 1 };
 
 using LanczosCallback = void (*)(int iteration, double residual, void* user);
 
+// Device memory of the allocation-free Mdot (one static array per GPU).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+
 class NBodySolver {
  public:
   NBodySolver(int px, int py, int pz);
@@ -140,6 +143,11 @@ class NBodySolver {
   // Device-pointer cores (no staging, no synchronisation beyond what the
   // algorithm needs).
   void mdot_dev(const double* F, const double* T, double* U, double* W);
+  // Mdot that allocates no device memory (device pointers only, no wait):
+  // the ordered evaluation with its work items and chunk slabs in a static
+  // device array of kNoAllocScratchBytes per GPU (noalloc.hip).  One system
+  // or equal batches; calls overlapping on one GPU race for that array.
+  void mdotNoAlloc(const double* F, const double* T, double* U, double* W);
   void sqrtMdotW_dev(double* U, double* W, double prefactor, const double* noise);
   void divM_dev(double* U, double* W, double prefactor, const double* noise = nullptr);
 
