@@ -4,8 +4,9 @@ The kernel (lanczos_kernels.hip, tridiag_sqrt_body) replaces host
 LAPACKE ?steqr + cblas ?gemv (include/lanczos/LanczosAlgorithm.cu:48-85).
 lm_selftest_tridiag_sqrt runs the same device code on a given T:
   * against the numpy restatement of the kernel (tests/tridiag_rule.py), whose
-    range is pinned on the CPU against a 30-digit reference (1e-13: the two
-    differ only in the order of the node sum and in exp/sinh rounding);
+    range is pinned on the CPU against a 30-digit reference (1e-13 below kappa
+    1e16: the two differ only in the order of the node sum and in exp/sinh
+    rounding; looser where the bracket sits on the 1e-16 eigenvalue floor);
   * against the 30-digit reference directly, kappa 1e2 .. 1e16 (<= 1e-12)
     and 1e18 beyond the eigenvalue floor (<= 1e-7);
   * sizes 1 and 1024 (the iteration limit), and NaN for a negative
@@ -48,7 +49,10 @@ def _rel(x, y):
 def test_device_tridiagonal_square_root_range(kappa, bound):
     a, b = _T(kappa)
     got = _device(a, b)
-    assert _rel(got, tridiag_rule.sqrt_e1(a, b)) <= 1e-13
+    # at 1e16 and beyond, the bracket sits on the 1e-16 floor, which rounds
+    # differently on the two sides
+    agree = 1e-13 if kappa < 1e16 else (1e-12 if kappa == 1e16 else 1e-9)
+    assert _rel(got, tridiag_rule.sqrt_e1(a, b)) <= agree
     assert _rel(got, _mp_sqrt_e1(a, b)) <= bound
 
 
