@@ -9,7 +9,7 @@ included) in FP32-pair / FP64-accumulate mode, inputs and outputs resident in
 HBM.  K steps are timed with HIP events on the solver's stream.  The Lanczos
 square root (tol 1e-6) is timed separately (median of a few calls) and
 reported with its iteration count, for the plain FP32-pair and FP64-pair
-operators and the learned-preconditioned FP32-pair operator.  The CPU baseline
+operators and the two-level preconditioned FP32-pair operator.  The CPU baseline
 is the oracle's OpenMP FP64 matrix-free product on a bounded sample of target
 rows, run on this host's cores.
 
@@ -23,8 +23,8 @@ Other workloads (``--workload``), each printing one JSON line of its own:
       single wall: one persistent launch vs per-system solves
   c5  single wall, N = 1048576, x,y in [0,1024a), z in [1.5a,30a], forces and
       torques: FP32/FP64 Mdot and hydrodynamicVelocities (Mdot + sqrtMdotW at
-      tol 1e-6 + RFD drift with delta 1e-4 a), FP32 pairs, learned
-      preconditioner
+      tol 1e-6 + RFD drift with delta 1e-4 a), FP32 pairs, two-level
+      preconditioner (or plain Lanczos with --c5-plain)
 
 Prints exactly one JSON line on stdout; progress goes to stderr.
 """
@@ -210,6 +210,16 @@ def result_line(metric, value, unit, steps, warmup, ms, higher, dtype, config, r
 # and WRITE_SIZE (x 1.00); the scales come from a 2 GiB device copy.  Keyed by
 # (workload, N, pair evaluation, targets per lane, source chunks) of the
 # profiled configuration.
+# C3 tiling of the bench line and its profiles: 4 targets per lane, 32
+# offset chunks
+C3_TILING = (4, 32)
+
+# FP32 VALU flops issued per launch, SQ_INSTS_VALU_FLOPS_FP32 x 64 lanes from
+# rocprofv3 --pmc, same keys as MEASURED_TRAFFIC_BYTES.
+MEASURED_FP32_FLOPS = {
+    ("c3", 131072, "symmetric", 4, 64): 7.134e9 * 64,
+}
+
 MEASURED_TRAFFIC_BYTES = {
     ("c3", 131072, "ordered", 2, 32): (10454.1 * 2.0 + 98304.0) * 1024.0,
     ("c3", 131072, "symmetric", 4, 64): (152410.0 * 2.0 + 394750.0) * 1024.0,
@@ -347,14 +357,17 @@ def bench_c3(args):
     fpp = flops_per_pair()["open_UF"]
     pairs = float(N) * float(N)
 
-    log("FP32-pair Mdot")
-    h32 = g.solver(pos_d, "fp32", tiling=args.tiling)
+    # pinned tiling (4 targets per lane, 32 offset chunks) unless given: the
+    # profile summaries are taken at this tiling
+    tiling = args.tiling or C3_TILING
+    log(f"FP32-pair Mdot, tiling {tiling}")
+    h32 = g.solver(pos_d, "fp32", tiling=tiling)
     ms32 = g.time_events(lambda: g.mdot(h32, F_d, None, U_d), args.steps, args.warmup)
     U32 = U_d.cpu().numpy()
     gpair32 = pairs / (ms32 * 1e-3) / 1e9
     log(f"  {ms32:.3f} ms/Mdot, {gpair32:.1f} Gpair/s")
     log("FP64-pair Mdot")
-    h64 = g.solver(pos_d, "fp64", tiling=args.tiling)
+    h64 = g.solver(pos_d, "fp64", tiling=tiling)
     ms64 = g.time_events(lambda: g.mdot(h64, F_d, None, U_d), max(3, args.steps // 4), 1)
     U64 = U_d.cpu().numpy()
     gpair64 = pairs / (ms64 * 1e-3) / 1e9
@@ -374,7 +387,7 @@ def bench_c3(args):
     }
     if extra["pair_evaluation_fp32pair"] == "symmetric":
         log("FP32-pair Mdot, ordered evaluation (every ordered pair, as the reference)")
-        ho = g.solver(pos_d, "fp32", tiling=args.tiling)
+        ho = g.solver(pos_d, "fp32", tiling=tiling)
         g.ffi.check(g.lib.lm_nbody_set_pair_symmetric(ho, 0))
         ms_o = g.time_events(lambda: g.mdot(ho, F_d, None, U_d), max(3, args.steps // 4), 1)
         g.destroy(ho)
@@ -405,8 +418,8 @@ def bench_c3(args):
         log("plain Lanczos sqrtMdotW, FP32 pairs, tol 1e-6, new positions before each call")
         ms_l32p, _ = g.time_lanczos(h32, W_d, 3 * N, args.lanczos_calls, fresh_pos=pos_d)
         extra["sqrtMdotW_plain_fp32pair_with_setPositions_ms"] = ms_l32p
-        from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
-        net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+        from libmobility_amd.precond import default_network
+        net = default_network(False)
         log(f"two-level preconditioned Lanczos sqrtMdotW (learned 3x3 blocks + coarse level of "
             f"{COARSE_PPC} particles per cell), FP32 pairs, tol 1e-6")
         hl = g.solver(pos_d, "fp32", tiling=lt)
@@ -438,17 +451,20 @@ def bench_c3(args):
         g.ffi.check(g.lib.lm_nbody_coarse_builds(hl, ctypes.byref(nb)))
         g.destroy(hl)
         log(f"  {ms_refresh:.1f} ms per call, {nb.value - nb0.value} coarse build(s)")
-        extra["sqrtMdotW_learned_preconditioner"] = (
-            f"learned 3x3 blocks (16-neighbour MLP, BF16 MFMA) + coarse level "
-            f"({COARSE_PPC} particles per cell target, {cells} cells)")
-        extra["sqrtMdotW_learned_fp32pair_ms"] = ms_ll
-        extra["sqrtMdotW_learned_fp32pair_iterations"] = itl
-        extra["sqrtMdotW_learned_speedup_vs_plain"] = ms_l32 / ms_ll
-        extra["sqrtMdotW_learned_with_setPositions_ms"] = ms_llp
-        extra["sqrtMdotW_learned_speedup_vs_plain_with_setPositions"] = ms_l32p / ms_llp
-        extra["sqrtMdotW_learned_with_setPositions_refresh10_ms"] = ms_refresh
-        extra["sqrtMdotW_learned_refresh10_coarse_builds"] = nb.value - nb0.value
-        extra["sqrtMdotW_learned_speedup_vs_plain_with_setPositions_refresh10"] = ms_l32p / ms_refresh
+        extra["sqrtMdotW_two_level_preconditioner"] = (
+            f"learned 3x3 blocks (64-feature MLP, BF16 MFMA, trained on the GPU) + coarse level "
+            f"({COARSE_PPC} particles per cell target, {cells} cells); on this open packing the "
+            f"speedup comes from the coarse level: the learned blocks neither gain nor lose "
+            f"(see sqrtMdotW_ablation)")
+        extra["sqrtMdotW_two_level_fp32pair_ms"] = ms_ll
+        extra["sqrtMdotW_two_level_fp32pair_iterations"] = itl
+        extra["sqrtMdotW_two_level_speedup_vs_plain"] = ms_l32 / ms_ll
+        extra["sqrtMdotW_two_level_with_setPositions_ms"] = ms_llp
+        extra["sqrtMdotW_two_level_speedup_vs_plain_with_setPositions"] = ms_l32p / ms_llp
+        extra["sqrtMdotW_two_level_with_setPositions_refresh10_ms"] = ms_refresh
+        extra["sqrtMdotW_two_level_refresh10_coarse_builds"] = nb.value - nb0.value
+        extra["sqrtMdotW_two_level_speedup_vs_plain_with_setPositions_refresh10"] = (
+            ms_l32p / ms_refresh)
         extra["coarse_build_ms"] = build
         log("tol 1e-4: plain and two-level sqrtMdotW, FP32 pairs")
         hp4 = g.solver(pos_d, "fp32", tol=1e-4, tiling=lt)
@@ -461,8 +477,8 @@ def bench_c3(args):
         g.destroy(ht4)
         extra["sqrtMdotW_tol1e-4"] = {
             "plain_fp32pair_ms": ms_p4, "plain_fp32pair_iterations": it_p4,
-            "learned_fp32pair_ms": ms_t4, "learned_fp32pair_iterations": it_t4,
-            "learned_speedup_vs_plain": ms_p4 / ms_t4}
+            "two_level_fp32pair_ms": ms_t4, "two_level_fp32pair_iterations": it_t4,
+            "two_level_speedup_vs_plain": ms_p4 / ms_t4}
         log("ablation: coarse level only (no network)")
         hc = g.solver(pos_d, "fp32", tiling=lt)
         g.ffi.check(g.lib.lm_nbody_set_coarse_level(hc, COARSE_PPC))
@@ -475,7 +491,8 @@ def bench_c3(args):
         g.destroy(hb)
         extra["sqrtMdotW_ablation"] = {
             "coarse_only_ms": ms_c, "coarse_only_iterations": itc,
-            "learned_blocks_only_ms": ms_b, "learned_blocks_only_iterations": itb}
+            "learned_blocks_only_ms": ms_b, "learned_blocks_only_iterations": itb,
+            "learned_blocks_gain_over_coarse_only": ms_c / ms_ll}
         if not args.no_bbt:
             log("square-root check at small N (columns of B from unit vectors)")
             extra["bbt_rel_err_small_N"] = bbt_errors(g, net)
@@ -491,15 +508,30 @@ def bench_c3(args):
         "pairs_per_step": pairs,
     }
     t32 = extra["tiling_fp32pair"]
-    traffic = MEASURED_TRAFFIC_BYTES.get((
-        "c3", N, extra["pair_evaluation_fp32pair"], t32["targets_per_lane"], t32["source_chunks"]))
+    key = ("c3", N, extra["pair_evaluation_fp32pair"], t32["targets_per_lane"], t32["source_chunks"])
+    roof = compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS, MEASURED_TRAFFIC_BYTES.get(key))
+    issued = MEASURED_FP32_FLOPS.get(key)
+    if issued is not None:
+        # what the hardware issues: FP32 VALU flops counted by the SQ counters
+        roof["hw_achieved"] = issued / (ms32 * 1e-3) / 1e12
+        roof["hw_frac"] = roof["hw_achieved"] / PEAK_FP32_TFLOPS
     res = result_line(METRIC, gpair32, "Gpair/s", args.steps, args.warmup, ms32, True, "f32",
-                      config, compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS, traffic),
-                      extra, None)
+                      config, roof, extra, None)
     if not args.no_cpu:
         log("CPU baseline (oracle, OpenMP FP64)")
         cpu, Uo = cpu_baseline_rows(pos, F)
         res["cpu_baseline"] = cpu
+        if "sqrtMdotW_plain_fp64pair_iterations" in extra:
+            # CPU plain Lanczos projected from the measured CPU product rate:
+            # (iterations + 1) products of N^2 pairs (the vector work is
+            # O(N) per iteration and not counted)
+            its = int(np.median(extra["sqrtMdotW_plain_fp64pair_iterations"]))
+            ms_cpu = (its + 1) * pairs / (cpu["value"] * 1e9) * 1e3
+            extra["cpu_sqrtMdotW_plain_projected_ms"] = ms_cpu
+            extra["cpu_sqrtMdotW_plain_projection"] = (
+                f"projection, not a run: ({its} iterations of the FP64 plain solve + 1) x N^2 "
+                f"pairs at the measured CPU product rate {cpu['value']:.2f} Gpair/s on "
+                f"{cpu['cores']} threads")
         k = Uo.size
         extra["mdot_rel_err_fp32pair_vs_oracle_rows"] = rel(U32[:k], Uo)
         extra["mdot_rel_err_fp64pair_vs_oracle_rows"] = rel(U64[:k], Uo)
@@ -602,6 +634,9 @@ def bench_c2(args):
     for prec in ("fp32", "fp64"):
         log(f"{prec}-pair wall Mdot (F, T)")
         h = g.solver(pos_d, prec, wall=True, angular=True, tiling=args.tiling)
+        # the drift's products follow the pair precision here, to state the
+        # FP32 drift error (the library default is FP64 drift products)
+        g.ffi.check(g.lib.lm_nbody_set_drift_precision(h, g.ffi.PAIR_PRECISION[prec]))
         hs[prec] = h
         steps = args.steps if prec == "fp32" else max(3, args.steps // 4)
         ms = g.time_events(lambda: g.mdot(h, F_d, T_d, U_d, W_d), steps, args.warmup)
@@ -629,16 +664,20 @@ def bench_c2(args):
         out[prec] = rec
     two_level = None
     if not args.no_lanczos:
-        from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
-        net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+        from libmobility_amd.precond import default_network
+        net = default_network(True)
         h = hs["fp32"]
-        g.ffi.check(g.lib.lm_nbody_set_preconditioner(h, net._h))
         g.ffi.check(g.lib.lm_nbody_set_coarse_level(h, COARSE_PPC))
-        log(f"  fp32-pair sqrtMdotW (6N, tol 1e-6), two-level preconditioner (learned blocks + "
+        log(f"  fp32-pair sqrtMdotW (6N, tol 1e-6), coarse level of {COARSE_PPC} particles per "
+            f"cell only")
+        ms_c, it_c = g.time_lanczos(h, Wn_d, 3 * N, 1, angular=True)
+        g.ffi.check(g.lib.lm_nbody_set_preconditioner(h, net._h))
+        log(f"  fp32-pair sqrtMdotW (6N, tol 1e-6), two-level preconditioner (learned 6x6 blocks + "
             f"coarse level of {COARSE_PPC} particles per cell)")
         ms_t, it_t = g.time_lanczos(h, Wn_d, 3 * N, 1, angular=True)
         two_level = {"ms": ms_t, "iterations": it_t, "cells": g.coarse_cells(h),
-                     "coarse_build_ms": g.coarse_build_ms(h)}
+                     "coarse_build_ms": g.coarse_build_ms(h), "coarse_only_ms": ms_c,
+                     "coarse_only_iterations": it_c}
         g.ffi.check(g.lib.lm_nbody_set_coarse_level(h, 0))
         g.ffi.check(g.lib.lm_nbody_set_preconditioner(h, None))
     g.destroy(*hs.values())
@@ -665,7 +704,8 @@ def bench_c2(args):
         "pair_evaluation_fp64pair": out["fp64"]["evaluation"],
         "flops_per_pair": fpp,
         "rel_err_fp32pair_vs_fp64pair_all_rows": err32_vs_64,
-        "divM_rel_diff_fp32pair_vs_fp64pair": rel(out["fp32"]["drift"], out["fp64"]["drift"]),
+        "divM_rel_diff_fp32drift_vs_fp64drift_delta1e-3": rel(out["fp32"]["drift"],
+                                                               out["fp64"]["drift"]),
     }
     if not args.no_lanczos:
         extra.update({
@@ -679,6 +719,12 @@ def bench_c2(args):
             "sqrtMdotW_two_level_cells": two_level["cells"],
             "sqrtMdotW_two_level_coarse_build_ms": two_level["coarse_build_ms"],
             "sqrtMdotW_two_level_speedup_vs_plain": out["fp32"]["sqrt_ms"] / two_level["ms"],
+            "sqrtMdotW_coarse_only_fp32pair_ms": two_level["coarse_only_ms"],
+            "sqrtMdotW_coarse_only_fp32pair_iterations": two_level["coarse_only_iterations"],
+            "sqrtMdotW_learned_blocks_gain_over_coarse_only":
+                two_level["coarse_only_ms"] / two_level["ms"],
+            "sqrtMdotW_two_level_preconditioner": "learned 6x6 blocks (64-feature MLP, BF16 MFMA, "
+                                                  "trained on the GPU) + coarse level",
         })
     config = {"workload": f"single-wall RPB/Swan-Brady, N={N}, x,y uniform in [0,256a), z uniform "
                           f"in [1.5a,20a], a=1, eta=1/(6 pi), F,T~N(0,1); 6x6 Mdot (UF,UT,WF,WT), "
@@ -815,8 +861,8 @@ def bench_c4(args):
                                                                       len(sizes)))
         log(f"sqrtMdotW: one recurrence per system, two-level preconditioner (learned blocks + "
             f"coarse level of {COARSE_PPC} particles per cell, per system)")
-        from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
-        net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+        from libmobility_amd.precond import default_network
+        net = default_network(False)
         g.ffi.check(g.lib.lm_nbody_set_preconditioner(he, net._h))
         g.ffi.check(g.lib.lm_nbody_set_coarse_level(he, COARSE_PPC))
         ms_lt, it_t = g.time_lanczos(he, W_d, 3 * N, args.lanczos_calls)
@@ -889,19 +935,63 @@ def bench_c5(args):
     ms64 = g.time_events(lambda: g.mdot(h64, F_d, T_d, U_d, W_d), 1, 0)
     ev64 = g.evaluation(h64)
     log(f"  {ms64:.1f} ms ({ev64} evaluation)")
-    g.destroy(h64)
     extra = {"mdot_fp32pair_ms": ms32, "mdot_fp64pair_ms": ms64, "flops_per_pair": fpp,
              "pair_evaluation_fp32pair": ev32, "pair_evaluation_fp64pair": ev64,
              "tiling": {"targets_per_lane": tiling[0], "source_chunks": tiling[1]},
              "lanczos_iteration_limit": 1024}
+    if not args.c5_plain:
+        # thermal drift at delta = 1e-4 a on one random displacement: FP64 drift
+        # products (the library default) against FP32 ones
+        Wd = g.put(np.random.default_rng(13).normal(size=3 * N))
+        drift = {}
+        for prec in ("fp64", "fp32"):
+            g.ffi.check(g.lib.lm_nbody_set_drift_precision(h32, g.ffi.PAIR_PRECISION[prec]))
+            du, dw = g.zeros(3 * N), g.zeros(3 * N)
+            g.sync()
+            t = time.perf_counter()
+            g.ffi.check(g.lib.lm_nbody_div_m_noise(h32, Wd.data_ptr(), du.data_ptr(), dw.data_ptr(),
+                                                   1.0, g.ffi.LM_MEM_DEVICE))
+            g.sync()
+            drift[prec] = (time.perf_counter() - t, np.concatenate([du.cpu().numpy(),
+                                                                    dw.cpu().numpy()]))
+            log(f"RFD drift with {prec} products: {drift[prec][0]:.1f} s")
+        g.ffi.check(g.lib.lm_nbody_set_drift_precision(h32, g.ffi.PAIR_PRECISION["fp64"]))
+        extra.update({
+            "divM_fp64drift_ms": drift["fp64"][0] * 1e3,
+            "divM_fp32drift_ms": drift["fp32"][0] * 1e3,
+            "divM_rel_err_fp32drift_vs_fp64drift_delta1e-4": rel(drift["fp32"][1],
+                                                                 drift["fp64"][1]),
+        })
+        del Wd
+        # FP64 plain Lanczos: cost per iteration over a bounded run (the full
+        # FP64 plain step does not fit one call: ~200 iterations x the FP64 product)
+        probe = 12
+        g.ffi.check(g.lib.lm_nbody_set_lanczos_iteration_limit(h64, probe))
+        Wn = g.put(np.random.default_rng(14).normal(size=6 * N))
+        su, sw = g.zeros(3 * N), g.zeros(3 * N)
+        g.sync()
+        t = time.perf_counter()
+        status = g.lib.lm_nbody_sqrt_mdot_w_noise(h64, Wn.data_ptr(), su.data_ptr(), sw.data_ptr(),
+                                                  1.0, g.ffi.LM_MEM_DEVICE)
+        g.sync()
+        per_it64 = (time.perf_counter() - t) * 1e3 / probe
+        log(f"FP64 plain Lanczos: {per_it64 / 1e3:.2f} s per iteration over {probe} iterations "
+            f"(status {status}: stopped at the limit)")
+        extra["sqrtMdotW_plain_fp64pair_ms_per_iteration"] = per_it64
+        del Wn, su, sw
+    g.destroy(h64)
     step_ms = None
     if not args.no_lanczos:
-        from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
-        net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
-        g.ffi.check(g.lib.lm_nbody_set_preconditioner(h32, net._h))
-        g.ffi.check(g.lib.lm_nbody_set_coarse_level(h32, COARSE_PPC))
-        log("hydrodynamicVelocities (Mdot + sqrtMdotW tol 1e-6 + RFD drift), FP32 pairs, "
-            "two-level preconditioner (learned blocks + coarse level)")
+        if args.c5_plain:
+            log("hydrodynamicVelocities (Mdot + sqrtMdotW tol 1e-6 + RFD drift), FP32 pairs, plain "
+                "Lanczos")
+        else:
+            from libmobility_amd.precond import default_network
+            net = default_network(True)
+            g.ffi.check(g.lib.lm_nbody_set_preconditioner(h32, net._h))
+            g.ffi.check(g.lib.lm_nbody_set_coarse_level(h32, COARSE_PPC))
+            log("hydrodynamicVelocities (Mdot + sqrtMdotW tol 1e-6 + RFD drift), FP32 pairs, "
+                "two-level preconditioner (learned 6x6 blocks + coarse level)")
         g.sync()
         t = t_start = time.perf_counter()
         g.ffi.check(g.lib.lm_nbody_hydrodynamic_velocities(
@@ -912,25 +1002,28 @@ def bench_c5(args):
         it = g.lib.lm_nbody_last_lanczos_iterations(h32)
         log(f"  {step_ms / 1e3:.1f} s, Lanczos iterations {it}")
         u = U_d.cpu().numpy()
+        label = "plain" if args.c5_plain else "two_level"
         extra.update({
-            "hydrodynamicVelocities_fp32pair_learned_ms": step_ms,
-            "preconditioner": f"learned 3x3 blocks + coarse level ({g.coarse_cells(h32)} cells, "
-                              f"built in {g.coarse_build_ms(h32):.0f} ms)",
+            f"hydrodynamicVelocities_fp32pair_{label}_ms": step_ms,
             "lanczos_iterations": it,
             "output_finite": bool(np.all(np.isfinite(u))),
-            "fp64pair_step_projected_ms": (it + 4) * ms64,
-            "fp64pair_step_projection": "(iterations + 4 pair products: Mdot, 2 RFD products, "
-                                        "1 estimate slack) x measured FP64 Mdot time; not run",
         })
+        if not args.c5_plain:
+            extra["preconditioner"] = (f"learned 6x6 blocks + coarse level ({g.coarse_cells(h32)} "
+                                       f"cells, built in {g.coarse_build_ms(h32):.0f} ms)")
     g.destroy(h32)
     config = {"workload": f"single-wall RPB/Swan-Brady, N={N}, x,y uniform in [0,{Lxy:.0f}a), z "
                           f"uniform in [1.5a,30a], a=1, eta=1/(6 pi), F,T~N(0,1); 6x6 operator; "
                           f"hydrodynamicVelocities = Mdot + sqrtMdotW (prefactor 1, tol 1e-6) + "
-                          f"RFD drift (delta 1e-4 a); FP32 pairs, FP64 accumulation, learned "
-                          f"preconditioner", "N": N}
+                          f"RFD drift (delta 1e-4 a, FP64 drift products); FP32 pairs, FP64 "
+                          f"accumulation, "
+                          + ("plain Lanczos" if args.c5_plain else
+                             "two-level preconditioner (learned 6x6 blocks + coarse level)"),
+            "N": N}
     value = step_ms if step_ms is not None else ms32
-    return result_line("hydrodynamicVelocities ms at N=1048576 single wall (FP32 pairs, learned "
-                       "preconditioner)", value, "ms", 1, 1, value, False, "f32", config,
+    metric = ("hydrodynamicVelocities ms at N=1048576 single wall (FP32 pairs, "
+              + ("plain Lanczos)" if args.c5_plain else "learned two-level preconditioner)"))
+    return result_line(metric, value, "ms", 1, 1, value, False, "f32", config,
                        compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS), extra, None)
 
 
@@ -966,6 +1059,8 @@ def main():
                     help="c3: skip the small-N ||BB^T - M|| check (thousands of small solves)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device-to-device copy (reference for the HBM byte counters)")
+    ap.add_argument("--c5-plain", action="store_true",
+                    help="c5: the step with plain Lanczos instead of the two-level preconditioner")
     ap.add_argument("--coarse-ppc", type=int, default=COARSE_PPC,
                     help="particles per cell of the preconditioner's coarse level")
     args = ap.parse_args()

@@ -253,41 +253,59 @@ int lm_nbody_synchronize(lm_nbody* h);
 
 /* ---- learned block preconditioner (no counterpart in the reference) ------
  * A per-particle network maps the relative positions of a particle's 16
- * nearest neighbours (48 floats, units of a) through 48-128-128-6 (GELU,
- * BF16 MFMA, FP32 accumulation) to the Cholesky factor of an SPD 3x3 block
- * G_i.  With a preconditioner attached, sqrtMdotW runs Lanczos on
- * G M G and returns G^{-1} (G M G)^{1/2} W, another square root of M (with
- * the coarse level too, G = D Gc; see lm_nbody_set_coarse_level).
+ * nearest neighbours and its height above a wall (64 floats: 48 coordinates
+ * in units of a, then u = a/max(z, a), b = clamp(z/a, 0, 1), u^3 -- 0, 1, 0 in
+ * an open system -- and zeros) through 64-128-128-out (GELU, BF16 MFMA, FP32
+ * accumulation) to the Cholesky factor of an SPD block: 3x3 on the particle's
+ * linear entries (6 outputs), or 6x6 on its linear and angular entries (21
+ * outputs) for solvers initialised with includeAngular.  With a
+ * preconditioner attached, sqrtMdotW runs Lanczos on G M G^T and returns
+ * G^{-1} (G M G^T)^{1/2} W, another square root of M (with the coarse level
+ * too, G = D Gc; see lm_nbody_set_coarse_level).
  * All arrays of this group are host arrays. */
 typedef struct lm_precond lm_precond;
 
+/* A network of 3x3 blocks (block_dim 3) or 6x6 blocks (block_dim 6). */
+int lm_precond_create_blocks(int block_dim, lm_precond** out);
+/* lm_precond_create_blocks(3, out). */
 int lm_precond_create(lm_precond** out);
 int lm_precond_destroy(lm_precond* p);
-/* Number of FP32 weights: W1[48][128] b1[128] W2[128][128] b2[128] W3[128][6] b3[6]. */
+/* Number of FP32 weights of a 3x3-block network:
+ * W1[64][128] b1[128] W2[128][128] b2[128] W3[128][out] b3[out]. */
 int64_t lm_precond_num_weights(void);
+/* Block size, number of weights, inputs and outputs of a network (any may be NULL). */
+int lm_precond_describe(lm_precond* p, int* block_dim, int64_t* num_weights, int* inputs,
+                        int* outputs);
 int lm_precond_init_weights(lm_precond* p, uint64_t seed);
 int lm_precond_set_weights(lm_precond* p, const float* weights, int64_t n);
 int lm_precond_get_weights(lm_precond* p, float* weights, int64_t n);
-/* Mini-batch Adam on (X: n x 48, T9: n x 9) with the GPU forward/backward
+/* Mini-batch Adam on (X: n x 64, T: n x d*d) with the GPU forward/backward
  * kernels; loss_history (epochs entries, may be NULL) gets each epoch's mean
  * loss sum_ij (G - T)_ij^2. */
-int lm_precond_train(lm_precond* p, const float* X, const float* T9, int64_t n, int epochs,
+int lm_precond_train(lm_precond* p, const float* X, const float* T, int64_t n, int epochs,
                      int batch, float learning_rate, uint64_t seed, float* loss_history);
-/* Network outputs (n x 6) for inputs X (n x 48). */
-int lm_precond_forward(lm_precond* p, const float* X, int64_t n, float* out6);
-/* Mean loss and its gradient w.r.t. all weights (lm_precond_num_weights
- * entries) for one batch. */
-int lm_precond_loss_grad(lm_precond* p, const float* X, const float* T9, int64_t n, float* grad,
+/* Network outputs (n x outputs) for inputs X (n x 64). */
+int lm_precond_forward(lm_precond* p, const float* X, int64_t n, float* out);
+/* Mean loss and its gradient w.r.t. all weights for one batch (T: n x d*d). */
+int lm_precond_loss_grad(lm_precond* p, const float* X, const float* T, int64_t n, float* grad,
                          float* loss);
 /* Training pairs from the solver's current positions: features of ncenters
- * random particles and the center 3x3 block of (M_local/t0)^{-1/2} of each
- * (open RPY, particle + 16 neighbours, FP64 on the device). */
-int lm_nbody_precond_dataset(lm_nbody* h, int64_t ncenters, uint64_t seed, float* X, float* T9);
+ * random particles and the center d x d block (block_dim 3 or 6) of the local
+ * (M/t0)^{-1/2} of each (open or bottom-wall mobility by the particle's
+ * system; the particle and its 16 nearest neighbours for 3x3 blocks, 8 for
+ * 6x6 blocks, whose linear and angular entries are scaled by sqrt(t0) and
+ * sqrt(r0); FP64 on the device).  status (ncenters ints, may be NULL) gets 0
+ * where the local iteration converged; without it a center that did not
+ * converge is an error. */
+int lm_nbody_precond_dataset(lm_nbody* h, int block_dim, int64_t ncenters, uint64_t seed,
+                             float* X, float* T, int* status);
 /* Attach (p != NULL) or detach (p == NULL) a preconditioner; p must outlive
- * its use by h. */
+ * its use by h.  6x6 blocks need a solver initialised with includeAngular. */
 int lm_nbody_set_preconditioner(lm_nbody* h, lm_precond* p);
-/* The blocks G (9N doubles, row-major 3x3 per particle) for the positions. */
-int lm_nbody_precond_blocks(lm_nbody* h, double* G9);
+/* The blocks G of the attached network for the positions: d*d doubles per
+ * particle, row-major (6x6 blocks: G = L L^T diag(1, 1, 1, c, c, c) with
+ * c = 2a/sqrt(3), acting on (linear, angular) entries). */
+int lm_nbody_precond_blocks(lm_nbody* h, double* G);
 /* The 16 nearest neighbours of every particle within its system (16N ints,
  * nearest first, ties by index; -1 past the system's other particles), the
  * network's input geometry. */

@@ -70,6 +70,8 @@ SIGNATURES = {
     "lm_nbody_last_lanczos_iterations": (ctypes.c_int, [_vp]),
     "lm_nbody_synchronize": (ctypes.c_int, [_vp]),
     "lm_precond_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "lm_precond_create_blocks": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "lm_precond_describe": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "lm_precond_destroy": (ctypes.c_int, [_vp]),
     "lm_precond_num_weights": (_i64, []),
     "lm_precond_init_weights": (ctypes.c_int, [_vp, ctypes.c_uint64]),
@@ -79,7 +81,8 @@ SIGNATURES = {
                                         ctypes.c_float, ctypes.c_uint64, _vp]),
     "lm_precond_forward": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
     "lm_precond_loss_grad": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
-    "lm_nbody_precond_dataset": (ctypes.c_int, [_vp, _i64, ctypes.c_uint64, _vp, _vp]),
+    "lm_nbody_precond_dataset": (ctypes.c_int, [_vp, ctypes.c_int, _i64, ctypes.c_uint64, _vp,
+                                                _vp, _vp]),
     "lm_nbody_set_preconditioner": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_precond_blocks": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_precond_neighbours": (ctypes.c_int, [_vp, _vp]),

@@ -23,6 +23,7 @@ struct lm_nbody {
 
 struct lm_precond {
   lm::LearnedPreconditioner net;
+  explicit lm_precond(int kind) : net(kind) {}
 };
 
 namespace {
@@ -240,11 +241,15 @@ int lm_nbody_synchronize(lm_nbody* h) {
   return guarded([&] { LM_HIP(hipStreamSynchronize(S(h).stream())); });
 }
 
-int lm_precond_create(lm_precond** out) {
+int lm_precond_create(lm_precond** out) { return lm_precond_create_blocks(3, out); }
+
+int lm_precond_create_blocks(int block_dim, lm_precond** out) {
   return guarded([&] {
     if (!out) throw lm::Error("[libmobility_amd] null output handle");
     *out = nullptr;
-    *out = new lm_precond();
+    if (block_dim != 3 && block_dim != 6)
+      throw lm::Error("[libmobility_amd] block size must be 3 or 6");
+    *out = new lm_precond(block_dim == 6 ? 1 : 0);
   });
 }
 
@@ -252,11 +257,21 @@ int lm_precond_destroy(lm_precond* p) {
   return guarded([&] { delete p; });
 }
 
-int64_t lm_precond_num_weights(void) { return lm::kPrecondNumWeights; }
+int64_t lm_precond_num_weights(void) { return lm::precond_layout(0).total; }
 
 static lm::LearnedPreconditioner& P(lm_precond* p) {
   if (!p) throw lm::Error("[libmobility_amd] null preconditioner handle");
   return p->net;
+}
+
+int lm_precond_describe(lm_precond* p, int* block_dim, int64_t* num_weights, int* inputs,
+                        int* outputs) {
+  return guarded([&] {
+    if (block_dim) *block_dim = P(p).blockDim();
+    if (num_weights) *num_weights = P(p).numWeights();
+    if (inputs) *inputs = lm::kPrecondIn;
+    if (outputs) *outputs = P(p).outputs();
+  });
 }
 
 int lm_precond_init_weights(lm_precond* p, uint64_t seed) {
@@ -276,35 +291,40 @@ int lm_precond_train(lm_precond* p, const float* X, const float* T9, int64_t n, 
   return guarded([&] { P(p).train(X, T9, n, epochs, batch, lr, seed, loss_history, nullptr); });
 }
 
-int lm_precond_forward(lm_precond* p, const float* X, int64_t n, float* out6) {
+int lm_precond_forward(lm_precond* p, const float* X, int64_t n, float* out) {
   return guarded([&] {
+    const int no = P(p).outputs();
     lm::DevBuf x, o;
     x.reserve(sizeof(float) * n * lm::kPrecondIn);
-    o.reserve(sizeof(float) * n * lm::kPrecondOut);
+    o.reserve(sizeof(float) * n * no);
     LM_HIP(hipMemcpy(x.as<float>(), X, sizeof(float) * n * lm::kPrecondIn, hipMemcpyHostToDevice));
     P(p).forward(x.as<float>(), (int)n, o.as<float>(), nullptr);
-    LM_HIP(hipMemcpy(out6, o.as<float>(), sizeof(float) * n * lm::kPrecondOut, hipMemcpyDeviceToHost));
+    LM_HIP(hipMemcpy(out, o.as<float>(), sizeof(float) * n * no, hipMemcpyDeviceToHost));
   });
 }
 
-int lm_precond_loss_grad(lm_precond* p, const float* X, const float* T9, int64_t n, float* grad,
+int lm_precond_loss_grad(lm_precond* p, const float* X, const float* T, int64_t n, float* grad,
                          float* loss) {
   return guarded([&] {
+    const int td = P(p).blockDim() * P(p).blockDim();
     lm::DevBuf x, t;
     x.reserve(sizeof(float) * n * lm::kPrecondIn);
-    t.reserve(sizeof(float) * n * 9);
+    t.reserve(sizeof(float) * n * td);
     LM_HIP(hipMemcpy(x.as<float>(), X, sizeof(float) * n * lm::kPrecondIn, hipMemcpyHostToDevice));
-    LM_HIP(hipMemcpy(t.as<float>(), T9, sizeof(float) * n * 9, hipMemcpyHostToDevice));
+    LM_HIP(hipMemcpy(t.as<float>(), T, sizeof(float) * n * td, hipMemcpyHostToDevice));
     const float l = P(p).lossGrad(x.as<float>(), t.as<float>(), (int)n, nullptr);
     if (loss) *loss = l;
-    LM_HIP(hipMemcpy(grad, P(p).grad(), sizeof(float) * lm::kPrecondNumWeights, hipMemcpyDeviceToHost));
+    LM_HIP(hipMemcpy(grad, P(p).grad(), sizeof(float) * P(p).numWeights(), hipMemcpyDeviceToHost));
   });
 }
 
-int lm_nbody_precond_dataset(lm_nbody* h, int64_t ncenters, uint64_t seed, float* X, float* T9) {
+int lm_nbody_precond_dataset(lm_nbody* h, int block_dim, int64_t ncenters, uint64_t seed,
+                             float* X, float* T, int* status) {
   return guarded([&] {
-    if (!X || !T9) throw lm::Error("[libmobility_amd] null output");
-    S(h).precondDataset(ncenters, seed, X, T9);
+    if (!X || !T) throw lm::Error("[libmobility_amd] null output");
+    if (block_dim != 3 && block_dim != 6)
+      throw lm::Error("[libmobility_amd] block size must be 3 or 6");
+    S(h).precondDataset(block_dim == 6 ? 1 : 0, ncenters, seed, X, T, status);
   });
 }
 

@@ -38,6 +38,7 @@ struct PairArgs {
 };
 
 constexpr int kPairThreads = 256;  // threads per workgroup = sources per LDS tile
+constexpr int kChunkGrain = 64;    // source chunks of the ordered kernel: multiples of 64
 
 void launch_pair(const PairArgs& a, hipStream_t s);
 
@@ -160,6 +161,16 @@ struct ReduceScratch {
 // Largest Krylov dimension the device-side Lanczos state can hold.
 constexpr int kLanczosMaxIterations = 1024;
 
+// Convergence state of a Lanczos solve, decided on the device by the estimate
+// kernel and copied to the host after each check.
+struct LanczosStatus {
+  int conv;       // 0 iterating, 1 converged, 2 NaN estimate
+  int conv_iter;  // iteration that converged (-1 before)
+  int cur;        // estimate buffer holding the previous estimate (oldBz)
+  int checked;    // last iteration whose estimate ran
+  double err;     // its error estimate
+};
+
 // Lanczos scalar state resident on the device.
 struct LanczosScalars {
   double normz;    // ||z||
@@ -167,10 +178,12 @@ struct LanczosScalars {
   double beta[kLanczosMaxIterations];   // hsup
   double err;      // last error estimate
   double errnum, errden;
+  LanczosStatus status;
 };
 
+// ||z|| and v0 = z / ||z||; resets the status (previous estimate in buffer `cur`)
 void launch_lanczos_start(const double* z, double* v0, int64_t n, ReduceScratch rs,
-                          LanczosScalars* sc, hipStream_t s);
+                          LanczosScalars* sc, int cur, hipStream_t s);
 // w -= beta[i-1] v_prev (if i > 0); alpha[i] = w . v_i
 void launch_lanczos_alpha(double* w, const double* v_i, const double* v_prev, int64_t n, int i,
                           ReduceScratch rs, LanczosScalars* sc, hipStream_t s);
@@ -182,16 +195,20 @@ void launch_lanczos_next(const double* w, double* v_next, int64_t n, int i,
                          const LanczosScalars* sc, hipStream_t s);
 // s[0..m) = T_m^{1/2} e1 (FP64, one workgroup); work has tsqrt_scratch(m) doubles
 constexpr int kTsqrtNodes = 512;
-constexpr int64_t tsqrt_scratch(int m) { return (int64_t)m * kTsqrtNodes; }
+constexpr int64_t tsqrt_scratch(int m) { return (int64_t)m * (kTsqrtNodes + kTsqrtNodes / 64); }
 // The same kernel on host arrays alpha (m) and beta (m - 1): out = T^{1/2} e1
 // (synchronous; temporary device buffers).  For tests.
 void tridiag_sqrt_selftest(const double* alpha, const double* beta, int m, double* out);
 void launch_tridiag_sqrt(const LanczosScalars* sc, int m, double* s_out, double* work,
                          hipStream_t s);
-// y = normz * V[:, 0:m] s ; err = ||y - old|| / ||old||
+// Check of iteration i (m = i + 1 columns), skipped once the status says
+// converged: with old = buf[cur] and y = buf[1 - cur], y = normz V[:, 0:m] s,
+// old <- old - y, err = ||old - y|| / ||old|| (LanczosAlgorithm.cu:158-167,
+// 202-222); from i = 1 on, err <= tol converges (y is the result), NaN
+// fails, anything else makes y the previous estimate (cur flips, cu:196).
 void launch_lanczos_estimate(const double* V, int64_t ldv, int m, const double* s_coef,
-                             double* y, const double* old, int64_t n, ReduceScratch rs,
-                             LanczosScalars* sc, hipStream_t s);
+                             double* buf0, double* buf1, int64_t n, int i, double tol,
+                             ReduceScratch rs, LanczosScalars* sc, hipStream_t s);
 // out[k] += pref * y[k]
 void launch_axpy_accumulate(double* out, const double* y, double pref, int64_t n, hipStream_t s);
 // dst = src + c * w

@@ -113,7 +113,7 @@ static inline int reduce_blocks(int64_t n) {
 // ||z|| -> sc->normz
 __global__ __launch_bounds__(kRT) void k_norm_start(const double* __restrict__ z, int64_t n,
                                                    double* partial, unsigned* counter,
-                                                   LanczosScalars* sc) {
+                                                   LanczosScalars* sc, int cur) {
   __shared__ double sh[kRT / 64];
   double v = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT)
@@ -123,6 +123,11 @@ __global__ __launch_bounds__(kRT) void k_norm_start(const double* __restrict__ z
     const double t = gather_partials(partial, 0, gridDim.x, sh);
     if (threadIdx.x == 0) {
       sc->normz = sqrt(t);
+      sc->status.conv = 0;
+      sc->status.conv_iter = -1;
+      sc->status.cur = cur;
+      sc->status.checked = -1;
+      sc->status.err = 0.0;
       *counter = 0;
     }
   }
@@ -137,9 +142,10 @@ __global__ void k_scale_start(const double* __restrict__ z, double* __restrict__
 }
 
 void launch_lanczos_start(const double* z, double* v0, int64_t n, ReduceScratch rs,
-                          LanczosScalars* sc, hipStream_t s) {
+                          LanczosScalars* sc, int cur, hipStream_t s) {
   const int nb = reduce_blocks(n);
-  hipLaunchKernelGGL(k_norm_start, dim3(nb), dim3(kRT), 0, s, z, n, rs.partial, rs.counter, sc);
+  hipLaunchKernelGGL(k_norm_start, dim3(nb), dim3(kRT), 0, s, z, n, rs.partial, rs.counter, sc,
+                     cur);
   LM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_scale_start, dim3(nb), dim3(kRT), 0, s, z, v0, n, sc);
   LM_HIP(hipGetLastError());
@@ -230,8 +236,21 @@ void launch_lanczos_next(const double* w, double* v_next, int64_t n, int i,
 }
 
 // ---- tridiagonal square root (one workgroup) -----------------------------------
-constexpr int kTS = kTsqrtNodes;  // threads = quadrature nodes
+constexpr int kTS = kTsqrtNodes;  // threads: bracket points, the first K are quadrature nodes
+constexpr int kTSWaves = kTS / 64;
+constexpr int kTSChunk = 16;      // multipliers prefetched per step of the top-down pass
 
+// 1/x to within an ulp: the hardware reciprocal refined by two Newton steps
+// (each doubles the correct bits).
+__device__ inline double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Number of eigenvalues of T below x (Sturm sequence of the LDL^T pivots).
 __device__ inline int sturm_count(const double* a, const double* b2, int m, double x,
                                   double pivmin) {
   int cnt = 0;
@@ -239,7 +258,7 @@ __device__ inline int sturm_count(const double* a, const double* b2, int m, doub
   if (fabs(q) < pivmin) q = -pivmin;
   cnt += q < 0.0;
   for (int i = 1; i < m; ++i) {
-    q = (a[i] - x) - b2[i - 1] / q;
+    q = (a[i] - x) - b2[i - 1] * rcp64(q);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
   }
@@ -253,83 +272,66 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
                                   double* __restrict__ s_out, double* __restrict__ work) {
   __shared__ double a[kLanczosMaxIterations], b[kLanczosMaxIterations],
       b2[kLanczosMaxIterations];
-  __shared__ double red[kTS / 64];
-  __shared__ int ired[kTS / 64];
-  __shared__ double s_lo, s_hi;
+  __shared__ double rhi[kTSWaves], rb[kTSWaves];
+  __shared__ int ired[kTSWaves];
+  __shared__ double s_hi, s_pivmin;
   __shared__ int s_bad;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   for (int i = t; i < m; i += kTS) {
     a[i] = alpha[i];
     b[i] = (i < m - 1) ? beta[i] : 0.0;
     b2[i] = b[i] * b[i];
   }
   __syncthreads();
-  // Gershgorin interval
-  double glo = 1e300, ghi = -1e300, bmax = 0.0;
+  // Gershgorin upper bound of the spectrum
+  double ghi = -1e300, bmax = 0.0;
   for (int i = t; i < m; i += kTS) {
     const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
-    glo = fmin(glo, a[i] - r);
     ghi = fmax(ghi, a[i] + r);
     bmax = fmax(bmax, b2[i]);
   }
   for (int o = 32; o > 0; o >>= 1) {
-    glo = fmin(glo, __shfl_xor(glo, o));
     ghi = fmax(ghi, __shfl_xor(ghi, o));
     bmax = fmax(bmax, __shfl_xor(bmax, o));
   }
-  __shared__ double rlo[kTS / 64], rhi[kTS / 64], rb[kTS / 64];
-  if ((t & 63) == 0) {
-    rlo[t >> 6] = glo;
-    rhi[t >> 6] = ghi;
-    rb[t >> 6] = bmax;
+  if (lane == 0) {
+    rhi[wid] = ghi;
+    rb[wid] = bmax;
   }
   __syncthreads();
   if (t == 0) {
-    double lo = rlo[0], hi = rhi[0], bm = rb[0];
-    for (int w = 1; w < kTS / 64; ++w) {
-      lo = fmin(lo, rlo[w]);
+    double hi = rhi[0], bm = rb[0];
+    for (int w = 1; w < kTSWaves; ++w) {
       hi = fmax(hi, rhi[w]);
       bm = fmax(bm, rb[w]);
     }
-    const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
-    s_lo = lo - pad;
-    s_hi = hi + pad;
-    red[0] = bm;
-    const double pivmin = 1e-290 * fmax(1.0, bm);
-    s_bad = sturm_count(a, b2, m, 0.0, pivmin) > 0;  // an eigenvalue below zero
+    s_hi = hi + 1e-12 * fabs(hi) + 1e-300;
+    s_pivmin = 1e-290 * fmax(1.0, bm);
   }
   __syncthreads();
-  const double pivmin = 1e-290 * fmax(1.0, red[0]);
+  const double lmax = s_hi, pivmin = s_pivmin;
+  // One round brackets the smallest eigenvalue: thread 0 counts the
+  // eigenvalues below 0 (a non-SPD T gives NaN), threads 1..kTS-1 the
+  // eigenvalues below x_t = lmax 1e-16^{1 - t/(kTS-1)}, log-spaced 7.5% apart
+  // from the floor 1e-16 lmax (eigenvalues below it, a numerically singular
+  // operator, are treated as lying on it) up to lmax.
+  const double floor_l = 1e-16 * lmax;
+  const double xt = (t == 0) ? 0.0 : lmax * exp(-36.841361487904734 * (1.0 - (double)t / (kTS - 1)));
+  const int cnt = sturm_count(a, b2, m, xt, pivmin);
+  int first = (t > 0 && cnt >= 1) ? t : kTS;
+  for (int o = 32; o > 0; o >>= 1) first = min(first, __shfl_xor(first, o));
+  if (lane == 0) ired[wid] = first;
+  if (t == 0) s_bad = cnt > 0;
+  __syncthreads();
   if (s_bad) {
     for (int i = t; i < m; i += kTS) s_out[i] = NAN;
     return;
   }
-  // Multisection for the smallest eigenvalue: each round over kTS points
-  // shrinks the bracket [L, H] by kTS; rounds continue until L > 0 brackets it
-  // within a factor 2 (at most 8 rounds: (range) / kTS^8 is far below the
-  // 1e-16 floor).  The lower end L is a lower bound.
-  double L = s_lo, H = s_hi;
-  for (int round = 0; round < 8; ++round) {
-    if (round >= 2 && L > 0.0 && H - L <= 0.5 * L) break;  // uniform over the workgroup
-    const double x = L + (H - L) * (double)(t + 1) / (double)kTS;
-    const int c = sturm_count(a, b2, m, x, pivmin);
-    int first = (c >= 1) ? t : kTS;
-    for (int o = 32; o > 0; o >>= 1) first = min(first, __shfl_xor(first, o));
-    __syncthreads();
-    if ((t & 63) == 0) ired[t >> 6] = first;
-    __syncthreads();
-    int f = ired[0];
-    for (int w = 1; w < kTS / 64; ++w) f = min(f, ired[w]);
-    if (f >= kTS) f = kTS - 1;
-    const double nH = L + (H - L) * (double)(f + 1) / (double)kTS;
-    const double nL = L + (H - L) * (double)f / (double)kTS;
-    L = nL;
-    H = nH;
-  }
-  // Eigenvalues below 1e-16 of the largest (a numerically singular operator)
-  // are treated as lying on that floor.
-  const double lmin = fmax(L, 1e-300 + 1e-16 * s_hi);
-  const double lmax = s_hi;
+  int f = ired[0];
+  for (int w = 1; w < kTSWaves; ++w) f = min(f, ired[w]);
+  const double lmin =
+      (f <= 1 || f >= kTS) ? floor_l
+                            : fmax(floor_l, lmax * exp(-36.841361487904734 * (1.0 - (double)(f - 1) / (kTS - 1))));
   // Nodes u_k = c + s sinh(t_k), t_k = (k - half) h: the map makes the sech
   // tails of the integrand decay double-exponentially; s and h grow and
   // shrink with log10 of the condition number, from 71 nodes at kappa 1e2 to
@@ -344,6 +346,7 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
     h = 4.0 / (double)half;
   }
   const int K = 2 * half + 1;
+  const int nw = (K + 63) / 64;  // waves holding nodes
   const double c0 = 0.25 * (log(lmin) + log(lmax));
   // T^{1/2} e1 = sum_k w_k T x_k with (sig_k I + T) x_k = e1.  Each node forms
   // its own T x_k: as e1 - sig_k x_k below the geometric middle of the
@@ -351,60 +354,76 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
   // and multiplying by T once amplifies the solves' rounding by about
   // sqrt(kappa): 2e-10 instead of 2e-13 at kappa 1e16.)
   const double tau = sqrt(lmin * lmax);
-  double* q = work;  // [m][kTS]: UL multipliers, then weighted entries of T x_k
-  if (t < K) {
+  double* q = work;                          // [m][kTS]: UL multipliers
+  double* part = work + (int64_t)m * kTS;    // [m][kTSWaves]: per-wave node sums
+  const bool node = t < K;
+  double sig = 0.0, wgt = 0.0, rg = 0.0;
+  if (node) {
     const double tk = (double)(t - half) * h;
     const double u = c0 + sc_s * sinh(tk);
-    const double sig = exp(2.0 * u);
-    const double wgt = (2.0 * h / M_PI) * exp(u) * sc_s * cosh(tk);
+    sig = exp(2.0 * u);
+    wgt = (2.0 * h / M_PI) * exp(u) * sc_s * cosh(tk);
     // (sig I + T) x = e1 by UL elimination: g_{m-1} = sig + a_{m-1},
     // g_i = sig + a_i - b_i^2 / g_{i+1}; x_0 = 1/g_0, x_i = q_i x_{i-1} with
-    // q_i = -b_{i-1} / g_i.  The second pass only multiplies.
-    double rg = 1.0 / (sig + a[m - 1]);
+    // q_i = -b_{i-1} / g_i.
+    rg = rcp64(sig + a[m - 1]);
     if (m > 1) q[(int64_t)(m - 1) * kTS + t] = -b[m - 2] * rg;
     for (int i = m - 2; i >= 0; --i) {
-      rg = 1.0 / (sig + a[i] - b2[i] * rg);
+      rg = rcp64(sig + a[i] - b2[i] * rg);
       if (i >= 1) q[(int64_t)i * kTS + t] = -b[i - 1] * rg;
     }
-    if (sig < tau) {
-      // T x = e1 - sig x
-      double x = rg;
-      q[t] = wgt * (1.0 - sig * x);
-      for (int i = 1; i < m; ++i) {
-        x *= q[(int64_t)i * kTS + t];
-        q[(int64_t)i * kTS + t] = -wgt * sig * x;
+  }
+  const bool small = sig < tau;
+  // Top-down pass: step i forms x_i = q_i x_{i-1} and emits (T x)_{i-1}; the
+  // node sums of each step are reduced per wave in a fixed order.
+  double xp2 = 0.0, xp = rg;  // x_{i-2}, x_{i-1}
+  for (int i0 = 1; i0 <= m; i0 += kTSChunk) {
+    double qv[kTSChunk], yv[kTSChunk];
+#pragma unroll
+    for (int k = 0; k < kTSChunk; ++k) {
+      const int i = i0 + k;
+      qv[k] = (node && i < m) ? q[(int64_t)i * kTS + t] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kTSChunk; ++k) {
+      const int i = i0 + k;
+      yv[k] = 0.0;
+      if (i <= m) {
+        const int j = i - 1;
+        const double xi = (i < m) ? xp * qv[k] : 0.0;
+        double y;
+        if (small) {
+          y = (j == 0 ? 1.0 : 0.0) - sig * xp;
+        } else {
+          y = a[j] * xp + (i < m ? b[j] * xi : 0.0);
+          if (j >= 1) y += b[j - 1] * xp2;
+        }
+        yv[k] = wgt * y;
+        xp2 = xp;
+        xp = xi;
       }
-    } else {
-      // (T x)_{i-1} = b_{i-2} x_{i-2} + a_{i-1} x_{i-1} + b_{i-1} x_i, written
-      // once x_i is known (over entry i-1 of q, whose multiplier is used)
-      double xm2 = 0.0, xm1 = rg;
-      for (int i = 1; i < m; ++i) {
-        const double x = xm1 * q[(int64_t)i * kTS + t];
-        double y = a[i - 1] * xm1 + b[i - 1] * x;
-        if (i >= 2) y += b[i - 2] * xm2;
-        q[(int64_t)(i - 1) * kTS + t] = wgt * y;
-        xm2 = xm1;
-        xm1 = x;
-      }
-      double y = a[m - 1] * xm1;
-      if (m >= 2) y += b[m - 2] * xm2;
-      q[(int64_t)(m - 1) * kTS + t] = wgt * y;
+    }
+#pragma unroll
+    for (int k = 0; k < kTSChunk; ++k)
+      for (int o = 32; o > 0; o >>= 1) yv[k] += __shfl_xor(yv[k], o);
+    if (lane == 0 && wid < nw) {
+#pragma unroll
+      for (int k = 0; k < kTSChunk; ++k)
+        if (i0 + k <= m) part[(int64_t)(i0 + k - 1) * kTSWaves + wid] = yv[k];
     }
   }
   __syncthreads();
-  // (T^{1/2} e1)_j = sum_k q[j][k] : one wave per component, fixed order
-  const int lane = t & 63, wid = t >> 6;
-  for (int j = wid; j < m; j += kTS / 64) {
+  for (int j = t; j < m; j += kTS) {
     double v = 0.0;
-    for (int k = lane; k < K; k += 64) v += q[(int64_t)j * kTS + k];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane == 0) s_out[j] = v;
+    for (int w = 0; w < nw; ++w) v += part[(int64_t)j * kTSWaves + w];
+    s_out[j] = v;
   }
 }
 
 __global__ __launch_bounds__(kTS) void k_tridiag_sqrt(const LanczosScalars* __restrict__ sc,
                                                       int m, double* __restrict__ s_out,
                                                       double* __restrict__ work) {
+  if (sc->status.conv) return;  // an earlier check converged
   tridiag_sqrt_body(sc->alpha, sc->beta, m, s_out, work);
 }
 
@@ -467,14 +486,19 @@ void launch_seg_tridiag_sqrt(const int* list, int nlist, int m, const SegScalars
 }
 
 // y = normz V[:, :m] s ; old <- old - y ; err = ||old - y|| / ||old||
-// (LanczosAlgorithm.cu:158-167 and :202-222)
+// (LanczosAlgorithm.cu:158-167 and :202-222), then the convergence decision.
 __global__ __launch_bounds__(kRT) void k_estimate(const double* __restrict__ V, int64_t ldv,
                                                  int m, const double* __restrict__ coef,
-                                                 double* __restrict__ y, double* __restrict__ old,
-                                                 int64_t n, double* partial, unsigned* counter,
+                                                 double* __restrict__ buf0,
+                                                 double* __restrict__ buf1, int64_t n, int it,
+                                                 double tol, double* partial, unsigned* counter,
                                                  LanczosScalars* sc) {
   __shared__ double sc_s[kLanczosMaxIterations];
   __shared__ double sh[kRT / 64];
+  if (sc->status.conv) return;  // an earlier check converged (same value in every block)
+  const int cur = sc->status.cur;
+  double* __restrict__ old = cur ? buf1 : buf0;
+  double* __restrict__ y = cur ? buf0 : buf1;
   const double nz = sc->normz;
   for (int k = threadIdx.x; k < m; k += kRT) sc_s[k] = nz * coef[k];
   __syncthreads();
@@ -496,19 +520,31 @@ __global__ __launch_bounds__(kRT) void k_estimate(const double* __restrict__ V, 
     const double tn = gather_partials(partial, 0, gridDim.x, sh);
     const double td = gather_partials(partial, 1, gridDim.x, sh);
     if (threadIdx.x == 0) {
+      const double err = fabs(sqrt(tn) / sqrt(td));
       sc->errnum = tn;
       sc->errden = td;
-      sc->err = fabs(sqrt(tn) / sqrt(td));
+      sc->err = err;
+      sc->status.err = err;
+      sc->status.checked = it;
+      if (it > 0 && isnan(err)) {  // LanczosAlgorithm.cu:217-220
+        sc->status.conv = 2;
+        sc->status.conv_iter = it;
+      } else if (it > 0 && err <= tol) {  // cu:185-192
+        sc->status.conv = 1;
+        sc->status.conv_iter = it;
+      } else {
+        sc->status.cur = 1 - cur;  // oldBz <- Bz (cu:196)
+      }
       *counter = 0;
     }
   }
 }
 
-void launch_lanczos_estimate(const double* V, int64_t ldv, int m, const double* s_coef, double* y,
-                             const double* old, int64_t n, ReduceScratch rs, LanczosScalars* sc,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(k_estimate, dim3(reduce_blocks(n)), dim3(kRT), 0, s, V, ldv, m, s_coef, y,
-                     const_cast<double*>(old), n, rs.partial, rs.counter, sc);
+void launch_lanczos_estimate(const double* V, int64_t ldv, int m, const double* s_coef,
+                             double* buf0, double* buf1, int64_t n, int i, double tol,
+                             ReduceScratch rs, LanczosScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_estimate, dim3(reduce_blocks(n)), dim3(kRT), 0, s, V, ldv, m, s_coef, buf0,
+                     buf1, n, i, tol, rs.partial, rs.counter, sc);
   LM_HIP(hipGetLastError());
 }
 

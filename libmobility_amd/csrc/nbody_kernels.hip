@@ -73,6 +73,9 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
 
   for (int64_t base = it.src0; base < it.src1; base += kPairThreads) {
     const int64_t j = base + tid;
+    // sources of this tile (a chunk's last tile may be partial; its padding
+    // slots hold far records and zero forces and are skipped)
+    const int cnt = (int)min((int64_t)kPairThreads, it.src1 - base);
     __syncthreads();
     if (j < it.src1) {
       s_pos[tid] = pos[j];
@@ -86,7 +89,7 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
     __syncthreads();
     if constexpr (WALL) {
 #pragma unroll (TPL >= 4 ? 1 : 4 / TPL)
-      for (int s = 0; s < kPairThreads; ++s) {
+      for (int s = 0; s < cnt; ++s) {
         const R4 pj = s_pos[s];
         R4 fj, tj;
         if (kNeedF) fj = s_f[s]; else fj = R4{R(0), R(0), R(0), R(0)};
@@ -102,7 +105,8 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
       // group; the overlap values (rare: overlapping spheres and the self
       // pair) are patched in by selects inside that single branch.
       constexpr int SG = TPL >= 4 ? 1 : 4 / TPL;
-      for (int s = 0; s < kPairThreads; s += SG) {
+      const int lim = (cnt + SG - 1) / SG * SG;  // padding slots up to a whole group
+      for (int s = 0; s < lim; s += SG) {
         R dx[SG][TPL], dy[SG][TPL], dz[SG][TPL], r2[SG][TPL], ir[SG][TPL];
         RpyCoef<R> cf[SG][TPL];
         R4 fj[SG], tj[SG];

@@ -121,7 +121,7 @@ void NBodySolver::mdotNoAlloc(const double* F, const double* T, double* U, doubl
                 std::to_string(kNoAllocScratchBytes) + " bytes");
   const int nitems = (int)(tiles64 * nb * nch);
   int64_t cs = (ns + nch - 1) / nch;
-  cs = ((cs + kPairThreads - 1) / kPairThreads) * kPairThreads;
+  cs = ((cs + kChunkGrain - 1) / kChunkGrain) * kChunkGrain;
   PairItem* items = reinterpret_cast<PairItem*>(base);
   hipLaunchKernelGGL(k_uniform_items, dim3((unsigned)((nitems + 255) / 256)), dim3(256), 0,
                      stream_, items, nitems, ns, (int)tiles64, nch, cs, bt, wall_ ? 1 : 0);

@@ -3,15 +3,20 @@
 //
 //   features : for each particle, the positions of its K = 16 nearest
 //              neighbours (same system) relative to it, in units of a, sorted
-//              by distance -> 48 floats.
-//   network  : 48 -> 128 -> 128 -> 6, GELU, BF16 MFMA products with FP32
-//              accumulation; the 6 outputs are a Cholesky factor L of a 3x3
-//              SPD block G = L L^T (diagonal through softplus).
-//   targets  : center 3x3 block of (M_local / t0)^{-1/2}, M_local the open-RPY
-//              mobility of the particle and its K neighbours, computed here in
-//              FP64 (coupled Newton-Schulz iteration in LDS).
-//   use      : Lanczos runs on A = G M G (G block diagonal, symmetric) and
-//              returns G^{-1} A^{1/2} W, whose covariance is M.
+//              by distance (48 floats), then its wall features u = a/max(z, a),
+//              b = clamp(z/a, 0, 1), u^3 (open systems: 0, 1, 0) and zeros:
+//              64 floats.
+//   network  : 64 -> 128 -> 128 -> d(d+1)/2, GELU, BF16 MFMA products with FP32
+//              accumulation; the outputs are a Cholesky factor L of a d x d
+//              SPD block (diagonal through softplus): d = 3 on the linear
+//              entries of a particle, d = 6 on its linear and angular entries.
+//   targets  : center d x d block of (M_local / t0)^{-1/2}, M_local the open or
+//              bottom-wall mobility of the particle and its 16 (d = 3) or 8
+//              (d = 6) nearest neighbours, linear and angular entries scaled
+//              by sqrt(t0) and sqrt(r0), computed here in FP64 (coupled
+//              Newton-Schulz iteration in LDS).
+//   use      : Lanczos runs on A = G M G^T (G block diagonal) and returns
+//              G^{-1} A^{1/2} W, whose covariance is M.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -34,8 +39,8 @@ template <int K>
 __global__ __launch_bounds__(256) void k_knn_cells(
     const double4* __restrict__ rec, const int64_t* __restrict__ sys_off, int nsys,
     const KnnSys* __restrict__ sys, const int* __restrict__ cell_of, const int* __restrict__ perm,
-    const int64_t* __restrict__ start, int64_t N, float* __restrict__ feat,
-    int* __restrict__ nbr) {
+    const int64_t* __restrict__ start, int64_t N, const int* __restrict__ kinds, int wall_all,
+    float* __restrict__ feat, int* __restrict__ nbr) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   int lo = 0, hi = nsys;
@@ -109,18 +114,27 @@ __global__ __launch_bounds__(256) void k_knn_cells(
       fy = (float)(pj.y - pi.y);
       fz = (float)(pj.z - pi.z);
     }
-    feat[i * 3 * K + 3 * k + 0] = fx;
-    feat[i * 3 * K + 3 * k + 1] = fy;
-    feat[i * 3 * K + 3 * k + 2] = fz;
+    feat[i * kPrecondIn + 3 * k + 0] = fx;
+    feat[i * kPrecondIn + 3 * k + 1] = fy;
+    feat[i * kPrecondIn + 3 * k + 2] = fz;
   }
+  // wall features of the particle (records: z = max(z, a)/a >= 1, w = b)
+  const bool wall = kinds ? kinds[lo] != 0 : wall_all != 0;
+  const float u = wall ? (float)(1.0 / pi.z) : 0.f;
+  float* f = feat + i * kPrecondIn + 3 * K;
+  f[0] = u;
+  f[1] = wall ? (float)pi.w : 1.f;
+  f[2] = u * u * u;
+  for (int k = 3 * K + 3; k < kPrecondIn; ++k) feat[i * kPrecondIn + k] = 0.f;
 }
 
 void launch_knn_cells(const double4* rec, const int64_t* sys_off, int nsys, const KnnSys* sys,
                       const int* cell_of, const int* perm, const int64_t* start, int64_t N,
-                      float* feat, int* nbr, hipStream_t s) {
+                      const int* kinds, int wall, float* feat, int* nbr, hipStream_t s) {
   if (N <= 0) return;
+  static_assert(3 * kPrecondK + 3 <= kPrecondIn, "feature layout");
   hipLaunchKernelGGL(k_knn_cells<kPrecondK>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s,
-                     rec, sys_off, nsys, sys, cell_of, perm, start, N, feat, nbr);
+                     rec, sys_off, nsys, sys, cell_of, perm, start, N, kinds, wall, feat, nbr);
   LM_HIP(hipGetLastError());
 }
 
@@ -181,16 +195,14 @@ void knn_grid(const double* xyz4, const int64_t* sys_off, int nsys, std::vector<
 
 
 // ---- FP64 targets: center block of (M_local/t0)^{-1/2} -------------------------
-constexpr int kC = kPrecondK + 1;  // cluster size
-constexpr int kD = 3 * kC;         // 51
-constexpr int kLD = kD + 1;        // padded row length
-
+template <int KD>
 __device__ inline void lds_matmul(const double* A, const double* B, double* C, int tid, int nt) {
-  for (int e = tid; e < kD * kD; e += nt) {
-    const int r = e / kD, c = e % kD;
+  constexpr int LD = KD + 1;
+  for (int e = tid; e < KD * KD; e += nt) {
+    const int r = e / KD, c = e % KD;
     double s = 0.0;
-    for (int k = 0; k < kD; ++k) s += A[r * kLD + k] * B[k * kLD + c];
-    C[r * kLD + c] = s;
+    for (int k = 0; k < KD; ++k) s += A[r * LD + k] * B[k * LD + c];
+    C[r * LD + c] = s;
   }
 }
 
@@ -204,84 +216,116 @@ __device__ inline double block_max(double v, double* red) {
   return m;
 }
 
+// One workgroup per center: the dimensionless local mobility of the center
+// and its KC - 1 nearest neighbours (entries [U; W] per particle with D = 6:
+// UF, sqrt(3)/2 UT, sqrt(3)/2 WF, WT -- the physical blocks scaled by
+// 1/sqrt(t0 t0), 1/sqrt(t0 r0), 1/sqrt(r0 r0); D = 3: UF), open or wall.
 // Coupled Newton-Schulz: Y0 = A/c, Z0 = I;  T = (3I - Z Y)/2;  Y <- Y T,
 // Z <- T Z.  Z -> (A/c)^{-1/2} when the eigenvalues of A/c lie in (0, 1]
 // (c = Gershgorin bound).  Stops when max|Z Y - I| < 1e-13 (status 0) or
-// after 100 steps (status = 1).
+// after 100 steps (status 1).
+template <int KC, int D>
 __global__ __launch_bounds__(256) void k_local_targets(const double4* __restrict__ rec,
                                                        const int* __restrict__ nbr,
                                                        const int64_t* __restrict__ centers,
-                                                       float* __restrict__ target9,
+                                                       const int* __restrict__ wall_of,
+                                                       float* __restrict__ target,
                                                        int* __restrict__ status) {
-  __shared__ double buf[4][kD * kLD];
-  __shared__ double3 P[kC];
+  constexpr int KD = D * KC, LD = KD + 1;
+  constexpr int BLK = D == 6 ? (kUF | kUT | kWT | kWF) : kUF;
+  __shared__ double buf[4][KD * LD];
+  __shared__ double4 P[KC];
   __shared__ double red[4];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int64_t ctr = centers[blockIdx.x];
-  if (tid < kC) {
+  const bool wall = wall_of[blockIdx.x] != 0;
+  if (tid < KC) {
     int64_t j = (tid == 0) ? ctr : (int64_t)nbr[ctr * kPrecondK + tid - 1];
     if (j < 0) j = ctr;
-    const double4 r = rec[j];
-    P[tid] = double3{r.x, r.y, r.z};
+    P[tid] = rec[j];
   }
   __syncthreads();
   double* Y = buf[0];
   double* Z = buf[1];
   double* T = buf[2];
   double* S = buf[3];
-  for (int e = tid; e < kC * kC; e += nt) {
-    const int a = e / kC, b = e % kC;
-    const double dx = P[a].x - P[b].x, dy = P[a].y - P[b].y, dz = P[a].z - P[b].z;
-    for (int col = 0; col < 3; ++col) {
+  const double s3 = 0.86602540378443865;  // sqrt(3)/2
+  for (int e = tid; e < KC * KC; e += nt) {
+    const int a = e / KC, b = e % KC;
+    const double4 pa = P[a], pb = P[b];
+    for (int col = 0; col < D; ++col) {
+      double v[6] = {0, 0, 0, 0, 0, 0};
+      v[col] = 1.0;
       PairAcc<double> acc;
       acc_zero(acc);
-      open_pair<double, kUF>(dx, dy, dz, col == 0 ? 1.0 : 0.0, col == 1 ? 1.0 : 0.0,
-                             col == 2 ? 1.0 : 0.0, 0.0, 0.0, 0.0, acc);
-      Y[(3 * a + 0) * kLD + 3 * b + col] = acc.uf.x;
-      Y[(3 * a + 1) * kLD + 3 * b + col] = acc.uf.y;
-      Y[(3 * a + 2) * kLD + 3 * b + col] = acc.uf.z;
+      if (wall)
+        wall_pair<double, BLK>(pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w, v[0], v[1], v[2],
+                               v[3], v[4], v[5], acc);
+      else
+        open_pair<double, BLK>(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, v[0], v[1], v[2], v[3],
+                               v[4], v[5], acc);
+      double* c = Y + (D * a) * LD + D * b + col;
+      if (D == 3) {
+        c[0 * LD] = acc.uf.x;
+        c[1 * LD] = acc.uf.y;
+        c[2 * LD] = acc.uf.z;
+      } else if (col < 3) {  // a force: U from UF, W from WF
+        c[0 * LD] = acc.uf.x;
+        c[1 * LD] = acc.uf.y;
+        c[2 * LD] = acc.uf.z;
+        c[3 * LD] = s3 * acc.wf.x;
+        c[4 * LD] = s3 * acc.wf.y;
+        c[5 * LD] = s3 * acc.wf.z;
+      } else {  // a torque: U from UT, W from WT
+        c[0 * LD] = s3 * acc.ut.x;
+        c[1 * LD] = s3 * acc.ut.y;
+        c[2 * LD] = s3 * acc.ut.z;
+        c[3 * LD] = acc.wt.x;
+        c[4 * LD] = acc.wt.y;
+        c[5 * LD] = acc.wt.z;
+      }
     }
   }
   __syncthreads();
   double g = 0.0;
-  for (int r = tid; r < kD; r += nt) {
-    double s = 0.0;
-    for (int c = 0; c < kD; ++c) s += fabs(Y[r * kLD + c]);
-    g = fmax(g, s);
+  for (int r = tid; r < KD; r += nt) {
+    double sum = 0.0;
+    for (int c = 0; c < KD; ++c) sum += fabs(Y[r * LD + c]);
+    g = fmax(g, sum);
   }
   const double cb = block_max(g, red);
   const double ic = 1.0 / cb;
-  for (int e = tid; e < kD * kD; e += nt) {
-    const int r = e / kD, c = e % kD;
-    Y[r * kLD + c] *= ic;
-    Z[r * kLD + c] = (r == c) ? 1.0 : 0.0;
+  for (int e = tid; e < KD * KD; e += nt) {
+    const int r = e / KD, c = e % KD;
+    Y[r * LD + c] *= ic;
+    Z[r * LD + c] = (r == c) ? 1.0 : 0.0;
   }
   __syncthreads();
   int converged = 0;
   for (int it = 0; it < 100; ++it) {
-    lds_matmul(Z, Y, T, tid, nt);  // T = Z Y
+    lds_matmul<KD>(Z, Y, T, tid, nt);  // T = Z Y
     __syncthreads();
     double err = 0.0;
-    for (int e = tid; e < kD * kD; e += nt) {
-      const int r = e / kD, c = e % kD;
-      const double v = T[r * kLD + c];
+    for (int e = tid; e < KD * KD; e += nt) {
+      const int r = e / KD, c = e % KD;
+      const double v = T[r * LD + c];
       const double id = (r == c) ? 1.0 : 0.0;
       err = fmax(err, fabs(v - id));
-      T[r * kLD + c] = 0.5 * (3.0 * id - v);
+      T[r * LD + c] = 0.5 * (3.0 * id - v);
     }
     const double m = block_max(err, red);  // includes the barrier for T
     if (m < 1e-13) {
       converged = 1;
       break;
     }
-    lds_matmul(Y, T, S, tid, nt);  // S = Y T
+    lds_matmul<KD>(Y, T, S, tid, nt);  // S = Y T
     __syncthreads();
     {
       double* t = Y;
       Y = S;
       S = t;
     }
-    lds_matmul(T, Z, S, tid, nt);  // S = T Z
+    lds_matmul<KD>(T, Z, S, tid, nt);  // S = T Z
     __syncthreads();
     {
       double* t = Z;
@@ -291,15 +335,20 @@ __global__ __launch_bounds__(256) void k_local_targets(const double4* __restrict
   }
   // (A/t0)^{-1/2} = c^{-1/2} (A/(t0 c))^{-1/2}: scale the center block
   const double sc = sqrt(ic);
-  if (tid < 9) target9[blockIdx.x * 9 + tid] = (float)(sc * Z[(tid / 3) * kLD + (tid % 3)]);
+  if (tid < D * D) target[blockIdx.x * D * D + tid] = (float)(sc * Z[(tid / D) * LD + (tid % D)]);
   if (tid == 0) status[blockIdx.x] = converged ? 0 : 1;
 }
 
 void launch_local_targets(const double4* rec, const int* nbr, const int64_t* centers,
-                          int64_t ncenters, float* target9, int* status, hipStream_t s) {
+                          const int* wall, int64_t ncenters, int kind, float* target, int* status,
+                          hipStream_t s) {
   if (ncenters <= 0) return;
-  hipLaunchKernelGGL(k_local_targets, dim3((unsigned)ncenters), dim3(256), 0, s, rec, nbr,
-                     centers, target9, status);
+  if (kind == 0)
+    hipLaunchKernelGGL((k_local_targets<kPrecondK + 1, 3>), dim3((unsigned)ncenters), dim3(256), 0,
+                       s, rec, nbr, centers, wall, target, status);
+  else
+    hipLaunchKernelGGL((k_local_targets<kPrecondK6 + 1, 6>), dim3((unsigned)ncenters), dim3(256),
+                       0, s, rec, nbr, centers, wall, target, status);
   LM_HIP(hipGetLastError());
 }
 
@@ -403,46 +452,53 @@ void launch_gemm_bf16(int M, int N, int K, const float* A, int64_t sam, int64_t 
 __device__ inline float softplus_f(float x) { return x > 20.f ? x : log1pf(expf(x)); }
 __device__ inline float sigmoid_f(float x) { return 1.f / (1.f + expf(-x)); }
 
-// L = [[s0,0,0],[o3,s1,0],[o4,o5,s2]], s_i = softplus(o_i) + eps
-__device__ inline void head_L(const float* o, float L[3][3]) {
-  L[0][0] = softplus_f(o[0]) + kPrecondEps; L[0][1] = 0.f; L[0][2] = 0.f;
-  L[1][0] = o[3]; L[1][1] = softplus_f(o[1]) + kPrecondEps; L[1][2] = 0.f;
-  L[2][0] = o[4]; L[2][1] = o[5]; L[2][2] = softplus_f(o[2]) + kPrecondEps;
+// L lower triangular d x d: L_ii = softplus(o_i) + eps, then the strictly
+// lower entries in row order: o_d = L_10, o_{d+1} = L_20, o_{d+2} = L_21, ...
+template <int D>
+__device__ inline void head_L(const float* o, float L[D][D]) {
+  int q = D;
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) {
+      if (j == i)
+        L[i][j] = softplus_f(o[i]) + kPrecondEps;
+      else if (j < i)
+        L[i][j] = o[q++];
+      else
+        L[i][j] = 0.f;
+    }
 }
 
 // loss = mean_b sum_ij (G - T)_ij^2 ; dO = d loss / d outputs (divided by B)
-__global__ void k_head_loss(const float* __restrict__ out, const float* __restrict__ tgt9, int Bn,
+template <int D>
+__global__ void k_head_loss(const float* __restrict__ out, const float* __restrict__ tgt, int Bn,
                             float* __restrict__ dout, float* __restrict__ loss_part) {
+  constexpr int NO = D * (D + 1) / 2;
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.f;
   if (b < Bn) {
-    const float* o = out + 6 * b;
-    float L[3][3];
-    head_L(o, L);
-    float G[3][3], R[3][3];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
+    const float* o = out + NO * b;
+    float L[D][D], R[D][D];
+    head_L<D>(o, L);
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) {
         float s = 0.f;
-        for (int k = 0; k < 3; ++k) s += L[i][k] * L[j][k];
-        G[i][j] = s;
-        R[i][j] = s - tgt9[9 * b + 3 * i + j];
+        for (int k = 0; k < D; ++k) s += L[i][k] * L[j][k];
+        R[i][j] = s - tgt[D * D * b + D * i + j];
         l += R[i][j] * R[i][j];
       }
-    // dloss/dG = 2 R (symmetric) ; dloss/dL = 2 (R + R^T) L = 4 R L
-    float dL[3][3];
-    for (int i = 0; i < 3; ++i)
-      for (int k = 0; k < 3; ++k) {
+    // dloss/dG = 2 R ; dloss/dL = 2 (R + R^T) L
+    float* d = dout + NO * b;
+    int q = D;
+    for (int i = 0; i < D; ++i)
+      for (int k = 0; k <= i; ++k) {
         float s = 0.f;
-        for (int j = 0; j < 3; ++j) s += (R[i][j] + R[j][i]) * L[j][k];
-        dL[i][k] = 2.f * s / (float)Bn;
+        for (int j = 0; j < D; ++j) s += (R[i][j] + R[j][i]) * L[j][k];
+        const float dl = 2.f * s / (float)Bn;
+        if (k == i)
+          d[i] = dl * sigmoid_f(o[i]);
+        else
+          d[q++] = dl;
       }
-    float* d = dout + 6 * b;
-    d[0] = dL[0][0] * sigmoid_f(o[0]);
-    d[1] = dL[1][1] * sigmoid_f(o[1]);
-    d[2] = dL[2][2] * sigmoid_f(o[2]);
-    d[3] = dL[1][0];
-    d[4] = dL[2][0];
-    d[5] = dL[2][1];
   }
   // block sum of the loss
   for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
@@ -456,10 +512,13 @@ __global__ void k_head_loss(const float* __restrict__ out, const float* __restri
   }
 }
 
-void launch_head_loss(const float* out, const float* tgt9, int Bn, float* dout, float* loss_part,
-                      hipStream_t s) {
-  hipLaunchKernelGGL(k_head_loss, dim3((unsigned)((Bn + 255) / 256)), dim3(256), 0, s, out, tgt9,
-                     Bn, dout, loss_part);
+void launch_head_loss(const float* out, const float* tgt, int Bn, int kind, float* dout,
+                      float* loss_part, hipStream_t s) {
+  const dim3 grid((unsigned)((Bn + 255) / 256)), block(256);
+  if (kind == 0)
+    hipLaunchKernelGGL(k_head_loss<3>, grid, block, 0, s, out, tgt, Bn, dout, loss_part);
+  else
+    hipLaunchKernelGGL(k_head_loss<6>, grid, block, 0, s, out, tgt, Bn, dout, loss_part);
   LM_HIP(hipGetLastError());
 }
 
@@ -535,78 +594,102 @@ void launch_gather_rows(const float* src, const int64_t* idx, int rows, int cols
   LM_HIP(hipGetLastError());
 }
 
-// ---- per-particle blocks: G = L L^T and G^{-1} (FP64) ----------------------------
-// Systems smaller than K+1 particles (no full neighbourhood) get G = I.
+// ---- per-particle blocks (FP64) --------------------------------------------------
+// 3x3: G = L L^T, G^{-1} = L^{-T} L^{-1}.  6x6: G = L L^T S, G^{-1} = S^{-1}
+// L^{-T} L^{-1}, S = diag(1, 1, 1, c, c, c).  Systems smaller than K + 1
+// particles (no full neighbourhood) get G = I.
+template <int D>
 __global__ void k_make_blocks(const float* __restrict__ out, const int* __restrict__ nbr, int64_t N,
-                              double* __restrict__ G9, double* __restrict__ Gi9) {
+                              double c, double* __restrict__ G, double* __restrict__ Gi) {
+  constexpr int NO = D * (D + 1) / 2;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  double G[3][3];
+  double L[D][D], X[D][D];  // X = L^{-1}
   const bool full = nbr[i * kPrecondK + kPrecondK - 1] >= 0;
   if (full) {
-    float Lf[3][3];
-    head_L(out + 6 * i, Lf);
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) {
-        double s = 0.0;
-        for (int k = 0; k < 3; ++k) s += (double)Lf[a][k] * (double)Lf[b][k];
-        G[a][b] = s;
-      }
+    float Lf[D][D];
+    head_L<D>(out + NO * i, Lf);
+    for (int a = 0; a < D; ++a)
+      for (int b = 0; b < D; ++b) L[a][b] = (double)Lf[a][b];
   } else {
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) G[a][b] = (a == b) ? 1.0 : 0.0;
+    for (int a = 0; a < D; ++a)
+      for (int b = 0; b < D; ++b) L[a][b] = (a == b) ? 1.0 : 0.0;
   }
-  // inverse of the symmetric 3x3 by cofactors
-  const double c00 = G[1][1] * G[2][2] - G[1][2] * G[2][1];
-  const double c01 = G[1][2] * G[2][0] - G[1][0] * G[2][2];
-  const double c02 = G[1][0] * G[2][1] - G[1][1] * G[2][0];
-  const double det = G[0][0] * c00 + G[0][1] * c01 + G[0][2] * c02;
-  const double id = 1.0 / det;
-  double I[3][3];
-  I[0][0] = c00 * id;
-  I[0][1] = (G[0][2] * G[2][1] - G[0][1] * G[2][2]) * id;
-  I[0][2] = (G[0][1] * G[1][2] - G[0][2] * G[1][1]) * id;
-  I[1][0] = c01 * id;
-  I[1][1] = (G[0][0] * G[2][2] - G[0][2] * G[2][0]) * id;
-  I[1][2] = (G[0][2] * G[1][0] - G[0][0] * G[1][2]) * id;
-  I[2][0] = c02 * id;
-  I[2][1] = (G[0][1] * G[2][0] - G[0][0] * G[2][1]) * id;
-  I[2][2] = (G[0][0] * G[1][1] - G[0][1] * G[1][0]) * id;
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) {
-      G9[9 * i + 3 * a + b] = G[a][b];
-      Gi9[9 * i + 3 * a + b] = I[a][b];
+  // forward substitution, column by column: L X = I
+  for (int j = 0; j < D; ++j)
+    for (int r = 0; r < D; ++r) {
+      double s = (r == j) ? 1.0 : 0.0;
+      for (int k = 0; k < r; ++k) s -= L[r][k] * X[k][j];
+      X[r][j] = (r >= j) ? s / L[r][r] : 0.0;
+    }
+  for (int a = 0; a < D; ++a)
+    for (int b = 0; b < D; ++b) {
+      double g = 0.0, gi = 0.0;
+      for (int k = 0; k < D; ++k) {
+        g += L[a][k] * L[b][k];
+        gi += X[k][a] * X[k][b];
+      }
+      if (D == 6) {
+        if (b >= 3) g *= c;
+        if (a >= 3) gi /= c;
+      }
+      G[(int64_t)D * D * i + D * a + b] = g;
+      Gi[(int64_t)D * D * i + D * a + b] = gi;
     }
 }
 
-void launch_make_blocks(const float* out, const int* nbr, int64_t N, double* G9, double* Gi9,
-                        hipStream_t s) {
+void launch_make_blocks(const float* out, const int* nbr, int64_t N, int kind, double a,
+                        double* G, double* Gi, hipStream_t s) {
   if (N <= 0) return;
-  hipLaunchKernelGGL(k_make_blocks, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, out, nbr, N,
-                     G9, Gi9);
+  const dim3 grid((unsigned)((N + 255) / 256)), block(256);
+  if (kind == 0)
+    hipLaunchKernelGGL(k_make_blocks<3>, grid, block, 0, s, out, nbr, N, 1.0, G, Gi);
+  else
+    hipLaunchKernelGGL(k_make_blocks<6>, grid, block, 0, s, out, nbr, N,
+                       2.0 * a / 1.7320508075688772, G, Gi);
   LM_HIP(hipGetLastError());
 }
 
-// y[3i..3i+2] = B_i x[3i..3i+2] for i < N; entries beyond 3N (rotational
-// part of a 6N vector) are copied unchanged.
-__global__ void k_apply_blocks(const double* __restrict__ B9, const double* __restrict__ x,
-                               int64_t N, int64_t n, double* __restrict__ y) {
+// 3x3: y[3i..3i+2] = B_i x[3i..3i+2] (or B_i^T) for i < N; entries beyond 3N
+// (rotational part of a 6N vector) are copied unchanged.  6x6: the block acts
+// on (x[3i..3i+2], x[3N+3i..3N+3i+2]).
+template <int D>
+__global__ void k_apply_blocks(const double* __restrict__ B, const double* __restrict__ x,
+                               int64_t N, int64_t n, int transpose, double* __restrict__ y) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < N) {
-    const double* b = B9 + 9 * i;
-    const double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
-    y[3 * i] = b[0] * x0 + b[1] * x1 + b[2] * x2;
-    y[3 * i + 1] = b[3] * x0 + b[4] * x1 + b[5] * x2;
-    y[3 * i + 2] = b[6] * x0 + b[7] * x1 + b[8] * x2;
+    const double* b = B + (int64_t)D * D * i;
+    int64_t idx[D];
+    double v[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      idx[k] = (k < 3) ? 3 * i + k : 3 * N + 3 * i + (k - 3);
+      v[k] = x[idx[k]];
+    }
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) s += (transpose ? b[D * k + r] : b[D * r + k]) * v[k];
+      y[idx[r]] = s;
+    }
   }
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = 3 * N + i; k < n; k += stride) y[k] = x[k];
+  if (D == 3) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = 3 * N + i; k < n; k += stride) y[k] = x[k];
+  }
 }
 
-void launch_apply_blocks(const double* B9, const double* x, int64_t N, int64_t n, double* y,
-                         hipStream_t s) {
+void launch_apply_blocks(const double* B, const double* x, int64_t N, int64_t n, int kind,
+                         int transpose, double* y, hipStream_t s) {
   if (N <= 0) return;
-  hipLaunchKernelGGL(k_apply_blocks, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, B9, x, N, n, y);
+  if (kind == 1 && n != 6 * N)
+    throw Error("[libmobility_amd] 6x6 preconditioner blocks need linear and angular entries");
+  const dim3 grid((unsigned)((N + 255) / 256)), block(256);
+  if (kind == 0)
+    hipLaunchKernelGGL(k_apply_blocks<3>, grid, block, 0, s, B, x, N, n, transpose, y);
+  else
+    hipLaunchKernelGGL(k_apply_blocks<6>, grid, block, 0, s, B, x, N, n, transpose, y);
   LM_HIP(hipGetLastError());
 }
 

@@ -33,7 +33,13 @@ NBodySolver::NBodySolver(int px, int py, int pz) {
     throw Error("[Mobility] Invalid periodicity");
 }
 
-NBodySolver::~NBodySolver() = default;
+NBodySolver::~NBodySolver() {
+  if (chk_stream_) (void)hipStreamSynchronize(chk_stream_);
+  for (auto& e : lz_ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (lz_vec_ev_) (void)hipEventDestroy(lz_vec_ev_);
+  if (chk_stream_) (void)hipStreamDestroy(chk_stream_);
+}
 
 void NBodySolver::setParameters(int algorithm, int nbatch, int nperbatch, bool has_wall_height,
                                 double wall_height, double delta) {
@@ -100,7 +106,8 @@ void NBodySolver::clean() {
   w_.release();
   bufA_.release();
   bufB_.release();
-  y_buf_ = old_buf_ = nullptr;
+  est_valid_ = false;
+  cur_host_ = 0;
   V_cols_ = V_n_ = 0;
 }
 
@@ -228,7 +235,8 @@ void NBodySolver::setPositions(const double* pos, int64_t n3, bool dev) {
 // Work decomposition: target tiles of 256*tpl particles x source chunks.
 // Source chunks per target block: the hint (auto: about 8192 items, ~32
 // workgroups per CU for a short tail on every XCD), clamped to [1, 32] and to
-// one chunk per 256 sources of the largest system.
+// one chunk per 64 sources of the largest system (a small system still
+// spreads over several workgroups per target tile).
 int NBodySolver::resolve_nch(int tpl, int nch_hint, const int* act) const {
   const std::vector<int64_t>& off = sys_off_host_;
   const int nsys = (int)off.size() - 1;
@@ -245,7 +253,7 @@ int NBodySolver::resolve_nch(int tpl, int nch_hint, const int* act) const {
     const int64_t want = 8192;
     nch = (int)std::max<int64_t>(1, (want + std::max<int64_t>(tiles, 1) - 1) / std::max<int64_t>(tiles, 1));
   }
-  const int maxch = (int)std::max<int64_t>(1, (maxn + kPairThreads - 1) / kPairThreads);
+  const int maxch = (int)std::max<int64_t>(1, (maxn + kChunkGrain - 1) / kChunkGrain);
   return std::max(1, std::min({nch, 32, maxch}));
 }
 
@@ -262,7 +270,7 @@ void NBodySolver::build_items(int tpl, int nch, bool persist, std::vector<PairIt
     if (ns <= 0) continue;
     const int kind = s < (int)sys_kind_host_.size() ? sys_kind_host_[s] : (wall_ ? 1 : 0);
     int64_t cs = (ns + nch - 1) / nch;
-    cs = ((cs + kPairThreads - 1) / kPairThreads) * kPairThreads;
+    cs = ((cs + kChunkGrain - 1) / kChunkGrain) * kChunkGrain;
     for (int64_t t0 = b; t0 < e; t0 += bt) {
       for (int c = 0; c < nch; ++c) {
         PairItem it;
@@ -912,7 +920,6 @@ void NBodySolver::setLanczosIterationLimit(int limit) {
 
 int NBodySolver::lanczos_run(const double* z, double** result) {
   const int64_t n = (angular_ ? 6 : 3) * N_;
-  const int64_t N3 = 3 * N_;
   ensure_records();
   // workspace
   red_partial_.reserve(sizeof(double) * 4 * kReduceBlocks);
@@ -924,14 +931,32 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
   tsq_work_.reserve(sizeof(double) * tsqrt_scratch(iter_limit_));
   coef_.reserve(sizeof(double) * iter_limit_);
   w_.reserve(sizeof(double) * n);
+  lz_status_.reserve(kLzSlots * sizeof(LanczosStatus));
+  for (auto& e : lz_ev_)
+    if (!e) LM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const bool lookahead = N_ <= kLanczosLookaheadN;
+  if (lookahead) {
+    if (!chk_stream_) LM_HIP(hipStreamCreateWithFlags(&chk_stream_, hipStreamNonBlocking));
+    if (!lz_vec_ev_) LM_HIP(hipEventCreateWithFlags(&lz_vec_ev_, hipEventDisableTiming));
+    chk_partial_.reserve(sizeof(double) * 4 * kReduceBlocks);
+    if (chk_counter_.capacity() == 0) {
+      chk_counter_.reserve(sizeof(unsigned int) * 64);
+      launch_zero_u32(chk_counter_.as<unsigned int>(), 64, stream_);
+    }
+  }
   ReduceScratch rs{red_partial_.as<double>(), red_counter_.as<unsigned int>()};
+  ReduceScratch rs_chk = lookahead
+                             ? ReduceScratch{chk_partial_.as<double>(), chk_counter_.as<unsigned int>()}
+                             : rs;
+  hipStream_t cs = lookahead ? chk_stream_ : stream_;
   LanczosScalars* sc = scalars_.as<LanczosScalars>();
   // previous estimate (oldBz) keeps its prefix across calls (LanczosAlgorithm.cu:175)
-  if (old_n_ != n || old_buf_ == nullptr) {
-    const int64_t keep = old_buf_ ? std::min(old_n_, n) : 0;
+  if (old_n_ != n || !est_valid_) {
+    const int64_t keep = est_valid_ ? std::min(old_n_, n) : 0;
     std::vector<double> host_prev(keep);
     if (keep > 0) {
-      LM_HIP(hipMemcpyAsync(host_prev.data(), old_buf_, sizeof(double) * keep,
+      const double* prev = cur_host_ ? bufB_.as<double>() : bufA_.as<double>();
+      LM_HIP(hipMemcpyAsync(host_prev.data(), prev, sizeof(double) * keep,
                             hipMemcpyDeviceToHost, stream_));
       LM_HIP(hipStreamSynchronize(stream_));
     }
@@ -941,16 +966,21 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     bufB_.reserve(sizeof(double) * (n + 1));
     launch_zero(bufA_.as<double>(), n + 1, stream_);
     launch_zero(bufB_.as<double>(), n + 1, stream_);
-    old_buf_ = bufA_.as<double>();
-    y_buf_ = bufB_.as<double>();
+    cur_host_ = 0;
     if (keep > 0) {
-      LM_HIP(hipMemcpyAsync(old_buf_, host_prev.data(), sizeof(double) * keep,
+      LM_HIP(hipMemcpyAsync(bufA_.as<double>(), host_prev.data(), sizeof(double) * keep,
                             hipMemcpyHostToDevice, stream_));
       LM_HIP(hipStreamSynchronize(stream_));
     }
     old_n_ = n;
+    est_valid_ = true;
   }
-  auto ensure_cols = [&](int64_t cols) { ensure_basis(n, cols); };
+  // A basis that grows is copied and the old one freed: the check stream
+  // must be done with it first.
+  auto ensure_cols = [&](int64_t cols) {
+    if (lookahead && cols > V_cols_) LM_HIP(hipStreamSynchronize(chk_stream_));
+    ensure_basis(n, cols);
+  };
   if (pre_on()) {
     ensure_pre();
     pc_t_.reserve(sizeof(double) * n);
@@ -959,8 +989,52 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
   ensure_cols(2);
   double* V = V_.as<double>();
   double* w = w_.as<double>();
-  launch_lanczos_start(z, V, n, rs, sc, stream_);  // LanczosAlgorithm.cu:110-118
+  double* buf0 = bufA_.as<double>();
+  double* buf1 = bufB_.as<double>();
+  launch_lanczos_start(z, V, n, rs, sc, cur_host_, stream_);  // LanczosAlgorithm.cu:110-118
   const int check = std::min(check_steps_, iter_limit_ - 2);
+  LanczosStatus* mirror = lz_status_.as<LanczosStatus>();
+  // On every exit the check stream has finished (the next solve resets the
+  // status it reads) and the main stream is ordered after it.
+  auto drain = [&]() {
+    if (!lookahead) return;
+    LM_HIP(hipStreamSynchronize(chk_stream_));
+  };
+  // Reads the status copied after the check of iteration i; returns true
+  // once the solve is finished (the tolerance met at i).
+  auto settle = [&](int i, int slot) -> bool {
+    LM_HIP(hipEventSynchronize(lz_ev_[slot]));
+    const LanczosStatus st = mirror[slot];
+    if (st.checked != i) return false;  // skipped: an earlier check finished the solve
+    if (st.conv == 2) {
+      drain();
+      throw Error("[Lanczos] Unknown error (found NaN in result guess)");
+    }
+    if (i > 0 && cb_) cb_(i, st.err, cb_user_);
+    cur_host_ = st.cur;
+    if (st.conv != 1) return false;
+    // registerRequiredStepsForConverge, LanczosAlgorithm.cu:224-232
+    if (i - 2 > check_steps_)
+      check_steps_ += 1;
+    else
+      check_steps_ = std::max(1, check_steps_ - 2);
+    last_iters_ = i;
+    *result = st.cur ? buf0 : buf1;  // the estimate written by this check
+    drain();
+    return true;
+  };
+  // checks in flight, oldest first: (iteration, slot)
+  int pend_i[kLzSlots], pend_s[kLzSlots];
+  int npend = 0, nchecks = 0;
+  auto settle_oldest = [&]() -> bool {
+    const bool done = settle(pend_i[0], pend_s[0]);
+    for (int k = 1; k < npend; ++k) {
+      pend_i[k - 1] = pend_i[k];
+      pend_s[k - 1] = pend_s[k];
+    }
+    --npend;
+    return done;
+  };
   for (int i = 0; i < iter_limit_; ++i) {
     ensure_cols(i + 2);
     V = V_.as<double>();
@@ -971,28 +1045,44 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     launch_lanczos_next(w, V + (int64_t)(i + 1) * n, n, i, sc, stream_);
     if (i >= check || cb_ != nullptr) {
       const int m = i + 1;
-      launch_tridiag_sqrt(sc, m, coef_.as<double>(), tsq_work_.as<double>(), stream_);
-      launch_lanczos_estimate(V, n, m, coef_.as<double>(), y_buf_, old_buf_, n, rs, sc, stream_);
-      if (i > 0) {
-        double err = 0.0;
-        LM_HIP(hipMemcpyAsync(&err, &sc->err, sizeof(double), hipMemcpyDeviceToHost, stream_));
-        LM_HIP(hipStreamSynchronize(stream_));
-        if (std::isnan(err)) throw Error("[Lanczos] Unknown error (found NaN in result guess)");
-        if (cb_) cb_(i, err, cb_user_);
-        if (err <= tol_) {
-          // registerRequiredStepsForConverge, LanczosAlgorithm.cu:224-232
-          if (i - 2 > check_steps_)
-            check_steps_ += 1;
-          else
-            check_steps_ = std::max(1, check_steps_ - 2);
-          last_iters_ = i;
-          *result = y_buf_;
-          return i;
+      if (lookahead) {
+        LM_HIP(hipEventRecord(lz_vec_ev_, stream_));
+        LM_HIP(hipStreamWaitEvent(chk_stream_, lz_vec_ev_, 0));
+      }
+      launch_tridiag_sqrt(sc, m, coef_.as<double>(), tsq_work_.as<double>(), cs);
+      launch_lanczos_estimate(V, n, m, coef_.as<double>(), buf0, buf1, n, i, tol_, rs_chk, sc, cs);
+      const int slot = nchecks++ % kLzSlots;
+      LM_HIP(hipMemcpyAsync(&mirror[slot], &sc->status, sizeof(LanczosStatus),
+                            hipMemcpyDeviceToHost, cs));
+      LM_HIP(hipEventRecord(lz_ev_[slot], cs));
+      if (!lookahead) {
+        if (settle(i, slot)) return i;
+        continue;
+      }
+      pend_i[npend] = i;
+      pend_s[npend] = slot;
+      ++npend;
+      // read every finished check in order; wait when too many are in flight
+      while (npend > 0) {
+        const bool must = npend > kLanczosMaxLag;
+        if (!must) {
+          const hipError_t q = hipEventQuery(lz_ev_[pend_s[0]]);
+          if (q == hipErrorNotReady) break;
+          if (q != hipSuccess) LM_HIP(q);
+        }
+        const int it = pend_i[0];
+        if (settle_oldest()) {
+          LM_HIP(hipStreamWaitEvent(stream_, lz_ev_[(nchecks - 1) % kLzSlots], 0));
+          return it;
         }
       }
-      std::swap(y_buf_, old_buf_);  // oldBz <- Bz (LanczosAlgorithm.cu:196)
     }
   }
+  while (npend > 0) {
+    const int it = pend_i[0];
+    if (settle_oldest()) return it;
+  }
+  drain();
   throw Error("[Lanczos] Could not converge");
 }
 
@@ -1499,7 +1589,7 @@ void NBodySolver::ensure_knn() {
   upload(knn_start_, knn_start_host_.data(), sizeof(int64_t) * knn_start_host_.size());
   launch_knn_cells(rec64_.as<double4>(), sys_off_.as<int64_t>(), nsys, knn_sys_.as<KnnSys>(),
                    knn_cell_.as<int>(), knn_perm_.as<int>(), knn_start_.as<int64_t>(), N_,
-                   pc_feat_.as<float>(), pc_nbr_.as<int>(), stream_);
+                   kinds_dev(), wall_ ? 1 : 0, pc_feat_.as<float>(), pc_nbr_.as<int>(), stream_);
   knn_layout_ = layout_id_;
 }
 
@@ -1528,12 +1618,17 @@ void NBodySolver::ensure_precond_blocks() {
   if (!precond_) throw Error("[libmobility_amd] no preconditioner attached");
   ensure_records();
   if (pc_version_ == precond_->version() && pre_current(pc_layout_, pc_struct_, pc_a_)) return;
+  const int kind = precond_->kind();
+  if (kind == 1 && !angular_)
+    throw Error("[libmobility_amd] a preconditioner of 6x6 blocks needs a solver initialised with "
+                "includeAngular");
   ensure_knn();
-  pc_out_.reserve(sizeof(float) * N_ * kPrecondOut);
-  pc_G_.reserve(sizeof(double) * 9 * N_);
-  pc_Gi_.reserve(sizeof(double) * 9 * N_);
+  const int d = precond_->blockDim();
+  pc_out_.reserve(sizeof(float) * N_ * precond_->outputs());
+  pc_G_.reserve(sizeof(double) * d * d * N_);
+  pc_Gi_.reserve(sizeof(double) * d * d * N_);
   precond_->forward(pc_feat_.as<float>(), (int)N_, pc_out_.as<float>(), stream_);
-  launch_make_blocks(pc_out_.as<float>(), pc_nbr_.as<int>(), N_, pc_G_.as<double>(),
+  launch_make_blocks(pc_out_.as<float>(), pc_nbr_.as<int>(), N_, kind, a_, pc_G_.as<double>(),
                      pc_Gi_.as<double>(), stream_);
   pc_layout_ = layout_id_;
   pc_struct_ = struct_id_;
@@ -1570,10 +1665,10 @@ void NBodySolver::ensure_pre() {
   ensure_coarse();
 }
 
-// G = D Gc: t = G^T v = Gc^T D v
+// G = D Gc: t = G^T v = Gc^T D^T v
 void NBodySolver::pre_forward(const double* v, double* t, int64_t n) {
   if (precond_) {
-    launch_apply_blocks(pc_G_.as<double>(), v, N_, n, t, stream_);
+    launch_apply_blocks(pc_G_.as<double>(), v, N_, n, precond_->kind(), 1, t, stream_);
     if (coarse_ppc_ > 0) coarse_.apply(CoarseLevel::kGt, t, t, n, stream_);
   } else {
     coarse_.apply(CoarseLevel::kGt, v, t, n, stream_);
@@ -1584,7 +1679,7 @@ void NBodySolver::pre_forward(const double* v, double* t, int64_t n) {
 void NBodySolver::pre_back(double* u, double* w, int64_t n) {
   if (precond_) {
     if (coarse_ppc_ > 0) coarse_.apply(CoarseLevel::kG, u, u, n, stream_);
-    launch_apply_blocks(pc_G_.as<double>(), u, N_, n, w, stream_);
+    launch_apply_blocks(pc_G_.as<double>(), u, N_, n, precond_->kind(), 0, w, stream_);
   } else {
     coarse_.apply(CoarseLevel::kG, u, w, n, stream_);
   }
@@ -1593,7 +1688,7 @@ void NBodySolver::pre_back(double* u, double* w, int64_t n) {
 // out = G^{-1} y = Gc^{-1} D^{-1} y
 void NBodySolver::pre_inverse(const double* y, double* out, int64_t n) {
   if (precond_) {
-    launch_apply_blocks(pc_Gi_.as<double>(), y, N_, n, out, stream_);
+    launch_apply_blocks(pc_Gi_.as<double>(), y, N_, n, precond_->kind(), 0, out, stream_);
     if (coarse_ppc_ > 0) coarse_.apply(CoarseLevel::kGinv, out, out, n, stream_);
   } else {
     coarse_.apply(CoarseLevel::kGinv, y, out, n, stream_);
@@ -1620,40 +1715,58 @@ int NBodySolver::coarseSystemCells(int system) const {
   return coarse_.system_cells(system);
 }
 
-void NBodySolver::precondDataset(int64_t ncenters, uint64_t seed, float* X, float* T9) {
+void NBodySolver::precondDataset(int kind, int64_t ncenters, uint64_t seed, float* X, float* T,
+                                 int* status_out) {
   ensure_knn();
   if (ncenters <= 0 || ncenters > N_) throw Error("[libmobility_amd] invalid number of centers");
+  const int d = precond_dim(kind);
   // distinct random centers
   std::vector<int64_t> idx(N_);
   for (int64_t i = 0; i < N_; ++i) idx[i] = i;
   std::mt19937_64 rng(seed);
   for (int64_t i = 0; i < ncenters; ++i) {
-    std::uniform_int_distribution<int64_t> d(i, N_ - 1);
-    std::swap(idx[i], idx[d(rng)]);
+    std::uniform_int_distribution<int64_t> dist(i, N_ - 1);
+    std::swap(idx[i], idx[dist(rng)]);
   }
-  DevBuf centers, tg, st, xf;
+  // the mobility of each center's system (open or bottom wall)
+  std::vector<int> wall(ncenters);
+  for (int64_t c = 0; c < ncenters; ++c) {
+    const int64_t i = idx[c];
+    const int sys = (int)(std::upper_bound(sys_off_host_.begin(), sys_off_host_.end(), i) -
+                          sys_off_host_.begin()) - 1;
+    wall[c] = sys < (int)sys_kind_host_.size() ? (sys_kind_host_[sys] != 0) : (wall_ ? 1 : 0);
+  }
+  DevBuf centers, walls, tg, st, xf;
   centers.reserve(sizeof(int64_t) * ncenters);
-  tg.reserve(sizeof(float) * 9 * ncenters);
+  walls.reserve(sizeof(int) * ncenters);
+  tg.reserve(sizeof(float) * d * d * ncenters);
   st.reserve(sizeof(int) * ncenters);
   xf.reserve(sizeof(float) * kPrecondIn * ncenters);
   LM_HIP(hipMemcpyAsync(centers.as<int64_t>(), idx.data(), sizeof(int64_t) * ncenters,
                         hipMemcpyHostToDevice, stream_));
-  launch_local_targets(rec64_.as<double4>(), pc_nbr_.as<int>(), centers.as<int64_t>(), ncenters,
-                       tg.as<float>(), st.as<int>(), stream_);
+  LM_HIP(hipMemcpyAsync(walls.as<int>(), wall.data(), sizeof(int) * ncenters,
+                        hipMemcpyHostToDevice, stream_));
+  launch_local_targets(rec64_.as<double4>(), pc_nbr_.as<int>(), centers.as<int64_t>(),
+                       walls.as<int>(), ncenters, kind, tg.as<float>(), st.as<int>(), stream_);
   launch_gather_rows(pc_feat_.as<float>(), centers.as<int64_t>(), (int)ncenters, kPrecondIn,
                      xf.as<float>(), stream_);
   std::vector<int> status(ncenters);
   LM_HIP(hipMemcpyAsync(status.data(), st.as<int>(), sizeof(int) * ncenters, hipMemcpyDeviceToHost, stream_));
-  LM_HIP(hipMemcpyAsync(T9, tg.as<float>(), sizeof(float) * 9 * ncenters, hipMemcpyDeviceToHost, stream_));
+  LM_HIP(hipMemcpyAsync(T, tg.as<float>(), sizeof(float) * d * d * ncenters, hipMemcpyDeviceToHost, stream_));
   LM_HIP(hipMemcpyAsync(X, xf.as<float>(), sizeof(float) * kPrecondIn * ncenters, hipMemcpyDeviceToHost, stream_));
   LM_HIP(hipStreamSynchronize(stream_));
+  if (status_out) {
+    std::copy(status.begin(), status.end(), status_out);
+    return;
+  }
   for (int64_t c = 0; c < ncenters; ++c)
     if (status[c] != 0) throw Error("[libmobility_amd] local inverse square root did not converge");
 }
 
-void NBodySolver::precondBlocks(double* G9) {
+void NBodySolver::precondBlocks(double* G) {
   ensure_precond_blocks();
-  LM_HIP(hipMemcpyAsync(G9, pc_G_.as<double>(), sizeof(double) * 9 * N_, hipMemcpyDeviceToHost, stream_));
+  const int d = precond_->blockDim();
+  LM_HIP(hipMemcpyAsync(G, pc_G_.as<double>(), sizeof(double) * d * d * N_, hipMemcpyDeviceToHost, stream_));
   LM_HIP(hipStreamSynchronize(stream_));
 }
 

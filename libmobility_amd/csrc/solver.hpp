@@ -114,9 +114,12 @@ class NBodySolver {
   void setPreconditioner(LearnedPreconditioner* p) { precond_ = p; pc_layout_ = 0; }
   // Training pairs from the current positions: X (ncenters x 48) and the FP64
   // local targets (ncenters x 9), host arrays.
-  void precondDataset(int64_t ncenters, uint64_t seed, float* X, float* T9);
-  // The 3x3 blocks G (9N, host) the attached network gives for the positions.
-  void precondBlocks(double* G9);
+  // kind 0: 3x3 targets (T: ncenters x 9); 1: 6x6 targets (ncenters x 36).
+  // status (may be null): 0 where the local iteration converged; without it
+  // a center that did not converge is an error.
+  void precondDataset(int kind, int64_t ncenters, uint64_t seed, float* X, float* T, int* status);
+  // The blocks G (d*d per particle, host) the attached network gives.
+  void precondBlocks(double* G);
   // The K nearest neighbours of every particle within its system (N x K,
   // host; -1 past the system's other particles), nearest first.
   void precondNeighbours(int* nbr);
@@ -187,6 +190,11 @@ class NBodySolver {
   void pair_product(const void* rec64, const void* rec32, const double* F, const double* T,
                     double* U, double* W, int prec = -1);
   int lanczos_run(const double* z, double** result);
+  // Systems up to this many particles look one iteration ahead instead of
+  // waiting for each convergence check (the wasted product is cheap there).
+  static constexpr int64_t kLanczosLookaheadN = 32768;
+  // Checks in flight before the host waits for the oldest.
+  static constexpr int kLanczosMaxLag = 4;
   int lanczos_run_seg(SegSolve& S, const std::vector<int64_t>& off, bool angular,
                       bool skip_converged_pairs,
                       const std::function<void(const double*, double*)>& product, const double* z,
@@ -260,9 +268,21 @@ class NBodySolver {
   // Lanczos
   DevBuf V_, w_, coef_, bufA_, bufB_, noise_, red_partial_, red_counter_, scalars_, tsq_work_;
   int64_t V_cols_ = 0, V_n_ = 0;
-  double* y_buf_ = nullptr;
-  double* old_buf_ = nullptr;
+  // the two estimate buffers (bufA_, bufB_); cur_host_ names the one holding
+  // the previous estimate (oldBz), as the device status last reported it
+  int cur_host_ = 0;
+  bool est_valid_ = false;
   int64_t old_n_ = 0;
+  // Small systems run each convergence check (tridiagonal square root and
+  // estimate) on a second stream of default priority, overlapped with the
+  // next iterations' products; the host reads the checks' status as they
+  // complete.  The check has its own reduction scratch.
+  static constexpr int kLzSlots = 8;
+  HostBuf lz_status_;       // pinned copies of the device LanczosStatus, one per slot
+  hipEvent_t lz_ev_[kLzSlots] = {};
+  hipEvent_t lz_vec_ev_ = nullptr;  // main stream: iteration's vectors written
+  hipStream_t chk_stream_ = nullptr;
+  DevBuf chk_partial_, chk_counter_;
   int check_steps_ = 3;          // LanczosAlgorithm.cu:171
   int iter_limit_ = 200;         // LanczosAlgorithm.h:49
   int last_iters_ = 0;

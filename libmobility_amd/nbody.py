@@ -6,7 +6,8 @@ Mirrors the class that ``MOBILITY_PYTHONIFY_WITH_EXTRA_CODE`` generates
 argument names and defaults, shape rules, output framework/device, and
 ``RuntimeError`` texts.  All arithmetic runs in the HIP library through the C
 ABI of include/libmobility_amd.h; arrays that already live on the GPU (torch
-ROCm tensors) are passed by device pointer without copies.
+ROCm tensors, or any DLPack producer's device arrays) are passed by device
+pointer without copies.
 
 Extensions (no reference counterpart): ``setPairPrecision``,
 ``sqrtMdotW_with_noise``, ``hydrodynamicVelocities``, ``setSystems``,
@@ -34,6 +35,20 @@ def _is_torch(x):
     return isinstance(x, torch.Tensor)
 
 
+def _adopt_dlpack(x):
+    """Arrays of other frameworks that speak DLPack (``__dlpack__`` and
+    ``__dlpack_device__``: CuPy, JAX, ...), host or ROCm device memory, are
+    taken over without a copy as torch tensors sharing their memory
+    (torch.from_dlpack); outputs are then torch tensors on the same device.
+    numpy arrays and torch tensors pass unchanged."""
+    if x is None or isinstance(x, np.ndarray) or _is_torch(x):
+        return x
+    if hasattr(x, "__dlpack__") and hasattr(x, "__dlpack_device__"):
+        import torch
+        return torch.from_dlpack(x)
+    return x
+
+
 class _Arr:
     """An FP64, C-contiguous view of a caller array plus where it lives."""
 
@@ -42,6 +57,7 @@ class _Arr:
     def __init__(self, x):
         if x is None:
             x = np.zeros(0)
+        x = _adopt_dlpack(x)
         if _is_torch(x):
             import torch
 
@@ -398,11 +414,12 @@ class NBody(_Solver):
     def setPreconditioner(self, preconditioner="learned"):
         """Extension: attach a learned block preconditioner to sqrtMdotW.
 
-        ``"learned"`` loads the packaged trained weights; a
+        ``"learned"`` loads the packaged trained weights (3x3 blocks, or 6x6
+        blocks when the solver was initialised with includeAngular); a
         :class:`libmobility_amd.precond.LearnedPreconditioner` is used as is;
         ``None`` returns to plain Lanczos.
         """
-        from .precond import DEFAULT_WEIGHTS, LearnedPreconditioner
+        from .precond import default_network
         if preconditioner is None:
             self._precond = None
             _ffi.check(self._lib.lm_nbody_set_preconditioner(self._h, None))
@@ -410,14 +427,16 @@ class NBody(_Solver):
         if isinstance(preconditioner, str):
             if preconditioner != "learned":
                 raise RuntimeError("[libmobility_amd] preconditioner must be 'learned', an object or None")
-            preconditioner = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+            preconditioner = default_network(self._include_angular)
         self._precond = preconditioner  # keep alive while attached
         _ffi.check(self._lib.lm_nbody_set_preconditioner(self._h, preconditioner._h))
 
     def preconditionerBlocks(self):
-        """Extension: the 3x3 blocks G_i (N, 3, 3) of the attached preconditioner."""
+        """Extension: the blocks G_i (N, d, d) of the attached preconditioner
+        (d = 3, or 6 acting on a particle's linear and angular entries)."""
         N = self.getNumberParticles()
-        G = np.zeros((N, 3, 3))
+        d = getattr(self._precond, "block_dim", 3)
+        G = np.zeros((N, d, d))
         _ffi.check(self._lib.lm_nbody_precond_blocks(self._h, G.ctypes.data))
         return G
 

@@ -239,3 +239,31 @@ def test_coarse_cells_match_the_oracle_rule(shape):
     assert nc.value == C and list(grid) == list(g)
     assert np.array_equal(cell_of, ref)
     assert 3 * C <= 24576
+
+
+class _DLWrap:
+    """An array of an unknown framework that only speaks DLPack."""
+
+    def __init__(self, a):
+        self._a = a
+
+    def __dlpack__(self, *args, **kwargs):
+        return self._a.__dlpack__(*args, **kwargs)
+
+    def __dlpack_device__(self):
+        return self._a.__dlpack_device__()
+
+
+def test_dlpack_arrays_are_taken_without_a_copy():
+    """Any DLPack producer (python_tensor.h:60-93 accepts every nanobind
+    framework) is adopted through torch.from_dlpack: the library receives the
+    producer's own pointer (no copy) when it is FP64 and contiguous; other
+    dtypes are converted, as nanobind converts them (test_precision.py:38-62)."""
+    from libmobility_amd import nbody
+    a = np.random.default_rng(0).normal(size=(50, 3))
+    arr = nbody._Arr(_DLWrap(a))
+    assert arr.ptr == a.ctypes.data
+    assert arr.framework == "torch" and arr.mem == nbody._ffi.LM_MEM_HOST and arr.shape == (50, 3)
+    b = a.astype(np.float32)
+    arr32 = nbody._Arr(_DLWrap(b))
+    assert arr32.ptr != b.ctypes.data and np.allclose(arr32.obj.numpy(), b)

@@ -150,10 +150,10 @@ def test_c3_two_level_square_root_shares_the_norm_identity(c3):
     at this size: it takes fewer iterations than plain Lanczos, repeats
     bitwise, and E||B W||^2 = tr(M) = E||M^{1/2} W||^2, both concentrated to
     well under 1% for 3N = 393216 (within 5% asserted)."""
-    from libmobility_amd.precond import DEFAULT_WEIGHTS, LearnedPreconditioner
+    from libmobility_amd.precond import default_network
     pos, _, Wn, _, _ = c3
     s = _nbody(False, "fp32", tol=1e-6)
-    net = LearnedPreconditioner(weights=DEFAULT_WEIGHTS)
+    net = default_network(False)
     s.setPreconditioner(net)
     s.setCoarseLevel(40)
     s.setPositions(pos)

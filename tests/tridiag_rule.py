@@ -7,9 +7,9 @@ include/lanczos/LanczosAlgorithm.cu:48-85).  The HIP kernel
     T^{1/2} e1 = (2/pi) Int_R e^u T (e^{2u} I + T)^{-1} e1 du
 on sinh-mapped trapezoid nodes; this module follows it step for step so that
 its accuracy over condition numbers can be measured on the CPU:
-  * spectrum bounds: Gershgorin above; below, Sturm-count multisection over
-    512 points per round until the bracket holds the smallest eigenvalue
-    within a factor 2 (at most 8 rounds), floored at 1e-16 of the largest;
+  * spectrum bounds: Gershgorin above; below, one round of Sturm counts at
+    511 points log-spaced from 1e-16 of the largest up to it, which brackets
+    the smallest eigenvalue within 7.5% (or puts it on that floor);
   * nodes u_k = c + s sinh(t_k), t_k = (k - half) h, |t_k| <= 4, with
     s = 1.2 + 0.2 log10(kappa), h = 0.18 / (1 + 0.3 log10(kappa)),
     c = (ln lmin + ln lmax) / 4, weights (2h/pi) e^u s cosh(t);
@@ -36,30 +36,28 @@ def _sturm(a, b2, x, pivmin):
 
 
 def bounds(a, b):
-    """(lmin, lmax, rounds) as the kernel brackets them."""
+    """(lmin, lmax) as the kernel brackets them: Gershgorin above; one round of
+    Sturm counts at x_t = lmax 1e-16^{1 - t/511}, t = 1..511 (log-spaced 7.5%
+    apart); lmin = the last point below the smallest eigenvalue, or the floor
+    1e-16 lmax."""
     m = len(a)
     bb = np.zeros(m)
     bb[:m - 1] = b
     b2 = bb * bb
     r = np.abs(np.concatenate([[0.0], bb[:m - 1]])) + np.abs(bb)
-    lo, hi = np.min(a - r), np.max(a + r)
-    pad = 1e-12 * max(abs(lo), abs(hi)) + 1e-300
-    lo, hi = lo - pad, hi + pad
+    hi = np.max(a + r)
+    hi = hi + 1e-12 * abs(hi) + 1e-300
     pivmin = 1e-290 * max(1.0, b2.max())
-    L, H = lo, hi
-    rounds = 0
-    for rnd in range(8):
-        if rnd >= 2 and L > 0 and H - L <= 0.5 * L:
+    floor = 1e-16 * hi
+    f = NODES
+    for t in range(1, NODES):
+        x = hi * np.exp(np.log(1e-16) * (1.0 - t / (NODES - 1)))
+        if _sturm(a, b2, x, pivmin) >= 1:
+            f = t
             break
-        xs = L + (H - L) * np.arange(1, NODES + 1) / NODES
-        f = NODES - 1
-        for t, x in enumerate(xs):
-            if _sturm(a, b2, x, pivmin) >= 1:
-                f = t
-                break
-        L, H = L + (H - L) * f / NODES, L + (H - L) * (f + 1) / NODES
-        rounds += 1
-    return max(L, 1e-300 + 1e-16 * hi), hi, rounds
+    if f <= 1 or f >= NODES:
+        return floor, hi
+    return max(floor, hi * np.exp(np.log(1e-16) * (1.0 - (f - 1) / (NODES - 1)))), hi
 
 
 def nodes(lmin, lmax):
@@ -81,7 +79,7 @@ def sqrt_e1(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     m = len(a)
-    lmin, lmax, _ = bounds(a, b)
+    lmin, lmax = bounds(a, b)
     sig, w = nodes(lmin, lmax)
     tau = np.sqrt(lmin * lmax)
     out = np.zeros(m)
