@@ -218,6 +218,7 @@ C3_TILING = (4, 32)
 # rocprofv3 --pmc, same keys as MEASURED_TRAFFIC_BYTES.
 MEASURED_FP32_FLOPS = {
     ("c3", 131072, "symmetric", 4, 64): 7.134e9 * 64,
+    ("c3", 131072, "symmetric", 4, 32): 7.00029e9 * 64,
 }
 
 MEASURED_TRAFFIC_BYTES = {

@@ -34,6 +34,10 @@ struct PairArgs {
   const PairItem* items;
   int nitems;
   const int* active;    // per-system flags; items of systems with 0 are skipped (nullptr: all)
+  // Non-persistent launches: a workgroup that finds *stop != 0 when it starts
+  // does nothing (products a finished Lanczos solve no longer needs; nullptr:
+  // never).  Its outputs are then undefined.
+  const int* stop;
   double t0, rt0, r0;
 };
 
@@ -161,14 +165,21 @@ struct ReduceScratch {
 // Largest Krylov dimension the device-side Lanczos state can hold.
 constexpr int kLanczosMaxIterations = 1024;
 
+// Convergence checks evaluated by one launch: the square roots of T_m for
+// consecutive m (one workgroup each) and their estimates in one pass over V.
+constexpr int kCheckBatch = 4;
+
 // Convergence state of a Lanczos solve, decided on the device by the estimate
-// kernel and copied to the host after each check.
+// kernel; the kernel also writes it to a host-mapped mirror.
 struct LanczosStatus {
   int conv;       // 0 iterating, 1 converged, 2 NaN estimate
   int conv_iter;  // iteration that converged (-1 before)
-  int cur;        // estimate buffer holding the previous estimate (oldBz)
-  int checked;    // last iteration whose estimate ran
-  double err;     // its error estimate
+  int old;        // estimate slot holding the previous estimate (oldBz)
+  int result;     // slot holding the converged estimate
+  int first;      // first iteration of the last batch of checks
+  int nchk;       // checks of that batch that ran (up to the deciding one)
+  int pad[2];
+  double err[kCheckBatch];  // their error estimates, in order
 };
 
 // Lanczos scalar state resident on the device.
@@ -181,9 +192,35 @@ struct LanczosScalars {
   LanczosStatus status;
 };
 
-// ||z|| and v0 = z / ||z||; resets the status (previous estimate in buffer `cur`)
+// ||z|| and v0 = z / ||z||; resets the status (previous estimate in slot `old`)
 void launch_lanczos_start(const double* z, double* v0, int64_t n, ReduceScratch rs,
-                          LanczosScalars* sc, int cur, hipStream_t s);
+                          LanczosScalars* sc, int old, hipStream_t s);
+// One Lanczos step after the product, in one launch (n <= kLanczosFusedMaxN):
+// w = M v_i (from `w`, or summed over the nch source-chunk slabs of the pair
+// kernel: entries k < n3 from slab_u, the rest from slab_w); w -= beta[i-1]
+// v_prev; alpha[i] = w . v_i; w -= alpha[i] v_i; beta[i] = ||w|| with the
+// breakdown rule; v_next = w / beta[i] or e1.  w_out (n doubles) holds w in
+// between; partial / counter: reduction scratch (counter zero at rest).
+constexpr int kLanczosFusedThreads = 1024;
+constexpr int kLanczosFusedPerThread = 16;
+constexpr int64_t kLanczosFusedMaxN = (int64_t)kLanczosFusedThreads * kLanczosFusedPerThread;
+struct LanczosStepArgs {
+  const double* w;
+  const double* slab_u;
+  const double* slab_w;
+  int nch;
+  int64_t n3;
+  const double* v_prev;
+  const double* v_i;
+  double* v_next;
+  double* w_out;
+  double* partial;
+  unsigned int* counter;
+  int64_t n;
+  int i;
+  LanczosScalars* sc;
+};
+void launch_lanczos_step(const LanczosStepArgs& a, hipStream_t s);
 // w -= beta[i-1] v_prev (if i > 0); alpha[i] = w . v_i
 void launch_lanczos_alpha(double* w, const double* v_i, const double* v_prev, int64_t n, int i,
                           ReduceScratch rs, LanczosScalars* sc, hipStream_t s);
@@ -199,16 +236,35 @@ constexpr int64_t tsqrt_scratch(int m) { return (int64_t)m * (kTsqrtNodes + kTsq
 // The same kernel on host arrays alpha (m) and beta (m - 1): out = T^{1/2} e1
 // (synchronous; temporary device buffers).  For tests.
 void tridiag_sqrt_selftest(const double* alpha, const double* beta, int m, double* out);
-void launch_tridiag_sqrt(const LanczosScalars* sc, int m, double* s_out, double* work,
-                         hipStream_t s);
-// Check of iteration i (m = i + 1 columns), skipped once the status says
-// converged: with old = buf[cur] and y = buf[1 - cur], y = normz V[:, 0:m] s,
-// old <- old - y, err = ||old - y|| / ||old|| (LanczosAlgorithm.cu:158-167,
-// 202-222); from i = 1 on, err <= tol converges (y is the result), NaN
-// fails, anything else makes y the previous estimate (cur flips, cu:196).
-void launch_lanczos_estimate(const double* V, int64_t ldv, int m, const double* s_coef,
-                             double* buf0, double* buf1, int64_t n, int i, double tol,
-                             ReduceScratch rs, LanczosScalars* sc, hipStream_t s);
+// Checks of iterations i0 .. i0+c-1 (c <= kCheckBatch), all skipped once the
+// status says converged.  Launch 1: coef row j = T_{m_j}^{1/2} e1 with
+// m_j = i0 + 1 + j, zero-padded to i0 + c entries (one workgroup per row,
+// `work` holds c slices of tsqrt_scratch(i0 + c) doubles).  Launch 2: the
+// estimates y_j = normz V[:, 0:m_j] coef_j in one pass over V, each compared
+// with the one before it (the first with slot `old`): err_j = ||y_{j-1} -
+// y_j|| / ||y_{j-1}|| (LanczosAlgorithm.cu:158-167, 202-222).  From i = 1 on
+// the first err_j <= tol converges (y_j is the result and, as in the
+// reference, the previous estimate becomes y_{j-1} - y_j), NaN fails;
+// otherwise y_{c-1} becomes the previous estimate (cu:196).  Estimates live
+// in a ring of nslots >= c + 1 slots of `ystride` doubles.  The status is also
+// written to `mirror` (host-mapped, may be null).  Launch 3 forms that
+// difference when this batch converged.
+struct CheckArgs {
+  const double* V;
+  int64_t ldv;
+  int i0, c;
+  double* coef;  // kCheckBatch rows of kLanczosMaxIterations
+  double* work;
+  double* Y;
+  int64_t ystride;
+  int nslots;
+  int64_t n;
+  double tol;
+  ReduceScratch rs;  // partials: 2 * kCheckBatch slots
+  LanczosScalars* sc;
+  LanczosStatus* mirror;
+};
+void launch_lanczos_checks(const CheckArgs& a, hipStream_t s);
 // out[k] += pref * y[k]
 void launch_axpy_accumulate(double* out, const double* y, double pref, int64_t n, hipStream_t s);
 // dst = src + c * w

@@ -113,7 +113,7 @@ static inline int reduce_blocks(int64_t n) {
 // ||z|| -> sc->normz
 __global__ __launch_bounds__(kRT) void k_norm_start(const double* __restrict__ z, int64_t n,
                                                    double* partial, unsigned* counter,
-                                                   LanczosScalars* sc, int cur) {
+                                                   LanczosScalars* sc, int old) {
   __shared__ double sh[kRT / 64];
   double v = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT)
@@ -123,11 +123,14 @@ __global__ __launch_bounds__(kRT) void k_norm_start(const double* __restrict__ z
     const double t = gather_partials(partial, 0, gridDim.x, sh);
     if (threadIdx.x == 0) {
       sc->normz = sqrt(t);
-      sc->status.conv = 0;
-      sc->status.conv_iter = -1;
-      sc->status.cur = cur;
-      sc->status.checked = -1;
-      sc->status.err = 0.0;
+      LanczosStatus st{};
+      st.conv = 0;
+      st.conv_iter = -1;
+      st.old = old;
+      st.result = -1;
+      st.first = -1;
+      st.nchk = 0;
+      sc->status = st;
       *counter = 0;
     }
   }
@@ -142,10 +145,10 @@ __global__ void k_scale_start(const double* __restrict__ z, double* __restrict__
 }
 
 void launch_lanczos_start(const double* z, double* v0, int64_t n, ReduceScratch rs,
-                          LanczosScalars* sc, int cur, hipStream_t s) {
+                          LanczosScalars* sc, int old, hipStream_t s) {
   const int nb = reduce_blocks(n);
   hipLaunchKernelGGL(k_norm_start, dim3(nb), dim3(kRT), 0, s, z, n, rs.partial, rs.counter, sc,
-                     cur);
+                     old);
   LM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_scale_start, dim3(nb), dim3(kRT), 0, s, z, v0, n, sc);
   LM_HIP(hipGetLastError());
@@ -232,6 +235,115 @@ __global__ void k_next(const double* __restrict__ w, double* __restrict__ vn, in
 void launch_lanczos_next(const double* w, double* v_next, int64_t n, int i,
                          const LanczosScalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_next, dim3(reduce_blocks(n)), dim3(kRT), 0, s, w, v_next, n, i, sc);
+  LM_HIP(hipGetLastError());
+}
+
+// Sum over a workgroup of kLanczosFusedThreads in a fixed order; every thread
+// gets the total.  `sh` holds one double per wave plus one.
+__device__ inline double group_sum_all(double v, double* sh) {
+  constexpr int kW = kLanczosFusedThreads / 64;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < kW; ++k) t += sh[k];
+    sh[kW] = t;
+  }
+  __syncthreads();
+  const double t = sh[kW];
+  __syncthreads();  // sh is reused by the next sum
+  return t;
+}
+
+// The step of LanczosAlgorithm.cu:128-152 in one launch.  Every workgroup
+// forms its entries of w = M v_i (summing the chunk slabs) - beta[i-1]
+// v_{i-1}, stores them and publishes its part of alpha = w . v_i; the last
+// workgroup to arrive finishes alone (the vectors are short): w -= alpha v_i,
+// beta = ||w|| with the breakdown rule, v_next = w / beta or e1.
+__global__ __launch_bounds__(kLanczosFusedThreads) void k_lanczos_step(LanczosStepArgs a) {
+  __shared__ double sh[kLanczosFusedThreads / 64 + 1];
+  constexpr int R = kLanczosFusedPerThread;
+  const int t = threadIdx.x;
+  const int64_t n = a.n;
+  const int i = a.i;
+  LanczosScalars* sc = a.sc;
+  const double bprev = i > 0 ? sc->beta[i - 1] : 0.0;
+  double p = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * kLanczosFusedThreads + t; k < n;
+       k += (int64_t)gridDim.x * kLanczosFusedThreads) {
+    double v = 0.0;
+    if (a.w) {
+      v = a.w[k];
+    } else {
+      const double* sl = k < a.n3 ? a.slab_u + k : a.slab_w + (k - a.n3);
+      const int nch = a.nch;
+      // the first 16 slabs' loads all in flight at once, then added in order
+      constexpr int kQ = 16;
+      double q[kQ];
+#pragma unroll
+      for (int c = 0; c < kQ; ++c) q[c] = c < nch ? sl[(int64_t)c * a.n3] : 0.0;
+#pragma unroll
+      for (int c = 0; c < kQ; ++c)
+        if (c < nch) v += q[c];
+      for (int c = kQ; c < nch; ++c) v += sl[(int64_t)c * a.n3];
+    }
+    if (i > 0) v -= bprev * a.v_prev[k];
+    a.w_out[k] = v;
+    p += v * a.v_i[k];
+  }
+  const double part = group_sum_all(p, sh);
+  __shared__ bool s_last;
+  if (t == 0) {
+    __hip_atomic_store(&a.partial[blockIdx.x], part, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned prev =
+        __hip_atomic_fetch_add(a.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  double q = 0.0;
+  for (int b = t; b < (int)gridDim.x; b += kLanczosFusedThreads)
+    q += __hip_atomic_load(&a.partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const double alpha = group_sum_all(q, sh);
+  double w[R];
+  p = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t k = t + (int64_t)r * kLanczosFusedThreads;
+    w[r] = 0.0;
+    if (k < n) {
+      w[r] = a.w_out[k] - alpha * a.v_i[k];
+      p += w[r] * w[r];
+    }
+  }
+  double beta = sqrt(group_sum_all(p, sh));
+  if (beta < 1e-3 * alpha / sc->normz) beta = 0.0;  // LanczosAlgorithm.cu:140-142
+  if (t == 0) {
+    sc->alpha[i] = alpha;
+    sc->beta[i] = beta;
+    *a.counter = 0;
+  }
+  const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t k = t + (int64_t)r * kLanczosFusedThreads;
+    if (k < n) a.v_next[k] = beta > 0.0 ? w[r] * inv : (k == 0 ? 1.0 : 0.0);
+  }
+}
+
+void launch_lanczos_step(const LanczosStepArgs& a, hipStream_t s) {
+  if (a.n <= 0 || a.n > kLanczosFusedMaxN)
+    throw Error("[libmobility_amd] fused Lanczos step: vector length out of range");
+  if (!a.w && (a.nch < 1 || !a.slab_u || (a.n > a.n3 && !a.slab_w)))
+    throw Error("[libmobility_amd] fused Lanczos step: no product input");
+  if (!a.w_out || !a.partial || !a.counter)
+    throw Error("[libmobility_amd] fused Lanczos step: no workspace");
+  const int grid = (int)((a.n + kLanczosFusedThreads - 1) / kLanczosFusedThreads);
+  hipLaunchKernelGGL(k_lanczos_step, dim3(grid), dim3(kLanczosFusedThreads), 0, s, a);
   LM_HIP(hipGetLastError());
 }
 
@@ -420,19 +532,20 @@ __device__ void tridiag_sqrt_body(const double* __restrict__ alpha,
   }
 }
 
-__global__ __launch_bounds__(kTS) void k_tridiag_sqrt(const LanczosScalars* __restrict__ sc,
-                                                      int m, double* __restrict__ s_out,
-                                                      double* __restrict__ work) {
+// ---- convergence checks, up to kCheckBatch per launch ---------------------------
+// One workgroup per check j: coef row j = T_{m0+j}^{1/2} e1, zero-padded to
+// m0 + c - 1 entries so that every row spans the batch's widest estimate.
+__global__ __launch_bounds__(kTS) void k_tridiag_sqrt_batch(const LanczosScalars* __restrict__ sc,
+                                                            int m0, int c,
+                                                            double* __restrict__ coef,
+                                                            double* __restrict__ work,
+                                                            int64_t wstride) {
   if (sc->status.conv) return;  // an earlier check converged
-  tridiag_sqrt_body(sc->alpha, sc->beta, m, s_out, work);
-}
-
-void launch_tridiag_sqrt(const LanczosScalars* sc, int m, double* s_out, double* work,
-                         hipStream_t s) {
-  if (m < 1 || m > kLanczosMaxIterations)
-    throw Error("[libmobility_amd] tridiagonal size out of range");
-  hipLaunchKernelGGL(k_tridiag_sqrt, dim3(1), dim3(kTS), 0, s, sc, m, s_out, work);
-  LM_HIP(hipGetLastError());
+  const int j = blockIdx.x;
+  const int m = m0 + j;
+  double* row = coef + (int64_t)j * kLanczosMaxIterations;
+  tridiag_sqrt_body(sc->alpha, sc->beta, m, row, work + (int64_t)j * wstride);
+  for (int k = m + threadIdx.x; k < m0 + c - 1; k += kTS) row[k] = 0.0;
 }
 
 __global__ __launch_bounds__(kTS) void k_tridiag_sqrt_arrays(const double* __restrict__ alpha,
@@ -485,66 +598,142 @@ void launch_seg_tridiag_sqrt(const int* list, int nlist, int m, const SegScalars
   LM_HIP(hipGetLastError());
 }
 
-// y = normz V[:, :m] s ; old <- old - y ; err = ||old - y|| / ||old||
-// (LanczosAlgorithm.cu:158-167 and :202-222), then the convergence decision.
-__global__ __launch_bounds__(kRT) void k_estimate(const double* __restrict__ V, int64_t ldv,
-                                                 int m, const double* __restrict__ coef,
-                                                 double* __restrict__ buf0,
-                                                 double* __restrict__ buf1, int64_t n, int it,
-                                                 double tol, double* partial, unsigned* counter,
-                                                 LanczosScalars* sc) {
-  __shared__ double sc_s[kLanczosMaxIterations];
+// Estimates of a batch of checks: y_j = normz V[:, 0:m_j] coef_j for the c
+// checks at once (each entry of V read once), each compared with the one
+// before it (LanczosAlgorithm.cu:158-167 and :202-222), then the decision of
+// LanczosAlgorithm.cu:181-197 in check order.  A tile is 64 rows, one per
+// lane; the workgroup's four waves split the columns and meet in LDS.
+constexpr int kEstRows = 64;
+constexpr int kEstWaves = kRT / 64;
+
+__global__ __launch_bounds__(kRT) void k_estimate_batch(CheckArgs a) {
+  __shared__ double cs[kCheckBatch][kLanczosMaxIterations];
+  __shared__ double red[kEstWaves][kCheckBatch][kEstRows];
   __shared__ double sh[kRT / 64];
+  LanczosScalars* sc = a.sc;
   if (sc->status.conv) return;  // an earlier check converged (same value in every block)
-  const int cur = sc->status.cur;
-  double* __restrict__ old = cur ? buf1 : buf0;
-  double* __restrict__ y = cur ? buf0 : buf1;
+  const int c = a.c;
+  const int mm = a.i0 + c;  // columns of the widest estimate
+  const int o = sc->status.old;
+  const int ns = a.nslots;
   const double nz = sc->normz;
-  for (int k = threadIdx.x; k < m; k += kRT) sc_s[k] = nz * coef[k];
-  __syncthreads();
-  double num = 0.0, den = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
-    double v = 0.0;
-    for (int k = 0; k < m; ++k) v += V[(int64_t)k * ldv + i] * sc_s[k];
-    y[i] = v;
-    const double o = old[i];
-    const double d = o - v;
-    old[i] = d;
-    num += d * d;
-    den += o * o;
+  for (int idx = threadIdx.x; idx < kCheckBatch * mm; idx += kRT) {
+    const int j = idx / mm, k = idx - j * mm;
+    cs[j][k] = j < c ? nz * a.coef[(int64_t)j * kLanczosMaxIterations + k] : 0.0;
   }
-  num = block_sum(num, sh);
-  den = block_sum(den, sh);
-  double vals[2] = {num, den};
-  if (publish_and_arrive(partial, 2, vals, counter)) {
-    const double tn = gather_partials(partial, 0, gridDim.x, sh);
-    const double td = gather_partials(partial, 1, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      const double err = fabs(sqrt(tn) / sqrt(td));
-      sc->errnum = tn;
-      sc->errden = td;
-      sc->err = err;
-      sc->status.err = err;
-      sc->status.checked = it;
-      if (it > 0 && isnan(err)) {  // LanczosAlgorithm.cu:217-220
-        sc->status.conv = 2;
-        sc->status.conv_iter = it;
-      } else if (it > 0 && err <= tol) {  // cu:185-192
-        sc->status.conv = 1;
-        sc->status.conv_iter = it;
-      } else {
-        sc->status.cur = 1 - cur;  // oldBz <- Bz (cu:196)
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double num[kCheckBatch], den[kCheckBatch];
+#pragma unroll
+  for (int j = 0; j < kCheckBatch; ++j) num[j] = den[j] = 0.0;
+  const double* __restrict__ yold = a.Y + (int64_t)o * a.ystride;
+  for (int64_t tile = blockIdx.x; tile * kEstRows < a.n; tile += gridDim.x) {
+    const int64_t row = tile * kEstRows + lane;
+    const bool ok = row < a.n;
+    double acc[kCheckBatch];
+#pragma unroll
+    for (int j = 0; j < kCheckBatch; ++j) acc[j] = 0.0;
+#pragma unroll 4
+    for (int k = wid; k < mm; k += kEstWaves) {
+      const double v = ok ? a.V[(int64_t)k * a.ldv + row] : 0.0;
+#pragma unroll
+      for (int j = 0; j < kCheckBatch; ++j) acc[j] += v * cs[j][k];
+    }
+#pragma unroll
+    for (int j = 0; j < kCheckBatch; ++j) red[wid][j][lane] = acc[j];
+    __syncthreads();
+    if (wid == 0 && ok) {
+      double prev = yold[row];
+#pragma unroll
+      for (int j = 0; j < kCheckBatch; ++j) {
+        if (j < c) {
+          double y = 0.0;
+#pragma unroll
+          for (int w = 0; w < kEstWaves; ++w) y += red[w][j][lane];
+          const double d = prev - y;
+          num[j] += d * d;
+          den[j] += prev * prev;
+          a.Y[(int64_t)((o + 1 + j) % ns) * a.ystride + row] = y;
+          prev = y;
+        }
       }
-      *counter = 0;
+    }
+    __syncthreads();
+  }
+  double vals[2 * kCheckBatch];
+#pragma unroll
+  for (int j = 0; j < kCheckBatch; ++j) {
+    vals[2 * j] = block_sum(num[j], sh);
+    vals[2 * j + 1] = block_sum(den[j], sh);
+  }
+  // every slot is published (fixed count: no dynamically indexed arrays)
+  if (publish_and_arrive(a.rs.partial, 2 * kCheckBatch, vals, a.rs.counter)) {
+    double tn[kCheckBatch], td[kCheckBatch];
+#pragma unroll
+    for (int j = 0; j < kCheckBatch; ++j) {
+      tn[j] = gather_partials(a.rs.partial, 2 * j, gridDim.x, sh);
+      td[j] = gather_partials(a.rs.partial, 2 * j + 1, gridDim.x, sh);
+    }
+    if (threadIdx.x == 0) {
+      LanczosStatus st = sc->status;
+      st.first = a.i0;
+      st.nchk = c;
+      bool done = false;
+#pragma unroll
+      for (int j = 0; j < kCheckBatch; ++j) {
+        if (j < c && !done) {
+          const int it = a.i0 + j;
+          const double err = fabs(sqrt(tn[j]) / sqrt(td[j]));
+          st.err[j] = err;
+          if (it > 0 && isnan(err)) {  // LanczosAlgorithm.cu:217-220
+            st.conv = 2;
+            st.conv_iter = it;
+            st.nchk = j + 1;
+            done = true;
+          } else if (it > 0 && err <= a.tol) {  // cu:185-192
+            st.conv = 1;
+            st.conv_iter = it;
+            st.result = (o + 1 + j) % ns;
+            st.old = j == 0 ? o : (o + j) % ns;  // becomes y_{j-1} - y_j (cu:209-212)
+            st.nchk = j + 1;
+            done = true;
+          }
+        }
+      }
+      if (st.conv == 0) st.old = (o + c) % ns;  // oldBz <- Bz (cu:196)
+      sc->status = st;
+      if (a.mirror) *a.mirror = st;
+      *a.rs.counter = 0;
     }
   }
 }
 
-void launch_lanczos_estimate(const double* V, int64_t ldv, int m, const double* s_coef,
-                             double* buf0, double* buf1, int64_t n, int i, double tol,
-                             ReduceScratch rs, LanczosScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_estimate, dim3(reduce_blocks(n)), dim3(kRT), 0, s, V, ldv, m, s_coef, buf0,
-                     buf1, n, i, tol, rs.partial, rs.counter, sc);
+// The batch that converged leaves in slot `old` the reference's oldBz after
+// computeError: previous estimate minus result.
+__global__ void k_check_post(CheckArgs a) {
+  const LanczosStatus& st = a.sc->status;
+  if (st.conv != 1 || st.conv_iter < a.i0 || st.conv_iter >= a.i0 + a.c) return;
+  double* __restrict__ dst = a.Y + (int64_t)st.old * a.ystride;
+  const double* __restrict__ y = a.Y + (int64_t)st.result * a.ystride;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    dst[k] -= y[k];
+}
+
+void launch_lanczos_checks(const CheckArgs& a, hipStream_t s) {
+  if (a.c < 1 || a.c > kCheckBatch || a.i0 < 0 || a.i0 + a.c > kLanczosMaxIterations)
+    throw Error("[libmobility_amd] convergence check batch out of range");
+  if (a.nslots < a.c + 1 || a.ystride < a.n || a.n <= 0)
+    throw Error("[libmobility_amd] convergence check: estimate ring too small");
+  const int mmax = a.i0 + a.c;
+  hipLaunchKernelGGL(k_tridiag_sqrt_batch, dim3(a.c), dim3(kTS), 0, s, a.sc, a.i0 + 1, a.c,
+                     a.coef, a.work, tsqrt_scratch(mmax));
+  LM_HIP(hipGetLastError());
+  const int64_t tiles = (a.n + kEstRows - 1) / kEstRows;
+  const int grid = (int)(tiles < kReduceBlocks ? tiles : kReduceBlocks);
+  hipLaunchKernelGGL(k_estimate_batch, dim3(grid), dim3(kRT), 0, s, a);
+  LM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_check_post, dim3(reduce_blocks(a.n)), dim3(kRT), 0, s, a);
   LM_HIP(hipGetLastError());
 }
 

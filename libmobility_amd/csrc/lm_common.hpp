@@ -77,14 +77,23 @@ class HostBuf {
   HostBuf(const HostBuf&) = delete;
   HostBuf& operator=(const HostBuf&) = delete;
   ~HostBuf() { release(); }
-  void* reserve(size_t bytes) {
+  // flags: hipHostMalloc flags of a new allocation (the first reserve that
+  // allocates decides them)
+  void* reserve(size_t bytes, unsigned flags = hipHostMallocDefault) {
     if (bytes > cap_) {
       release();
       size_t want = bytes < 256 ? 256 : bytes;
-      LM_HIP(hipHostMalloc(&p_, want, hipHostMallocDefault));
+      LM_HIP(hipHostMalloc(&p_, want, flags));
       cap_ = want;
     }
     return p_;
+  }
+  // Address of the allocation in the current device's address space.
+  template <class T>
+  T* device_as() const {
+    void* d = nullptr;
+    LM_HIP(hipHostGetDevicePointer(&d, p_, 0));
+    return static_cast<T*>(d);
   }
   void release() {
     if (p_) (void)hipHostFree(p_);

@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(
     const PairItem* __restrict__ items, int nitems, unsigned int* __restrict__ counter,
     const typename Rec<R>::T* __restrict__ pos, const double* __restrict__ F,
     const double* __restrict__ T, double* __restrict__ U, double* __restrict__ W, int64_t slab,
-    const int* __restrict__ active, double t0, double rt0, double r0) {
+    const int* __restrict__ active, const int* stop, double t0, double rt0, double r0) {
   using R4 = typename Rec<R>::T;
   constexpr bool kNeedF = (BLK & (kUF | kWF)) != 0;
   constexpr bool kNeedT = (BLK & (kUT | kWT)) != 0;
@@ -196,6 +196,14 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(
   __shared__ R4 s_t[kNeedT ? kPairThreads : 1];
   __shared__ int s_item;
   int item = blockIdx.x;
+  if (!PERSIST && stop != nullptr) {
+    // one read per workgroup, so that all its threads take the same branch
+    // (the flag may change while the workgroup starts)
+    __shared__ int s_stop;
+    if (threadIdx.x == 0) s_stop = *(volatile const int*)stop;
+    __syncthreads();
+    if (s_stop != 0) return;
+  }
   while (true) {
     if (PERSIST) {
       __syncthreads();  // previous item finished with s_item and the tiles
@@ -808,11 +816,11 @@ static void launch_pair_t(const PairArgs& a, hipStream_t s) {
     launch_zero_u32(a.counter, 1, s);
     hipLaunchKernelGGL((k_pair<R, KIND, BLK, TPL, true>), dim3(a.grid), dim3(kPairThreads), 0, s,
                        a.items, a.nitems, a.counter, static_cast<const R4*>(a.pos), a.F, a.T, a.U,
-                       a.W, a.slab, a.active, a.t0, a.rt0, a.r0);
+                       a.W, a.slab, a.active, a.stop, a.t0, a.rt0, a.r0);
   } else {
     hipLaunchKernelGGL((k_pair<R, KIND, BLK, TPL, false>), dim3(a.nitems), dim3(kPairThreads), 0,
                        s, a.items, a.nitems, a.counter, static_cast<const R4*>(a.pos), a.F, a.T,
-                       a.U, a.W, a.slab, a.active, a.t0, a.rt0, a.r0);
+                       a.U, a.W, a.slab, a.active, a.stop, a.t0, a.rt0, a.r0);
   }
   LM_HIP(hipGetLastError());
 }

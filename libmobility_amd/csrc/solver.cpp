@@ -104,10 +104,11 @@ void NBodySolver::clean() {
   old_n_ = 0;
   V_.release();
   w_.release();
-  bufA_.release();
-  bufB_.release();
+  est_ring_.release();
+  ring_slots_ = 0;
+  ring_stride_ = 0;
   est_valid_ = false;
-  cur_host_ = 0;
+  old_host_ = 0;
   V_cols_ = V_n_ = 0;
 }
 
@@ -650,6 +651,7 @@ void NBodySolver::launch_pair_items(const PairItem* list, int count, int tpl, in
   a.items = list;
   a.nitems = count;
   a.active = pair_active_;
+  a.stop = a.persistent ? nullptr : pair_stop_;
   a.t0 = t0_;
   a.rt0 = rt0_;
   a.r0 = r0_;
@@ -666,6 +668,11 @@ void NBodySolver::launch_pair_items(const PairItem* list, int count, int tpl, in
     a.W = ang ? part_w_.as<double>() : nullptr;
     a.slab = n3;
     launch_pair(a, stream_);
+    if (defer_chunk_sum_ && !(seg_blocks_ && pair_active_)) {
+      // the caller sums the slabs itself (the fused Lanczos step)
+      deferred_nch_ = nch;
+      return;
+    }
     auto reduce = [&](const double* part, double* out) {
       if (seg_blocks_ && pair_active_)
         launch_seg_chunk_reduce(part, nch, n3, out, seg_blocks_, seg_nblocks_, pair_active_,
@@ -777,14 +784,20 @@ void NBodySolver::choose_tiling(int blocks, bool fp32, const double* F, const do
     for (int ci = 0; ci < 6; ++ci) {
       ensure_items(tpls[ti], nchs[ci]);
       if (nchs[ci] != 0 && items_nch_ != nchs[ci]) continue;  // clamped duplicate
-      launch_main_items(blocks, fp32, rec, F, T, U, W);
-      LM_HIP(hipEventRecord(e0, stream_));
-      launch_main_items(blocks, fp32, rec, F, T, U, W);
-      launch_main_items(blocks, fp32, rec, F, T, U, W);
-      LM_HIP(hipEventRecord(e1, stream_));
-      LM_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      LM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      // per-launch time of `reps` back-to-back launches after one untimed
+      auto time_reps = [&](int reps) {
+        launch_main_items(blocks, fp32, rec, F, T, U, W);
+        LM_HIP(hipEventRecord(e0, stream_));
+        for (int r = 0; r < reps; ++r) launch_main_items(blocks, fp32, rec, F, T, U, W);
+        LM_HIP(hipEventRecord(e1, stream_));
+        LM_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        LM_HIP(hipEventElapsedTime(&t, e0, e1));
+        return t / reps;
+      };
+      float ms = time_reps(2);
+      // short products: two launches are within the clock's noise
+      if (ms < 0.1f) ms = time_reps(16);
       if (ms < best) {
         best = ms;
         best_tpl = items_tpl_;
@@ -819,6 +832,7 @@ void NBodySolver::pair_product(const void* rec64, const void* rec32, const doubl
   const bool fp32 = (prec < 0 ? pair_prec_ : prec) == kPairFP32;
   const void* rec = fp32 ? rec32 : rec64;
   last_sym_ = false;
+  deferred_nch_ = 0;
   if (use_sym(blocks)) {
     launch_sym(blocks, fp32, rec, F, T, U, W);
     last_sym_ = true;
@@ -921,6 +935,9 @@ void NBodySolver::setLanczosIterationLimit(int limit) {
 int NBodySolver::lanczos_run(const double* z, double** result) {
   const int64_t n = (angular_ ? 6 : 3) * N_;
   ensure_records();
+  const bool lookahead = N_ <= kLanczosLookaheadN;
+  const int batch = lookahead ? kCheckBatch : 1;
+  const bool fused = n <= kLanczosFusedMaxN;
   // workspace
   red_partial_.reserve(sizeof(double) * 4 * kReduceBlocks);
   if (red_counter_.capacity() == 0) {
@@ -928,51 +945,49 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     launch_zero_u32(red_counter_.as<unsigned int>(), 64, stream_);
   }
   scalars_.reserve(sizeof(LanczosScalars));
-  tsq_work_.reserve(sizeof(double) * tsqrt_scratch(iter_limit_));
-  coef_.reserve(sizeof(double) * iter_limit_);
+  tsq_work_.reserve(sizeof(double) * kCheckBatch * tsqrt_scratch(iter_limit_));
+  coef_.reserve(sizeof(double) * kCheckBatch * kLanczosMaxIterations);
   w_.reserve(sizeof(double) * n);
-  lz_status_.reserve(kLzSlots * sizeof(LanczosStatus));
+  lz_status_.reserve(kLzSlots * sizeof(LanczosStatus), hipHostMallocMapped | hipHostMallocCoherent);
   for (auto& e : lz_ev_)
     if (!e) LM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  const bool lookahead = N_ <= kLanczosLookaheadN;
   if (lookahead) {
     if (!chk_stream_) LM_HIP(hipStreamCreateWithFlags(&chk_stream_, hipStreamNonBlocking));
     if (!lz_vec_ev_) LM_HIP(hipEventCreateWithFlags(&lz_vec_ev_, hipEventDisableTiming));
-    chk_partial_.reserve(sizeof(double) * 4 * kReduceBlocks);
-    if (chk_counter_.capacity() == 0) {
-      chk_counter_.reserve(sizeof(unsigned int) * 64);
-      launch_zero_u32(chk_counter_.as<unsigned int>(), 64, stream_);
-    }
+  }
+  chk_partial_.reserve(sizeof(double) * 2 * kCheckBatch * kReduceBlocks);
+  if (chk_counter_.capacity() == 0) {
+    chk_counter_.reserve(sizeof(unsigned int) * 64);
+    launch_zero_u32(chk_counter_.as<unsigned int>(), 64, stream_);
   }
   ReduceScratch rs{red_partial_.as<double>(), red_counter_.as<unsigned int>()};
-  ReduceScratch rs_chk = lookahead
-                             ? ReduceScratch{chk_partial_.as<double>(), chk_counter_.as<unsigned int>()}
-                             : rs;
+  ReduceScratch rs_chk{chk_partial_.as<double>(), chk_counter_.as<unsigned int>()};
   hipStream_t cs = lookahead ? chk_stream_ : stream_;
   LanczosScalars* sc = scalars_.as<LanczosScalars>();
-  // previous estimate (oldBz) keeps its prefix across calls (LanczosAlgorithm.cu:175)
-  if (old_n_ != n || !est_valid_) {
+  // Estimate ring; the previous estimate (oldBz) keeps its prefix across
+  // calls (LanczosAlgorithm.cu:175).
+  const int nslots = batch + 1;
+  const int64_t stride = (n + 31) / 32 * 32;
+  if (old_n_ != n || !est_valid_ || ring_slots_ != nslots) {
     const int64_t keep = est_valid_ ? std::min(old_n_, n) : 0;
     std::vector<double> host_prev(keep);
     if (keep > 0) {
-      const double* prev = cur_host_ ? bufB_.as<double>() : bufA_.as<double>();
-      LM_HIP(hipMemcpyAsync(host_prev.data(), prev, sizeof(double) * keep,
-                            hipMemcpyDeviceToHost, stream_));
+      LM_HIP(hipMemcpyAsync(host_prev.data(), est_ring_.as<double>() + (int64_t)old_host_ * ring_stride_,
+                            sizeof(double) * keep, hipMemcpyDeviceToHost, stream_));
       LM_HIP(hipStreamSynchronize(stream_));
     }
-    bufA_.release();
-    bufB_.release();
-    bufA_.reserve(sizeof(double) * (n + 1));
-    bufB_.reserve(sizeof(double) * (n + 1));
-    launch_zero(bufA_.as<double>(), n + 1, stream_);
-    launch_zero(bufB_.as<double>(), n + 1, stream_);
-    cur_host_ = 0;
+    est_ring_.release();
+    est_ring_.reserve(sizeof(double) * nslots * stride);
+    launch_zero(est_ring_.as<double>(), nslots * stride, stream_);
     if (keep > 0) {
-      LM_HIP(hipMemcpyAsync(bufA_.as<double>(), host_prev.data(), sizeof(double) * keep,
+      LM_HIP(hipMemcpyAsync(est_ring_.as<double>(), host_prev.data(), sizeof(double) * keep,
                             hipMemcpyHostToDevice, stream_));
       LM_HIP(hipStreamSynchronize(stream_));
     }
+    old_host_ = 0;
     old_n_ = n;
+    ring_slots_ = nslots;
+    ring_stride_ = stride;
     est_valid_ = true;
   }
   // A basis that grows is copied and the old one freed: the check stream
@@ -986,102 +1001,161 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     pc_t_.reserve(sizeof(double) * n);
     pc_u_.reserve(sizeof(double) * n);
   }
+  // the fused step sums the pair kernel's chunk slabs itself
+  const bool defer = fused && !pre_on();
+  // Products launched after the solve has converged on the device (the main
+  // stream runs ahead of the checks) skip their pair work.
+  struct StopGuard {
+    const int*& p;
+    ~StopGuard() { p = nullptr; }
+  } stop_guard{pair_stop_};
   ensure_cols(2);
   double* V = V_.as<double>();
   double* w = w_.as<double>();
-  double* buf0 = bufA_.as<double>();
-  double* buf1 = bufB_.as<double>();
-  launch_lanczos_start(z, V, n, rs, sc, cur_host_, stream_);  // LanczosAlgorithm.cu:110-118
+  launch_lanczos_start(z, V, n, rs, sc, old_host_, stream_);  // LanczosAlgorithm.cu:110-118
+  if (lookahead) pair_stop_ = &sc->status.conv;
   const int check = std::min(check_steps_, iter_limit_ - 2);
   LanczosStatus* mirror = lz_status_.as<LanczosStatus>();
+  LanczosStatus* mirror_dev = lz_status_.device_as<LanczosStatus>();
   // On every exit the check stream has finished (the next solve resets the
-  // status it reads) and the main stream is ordered after it.
+  // status it reads).
   auto drain = [&]() {
-    if (!lookahead) return;
-    LM_HIP(hipStreamSynchronize(chk_stream_));
+    if (lookahead) LM_HIP(hipStreamSynchronize(chk_stream_));
   };
-  // Reads the status copied after the check of iteration i; returns true
-  // once the solve is finished (the tolerance met at i).
-  auto settle = [&](int i, int slot) -> bool {
+  // Reads the status of the batch of checks starting at iteration `first`;
+  // returns true once the solve is finished (the tolerance met).
+  auto settle = [&](int first, int slot) -> bool {
     LM_HIP(hipEventSynchronize(lz_ev_[slot]));
     const LanczosStatus st = mirror[slot];
-    if (st.checked != i) return false;  // skipped: an earlier check finished the solve
-    if (st.conv == 2) {
+    if (st.first != first || st.nchk < 1 || st.nchk > kCheckBatch) {
       drain();
-      throw Error("[Lanczos] Unknown error (found NaN in result guess)");
+      throw Error("[libmobility_amd] Lanczos convergence status out of order");
     }
-    if (i > 0 && cb_) cb_(i, st.err, cb_user_);
-    cur_host_ = st.cur;
+    for (int j = 0; j < st.nchk; ++j) {
+      if (st.conv == 2 && j == st.nchk - 1) {
+        drain();
+        throw Error("[Lanczos] Unknown error (found NaN in result guess)");
+      }
+      if (first + j > 0 && cb_) cb_(first + j, st.err[j], cb_user_);
+    }
+    old_host_ = st.old;
     if (st.conv != 1) return false;
+    const int i = st.conv_iter;
     // registerRequiredStepsForConverge, LanczosAlgorithm.cu:224-232
     if (i - 2 > check_steps_)
       check_steps_ += 1;
     else
       check_steps_ = std::max(1, check_steps_ - 2);
     last_iters_ = i;
-    *result = st.cur ? buf0 : buf1;  // the estimate written by this check
+    *result = est_ring_.as<double>() + (int64_t)st.result * ring_stride_;
     drain();
     return true;
   };
-  // checks in flight, oldest first: (iteration, slot)
-  int pend_i[kLzSlots], pend_s[kLzSlots];
-  int npend = 0, nchecks = 0;
+  // batches in flight, oldest first: (first iteration, slot)
+  int pend_f[kLzSlots], pend_s[kLzSlots];
+  int npend = 0, nbatches = 0, bfirst = -1;
   auto settle_oldest = [&]() -> bool {
-    const bool done = settle(pend_i[0], pend_s[0]);
+    const bool done = settle(pend_f[0], pend_s[0]);
     for (int k = 1; k < npend; ++k) {
-      pend_i[k - 1] = pend_i[k];
+      pend_f[k - 1] = pend_f[k];
       pend_s[k - 1] = pend_s[k];
     }
     --npend;
     return done;
   };
+  auto finish = [&]() {
+    if (lookahead) LM_HIP(hipStreamWaitEvent(stream_, lz_ev_[(nbatches - 1) % kLzSlots], 0));
+    return last_iters_;
+  };
   for (int i = 0; i < iter_limit_; ++i) {
     ensure_cols(i + 2);
     V = V_.as<double>();
     const double* vi = V + (int64_t)i * n;
-    lanczos_product(vi, w);
-    launch_lanczos_alpha(w, vi, i > 0 ? V + (int64_t)(i - 1) * n : nullptr, n, i, rs, sc, stream_);
-    launch_lanczos_beta(w, vi, n, i, rs, sc, stream_);
-    launch_lanczos_next(w, V + (int64_t)(i + 1) * n, n, i, sc, stream_);
-    if (i >= check || cb_ != nullptr) {
-      const int m = i + 1;
-      if (lookahead) {
-        LM_HIP(hipEventRecord(lz_vec_ev_, stream_));
-        LM_HIP(hipStreamWaitEvent(chk_stream_, lz_vec_ev_, 0));
+    const double* vp = i > 0 ? V + (int64_t)(i - 1) * n : nullptr;
+    double* vn = V + (int64_t)(i + 1) * n;
+    if (fused) {
+      defer_chunk_sum_ = defer;
+      try {
+        lanczos_product(vi, w);
+      } catch (...) {
+        defer_chunk_sum_ = false;
+        drain();
+        throw;
       }
-      launch_tridiag_sqrt(sc, m, coef_.as<double>(), tsq_work_.as<double>(), cs);
-      launch_lanczos_estimate(V, n, m, coef_.as<double>(), buf0, buf1, n, i, tol_, rs_chk, sc, cs);
-      const int slot = nchecks++ % kLzSlots;
-      LM_HIP(hipMemcpyAsync(&mirror[slot], &sc->status, sizeof(LanczosStatus),
-                            hipMemcpyDeviceToHost, cs));
-      LM_HIP(hipEventRecord(lz_ev_[slot], cs));
-      if (!lookahead) {
-        if (settle(i, slot)) return i;
-        continue;
+      defer_chunk_sum_ = false;
+      LanczosStepArgs sa{};
+      if (deferred_nch_ > 0) {
+        sa.slab_u = part_u_.as<double>();
+        sa.slab_w = angular_ ? part_w_.as<double>() : nullptr;
+        sa.nch = deferred_nch_;
+      } else {
+        sa.w = w;
       }
-      pend_i[npend] = i;
-      pend_s[npend] = slot;
-      ++npend;
-      // read every finished check in order; wait when too many are in flight
-      while (npend > 0) {
-        const bool must = npend > kLanczosMaxLag;
-        if (!must) {
-          const hipError_t q = hipEventQuery(lz_ev_[pend_s[0]]);
-          if (q == hipErrorNotReady) break;
-          if (q != hipSuccess) LM_HIP(q);
-        }
-        const int it = pend_i[0];
-        if (settle_oldest()) {
-          LM_HIP(hipStreamWaitEvent(stream_, lz_ev_[(nchecks - 1) % kLzSlots], 0));
-          return it;
-        }
+      sa.n3 = 3 * N_;
+      sa.w_out = w;
+      sa.partial = rs.partial;
+      sa.counter = rs.counter;
+      sa.v_prev = vp;
+      sa.v_i = vi;
+      sa.v_next = vn;
+      sa.n = n;
+      sa.i = i;
+      sa.sc = sc;
+      launch_lanczos_step(sa, stream_);  // LanczosAlgorithm.cu:120-153
+    } else {
+      lanczos_product(vi, w);
+      launch_lanczos_alpha(w, vi, vp, n, i, rs, sc, stream_);
+      launch_lanczos_beta(w, vi, n, i, rs, sc, stream_);
+      launch_lanczos_next(w, vn, n, i, sc, stream_);
+    }
+    if (i < check && cb_ == nullptr) continue;
+    // checks from iteration `check` on (every one with a callback), in batches
+    if (bfirst < 0) bfirst = i;
+    if (i - bfirst + 1 < batch && i < iter_limit_ - 1) continue;
+    if (lookahead) {
+      LM_HIP(hipEventRecord(lz_vec_ev_, stream_));
+      LM_HIP(hipStreamWaitEvent(chk_stream_, lz_vec_ev_, 0));
+    }
+    const int slot = nbatches++ % kLzSlots;
+    mirror[slot].first = -2;
+    CheckArgs ca{};
+    ca.V = V;
+    ca.ldv = n;
+    ca.i0 = bfirst;
+    ca.c = i - bfirst + 1;
+    ca.coef = coef_.as<double>();
+    ca.work = tsq_work_.as<double>();
+    ca.Y = est_ring_.as<double>();
+    ca.ystride = ring_stride_;
+    ca.nslots = ring_slots_;
+    ca.n = n;
+    ca.tol = tol_;
+    ca.rs = rs_chk;
+    ca.sc = sc;
+    ca.mirror = mirror_dev + slot;
+    launch_lanczos_checks(ca, cs);
+    LM_HIP(hipEventRecord(lz_ev_[slot], cs));
+    const int first = bfirst;
+    bfirst = -1;
+    if (!lookahead) {
+      if (settle(first, slot)) return last_iters_;
+      continue;
+    }
+    pend_f[npend] = first;
+    pend_s[npend] = slot;
+    ++npend;
+    // read every finished batch in order; wait when too many are in flight
+    while (npend > 0) {
+      if (npend <= kLanczosMaxLag) {
+        const hipError_t q = hipEventQuery(lz_ev_[pend_s[0]]);
+        if (q == hipErrorNotReady) break;
+        if (q != hipSuccess) LM_HIP(q);
       }
+      if (settle_oldest()) return finish();
     }
   }
-  while (npend > 0) {
-    const int it = pend_i[0];
-    if (settle_oldest()) return it;
-  }
+  while (npend > 0)
+    if (settle_oldest()) return finish();
   drain();
   throw Error("[Lanczos] Could not converge");
 }

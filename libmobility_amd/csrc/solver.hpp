@@ -190,11 +190,13 @@ class NBodySolver {
   void pair_product(const void* rec64, const void* rec32, const double* F, const double* T,
                     double* U, double* W, int prec = -1);
   int lanczos_run(const double* z, double** result);
-  // Systems up to this many particles look one iteration ahead instead of
-  // waiting for each convergence check (the wasted product is cheap there).
+  // Systems up to this many particles run their convergence checks on a
+  // second stream, kCheckBatch at a time, while the main stream goes on with
+  // the next iterations (the products wasted past convergence are cheap
+  // there); larger ones check every iteration on the main stream and wait.
   static constexpr int64_t kLanczosLookaheadN = 32768;
-  // Checks in flight before the host waits for the oldest.
-  static constexpr int kLanczosMaxLag = 4;
+  // Batches of checks in flight before the host waits for the oldest.
+  static constexpr int kLanczosMaxLag = 1;
   int lanczos_run_seg(SegSolve& S, const std::vector<int64_t>& off, bool angular,
                       bool skip_converged_pairs,
                       const std::function<void(const double*, double*)>& product, const double* z,
@@ -260,25 +262,38 @@ class NBodySolver {
   // offsets or kinds changed (work lists depend on these only)
   uint64_t items_struct_ = 0, layout_id_ = 0, struct_id_ = 0;
   DevBuf part_u_, part_w_;
+  // With defer_chunk_sum_ set, an ordered product over nch > 1 source chunks
+  // leaves its partial sums in part_u_ / part_w_ (deferred_nch_ = nch) instead
+  // of summing them into the outputs; deferred_nch_ = 0 when the outputs hold
+  // the product.
+  bool defer_chunk_sum_ = false;
+  int deferred_nch_ = 0;
+  // device flag that stops non-persistent pair workgroups (PairArgs::stop)
+  const int* pair_stop_ = nullptr;
   int tile_tpl_ = 0, tile_nch_ = 0;  // user override (0 = auto)
   bool autotune_ = false;
   std::map<std::tuple<int64_t, int, int, int, int>, std::pair<int, int>> tuned_;
   // staging for host arrays
   DevBuf st_pos_, st_f_, st_t_, st_u_, st_w_;
   // Lanczos
-  DevBuf V_, w_, coef_, bufA_, bufB_, noise_, red_partial_, red_counter_, scalars_, tsq_work_;
+  DevBuf V_, w_, coef_, noise_, red_partial_, red_counter_, scalars_, tsq_work_;
   int64_t V_cols_ = 0, V_n_ = 0;
-  // the two estimate buffers (bufA_, bufB_); cur_host_ names the one holding
-  // the previous estimate (oldBz), as the device status last reported it
-  int cur_host_ = 0;
+  // Estimates of the convergence checks: a ring of ring_slots_ vectors of
+  // ring_stride_ doubles; old_host_ names the slot holding the previous
+  // estimate (oldBz), as the device status last reported it.
+  DevBuf est_ring_;
+  int ring_slots_ = 0;
+  int64_t ring_stride_ = 0;
+  int old_host_ = 0;
   bool est_valid_ = false;
   int64_t old_n_ = 0;
-  // Small systems run each convergence check (tridiagonal square root and
-  // estimate) on a second stream of default priority, overlapped with the
-  // next iterations' products; the host reads the checks' status as they
-  // complete.  The check has its own reduction scratch.
+  // Small systems run the checks (tridiagonal square roots and estimates) on
+  // a second stream of default priority, overlapped with the next
+  // iterations; the estimate kernel writes each batch's status into a
+  // host-mapped slot, read once the batch's event has completed.  The checks
+  // have their own reduction scratch.
   static constexpr int kLzSlots = 8;
-  HostBuf lz_status_;       // pinned copies of the device LanczosStatus, one per slot
+  HostBuf lz_status_;       // host-mapped LanczosStatus, one per slot
   hipEvent_t lz_ev_[kLzSlots] = {};
   hipEvent_t lz_vec_ev_ = nullptr;  // main stream: iteration's vectors written
   hipStream_t chk_stream_ = nullptr;

@@ -341,3 +341,42 @@ def test_empty_system_in_ensemble():
     exact = oracle.dense_sqrt(M) @ W
     assert np.linalg.norm(y.ravel() - exact) <= 1e-5 * np.linalg.norm(exact)
     assert list(s.lastLanczosIterationsPerSystem()[[1]]) == [0]
+
+
+class _DLWrap:
+    """A device array of a framework the library does not know, speaking only
+    DLPack (python_tensor.h:30-93 takes any nanobind framework)."""
+
+    def __init__(self, a):
+        self._a = a
+
+    def __dlpack__(self, *args, **kwargs):
+        return self._a.__dlpack__(*args, **kwargs)
+
+    def __dlpack_device__(self):
+        return self._a.__dlpack_device__()
+
+
+@pytest.mark.parametrize("geom", GEOMS)
+def test_dlpack_device_arrays_without_a_copy(geom):
+    """Device arrays that only speak DLPack reach the library as the
+    producer's own device pointer (no copy, no host round trip), and the
+    products equal those of the same torch tensors passed directly."""
+    import torch
+    from libmobility_amd import nbody
+    rng = np.random.default_rng(21)
+    pos = torch.from_numpy(_pos(64, rng)).to("cuda:0")
+    f = torch.from_numpy(rng.normal(size=(64, 3))).to("cuda:0")
+    t = torch.from_numpy(rng.normal(size=(64, 3))).to("cuda:0")
+    for x in (pos, f, t):
+        arr = nbody._Arr(_DLWrap(x))
+        assert arr.ptr == x.data_ptr()
+        assert arr.mem == nbody._ffi.LM_MEM_DEVICE and arr.device == str(x.device)
+    s_dl = _solver(geom, includeAngular=True)
+    s_dl.setPositions(_DLWrap(pos))
+    u_dl, w_dl = s_dl.Mdot(_DLWrap(f), _DLWrap(t))
+    s_t = _solver(geom, includeAngular=True)
+    s_t.setPositions(pos)
+    u_t, w_t = s_t.Mdot(f, t)
+    assert torch.is_tensor(u_dl) and u_dl.device == pos.device
+    assert torch.equal(u_dl, u_t) and torch.equal(w_dl, w_t)
