@@ -213,6 +213,9 @@ def result_line(metric, value, unit, steps, warmup, ms, higher, dtype, config, r
 # C3 tiling of the bench line and its profiles: 4 targets per lane, 32
 # offset chunks
 C3_TILING = (4, 32)
+# C1 tiling (the autotuner's usual pick there), pinned so that profiles of
+# the same command run the same kernels: 1 target per lane, 16 source chunks
+C1_TILING = (1, 16)
 
 # FP32 VALU flops issued per launch, SQ_INSTS_VALU_FLOPS_FP32 x 64 lanes from
 # rocprofv3 --pmc, same keys as MEASURED_TRAFFIC_BYTES.
@@ -580,7 +583,7 @@ def bench_c1(args):
     N = args.N or 1024
     pos, F, Wn = workloads.config_c1(N)
     pos_d, F_d, W_d, U_d = g.put(pos), g.put(F), g.put(Wn), g.zeros(3 * N)
-    h = g.solver(pos_d, "fp64", tiling=args.tiling)
+    h = g.solver(pos_d, "fp64", tiling=args.tiling or C1_TILING)
     ms_mdot = g.time_events(lambda: g.mdot(h, F_d, None, U_d), args.steps, args.warmup)
     U = U_d.cpu().numpy()
     ms_sqrt, its = g.time_lanczos(h, W_d, 3 * N, args.lanczos_calls)

@@ -216,6 +216,10 @@ struct LanczosStepArgs {
   double* w_out;
   double* partial;
   unsigned int* counter;
+  // a workgroup that finds *stop != 0 when it starts skips the arithmetic
+  // (the step is past convergence; its outputs are then undefined) but still
+  // takes part in the arrival count (nullptr: never)
+  const int* stop;
   int64_t n;
   int i;
   LanczosScalars* sc;

@@ -269,9 +269,13 @@ __global__ __launch_bounds__(kLanczosFusedThreads) void k_lanczos_step(LanczosSt
   const int64_t n = a.n;
   const int i = a.i;
   LanczosScalars* sc = a.sc;
+  __shared__ int s_stop;
+  if (t == 0) s_stop = a.stop != nullptr ? *(volatile const int*)a.stop : 0;
+  __syncthreads();
+  const bool skip = s_stop != 0;  // the same in every thread of the workgroup
   const double bprev = i > 0 ? sc->beta[i - 1] : 0.0;
   double p = 0.0;
-  for (int64_t k = (int64_t)blockIdx.x * kLanczosFusedThreads + t; k < n;
+  for (int64_t k = (int64_t)blockIdx.x * kLanczosFusedThreads + t; !skip && k < n;
        k += (int64_t)gridDim.x * kLanczosFusedThreads) {
     double v = 0.0;
     if (a.w) {
@@ -304,6 +308,10 @@ __global__ __launch_bounds__(kLanczosFusedThreads) void k_lanczos_step(LanczosSt
   }
   __syncthreads();
   if (!s_last) return;
+  if (skip) {
+    if (t == 0) *a.counter = 0;
+    return;
+  }
   __threadfence();
   double q = 0.0;
   for (int b = t; b < (int)gridDim.x; b += kLanczosFusedThreads)
@@ -335,11 +343,101 @@ __global__ __launch_bounds__(kLanczosFusedThreads) void k_lanczos_step(LanczosSt
   }
 }
 
+// The same step in one workgroup for the shortest vectors (n <= kStep1Max):
+// no arrival counter and no device-scope fences (those alone cost about 5 µs
+// per launch on gfx950).  Every slab load of a thread's entries is issued
+// before the first add, so the sum costs one memory round trip; w stays in
+// registers throughout.
+constexpr int kStep1R = 4;
+constexpr int64_t kStep1Max = (int64_t)kLanczosFusedThreads * kStep1R;
+
+__global__ __launch_bounds__(kLanczosFusedThreads) void k_lanczos_step_1wg(LanczosStepArgs a) {
+  __shared__ double sh[kLanczosFusedThreads / 64 + 1];
+  __shared__ int s_stop;
+  const int t = threadIdx.x;
+  if (t == 0) s_stop = a.stop != nullptr ? *(volatile const int*)a.stop : 0;
+  __syncthreads();
+  if (s_stop != 0) return;  // past convergence (one workgroup: nothing to keep consistent)
+  const int64_t n = a.n;
+  const int i = a.i;
+  LanczosScalars* sc = a.sc;
+  constexpr int kQ = 6;  // slabs per round: kStep1R x kQ loads in flight per thread
+  double w[kStep1R];
+  if (a.w) {
+#pragma unroll
+    for (int r = 0; r < kStep1R; ++r) {
+      const int64_t k = t + (int64_t)r * kLanczosFusedThreads;
+      w[r] = k < n ? a.w[k] : 0.0;
+    }
+  } else {
+    const int nch = a.nch;
+#pragma unroll
+    for (int r = 0; r < kStep1R; ++r) w[r] = 0.0;
+    for (int c0 = 0; c0 < nch; c0 += kQ) {
+      double q[kStep1R][kQ];
+#pragma unroll
+      for (int r = 0; r < kStep1R; ++r) {
+        const int64_t k = t + (int64_t)r * kLanczosFusedThreads;
+        const double* sl = k < a.n3 ? a.slab_u + k : a.slab_w + (k - a.n3);
+#pragma unroll
+        for (int c = 0; c < kQ; ++c)
+          q[r][c] = (k < n && c0 + c < nch) ? sl[(int64_t)(c0 + c) * a.n3] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < kStep1R; ++r)
+#pragma unroll
+        for (int c = 0; c < kQ; ++c)
+          if (c0 + c < nch) w[r] += q[r][c];
+    }
+  }
+  // w -= beta[i-1] v_{i-1}; alpha = w . v_i
+  const double bprev = i > 0 ? sc->beta[i - 1] : 0.0;
+  double vi[kStep1R];
+  double p = 0.0;
+#pragma unroll
+  for (int r = 0; r < kStep1R; ++r) {
+    const int64_t k = t + (int64_t)r * kLanczosFusedThreads;
+    vi[r] = 0.0;
+    if (k < n) {
+      vi[r] = a.v_i[k];
+      if (i > 0) w[r] -= bprev * a.v_prev[k];
+      p += w[r] * vi[r];
+    }
+  }
+  const double alpha = group_sum_all(p, sh);
+  p = 0.0;
+#pragma unroll
+  for (int r = 0; r < kStep1R; ++r) {
+    const int64_t k = t + (int64_t)r * kLanczosFusedThreads;
+    if (k < n) {
+      w[r] -= alpha * vi[r];
+      p += w[r] * w[r];
+    }
+  }
+  double beta = sqrt(group_sum_all(p, sh));
+  if (beta < 1e-3 * alpha / sc->normz) beta = 0.0;  // LanczosAlgorithm.cu:140-142
+  if (t == 0) {
+    sc->alpha[i] = alpha;
+    sc->beta[i] = beta;
+  }
+  const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
+#pragma unroll
+  for (int r = 0; r < kStep1R; ++r) {
+    const int64_t k = t + (int64_t)r * kLanczosFusedThreads;
+    if (k < n) a.v_next[k] = beta > 0.0 ? w[r] * inv : (k == 0 ? 1.0 : 0.0);
+  }
+}
+
 void launch_lanczos_step(const LanczosStepArgs& a, hipStream_t s) {
   if (a.n <= 0 || a.n > kLanczosFusedMaxN)
     throw Error("[libmobility_amd] fused Lanczos step: vector length out of range");
   if (!a.w && (a.nch < 1 || !a.slab_u || (a.n > a.n3 && !a.slab_w)))
     throw Error("[libmobility_amd] fused Lanczos step: no product input");
+  if (a.n <= kStep1Max) {
+    hipLaunchKernelGGL(k_lanczos_step_1wg, dim3(1), dim3(kLanczosFusedThreads), 0, s, a);
+    LM_HIP(hipGetLastError());
+    return;
+  }
   if (!a.w_out || !a.partial || !a.counter)
     throw Error("[libmobility_amd] fused Lanczos step: no workspace");
   const int grid = (int)((a.n + kLanczosFusedThreads - 1) / kLanczosFusedThreads);

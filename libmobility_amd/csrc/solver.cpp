@@ -1095,6 +1095,7 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
       sa.w_out = w;
       sa.partial = rs.partial;
       sa.counter = rs.counter;
+      sa.stop = pair_stop_;
       sa.v_prev = vp;
       sa.v_i = vi;
       sa.v_next = vn;
