@@ -419,6 +419,20 @@ def bench_c3(args):
         g.sqrt_noise(h64, W_d, y64)
         g.sync()
         extra["sqrtMdotW_rel_diff_fp32pair_vs_fp64pair"] = rel(y32.cpu().numpy(), y64.cpu().numpy())
+        log("Lanczos sqrtMdotW with FP64 reorthogonalisation (extension), FP32 pairs, tol 1e-6")
+        hr = g.solver(pos_d, "fp32", tiling=lt)
+        g.ffi.check(g.lib.lm_nbody_set_lanczos_reorthogonalization(hr, 1))
+        ms_lr, itr = g.time_lanczos(hr, W_d, 3 * N, args.lanczos_calls)
+        yr = g.zeros(3 * N)
+        g.sqrt_noise(hr, W_d, yr)
+        g.sync()
+        g.destroy(hr)
+        extra["sqrtMdotW_reorth_fp32pair_ms"] = ms_lr
+        extra["sqrtMdotW_reorth_fp32pair_iterations"] = itr
+        extra["sqrtMdotW_reorth_speedup_vs_plain"] = ms_l32 / ms_lr
+        extra["sqrtMdotW_rel_diff_reorth_fp32pair_vs_plain_fp64pair"] = rel(yr.cpu().numpy(),
+                                                                          y64.cpu().numpy())
+        del yr
         log("plain Lanczos sqrtMdotW, FP32 pairs, tol 1e-6, new positions before each call")
         ms_l32p, _ = g.time_lanczos(h32, W_d, 3 * N, args.lanczos_calls, fresh_pos=pos_d)
         extra["sqrtMdotW_plain_fp32pair_with_setPositions_ms"] = ms_l32p
@@ -455,6 +469,15 @@ def bench_c3(args):
         g.ffi.check(g.lib.lm_nbody_coarse_builds(hl, ctypes.byref(nb)))
         g.destroy(hl)
         log(f"  {ms_refresh:.1f} ms per call, {nb.value - nb0.value} coarse build(s)")
+        log("  the same (positions fixed) with FP64 reorthogonalisation")
+        hlr = g.solver(pos_d, "fp32", tiling=lt)
+        g.ffi.check(g.lib.lm_nbody_set_preconditioner(hlr, net._h))
+        g.ffi.check(g.lib.lm_nbody_set_coarse_level(hlr, COARSE_PPC))
+        g.ffi.check(g.lib.lm_nbody_set_lanczos_reorthogonalization(hlr, 1))
+        ms_llr, itlr = g.time_lanczos(hlr, W_d, 3 * N, max(1, args.lanczos_calls - 1))
+        g.destroy(hlr)
+        extra["sqrtMdotW_two_level_reorth_fp32pair_ms"] = ms_llr
+        extra["sqrtMdotW_two_level_reorth_fp32pair_iterations"] = itlr
         extra["sqrtMdotW_two_level_preconditioner"] = (
             f"learned 3x3 blocks (64-feature MLP, BF16 MFMA, trained on the GPU) + coarse level "
             f"({COARSE_PPC} particles per cell target, {cells} cells); on this open packing the "

@@ -149,6 +149,17 @@ int lm_nbody_set_persistent(lm_nbody* h, int mode);
  * Range [3, 1024]; the default stays 200. */
 int lm_nbody_set_lanczos_iteration_limit(lm_nbody* h, int limit);
 
+/* Extension, no counterpart in the reference (its recurrence,
+ * LanczosAlgorithm.cu:120-153, never reorthogonalises): mode 1 orthogonalises
+ * every new basis vector against all earlier ones in FP64 (classical
+ * Gram-Schmidt) before it is normalised; the recurrence's beta becomes the norm
+ * of the reorthogonalised vector.  It removes the loss of orthogonality that
+ * lets FP32 products and the reference's rounding-sensitive stopping rule
+ * move the converged iteration, at the cost of one pass over the basis per
+ * iteration.  Mode 0 (default) is the reference's recurrence.  Applies to
+ * single-recurrence solves (not to per-system or multi-vector recurrences). */
+int lm_nbody_set_lanczos_reorthogonalization(lm_nbody* h, int mode);
+
 /* Ragged ensembles: one lanczos::Solver per system (LanczosAlgorithm.cu:171-232
  * applied to each diagonal block, as separate per-system NBody solvers would
  * run it) instead of one recurrence over the whole block-diagonal operator

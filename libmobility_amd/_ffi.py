@@ -49,6 +49,7 @@ SIGNATURES = {
     "lm_nbody_set_systems": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "lm_nbody_set_persistent": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lm_nbody_set_lanczos_iteration_limit": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lm_nbody_set_lanczos_reorthogonalization": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lm_nbody_set_lanczos_per_system": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lm_nbody_last_lanczos_iterations_per_system": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
     "lm_nbody_num_systems": (ctypes.c_int, [_vp]),

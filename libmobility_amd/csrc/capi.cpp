@@ -155,6 +155,10 @@ int lm_nbody_set_lanczos_iteration_limit(lm_nbody* h, int limit) {
   return guarded([&] { S(h).setLanczosIterationLimit(limit); });
 }
 
+int lm_nbody_set_lanczos_reorthogonalization(lm_nbody* h, int mode) {
+  return guarded([&] { S(h).setLanczosReorthogonalization(mode); });
+}
+
 int lm_nbody_set_lanczos_per_system(lm_nbody* h, int on) {
   return guarded([&] { S(h).setLanczosPerSystem(on != 0); });
 }

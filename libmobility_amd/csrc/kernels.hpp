@@ -189,6 +189,7 @@ struct LanczosScalars {
   double beta[kLanczosMaxIterations];   // hsup
   double err;      // last error estimate
   double errnum, errden;
+  double rscale;   // reorthogonalisation: 1 / norm of the reorthogonalised v_{i+1}
   LanczosStatus status;
 };
 
@@ -225,6 +226,15 @@ struct LanczosStepArgs {
   LanczosScalars* sc;
 };
 void launch_lanczos_step(const LanczosStepArgs& a, hipStream_t s);
+// Full reorthogonalisation of v_{i+1} (column i + 1 of V) against columns
+// 0..i, classical Gram-Schmidt in FP64 (an extension; the reference's
+// recurrence has none): c = V^T v, v -= V c, s = ||v||, beta[i] *= s (the
+// breakdown rule applied to the result), v /= s, or e1 after a breakdown.
+// Skipped when beta[i] is already 0.  partial: kLanczosMaxIterations *
+// kReduceBlocks doubles; c: kLanczosMaxIterations doubles; counter zero at
+// rest.
+void launch_lanczos_reorth(const double* V, int64_t ldv, int i, int64_t n, double* partial,
+                           unsigned int* counter, double* c, LanczosScalars* sc, hipStream_t s);
 // w -= beta[i-1] v_prev (if i > 0); alpha[i] = w . v_i
 void launch_lanczos_alpha(double* w, const double* v_i, const double* v_prev, int64_t n, int i,
                           ReduceScratch rs, LanczosScalars* sc, hipStream_t s);

@@ -445,6 +445,146 @@ void launch_lanczos_step(const LanczosStepArgs& a, hipStream_t s) {
   LM_HIP(hipGetLastError());
 }
 
+// ---- full reorthogonalisation (extension) -----------------------------------------
+// Three launches: projections c_j = v_j . v (one wave per column slice of a
+// block's rows, fixed-order sums over blocks), v -= V c with ||v||^2, and the
+// rescale.  All three read beta[i] first and do nothing after a breakdown.
+// Projections: blockIdx.x takes a slice of rows, blockIdx.y a group of
+// kRoCols columns; each thread loads its rows' entries of v once and
+// multiplies them into kRoCols running dots, so V is read once.
+constexpr int kRoCols = 8;
+
+__global__ __launch_bounds__(kRT) void k_reorth_dots(const double* __restrict__ V, int64_t ldv,
+                                                     int m, const double* __restrict__ v,
+                                                     int64_t n, int i, double* partial,
+                                                     unsigned* counter, double* __restrict__ c,
+                                                     const LanczosScalars* sc) {
+  __shared__ double red[kRT / 64][kRoCols];
+  if (sc->beta[i] == 0.0) return;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int rb = blockIdx.x, nrb = gridDim.x;
+  const int j0 = blockIdx.y * kRoCols;
+  const int64_t per = (n + nrb - 1) / nrb;
+  const int64_t r0 = (int64_t)rb * per;
+  const int64_t r1 = r0 + per < n ? r0 + per : n;
+  double acc[kRoCols];
+#pragma unroll
+  for (int q = 0; q < kRoCols; ++q) acc[q] = 0.0;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kRT) {
+    const double x = v[r];
+#pragma unroll
+    for (int q = 0; q < kRoCols; ++q)
+      if (j0 + q < m) acc[q] += V[(int64_t)(j0 + q) * ldv + r] * x;
+  }
+#pragma unroll
+  for (int q = 0; q < kRoCols; ++q) {
+    double t = acc[q];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if (lane == 0) red[wid][q] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < kRoCols && j0 + (int)threadIdx.x < m) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kRT / 64; ++w) t += red[w][threadIdx.x];
+    __hip_atomic_store(&partial[(int64_t)(j0 + threadIdx.x) * kReduceBlocks + rb], t,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __shared__ bool s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned prev =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x * gridDim.y - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  for (int j = threadIdx.x; j < m; j += kRT) {
+    double t = 0.0;
+    for (int b = 0; b < nrb; ++b)
+      t += __hip_atomic_load(&partial[(int64_t)j * kReduceBlocks + b], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    c[j] = t;
+  }
+  if (threadIdx.x == 0) *counter = 0;
+}
+
+__global__ __launch_bounds__(kRT) void k_reorth_apply(const double* __restrict__ V, int64_t ldv,
+                                                      int m, double* __restrict__ v, int64_t n,
+                                                      int i, const double* __restrict__ c,
+                                                      double* partial, unsigned* counter,
+                                                      LanczosScalars* sc) {
+  __shared__ double cs[kLanczosMaxIterations];
+  __shared__ double red[kRT / 64][64];
+  __shared__ double sh[kRT / 64];
+  if (sc->beta[i] == 0.0) return;
+  for (int j = threadIdx.x; j < m; j += kRT) cs[j] = c[j];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double p = 0.0;
+  for (int64_t tile = blockIdx.x; tile * 64 < n; tile += gridDim.x) {
+    const int64_t row = tile * 64 + lane;
+    const bool ok = row < n;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int j = wid; j < m; j += kRT / 64) acc += (ok ? V[(int64_t)j * ldv + row] : 0.0) * cs[j];
+    red[wid][lane] = acc;
+    __syncthreads();
+    if (wid == 0 && ok) {
+      double y = 0.0;
+#pragma unroll
+      for (int w = 0; w < kRT / 64; ++w) y += red[w][lane];
+      const double x = v[row] - y;
+      v[row] = x;
+      p += x * x;
+    }
+    __syncthreads();
+  }
+  p = block_sum(p, sh);
+  if (publish_and_arrive(partial, 1, &p, counter)) {
+    const double t = gather_partials(partial, 0, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      const double s = sqrt(t);
+      double be = sc->beta[i] * s;
+      if (be < 1e-3 * sc->alpha[i] / sc->normz) be = 0.0;  // LanczosAlgorithm.cu:140-142
+      sc->beta[i] = be;
+      sc->rscale = be > 0.0 ? 1.0 / s : 0.0;
+      *counter = 0;
+    }
+  }
+}
+
+__global__ void k_reorth_scale(double* __restrict__ v, int64_t n, int i,
+                               const LanczosScalars* sc) {
+  const double be = sc->beta[i];
+  const double f = sc->rscale;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    v[k] = be > 0.0 ? v[k] * f : (k == 0 ? 1.0 : 0.0);
+}
+
+void launch_lanczos_reorth(const double* V, int64_t ldv, int i, int64_t n, double* partial,
+                           unsigned int* counter, double* c, LanczosScalars* sc, hipStream_t s) {
+  const int m = i + 1;
+  if (i < 0 || m > kLanczosMaxIterations || n <= 0)
+    throw Error("[libmobility_amd] reorthogonalisation: size out of range");
+  double* v = const_cast<double*>(V) + (int64_t)m * ldv;
+  const int g = reduce_blocks(n);
+  const int cg = (m + kRoCols - 1) / kRoCols;
+  hipLaunchKernelGGL(k_reorth_dots, dim3(g, cg), dim3(kRT), 0, s, V, ldv, m, v, n, i, partial,
+                     counter, c, sc);
+  LM_HIP(hipGetLastError());
+  const int64_t tiles = (n + 63) / 64;
+  const int ga = (int)(tiles < kReduceBlocks ? tiles : kReduceBlocks);
+  hipLaunchKernelGGL(k_reorth_apply, dim3(ga), dim3(kRT), 0, s, V, ldv, m, v, n, i, c, partial,
+                     counter, sc);
+  LM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reorth_scale, dim3(g), dim3(kRT), 0, s, v, n, i, sc);
+  LM_HIP(hipGetLastError());
+}
+
 // ---- tridiagonal square root (one workgroup) -----------------------------------
 constexpr int kTS = kTsqrtNodes;  // threads: bracket points, the first K are quadrature nodes
 constexpr int kTSWaves = kTS / 64;

@@ -925,6 +925,12 @@ void NBodySolver::ensure_basis(int64_t n, int64_t cols) {
   V_cols_ = want;
 }
 
+void NBodySolver::setLanczosReorthogonalization(int mode) {
+  if (mode != 0 && mode != 1)
+    throw Error("[libmobility_amd] Lanczos reorthogonalisation mode must be 0 (off) or 1 (full)");
+  reorth_ = mode;
+}
+
 void NBodySolver::setLanczosIterationLimit(int limit) {
   if (limit < 3 || limit > kLanczosMaxIterations)
     throw Error("[libmobility_amd] Lanczos iteration limit must be in [3, " +
@@ -1000,6 +1006,14 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     ensure_pre();
     pc_t_.reserve(sizeof(double) * n);
     pc_u_.reserve(sizeof(double) * n);
+  }
+  if (reorth_) {
+    reorth_partial_.reserve(sizeof(double) * kLanczosMaxIterations * kReduceBlocks);
+    reorth_c_.reserve(sizeof(double) * kLanczosMaxIterations);
+    if (reorth_counter_.capacity() == 0) {
+      reorth_counter_.reserve(sizeof(unsigned int) * 64);
+      launch_zero_u32(reorth_counter_.as<unsigned int>(), 64, stream_);
+    }
   }
   // the fused step sums the pair kernel's chunk slabs itself
   const bool defer = fused && !pre_on();
@@ -1109,6 +1123,10 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
       launch_lanczos_beta(w, vi, n, i, rs, sc, stream_);
       launch_lanczos_next(w, vn, n, i, sc, stream_);
     }
+    if (reorth_)
+      launch_lanczos_reorth(V, n, i, n, reorth_partial_.as<double>(),
+                            reorth_counter_.as<unsigned int>(), reorth_c_.as<double>(), sc,
+                            stream_);
     if (i < check && cb_ == nullptr) continue;
     // checks from iteration `check` on (every one with a callback), in batches
     if (bfirst < 0) bfirst = i;

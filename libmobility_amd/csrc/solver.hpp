@@ -59,6 +59,10 @@ class NBodySolver {
   // Lanczos iterations before "[Lanczos] Could not converge" (the public
   // iterationHardLimit member of lanczos::Solver, LanczosAlgorithm.h:49).
   void setLanczosIterationLimit(int limit);
+  // 0: the reference's plain recurrence; 1: full FP64 reorthogonalisation of
+  // every new basis vector (single-recurrence solves)
+  void setLanczosReorthogonalization(int mode);
+  int lanczosReorthogonalization() const { return reorth_; }
   int lanczosIterationLimit() const { return iter_limit_; }
   // Ensembles: one Lanczos recurrence per system (each with its own tolerance
   // test, check schedule, previous estimate and iteration count, as separate
@@ -300,6 +304,8 @@ class NBodySolver {
   DevBuf chk_partial_, chk_counter_;
   int check_steps_ = 3;          // LanczosAlgorithm.cu:171
   int iter_limit_ = 200;         // LanczosAlgorithm.h:49
+  int reorth_ = 0;
+  DevBuf reorth_partial_, reorth_counter_, reorth_c_;
   int last_iters_ = 0;
   // per-system Lanczos
   bool seg_mode_ = false;

@@ -411,6 +411,12 @@ class NBody(_Solver):
         LanczosAlgorithm.h:49; default 200, at most 1024)."""
         _ffi.check(self._lib.lm_nbody_set_lanczos_iteration_limit(self._h, int(limit)))
 
+    def setLanczosReorthogonalization(self, on=True):
+        """Extension: orthogonalise every new Lanczos basis vector against all
+        earlier ones in FP64 (the reference's recurrence never does).  Off by
+        default; applies to single-recurrence solves."""
+        _ffi.check(self._lib.lm_nbody_set_lanczos_reorthogonalization(self._h, 1 if on else 0))
+
     def setPreconditioner(self, preconditioner="learned"):
         """Extension: attach a learned block preconditioner to sqrtMdotW.
 
