@@ -446,9 +446,9 @@ void launch_lanczos_step(const LanczosStepArgs& a, hipStream_t s) {
 }
 
 // ---- full reorthogonalisation (extension) -----------------------------------------
-// Three launches: projections c_j = v_j . v (one wave per column slice of a
-// block's rows, fixed-order sums over blocks), v -= V c with ||v||^2, and the
-// rescale.  All three read beta[i] first and do nothing after a breakdown.
+// Four launches: projections c_j = v_j . v (partial dots per row slice, then
+// their fixed-order sums), v -= V c with ||v||^2, and the rescale.  All read
+// beta[i] first and do nothing after a breakdown.
 // Projections: blockIdx.x takes a slice of rows, blockIdx.y a group of
 // kRoCols columns; each thread loads its rows' entries of v once and
 // multiplies them into kRoCols running dots, so V is read once.
@@ -456,8 +456,7 @@ constexpr int kRoCols = 8;
 
 __global__ __launch_bounds__(kRT) void k_reorth_dots(const double* __restrict__ V, int64_t ldv,
                                                      int m, const double* __restrict__ v,
-                                                     int64_t n, int i, double* partial,
-                                                     unsigned* counter, double* __restrict__ c,
+                                                     int64_t n, int i, double* __restrict__ partial,
                                                      const LanczosScalars* sc) {
   __shared__ double red[kRT / 64][kRoCols];
   if (sc->beta[i] == 0.0) return;
@@ -487,28 +486,22 @@ __global__ __launch_bounds__(kRT) void k_reorth_dots(const double* __restrict__ 
     double t = 0.0;
 #pragma unroll
     for (int w = 0; w < kRT / 64; ++w) t += red[w][threadIdx.x];
-    __hip_atomic_store(&partial[(int64_t)(j0 + threadIdx.x) * kReduceBlocks + rb], t,
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    partial[(int64_t)(j0 + threadIdx.x) * kReduceBlocks + rb] = t;
   }
-  __shared__ bool s_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned prev =
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (prev == gridDim.x * gridDim.y - 1);
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  for (int j = threadIdx.x; j < m; j += kRT) {
+}
+
+// c_j = fixed-order sum of the row slices' partial dots (the launch boundary
+// orders it after every slice; no arrival counter, which thousands of
+// workgroups would contend on).
+__global__ __launch_bounds__(kRT) void k_reorth_sum(const double* __restrict__ partial, int m,
+                                                    int nrb, int i, double* __restrict__ c,
+                                                    const LanczosScalars* sc) {
+  if (sc->beta[i] == 0.0) return;
+  for (int j = blockIdx.x * kRT + threadIdx.x; j < m; j += gridDim.x * kRT) {
     double t = 0.0;
-    for (int b = 0; b < nrb; ++b)
-      t += __hip_atomic_load(&partial[(int64_t)j * kReduceBlocks + b], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = 0; b < nrb; ++b) t += partial[(int64_t)j * kReduceBlocks + b];
     c[j] = t;
   }
-  if (threadIdx.x == 0) *counter = 0;
 }
 
 __global__ __launch_bounds__(kRT) void k_reorth_apply(const double* __restrict__ V, int64_t ldv,
@@ -574,7 +567,10 @@ void launch_lanczos_reorth(const double* V, int64_t ldv, int i, int64_t n, doubl
   const int g = reduce_blocks(n);
   const int cg = (m + kRoCols - 1) / kRoCols;
   hipLaunchKernelGGL(k_reorth_dots, dim3(g, cg), dim3(kRT), 0, s, V, ldv, m, v, n, i, partial,
-                     counter, c, sc);
+                     sc);
+  LM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reorth_sum, dim3((m + kRT - 1) / kRT), dim3(kRT), 0, s, partial, m, g, i, c,
+                     sc);
   LM_HIP(hipGetLastError());
   const int64_t tiles = (n + 63) / 64;
   const int ga = (int)(tiles < kReduceBlocks ? tiles : kReduceBlocks);
