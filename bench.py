@@ -47,6 +47,10 @@ from libmobility_amd import workloads  # noqa: E402  (numpy generators only)
 METRIC = "Mdot Gpair/s + sqrtMdotW ms at N=131072 tol 1e-6: learned-prec vs plain"
 # coarse level of the preconditioned sqrtMdotW: target particles per cell
 COARSE_PPC = 40
+# C5: the coarse dimension cap raised from the library's default 24576 to
+# 49152 (16200 cells instead of 7938: 53 Lanczos iterations instead of 58,
+# build 3.9 s, three dense 49152^2 FP64 matrices = 58 GB of HBM)
+C5_COARSE_MAX_DIM = 49152
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
 PEAK_FP64_TFLOPS = 78.6
 ETA = 1.0 / (6.0 * np.pi)
@@ -1017,6 +1021,7 @@ def bench_c5(args):
             net = default_network(True)
             g.ffi.check(g.lib.lm_nbody_set_preconditioner(h32, net._h))
             g.ffi.check(g.lib.lm_nbody_set_coarse_level(h32, COARSE_PPC))
+            g.ffi.check(g.lib.lm_nbody_set_coarse_max_dim(h32, C5_COARSE_MAX_DIM))
             log("hydrodynamicVelocities (Mdot + sqrtMdotW tol 1e-6 + RFD drift), FP32 pairs, "
                 "two-level preconditioner (learned 6x6 blocks + coarse level)")
         g.sync()
@@ -1037,7 +1042,8 @@ def bench_c5(args):
         })
         if not args.c5_plain:
             extra["preconditioner"] = (f"learned 6x6 blocks + coarse level ({g.coarse_cells(h32)} "
-                                       f"cells, built in {g.coarse_build_ms(h32):.0f} ms)")
+                                       f"cells, 3C capped at {C5_COARSE_MAX_DIM}, built in "
+                                       f"{g.coarse_build_ms(h32):.0f} ms)")
     g.destroy(h32)
     config = {"workload": f"single-wall RPB/Swan-Brady, N={N}, x,y uniform in [0,{Lxy:.0f}a), z "
                           f"uniform in [1.5a,30a], a=1, eta=1/(6 pi), F,T~N(0,1); 6x6 operator; "

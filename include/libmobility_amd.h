@@ -324,13 +324,25 @@ int lm_nbody_precond_neighbours(lm_nbody* h, int* nbr);
 
 /* Extension, no counterpart in the reference: coarse level of the sqrtMdotW
  * preconditioner.  The particles of every system are grouped into spatial
- * cells of about particles_per_cell particles (coarse dimension 3C_s <= 24576
- * per system); with Q the orthonormal cell columns and Q^T M Q = L L^T (block
- * diagonal over the systems), Lanczos runs on G M G^T with
+ * cells of about particles_per_cell particles (coarse dimension 3C_s at most
+ * 24576 per system by default, see lm_nbody_set_coarse_max_dim); with Q the
+ * orthonormal cell columns and Q^T M Q = L L^T (block diagonal over the
+ * systems), Lanczos runs on G M G^T with
  * G = D (I + Q (L^{-1} - I) Q^T) (D the learned blocks when a network is
  * attached, else I) and the result is G^{-1} (G M G^T)^{1/2} W.  0 switches
  * it off.  Built on the first sqrtMdotW after each setPositions. */
 int lm_nbody_set_coarse_level(lm_nbody* h, int particles_per_cell);
+/* Extension: the largest coarse dimension 3C_s of one system (default 24576,
+ * at most 98304); denser requests get bigger cells.  The build holds three
+ * dense (3C_s)^2 FP64 matrices per system (14 GB at 24576, 55 GB at 49152). */
+int lm_nbody_set_coarse_max_dim(lm_nbody* h, int64_t max_dim);
+/* Extension: pair arithmetic of the coarse matrix Q^T M Q.  LM_PAIR_FP32
+ * (default) falls back to FP64 by itself when the FP32 matrix turns out
+ * indefinite (a non-positive Cholesky pivot); LM_PAIR_FP64 always uses FP64
+ * (about twice the build's pair work).  lm_nbody_coarse_used_fp64 reports
+ * whether the last build used FP64. */
+int lm_nbody_set_coarse_pair_precision(lm_nbody* h, int precision);
+int lm_nbody_coarse_used_fp64(lm_nbody* h, int* used);
 /* Cells of the last coarse build, over all systems: *ncells, grid[3] (cells
  * per axis of the first system, before empty ones are dropped) and, if
  * cell_of != NULL, the cell of each particle (N ints). */

@@ -88,6 +88,9 @@ SIGNATURES = {
     "lm_nbody_precond_blocks": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_precond_neighbours": (ctypes.c_int, [_vp, _vp]),
     "lm_nbody_set_coarse_level": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lm_nbody_set_coarse_max_dim": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "lm_nbody_set_coarse_pair_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lm_nbody_coarse_used_fp64": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "lm_nbody_coarse_cells": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "lm_nbody_coarse_factors": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "lm_nbody_coarse_system_cells": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),

@@ -354,6 +354,22 @@ int lm_nbody_set_coarse_level(lm_nbody* h, int particles_per_cell) {
   return guarded([&] { S(h).setCoarseLevel(particles_per_cell); });
 }
 
+int lm_nbody_set_coarse_pair_precision(lm_nbody* h, int precision) {
+  return guarded([&] {
+    if (precision != LM_PAIR_FP64 && precision != LM_PAIR_FP32)
+      throw lm::Error("[libmobility_amd] pair precision must be LM_PAIR_FP64 or LM_PAIR_FP32");
+    S(h).setCoarseFP64(precision == LM_PAIR_FP64);
+  });
+}
+
+int lm_nbody_coarse_used_fp64(lm_nbody* h, int* used) {
+  return guarded([&] { *used = S(h).coarseUsedFP64() ? 1 : 0; });
+}
+
+int lm_nbody_set_coarse_max_dim(lm_nbody* h, int64_t max_dim) {
+  return guarded([&] { S(h).setCoarseMaxDim(max_dim); });
+}
+
 int lm_nbody_coarse_cells(lm_nbody* h, int* ncells, int* grid, int* cell_of) {
   return guarded([&] {
     if (!ncells || !grid) throw lm::Error("[libmobility_amd] null output");
@@ -574,7 +590,7 @@ int lm_selftest_coarse_cells_host(const double* xyz, int64_t N, int particles_pe
   return guarded([&] {
     if (!xyz || !ncells || !grid || !cell_of) throw lm::Error("[libmobility_amd] null argument");
     lm::CoarseCells c;
-    lm::coarse_cells(xyz, 3, N, particles_per_cell, c);
+    lm::coarse_cells(xyz, 3, N, particles_per_cell, c, lm::CoarseLevel::kMaxDim);
     *ncells = c.C;
     for (int k = 0; k < 3; ++k) grid[k] = c.g[k];
     std::copy(c.cell_of.begin(), c.cell_of.end(), cell_of);

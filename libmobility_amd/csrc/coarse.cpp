@@ -10,7 +10,8 @@
 
 namespace lm {
 
-void coarse_cells(const double* xyz, int64_t stride, int64_t N, int ppc, CoarseCells& out) {
+void coarse_cells(const double* xyz, int64_t stride, int64_t N, int ppc, CoarseCells& out,
+                  int64_t max_dim) {
   if (N <= 0) throw Error("[libmobility_amd] coarse level: no particles");
   double lo[3], hi[3], ext[3];
   for (int k = 0; k < 3; ++k) lo[k] = hi[k] = xyz[k];
@@ -50,7 +51,7 @@ void coarse_cells(const double* xyz, int64_t stride, int64_t N, int ppc, CoarseC
     for (int64_t i = 0; i < N; ++i) ++count[out.cell_of[i]];
     int C = 0;
     for (int64_t c = 0; c < total; ++c) C += count[c] > 0;
-    if (3 * (int64_t)C <= CoarseLevel::kMaxDim) break;
+    if (3 * (int64_t)C <= max_dim) break;
     side *= 1.03;  // too many cells: coarser grid
   }
   // compact the non-empty cells in raw order, then a stable counting sort
@@ -86,7 +87,8 @@ void SideStream::ensure() {
 }
 
 void CoarseLevel::build(const void* rec64, const void* rec32, const std::vector<int64_t>& off,
-                        const std::vector<int>& wall, int ppc_req, hipStream_t s) {
+                        const std::vector<int>& wall, int ppc_req, int64_t max_dim,
+                        bool force_fp64, hipStream_t s) {
   const auto t_start = std::chrono::steady_clock::now();
   const int nsys = (int)off.size() - 1;
   const int64_t N = off[nsys];
@@ -113,7 +115,7 @@ void CoarseLevel::build(const void* rec64, const void* rec32, const std::vector<
     S.wall = wall[q];
     if (n_q == 0) continue;
     CoarseCells cq;
-    coarse_cells(h.data() + 4 * off[q], 4, n_q, ppc_req, cq);
+    coarse_cells(h.data() + 4 * off[q], 4, n_q, ppc_req, cq, max_dim);
     for (int64_t i = 0; i < n_q; ++i) {
       cells_.cell_of[off[q] + i] = C + cq.cell_of[i];
       cells_.perm[off[q] + i] = (int)(off[q] + cq.perm[i]);
@@ -163,9 +165,6 @@ void CoarseLevel::build(const void* rec64, const void* rec32, const std::vector<
   upload(row_sys_, row_sys.data(), sizeof(int) * row_sys.size());
   upload(pair_off_, pair_off.data(), sizeof(int64_t) * (nsys + 1));
   upload(small_, small.data(), sizeof(int) * small.size());
-  sorted_.reserve(sizeof(float4) * N);
-  launch_coarse_gather(static_cast<const float4*>(rec32), perm_.as<int>(), N, sorted_.as<float4>(),
-                       s);
   A_.reserve(sizeof(double) * moff);
   X_.reserve(sizeof(double) * moff);
   Xt_.reserve(sizeof(double) * moff);
@@ -173,23 +172,32 @@ void CoarseLevel::build(const void* rec64, const void* rec32, const std::vector<
   info_.reserve(sizeof(unsigned int));
   cvec_.reserve(sizeof(double) * 3 * C);
   yvec_.reserve(sizeof(double) * 3 * C);
-  launch_zero_u32(info_.as<unsigned int>(), 1, s);
-  launch_coarse_matrix(sorted_.as<float4>(), start_.as<int>(), scale_.as<double>(),
-                       sys_dev_.as<CoarseSys>(), pair_off_.as<int64_t>(), nsys, pair_off[nsys],
-                       A_.as<double>(), moff, s);
-  if (!large.empty()) side_.ensure();
-  for (int q : large) {
-    const CoarseSys& S = sys_[q];
-    launch_coarse_factor(A_.as<double>() + S.moff, X_.as<double>() + S.moff,
-                         Xt_.as<double>() + S.moff, Dinv_.as<double>() + S.doff, info_.as<int>(),
-                         S.n, S.n, s, side_);
-  }
-  launch_coarse_factor_small(A_.as<double>(), X_.as<double>(), Xt_.as<double>(),
-                             Dinv_.as<double>(), info_.as<int>(), sys_dev_.as<CoarseSys>(),
-                             small_.as<int>(), (int)small.size(), s);
+  // Q^T M Q from FP32 pair terms; when rounding leaves it indefinite (a
+  // non-positive pivot), once more from FP64 pair terms.
   int info = 0;
-  LM_HIP(hipMemcpyAsync(&info, info_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, s));
-  LM_HIP(hipStreamSynchronize(s));
+  for (int fp64 = force_fp64 ? 1 : 0; fp64 < 2; ++fp64) {
+    sorted_.reserve((fp64 ? sizeof(double4) : sizeof(float4)) * N);
+    launch_coarse_gather(fp64 ? rec64 : rec32, perm_.as<int>(), N, sorted_.as<void>(), fp64 != 0,
+                         s);
+    launch_zero_u32(info_.as<unsigned int>(), 1, s);
+    launch_coarse_matrix(sorted_.as<void>(), fp64 != 0, start_.as<int>(), scale_.as<double>(),
+                         sys_dev_.as<CoarseSys>(), pair_off_.as<int64_t>(), nsys, pair_off[nsys],
+                         A_.as<double>(), moff, s);
+    if (!large.empty()) side_.ensure();
+    for (int q : large) {
+      const CoarseSys& S = sys_[q];
+      launch_coarse_factor(A_.as<double>() + S.moff, X_.as<double>() + S.moff,
+                           Xt_.as<double>() + S.moff, Dinv_.as<double>() + S.doff,
+                           info_.as<int>(), S.n, S.n, s, side_);
+    }
+    launch_coarse_factor_small(A_.as<double>(), X_.as<double>(), Xt_.as<double>(),
+                               Dinv_.as<double>(), info_.as<int>(), sys_dev_.as<CoarseSys>(),
+                               small_.as<int>(), (int)small.size(), s);
+    LM_HIP(hipMemcpyAsync(&info, info_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, s));
+    LM_HIP(hipStreamSynchronize(s));
+    fp64_pairs_ = fp64 != 0;
+    if (info == 0) break;
+  }
   if (info != 0) throw Error("[libmobility_amd] coarse level: the coarse matrix is not positive definite");
   build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
                  .count();

@@ -32,12 +32,14 @@ struct CoarseSys {
   int c0, C, n, wall;
 };
 
-void launch_coarse_gather(const float4* rec, const int* perm, int64_t N, float4* out,
+// out[k] = rec[perm[k]]: float4 records, or double4 with fp64
+void launch_coarse_gather(const void* rec, const int* perm, int64_t N, void* out, bool fp64,
                           hipStream_t s);
 // Mc (all systems' blocks, `total` elements, zero-filled here) from
-// cell-sorted FP32 records; pair_off[s] = cell pairs (c >= d) of the systems
-// before s, npairs = all of them.
-void launch_coarse_matrix(const float4* rec, const int* start, const double* scale,
+// cell-sorted records, FP32 pair terms (float4) or FP64 (double4, fp64);
+// pair_off[s] = cell pairs (c >= d) of the systems before s, npairs = all of
+// them.
+void launch_coarse_matrix(const void* rec, bool fp64, const int* start, const double* scale,
                           const CoarseSys* sys, const int64_t* pair_off, int nsys, int64_t npairs,
                           double* Mc, int64_t total, hipStream_t s);
 // A second stream (default priority, non-blocking) and two events, created
@@ -84,14 +86,17 @@ struct CoarseCells {
   int g[3] = {1, 1, 1};
   std::vector<int> cell_of, perm, start;
 };
-void coarse_cells(const double* xyz, int64_t stride, int64_t N, int ppc, CoarseCells& out);
+// max_dim: largest coarse dimension 3C; larger requests get bigger cells
+void coarse_cells(const double* xyz, int64_t stride, int64_t N, int ppc, CoarseCells& out,
+                  int64_t max_dim);
 
 class CoarseLevel {
  public:
   enum Op : int { kG = 0, kGt = 1, kGinv = 2 };
-  // Largest coarse dimension 3C_s of one system; larger requests get bigger
-  // cells.
+  // Default largest coarse dimension 3C_s of one system; larger requests get
+  // bigger cells.  The build holds three dense (3C_s)^2 FP64 matrices.
   static constexpr int kMaxDim = 24576;
+  static constexpr int64_t kMaxDimLimit = 98304;
 
   // Largest number of 64-tiles of a block factored by the one-workgroup path
   // (used when at least kSmallBatch systems qualify).
@@ -100,8 +105,10 @@ class CoarseLevel {
   // rec64 / rec32: the solver's position records; off[0..nsys] the systems'
   // particle offsets, wall[s] 1 for a single-wall system.  Every system gets
   // its own cells and coarse block.
+  // fp64: the coarse matrix from FP64 pair terms (otherwise FP32, and FP64
+  // only when the FP32 one turns out indefinite)
   void build(const void* rec64, const void* rec32, const std::vector<int64_t>& off,
-             const std::vector<int>& wall, int ppc, hipStream_t s);
+             const std::vector<int>& wall, int ppc, int64_t max_dim, bool fp64, hipStream_t s);
   // out = Gc in (kG), Gc^T in (kGt) or Gc^{-1} in (kGinv); vectors of length
   // n >= 3N whose entries past 3N pass through.  in may equal out.
   void apply(Op op, const double* in, double* out, int64_t n, hipStream_t s);
@@ -121,9 +128,15 @@ class CoarseLevel {
   uint64_t layout = 0, structure = 0;
   double a = 0.0;
   int ppc = 0;
+  int64_t max_dim = 0;
+  bool fp64_requested = false;
   double build_ms = 0.0;
+  // the last build's coarse matrix came from FP64 pair terms (the FP32 one
+  // was indefinite)
+  bool fp64_pairs() const { return fp64_pairs_; }
 
  private:
+  bool fp64_pairs_ = false;
   CoarseCells cells_;
   std::vector<CoarseSys> sys_;
   int64_t N_ = 0;

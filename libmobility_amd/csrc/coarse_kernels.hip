@@ -31,8 +31,9 @@ constexpr int64_t kQuadTiles = 1024;
 
 // ---- cell-sorted records ---------------------------------------------------
 
-__global__ void k_gather_records(const float4* __restrict__ rec, const int* __restrict__ perm,
-                                 int64_t N, float4* __restrict__ out) {
+template <class R4>
+__global__ void k_gather_records(const R4* __restrict__ rec, const int* __restrict__ perm,
+                                 int64_t N, R4* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < N) out[k] = rec[perm[k]];
 }
@@ -50,26 +51,26 @@ __device__ inline void tri_index(int64_t p, int& c, int& d) {
 
 // s += the 3x3 translational block M_ij / t0 of the pair (ri, rj),
 // row-major (open: the six entries of the upper triangle)
-template <bool WALL>
-__device__ inline void pair_block_sum(const float4 ri, const float4 rj, float s[9]) {
+template <bool WALL, class R, class R4>
+__device__ inline void pair_block_sum(const R4 ri, const R4 rj, R s[9]) {
   if (WALL) {
-    UFCoef<float> k;
-    uf_coef_wall<float>(ri.x, ri.y, ri.z, ri.w, rj.x, rj.y, rj.z, rj.w, k);
-    const float dv[3] = {k.dx, k.dy, k.dz};
-    const float ev[3] = {k.ex, k.ey, k.ez};
-    const float gv[3] = {k.gx, k.gy, k.gz};
-    const float hv[3] = {k.hx, k.hy, k.hz};
+    UFCoef<R> k;
+    uf_coef_wall<R>(ri.x, ri.y, ri.z, ri.w, rj.x, rj.y, rj.z, rj.w, k);
+    const R dv[3] = {k.dx, k.dy, k.dz};
+    const R ev[3] = {k.ex, k.ey, k.ez};
+    const R gv[3] = {k.gx, k.gy, k.gz};
+    const R hv[3] = {k.hx, k.hy, k.hz};
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
       for (int b = 0; b < 3; ++b)
-        s[3 * a + b] += (a == b ? k.a0 : 0.0f) + k.cd * dv[a] * dv[b] + gv[a] * ev[b] +
-                        (b == 2 ? hv[a] : 0.0f);
+        s[3 * a + b] += (a == b ? k.a0 : R(0)) + k.cd * dv[a] * dv[b] + gv[a] * ev[b] +
+                        (b == 2 ? hv[a] : R(0));
   } else {
-    const float dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    float fuf, guf, fwt, gwt, cc;
-    rpy_coeffs<float, kUF>(r2, fuf, guf, fwt, gwt, cc);
+    const R dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
+    const R r2 = dx * dx + dy * dy + dz * dz;
+    R fuf, guf, fwt, gwt, cc;
+    rpy_coeffs<R, kUF>(r2, fuf, guf, fwt, gwt, cc);
     s[0] += fuf + guf * dx * dx;
     s[1] += guf * dx * dy;
     s[2] += guf * dx * dz;
@@ -84,9 +85,9 @@ __device__ inline void pair_block_sum(const float4 ri, const float4 rj, float s[
 // (sh: 128 records) in chunks of up to 64 x 64 particles, and the 64 lanes
 // walk a chunk's ni x nj pairs in row-major order, lane l taking pairs
 // l, l + 64, ...: every lane works whatever the cell sizes.
-template <bool WALL>
-__device__ inline void cell_pair_sum(const float4* __restrict__ rec, int i0, int i1, int j0,
-                                     int j1, int lane, float4* sh, float s[9]) {
+template <bool WALL, class R, class R4>
+__device__ inline void cell_pair_sum(const R4* __restrict__ rec, int i0, int i1, int j0, int j1,
+                                     int lane, R4* sh, R s[9]) {
   for (int ib = i0; ib < i1; ib += 64) {
     const int ni = min(64, i1 - ib);
     for (int jb = j0; jb < j1; jb += 64) {
@@ -100,7 +101,7 @@ __device__ inline void cell_pair_sum(const float4* __restrict__ rec, int i0, int
       const int di = 64 / nj, dj = 64 - di * nj;  // a step of 64 pairs
       int ii = lane / nj, jj = lane - ii * nj;
       for (int p = lane; p < ni * nj; p += 64) {
-        pair_block_sum<WALL>(sh[ii], sh[64 + jj], s);
+        pair_block_sum<WALL, R, R4>(sh[ii], sh[64 + jj], s);
         ii += di;
         jj += dj;
         if (jj >= nj) {
@@ -121,51 +122,55 @@ __device__ inline void cell_pair_sum(const float4* __restrict__ rec, int i0, int
 // of the two cells, then a butterfly sums the lanes.  The block and its
 // transpose go to the system's matrix (Mc is symmetric because
 // M_ji = M_ij^T).  pair_off[s] counts the cell pairs of the systems before s.
-__global__ __launch_bounds__(256) void k_coarse_matrix(const float4* __restrict__ rec,
+template <class R, class R4>
+__global__ __launch_bounds__(256) void k_coarse_matrix(const R4* __restrict__ rec,
                                                        const int* __restrict__ start,
                                                        const double* __restrict__ scale,
                                                        const CoarseSys* __restrict__ sys,
                                                        const int64_t* __restrict__ pair_off,
                                                        int nsys, int64_t npairs,
                                                        double* __restrict__ Mc) {
-  __shared__ float4 sh[4][128];
+  __shared__ R4 sh[4][128];
   const int lane = threadIdx.x & 63;
-  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (p >= npairs) return;  // uniform per wave
-  int lo = 0, hi = nsys;  // the last system with pair_off <= p
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (pair_off[mid] <= p) lo = mid; else hi = mid;
-  }
-  const CoarseSys S = sys[lo];
-  int c, d;
-  tri_index(p - pair_off[lo], c, d);
-  const int gc = S.c0 + c, gd = S.c0 + d;
-  float s[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) s[k] = 0.0f;
-  float4* w = sh[threadIdx.x >> 6];
-  if (S.wall)
-    cell_pair_sum<true>(rec, start[gc], start[gc + 1], start[gd], start[gd + 1], lane, w, s);
-  else
-    cell_pair_sum<false>(rec, start[gc], start[gc + 1], start[gd], start[gd + 1], lane, w, s);
-  double v[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    v[k] = (double)s[k];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
-  }
-  if (lane == 0) {
-    const double f = scale[gc] * scale[gd];
-    double* M = Mc + S.moff;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        M[(int64_t)(3 * c + a) * S.n + 3 * d + b] = f * v[3 * a + b];
-        if (c != d) M[(int64_t)(3 * d + b) * S.n + 3 * c + a] = f * v[3 * a + b];
-      }
+  // grid-stride over the cell pairs: one launch may not exceed 2^32 work-items
+  for (int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); p < npairs;
+       p += (int64_t)gridDim.x * 4) {  // uniform per wave
+    int lo = 0, hi = nsys;  // the last system with pair_off <= p
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (pair_off[mid] <= p) lo = mid; else hi = mid;
+    }
+    const CoarseSys S = sys[lo];
+    int c, d;
+    tri_index(p - pair_off[lo], c, d);
+    const int gc = S.c0 + c, gd = S.c0 + d;
+    R s[9];
+  #pragma unroll
+    for (int k = 0; k < 9; ++k) s[k] = R(0);
+    R4* w = sh[threadIdx.x >> 6];
+    if (S.wall)
+      cell_pair_sum<true, R, R4>(rec, start[gc], start[gc + 1], start[gd], start[gd + 1], lane, w, s);
+    else
+      cell_pair_sum<false, R, R4>(rec, start[gc], start[gc + 1], start[gd], start[gd + 1], lane, w,
+                                  s);
+    double v[9];
+  #pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      v[k] = (double)s[k];
+  #pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
+    }
+    if (lane == 0) {
+      const double f = scale[gc] * scale[gd];
+      double* M = Mc + S.moff;
+  #pragma unroll
+      for (int a = 0; a < 3; ++a)
+  #pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          M[(int64_t)(3 * c + a) * S.n + 3 * d + b] = f * v[3 * a + b];
+          if (c != d) M[(int64_t)(3 * d + b) * S.n + 3 * c + a] = f * v[3 * a + b];
+        }
+    }
   }
 }
 
@@ -595,10 +600,11 @@ __global__ __launch_bounds__(256) void k_trinv_trailing(double* __restrict__ X,
 }
 
 __global__ void k_identity(double* __restrict__ X, int64_t ld, int n) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)n * n) return;
-  const int64_t r = e / n, c = e % n;
-  X[r * ld + c] = r == c ? 1.0 : 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)n * n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / n, c = e % n;
+    X[r * ld + c] = r == c ? 1.0 : 0.0;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ X,
@@ -752,24 +758,39 @@ __global__ void k_coarse_prolong(const double* in, const int* __restrict__ cell_
 }
 
 inline unsigned blocks_for(int64_t n, int nt) { return (unsigned)((n + nt - 1) / nt); }
+// Workgroups of a grid-stride launch: a dispatch holds at most 2^32 work-items.
+inline unsigned stride_blocks(int64_t n, int per_block) {
+  const int64_t b = (n + per_block - 1) / per_block;
+  return (unsigned)(b < (int64_t)1 << 20 ? b : (int64_t)1 << 20);
+}
 
 }  // namespace
 
-void launch_coarse_gather(const float4* rec, const int* perm, int64_t N, float4* out,
+void launch_coarse_gather(const void* rec, const int* perm, int64_t N, void* out, bool fp64,
                           hipStream_t s) {
   if (N <= 0) return;
-  hipLaunchKernelGGL(k_gather_records, dim3(blocks_for(N, 256)), dim3(256), 0, s, rec, perm, N,
-                     out);
+  if (fp64)
+    hipLaunchKernelGGL(k_gather_records<double4>, dim3(blocks_for(N, 256)), dim3(256), 0, s,
+                       static_cast<const double4*>(rec), perm, N, static_cast<double4*>(out));
+  else
+    hipLaunchKernelGGL(k_gather_records<float4>, dim3(blocks_for(N, 256)), dim3(256), 0, s,
+                       static_cast<const float4*>(rec), perm, N, static_cast<float4*>(out));
   LM_HIP(hipGetLastError());
 }
 
-void launch_coarse_matrix(const float4* rec, const int* start, const double* scale,
+void launch_coarse_matrix(const void* rec, bool fp64, const int* start, const double* scale,
                           const CoarseSys* sys, const int64_t* pair_off, int nsys, int64_t npairs,
                           double* Mc, int64_t total, hipStream_t s) {
   launch_zero(Mc, total, s);
   if (npairs > 0) {
-    hipLaunchKernelGGL(k_coarse_matrix, dim3(blocks_for(npairs, 4)), dim3(256), 0, s, rec, start,
-                       scale, sys, pair_off, nsys, npairs, Mc);
+    if (fp64)
+      hipLaunchKernelGGL((k_coarse_matrix<double, double4>), dim3(stride_blocks(npairs, 4)),
+                         dim3(256), 0, s, static_cast<const double4*>(rec), start, scale, sys,
+                         pair_off, nsys, npairs, Mc);
+    else
+      hipLaunchKernelGGL((k_coarse_matrix<float, float4>), dim3(stride_blocks(npairs, 4)), dim3(256),
+                         0, s, static_cast<const float4*>(rec), start, scale, sys, pair_off, nsys,
+                         npairs, Mc);
     LM_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_pad_identity, dim3(nsys), dim3(64), 0, s, Mc, sys);
@@ -783,7 +804,8 @@ void launch_coarse_factor(double* A, double* X, double* Xt, double* Dinv, int* i
   // X may still be read by work queued on s before this call.
   LM_HIP(hipEventRecord(side.block, s));
   LM_HIP(hipStreamWaitEvent(x, side.block, 0));
-  hipLaunchKernelGGL(k_identity, dim3(blocks_for((int64_t)n * n, 256)), dim3(256), 0, x, X, ld, n);
+  hipLaunchKernelGGL(k_identity, dim3(stride_blocks((int64_t)n * n, 256)), dim3(256), 0, x, X, ld,
+                     n);
   for (int kb = 0; kb < nb; kb += kKB) {
     const int kend = kb + kKB < nb ? kb + kKB : nb;
     for (int k = kb; k < kend; ++k) {

@@ -1735,21 +1735,32 @@ void NBodySolver::setCoarseLevel(int particles_per_cell) {
   coarse_ppc_ = particles_per_cell;
 }
 
+void NBodySolver::setCoarseMaxDim(int64_t max_dim) {
+  if (max_dim < 3 || max_dim > CoarseLevel::kMaxDimLimit)
+    throw Error("[libmobility_amd] coarse dimension cap must be in [3, " +
+                std::to_string(CoarseLevel::kMaxDimLimit) + "]");
+  coarse_max_dim_ = max_dim;
+}
+
 void NBodySolver::ensure_coarse() {
   if (coarse_ppc_ <= 0) return;
   ensure_records();
-  if (coarse_.ppc == coarse_ppc_ && coarse_.cells() > 0 &&
+  if (coarse_.ppc == coarse_ppc_ && coarse_.max_dim == coarse_max_dim_ &&
+      coarse_.fp64_requested == coarse_fp64_ && coarse_.cells() > 0 &&
       pre_current(coarse_.layout, coarse_.structure, coarse_.a))
     return;
   const int nsys = numSystems();
   std::vector<int> wall(nsys);
   for (int q = 0; q < nsys; ++q)
     wall[q] = q < (int)sys_kind_host_.size() ? (sys_kind_host_[q] != 0) : (wall_ ? 1 : 0);
-  coarse_.build(rec64_.as<void>(), rec32_.as<void>(), sys_off_host_, wall, coarse_ppc_, stream_);
+  coarse_.build(rec64_.as<void>(), rec32_.as<void>(), sys_off_host_, wall, coarse_ppc_,
+                coarse_max_dim_, coarse_fp64_, stream_);
   coarse_.layout = layout_id_;
   coarse_.structure = struct_id_;
   coarse_.a = a_;
   coarse_.ppc = coarse_ppc_;
+  coarse_.max_dim = coarse_max_dim_;
+  coarse_.fp64_requested = coarse_fp64_;
   ++coarse_builds_;
 }
 

@@ -133,6 +133,14 @@ class NBodySolver {
   // G^{-1} (G M G^T)^{1/2} W.  0 switches it off.
   void setCoarseLevel(int particles_per_cell);
   int coarseLevel() const { return coarse_ppc_; }
+  // Largest coarse dimension 3C per system (default CoarseLevel::kMaxDim);
+  // denser systems get bigger cells.  The build holds three dense (3C)^2 FP64
+  // matrices per system.
+  void setCoarseMaxDim(int64_t max_dim);
+  // Pair arithmetic of the coarse matrix Q^T M Q: FP32 by default (FP64 when
+  // the FP32 matrix is indefinite), FP64 always with fp64.
+  void setCoarseFP64(bool fp64) { coarse_fp64_ = fp64; }
+  bool coarseUsedFP64() const { return coarse_.fp64_pairs(); }
   // Cells of the last coarse build (cell_of: N entries, host) and the grid;
   // returns the number of cells.
   int coarseCells(int* cell_of, int* grid) const;
@@ -383,6 +391,8 @@ class NBodySolver {
   void ensure_coarse();
   CoarseLevel coarse_;
   int coarse_ppc_ = 0;
+  int64_t coarse_max_dim_ = CoarseLevel::kMaxDim;
+  bool coarse_fp64_ = false;
   int64_t coarse_builds_ = 0;
   // RFD
   DevBuf rfd_noise_, rfd_pos_, rfd_rec64_, rfd_rec32_, rfd_mp_, rfd_mm_, rfd_wp_, rfd_wm_;

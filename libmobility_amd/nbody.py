@@ -446,6 +446,27 @@ class NBody(_Solver):
         _ffi.check(self._lib.lm_nbody_precond_blocks(self._h, G.ctypes.data))
         return G
 
+    def setCoarseMaxDim(self, maxDim=24576):
+        """Extension: the largest coarse dimension 3C of one system (default
+        24576); denser requests get bigger cells.  The build holds three dense
+        (3C)^2 FP64 matrices per system."""
+        _ffi.check(self._lib.lm_nbody_set_coarse_max_dim(self._h, int(maxDim)))
+
+    def setCoarsePairPrecision(self, precision="fp32"):
+        """Extension: pair arithmetic of the coarse matrix Q^T M Q; 'fp32'
+        (default) falls back to FP64 by itself when the FP32 matrix turns out
+        indefinite, 'fp64' always uses FP64."""
+        if precision not in _ffi.PAIR_PRECISION:
+            raise RuntimeError("[libmobility_amd] pair precision must be 'fp64' or 'fp32'")
+        _ffi.check(self._lib.lm_nbody_set_coarse_pair_precision(self._h,
+                                                                 _ffi.PAIR_PRECISION[precision]))
+
+    def coarseUsedFP64(self):
+        """Extension: whether the last coarse build's matrix came from FP64 pairs."""
+        used = ctypes.c_int(0)
+        _ffi.check(self._lib.lm_nbody_coarse_used_fp64(self._h, ctypes.byref(used)))
+        return bool(used.value)
+
     def setCoarseLevel(self, particlesPerCell=40):
         """Extension: coarse level of the sqrtMdotW preconditioner (every system
         of an ensemble gets its own cells and coarse block).

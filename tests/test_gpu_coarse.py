@@ -449,3 +449,27 @@ def test_factors_on_the_two_by_two_tile_path():
     print(f"[coarse] C={C}, {nb} tiles: |LL^T - Mc|/|Mc| = {rel:.2e}, max|XL - I| = {err:.2e}")
     assert rel <= 1e-5
     assert err <= 1e-11
+
+
+@pytest.mark.parametrize("wall", [False, True])
+def test_fp64_coarse_matrix_matches_the_oracle(wall):
+    """The coarse matrix from FP64 pair terms (the fallback for an FP32 matrix
+    that rounding leaves indefinite, or asked for): L L^T equals the oracle's
+    Q^T M Q to FP64 rounding, where the FP32 build agrees to ~1e-7."""
+    from libmobility_amd.workloads import rsa_cube
+    N = 1500
+    pos = _wall_positions(N, 3) if wall else rsa_cube(N, 0.2, seed=3)[0]
+    s = _solver(pos, wall=wall, ppc=40)
+    s.setCoarsePairPrecision("fp64")
+    s.sqrtMdotW_with_noise(np.random.default_rng(0).normal(size=3 * N))
+    assert s.coarseUsedFP64()
+    cell_of, grid, C = s.coarseCells()
+    L, X = s.coarseFactors()
+    M = oracle.dense_matrix(pos, ETA, 1.0, wall=wall) / (1.0 / (6 * np.pi * ETA))
+    Mc = coarse.coarse_matrix(M, cell_of, C)
+    rel = np.linalg.norm(L @ L.T - Mc) / np.linalg.norm(Mc)
+    assert rel <= 1e-12, rel
+    assert np.abs(X @ L - np.eye(3 * C)).max() <= 1e-11
+    s32 = _solver(pos, wall=wall, ppc=40)
+    s32.sqrtMdotW_with_noise(np.random.default_rng(0).normal(size=3 * N))
+    assert not s32.coarseUsedFP64()
