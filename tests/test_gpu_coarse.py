@@ -473,3 +473,27 @@ def test_fp64_coarse_matrix_matches_the_oracle(wall):
     s32 = _solver(pos, wall=wall, ppc=40)
     s32.sqrtMdotW_with_noise(np.random.default_rng(0).normal(size=3 * N))
     assert not s32.coarseUsedFP64()
+
+
+def test_more_than_8192_cells():
+    """A coarse level of 15625 cells (open RSA packing, N = 131072, 8 per
+    cell, dimension cap raised to 49152): its coarse matrix has 1.2e8 cell
+    pairs, more than one dispatch of one wave per pair can hold (2^32
+    work-items), so the kernel strides over them.  Free-space RPY is SPD for
+    any configuration, so the build must succeed, and the finer cells must
+    cut the iterations below those of the default 40 per cell (38 at this
+    size)."""
+    from libmobility_amd.workloads import config_c3
+    pos = config_c3(131072, 0.2)[0]
+    W = np.random.default_rng(1).normal(size=3 * pos.shape[0])
+    s = _solver(pos, tol=1e-6, ppc=8)
+    s.setPairPrecision("fp32")
+    s.setCoarseMaxDim(49152)
+    s.sqrtMdotW_with_noise(W)
+    C = s.coarseCells()[2]
+    assert 8192 < C <= 16384, C
+    fine = s.lastLanczosIterations()
+    s40 = _solver(pos, tol=1e-6, ppc=40)
+    s40.setPairPrecision("fp32")
+    s40.sqrtMdotW_with_noise(W)
+    assert fine < s40.lastLanczosIterations(), (fine, s40.lastLanczosIterations())
