@@ -111,6 +111,9 @@ struct SymArgs {
   const SymItem* items;
   int nitems;
   const int* active;  // per-system flags; items of systems with 0 return at once (nullptr: all)
+  // a workgroup that finds *stop != 0 when it starts does nothing (products a
+  // finished Lanczos solve no longer needs; nullptr: never)
+  const int* stop;
   double t0, rt0, r0;
 };
 void launch_pair_sym(const SymArgs& a, hipStream_t s);

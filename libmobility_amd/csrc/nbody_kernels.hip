@@ -616,6 +616,13 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_sym(SymArgs a) {
   __shared__ R4 s_f[kNeedF ? 4 * 128 : 1];
   __shared__ R4 s_t[kNeedT ? 4 * 128 : 1];
   __shared__ R s_red[4 * kComp * kPairThreads];
+  if (a.stop != nullptr) {
+    // one read per workgroup, so that all its threads take the same branch
+    __shared__ int s_stop;
+    if (threadIdx.x == 0) s_stop = *(volatile const int*)a.stop;
+    __syncthreads();
+    if (s_stop != 0) return;
+  }
   const SymItem it = a.items[blockIdx.x];
   if (a.active != nullptr && a.active[it.sys] == 0) return;  // system already converged
   const SymSys S = a.sys[it.sys];

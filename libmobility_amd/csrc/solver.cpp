@@ -532,6 +532,7 @@ void NBodySolver::run_sym(int tpl, int nch, int h, int blocks, bool fp32, const 
     a.items = items + ps.item0;
     a.nitems = ps.nitems;
     a.active = pair_active_;
+    a.stop = pair_stop_;
     a.t0 = t0_;
     a.rt0 = rt0_;
     a.r0 = r0_;
@@ -941,8 +942,11 @@ void NBodySolver::setLanczosIterationLimit(int limit) {
 int NBodySolver::lanczos_run(const double* z, double** result) {
   const int64_t n = (angular_ ? 6 : 3) * N_;
   ensure_records();
-  const bool lookahead = N_ <= kLanczosLookaheadN;
-  const int batch = lookahead ? kCheckBatch : 1;
+  // The checks run on a second stream, overlapped with the next iterations:
+  // kCheckBatch at a time for small systems, one at a time for large ones
+  // (where a product costs milliseconds, a check of every iteration decides
+  // before the next product is far along).
+  const int batch = N_ <= kLanczosLookaheadN ? kCheckBatch : 1;
   const bool fused = n <= kLanczosFusedMaxN;
   // workspace
   red_partial_.reserve(sizeof(double) * 4 * kReduceBlocks);
@@ -957,10 +961,8 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
   lz_status_.reserve(kLzSlots * sizeof(LanczosStatus), hipHostMallocMapped | hipHostMallocCoherent);
   for (auto& e : lz_ev_)
     if (!e) LM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  if (lookahead) {
-    if (!chk_stream_) LM_HIP(hipStreamCreateWithFlags(&chk_stream_, hipStreamNonBlocking));
-    if (!lz_vec_ev_) LM_HIP(hipEventCreateWithFlags(&lz_vec_ev_, hipEventDisableTiming));
-  }
+  if (!chk_stream_) LM_HIP(hipStreamCreateWithFlags(&chk_stream_, hipStreamNonBlocking));
+  if (!lz_vec_ev_) LM_HIP(hipEventCreateWithFlags(&lz_vec_ev_, hipEventDisableTiming));
   chk_partial_.reserve(sizeof(double) * 2 * kCheckBatch * kReduceBlocks);
   if (chk_counter_.capacity() == 0) {
     chk_counter_.reserve(sizeof(unsigned int) * 64);
@@ -968,7 +970,7 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
   }
   ReduceScratch rs{red_partial_.as<double>(), red_counter_.as<unsigned int>()};
   ReduceScratch rs_chk{chk_partial_.as<double>(), chk_counter_.as<unsigned int>()};
-  hipStream_t cs = lookahead ? chk_stream_ : stream_;
+  hipStream_t cs = chk_stream_;
   LanczosScalars* sc = scalars_.as<LanczosScalars>();
   // Estimate ring; the previous estimate (oldBz) keeps its prefix across
   // calls (LanczosAlgorithm.cu:175).
@@ -999,7 +1001,7 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
   // A basis that grows is copied and the old one freed: the check stream
   // must be done with it first.
   auto ensure_cols = [&](int64_t cols) {
-    if (lookahead && cols > V_cols_) LM_HIP(hipStreamSynchronize(chk_stream_));
+    if (cols > V_cols_) LM_HIP(hipStreamSynchronize(chk_stream_));
     ensure_basis(n, cols);
   };
   if (pre_on()) {
@@ -1027,14 +1029,14 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
   double* V = V_.as<double>();
   double* w = w_.as<double>();
   launch_lanczos_start(z, V, n, rs, sc, old_host_, stream_);  // LanczosAlgorithm.cu:110-118
-  if (lookahead) pair_stop_ = &sc->status.conv;
+  pair_stop_ = &sc->status.conv;
   const int check = std::min(check_steps_, iter_limit_ - 2);
   LanczosStatus* mirror = lz_status_.as<LanczosStatus>();
   LanczosStatus* mirror_dev = lz_status_.device_as<LanczosStatus>();
   // On every exit the check stream has finished (the next solve resets the
   // status it reads).
   auto drain = [&]() {
-    if (lookahead) LM_HIP(hipStreamSynchronize(chk_stream_));
+    LM_HIP(hipStreamSynchronize(chk_stream_));
   };
   // Reads the status of the batch of checks starting at iteration `first`;
   // returns true once the solve is finished (the tolerance met).
@@ -1078,7 +1080,7 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     return done;
   };
   auto finish = [&]() {
-    if (lookahead) LM_HIP(hipStreamWaitEvent(stream_, lz_ev_[(nbatches - 1) % kLzSlots], 0));
+    LM_HIP(hipStreamWaitEvent(stream_, lz_ev_[(nbatches - 1) % kLzSlots], 0));
     return last_iters_;
   };
   for (int i = 0; i < iter_limit_; ++i) {
@@ -1131,10 +1133,8 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     // checks from iteration `check` on (every one with a callback), in batches
     if (bfirst < 0) bfirst = i;
     if (i - bfirst + 1 < batch && i < iter_limit_ - 1) continue;
-    if (lookahead) {
-      LM_HIP(hipEventRecord(lz_vec_ev_, stream_));
-      LM_HIP(hipStreamWaitEvent(chk_stream_, lz_vec_ev_, 0));
-    }
+    LM_HIP(hipEventRecord(lz_vec_ev_, stream_));
+    LM_HIP(hipStreamWaitEvent(chk_stream_, lz_vec_ev_, 0));
     const int slot = nbatches++ % kLzSlots;
     mirror[slot].first = -2;
     CheckArgs ca{};
@@ -1156,10 +1156,6 @@ int NBodySolver::lanczos_run(const double* z, double** result) {
     LM_HIP(hipEventRecord(lz_ev_[slot], cs));
     const int first = bfirst;
     bfirst = -1;
-    if (!lookahead) {
-      if (settle(first, slot)) return last_iters_;
-      continue;
-    }
     pend_f[npend] = first;
     pend_s[npend] = slot;
     ++npend;

@@ -202,10 +202,10 @@ class NBodySolver {
   void pair_product(const void* rec64, const void* rec32, const double* F, const double* T,
                     double* U, double* W, int prec = -1);
   int lanczos_run(const double* z, double** result);
-  // Systems up to this many particles run their convergence checks on a
-  // second stream, kCheckBatch at a time, while the main stream goes on with
-  // the next iterations (the products wasted past convergence are cheap
-  // there); larger ones check every iteration on the main stream and wait.
+  // The convergence checks run on a second stream while the main stream goes
+  // on with the next iterations; systems up to this many particles check
+  // kCheckBatch iterations per launch, larger ones every iteration on its own
+  // (pair workgroups launched after the device has decided return at once).
   static constexpr int64_t kLanczosLookaheadN = 32768;
   // Batches of checks in flight before the host waits for the oldest.
   static constexpr int kLanczosMaxLag = 1;
