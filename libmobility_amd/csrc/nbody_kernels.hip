@@ -44,6 +44,27 @@ __device__ inline R4 far_record() {
   return r;
 }
 
+// Overlap test of a group of pairs: is any r^2 below 4 (radius units)?  In
+// FP32 it keeps the smallest r^2 by its bit pattern, which orders like the
+// value for r^2 >= 0 (NaN sorts above 4 and stays on the far formula), so a
+// group costs one unsigned min per pair (the compiler forms v_min3_u32) and a
+// single compare instead of a compare per pair.
+template <class R>
+struct NearTest {
+  bool any = false;
+  __device__ void add(R r2) { any |= r2 < R(4); }
+  __device__ bool hit() const { return any; }
+};
+template <>
+struct NearTest<float> {
+  uint32_t m = 0xffffffffu;
+  __device__ void add(float r2) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, r2);
+    m = b < m ? b : m;
+  }
+  __device__ bool hit() const { return m < 0x40800000u; }  // the bits of 4.0f
+};
+
 // One item: targets [tgt0, tgt0 + ntgt) x sources [src0, src1) of one system.
 template <class R, bool WALL, int BLK, int TPL>
 __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* __restrict__ pos,
@@ -110,7 +131,7 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
         R dx[SG][TPL], dy[SG][TPL], dz[SG][TPL], r2[SG][TPL], ir[SG][TPL];
         RpyCoef<R> cf[SG][TPL];
         R4 fj[SG], tj[SG];
-        bool near = false;
+        NearTest<R> near;
 #pragma unroll
         for (int g = 0; g < SG; ++g) {
           const R4 pj = s_pos[s + g];
@@ -125,10 +146,10 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
             ir[g][k] = lm_rsqrt(r2[g][k]);
             cf[g][k] = RpyCoef<R>{};
             rpy_far<R, BLK>(ir[g][k], cf[g][k]);
-            near |= r2[g][k] < R(4);
+            near.add(r2[g][k]);
           }
         }
-        if (__builtin_expect(near, 0)) {
+        if (__builtin_expect(near.hit(), 0)) {
           // opaque statement: keep this a branch (not speculated onto every group)
           __asm__ __volatile__("");
 #pragma unroll
@@ -321,6 +342,7 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
     int J = it.tile + o;
     if (J >= S.ntiles) J -= S.ntiles;
     double* __restrict__ So = a.slab_s + (int64_t)(o - a.obase) * a.n3;
+#pragma unroll 1
     for (int sub = 0; sub < TPL; ++sub) {
       const int64_t base = S.off + (int64_t)J * kTile + sub * kPairThreads;
       if (base >= N) break;
@@ -337,6 +359,7 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
       s_f[wave * 128 + lane] = fj;
       s_f[wave * 128 + 64 + lane] = fj;
       __syncthreads();
+#pragma unroll 1  // one copy of the step loop (the blocks differ only in address)
       for (int blk = 0; blk < 4; ++blk) {
         const R4* sp = s_pos + blk * 128 + lane;
         const R4* sf = s_f + blk * 128 + lane;
@@ -346,7 +369,7 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
           R dx[SG][TPL], dy[SG][TPL], dz[SG][TPL], r2[SG][TPL], ir[SG][TPL];
           RpyCoef<R> cf[SG][TPL];
           R4 pg[SG], fg[SG];
-          bool near = false;
+          NearTest<R> near;
 #pragma unroll
           for (int g = 0; g < SG; ++g) {
             pg[g] = sp[k + g];
@@ -360,10 +383,10 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
               ir[g][t] = lm_rsqrt(r2[g][t]);
               cf[g][t] = RpyCoef<R>{};
               rpy_far_uf34<R>(ir[g][t], cf[g][t]);
-              near |= r2[g][t] < R(4);
+              near.add(r2[g][t]);
             }
           }
-          if (__builtin_expect(near, 0)) {
+          if (__builtin_expect(near.hit(), 0)) {
             __asm__ __volatile__("");  // keep this a branch
 #pragma unroll
             for (int g = 0; g < SG; ++g)
@@ -372,9 +395,11 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
           }
 #pragma unroll
           for (int g = 0; g < SG; ++g) {
-            // this step's source contribution first; only the final add (with
-            // the move) depends on the previous step
-            Vec3<R> c;
+            // the running sum moves first and the step's terms accumulate
+            // into it: one move per component and step, no separate add
+            sa.x = lane_rol1(sa.x);
+            sa.y = lane_rol1(sa.y);
+            sa.z = lane_rol1(sa.z);
 #pragma unroll
             for (int t = 0; t < TPL; ++t) {
               const R fuf = cf[g][t].fuf, guf = cf[g][t].guf;
@@ -383,19 +408,10 @@ __device__ inline void sym_item(const SymItem& it, const SymArgs& a, typename Re
               acc[t].y = lm_fma(st, dy[g][t], lm_fma(fuf, fg[g].y, acc[t].y));
               acc[t].z = lm_fma(st, dz[g][t], lm_fma(fuf, fg[g].z, acc[t].z));
               const R ss = guf * (dx[g][t] * fix[t] + dy[g][t] * fiy[t] + dz[g][t] * fiz[t]);
-              if (t == 0) {
-                c.x = lm_fma(ss, dx[g][t], fuf * fix[t]);
-                c.y = lm_fma(ss, dy[g][t], fuf * fiy[t]);
-                c.z = lm_fma(ss, dz[g][t], fuf * fiz[t]);
-              } else {
-                c.x = lm_fma(ss, dx[g][t], lm_fma(fuf, fix[t], c.x));
-                c.y = lm_fma(ss, dy[g][t], lm_fma(fuf, fiy[t], c.y));
-                c.z = lm_fma(ss, dz[g][t], lm_fma(fuf, fiz[t], c.z));
-              }
+              sa.x = lm_fma(ss, dx[g][t], lm_fma(fuf, fix[t], sa.x));
+              sa.y = lm_fma(ss, dy[g][t], lm_fma(fuf, fiy[t], sa.y));
+              sa.z = lm_fma(ss, dz[g][t], lm_fma(fuf, fiz[t], sa.z));
             }
-            sa.x = lane_rol1(sa.x) + c.x;
-            sa.y = lane_rol1(sa.y) + c.y;
-            sa.z = lane_rol1(sa.z) + c.z;
           }
         }
         // after 64 steps lane l holds source l - 1: one more move puts source l in lane l
