@@ -44,6 +44,12 @@ __host__ __device__ inline void acc_zero(PairAcc<R>& a) {
   a.wf = {R(0), R(0), R(0)};
 }
 
+// Marks the branch it is placed in as one that stays a branch: an opaque
+// statement, so the compiler does not evaluate that branch's arithmetic for
+// every lane and select afterwards.  For the self-pair terms (r == 0), which
+// are rare: off the diagonal tiles only coincident distinct particles take them.
+__host__ __device__ inline void lm_rare_path() { __asm__ __volatile__(""); }
+
 __host__ __device__ inline float lm_rsqrt(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __builtin_amdgcn_rsqf(x);
@@ -277,6 +283,7 @@ __host__ __device__ inline void uf_coef_wall(R xi, R yi, R hi, R bi, R xj, R yj,
     k.hz = b * (f3 * ez + f5);
   } else {
     // self (Swan & Brady B1), as in wall_pair
+    lm_rare_path();
     const R ih = R(1) / hj;
     const R ih2 = ih * ih;
     const R ih3 = ih2 * ih;
@@ -374,6 +381,7 @@ __host__ __device__ inline void wall_pair(R xi, R yi, R hi, R bi, R xj, R yj, R 
   } else {
     // Self terms (r == 0), Swan & Brady B1, B2, B3 as in :190-202, :256-261,
     // :290-294, :327-331.  h_i == h_j here.
+    lm_rare_path();
     const R ih = R(1) / hj;
     const R ih2 = ih * ih;
     const R ih3 = ih2 * ih;
@@ -539,6 +547,7 @@ __host__ __device__ inline void pair_both(R xi, R yi, R hi, R bi, R xj, R yj, R 
     } else {
       // Self terms (r == 0, h_i == h_j), Swan & Brady B1-B3 as in wall_pair;
       // each block equals its transpose's counterpart, so j gets the same forms.
+      lm_rare_path();
       const R ih = R(1) / hj;
       const R ih2 = ih * ih;
       const R ih3 = ih2 * ih;
