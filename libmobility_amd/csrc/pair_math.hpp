@@ -77,7 +77,19 @@ __host__ __device__ inline float lm_rcp(float x) {
   return 1.0f / x;
 #endif
 }
-__host__ __device__ inline double lm_rcp(double x) { return 1.0 / x; }
+// FP64: hardware estimate refined by two Newton steps y (2 - x y) (relative
+// error e0^4 for an estimate error e0).  Callers pass finite x > 0.
+__host__ __device__ inline double lm_rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double y = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-x, y, 1.0);
+  return __builtin_fma(y, e, y);
+#else
+  return 1.0 / x;
+#endif
+}
 __host__ __device__ inline float lm_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __host__ __device__ inline double lm_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __host__ __device__ inline float lm_sqrt(float x) { return __builtin_sqrtf(x); }
@@ -450,6 +462,243 @@ __host__ __device__ inline void wall_pair(R xi, R yi, R hi, R bi, R xj, R yj, R 
 // The target accumulates M_ij [F_j; T_j] into a (as wall_pair / open_pair);
 // the source sums are kept in units of t0 (su: UF part + kut * UT part,
 // kut = rt0/t0) and r0 (sw: WT part + kwf * WF part, kwf = rt0/r0).
+// Open boundaries: both directions of every block from the RPY coefficients
+// (UF, WT even in d; c (v x d) transposes to -c (v x d)).
+template <class R, int BLK>
+__host__ __device__ inline void open_both(R dx, R dy, R dz, R fuf, R guf, R fwt, R gwt, R c,
+                                          const Vec3<R>& Fj, const Vec3<R>& Tj,
+                                          const Vec3<R>& Fi, const Vec3<R>& Ti, R kut, R kwf,
+                                          PairAcc<R>& a, Vec3<R>& su, Vec3<R>& sw) {
+  if (BLK & kUF) {
+    const R s = guf * (dx * Fj.x + dy * Fj.y + dz * Fj.z);
+    a.uf.x = lm_fma(s, dx, lm_fma(fuf, Fj.x, a.uf.x));
+    a.uf.y = lm_fma(s, dy, lm_fma(fuf, Fj.y, a.uf.y));
+    a.uf.z = lm_fma(s, dz, lm_fma(fuf, Fj.z, a.uf.z));
+    const R ss = guf * (dx * Fi.x + dy * Fi.y + dz * Fi.z);
+    su.x = lm_fma(ss, dx, lm_fma(fuf, Fi.x, su.x));
+    su.y = lm_fma(ss, dy, lm_fma(fuf, Fi.y, su.y));
+    su.z = lm_fma(ss, dz, lm_fma(fuf, Fi.z, su.z));
+  }
+  if (BLK & kWT) {
+    const R s = gwt * (dx * Tj.x + dy * Tj.y + dz * Tj.z);
+    a.wt.x = lm_fma(s, dx, lm_fma(fwt, Tj.x, a.wt.x));
+    a.wt.y = lm_fma(s, dy, lm_fma(fwt, Tj.y, a.wt.y));
+    a.wt.z = lm_fma(s, dz, lm_fma(fwt, Tj.z, a.wt.z));
+    const R ss = gwt * (dx * Ti.x + dy * Ti.y + dz * Ti.z);
+    sw.x = lm_fma(ss, dx, lm_fma(fwt, Ti.x, sw.x));
+    sw.y = lm_fma(ss, dy, lm_fma(fwt, Ti.y, sw.y));
+    sw.z = lm_fma(ss, dz, lm_fma(fwt, Ti.z, sw.z));
+  }
+  if (BLK & kUT) {  // c (T_j x d) to i; -c (T_i x d) to j, in units of t0
+    a.ut.x = lm_fma(c, Tj.y * dz - Tj.z * dy, a.ut.x);
+    a.ut.y = lm_fma(c, Tj.z * dx - Tj.x * dz, a.ut.y);
+    a.ut.z = lm_fma(c, Tj.x * dy - Tj.y * dx, a.ut.z);
+    const R k = -kut * c;
+    su.x = lm_fma(k, Ti.y * dz - Ti.z * dy, su.x);
+    su.y = lm_fma(k, Ti.z * dx - Ti.x * dz, su.y);
+    su.z = lm_fma(k, Ti.x * dy - Ti.y * dx, su.z);
+  }
+  if (BLK & kWF) {  // c (F_j x d) to i; -c (F_i x d) to j, in units of r0
+    a.wf.x = lm_fma(c, Fj.y * dz - Fj.z * dy, a.wf.x);
+    a.wf.y = lm_fma(c, Fj.z * dx - Fj.x * dz, a.wf.y);
+    a.wf.z = lm_fma(c, Fj.x * dy - Fj.y * dx, a.wf.z);
+    const R k = -kwf * c;
+    sw.x = lm_fma(k, Fi.y * dz - Fi.z * dy, sw.x);
+    sw.y = lm_fma(k, Fi.z * dx - Fi.x * dz, sw.y);
+    sw.z = lm_fma(k, Fi.x * dy - Fi.y * dx, sw.z);
+  }
+}
+
+// Bottom wall, r > 0: the open part and the image terms of each block merged
+// into one set of coefficients per pair, with the damping b = b_i b_j and the
+// unit factors kut, kwf folded in, applied to both directions.  e = (d_x, d_y,
+// R_z)/R, so e's in-plane part is d's divided by R; u = 1/R^2; for a vector v
+// (F or T of the other particle) c_v = d_y v_x - d_x v_y is the in-plane
+// perpendicular component.
+//   UF:  x, y: A v + (G (d.v) + Q/R) d,  z: (A + F5) v_z + G (d.v) d_z + Q e_z + F4 (e.v)
+//        with A = b (fuf + f1), G = b guf, Fk = b fk, Q = F2 (e.v) + F3 v_z; f3 and f4
+//        swap for j.
+//   WT:  x: Aw v_x + (Gw (d.v) + G2 (e.v)/R) d_x + G4 c_v d_y,  y: ... - G4 c_v d_x,
+//        z: Aw v_z + Gw (d.v) d_z + b (g2 e_z + g3) (e.v); for j, G2 (e.v) + G3 v_z
+//        takes the place of G2 (e.v) and of the z coefficient.
+//   WF, UT and their transposes: x, y from (v_y, -v_x) and (d_x, d_y), z from c_v.
+// The coefficient polynomials are those of wall_pair (Swan & Brady B1-B3), in
+// Horner form in u.
+template <class R, int BLK>
+__host__ __device__ inline void wall_image_both(R dx, R dy, R dz, R hi, R hj, R b, R fuf, R guf,
+                                                R fwt, R gwt, R c, const Vec3<R>& Fj,
+                                                const Vec3<R>& Tj, const Vec3<R>& Fi,
+                                                const Vec3<R>& Ti, R kut, R kwf, PairAcc<R>& a,
+                                                Vec3<R>& su, Vec3<R>& sw) {
+  constexpr bool kCross = (BLK & (kUT | kWF)) != 0;
+  const R Rz = hi + hj;
+  const R iR = lm_rsqrt(dx * dx + dy * dy + Rz * Rz);
+  const R u = iR * iR;
+  const R ez = Rz * iR;
+  const R ez2 = ez * ez;
+  const R iRz = lm_rcp(Rz);
+  const R hw = hj * iRz;  // h_j / (h_i + h_j)
+  const R bR = b * iR;
+  const R bR3 = bR * u;
+  // projections of the source's (j's) force and torque, and of the target's
+  const R pF = dx * Fj.x + dy * Fj.y, pFi = dx * Fi.x + dy * Fi.y;
+  const R eF = lm_fma(pF, iR, ez * Fj.z), eFi = lm_fma(pFi, iR, ez * Fi.z);  // e.F
+  const R pT = dx * Tj.x + dy * Tj.y, pTi = dx * Ti.x + dy * Ti.y;
+  const R cT = dy * Tj.x - dx * Tj.y, cTi = dy * Ti.x - dx * Ti.y;  // in-plane perpendicular
+  if (BLK & kUF) {
+    const R hq = hw * (R(1) - hw) * ez2;
+    const R F1 = bR * lm_fma(u, lm_fma(u, lm_fma(R(-2.5), ez2, R(0.5)), lm_fma(R(1.5), ez2, R(-0.5))),
+                             lm_fma(R(-1.5), hq, R(-0.75)));
+    const R F2 = bR * lm_fma(u, lm_fma(u, lm_fma(R(17.5), ez2, R(-2.5)), lm_fma(R(-7.5), ez2, R(1.5))),
+                             lm_fma(R(4.5), hq, R(-0.75)));
+    const R ezbR = ez * bR;
+    const R c30 = R(1.5) * hw * lm_fma(R(-6) * (R(1) - hw), ez2, R(1));
+    const R F3 = ezbR * lm_fma(u, lm_fma(u, lm_fma(R(-35), ez2, R(10)), lm_fma(R(15), ez2, R(-3))), c30);
+    const R F4 = ezbR * lm_fma(R(-5) * u, u, R(1.5) * hw);
+    const R F5 = bR * lm_fma(u, lm_fma(u, lm_fma(R(15), ez2, R(-2)), R(-3) * ez2), R(-3) * hw * hw * ez2);
+    const R A = lm_fma(b, fuf, F1);
+    const R A5 = A + F5;
+    const R G = b * guf;
+    // i from F_j
+    const R GdF = G * lm_fma(dz, Fj.z, pF);
+    const R Q = lm_fma(F2, eF, F3 * Fj.z);
+    const R S = lm_fma(Q, iR, GdF);
+    a.uf.x = lm_fma(S, dx, lm_fma(A, Fj.x, a.uf.x));
+    a.uf.y = lm_fma(S, dy, lm_fma(A, Fj.y, a.uf.y));
+    a.uf.z = lm_fma(F4, eF, lm_fma(Q, ez, lm_fma(GdF, dz, lm_fma(A5, Fj.z, a.uf.z))));
+    // j from F_i (f3 and f4 swap)
+    const R GdFi = G * lm_fma(dz, Fi.z, pFi);
+    const R Qs = lm_fma(F2, eFi, F4 * Fi.z);
+    const R Ss = lm_fma(Qs, iR, GdFi);
+    su.x = lm_fma(Ss, dx, lm_fma(A, Fi.x, su.x));
+    su.y = lm_fma(Ss, dy, lm_fma(A, Fi.y, su.y));
+    su.z = lm_fma(F3, eFi, lm_fma(Qs, ez, lm_fma(GdFi, dz, lm_fma(A5, Fi.z, su.z))));
+  }
+  if (BLK & kWT) {
+    const R Aw = lm_fma(b, fwt, bR3 * lm_fma(R(-3), ez2, R(0.5)));  // b (fwt + g1)
+    const R Gw = b * gwt;
+    const R G2 = R(-1.5) * bR3;
+    const R G2iR = G2 * iR;
+    const R G3 = R(3) * bR3 * ez;
+    const R G23 = R(1.5) * bR3 * ez;  // b (g2 e_z + g3)
+    const R G4 = R(3) * bR3 * u;
+    // i from T_j
+    const R eT = lm_fma(pT, iR, ez * Tj.z);
+    const R GdT = Gw * lm_fma(dz, Tj.z, pT);
+    const R Sw = lm_fma(G2iR, eT, GdT);
+    const R P = G4 * cT;
+    a.wt.x = lm_fma(P, dy, lm_fma(Sw, dx, lm_fma(Aw, Tj.x, a.wt.x)));
+    a.wt.y = lm_fma(-P, dx, lm_fma(Sw, dy, lm_fma(Aw, Tj.y, a.wt.y)));
+    a.wt.z = lm_fma(G23, eT, lm_fma(GdT, dz, lm_fma(Aw, Tj.z, a.wt.z)));
+    // j from T_i
+    const R eTi = lm_fma(pTi, iR, ez * Ti.z);
+    const R SS = lm_fma(G2, eTi, G3 * Ti.z);
+    const R GdTi = Gw * lm_fma(dz, Ti.z, pTi);
+    const R Ss = lm_fma(SS, iR, GdTi);
+    const R Ps = G4 * cTi;
+    sw.x = lm_fma(Ps, dy, lm_fma(Ss, dx, lm_fma(Aw, Ti.x, sw.x)));
+    sw.y = lm_fma(-Ps, dx, lm_fma(Ss, dy, lm_fma(Aw, Ti.y, sw.y)));
+    sw.z = lm_fma(SS, ez, lm_fma(GdTi, dz, lm_fma(Aw, Ti.z, sw.z)));
+  }
+  if (kCross) {
+    // k1 = u; k2, k3, kz with h = h_j (w: WF to i, UT to j) and h = h_i (u: UT to i, WF to j)
+    const R hu = hi * iRz;
+    const R ezu = ez * u;
+    const R t20 = lm_fma(R(-20), ez2, R(2)) * u;
+    const R k2w = u * lm_fma(R(12) * hw, ez2, t20);
+    const R k3w = ezu * lm_fma(R(10), u, R(-6) * hw);
+    const R kzw = ezu * lm_fma(R(2), u - hw, R(1));
+    const R k2u = u * lm_fma(R(12) * hu, ez2, t20);
+    const R k3u = ezu * lm_fma(R(10), u, R(-6) * hu);
+    const R kzu = ezu * lm_fma(R(2), u - hu, R(1));
+    const R bc = b * c;
+    const R Cz = bc - bR3;                   // b (c - k1/R)
+    const R Cw = lm_fma(bc, dz, -b * kzw);  // b (c d_z - kz_w)
+    const R Cu = lm_fma(bc, dz, b * kzu);   // b (c d_z + kz_u)
+    if (BLK & kWF) {
+      // i from F_j: x: F_y Cw + d_y Wv, y: -F_x Cw - d_x Wv, z: Cz (d_y F_x - d_x F_y)
+      const R K3w = bR * k3w;
+      const R KCw = lm_fma(bR, u + k2w, -bc);
+      const R Wv = lm_fma(K3w, eF, KCw * Fj.z);
+      a.wf.x = lm_fma(dy, Wv, lm_fma(Fj.y, Cw, a.wf.x));
+      a.wf.y = lm_fma(-dx, Wv, lm_fma(-Fj.x, Cw, a.wf.y));
+      a.wf.z = lm_fma(Cz, dy * Fj.x - dx * Fj.y, a.wf.z);
+      // j from F_i (units of r0): x: -F_y Cf + d_y Vs, y: F_x Cf - d_x Vs, z: -kwf Cz (d_y F_x - d_x F_y)
+      const R Cf = kwf * Cu;
+      const R gam = -lm_fma(R(2) * k3u, ez, u + k2u);
+      const R Vs = kwf * lm_fma(bR * k3u, eFi, lm_fma(bR, gam, bc) * Fi.z);
+      sw.x = lm_fma(dy, Vs, lm_fma(-Fi.y, Cf, sw.x));
+      sw.y = lm_fma(-dx, Vs, lm_fma(Fi.x, Cf, sw.y));
+      sw.z = lm_fma(-kwf * Cz, dy * Fi.x - dx * Fi.y, sw.z);
+    }
+    if (BLK & kUT) {
+      // i from T_j: x: T_y Cu + d_x m + d_y n, y: -T_x Cu + d_y m - d_x n, z: Zu (d_y T_x - d_x T_y)
+      const R m = b * u * k3u * cT;
+      const R n = -Cz * Tj.z;
+      const R Zu = bc - bR * lm_fma(k3u, ez, u + k2u);
+      a.ut.x = lm_fma(dy, n, lm_fma(dx, m, lm_fma(Tj.y, Cu, a.ut.x)));
+      a.ut.y = lm_fma(-dx, n, lm_fma(dy, m, lm_fma(-Tj.x, Cu, a.ut.y)));
+      a.ut.z = lm_fma(Zu, cT, a.ut.z);
+      // j from T_i (units of t0): x: T_y Cs + d_x ms + d_y ns, y: -T_x Cs + d_y ms - d_x ns
+      const R Cs = -kut * Cw;
+      const R ms = kut * b * u * k3w * cTi;
+      const R ns = kut * Cz * Ti.z;
+      const R Zs = kut * (bR * lm_fma(k3w, ez, u + k2w) - bc);
+      su.x = lm_fma(dy, ns, lm_fma(dx, ms, lm_fma(Ti.y, Cs, su.x)));
+      su.y = lm_fma(-dx, ns, lm_fma(dy, ms, lm_fma(-Ti.x, Cs, su.y)));
+      su.z = lm_fma(Zs, cTi, su.z);
+    }
+  }
+}
+
+// Bottom wall, self pair (r = 0, h_i = h_j): Swan & Brady B1-B3 as in
+// wall_pair; each block equals its transpose's counterpart, so j gets the same
+// forms.  With d = 0 the open part is fuf v (UF), fwt v (WT) and nothing for
+// the cross blocks.
+template <class R, int BLK>
+__host__ __device__ inline void wall_self_both(R hj, R b, R fuf, R fwt, const Vec3<R>& Fj,
+                                               const Vec3<R>& Tj, const Vec3<R>& Fi,
+                                               const Vec3<R>& Ti, R kut, R kwf, PairAcc<R>& a,
+                                               Vec3<R>& su, Vec3<R>& sw) {
+  const R ih = R(1) / hj;
+  const R ih2 = ih * ih;
+  const R ih3 = ih2 * ih;
+  if (BLK & kUF) {
+    const R ih5 = ih3 * ih2;
+    const R cp = b * (fuf + R(-0.0625) * (R(9) * ih - R(2) * ih3 + ih5));
+    const R cz = b * (fuf + R(-0.125) * (R(9) * ih - R(4) * ih3 + ih5));
+    a.uf.x += cp * Fj.x;
+    a.uf.y += cp * Fj.y;
+    a.uf.z += cz * Fj.z;
+    su.x += cp * Fi.x;
+    su.y += cp * Fi.y;
+    su.z += cz * Fi.z;
+  }
+  if (BLK & kWT) {
+    const R cp = b * (fwt + R(-0.3125) * ih3);
+    const R cz = b * (fwt + R(-0.125) * ih3);
+    a.wt.x += cp * Tj.x;
+    a.wt.y += cp * Tj.y;
+    a.wt.z += cz * Tj.z;
+    sw.x += cp * Ti.x;
+    sw.y += cp * Ti.y;
+    sw.z += cz * Ti.z;
+  }
+  const R c4 = b * R(0.125) * ih3 * ih;
+  if (BLK & kWF) {  // (-c4 v_y, c4 v_x, 0)
+    a.wf.x -= c4 * Fj.y;
+    a.wf.y += c4 * Fj.x;
+    sw.x -= kwf * c4 * Fi.y;
+    sw.y += kwf * c4 * Fi.x;
+  }
+  if (BLK & kUT) {  // (c4 v_y, -c4 v_x, 0)
+    a.ut.x += c4 * Tj.y;
+    a.ut.y -= c4 * Tj.x;
+    su.x += kut * c4 * Ti.y;
+    su.y -= kut * c4 * Ti.x;
+  }
+}
+
 template <class R, bool WALL, int BLK>
 __host__ __device__ inline void pair_both(R xi, R yi, R hi, R bi, R xj, R yj, R hj, R bj,
                                           const Vec3<R>& Fj, const Vec3<R>& Tj,
@@ -458,158 +707,16 @@ __host__ __device__ inline void pair_both(R xi, R yi, R hi, R bi, R xj, R yj, R 
   constexpr bool kCross = (BLK & (kUT | kWF)) != 0;
   const R dx = xi - xj, dy = yi - yj, dz = hi - hj;
   const R r2 = dx * dx + dy * dy + dz * dz;
-  const R bij = WALL ? bi * bj : R(1);
   R fuf, guf, fwt, gwt, c;
   rpy_coeffs<R, BLK | (kCross ? (kUT | kWF) : 0)>(r2, fuf, guf, fwt, gwt, c);
-  const Vec3<R> zero = {R(0), R(0), R(0)};
-  Vec3<R> uf = zero, ut = zero, wt = zero, wf = zero;      // image part, to i
-  Vec3<R> suf = zero, sut = zero, swt = zero, swf = zero;  // image part, to j
-  if (WALL) {
-    if (r2 != R(0)) {
-      const R Rz = hi + hj;
-      const R iR = lm_rsqrt(dx * dx + dy * dy + Rz * Rz);
-      const R iR2 = iR * iR;
-      const R iR3 = iR2 * iR;
-      const R ex = dx * iR, ey = dy * iR, ez = Rz * iR;
-      const R ez2 = ez * ez;
-      const R iRz = lm_rcp(Rz);
-      if (BLK & kUF) {
-        const R hh = hj * iRz;
-        const R iR5 = iR3 * iR2;
-        const R t1 = R(5) * ez2 - R(1);
-        const R hq = hh * (R(1) - hh) * ez2;
-        const R f1 = R(-0.25) * (R(3) * (R(1) + R(2) * hq) * iR + R(2) * (R(1) - R(3) * ez2) * iR3 +
-                                 R(2) * t1 * iR5);
-        const R f2 = R(-0.25) * (R(3) * (R(1) - R(6) * hq) * iR + R(6) * t1 * iR3 +
-                                 R(10) * (R(1) - R(7) * ez2) * iR5);
-        const R f3 = R(0.5) * ez *
-                     (R(3) * hh * (R(1) - R(6) * (R(1) - hh) * ez2) * iR + R(6) * t1 * iR3 +
-                      R(10) * (R(2) - R(7) * ez2) * iR5);
-        const R f4 = R(0.5) * ez * (R(3) * hh * iR - R(10) * iR5);
-        const R f5 = -(R(3) * hh * hh * ez2 * iR + R(3) * ez2 * iR3 + (R(2) - R(15) * ez2) * iR5);
-        const R ev = ex * Fj.x + ey * Fj.y + ez * Fj.z;
-        const R q = f2 * ev + f3 * Fj.z;
-        uf = {f1 * Fj.x + q * ex, f1 * Fj.y + q * ey, f1 * Fj.z + q * ez + f4 * ev + f5 * Fj.z};
-        const R evs = ex * Fi.x + ey * Fi.y + ez * Fi.z;
-        const R qs = f2 * evs + f4 * Fi.z;
-        suf = {f1 * Fi.x + qs * ex, f1 * Fi.y + qs * ey, f1 * Fi.z + qs * ez + f3 * evs + f5 * Fi.z};
-      }
-      if (BLK & kWT) {
-        const R g1 = R(0.5) * (R(1) - R(6) * ez2) * iR3;
-        const R g2 = R(-1.5) * iR3;
-        const R g3 = R(3) * iR3 * ez;
-        const R g4 = R(3) * iR3;
-        const R ev = ex * Tj.x + ey * Tj.y + ez * Tj.z;
-        const R p = g4 * (ey * Tj.x - ex * Tj.y);
-        wt = {g1 * Tj.x + g2 * ev * ex + p * ey, g1 * Tj.y + g2 * ev * ey - p * ex,
-              g1 * Tj.z + (g2 * ez + g3) * ev};
-        const R evs = ex * Ti.x + ey * Ti.y + ez * Ti.z;
-        const R ps = g4 * (ey * Ti.x - ex * Ti.y);
-        const R ss = g2 * evs + g3 * Ti.z;
-        swt = {g1 * Ti.x + ss * ex + ps * ey, g1 * Ti.y + ss * ey - ps * ex, g1 * Ti.z + ss * ez};
-      }
-      if (kCross) {
-        const R iR4 = iR2 * iR2;
-        const R k1 = iR2;
-        // WF factors (h = h_j) and UT factors (h = h_i)
-        const R hw = hj * iRz, hu = hi * iRz;
-        const R k2w = R(2) * (R(6) * hw * ez2 * iR2 + (R(1) - R(10) * ez2) * iR4);
-        const R k3w = R(-2) * ez * (R(3) * hw * iR2 - R(5) * iR4);
-        const R kzw = k1 * ez + R(-2) * ez * (hw * iR2 - iR4);
-        const R k2u = R(2) * (R(6) * hu * ez2 * iR2 + (R(1) - R(10) * ez2) * iR4);
-        const R k3u = R(-2) * ez * (R(3) * hu * iR2 - R(5) * iR4);
-        const R kzu = k1 * ez + R(-2) * ez * (hu * iR2 - iR4);
-        if (BLK & kWF) {
-          // to i: B F_j
-          const R ev = ex * Fj.x + ey * Fj.y + ez * Fj.z;
-          const R p = ey * Fj.x - ex * Fj.y;
-          const R k12 = k1 + k2w;
-          wf = {k3w * ey * ev - kzw * Fj.y + k12 * ey * Fj.z,
-                kzw * Fj.x - k3w * ex * ev - k12 * ex * Fj.z, -k1 * p};
-          // to j: C^T F_i (W_j from F_i is M_UT(ij)^T)
-          const R evs = ex * Fi.x + ey * Fi.y + ez * Fi.z;
-          const R gam = -(k1 + k2u + R(2) * k3u * ez);
-          const R s = k3u * evs + gam * Fi.z;
-          swf = {s * ey - kzu * Fi.y, -s * ex + kzu * Fi.x, k1 * (ey * Fi.x - ex * Fi.y)};
-        }
-        if (BLK & kUT) {
-          // to i: C T_j
-          const R p = ey * Tj.x - ex * Tj.y;
-          ut = {k3u * ex * p + kzu * Tj.y + k1 * ey * Tj.z,
-                k3u * ey * p - kzu * Tj.x - k1 * ex * Tj.z, -(k1 + k2u + k3u * ez) * p};
-          // to j: B^T T_i (U_j from T_i is M_WF(ij)^T)
-          const R ps = ey * Ti.x - ex * Ti.y;
-          const R k12 = k1 + k2w;
-          sut = {k3w * ex * ps + kzw * Ti.y - k1 * ey * Ti.z,
-                 k3w * ey * ps - kzw * Ti.x + k1 * ex * Ti.z, (k3w * ez + k12) * ps};
-        }
-      }
-    } else {
-      // Self terms (r == 0, h_i == h_j), Swan & Brady B1-B3 as in wall_pair;
-      // each block equals its transpose's counterpart, so j gets the same forms.
-      lm_rare_path();
-      const R ih = R(1) / hj;
-      const R ih2 = ih * ih;
-      const R ih3 = ih2 * ih;
-      if (BLK & kUF) {
-        const R ih5 = ih3 * ih2;
-        const R cp = R(-0.0625) * (R(9) * ih - R(2) * ih3 + ih5);
-        const R cz = R(-0.125) * (R(9) * ih - R(4) * ih3 + ih5);
-        uf = {cp * Fj.x, cp * Fj.y, cz * Fj.z};
-        suf = {cp * Fi.x, cp * Fi.y, cz * Fi.z};
-      }
-      if (BLK & kWT) {
-        wt = {R(-0.3125) * ih3 * Tj.x, R(-0.3125) * ih3 * Tj.y, R(-0.125) * ih3 * Tj.z};
-        swt = {R(-0.3125) * ih3 * Ti.x, R(-0.3125) * ih3 * Ti.y, R(-0.125) * ih3 * Ti.z};
-      }
-      const R c4 = R(0.125) * ih3 * ih;
-      if (BLK & kWF) {
-        wf = {-c4 * Fj.y, c4 * Fj.x, R(0)};
-        swf = {-c4 * Fi.y, c4 * Fi.x, R(0)};
-      }
-      if (BLK & kUT) {
-        ut = {c4 * Tj.y, -c4 * Tj.x, R(0)};
-        sut = {c4 * Ti.y, -c4 * Ti.x, R(0)};
-      }
-    }
-  }
-  if (BLK & kUF) {
-    const R s = guf * (dx * Fj.x + dy * Fj.y + dz * Fj.z);
-    a.uf.x += bij * (fuf * Fj.x + s * dx + uf.x);
-    a.uf.y += bij * (fuf * Fj.y + s * dy + uf.y);
-    a.uf.z += bij * (fuf * Fj.z + s * dz + uf.z);
-    const R ss = guf * (dx * Fi.x + dy * Fi.y + dz * Fi.z);
-    su.x += bij * (fuf * Fi.x + ss * dx + suf.x);
-    su.y += bij * (fuf * Fi.y + ss * dy + suf.y);
-    su.z += bij * (fuf * Fi.z + ss * dz + suf.z);
-  }
-  if (BLK & kWT) {
-    const R s = gwt * (dx * Tj.x + dy * Tj.y + dz * Tj.z);
-    a.wt.x += bij * (fwt * Tj.x + s * dx + wt.x);
-    a.wt.y += bij * (fwt * Tj.y + s * dy + wt.y);
-    a.wt.z += bij * (fwt * Tj.z + s * dz + wt.z);
-    const R ss = gwt * (dx * Ti.x + dy * Ti.y + dz * Ti.z);
-    sw.x += bij * (fwt * Ti.x + ss * dx + swt.x);
-    sw.y += bij * (fwt * Ti.y + ss * dy + swt.y);
-    sw.z += bij * (fwt * Ti.z + ss * dz + swt.z);
-  }
-  if (BLK & kUT) {
-    a.ut.x += bij * (c * (Tj.y * dz - Tj.z * dy) + ut.x);
-    a.ut.y += bij * (c * (Tj.z * dx - Tj.x * dz) + ut.y);
-    a.ut.z += bij * (c * (Tj.x * dy - Tj.y * dx) + ut.z);
-    const R b2 = kut * bij;
-    su.x += b2 * (sut.x - c * (Ti.y * dz - Ti.z * dy));
-    su.y += b2 * (sut.y - c * (Ti.z * dx - Ti.x * dz));
-    su.z += b2 * (sut.z - c * (Ti.x * dy - Ti.y * dx));
-  }
-  if (BLK & kWF) {
-    a.wf.x += bij * (c * (Fj.y * dz - Fj.z * dy) + wf.x);
-    a.wf.y += bij * (c * (Fj.z * dx - Fj.x * dz) + wf.y);
-    a.wf.z += bij * (c * (Fj.x * dy - Fj.y * dx) + wf.z);
-    const R b2 = kwf * bij;
-    sw.x += b2 * (swf.x - c * (Fi.y * dz - Fi.z * dy));
-    sw.y += b2 * (swf.y - c * (Fi.z * dx - Fi.x * dz));
-    sw.z += b2 * (swf.z - c * (Fi.x * dy - Fi.y * dx));
+  if (!WALL) {
+    open_both<R, BLK>(dx, dy, dz, fuf, guf, fwt, gwt, c, Fj, Tj, Fi, Ti, kut, kwf, a, su, sw);
+  } else if (r2 != R(0)) {
+    wall_image_both<R, BLK>(dx, dy, dz, hi, hj, bi * bj, fuf, guf, fwt, gwt, c, Fj, Tj, Fi, Ti,
+                            kut, kwf, a, su, sw);
+  } else {
+    lm_rare_path();
+    wall_self_both<R, BLK>(hj, bi * bj, fuf, fwt, Fj, Tj, Fi, Ti, kut, kwf, a, su, sw);
   }
 }
 
