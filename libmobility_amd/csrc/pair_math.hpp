@@ -310,160 +310,40 @@ __host__ __device__ inline void uf_coef_wall(R xi, R yi, R hi, R bi, R xj, R yj,
   }
 }
 
-// Bottom-wall contribution (open RPY + Rotne-Prager-Blake/Swan-Brady image
-// terms, damped by b_i b_j).  (xi, yi, hi, bi) and (xj, yj, hj, bj) are
-// position records as described at the top of this file.
-template <class R, int BLK>
-__host__ __device__ inline void wall_pair(R xi, R yi, R hi, R bi, R xj, R yj, R hj, R bj, R fx,
-                                          R fy, R fz, R tx, R ty, R tz, PairAcc<R>& a) {
-  const R dx = xi - xj, dy = yi - yj, dz = hi - hj;
-  const R r2 = dx * dx + dy * dy + dz * dz;
-  const R bij = bi * bj;
-  R fuf, guf, fwt, gwt, c;
-  rpy_coeffs<R, BLK>(r2, fuf, guf, fwt, gwt, c);
-  Vec3<R> uf = {R(0), R(0), R(0)}, ut = uf, wt = uf, wf = uf;
-  if (r2 != R(0)) {
-    // Image system: R = (dx, dy, hi + hj), e = R/|R|, hh = h_j/R_z, hh' = h_i/R_z.
-    const R Rz = hi + hj;
-    const R iR = lm_rsqrt(dx * dx + dy * dy + Rz * Rz);
-    const R iR2 = iR * iR;
-    const R iR3 = iR2 * iR;
-    const R ex = dx * iR, ey = dy * iR, ez = Rz * iR;
-    const R ez2 = ez * ez;
-    const R iRz = lm_rcp(Rz);
-    if (BLK & kUF) {
-      const R hh = hj * iRz;
-      const R iR5 = iR3 * iR2;
-      const R t1 = R(5) * ez2 - R(1);  // -(1 - 5 ez2)
-      const R hq = hh * (R(1) - hh) * ez2;
-      const R f1 = R(-0.25) * (R(3) * (R(1) + R(2) * hq) * iR + R(2) * (R(1) - R(3) * ez2) * iR3 +
-                               R(2) * t1 * iR5);
-      const R f2 = R(-0.25) * (R(3) * (R(1) - R(6) * hq) * iR + R(6) * t1 * iR3 +
-                               R(10) * (R(1) - R(7) * ez2) * iR5);
-      const R f3 = R(0.5) * ez *
-                   (R(3) * hh * (R(1) - R(6) * (R(1) - hh) * ez2) * iR + R(6) * t1 * iR3 +
-                    R(10) * (R(2) - R(7) * ez2) * iR5);
-      const R f4 = R(0.5) * ez * (R(3) * hh * iR - R(10) * iR5);
-      const R f5 = -(R(3) * hh * hh * ez2 * iR + R(3) * ez2 * iR3 + (R(2) - R(15) * ez2) * iR5);
-      const R ev = ex * fx + ey * fy + ez * fz;
-      const R q = f2 * ev + f3 * fz;
-      uf.x = f1 * fx + q * ex;
-      uf.y = f1 * fy + q * ey;
-      uf.z = f1 * fz + q * ez + f4 * ev + f5 * fz;
-    }
-    if (BLK & kWT) {
-      const R g1 = R(0.5) * (R(1) - R(6) * ez2) * iR3;
-      const R g2 = R(-1.5) * iR3;
-      const R g3 = R(3) * iR3 * ez;
-      const R g4 = R(3) * iR3;
-      const R ev = ex * tx + ey * ty + ez * tz;
-      const R p = g4 * (ey * tx - ex * ty);
-      wt.x = g1 * tx + g2 * ev * ex + p * ey;
-      wt.y = g1 * ty + g2 * ev * ey - p * ex;
-      wt.z = g1 * tz + (g2 * ez + g3) * ev;
-    }
-    if (BLK & (kWF | kUT)) {
-      const R iR4 = iR2 * iR2;
-      const R k1 = iR2;
-      if (BLK & kWF) {
-        const R hh = hj * iRz;
-        const R k2 = R(2) * (R(6) * hh * ez2 * iR2 + (R(1) - R(10) * ez2) * iR4);
-        const R k3 = R(-2) * ez * (R(3) * hh * iR2 - R(5) * iR4);
-        const R k4 = R(-2) * ez * (hh * iR2 - iR4);
-        const R ev = ex * fx + ey * fy + ez * fz;
-        const R p = ey * fx - ex * fy;
-        const R kz = k1 * ez + k4;
-        const R k12 = k1 + k2;
-        wf.x = k3 * ey * ev - kz * fy + k12 * ey * fz;
-        wf.y = kz * fx - k3 * ex * ev - k12 * ex * fz;
-        wf.z = -k1 * p;
-      }
-      if (BLK & kUT) {
-        const R hh = hi * iRz;
-        const R k2 = R(2) * (R(6) * hh * ez2 * iR2 + (R(1) - R(10) * ez2) * iR4);
-        const R k3 = R(-2) * ez * (R(3) * hh * iR2 - R(5) * iR4);
-        const R k4 = R(-2) * ez * (hh * iR2 - iR4);
-        const R p = ey * tx - ex * ty;
-        const R kz = k1 * ez + k4;
-        ut.x = k3 * ex * p + kz * ty + k1 * ey * tz;
-        ut.y = k3 * ey * p - kz * tx - k1 * ex * tz;
-        ut.z = -(k1 + k2 + k3 * ez) * p;
-      }
-    }
-  } else {
-    // Self terms (r == 0), Swan & Brady B1, B2, B3 as in :190-202, :256-261,
-    // :290-294, :327-331.  h_i == h_j here.
-    lm_rare_path();
-    const R ih = R(1) / hj;
-    const R ih2 = ih * ih;
-    const R ih3 = ih2 * ih;
-    if (BLK & kUF) {
-      const R ih5 = ih3 * ih2;
-      const R cp = R(-0.0625) * (R(9) * ih - R(2) * ih3 + ih5);
-      const R cz = R(-0.125) * (R(9) * ih - R(4) * ih3 + ih5);
-      uf.x = cp * fx;
-      uf.y = cp * fy;
-      uf.z = cz * fz;
-    }
-    if (BLK & kWT) {
-      wt.x = R(-0.3125) * ih3 * tx;
-      wt.y = R(-0.3125) * ih3 * ty;
-      wt.z = R(-0.125) * ih3 * tz;
-    }
-    const R c4 = R(0.125) * ih3 * ih;
-    if (BLK & kWF) {
-      wf.x = -c4 * fy;
-      wf.y = c4 * fx;
-    }
-    if (BLK & kUT) {
-      ut.x = c4 * ty;
-      ut.y = -c4 * tx;
-    }
-  }
-  // open RPY part
-  if (BLK & kUF) {
-    const R s = guf * (dx * fx + dy * fy + dz * fz);
-    a.uf.x += bij * (fuf * fx + s * dx + uf.x);
-    a.uf.y += bij * (fuf * fy + s * dy + uf.y);
-    a.uf.z += bij * (fuf * fz + s * dz + uf.z);
-  }
-  if (BLK & kWT) {
-    const R s = gwt * (dx * tx + dy * ty + dz * tz);
-    a.wt.x += bij * (fwt * tx + s * dx + wt.x);
-    a.wt.y += bij * (fwt * ty + s * dy + wt.y);
-    a.wt.z += bij * (fwt * tz + s * dz + wt.z);
-  }
-  if (BLK & kUT) {
-    a.ut.x += bij * (c * (ty * dz - tz * dy) + ut.x);
-    a.ut.y += bij * (c * (tz * dx - tx * dz) + ut.y);
-    a.ut.z += bij * (c * (tx * dy - ty * dx) + ut.z);
-  }
-  if (BLK & kWF) {
-    a.wf.x += bij * (c * (fy * dz - fz * dy) + wf.x);
-    a.wf.y += bij * (c * (fz * dx - fx * dz) + wf.y);
-    a.wf.z += bij * (c * (fx * dy - fy * dx) + wf.z);
-  }
-}
-
-// ---- one pair, both directions (symmetric evaluation) ----------------------
+// ---- bottom wall, and one pair in both directions -------------------------
+// Image system of the bottom wall (Swan & Brady, hydrodynamicKernels.cuh:187-359),
+// damped by b_i b_j: R = (d_x, d_y, h_i + h_j), e = R/|R|, u = 1/|R|^2,
+// q = (e_y, -e_x, 0), h = h_j/R_z (h_i/R_z where marked):
+//   UF:  M v = f1 v + (f2 (e.v) + f3 v_z) e + z^ (f4 (e.v) + f5 v_z), with t1 = 5 e_z^2 - 1,
+//        hq = h (1 - h) e_z^2 and
+//        f1 = -(3 (1 + 2 hq) + 2 (1 - 3 e_z^2) u + 2 t1 u^2) / (4 |R|)
+//        f2 = -(3 (1 - 6 hq) + 6 t1 u + 10 (1 - 7 e_z^2) u^2) / (4 |R|)
+//        f3 = e_z (3 h (1 - 6 (1 - h) e_z^2) + 6 t1 u + 10 (2 - 7 e_z^2) u^2) / (2 |R|)
+//        f4 = e_z (3 h - 10 u^2) / (2 |R|)
+//        f5 = -(3 h^2 e_z^2 + 3 e_z^2 u + (2 - 15 e_z^2) u^2) / |R|
+//   WT:  M v = g1 v + g2 (e.v) e + g3 (e.v) z^ + g4 (q.v) q, with
+//        g1 = (1 - 6 e_z^2) u / (2 |R|), g2 = -3 u / (2 |R|), g3 = 3 e_z u / |R|, g4 = 3 u / |R|
+//   WF:  B v = q (k3 (e.v) + (k1 + k2) v_z) + kz (z^ x v) - k1 (q.v) z^
+//   UT:  C v = k3 (q.v) e - kz (z^ x v) + k1 v_z q + gam (q.v) z^ (h = h_i),
+//        gam = -(k1 + k2 + 2 k3 e_z), with k1 = u, k2 = 2 u (6 h e_z^2 + (1 - 10 e_z^2) u),
+//        k3 = -2 e_z u (3 h - 5 u), kz = e_z u (1 - 2 h + 2 u).
+// Self pairs (r = 0, h_i = h_j) have their own forms (B1-B3, :190-202, :256-261,
+// :290-294, :327-331).
+//
 // The grand mobility matrix is symmetric: blockwise M_ji = M_ij^T, with UF and
 // WT transposing into themselves and the cross blocks swapping,
 // M_UT(ji) = M_WF(ij)^T and M_WF(ji) = M_UT(ij)^T.  So the coefficients of the
-// pair (i <- j) also give what j receives from i.  In the vector forms above,
-// with q = (e_y, -e_x, 0):
+// pair (i <- j) also give what j receives from i:
 //   UF^T v = f1 v + (f2 (e.v) + f4 v_z) e + z^ (f3 (e.v) + f5 v_z)   (f3, f4 swap)
 //   WT^T v = g1 v + (g2 (e.v) + g3 v_z) e + g4 (q.v) q
-//   WF:  B v = q (k3 (e.v) + (k1 + k2) v_z) + kz (z^ x v) - k1 (q.v) z^
-//        B^T v = (k3 e + (k1 + k2) z^) (q.v) - kz (z^ x v) - k1 v_z q
-//   UT:  C v = k3 (q.v) e - kz (z^ x v) + k1 v_z q + gam (q.v) z^,
-//        gam = -(k1 + k2 + 2 k3 e_z)  (coefficients with h = h_i)
-//        C^T v = (k3 (e.v) + gam v_z) q + kz (z^ x v) + k1 (q.v) z^
+//   B^T v = (k3 e + (k1 + k2) z^) (q.v) - kz (z^ x v) - k1 v_z q
+//   C^T v = (k3 (e.v) + gam v_z) q + kz (z^ x v) + k1 (q.v) z^
 // and the open parts: UF, WT symmetric; c (v x d) transposes to -c (v x d).
-// The target accumulates M_ij [F_j; T_j] into a (as wall_pair / open_pair);
-// the source sums are kept in units of t0 (su: UF part + kut * UT part,
-// kut = rt0/t0) and r0 (sw: WT part + kwf * WF part, kwf = rt0/r0).
-// Open boundaries: both directions of every block from the RPY coefficients
-// (UF, WT even in d; c (v x d) transposes to -c (v x d)).
+// The target accumulates M_ij [F_j; T_j] into a; the source sums are kept in
+// units of t0 (su: UF part + kut * UT part, kut = rt0/t0) and r0 (sw: WT part
+// + kwf * WF part, kwf = rt0/r0).
+
+// Open boundaries: both directions of every block from the RPY coefficients.
 template <class R, int BLK>
 __host__ __device__ inline void open_both(R dx, R dy, R dz, R fuf, R guf, R fwt, R gwt, R c,
                                           const Vec3<R>& Fj, const Vec3<R>& Tj,
@@ -511,20 +391,18 @@ __host__ __device__ inline void open_both(R dx, R dy, R dz, R fuf, R guf, R fwt,
 
 // Bottom wall, r > 0: the open part and the image terms of each block merged
 // into one set of coefficients per pair, with the damping b = b_i b_j and the
-// unit factors kut, kwf folded in, applied to both directions.  e = (d_x, d_y,
-// R_z)/R, so e's in-plane part is d's divided by R; u = 1/R^2; for a vector v
-// (F or T of the other particle) c_v = d_y v_x - d_x v_y is the in-plane
-// perpendicular component.
-//   UF:  x, y: A v + (G (d.v) + Q/R) d,  z: (A + F5) v_z + G (d.v) d_z + Q e_z + F4 (e.v)
+// unit factors kut, kwf folded in, applied to i and (kToJ) to j.  e's in-plane
+// part is d's divided by |R|; for a vector v (F or T of the other particle)
+// c_v = d_y v_x - d_x v_y = |R| (q.v).
+//   UF:  x, y: A v + (G (d.v) + Q/|R|) d,  z: (A + F5) v_z + G (d.v) d_z + Q e_z + F4 (e.v)
 //        with A = b (fuf + f1), G = b guf, Fk = b fk, Q = F2 (e.v) + F3 v_z; f3 and f4
 //        swap for j.
-//   WT:  x: Aw v_x + (Gw (d.v) + G2 (e.v)/R) d_x + G4 c_v d_y,  y: ... - G4 c_v d_x,
+//   WT:  x: Aw v_x + (Gw (d.v) + G2 (e.v)/|R|) d_x + G4 c_v d_y,  y: ... - G4 c_v d_x,
 //        z: Aw v_z + Gw (d.v) d_z + b (g2 e_z + g3) (e.v); for j, G2 (e.v) + G3 v_z
 //        takes the place of G2 (e.v) and of the z coefficient.
 //   WF, UT and their transposes: x, y from (v_y, -v_x) and (d_x, d_y), z from c_v.
-// The coefficient polynomials are those of wall_pair (Swan & Brady B1-B3), in
-// Horner form in u.
-template <class R, int BLK>
+// The polynomials in u are evaluated in Horner form.
+template <class R, int BLK, bool kToJ = true>
 __host__ __device__ inline void wall_image_both(R dx, R dy, R dz, R hi, R hj, R b, R fuf, R guf,
                                                 R fwt, R gwt, R c, const Vec3<R>& Fj,
                                                 const Vec3<R>& Tj, const Vec3<R>& Fi,
@@ -541,10 +419,10 @@ __host__ __device__ inline void wall_image_both(R dx, R dy, R dz, R hi, R hj, R 
   const R bR = b * iR;
   const R bR3 = bR * u;
   // projections of the source's (j's) force and torque, and of the target's
-  const R pF = dx * Fj.x + dy * Fj.y, pFi = dx * Fi.x + dy * Fi.y;
-  const R eF = lm_fma(pF, iR, ez * Fj.z), eFi = lm_fma(pFi, iR, ez * Fi.z);  // e.F
-  const R pT = dx * Tj.x + dy * Tj.y, pTi = dx * Ti.x + dy * Ti.y;
-  const R cT = dy * Tj.x - dx * Tj.y, cTi = dy * Ti.x - dx * Ti.y;  // in-plane perpendicular
+  const R pF = dx * Fj.x + dy * Fj.y, pFi = kToJ ? dx * Fi.x + dy * Fi.y : R(0);
+  const R eF = lm_fma(pF, iR, ez * Fj.z), eFi = kToJ ? lm_fma(pFi, iR, ez * Fi.z) : R(0);
+  const R pT = dx * Tj.x + dy * Tj.y, pTi = kToJ ? dx * Ti.x + dy * Ti.y : R(0);
+  const R cT = dy * Tj.x - dx * Tj.y, cTi = kToJ ? dy * Ti.x - dx * Ti.y : R(0);
   if (BLK & kUF) {
     const R hq = hw * (R(1) - hw) * ez2;
     const R F1 = bR * lm_fma(u, lm_fma(u, lm_fma(R(-2.5), ez2, R(0.5)), lm_fma(R(1.5), ez2, R(-0.5))),
@@ -566,13 +444,14 @@ __host__ __device__ inline void wall_image_both(R dx, R dy, R dz, R hi, R hj, R 
     a.uf.x = lm_fma(S, dx, lm_fma(A, Fj.x, a.uf.x));
     a.uf.y = lm_fma(S, dy, lm_fma(A, Fj.y, a.uf.y));
     a.uf.z = lm_fma(F4, eF, lm_fma(Q, ez, lm_fma(GdF, dz, lm_fma(A5, Fj.z, a.uf.z))));
-    // j from F_i (f3 and f4 swap)
-    const R GdFi = G * lm_fma(dz, Fi.z, pFi);
-    const R Qs = lm_fma(F2, eFi, F4 * Fi.z);
-    const R Ss = lm_fma(Qs, iR, GdFi);
-    su.x = lm_fma(Ss, dx, lm_fma(A, Fi.x, su.x));
-    su.y = lm_fma(Ss, dy, lm_fma(A, Fi.y, su.y));
-    su.z = lm_fma(F3, eFi, lm_fma(Qs, ez, lm_fma(GdFi, dz, lm_fma(A5, Fi.z, su.z))));
+    if (kToJ) {  // j from F_i (f3 and f4 swap)
+      const R GdFi = G * lm_fma(dz, Fi.z, pFi);
+      const R Qs = lm_fma(F2, eFi, F4 * Fi.z);
+      const R Ss = lm_fma(Qs, iR, GdFi);
+      su.x = lm_fma(Ss, dx, lm_fma(A, Fi.x, su.x));
+      su.y = lm_fma(Ss, dy, lm_fma(A, Fi.y, su.y));
+      su.z = lm_fma(F3, eFi, lm_fma(Qs, ez, lm_fma(GdFi, dz, lm_fma(A5, Fi.z, su.z))));
+    }
   }
   if (BLK & kWT) {
     const R Aw = lm_fma(b, fwt, bR3 * lm_fma(R(-3), ez2, R(0.5)));  // b (fwt + g1)
@@ -590,15 +469,16 @@ __host__ __device__ inline void wall_image_both(R dx, R dy, R dz, R hi, R hj, R 
     a.wt.x = lm_fma(P, dy, lm_fma(Sw, dx, lm_fma(Aw, Tj.x, a.wt.x)));
     a.wt.y = lm_fma(-P, dx, lm_fma(Sw, dy, lm_fma(Aw, Tj.y, a.wt.y)));
     a.wt.z = lm_fma(G23, eT, lm_fma(GdT, dz, lm_fma(Aw, Tj.z, a.wt.z)));
-    // j from T_i
-    const R eTi = lm_fma(pTi, iR, ez * Ti.z);
-    const R SS = lm_fma(G2, eTi, G3 * Ti.z);
-    const R GdTi = Gw * lm_fma(dz, Ti.z, pTi);
-    const R Ss = lm_fma(SS, iR, GdTi);
-    const R Ps = G4 * cTi;
-    sw.x = lm_fma(Ps, dy, lm_fma(Ss, dx, lm_fma(Aw, Ti.x, sw.x)));
-    sw.y = lm_fma(-Ps, dx, lm_fma(Ss, dy, lm_fma(Aw, Ti.y, sw.y)));
-    sw.z = lm_fma(SS, ez, lm_fma(GdTi, dz, lm_fma(Aw, Ti.z, sw.z)));
+    if (kToJ) {  // j from T_i
+      const R eTi = lm_fma(pTi, iR, ez * Ti.z);
+      const R SS = lm_fma(G2, eTi, G3 * Ti.z);
+      const R GdTi = Gw * lm_fma(dz, Ti.z, pTi);
+      const R Ss = lm_fma(SS, iR, GdTi);
+      const R Ps = G4 * cTi;
+      sw.x = lm_fma(Ps, dy, lm_fma(Ss, dx, lm_fma(Aw, Ti.x, sw.x)));
+      sw.y = lm_fma(-Ps, dx, lm_fma(Ss, dy, lm_fma(Aw, Ti.y, sw.y)));
+      sw.z = lm_fma(SS, ez, lm_fma(GdTi, dz, lm_fma(Aw, Ti.z, sw.z)));
+    }
   }
   if (kCross) {
     // k1 = u; k2, k3, kz with h = h_j (w: WF to i, UT to j) and h = h_i (u: UT to i, WF to j)
@@ -623,13 +503,16 @@ __host__ __device__ inline void wall_image_both(R dx, R dy, R dz, R hi, R hj, R 
       a.wf.x = lm_fma(dy, Wv, lm_fma(Fj.y, Cw, a.wf.x));
       a.wf.y = lm_fma(-dx, Wv, lm_fma(-Fj.x, Cw, a.wf.y));
       a.wf.z = lm_fma(Cz, dy * Fj.x - dx * Fj.y, a.wf.z);
-      // j from F_i (units of r0): x: -F_y Cf + d_y Vs, y: F_x Cf - d_x Vs, z: -kwf Cz (d_y F_x - d_x F_y)
-      const R Cf = kwf * Cu;
-      const R gam = -lm_fma(R(2) * k3u, ez, u + k2u);
-      const R Vs = kwf * lm_fma(bR * k3u, eFi, lm_fma(bR, gam, bc) * Fi.z);
-      sw.x = lm_fma(dy, Vs, lm_fma(-Fi.y, Cf, sw.x));
-      sw.y = lm_fma(-dx, Vs, lm_fma(Fi.x, Cf, sw.y));
-      sw.z = lm_fma(-kwf * Cz, dy * Fi.x - dx * Fi.y, sw.z);
+      if (kToJ) {
+        // j from F_i (units of r0): x: -F_y Cf + d_y Vs, y: F_x Cf - d_x Vs,
+        // z: -kwf Cz (d_y F_x - d_x F_y)
+        const R Cf = kwf * Cu;
+        const R gam = -lm_fma(R(2) * k3u, ez, u + k2u);
+        const R Vs = kwf * lm_fma(bR * k3u, eFi, lm_fma(bR, gam, bc) * Fi.z);
+        sw.x = lm_fma(dy, Vs, lm_fma(-Fi.y, Cf, sw.x));
+        sw.y = lm_fma(-dx, Vs, lm_fma(Fi.x, Cf, sw.y));
+        sw.z = lm_fma(-kwf * Cz, dy * Fi.x - dx * Fi.y, sw.z);
+      }
     }
     if (BLK & kUT) {
       // i from T_j: x: T_y Cu + d_x m + d_y n, y: -T_x Cu + d_y m - d_x n, z: Zu (d_y T_x - d_x T_y)
@@ -639,23 +522,24 @@ __host__ __device__ inline void wall_image_both(R dx, R dy, R dz, R hi, R hj, R 
       a.ut.x = lm_fma(dy, n, lm_fma(dx, m, lm_fma(Tj.y, Cu, a.ut.x)));
       a.ut.y = lm_fma(-dx, n, lm_fma(dy, m, lm_fma(-Tj.x, Cu, a.ut.y)));
       a.ut.z = lm_fma(Zu, cT, a.ut.z);
-      // j from T_i (units of t0): x: T_y Cs + d_x ms + d_y ns, y: -T_x Cs + d_y ms - d_x ns
-      const R Cs = -kut * Cw;
-      const R ms = kut * b * u * k3w * cTi;
-      const R ns = kut * Cz * Ti.z;
-      const R Zs = kut * (bR * lm_fma(k3w, ez, u + k2w) - bc);
-      su.x = lm_fma(dy, ns, lm_fma(dx, ms, lm_fma(Ti.y, Cs, su.x)));
-      su.y = lm_fma(-dx, ns, lm_fma(dy, ms, lm_fma(-Ti.x, Cs, su.y)));
-      su.z = lm_fma(Zs, cTi, su.z);
+      if (kToJ) {
+        // j from T_i (units of t0): x: T_y Cs + d_x ms + d_y ns, y: -T_x Cs + d_y ms - d_x ns
+        const R Cs = -kut * Cw;
+        const R ms = kut * b * u * k3w * cTi;
+        const R ns = kut * Cz * Ti.z;
+        const R Zs = kut * (bR * lm_fma(k3w, ez, u + k2w) - bc);
+        su.x = lm_fma(dy, ns, lm_fma(dx, ms, lm_fma(Ti.y, Cs, su.x)));
+        su.y = lm_fma(-dx, ns, lm_fma(dy, ms, lm_fma(-Ti.x, Cs, su.y)));
+        su.z = lm_fma(Zs, cTi, su.z);
+      }
     }
   }
 }
 
-// Bottom wall, self pair (r = 0, h_i = h_j): Swan & Brady B1-B3 as in
-// wall_pair; each block equals its transpose's counterpart, so j gets the same
-// forms.  With d = 0 the open part is fuf v (UF), fwt v (WT) and nothing for
-// the cross blocks.
-template <class R, int BLK>
+// Bottom wall, self pair (r = 0, h_i = h_j): Swan & Brady B1-B3; each block
+// equals its transpose's counterpart, so j (kToJ) gets the same forms.  With
+// d = 0 the open part is fuf v (UF), fwt v (WT) and nothing for the cross blocks.
+template <class R, int BLK, bool kToJ = true>
 __host__ __device__ inline void wall_self_both(R hj, R b, R fuf, R fwt, const Vec3<R>& Fj,
                                                const Vec3<R>& Tj, const Vec3<R>& Fi,
                                                const Vec3<R>& Ti, R kut, R kwf, PairAcc<R>& a,
@@ -670,9 +554,11 @@ __host__ __device__ inline void wall_self_both(R hj, R b, R fuf, R fwt, const Ve
     a.uf.x += cp * Fj.x;
     a.uf.y += cp * Fj.y;
     a.uf.z += cz * Fj.z;
-    su.x += cp * Fi.x;
-    su.y += cp * Fi.y;
-    su.z += cz * Fi.z;
+    if (kToJ) {
+      su.x += cp * Fi.x;
+      su.y += cp * Fi.y;
+      su.z += cz * Fi.z;
+    }
   }
   if (BLK & kWT) {
     const R cp = b * (fwt + R(-0.3125) * ih3);
@@ -680,22 +566,50 @@ __host__ __device__ inline void wall_self_both(R hj, R b, R fuf, R fwt, const Ve
     a.wt.x += cp * Tj.x;
     a.wt.y += cp * Tj.y;
     a.wt.z += cz * Tj.z;
-    sw.x += cp * Ti.x;
-    sw.y += cp * Ti.y;
-    sw.z += cz * Ti.z;
+    if (kToJ) {
+      sw.x += cp * Ti.x;
+      sw.y += cp * Ti.y;
+      sw.z += cz * Ti.z;
+    }
   }
   const R c4 = b * R(0.125) * ih3 * ih;
   if (BLK & kWF) {  // (-c4 v_y, c4 v_x, 0)
     a.wf.x -= c4 * Fj.y;
     a.wf.y += c4 * Fj.x;
-    sw.x -= kwf * c4 * Fi.y;
-    sw.y += kwf * c4 * Fi.x;
+    if (kToJ) {
+      sw.x -= kwf * c4 * Fi.y;
+      sw.y += kwf * c4 * Fi.x;
+    }
   }
   if (BLK & kUT) {  // (c4 v_y, -c4 v_x, 0)
     a.ut.x += c4 * Tj.y;
     a.ut.y -= c4 * Tj.x;
-    su.x += kut * c4 * Ti.y;
-    su.y -= kut * c4 * Ti.x;
+    if (kToJ) {
+      su.x += kut * c4 * Ti.y;
+      su.y -= kut * c4 * Ti.x;
+    }
+  }
+}
+
+// Bottom-wall contribution of source j to target i (open RPY + image terms,
+// damped by b_i b_j).  (xi, yi, hi, bi) and (xj, yj, hj, bj) are position
+// records as described at the top of this file.
+template <class R, int BLK>
+__host__ __device__ inline void wall_pair(R xi, R yi, R hi, R bi, R xj, R yj, R hj, R bj, R fx,
+                                          R fy, R fz, R tx, R ty, R tz, PairAcc<R>& a) {
+  const R dx = xi - xj, dy = yi - yj, dz = hi - hj;
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  R fuf, guf, fwt, gwt, c;
+  rpy_coeffs<R, BLK>(r2, fuf, guf, fwt, gwt, c);
+  const Vec3<R> Fj = {fx, fy, fz}, Tj = {tx, ty, tz}, none = {R(0), R(0), R(0)};
+  Vec3<R> su = none, sw = none;  // not written (target only)
+  if (r2 != R(0)) {
+    wall_image_both<R, BLK, false>(dx, dy, dz, hi, hj, bi * bj, fuf, guf, fwt, gwt, c, Fj, Tj,
+                                   none, none, R(0), R(0), a, su, sw);
+  } else {
+    lm_rare_path();
+    wall_self_both<R, BLK, false>(hj, bi * bj, fuf, fwt, Fj, Tj, none, none, R(0), R(0), a, su,
+                                  sw);
   }
 }
 
