@@ -226,6 +226,7 @@ C1_TILING = (1, 16)
 MEASURED_FP32_FLOPS = {
     ("c3", 131072, "symmetric", 4, 64): 7.134e9 * 64,
     ("c3", 131072, "symmetric", 4, 32): 7.00029e9 * 64,
+    ("c2", 65536, "symmetric", 1, 64): 1.25718e10 * 64,
 }
 
 MEASURED_TRAFFIC_BYTES = {
@@ -240,6 +241,17 @@ def compute_roofline(flops_per_launch, ms, peak_tflops, traffic=None):
     achieved = flops_per_launch / (ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": peak_tflops, "unit": "TFLOP/s",
             "frac": achieved / peak_tflops, "traffic": traffic}
+
+
+def counted_roofline(key, flops_per_launch, ms):
+    """FP32 roofline of a pair product, with the HBM traffic and the FP32 flops
+    the hardware issues (SQ counters) where they were measured for `key`."""
+    roof = compute_roofline(flops_per_launch, ms, PEAK_FP32_TFLOPS, MEASURED_TRAFFIC_BYTES.get(key))
+    issued = MEASURED_FP32_FLOPS.get(key)
+    if issued is not None:
+        roof["hw_achieved"] = issued / (ms * 1e-3) / 1e12
+        roof["hw_frac"] = roof["hw_achieved"] / PEAK_FP32_TFLOPS
+    return roof
 
 
 # ---------------------------------------------------------------------------
@@ -540,12 +552,7 @@ def bench_c3(args):
     }
     t32 = extra["tiling_fp32pair"]
     key = ("c3", N, extra["pair_evaluation_fp32pair"], t32["targets_per_lane"], t32["source_chunks"])
-    roof = compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS, MEASURED_TRAFFIC_BYTES.get(key))
-    issued = MEASURED_FP32_FLOPS.get(key)
-    if issued is not None:
-        # what the hardware issues: FP32 VALU flops counted by the SQ counters
-        roof["hw_achieved"] = issued / (ms32 * 1e-3) / 1e12
-        roof["hw_frac"] = roof["hw_achieved"] / PEAK_FP32_TFLOPS
+    roof = counted_roofline(key, pairs * fpp, ms32)
     res = result_line(METRIC, gpair32, "Gpair/s", args.steps, args.warmup, ms32, True, "f32",
                       config, roof, extra, None)
     if not args.no_cpu:
@@ -760,9 +767,11 @@ def bench_c2(args):
     config = {"workload": f"single-wall RPB/Swan-Brady, N={N}, x,y uniform in [0,256a), z uniform "
                           f"in [1.5a,20a], a=1, eta=1/(6 pi), F,T~N(0,1); 6x6 Mdot (UF,UT,WF,WT), "
                           f"FP32 pair terms with FP64 per-tile accumulation", "N": N}
+    t32 = out["fp32"]["tiling"]
+    key = ("c2", N, out["fp32"]["evaluation"], t32["targets_per_lane"], t32["source_chunks"])
     res = result_line("Wall 6x6 Mdot Gpair/s at N=65536 (FP32 pairs)", pairs / (ms32 * 1e-3) / 1e9,
                       "Gpair/s", args.steps, args.warmup, ms32, True, "f32", config,
-                      compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS), extra, None)
+                      counted_roofline(key, pairs * fpp, ms32), extra, None)
     if not args.no_cpu:
         rows = np.sort(np.random.default_rng(3).choice(N, size=min(N, args.oracle_rows),
                                                       replace=False))
@@ -825,10 +834,12 @@ def bench_c4(args):
     g.ffi.check(g.lib.lm_nbody_set_pair_symmetric(ho, 0))
     ms_o = g.time_events(lambda: g.mdot(ho, F_d, None, U_d), args.steps, args.warmup)
     Uo = U_d.cpu().numpy()
+    t_o = g.tiling(ho)
     g.destroy(ho)
     log(f"  {ms_o:.3f} ms with the ordered evaluation (persistent grid)")
+    # the same tiling as the persistent grid, so that the two sum in the same order
     hn = g.solver(pos_d, "fp32", wall=True, tol=1e-6, sizes=sizes, kinds=kinds, persistent=0,
-                  tiling=args.tiling)
+                  tiling=(t_o["targets_per_lane"], t_o["source_chunks"]))
     g.ffi.check(g.lib.lm_nbody_set_pair_symmetric(hn, 0))
     ms_n = g.time_events(lambda: g.mdot(hn, F_d, None, U_d), args.steps, args.warmup)
     Un = U_d.cpu().numpy()
@@ -861,6 +872,7 @@ def bench_c4(args):
         "mdot_rel_diff_ensemble_vs_ordered": rel(Ue, Uo),
         "mdot_ordered_persistent_equals_plain_grid_bitwise": bool(np.array_equal(Uo, Un)),
         "tiling_ensemble": g.tiling(he),
+        "tiling_ordered": t_o,
         "flops_per_launch": flops,
     }
     if not args.no_lanczos:
