@@ -1072,6 +1072,55 @@ def bench_c5(args):
                        compute_roofline(pairs * fpp, ms32, PEAK_FP32_TFLOPS), extra, None)
 
 
+def bench_c5_plain_fp64(args):
+    """The C5 step on the plain FP64 path: FP64 pair terms in every product
+    (Mdot, the plain Lanczos square root and the RFD drift)."""
+    g = Gpu()
+    N = args.N or 1048576
+    pos, F, T, Lxy = workloads.config_c5(N)
+    pos_d, F_d, T_d = g.put(pos), g.put(F), g.put(T)
+    U_d, W_d = g.zeros(3 * N), g.zeros(3 * N)
+    del pos, F, T
+    fpp = flops_per_pair()["wall_6x6"]
+    pairs = float(N) ** 2
+    tiling = args.tiling or (2, 0)
+    t_start = time.perf_counter()
+
+    def progress(i, err):
+        log(f"  Lanczos iteration {i}: error estimate {err:.3e}, "
+            f"{time.perf_counter() - t_start:.0f} s")
+
+    h64 = g.solver(pos_d, "fp64", wall=True, angular=True, delta=1e-4, tiling=tiling,
+                   iteration_limit=1024, callback=progress)
+    log("FP64-pair Mdot")
+    ms64 = g.time_events(lambda: g.mdot(h64, F_d, T_d, U_d, W_d), 1, 0)
+    log(f"  {ms64:.1f} ms ({g.evaluation(h64)} evaluation)")
+    log("hydrodynamicVelocities (Mdot + sqrtMdotW tol 1e-6 + RFD drift), FP64 pairs, plain Lanczos")
+    g.sync()
+    t = t_start = time.perf_counter()
+    g.ffi.check(g.lib.lm_nbody_hydrodynamic_velocities(
+        h64, F_d.data_ptr(), 3 * N, T_d.data_ptr(), 3 * N, U_d.data_ptr(), W_d.data_ptr(),
+        g.ffi.LM_MEM_DEVICE))
+    g.sync()
+    step_ms = (time.perf_counter() - t) * 1e3
+    it = g.lib.lm_nbody_last_lanczos_iterations(h64)
+    log(f"  {step_ms / 1e3:.1f} s, Lanczos iterations {it}")
+    u = U_d.cpu().numpy()
+    g.destroy(h64)
+    extra = {"mdot_fp64pair_ms": ms64, "flops_per_pair": fpp,
+             "hydrodynamicVelocities_fp64pair_plain_ms": step_ms, "lanczos_iterations": it,
+             "output_finite": bool(np.all(np.isfinite(u))),
+             "tiling": {"targets_per_lane": tiling[0], "source_chunks": tiling[1]}}
+    config = {"workload": f"single-wall RPB/Swan-Brady, N={N}, x,y uniform in [0,{Lxy:.0f}a), z "
+                          f"uniform in [1.5a,30a], a=1, eta=1/(6 pi), F,T~N(0,1); 6x6 operator; "
+                          f"hydrodynamicVelocities = Mdot + sqrtMdotW (prefactor 1, tol 1e-6) + "
+                          f"RFD drift (delta 1e-4 a); FP64 pairs everywhere, plain Lanczos",
+              "N": N}
+    return result_line("hydrodynamicVelocities ms at N=1048576 single wall (FP64 pairs, plain "
+                       "Lanczos)", step_ms, "ms", 1, 1, step_ms, False, "f64", config,
+                       compute_roofline(pairs * fpp, ms64, PEAK_FP64_TFLOPS), extra, None)
+
+
 def copy_calibration():
     import torch
     n = 1 << 28  # doubles: 2 GiB read + 2 GiB written per copy
@@ -1106,6 +1155,8 @@ def main():
                     help="only run a 2 GiB device-to-device copy (reference for the HBM byte counters)")
     ap.add_argument("--c5-plain", action="store_true",
                     help="c5: the step with plain Lanczos instead of the two-level preconditioner")
+    ap.add_argument("--c5-plain-fp64", action="store_true",
+                    help="c5: the step on the plain FP64 path (FP64 pairs everywhere, plain Lanczos)")
     ap.add_argument("--coarse-ppc", type=int, default=COARSE_PPC,
                     help="particles per cell of the preconditioner's coarse level")
     args = ap.parse_args()
@@ -1118,8 +1169,11 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
-    res = {"c3": bench_c3, "c1": bench_c1, "c2": bench_c2, "c4": bench_c4,
-           "c5": bench_c5}[args.workload](args)
+    if args.workload == "c5" and args.c5_plain_fp64:
+        res = bench_c5_plain_fp64(args)
+    else:
+        res = {"c3": bench_c3, "c1": bench_c1, "c2": bench_c2, "c4": bench_c4,
+               "c5": bench_c5}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 
