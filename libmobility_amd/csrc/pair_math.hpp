@@ -199,15 +199,18 @@ __host__ __device__ inline void open_accumulate(R dx, R dy, R dz, R fx, R fy, R 
     a.wt.y = lm_fma(s, dy, lm_fma(fwt, ty, a.wt.y));
     a.wt.z = lm_fma(s, dz, lm_fma(fwt, tz, a.wt.z));
   }
-  if (BLK & kUT) {  // c (T x d)
-    a.ut.x = lm_fma(c, ty * dz - tz * dy, a.ut.x);
-    a.ut.y = lm_fma(c, tz * dx - tx * dz, a.ut.y);
-    a.ut.z = lm_fma(c, tx * dy - ty * dx, a.ut.z);
-  }
-  if (BLK & kWF) {  // c (F x d)
-    a.wf.x = lm_fma(c, fy * dz - fz * dy, a.wf.x);
-    a.wf.y = lm_fma(c, fz * dx - fx * dz, a.wf.y);
-    a.wf.z = lm_fma(c, fx * dy - fy * dx, a.wf.z);
+  if (BLK & (kUT | kWF)) {  // v x (c d)
+    const R cdx = c * dx, cdy = c * dy, cdz = c * dz;
+    if (BLK & kUT) {  // c (T x d)
+      a.ut.x = lm_fma(ty, cdz, lm_fma(-tz, cdy, a.ut.x));
+      a.ut.y = lm_fma(tz, cdx, lm_fma(-tx, cdz, a.ut.y));
+      a.ut.z = lm_fma(tx, cdy, lm_fma(-ty, cdx, a.ut.z));
+    }
+    if (BLK & kWF) {  // c (F x d)
+      a.wf.x = lm_fma(fy, cdz, lm_fma(-fz, cdy, a.wf.x));
+      a.wf.y = lm_fma(fz, cdx, lm_fma(-fx, cdz, a.wf.y));
+      a.wf.z = lm_fma(fx, cdy, lm_fma(-fy, cdx, a.wf.z));
+    }
   }
 }
 
@@ -369,23 +372,26 @@ __host__ __device__ inline void open_both(R dx, R dy, R dz, R fuf, R guf, R fwt,
     sw.y = lm_fma(ss, dy, lm_fma(fwt, Ti.y, sw.y));
     sw.z = lm_fma(ss, dz, lm_fma(fwt, Ti.z, sw.z));
   }
+  // cross blocks: v x (c d), with the source side's factors -kut, -kwf on
+  // the target's own torque and force (fixed over the source loop)
+  const R cdx = c * dx, cdy = c * dy, cdz = c * dz;
   if (BLK & kUT) {  // c (T_j x d) to i; -c (T_i x d) to j, in units of t0
-    a.ut.x = lm_fma(c, Tj.y * dz - Tj.z * dy, a.ut.x);
-    a.ut.y = lm_fma(c, Tj.z * dx - Tj.x * dz, a.ut.y);
-    a.ut.z = lm_fma(c, Tj.x * dy - Tj.y * dx, a.ut.z);
-    const R k = -kut * c;
-    su.x = lm_fma(k, Ti.y * dz - Ti.z * dy, su.x);
-    su.y = lm_fma(k, Ti.z * dx - Ti.x * dz, su.y);
-    su.z = lm_fma(k, Ti.x * dy - Ti.y * dx, su.z);
+    a.ut.x = lm_fma(Tj.y, cdz, lm_fma(-Tj.z, cdy, a.ut.x));
+    a.ut.y = lm_fma(Tj.z, cdx, lm_fma(-Tj.x, cdz, a.ut.y));
+    a.ut.z = lm_fma(Tj.x, cdy, lm_fma(-Tj.y, cdx, a.ut.z));
+    const R kx = -kut * Ti.x, ky = -kut * Ti.y, kz = -kut * Ti.z;
+    su.x = lm_fma(ky, cdz, lm_fma(-kz, cdy, su.x));
+    su.y = lm_fma(kz, cdx, lm_fma(-kx, cdz, su.y));
+    su.z = lm_fma(kx, cdy, lm_fma(-ky, cdx, su.z));
   }
   if (BLK & kWF) {  // c (F_j x d) to i; -c (F_i x d) to j, in units of r0
-    a.wf.x = lm_fma(c, Fj.y * dz - Fj.z * dy, a.wf.x);
-    a.wf.y = lm_fma(c, Fj.z * dx - Fj.x * dz, a.wf.y);
-    a.wf.z = lm_fma(c, Fj.x * dy - Fj.y * dx, a.wf.z);
-    const R k = -kwf * c;
-    sw.x = lm_fma(k, Fi.y * dz - Fi.z * dy, sw.x);
-    sw.y = lm_fma(k, Fi.z * dx - Fi.x * dz, sw.y);
-    sw.z = lm_fma(k, Fi.x * dy - Fi.y * dx, sw.z);
+    a.wf.x = lm_fma(Fj.y, cdz, lm_fma(-Fj.z, cdy, a.wf.x));
+    a.wf.y = lm_fma(Fj.z, cdx, lm_fma(-Fj.x, cdz, a.wf.y));
+    a.wf.z = lm_fma(Fj.x, cdy, lm_fma(-Fj.y, cdx, a.wf.z));
+    const R kx = -kwf * Fi.x, ky = -kwf * Fi.y, kz = -kwf * Fi.z;
+    sw.x = lm_fma(ky, cdz, lm_fma(-kz, cdy, sw.x));
+    sw.y = lm_fma(kz, cdx, lm_fma(-kx, cdz, sw.y));
+    sw.z = lm_fma(kx, cdy, lm_fma(-ky, cdx, sw.z));
   }
 }
 
