@@ -253,6 +253,30 @@ __host__ __device__ inline void uf_apply_wall(const UFCoef<R>& k, R fx, R fy, R 
   u.z = lm_fma(fz, k.hz, lm_fma(ev, k.gz, lm_fma(s, k.dz, lm_fma(k.a0, fz, u.z))));
 }
 
+// The bottom wall's UF image coefficients f1..f5 (their forms are stated in
+// the comment on the bottom wall further down), each times the damping b, in
+// Horner form in u = 1/|R|^2; bR = b/|R|, hw = h_j/R_z, ez = R_z/|R|.
+template <class R>
+struct WallUF {
+  R F1, F2, F3, F4, F5;
+};
+
+template <class R>
+__host__ __device__ inline WallUF<R> wall_uf_coefs(R hw, R ez, R ez2, R u, R bR) {
+  WallUF<R> w;
+  const R hq = hw * (R(1) - hw) * ez2;
+  w.F1 = bR * lm_fma(u, lm_fma(u, lm_fma(R(-2.5), ez2, R(0.5)), lm_fma(R(1.5), ez2, R(-0.5))),
+                     lm_fma(R(-1.5), hq, R(-0.75)));
+  w.F2 = bR * lm_fma(u, lm_fma(u, lm_fma(R(17.5), ez2, R(-2.5)), lm_fma(R(-7.5), ez2, R(1.5))),
+                     lm_fma(R(4.5), hq, R(-0.75)));
+  const R ezbR = ez * bR;
+  const R c30 = R(1.5) * hw * lm_fma(R(-6) * (R(1) - hw), ez2, R(1));
+  w.F3 = ezbR * lm_fma(u, lm_fma(u, lm_fma(R(-35), ez2, R(10)), lm_fma(R(15), ez2, R(-3))), c30);
+  w.F4 = ezbR * lm_fma(R(-5) * u, u, R(1.5) * hw);
+  w.F5 = bR * lm_fma(u, lm_fma(u, lm_fma(R(15), ez2, R(-2)), R(-3) * ez2), R(-3) * hw * hw * ez2);
+  return w;
+}
+
 // Wall UF coefficients of target record (xi, yi, hi, bi) and source (xj, yj, hj, bj).
 template <class R>
 __host__ __device__ inline void uf_coef_wall(R xi, R yi, R hi, R bi, R xj, R yj, R hj, R bj,
@@ -269,33 +293,18 @@ __host__ __device__ inline void uf_coef_wall(R xi, R yi, R hi, R bi, R xj, R yj,
   if (r2 != R(0)) {
     const R Rz = hi + hj;
     const R iR = lm_rsqrt(dx * dx + dy * dy + Rz * Rz);
-    const R iR2 = iR * iR;
-    const R iR3 = iR2 * iR;
-    const R iR5 = iR3 * iR2;
     const R ex = dx * iR, ey = dy * iR, ez = Rz * iR;
-    const R ez2 = ez * ez;
-    const R hh = hj * lm_rcp(Rz);
-    const R t1 = R(5) * ez2 - R(1);
-    const R hq = hh * (R(1) - hh) * ez2;
-    const R f1 = R(-0.25) * (R(3) * (R(1) + R(2) * hq) * iR + R(2) * (R(1) - R(3) * ez2) * iR3 +
-                             R(2) * t1 * iR5);
-    const R f2 = R(-0.25) * (R(3) * (R(1) - R(6) * hq) * iR + R(6) * t1 * iR3 +
-                             R(10) * (R(1) - R(7) * ez2) * iR5);
-    const R f3 = R(0.5) * ez *
-                 (R(3) * hh * (R(1) - R(6) * (R(1) - hh) * ez2) * iR + R(6) * t1 * iR3 +
-                  R(10) * (R(2) - R(7) * ez2) * iR5);
-    const R f4 = R(0.5) * ez * (R(3) * hh * iR - R(10) * iR5);
-    const R f5 = -(R(3) * hh * hh * ez2 * iR + R(3) * ez2 * iR3 + (R(2) - R(15) * ez2) * iR5);
-    k.a0 = b * (fuf + f1);
+    const WallUF<R> f = wall_uf_coefs<R>(hj * lm_rcp(Rz), ez, ez * ez, iR * iR, b * iR);
+    k.a0 = lm_fma(b, fuf, f.F1);
     k.ex = ex;
     k.ey = ey;
     k.ez = ez;
-    k.gx = b * f2 * ex;
-    k.gy = b * f2 * ey;
-    k.gz = b * (f2 * ez + f4);
-    k.hx = b * f3 * ex;
-    k.hy = b * f3 * ey;
-    k.hz = b * (f3 * ez + f5);
+    k.gx = f.F2 * ex;
+    k.gy = f.F2 * ey;
+    k.gz = lm_fma(f.F2, ez, f.F4);
+    k.hx = f.F3 * ex;
+    k.hy = f.F3 * ey;
+    k.hz = lm_fma(f.F3, ez, f.F5);
   } else {
     // self (Swan & Brady B1), as in wall_pair
     lm_rare_path();
@@ -430,17 +439,9 @@ __host__ __device__ inline void wall_image_both(R dx, R dy, R dz, R hi, R hj, R 
   const R pT = dx * Tj.x + dy * Tj.y, pTi = kToJ ? dx * Ti.x + dy * Ti.y : R(0);
   const R cT = dy * Tj.x - dx * Tj.y, cTi = kToJ ? dy * Ti.x - dx * Ti.y : R(0);
   if (BLK & kUF) {
-    const R hq = hw * (R(1) - hw) * ez2;
-    const R F1 = bR * lm_fma(u, lm_fma(u, lm_fma(R(-2.5), ez2, R(0.5)), lm_fma(R(1.5), ez2, R(-0.5))),
-                             lm_fma(R(-1.5), hq, R(-0.75)));
-    const R F2 = bR * lm_fma(u, lm_fma(u, lm_fma(R(17.5), ez2, R(-2.5)), lm_fma(R(-7.5), ez2, R(1.5))),
-                             lm_fma(R(4.5), hq, R(-0.75)));
-    const R ezbR = ez * bR;
-    const R c30 = R(1.5) * hw * lm_fma(R(-6) * (R(1) - hw), ez2, R(1));
-    const R F3 = ezbR * lm_fma(u, lm_fma(u, lm_fma(R(-35), ez2, R(10)), lm_fma(R(15), ez2, R(-3))), c30);
-    const R F4 = ezbR * lm_fma(R(-5) * u, u, R(1.5) * hw);
-    const R F5 = bR * lm_fma(u, lm_fma(u, lm_fma(R(15), ez2, R(-2)), R(-3) * ez2), R(-3) * hw * hw * ez2);
-    const R A = lm_fma(b, fuf, F1);
+    const WallUF<R> f = wall_uf_coefs<R>(hw, ez, ez2, u, bR);
+    const R F2 = f.F2, F3 = f.F3, F4 = f.F4, F5 = f.F5;
+    const R A = lm_fma(b, fuf, f.F1);
     const R A5 = A + F5;
     const R G = b * guf;
     // i from F_j
