@@ -100,7 +100,7 @@ int lm_nbody_set_pair_tiling(lm_nbody* h, int targets_per_lane, int source_chunk
  * mobility matrix is symmetric (M_ji = M_ij^T blockwise), so each unordered
  * pair is evaluated once and applied to both particles; the reference
  * evaluates every ordered pair (NbodyRPY.cu:27-97, hydrodynamicKernels.cuh).
- * Auto uses it from N = 32768.  Results agree with the ordered evaluation to
+ * Auto uses it from N = 16384.  Results agree with the ordered evaluation to
  * rounding and are bitwise reproducible. */
 int lm_nbody_set_pair_symmetric(lm_nbody* h, int mode);
 

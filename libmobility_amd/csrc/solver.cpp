@@ -369,7 +369,7 @@ bool NBodySolver::use_sym(int blocks) const {
   if (blocks != kUF && blocks != (kUF | kWF) && blocks != (kUT | kWT) &&
       blocks != (kUF | kUT | kWT | kWF))
     return false;
-  return sym_mode_ == 1 || N_ >= 32768;
+  return sym_mode_ == 1 || N_ >= kSymAutoN;
 }
 
 // Items: per pass, for every system and tile, the system's chunks of the
