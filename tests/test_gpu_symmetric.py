@@ -126,6 +126,12 @@ def test_auto_mode_picks_the_symmetric_path_for_large_open_systems():
     small.setPositions(pos[:1000])
     small.Mdot(F[:1000])
     assert not small.lastPairSymmetric()
+    # the automatic rule starts at 16384 particles
+    for n, expect in ((16383, False), (16384, True)):
+        s = _solver("fp32", -1)
+        s.setPositions(pos[:n])
+        s.Mdot(F[:n])
+        assert s.lastPairSymmetric() == expect, n
 
 
 def _wall_solver(precision="fp64", sym=1, tpl=0, nch=0, angular=False, wall=True, memory=0):
