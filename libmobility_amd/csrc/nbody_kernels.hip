@@ -44,17 +44,22 @@ __device__ inline R4 far_record() {
   return r;
 }
 
-// Overlap test of a group of pairs: is any r^2 below 4 (radius units)?  In
-// FP32 it keeps the smallest r^2 by its bit pattern, which orders like the
-// value for r^2 >= 0 (NaN sorts above 4 and stays on the far formula), so a
-// group costs one unsigned min per pair (the compiler forms v_min3_u32) and a
-// single compare instead of a compare per pair.
+// Overlap test of a group of pairs: is any r^2 below 4 (radius units)?
+// NearTestCmp compares every r^2 (the mask lives in scalar registers).
+// NearTest in FP32 keeps the smallest r^2 by its bit pattern, which orders
+// like the value for r^2 >= 0 (NaN sorts above 4 and stays on the far
+// formula), so a group costs one unsigned min per pair (the compiler forms
+// v_min3_u32) and a single compare; it takes a vector register, which the
+// ordered kernel's 4-target tiling cannot spare without losing a wave per
+// SIMD, so that kernel keeps the compares.
 template <class R>
-struct NearTest {
+struct NearTestCmp {
   bool any = false;
   __device__ void add(R r2) { any |= r2 < R(4); }
   __device__ bool hit() const { return any; }
 };
+template <class R>
+struct NearTest : NearTestCmp<R> {};
 template <>
 struct NearTest<float> {
   uint32_t m = 0xffffffffu;
@@ -131,7 +136,7 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
         R dx[SG][TPL], dy[SG][TPL], dz[SG][TPL], r2[SG][TPL], ir[SG][TPL];
         RpyCoef<R> cf[SG][TPL];
         R4 fj[SG], tj[SG];
-        NearTest<R> near;
+        NearTestCmp<R> near;
 #pragma unroll
         for (int g = 0; g < SG; ++g) {
           const R4 pj = s_pos[s + g];
