@@ -176,6 +176,30 @@ def test_mdot_angular_vs_oracle(precision, tol, wall, which):
         assert err <= tol, err
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+@pytest.mark.parametrize("sym", [0, 1])
+def test_mdot_wall_far_above_and_spread_out(precision, tol, sym):
+    """Heights up to 300a and separations up to 400a: the image terms are then
+    small against the open part (f1..f5 of the merged wall coefficients in
+    their Horner form), in the ordered and the symmetric evaluation."""
+    rng = np.random.default_rng(17)
+    N = 600
+    a = 1.0
+    pos = rng.uniform(0, 400.0, (N, 3))
+    pos[:, 2] = np.exp(rng.uniform(np.log(0.5), np.log(300.0), N))
+    F = rng.normal(size=(N, 3))
+    T = rng.normal(size=(N, 3))
+    s = _solver(True, a, 0.7, angular=True, precision=precision)
+    s.setPairSymmetric(sym)
+    s.setPositions(pos)
+    U, W = s.Mdot(F, T)
+    assert s.lastPairSymmetric() == bool(sym)
+    Uo, Wo = oracle.mdot(pos, 0.7, a, F=F, T=T, angular=True, wall=True)
+    for got, ref in ((U, Uo), (W, Wo)):
+        err = np.linalg.norm(got.ravel() - ref) / np.linalg.norm(ref)
+        assert err <= tol, err
+
+
 def test_batched_equals_per_system():
     rng = np.random.default_rng(5)
     nb, npb = 6, 97
