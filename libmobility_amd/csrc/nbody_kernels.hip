@@ -70,6 +70,27 @@ struct NearTest<float> {
   __device__ bool hit() const { return m < 0x40800000u; }  // the bits of 4.0f
 };
 
+// Open-boundary UF coefficients times 4/3, so that the far field costs one
+// operation less: fuf = 1/r + (2/3)/r^3, guf = 1/r^3 - 2/r^5 (4/3 of
+// rpy_far's); the caller scales its sums by 3/4.
+template <class R>
+__device__ inline void rpy_far_uf34(R ir, RpyCoef<R>& k) {
+  const R ir2 = ir * ir;
+  const R ir3 = ir * ir2;
+  k.fuf = lm_fma(R(2.0 / 3.0), ir3, ir);
+  k.guf = lm_fma(R(-2) * ir2, ir3, ir3);
+}
+
+// rpy_near_select for the 4/3-scaled coefficients
+template <class R>
+__device__ inline void rpy_near_select_uf34(R r2, R ir, RpyCoef<R>& k) {
+  RpyCoef<R> n = k;
+  rpy_near<R, kUF>(r2, ir, n);
+  const bool is_near = r2 < R(4);
+  k.fuf = is_near ? n.fuf * R(4.0 / 3.0) : k.fuf;
+  k.guf = is_near ? n.guf * R(4.0 / 3.0) : k.guf;
+}
+
 // One item: targets [tgt0, tgt0 + ntgt) x sources [src0, src1) of one system.
 template <class R, bool WALL, int BLK, int TPL>
 __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* __restrict__ pos,
@@ -81,6 +102,9 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
   constexpr bool kNeedF = (BLK & (kUF | kWF)) != 0;
   constexpr bool kNeedT = (BLK & (kUT | kWT)) != 0;
   constexpr bool kAngular = (BLK & (kWT | kWF)) != 0;
+  // open translational block: coefficients times 4/3 (one operation less per
+  // pair), undone once on the output
+  constexpr bool kUF34 = !WALL && BLK == kUF;
   const int tid = threadIdx.x;
 
   R4 pi[TPL];
@@ -150,7 +174,10 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
             r2[g][k] = dx[g][k] * dx[g][k] + dy[g][k] * dy[g][k] + dz[g][k] * dz[g][k];
             ir[g][k] = lm_rsqrt(r2[g][k]);
             cf[g][k] = RpyCoef<R>{};
-            rpy_far<R, BLK>(ir[g][k], cf[g][k]);
+            if constexpr (kUF34)
+              rpy_far_uf34<R>(ir[g][k], cf[g][k]);
+            else
+              rpy_far<R, BLK>(ir[g][k], cf[g][k]);
             near.add(r2[g][k]);
           }
         }
@@ -160,7 +187,12 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
 #pragma unroll
           for (int g = 0; g < SG; ++g)
 #pragma unroll
-            for (int k = 0; k < TPL; ++k) rpy_near_select<R, BLK>(r2[g][k], ir[g][k], cf[g][k]);
+            for (int k = 0; k < TPL; ++k) {
+              if constexpr (kUF34)
+                rpy_near_select_uf34<R>(r2[g][k], ir[g][k], cf[g][k]);
+              else
+                rpy_near_select<R, BLK>(r2[g][k], ir[g][k], cf[g][k]);
+            }
         }
 #pragma unroll
         for (int g = 0; g < SG; ++g)
@@ -187,15 +219,16 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
   }
   double* Uo = U + (int64_t)it.chunk * slab;
   double* Wo = kAngular ? W + (int64_t)it.chunk * slab : nullptr;
+  const double tuf = kUF34 ? 0.75 * t0 : t0;
 #pragma unroll
   for (int k = 0; k < TPL; ++k) {
     const int loc = tid + k * kPairThreads;
     if (loc < it.ntgt) {
       const int64_t i = it.tgt0 + loc;
       const PairAcc<double>& a = acc64[k];
-      Uo[3 * i + 0] = t0 * a.uf.x + rt0 * a.ut.x;
-      Uo[3 * i + 1] = t0 * a.uf.y + rt0 * a.ut.y;
-      Uo[3 * i + 2] = t0 * a.uf.z + rt0 * a.ut.z;
+      Uo[3 * i + 0] = tuf * a.uf.x + rt0 * a.ut.x;
+      Uo[3 * i + 1] = tuf * a.uf.y + rt0 * a.ut.y;
+      Uo[3 * i + 2] = tuf * a.uf.z + rt0 * a.ut.z;
       if (kAngular) {
         Wo[3 * i + 0] = r0 * a.wt.x + rt0 * a.wf.x;
         Wo[3 * i + 1] = r0 * a.wt.y + rt0 * a.wf.y;
@@ -282,27 +315,6 @@ __device__ inline double lane_rol1<double>(double v) {
   const uint32_t hi =
       (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x134, 0xf, 0xf, true);
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
-// Open-boundary UF coefficients times 4/3, so that the far field costs one
-// operation less: fuf = 1/r + (2/3)/r^3, guf = 1/r^3 - 2/r^5 (4/3 of
-// rpy_far's); the caller scales its sums by 3/4.
-template <class R>
-__device__ inline void rpy_far_uf34(R ir, RpyCoef<R>& k) {
-  const R ir2 = ir * ir;
-  const R ir3 = ir * ir2;
-  k.fuf = lm_fma(R(2.0 / 3.0), ir3, ir);
-  k.guf = lm_fma(R(-2) * ir2, ir3, ir3);
-}
-
-// rpy_near_select for the 4/3-scaled coefficients
-template <class R>
-__device__ inline void rpy_near_select_uf34(R r2, R ir, RpyCoef<R>& k) {
-  RpyCoef<R> n = k;
-  rpy_near<R, kUF>(r2, ir, n);
-  const bool is_near = r2 < R(4);
-  k.fuf = is_near ? n.fuf * R(4.0 / 3.0) : k.fuf;
-  k.guf = is_near ? n.guf * R(4.0 / 3.0) : k.guf;
 }
 
 template <class R, int TPL>
