@@ -758,8 +758,15 @@ __device__ inline void pair_item_multi(const PairItem& it, const typename Rec<R>
           c.dy = pi[k].y - pj.y;
           c.dz = pi[k].z - pj.z;
           const R r2 = c.dx * c.dx + c.dy * c.dy + c.dz * c.dz;
-          R fwt, gwt, cc;
-          rpy_coeffs<R, kUF>(r2, c.a0, c.cd, fwt, gwt, cc);
+          const R ir = lm_rsqrt(r2);
+          RpyCoef<R> cf{};
+          rpy_far_uf34<R>(ir, cf);  // times 4/3, undone on the output
+          if (__builtin_expect(r2 < R(4), 0)) {
+            lm_rare_path();
+            rpy_near_select_uf34<R>(r2, ir, cf);
+          }
+          c.a0 = cf.fuf;
+          c.cd = cf.guf;
 #pragma unroll
           for (int r = 0; r < NR; ++r) uf_apply_open<R>(c, fj[r].x, fj[r].y, fj[r].z, acc[k][r]);
         }
@@ -791,9 +798,10 @@ __device__ inline void pair_item_multi(const PairItem& it, const typename Rec<R>
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         double* o = U + ((int64_t)r * nch + it.chunk) * n3;
-        o[3 * i + 0] = t0 * acc64[k][r].x;
-        o[3 * i + 1] = t0 * acc64[k][r].y;
-        o[3 * i + 2] = t0 * acc64[k][r].z;
+        const double tuf = WALL ? t0 : 0.75 * t0;
+        o[3 * i + 0] = tuf * acc64[k][r].x;
+        o[3 * i + 1] = tuf * acc64[k][r].y;
+        o[3 * i + 2] = tuf * acc64[k][r].z;
       }
     }
   }
