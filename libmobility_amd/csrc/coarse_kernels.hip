@@ -50,7 +50,8 @@ __device__ inline void tri_index(int64_t p, int& c, int& d) {
 }
 
 // s += the 3x3 translational block M_ij / t0 of the pair (ri, rj),
-// row-major (open: the six entries of the upper triangle)
+// row-major (open: the six entries of the upper triangle, times 4/3: the
+// caller scales the open sums by 3/4)
 template <bool WALL, class R, class R4>
 __device__ inline void pair_block_sum(const R4 ri, const R4 rj, R s[9]) {
   if (WALL) {
@@ -69,8 +70,14 @@ __device__ inline void pair_block_sum(const R4 ri, const R4 rj, R s[9]) {
   } else {
     const R dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
     const R r2 = dx * dx + dy * dy + dz * dz;
-    R fuf, guf, fwt, gwt, cc;
-    rpy_coeffs<R, kUF>(r2, fuf, guf, fwt, gwt, cc);
+    const R ir = lm_rsqrt(r2);
+    RpyCoef<R> cf{};
+    rpy_far_uf34<R>(ir, cf);
+    if (__builtin_expect(r2 < R(4), 0)) {
+      lm_rare_path();
+      rpy_near_select_uf34<R>(r2, ir, cf);
+    }
+    const R fuf = cf.fuf, guf = cf.guf;
     s[0] += fuf + guf * dx * dx;
     s[1] += guf * dx * dy;
     s[2] += guf * dx * dz;
@@ -161,7 +168,7 @@ __global__ __launch_bounds__(256) void k_coarse_matrix(const R4* __restrict__ re
       for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
     }
     if (lane == 0) {
-      const double f = scale[gc] * scale[gd];
+      const double f = scale[gc] * scale[gd] * (S.wall ? 1.0 : 0.75);  // open sums carry 4/3
       double* M = Mc + S.moff;
   #pragma unroll
       for (int a = 0; a < 3; ++a)

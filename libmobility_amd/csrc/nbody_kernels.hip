@@ -70,27 +70,6 @@ struct NearTest<float> {
   __device__ bool hit() const { return m < 0x40800000u; }  // the bits of 4.0f
 };
 
-// Open-boundary UF coefficients times 4/3, so that the far field costs one
-// operation less: fuf = 1/r + (2/3)/r^3, guf = 1/r^3 - 2/r^5 (4/3 of
-// rpy_far's); the caller scales its sums by 3/4.
-template <class R>
-__device__ inline void rpy_far_uf34(R ir, RpyCoef<R>& k) {
-  const R ir2 = ir * ir;
-  const R ir3 = ir * ir2;
-  k.fuf = lm_fma(R(2.0 / 3.0), ir3, ir);
-  k.guf = lm_fma(R(-2) * ir2, ir3, ir3);
-}
-
-// rpy_near_select for the 4/3-scaled coefficients
-template <class R>
-__device__ inline void rpy_near_select_uf34(R r2, R ir, RpyCoef<R>& k) {
-  RpyCoef<R> n = k;
-  rpy_near<R, kUF>(r2, ir, n);
-  const bool is_near = r2 < R(4);
-  k.fuf = is_near ? n.fuf * R(4.0 / 3.0) : k.fuf;
-  k.guf = is_near ? n.guf * R(4.0 / 3.0) : k.guf;
-}
-
 // One item: targets [tgt0, tgt0 + ntgt) x sources [src0, src1) of one system.
 template <class R, bool WALL, int BLK, int TPL>
 __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* __restrict__ pos,

@@ -179,6 +179,27 @@ __host__ __device__ inline void rpy_coeffs(R r2, R& fuf, R& guf, R& fwt, R& gwt,
   c = k.c;
 }
 
+// Open-boundary UF coefficients times 4/3, so that the far field costs one
+// operation less: fuf = 1/r + (2/3)/r^3, guf = 1/r^3 - 2/r^5 (4/3 of
+// rpy_far's); the caller scales its sums by 3/4.
+template <class R>
+__host__ __device__ inline void rpy_far_uf34(R ir, RpyCoef<R>& k) {
+  const R ir2 = ir * ir;
+  const R ir3 = ir * ir2;
+  k.fuf = lm_fma(R(2.0 / 3.0), ir3, ir);
+  k.guf = lm_fma(R(-2) * ir2, ir3, ir3);
+}
+
+// rpy_near_select for the 4/3-scaled coefficients
+template <class R>
+__host__ __device__ inline void rpy_near_select_uf34(R r2, R ir, RpyCoef<R>& k) {
+  RpyCoef<R> n = k;
+  rpy_near<R, kUF>(r2, ir, n);
+  const bool is_near = r2 < R(4);
+  k.fuf = is_near ? n.fuf * R(4.0 / 3.0) : k.fuf;
+  k.guf = is_near ? n.guf * R(4.0 / 3.0) : k.guf;
+}
+
 // Open-boundary contribution of one source (f = force, t = torque) to one
 // target with given coefficients, added into a (dimensionless; the caller
 // applies t0, rt0, r0).
