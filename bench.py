@@ -234,6 +234,7 @@ MEASURED_TRAFFIC_BYTES = {
     ("c3", 131072, "symmetric", 4, 64): (152410.0 * 2.0 + 394750.0) * 1024.0,
     ("c3", 131072, "symmetric", 4, 32): (104635.2 * 2.0 + 296448.0) * 1024.0,
     ("c3", 131072, "symmetric", 2, 64): (177902.1 * 2.0 + 591360.0) * 1024.0,
+    ("c2", 65536, "symmetric", 1, 64): (130223.0 * 2.0 + 591360.0) * 1024.0,
 }
 
 
