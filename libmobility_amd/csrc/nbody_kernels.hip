@@ -60,6 +60,18 @@ struct NearTestCmp {
 };
 template <class R>
 struct NearTest : NearTestCmp<R> {};
+// FP64: the high words of non-negative doubles order like the values, and
+// every double whose high word is that of 4.0 (0x40100000) is >= 4.0, so
+// r^2 < 4 exactly when its high word is below 0x40100000.
+template <>
+struct NearTest<double> {
+  uint32_t m = 0xffffffffu;
+  __device__ void add(double r2) {
+    const uint32_t h = (uint32_t)(__builtin_bit_cast(uint64_t, r2) >> 32);
+    m = h < m ? h : m;
+  }
+  __device__ bool hit() const { return m < 0x40100000u; }
+};
 template <>
 struct NearTest<float> {
   uint32_t m = 0xffffffffu;
