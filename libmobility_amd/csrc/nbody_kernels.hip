@@ -129,7 +129,6 @@ __device__ inline void pair_item(const PairItem& it, const typename Rec<R>::T* _
     }
     __syncthreads();
     if constexpr (WALL) {
-#pragma unroll (TPL >= 4 ? 1 : 4 / TPL)
       for (int s = 0; s < cnt; ++s) {
         const R4 pj = s_pos[s];
         R4 fj, tj;
