@@ -227,6 +227,7 @@ MEASURED_FP32_FLOPS = {
     ("c3", 131072, "symmetric", 4, 64): 7.134e9 * 64,
     ("c3", 131072, "symmetric", 4, 32): 7.00029e9 * 64,
     ("c2", 65536, "symmetric", 1, 64): 1.25718e10 * 64,
+    ("c4", 816182, "symmetric", 2, 8): 3.19152e9 * 64,
 }
 
 MEASURED_TRAFFIC_BYTES = {
@@ -235,6 +236,7 @@ MEASURED_TRAFFIC_BYTES = {
     ("c3", 131072, "symmetric", 4, 32): (104635.2 * 2.0 + 296448.0) * 1024.0,
     ("c3", 131072, "symmetric", 2, 64): (177902.1 * 2.0 + 591360.0) * 1024.0,
     ("c2", 65536, "symmetric", 1, 64): (130223.0 * 2.0 + 591360.0) * 1024.0,
+    ("c4", 816182, "symmetric", 2, 8): (108844.0 * 2.0 + 165989.0) * 1024.0,
 }
 
 
@@ -942,9 +944,11 @@ def bench_c4(args):
                           "even systems open (RSA cube), odd systems single wall (RSA slab, z in "
                           "[1.5a, 10a]), phi uniform in [0.05, 0.25], a=1, eta=1/(6 pi), F~N(0,1); "
                           "FP32 pairs, FP64 accumulation; one persistent launch", "N": N}
+    te = extra["tiling_ensemble"]
+    key = ("c4", N, ev, te["targets_per_lane"], te["source_chunks"])
     return result_line("Ensemble Mdot Gpair/s, 512 ragged systems (FP32 pairs)",
                        pairs / (ms_e * 1e-3) / 1e9, "Gpair/s", args.steps, args.warmup, ms_e, True,
-                       "f32", config, compute_roofline(flops, ms_e, PEAK_FP32_TFLOPS), extra, None)
+                       "f32", config, counted_roofline(key, flops, ms_e), extra, None)
 
 
 # ---------------------------------------------------------------------------
