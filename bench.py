@@ -237,6 +237,7 @@ MEASURED_TRAFFIC_BYTES = {
     ("c3", 131072, "symmetric", 2, 64): (177902.1 * 2.0 + 591360.0) * 1024.0,
     ("c2", 65536, "symmetric", 1, 64): (130223.0 * 2.0 + 591360.0) * 1024.0,
     ("c4", 816182, "symmetric", 2, 8): (108844.0 * 2.0 + 165989.0) * 1024.0,
+    ("c1", 1024, "ordered", 1, 16): (173.0 * 2.0 + 384.0) * 1024.0,
 }
 
 
@@ -654,7 +655,10 @@ def bench_c1(args):
     total = ms_mdot + ms_sqrt
     return result_line("Mdot + sqrtMdotW ms at N=1024 open FP64 tol 1e-6", total, "ms",
                        args.steps, args.warmup, total, False, "f64", config,
-                       compute_roofline(pairs * fpp, ms_mdot, PEAK_FP64_TFLOPS), extra, cpu)
+                       compute_roofline(pairs * fpp, ms_mdot, PEAK_FP64_TFLOPS,
+                                        MEASURED_TRAFFIC_BYTES.get(
+                                            ("c1", N, "ordered", tiling["targets_per_lane"],
+                                             tiling["source_chunks"]))), extra, cpu)
 
 
 # ---------------------------------------------------------------------------
